@@ -1,0 +1,81 @@
+"""ctypes signatures of the C ABI declared in include/afepack_amd.h."""
+import ctypes as C
+
+P = C.c_void_p
+I32 = C.c_int32
+I64 = C.c_int64
+U64 = C.c_uint64
+F64 = C.c_double
+PP = C.POINTER(C.c_void_p)
+PI32 = C.POINTER(C.c_int32)
+PI64 = C.POINTER(C.c_int64)
+PF64 = C.POINTER(C.c_double)
+
+
+class Field(C.Structure):
+    _fields_ = [("n_modes", I32), ("dim", I32), ("k", PF64), ("phi", PF64),
+                ("w", PF64), ("scale", F64)]
+
+
+class AmgParams(C.Structure):
+    _fields_ = [("alpha", F64), ("nmpo", C.c_uint32), ("nmpsd", F64), ("lazy", I32),
+                ("smooth", I32), ("build_fp32", I32)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", I32), ("converged", I32), ("rel_res_l2", F64),
+                ("rel_res_l1", F64), ("res_l1", F64), ("seconds", F64)]
+
+
+PField = C.POINTER(Field)
+
+SIGS = {
+    "afe_last_error": (C.c_char_p, []),
+    "afe_version": (C.c_int, []),
+    "afe_ctx_create": (C.c_int, [C.c_int, PP]),
+    "afe_ctx_destroy": (C.c_int, [P]),
+    "afe_ctx_synchronize": (C.c_int, [P]),
+    "afe_ctx_device_bytes": (C.c_int, [P, PI64]),
+    "afe_mesh_square": (C.c_int, [I32, F64, U64, P, P, P]),
+    "afe_mesh_cube": (C.c_int, [I32, F64, U64, P, P, P]),
+    "afe_mesh_displace": (C.c_int, [I32, I64, P, P, F64, I32]),
+    "afe_field_random": (C.c_int, [I32, I32, F64, U64, P, P, P, PF64]),
+    "afe_space_create": (C.c_int, [P, I32, I64, P, I64, P, P, PP]),
+    "afe_space_destroy": (C.c_int, [P]),
+    "afe_space_info": (C.c_int, [P, PI64, PI64, PI32]),
+    "afe_space_dofs": (C.c_int, [P, P, P, P]),
+    "afe_space_pattern": (C.c_int, [P, P, P]),
+    "afe_space_update_coords": (C.c_int, [P, P]),
+    "afe_matrix_create": (C.c_int, [P, PP]),
+    "afe_matrix_create_csr": (C.c_int, [P, I64, P, P, P, PP]),
+    "afe_matrix_destroy": (C.c_int, [P]),
+    "afe_assemble": (C.c_int, [P, I32, I32, PField]),
+    "afe_assemble_elements": (C.c_int, [P, P]),
+    "afe_matrix_values": (C.c_int, [P, P]),
+    "afe_matrix_set_values": (C.c_int, [P, P]),
+    "afe_matrix_device_values": (C.c_void_p, [P]),
+    "afe_matrix_vmult": (C.c_int, [P, P, P]),
+    "afe_rhs": (C.c_int, [P, I32, I32, PField, P, P]),
+    "afe_apply_dirichlet": (C.c_int, [P, P, P]),
+    "afe_error_l2": (C.c_int, [P, P, I32, PF64]),
+    "afe_error_h1": (C.c_int, [P, P, I32, PF64]),
+    "afe_amg_create": (C.c_int, [P, C.POINTER(AmgParams), PP]),
+    "afe_amg_destroy": (C.c_int, [P]),
+    "afe_amg_create_host": (C.c_int, [I64, P, P, P, C.POINTER(AmgParams), PP]),
+    "afe_amg_lazy_reinit_host": (C.c_int, [P, P]),
+    "afe_amg_coarse_inverse": (C.c_int, [P, P]),
+    "afe_amg_lazy_reinit": (C.c_int, [P, P]),
+    "afe_amg_info": (C.c_int, [P, PI32, P, P, PI32]),
+    "afe_amg_level_csr": (C.c_int, [P, I32, I32, PI64, PI64, PI64, P, P, P]),
+    "afe_amg_solve": (C.c_int, [P, P, P, F64, I64, I32, F64, C.POINTER(SolveStats)]),
+    "afe_amg_precondition": (C.c_int, [P, P, P, I32]),
+    "afe_amg_pcg": (C.c_int, [P, P, P, F64, I32, I32, C.POINTER(SolveStats)]),
+    "afe_amg_vcycle": (C.c_int, [P, P, P, I32]),
+}
+
+
+def declare(lib):
+    for name, (res, args) in SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args

@@ -1,0 +1,340 @@
+"""Python wrappers over the C ABI (include/afepack_amd.h).
+
+Host arrays are numpy; device vectors are torch float64 tensors on the
+context's device.  Every call goes through libafepack_amd.so.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _sigs
+
+__all__ = [
+    "P1_TRI", "P2_TRI", "P1_TET", "OP_STIFF", "OP_MASS", "OP_VARSTIFF",
+    "F_POISSON", "F_VARCOEF", "F_NODAL", "F_ONE", "PREC_FP64", "PREC_FP32",
+    "mesh_square", "mesh_cube", "mesh_displace", "field_random", "Field",
+    "Context", "Space", "Matrix", "Amg", "HostAmg", "AmgParams",
+]
+
+P1_TRI, P2_TRI, P1_TET = 1, 2, 3
+OP_STIFF, OP_MASS, OP_VARSTIFF = 0, 1, 2
+F_POISSON, F_VARCOEF, F_NODAL, F_ONE = 0, 1, 2, 3
+PREC_FP64, PREC_FP32 = 0, 1
+
+
+def _lib():
+    from . import load
+    return load()
+
+
+def _chk(code):
+    from . import check
+    return check(code)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return C.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def mesh_square(n, jitter=0.0, seed=0):
+    """Structured unit-square triangulation: (coords, elem_vert, vert_mark)."""
+    nv, ne = (n + 1) ** 2, 2 * n * n
+    coords = np.empty(2 * nv, dtype=np.float64)
+    ev = np.empty(3 * ne, dtype=np.int32)
+    vm = np.empty(nv, dtype=np.int32)
+    _chk(_lib().afe_mesh_square(n, jitter, seed, _ptr(coords), _ptr(ev), _ptr(vm)))
+    return coords, ev, vm
+
+
+def mesh_cube(n, jitter=0.0, seed=0):
+    """Unit cube, n^3 cubes x 6 tetrahedra: (coords, elem_vert, vert_mark)."""
+    nv, ne = (n + 1) ** 3, 6 * n ** 3
+    coords = np.empty(3 * nv, dtype=np.float64)
+    ev = np.empty(4 * ne, dtype=np.int32)
+    vm = np.empty(nv, dtype=np.int32)
+    _chk(_lib().afe_mesh_cube(n, jitter, seed, _ptr(coords), _ptr(ev), _ptr(vm)))
+    return coords, ev, vm
+
+
+def mesh_displace(dim, coords, vert_mark, amp_h, k):
+    out = np.array(coords, dtype=np.float64, copy=True)
+    _chk(_lib().afe_mesh_displace(dim, len(vert_mark), _ptr(out), _ptr(vert_mark), amp_h, k))
+    return out
+
+
+class Field:
+    """Seeded Fourier field g; a = exp(g).  Keeps the arrays alive."""
+
+    def __init__(self, k, phi, w, scale):
+        self.k = np.ascontiguousarray(k, dtype=np.float64)
+        self.phi = np.ascontiguousarray(phi, dtype=np.float64)
+        self.w = np.ascontiguousarray(w, dtype=np.float64)
+        self.scale = float(scale)
+        self.n_modes = len(self.phi)
+        self.dim = self.k.size // self.n_modes
+
+    def cstruct(self):
+        pf = C.POINTER(C.c_double)
+        return _sigs.Field(self.n_modes, self.dim, self.k.ctypes.data_as(pf),
+                           self.phi.ctypes.data_as(pf), self.w.ctypes.data_as(pf), self.scale)
+
+
+def field_random(n_modes, dim, amplitude, seed):
+    k = np.empty(n_modes * dim)
+    phi = np.empty(n_modes)
+    w = np.empty(n_modes)
+    sc = C.c_double()
+    _chk(_lib().afe_field_random(n_modes, dim, amplitude, seed, _ptr(k), _ptr(phi), _ptr(w),
+                                 C.byref(sc)))
+    return Field(k, phi, w, sc.value)
+
+
+def _field_ptr(f):
+    if f is None:
+        return None, None
+    s = f.cstruct()
+    return C.pointer(s), s
+
+
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _chk(_lib().afe_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            _lib().afe_ctx_destroy(self.h)
+            self.h = None
+
+    def device_bytes(self):
+        b = C.c_int64()
+        _chk(_lib().afe_ctx_device_bytes(self.h, C.byref(b)))
+        return b.value
+
+    def synchronize(self):
+        _chk(_lib().afe_ctx_synchronize(self.h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Space:
+    """FE space (DOF numbering + pattern); ctx=None builds it on the host only."""
+
+    def __init__(self, ctx, kind, coords, elem_vert, vert_mark):
+        self.ctx = ctx
+        self.kind = kind
+        self.dim = 3 if kind == P1_TET else 2
+        self.coords = np.ascontiguousarray(coords, dtype=np.float64)
+        self.elem_vert = np.ascontiguousarray(elem_vert, dtype=np.int32)
+        self.vert_mark = np.ascontiguousarray(vert_mark, dtype=np.int32)
+        nv = len(self.vert_mark)
+        ne = len(self.elem_vert) // (4 if kind == P1_TET else 3)
+        h = C.c_void_p()
+        _chk(_lib().afe_space_create(ctx.h if ctx else None, kind, nv, _ptr(self.coords), ne,
+                                     _ptr(self.elem_vert), _ptr(self.vert_mark), C.byref(h)))
+        self.h = h
+        self.n_vert, self.n_elem = nv, ne
+        nd, nz, nl = C.c_int64(), C.c_int64(), C.c_int32()
+        _chk(_lib().afe_space_info(h, C.byref(nd), C.byref(nz), C.byref(nl)))
+        self.n_dof, self.nnz, self.n_loc = nd.value, nz.value, nl.value
+
+    def dofs(self):
+        ed = np.empty(self.n_elem * self.n_loc, dtype=np.int32)
+        dm = np.empty(self.n_dof, dtype=np.int32)
+        dp = np.empty(self.n_dof * self.dim, dtype=np.float64)
+        _chk(_lib().afe_space_dofs(self.h, _ptr(ed), _ptr(dm), _ptr(dp)))
+        return ed, dm, dp
+
+    def pattern(self):
+        rp = np.empty(self.n_dof + 1, dtype=np.int64)
+        col = np.empty(self.nnz, dtype=np.int64)
+        _chk(_lib().afe_space_pattern(self.h, _ptr(rp), _ptr(col)))
+        return rp, col
+
+    def update_coords(self, coords):
+        self.coords = np.ascontiguousarray(coords, dtype=np.float64)
+        _chk(_lib().afe_space_update_coords(self.h, _ptr(self.coords)))
+
+    def rhs(self, acc, fkind, field=None, nodal=None, out=None):
+        import torch
+        if out is None:
+            out = torch.empty(self.n_dof, dtype=torch.float64, device=f"cuda:{self.ctx.device}")
+        fp, _keep = _field_ptr(field)
+        _chk(_lib().afe_rhs(self.h, acc, fkind, fp, _ptr(nodal), _ptr(out)))
+        return out
+
+    def error_l2(self, uh, acc):
+        e = C.c_double()
+        _chk(_lib().afe_error_l2(self.h, _ptr(uh), acc, C.byref(e)))
+        return e.value
+
+    def error_h1(self, uh, acc):
+        e = C.c_double()
+        _chk(_lib().afe_error_h1(self.h, _ptr(uh), acc, C.byref(e)))
+        return e.value
+
+    def close(self):
+        if self.h:
+            _lib().afe_space_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Matrix:
+    def __init__(self, space=None, ctx=None, csr=None):
+        h = C.c_void_p()
+        if space is not None:
+            _chk(_lib().afe_matrix_create(space.h, C.byref(h)))
+            self.n, self.nnz = space.n_dof, space.nnz
+        else:
+            rp, col, val = csr
+            rp = np.ascontiguousarray(rp, dtype=np.int64)
+            col = np.ascontiguousarray(col, dtype=np.int64)
+            val = np.ascontiguousarray(val, dtype=np.float64)
+            _chk(_lib().afe_matrix_create_csr(ctx.h, len(rp) - 1, _ptr(rp), _ptr(col), _ptr(val),
+                                              C.byref(h)))
+            self.n, self.nnz = len(rp) - 1, int(rp[-1])
+        self.space = space
+        self.h = h
+
+    def assemble(self, op, acc, field=None):
+        fp, _keep = _field_ptr(field)
+        _chk(_lib().afe_assemble(self.h, op, acc, fp))
+
+    def assemble_elements(self, elem_mats):
+        em = np.ascontiguousarray(elem_mats, dtype=np.float64)
+        _chk(_lib().afe_assemble_elements(self.h, _ptr(em)))
+
+    def values(self):
+        v = np.empty(self.nnz, dtype=np.float64)
+        _chk(_lib().afe_matrix_values(self.h, _ptr(v)))
+        return v
+
+    def set_values(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        _chk(_lib().afe_matrix_set_values(self.h, _ptr(v)))
+
+    def apply_dirichlet(self, f, u=None):
+        _chk(_lib().afe_apply_dirichlet(self.h, _ptr(u), _ptr(f)))
+
+    def vmult(self, x, y):
+        _chk(_lib().afe_matrix_vmult(self.h, _ptr(x), _ptr(y)))
+
+    def close(self):
+        if self.h:
+            _lib().afe_matrix_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def AmgParams(alpha=0.25, nmpo=50, nmpsd=0.382, lazy=0, smooth=3, build_fp32=1):
+    return _sigs.AmgParams(alpha, nmpo, nmpsd, lazy, smooth, build_fp32)
+
+
+class _AmgBase:
+    def info(self):
+        L = C.c_int32()
+        n = np.empty(128, dtype=np.int64)
+        nz = np.empty(128, dtype=np.int64)
+        full = C.c_int32()
+        _chk(_lib().afe_amg_info(self.h, C.byref(L), _ptr(n), _ptr(nz), C.byref(full)))
+        return {"levels": L.value, "n": n[:L.value].tolist(), "nnz": nz[:L.value].tolist(),
+                "coarse_full": bool(full.value)}
+
+    def level_csr(self, level, which=0):
+        m, n, nnz = C.c_int64(), C.c_int64(), C.c_int64()
+        _chk(_lib().afe_amg_level_csr(self.h, level, which, C.byref(m), C.byref(n), C.byref(nnz),
+                                      None, None, None))
+        rp = np.empty(m.value + 1, dtype=np.int64)
+        col = np.empty(nnz.value, dtype=np.int64)
+        val = np.empty(nnz.value, dtype=np.float64)
+        _chk(_lib().afe_amg_level_csr(self.h, level, which, C.byref(m), C.byref(n), C.byref(nnz),
+                                      _ptr(rp), _ptr(col), _ptr(val)))
+        return m.value, n.value, rp, col, val
+
+    def coarse_inverse(self):
+        info = self.info()
+        nc = info["n"][-1]
+        out = np.empty(nc * nc, dtype=np.float64)
+        _chk(_lib().afe_amg_coarse_inverse(self.h, _ptr(out)))
+        return out.reshape(nc, nc)
+
+    def close(self):
+        if self.h:
+            _lib().afe_amg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HostAmg(_AmgBase):
+    """Host-only hierarchy of a CSR matrix (no device)."""
+
+    def __init__(self, rp, col, val, params=None):
+        rp = np.ascontiguousarray(rp, dtype=np.int64)
+        col = np.ascontiguousarray(col, dtype=np.int64)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        h = C.c_void_p()
+        prm = params if params is not None else AmgParams(build_fp32=0)
+        _chk(_lib().afe_amg_create_host(len(rp) - 1, _ptr(rp), _ptr(col), _ptr(val), C.byref(prm),
+                                        C.byref(h)))
+        self.h = h
+
+    def lazy_reinit(self, val):
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        _chk(_lib().afe_amg_lazy_reinit_host(self.h, _ptr(val)))
+
+
+class Amg(_AmgBase):
+    def __init__(self, matrix, params=None):
+        h = C.c_void_p()
+        prm = params if params is not None else AmgParams()
+        _chk(_lib().afe_amg_create(matrix.h, C.byref(prm), C.byref(h)))
+        self.h = h
+        self.matrix = matrix
+
+    def lazy_reinit(self, matrix):
+        _chk(_lib().afe_amg_lazy_reinit(self.h, matrix.h))
+        self.matrix = matrix
+
+    def solve(self, x, b, tol=0.0, step=20, mode=1, rtol=0.0):
+        st = _sigs.SolveStats()
+        _chk(_lib().afe_amg_solve(self.h, _ptr(x), _ptr(b), tol, step, mode, rtol, C.byref(st)))
+        return st
+
+    def pcg(self, x, b, rtol=1e-10, max_it=500, prec=PREC_FP32, check_conv=True):
+        st = _sigs.SolveStats()
+        code = _lib().afe_amg_pcg(self.h, _ptr(x), _ptr(b), rtol, max_it, prec, C.byref(st))
+        if check_conv or code not in (0, -5):
+            _chk(code)
+        return st
+
+    def vcycle(self, z, r, prec=PREC_FP64):
+        _chk(_lib().afe_amg_vcycle(self.h, _ptr(z), _ptr(r), prec))
+
+    def precondition(self, x, b, iterate_step=5):
+        _chk(_lib().afe_amg_precondition(self.h, _ptr(x), _ptr(b), iterate_step))

@@ -1,0 +1,1261 @@
+// capi.hip -- the C ABI (include/afepack_amd.h) over the host algorithms
+// (host.cpp) and the gfx950 kernels (kernels.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <type_traits>
+#include <string>
+#include <vector>
+
+#include "../../include/afepack_amd.h"
+#include "host.hpp"
+#include "kernels.hpp"
+#include "quadrature.hpp"
+
+using namespace afe;
+
+// ===========================================================================
+// errors
+// ===========================================================================
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+struct HipFail {
+  hipError_t e;
+  const char *what;
+};
+
+#define HIPCHK(call)                                                           \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) throw HipFail{e_, #call};                            \
+  } while (0)
+
+#define API_BEGIN try {
+#define API_END                                                                \
+  }                                                                            \
+  catch (const HipFail &h) {                                                   \
+    return fail(AFE_E_HIP, std::string(h.what) + ": " + hipGetErrorString(h.e)); \
+  }                                                                            \
+  catch (const afe::Error &x) {                                                \
+    return fail(x.code, x.msg);                                                \
+  }                                                                            \
+  catch (const std::bad_alloc &) {                                             \
+    return fail(AFE_E_NOMEM, "host allocation failed");                        \
+  }                                                                            \
+  catch (const std::exception &x) {                                            \
+    return fail(AFE_E_ARG, x.what());                                          \
+  }                                                                            \
+  return AFE_OK;
+
+extern "C" const char *afe_last_error(void) { return g_err.c_str(); }
+extern "C" int afe_version(void) { return 1; }
+
+// ===========================================================================
+// context and device buffers
+// ===========================================================================
+struct afe_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::recursive_mutex mu;
+  int64_t bytes = 0;
+  double *d_quad = nullptr; // all quadrature rules
+};
+
+template <class T> struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  afe_ctx *ctx = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept { *this = std::move(o); }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p; n = o.n; ctx = o.ctx;
+      o.p = nullptr; o.n = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) {
+      (void)hipFree(p);
+      if (ctx) ctx->bytes -= (int64_t)(n * sizeof(T));
+    }
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(afe_ctx *c, size_t count) {
+    if (p && n == count) return;
+    release();
+    ctx = c;
+    if (count == 0) return;
+    if (hipMalloc((void **)&p, count * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      throw afe::Error{AFE_E_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed"};
+    }
+    n = count;
+    ctx->bytes += (int64_t)(n * sizeof(T));
+  }
+  void upload(afe_ctx *c, const T *h, size_t count) {
+    alloc(c, count);
+    if (count) HIPCHK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  }
+  void upload(afe_ctx *c, const std::vector<T> &v) { upload(c, v.data(), v.size()); }
+};
+
+static inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+extern "C" int afe_ctx_create(int device, afe_ctx **out) {
+  API_BEGIN
+  if (!out) return fail(AFE_E_ARG, "null output");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    return fail(AFE_E_NODEVICE, "no HIP device visible");
+  }
+  if (device < 0 || device >= n) return fail(AFE_E_NODEVICE, "device index out of range");
+  auto *c = new afe_ctx();
+  c->device = device;
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&c->ev0));
+  HIPCHK(hipEventCreate(&c->ev1));
+  HIPCHK(hipMalloc((void **)&c->d_quad, sizeof(kQuadData)));
+  c->bytes += sizeof(kQuadData);
+  HIPCHK(hipMemcpy(c->d_quad, kQuadData, sizeof(kQuadData), hipMemcpyHostToDevice));
+  *out = c;
+  API_END
+}
+
+extern "C" int afe_ctx_destroy(afe_ctx *c) {
+  if (!c) return AFE_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->d_quad);
+  (void)hipEventDestroy(c->ev0);
+  (void)hipEventDestroy(c->ev1);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return AFE_OK;
+}
+
+extern "C" int afe_ctx_synchronize(afe_ctx *c) {
+  API_BEGIN
+  HIPCHK(hipStreamSynchronize(c->stream));
+  API_END
+}
+
+extern "C" int afe_ctx_device_bytes(afe_ctx *c, int64_t *bytes) {
+  if (!c || !bytes) return fail(AFE_E_ARG, "null argument");
+  *bytes = c->bytes;
+  return AFE_OK;
+}
+
+// ===========================================================================
+// meshes and fields
+// ===========================================================================
+extern "C" int afe_mesh_square(int32_t n, double jitter, uint64_t seed, double *coords,
+                               int32_t *elem_vert, int32_t *vert_mark) {
+  API_BEGIN
+  if (n < 1 || !coords || !elem_vert || !vert_mark) return fail(AFE_E_ARG, "bad mesh arguments");
+  if (jitter < 0 || jitter >= 0.25) return fail(AFE_E_ARG, "jitter must be in [0, 0.25)");
+  mesh_square(n, jitter, seed, coords, elem_vert, vert_mark);
+  API_END
+}
+
+extern "C" int afe_mesh_cube(int32_t n, double jitter, uint64_t seed, double *coords,
+                             int32_t *elem_vert, int32_t *vert_mark) {
+  API_BEGIN
+  if (n < 1 || !coords || !elem_vert || !vert_mark) return fail(AFE_E_ARG, "bad mesh arguments");
+  if (jitter < 0 || jitter >= 0.1) return fail(AFE_E_ARG, "jitter must be in [0, 0.1)");
+  mesh_cube(n, jitter, seed, coords, elem_vert, vert_mark);
+  API_END
+}
+
+extern "C" int afe_mesh_displace(int32_t dim, int64_t n_vert, double *coords,
+                                 const int32_t *vert_mark, double amp_h, int32_t k) {
+  API_BEGIN
+  if ((dim != 2 && dim != 3) || !coords || !vert_mark) return fail(AFE_E_ARG, "bad arguments");
+  // smooth displacement amp_h * sin(2 pi k x_d) per coordinate, interior only
+  for (int64_t v = 0; v < n_vert; v++) {
+    if (vert_mark[v]) continue;
+    double *x = coords + v * dim;
+    double d[3];
+    for (int c = 0; c < dim; c++) d[c] = amp_h * std::sin(2.0 * M_PI * k * x[c]);
+    for (int c = 0; c < dim; c++) x[c] += d[c];
+  }
+  API_END
+}
+
+extern "C" int afe_field_random(int32_t n_modes, int32_t dim, double amplitude, uint64_t seed,
+                                double *k, double *phi, double *w, double *scale) {
+  API_BEGIN
+  if (n_modes < 1 || (dim != 2 && dim != 3) || !k || !phi || !w || !scale)
+    return fail(AFE_E_ARG, "bad field arguments");
+  // k_m in {1..4}^dim, phi_m in [0, 2 pi), w_m = +-(0.5 + 0.5 u); scale makes
+  // sum |w_m| * scale = amplitude, so |g| <= amplitude and a = exp(g) has a
+  // contrast of at most exp(2 amplitude)
+  Rng rng(seed);
+  double sw = 0.0;
+  for (int m = 0; m < n_modes; m++) {
+    for (int d = 0; d < dim; d++) k[m * dim + d] = (double)(1 + (int)(rng.uniform() * 4.0));
+    phi[m] = 2.0 * M_PI * rng.uniform();
+    const double mag = 0.5 + 0.5 * rng.uniform();
+    w[m] = rng.uniform() < 0.5 ? -mag : mag;
+    sw += mag;
+  }
+  *scale = amplitude / sw;
+  API_END
+}
+
+// ===========================================================================
+// FE space
+// ===========================================================================
+struct afe_space {
+  afe_ctx *ctx = nullptr;
+  Space h;
+  int64_t seg_max = 0; // largest 256-row value segment (gather LDS)
+  DevBuf<double> coords;
+  DevBuf<int32_t> ev, elem_dof, dof_mark, adj, col;
+  DevBuf<int64_t> adj_ptr, rp;
+  DevBuf<uint8_t> slot;
+};
+
+extern "C" int afe_space_create(afe_ctx *ctx, int32_t kind, int64_t n_vert, const double *coords,
+                                int64_t n_elem, const int32_t *elem_vert, const int32_t *vert_mark,
+                                afe_space **out) {
+  API_BEGIN
+  if (!out || !coords || !elem_vert || !vert_mark || n_vert <= 0 || n_elem <= 0)
+    return fail(AFE_E_ARG, "bad space arguments");
+  if (kind != AFE_P1_TRI && kind != AFE_P2_TRI && kind != AFE_P1_TET)
+    return fail(AFE_E_ARG, "unknown element kind");
+  auto s = std::make_unique<afe_space>();
+  s->ctx = ctx;
+  space_build(s->h, kind, n_vert, coords, n_elem, elem_vert, vert_mark);
+  if (!ctx) { // host-only space: numbering and pattern, nothing uploaded
+    *out = s.release();
+    return AFE_OK;
+  }
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  if (s->h.rp[s->h.n_dof] > INT32_MAX) return fail(AFE_E_ARG, "nnz exceeds 2^31");
+  for (int64_t r0 = 0; r0 < s->h.n_dof; r0 += 256) {
+    const int64_t r1 = std::min<int64_t>(r0 + 256, s->h.n_dof);
+    s->seg_max = std::max(s->seg_max, s->h.rp[r1] - s->h.rp[r0]);
+  }
+  if (s->seg_max * 8 > 160 * 1024) return fail(AFE_E_ARG, "rows too long for the LDS gather");
+  s->coords.upload(ctx, s->h.coords);
+  s->ev.upload(ctx, s->h.ev);
+  s->elem_dof.upload(ctx, s->h.elem_dof);
+  s->dof_mark.upload(ctx, s->h.dof_mark);
+  s->adj.upload(ctx, s->h.adj);
+  s->adj_ptr.upload(ctx, s->h.adj_ptr);
+  s->slot.upload(ctx, s->h.slot);
+  s->rp.upload(ctx, s->h.rp);
+  s->col.upload(ctx, s->h.col);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *out = s.release();
+  API_END
+}
+
+extern "C" int afe_space_destroy(afe_space *s) {
+  if (!s) return AFE_OK;
+  if (!s->ctx) {
+    delete s;
+    return AFE_OK;
+  }
+  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  (void)hipSetDevice(s->ctx->device);
+  delete s;
+  return AFE_OK;
+}
+
+extern "C" int afe_space_info(const afe_space *s, int64_t *n_dof, int64_t *nnz, int32_t *n_loc) {
+  if (!s) return fail(AFE_E_ARG, "null space");
+  if (n_dof) *n_dof = s->h.n_dof;
+  if (nnz) *nnz = s->h.rp[s->h.n_dof];
+  if (n_loc) *n_loc = s->h.n_loc;
+  return AFE_OK;
+}
+
+extern "C" int afe_space_dofs(const afe_space *s, int32_t *elem_dof, int32_t *dof_mark,
+                              double *dof_point) {
+  if (!s) return fail(AFE_E_ARG, "null space");
+  const Space &h = s->h;
+  if (elem_dof) std::memcpy(elem_dof, h.elem_dof.data(), h.elem_dof.size() * sizeof(int32_t));
+  if (dof_mark) std::memcpy(dof_mark, h.dof_mark.data(), h.dof_mark.size() * sizeof(int32_t));
+  if (dof_point) std::memcpy(dof_point, h.dof_point.data(), h.dof_point.size() * sizeof(double));
+  return AFE_OK;
+}
+
+extern "C" int afe_space_pattern(const afe_space *s, int64_t *row_ptr, int64_t *col) {
+  if (!s) return fail(AFE_E_ARG, "null space");
+  const Space &h = s->h;
+  if (row_ptr) std::memcpy(row_ptr, h.rp.data(), h.rp.size() * sizeof(int64_t));
+  if (col)
+    for (size_t k = 0; k < h.col.size(); k++) col[k] = h.col[k];
+  return AFE_OK;
+}
+
+extern "C" int afe_space_update_coords(afe_space *s, const double *coords) {
+  API_BEGIN
+  if (!s || !coords) return fail(AFE_E_ARG, "null argument");
+  if (!s->ctx) {
+    std::memcpy(s->h.coords.data(), coords, s->h.coords.size() * sizeof(double));
+    return AFE_OK;
+  }
+  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  HIPCHK(hipSetDevice(s->ctx->device));
+  std::memcpy(s->h.coords.data(), coords, s->h.coords.size() * sizeof(double));
+  s->coords.upload(s->ctx, s->h.coords);
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  API_END
+}
+
+// ===========================================================================
+// matrices and assembly
+// ===========================================================================
+struct afe_matrix {
+  afe_ctx *ctx = nullptr;
+  afe_space *s = nullptr; // null for a matrix given as CSR
+  int64_t m = 0, n = 0, nnz = 0;
+  std::vector<int64_t> h_rp; // only for a CSR-given matrix
+  std::vector<int32_t> h_col;
+  DevBuf<int64_t> own_rp;
+  DevBuf<int32_t> own_col, own_mark;
+  DevBuf<double> val;
+  DevBuf<double> Ke; // element matrices of the last assembly
+  const int64_t *rp() const { return s ? s->rp.p : own_rp.p; }
+  const int32_t *col() const { return s ? s->col.p : own_col.p; }
+  const std::vector<int64_t> &hrp() const { return s ? s->h.rp : h_rp; }
+  const std::vector<int32_t> &hcol() const { return s ? s->h.col : h_col; }
+};
+
+extern "C" int afe_matrix_create(afe_space *s, afe_matrix **out) {
+  API_BEGIN
+  if (!s || !out) return fail(AFE_E_ARG, "null argument");
+  if (!s->ctx) return fail(AFE_E_ARG, "host-only space (no device context)");
+  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  HIPCHK(hipSetDevice(s->ctx->device));
+  auto a = std::make_unique<afe_matrix>();
+  a->ctx = s->ctx;
+  a->s = s;
+  a->m = a->n = s->h.n_dof;
+  a->nnz = s->h.rp[s->h.n_dof];
+  a->val.alloc(s->ctx, a->nnz);
+  HIPCHK(hipMemsetAsync(a->val.p, 0, a->nnz * sizeof(double), s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  *out = a.release();
+  API_END
+}
+
+// a square matrix given as the reference's CSR (diagonal first in each row)
+extern "C" int afe_matrix_create_csr(afe_ctx *ctx, int64_t m, const int64_t *row_ptr,
+                                     const int64_t *col, const double *val, afe_matrix **out) {
+  API_BEGIN
+  if (!ctx || !out || !row_ptr || !col || m <= 0) return fail(AFE_E_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  auto a = std::make_unique<afe_matrix>();
+  a->ctx = ctx;
+  a->m = a->n = m;
+  a->nnz = row_ptr[m];
+  if (a->nnz > INT32_MAX) return fail(AFE_E_ARG, "nnz exceeds 2^31");
+  a->h_rp.assign(row_ptr, row_ptr + m + 1);
+  a->h_col.resize(a->nnz);
+  for (int64_t i = 0; i < m; i++) {
+    if (row_ptr[i + 1] <= row_ptr[i] || col[row_ptr[i]] != i)
+      return fail(AFE_E_ARG, "each row must start with its diagonal entry");
+    for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
+      if (col[k] < 0 || col[k] >= m) return fail(AFE_E_ARG, "column index out of range");
+      a->h_col[k] = (int32_t)col[k];
+    }
+  }
+  a->own_rp.upload(ctx, a->h_rp);
+  a->own_col.upload(ctx, a->h_col);
+  a->val.alloc(ctx, a->nnz);
+  if (val)
+    HIPCHK(hipMemcpyAsync(a->val.p, val, a->nnz * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  else
+    HIPCHK(hipMemsetAsync(a->val.p, 0, a->nnz * sizeof(double), ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *out = a.release();
+  API_END
+}
+
+extern "C" int afe_matrix_destroy(afe_matrix *a) {
+  if (!a) return AFE_OK;
+  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  (void)hipSetDevice(a->ctx->device);
+  delete a;
+  return AFE_OK;
+}
+
+static FieldDev upload_field(afe_ctx *ctx, const afe_field *f, DevBuf<double> &store) {
+  FieldDev fd{0, 0, nullptr, nullptr, nullptr, 0.0};
+  if (!f || f->n_modes <= 0) return fd;
+  const int nm = f->n_modes, dim = f->dim;
+  std::vector<double> h((size_t)nm * dim + 2 * nm);
+  std::memcpy(h.data(), f->k, sizeof(double) * nm * dim);
+  std::memcpy(h.data() + nm * dim, f->phi, sizeof(double) * nm);
+  std::memcpy(h.data() + nm * dim + nm, f->w, sizeof(double) * nm);
+  store.upload(ctx, h);
+  fd.n_modes = nm;
+  fd.dim = dim;
+  fd.k = store.p;
+  fd.phi = store.p + nm * dim;
+  fd.w = store.p + nm * dim + nm;
+  fd.scale = f->scale;
+  return fd;
+}
+
+template <int KIND>
+static void launch_elements(afe_matrix *a, int op, const QuadRule *q, FieldDev fd) {
+  afe_space *s = a->s;
+  const int64_t ne = s->h.n_elem;
+  const double *qd = s->ctx->d_quad + q->off;
+  hipStream_t st = s->ctx->stream;
+  if (op == AFE_OP_STIFF)
+    k_element_matrices<KIND, 0><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, qd, q->n, fd, a->Ke.p);
+  else if (op == AFE_OP_MASS)
+    k_element_matrices<KIND, 1><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, qd, q->n, fd, a->Ke.p);
+  else
+    k_element_matrices<KIND, 2><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, qd, q->n, fd, a->Ke.p);
+  HIPCHK(hipGetLastError());
+}
+
+static void launch_gather(afe_matrix *a) {
+  afe_space *s = a->s;
+  const int64_t n = s->h.n_dof;
+  const size_t lds = (size_t)s->seg_max * sizeof(double);
+  hipStream_t st = s->ctx->stream;
+  switch (s->h.n_loc) {
+  case 3:
+    k_gather<3><<<nblk(n), 256, lds, st>>>(n, s->rp.p, s->adj_ptr.p, s->adj.p, s->slot.p, a->Ke.p, a->val.p);
+    break;
+  case 4:
+    k_gather<4><<<nblk(n), 256, lds, st>>>(n, s->rp.p, s->adj_ptr.p, s->adj.p, s->slot.p, a->Ke.p, a->val.p);
+    break;
+  default:
+    k_gather<6><<<nblk(n), 256, lds, st>>>(n, s->rp.p, s->adj_ptr.p, s->adj.p, s->slot.p, a->Ke.p, a->val.p);
+    break;
+  }
+  HIPCHK(hipGetLastError());
+}
+
+extern "C" int afe_assemble(afe_matrix *a, int32_t op, int32_t acc, const afe_field *coef) {
+  API_BEGIN
+  if (!a || !a->s) return fail(AFE_E_ARG, "matrix has no FE space");
+  if (op < AFE_OP_STIFF || op > AFE_OP_VARSTIFF) return fail(AFE_E_ARG, "unknown operator");
+  if (op == AFE_OP_VARSTIFF && (!coef || coef->n_modes <= 0 || coef->dim != a->s->h.tdim))
+    return fail(AFE_E_ARG, "variable coefficient needs a field of the space dimension");
+  afe_space *s = a->s;
+  const QuadRule *q = quad_find(s->h.tdim, acc);
+  if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
+  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  HIPCHK(hipSetDevice(a->ctx->device));
+  DevBuf<double> fstore;
+  FieldDev fd = upload_field(a->ctx, op == AFE_OP_VARSTIFF ? coef : nullptr, fstore);
+  a->Ke.alloc(a->ctx, (size_t)s->h.n_elem * s->h.n_loc * s->h.n_loc);
+  switch (s->h.kind) {
+  case P1_TRI: launch_elements<P1_TRI>(a, op, q, fd); break;
+  case P2_TRI: launch_elements<P2_TRI>(a, op, q, fd); break;
+  default: launch_elements<P1_TET>(a, op, q, fd); break;
+  }
+  launch_gather(a);
+  HIPCHK(hipStreamSynchronize(a->ctx->stream));
+  API_END
+}
+
+extern "C" int afe_assemble_elements(afe_matrix *a, const double *h_elem_mat) {
+  API_BEGIN
+  if (!a || !a->s || !h_elem_mat) return fail(AFE_E_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  HIPCHK(hipSetDevice(a->ctx->device));
+  const Space &h = a->s->h;
+  a->Ke.upload(a->ctx, h_elem_mat, (size_t)h.n_elem * h.n_loc * h.n_loc);
+  launch_gather(a);
+  HIPCHK(hipStreamSynchronize(a->ctx->stream));
+  API_END
+}
+
+extern "C" int afe_matrix_values(const afe_matrix *a, double *h_val) {
+  API_BEGIN
+  if (!a || !h_val) return fail(AFE_E_ARG, "null argument");
+  HIPCHK(hipSetDevice(a->ctx->device));
+  HIPCHK(hipMemcpyAsync(h_val, a->val.p, a->nnz * sizeof(double), hipMemcpyDeviceToHost, a->ctx->stream));
+  HIPCHK(hipStreamSynchronize(a->ctx->stream));
+  API_END
+}
+
+extern "C" int afe_matrix_set_values(afe_matrix *a, const double *h_val) {
+  API_BEGIN
+  if (!a || !h_val) return fail(AFE_E_ARG, "null argument");
+  HIPCHK(hipSetDevice(a->ctx->device));
+  HIPCHK(hipMemcpyAsync(a->val.p, h_val, a->nnz * sizeof(double), hipMemcpyHostToDevice, a->ctx->stream));
+  HIPCHK(hipStreamSynchronize(a->ctx->stream));
+  API_END
+}
+
+extern "C" double *afe_matrix_device_values(afe_matrix *a) { return a ? a->val.p : nullptr; }
+
+__global__ __launch_bounds__(256) void k_csr_vmult(int64_t m, const int64_t *__restrict__ rp,
+                                                   const int32_t *__restrict__ col,
+                                                   const double *__restrict__ val,
+                                                   const double *__restrict__ x, double *__restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double s = 0.0;
+  for (int64_t k = rp[i]; k < rp[i + 1]; k++) s += val[k] * x[col[k]];
+  y[i] = s;
+}
+
+extern "C" int afe_matrix_vmult(afe_matrix *a, const double *d_x, double *d_y) {
+  API_BEGIN
+  if (!a || !d_x || !d_y) return fail(AFE_E_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  HIPCHK(hipSetDevice(a->ctx->device));
+  k_csr_vmult<<<nblk(a->m), 256, 0, a->ctx->stream>>>(a->m, a->rp(), a->col(), a->val.p, d_x, d_y);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(a->ctx->stream));
+  API_END
+}
+
+// ===========================================================================
+// right-hand side, Dirichlet elimination, error functionals
+// ===========================================================================
+template <int KIND>
+static void launch_loads(afe_space *s, const QuadRule *q, int fkind, FieldDev fd, const double *nodal,
+                         double *Fe) {
+  const int64_t ne = s->h.n_elem;
+  k_element_loads<KIND><<<nblk(ne), 256, 0, s->ctx->stream>>>(ne, s->coords.p, s->ev.p, s->elem_dof.p,
+                                                               s->ctx->d_quad + q->off, q->n, fkind, fd,
+                                                               nodal, Fe);
+  HIPCHK(hipGetLastError());
+}
+
+extern "C" int afe_rhs(afe_space *s, int32_t acc, int32_t fkind, const afe_field *coef,
+                       const double *d_nodal, double *d_f) {
+  API_BEGIN
+  if (!s || !d_f) return fail(AFE_E_ARG, "null argument");
+  if (!s->ctx) return fail(AFE_E_ARG, "host-only space (no device context)");
+  if (fkind < AFE_F_POISSON || fkind > AFE_F_ONE) return fail(AFE_E_ARG, "unknown rhs kind");
+  if (fkind == AFE_F_NODAL && !d_nodal) return fail(AFE_E_ARG, "nodal rhs needs DOF values");
+  if (fkind == AFE_F_VARCOEF && (!coef || coef->dim != s->h.tdim))
+    return fail(AFE_E_ARG, "variable-coefficient rhs needs a field");
+  const QuadRule *q = quad_find(s->h.tdim, acc);
+  if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
+  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  HIPCHK(hipSetDevice(s->ctx->device));
+  DevBuf<double> fstore, Fe;
+  FieldDev fd = upload_field(s->ctx, fkind == AFE_F_VARCOEF ? coef : nullptr, fstore);
+  Fe.alloc(s->ctx, (size_t)s->h.n_elem * s->h.n_loc);
+  switch (s->h.kind) {
+  case P1_TRI: launch_loads<P1_TRI>(s, q, fkind, fd, d_nodal, Fe.p); break;
+  case P2_TRI: launch_loads<P2_TRI>(s, q, fkind, fd, d_nodal, Fe.p); break;
+  default: launch_loads<P1_TET>(s, q, fkind, fd, d_nodal, Fe.p); break;
+  }
+  k_gather_vec<<<nblk(s->h.n_dof), 256, 0, s->ctx->stream>>>(s->h.n_dof, s->h.n_loc, s->adj_ptr.p, s->adj.p,
+                                                              Fe.p, d_f);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  API_END
+}
+
+extern "C" int afe_apply_dirichlet(afe_matrix *a, double *d_u, double *d_f) {
+  API_BEGIN
+  (void)d_u;
+  if (!a || !a->s || !d_f) return fail(AFE_E_ARG, "matrix without FE space or null rhs");
+  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  HIPCHK(hipSetDevice(a->ctx->device));
+  afe_space *s = a->s;
+  k_dirichlet_zero<<<nblk(s->h.n_dof), 256, 0, a->ctx->stream>>>(s->h.n_dof, s->rp.p, s->col.p, s->dof_mark.p,
+                                                                  a->val.p, d_f);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(a->ctx->stream));
+  API_END
+}
+
+template <int KIND>
+static void launch_error(afe_space *s, const QuadRule *q, int h1, const double *uh, double *out) {
+  const int64_t ne = s->h.n_elem;
+  k_element_error<KIND><<<nblk(ne), 256, 0, s->ctx->stream>>>(ne, s->coords.p, s->ev.p, s->elem_dof.p,
+                                                               s->ctx->d_quad + q->off, q->n, h1, uh, out);
+  HIPCHK(hipGetLastError());
+}
+
+static int fe_error(afe_space *s, const double *d_uh, int acc, int h1, double *err) {
+  API_BEGIN
+  if (!s || !d_uh || !err) return fail(AFE_E_ARG, "null argument");
+  if (!s->ctx) return fail(AFE_E_ARG, "host-only space (no device context)");
+  const QuadRule *q = quad_find(s->h.tdim, acc);
+  if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
+  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  HIPCHK(hipSetDevice(s->ctx->device));
+  DevBuf<double> pe, part, tot;
+  pe.alloc(s->ctx, s->h.n_elem);
+  part.alloc(s->ctx, RED_BLOCKS);
+  tot.alloc(s->ctx, 1);
+  switch (s->h.kind) {
+  case P1_TRI: launch_error<P1_TRI>(s, q, h1, d_uh, pe.p); break;
+  case P2_TRI: launch_error<P2_TRI>(s, q, h1, d_uh, pe.p); break;
+  default: launch_error<P1_TET>(s, q, h1, d_uh, pe.p); break;
+  }
+  k_sum_partial<<<RED_BLOCKS, 256, 0, s->ctx->stream>>>(s->h.n_elem, pe.p, part.p);
+  k_finalize<<<1, 256, 0, s->ctx->stream>>>(part.p, RED_BLOCKS, 1, tot.p);
+  double h = 0;
+  HIPCHK(hipMemcpyAsync(&h, tot.p, sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  *err = std::sqrt(std::fabs(h));
+  API_END
+}
+
+extern "C" int afe_error_l2(afe_space *s, const double *d_uh, int32_t acc, double *err) {
+  return fe_error(s, d_uh, acc, 0, err);
+}
+extern "C" int afe_error_h1(afe_space *s, const double *d_uh, int32_t acc, double *err) {
+  return fe_error(s, d_uh, acc, 1, err);
+}
+
+// ===========================================================================
+// AMG hierarchy on the device
+// ===========================================================================
+constexpr int64_t TAIL_MAX_ROWS = 4096;
+
+struct DevLevel {
+  int64_t n = 0;
+  int ncol = 0;
+  std::vector<int32_t> colour_slice; // host copy
+  int ns = 0, p_ns = 0, t_ns = 0;
+  DevBuf<SliceInfo> si, p_si, t_si;
+  DevBuf<int32_t> cols, p_cols, t_cols, col_slice;
+  DevBuf<double> v64, invd64, diag64, p_v64, t_v64;
+  DevBuf<float> v32, invd32, diag32, p_v32, t_v32;
+  DevBuf<double> x64, b64, t64;
+  DevBuf<float> x32, b32, t32;
+  std::vector<int32_t> perm; // new -> old
+};
+
+struct afe_amg {
+  afe_ctx *ctx = nullptr;
+  afe_matrix *A0 = nullptr;
+  afe_amg_params prm{};
+  Hierarchy h;
+  std::vector<DevLevel> lv;
+  int tail0 = 0;       // first level handled by the single-workgroup tail
+  int coarse_mode = 0; // 0 dense inverse, 1 diagonal, 2 GS
+  DevBuf<double> minv;
+  DevBuf<LevelDev<double>> ld64;
+  DevBuf<LevelDev<float>> ld32;
+  DevBuf<int32_t> perm0;
+  // CG workspace (level-0 permuted numbering)
+  DevBuf<double> wx, wr, wp, wq, wz, wb, part;
+  DevBuf<CgScalars> sc;
+  CgScalars *h_sc = nullptr; // pinned
+  bool has32 = false;
+  // captured V-cycles: [precision][symmetric]
+  hipGraphExec_t vc_graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  hipGraphExec_t cg_graph[2] = {nullptr, nullptr};
+  int graph_nu[2][2] = {{-1, -1}, {-1, -1}};
+  int cg_nu[2] = {-1, -1};
+};
+
+static void drop_graphs(afe_amg *g) {
+  for (auto &row : g->vc_graph)
+    for (auto &e : row)
+      if (e) {
+        (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
+  for (auto &e : g->cg_graph)
+    if (e) {
+      (void)hipGraphExecDestroy(e);
+      e = nullptr;
+    }
+}
+
+// host CSR of the fine matrix (values from the device)
+static Csr fine_csr(afe_matrix *a) {
+  Csr c;
+  c.m = a->m;
+  c.n = a->n;
+  c.rp = a->hrp();
+  c.col = a->hcol();
+  c.val.resize(a->nnz);
+  HIPCHK(hipMemcpyAsync(c.val.data(), a->val.p, a->nnz * sizeof(double), hipMemcpyDeviceToHost, a->ctx->stream));
+  HIPCHK(hipStreamSynchronize(a->ctx->stream));
+  return c;
+}
+
+template <class T> static std::vector<T> to_prec(const std::vector<double> &v) {
+  return std::vector<T>(v.begin(), v.end());
+}
+
+static std::vector<double> inv_of(const std::vector<double> &d) {
+  std::vector<double> r(d.size());
+  for (size_t i = 0; i < d.size(); i++) r[i] = 1.0 / d[i];
+  return r;
+}
+
+// colour, permute and convert every level of the host hierarchy; upload
+static void upload_hierarchy(afe_amg *g) {
+  afe_ctx *ctx = g->ctx;
+  const Hierarchy &h = g->h;
+  const int L = (int)h.A.size();
+  drop_graphs(g);
+  g->lv.clear();
+  g->lv.resize(L);
+  std::vector<std::vector<int32_t>> perm(L), iperm(L), groups(L);
+  for (int l = 0; l < L; l++) {
+    const Csr &A = h.A[l];
+    std::vector<int32_t> colour;
+    const int ncol = colour_rows(A, colour);
+    std::vector<int32_t> len(A.m, 0);
+    for (int64_t i = 0; i < A.m; i++)
+      for (int64_t k = A.rp[i]; k < A.rp[i + 1]; k++)
+        if (A.col[k] != i && A.val[k] != 0.0) len[i]++;
+    perm[l].resize(A.m);
+    std::iota(perm[l].begin(), perm[l].end(), 0);
+    std::stable_sort(perm[l].begin(), perm[l].end(), [&](int32_t a, int32_t b) {
+      if (colour[a] != colour[b]) return colour[a] < colour[b];
+      return len[a] > len[b];
+    });
+    iperm[l].resize(A.m);
+    for (int64_t i = 0; i < A.m; i++) iperm[l][perm[l][i]] = (int32_t)i;
+    groups[l].assign(ncol, 0);
+    for (int64_t i = 0; i < A.m; i++) groups[l][colour[i]]++;
+    g->lv[l].ncol = ncol;
+    g->lv[l].n = A.m;
+  }
+  for (int l = 0; l < L; l++) {
+    DevLevel &d = g->lv[l];
+    Sell sa;
+    build_sell(h.A[l], perm[l], iperm[l], groups[l], true, sa);
+    d.ns = (int)sa.slices.size();
+    d.colour_slice = sa.group_slice;
+    d.si.upload(ctx, sa.slices);
+    d.cols.upload(ctx, sa.cols);
+    d.col_slice.upload(ctx, sa.group_slice);
+    d.v64.upload(ctx, sa.vals);
+    d.diag64.upload(ctx, sa.diag);
+    d.invd64.upload(ctx, inv_of(sa.diag));
+    if (g->has32) {
+      d.v32.upload(ctx, to_prec<float>(sa.vals));
+      d.diag32.upload(ctx, to_prec<float>(sa.diag));
+      d.invd32.upload(ctx, to_prec<float>(inv_of(sa.diag)));
+    }
+    if (l + 1 < L) {
+      Sell sp, st;
+      build_sell(h.P[l], perm[l + 1], iperm[l], {(int32_t)h.P[l].m}, false, sp);
+      build_sell(h.Pt[l].m ? h.Pt[l] : transpose(h.P[l]), perm[l], iperm[l + 1], {(int32_t)h.A[l].m}, false, st);
+      d.p_ns = (int)sp.slices.size();
+      d.t_ns = (int)st.slices.size();
+      d.p_si.upload(ctx, sp.slices);
+      d.p_cols.upload(ctx, sp.cols);
+      d.p_v64.upload(ctx, sp.vals);
+      d.t_si.upload(ctx, st.slices);
+      d.t_cols.upload(ctx, st.cols);
+      d.t_v64.upload(ctx, st.vals);
+      if (g->has32) {
+        d.p_v32.upload(ctx, to_prec<float>(sp.vals));
+        d.t_v32.upload(ctx, to_prec<float>(st.vals));
+      }
+    }
+    d.x64.alloc(ctx, d.n);
+    d.b64.alloc(ctx, d.n);
+    d.t64.alloc(ctx, d.n);
+    if (g->has32) {
+      d.x32.alloc(ctx, d.n);
+      d.b32.alloc(ctx, d.n);
+      d.t32.alloc(ctx, d.n);
+    }
+    d.perm = perm[l];
+  }
+  g->perm0.upload(ctx, perm[0]);
+  // first level of the coarse tail
+  g->tail0 = L - 1;
+  while (g->tail0 > 0 && g->lv[g->tail0 - 1].n <= TAIL_MAX_ROWS) g->tail0--;
+  g->coarse_mode = h.full ? 0 : 1;
+  if (h.full) g->minv.upload(ctx, h.minv);
+  // level descriptors for the tail kernel
+  std::vector<LevelDev<double>> d64(L);
+  std::vector<LevelDev<float>> d32(L);
+  for (int l = 0; l < L; l++) {
+    DevLevel &d = g->lv[l];
+    LevelDev<double> a{};
+    a.n = (int)d.n; a.ncol = d.ncol; a.ns = d.ns; a.p_ns = d.p_ns; a.t_ns = d.t_ns;
+    a.col_slice = d.col_slice.p; a.si = d.si.p; a.cols = d.cols.p;
+    a.vals = d.v64.p; a.invd = d.invd64.p; a.diag = d.diag64.p;
+    a.p_si = d.p_si.p; a.p_cols = d.p_cols.p; a.p_vals = d.p_v64.p;
+    a.t_si = d.t_si.p; a.t_cols = d.t_cols.p; a.t_vals = d.t_v64.p;
+    a.x = d.x64.p; a.b = d.b64.p; a.t = d.t64.p;
+    d64[l] = a;
+    LevelDev<float> b{};
+    b.n = a.n; b.ncol = a.ncol; b.ns = a.ns; b.p_ns = a.p_ns; b.t_ns = a.t_ns;
+    b.col_slice = a.col_slice; b.si = a.si; b.cols = a.cols;
+    b.vals = d.v32.p; b.invd = d.invd32.p; b.diag = d.diag32.p;
+    b.p_si = a.p_si; b.p_cols = a.p_cols; b.p_vals = d.p_v32.p;
+    b.t_si = a.t_si; b.t_cols = a.t_cols; b.t_vals = d.t_v32.p;
+    b.x = d.x32.p; b.b = d.b32.p; b.t = d.t32.p;
+    d32[l] = b;
+  }
+  g->ld64.upload(ctx, d64);
+  if (g->has32) g->ld32.upload(ctx, d32);
+  const int64_t n0 = g->lv[0].n;
+  g->wx.alloc(ctx, n0);
+  g->wr.alloc(ctx, n0);
+  g->wp.alloc(ctx, n0);
+  g->wq.alloc(ctx, n0);
+  g->wz.alloc(ctx, n0);
+  g->wb.alloc(ctx, n0);
+  g->part.alloc(ctx, 3 * RED_BLOCKS);
+  g->sc.alloc(ctx, 1);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+}
+
+extern "C" int afe_amg_create(afe_matrix *a, const afe_amg_params *p, afe_amg **out) {
+  API_BEGIN
+  if (!a || !out) return fail(AFE_E_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  HIPCHK(hipSetDevice(a->ctx->device));
+  auto g = std::make_unique<afe_amg>();
+  g->ctx = a->ctx;
+  g->A0 = a;
+  afe_amg_params prm{0.25, 50u, 0.382, 0, 3, 1};
+  if (p) prm = *p;
+  if (prm.smooth < 0 || prm.nmpo < 1) return fail(AFE_E_ARG, "bad AMG parameters");
+  g->prm = prm;
+  g->has32 = prm.build_fp32 != 0;
+  AmgParams hp;
+  hp.alpha = prm.alpha;
+  hp.nmpo = prm.nmpo;
+  hp.nmpsd = prm.nmpsd;
+  hp.lazy = prm.lazy;
+  Csr A0 = fine_csr(a);
+  amg_build(A0, hp, g->h);
+  HIPCHK(hipHostMalloc((void **)&g->h_sc, sizeof(CgScalars), hipHostMallocDefault));
+  upload_hierarchy(g.get());
+  *out = g.release();
+  API_END
+}
+
+extern "C" int afe_amg_create_host(int64_t m, const int64_t *row_ptr, const int64_t *col, const double *val,
+                                   const afe_amg_params *p, afe_amg **out) {
+  API_BEGIN
+  if (!row_ptr || !col || !val || !out || m <= 0) return fail(AFE_E_ARG, "null argument");
+  auto g = std::make_unique<afe_amg>();
+  afe_amg_params prm{0.25, 50u, 0.382, 0, 3, 0};
+  if (p) prm = *p;
+  g->prm = prm;
+  AmgParams hp;
+  hp.alpha = prm.alpha;
+  hp.nmpo = prm.nmpo;
+  hp.nmpsd = prm.nmpsd;
+  hp.lazy = prm.lazy;
+  Csr A0;
+  A0.m = A0.n = m;
+  A0.rp.assign(row_ptr, row_ptr + m + 1);
+  A0.col.resize(row_ptr[m]);
+  for (int64_t k = 0; k < row_ptr[m]; k++) {
+    if (col[k] < 0 || col[k] >= m) return fail(AFE_E_ARG, "column index out of range");
+    A0.col[k] = (int32_t)col[k];
+  }
+  A0.val.assign(val, val + row_ptr[m]);
+  amg_build(A0, hp, g->h);
+  *out = g.release();
+  API_END
+}
+
+extern "C" int afe_amg_lazy_reinit_host(afe_amg *g, const double *val) {
+  API_BEGIN
+  if (!g || !val || g->ctx) return fail(AFE_E_ARG, "not a host-only hierarchy");
+  if (!g->prm.lazy) return fail(AFE_E_ARG, "hierarchy was not built in lazy mode");
+  Csr A0 = g->h.A[0];
+  std::memcpy(A0.val.data(), val, sizeof(double) * A0.nnz());
+  amg_lazy_refresh(g->h, A0);
+  API_END
+}
+
+extern "C" int afe_amg_coarse_inverse(const afe_amg *g, double *minv) {
+  if (!g || !minv) return fail(AFE_E_ARG, "null argument");
+  if (!g->h.full) return fail(AFE_E_ARG, "coarsest level is not solved by a dense inverse");
+  std::memcpy(minv, g->h.minv.data(), sizeof(double) * g->h.minv.size());
+  return AFE_OK;
+}
+
+extern "C" int afe_amg_destroy(afe_amg *g) {
+  if (!g) return AFE_OK;
+  if (!g->ctx) {
+    delete g;
+    return AFE_OK;
+  }
+  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  drop_graphs(g);
+  if (g->h_sc) (void)hipHostFree(g->h_sc);
+  delete g;
+  return AFE_OK;
+}
+
+extern "C" int afe_amg_lazy_reinit(afe_amg *g, afe_matrix *a) {
+  API_BEGIN
+  if (!g || !a) return fail(AFE_E_ARG, "null argument");
+  if (!g->prm.lazy) return fail(AFE_E_ARG, "hierarchy was not built in lazy mode");
+  if (a->s != g->A0->s || (a->s == nullptr && a != g->A0))
+    return fail(AFE_E_PATTERN, "The solver is not initialized with the same sparsity pattern.");
+  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  HIPCHK(hipSetDevice(g->ctx->device));
+  g->A0 = a;
+  Csr A0 = fine_csr(a);
+  amg_lazy_refresh(g->h, A0);
+  upload_hierarchy(g);
+  API_END
+}
+
+extern "C" int afe_amg_info(const afe_amg *g, int32_t *n_levels, int64_t *level_n, int64_t *level_nnz,
+                            int32_t *coarse_full) {
+  if (!g) return fail(AFE_E_ARG, "null argument");
+  const int L = (int)g->h.A.size();
+  if (n_levels) *n_levels = L;
+  for (int l = 0; l < L; l++) {
+    if (level_n) level_n[l] = g->h.A[l].m;
+    if (level_nnz) level_nnz[l] = g->h.A[l].nnz();
+  }
+  if (coarse_full) *coarse_full = g->h.full;
+  return AFE_OK;
+}
+
+extern "C" int afe_amg_level_csr(const afe_amg *g, int32_t level, int32_t which, int64_t *m, int64_t *n,
+                                 int64_t *nnz, int64_t *row_ptr, int64_t *col, double *val) {
+  if (!g) return fail(AFE_E_ARG, "null argument");
+  const int L = (int)g->h.A.size();
+  if (level < 0 || level >= L || (which == 1 && level >= L - 1) || which < 0 || which > 1)
+    return fail(AFE_E_ARG, "no such level matrix");
+  const Csr &c = which == 0 ? g->h.A[level] : g->h.P[level];
+  if (m) *m = c.m;
+  if (n) *n = c.n;
+  if (nnz) *nnz = c.nnz();
+  if (row_ptr) std::memcpy(row_ptr, c.rp.data(), sizeof(int64_t) * (c.m + 1));
+  if (col)
+    for (int64_t k = 0; k < c.nnz(); k++) col[k] = c.col[k];
+  if (val) std::memcpy(val, c.val.data(), sizeof(double) * c.nnz());
+  return AFE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle issue (capturable: kernels only)
+// ---------------------------------------------------------------------------
+template <class T> struct Prec;
+template <> struct Prec<double> {
+  static const double *v(const DevLevel &d) { return d.v64.p; }
+  static const double *invd(const DevLevel &d) { return d.invd64.p; }
+  static const double *diag(const DevLevel &d) { return d.diag64.p; }
+  static const double *pv(const DevLevel &d) { return d.p_v64.p; }
+  static const double *tv(const DevLevel &d) { return d.t_v64.p; }
+  static double *x(DevLevel &d) { return d.x64.p; }
+  static double *b(DevLevel &d) { return d.b64.p; }
+  static double *t(DevLevel &d) { return d.t64.p; }
+  static const LevelDev<double> *ld(afe_amg *g) { return g->ld64.p; }
+};
+template <> struct Prec<float> {
+  static const float *v(const DevLevel &d) { return d.v32.p; }
+  static const float *invd(const DevLevel &d) { return d.invd32.p; }
+  static const float *diag(const DevLevel &d) { return d.diag32.p; }
+  static const float *pv(const DevLevel &d) { return d.p_v32.p; }
+  static const float *tv(const DevLevel &d) { return d.t_v32.p; }
+  static float *x(DevLevel &d) { return d.x32.p; }
+  static float *b(DevLevel &d) { return d.b32.p; }
+  static float *t(DevLevel &d) { return d.t32.p; }
+  static const LevelDev<float> *ld(afe_amg *g) { return g->ld32.p; }
+};
+
+template <class T>
+static void gs_colour(DevLevel &d, int c, hipStream_t st) {
+  const int s0 = d.colour_slice[c], s1 = d.colour_slice[c + 1];
+  if (s1 <= s0) return;
+  k_gs<T><<<nblk(s1 - s0, WPB), BLK, 0, st>>>(d.si.p, s0, s1, d.cols.p, Prec<T>::v(d), Prec<T>::invd(d),
+                                              Prec<T>::b(d), Prec<T>::x(d));
+}
+
+// one V(nu, nu) cycle on level-0 vectors (b, x) of precision T; the
+// post-smoother runs the colours in reverse order when symmetric
+template <class T>
+static void issue_vcycle(afe_amg *g, int nu, bool symmetric, hipStream_t st) {
+  const int Lc = (int)g->lv.size() - 1;
+  for (int l = 0; l < g->tail0; l++) {
+    DevLevel &d = g->lv[l];
+    DevLevel &dn = g->lv[l + 1];
+    for (int it = 0; it < nu; it++)
+      for (int c = 0; c < d.ncol; c++) gs_colour<T>(d, c, st);
+    k_residual<T><<<nblk(d.ns, WPB), BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, Prec<T>::v(d), Prec<T>::diag(d),
+                                                    Prec<T>::b(d), Prec<T>::x(d), Prec<T>::t(d));
+    k_restrict<T><<<nblk(d.p_ns, WPB), BLK, 0, st>>>(d.p_si.p, d.p_ns, d.p_cols.p, Prec<T>::pv(d),
+                                                      Prec<T>::t(d), Prec<T>::b(dn), Prec<T>::x(dn));
+  }
+  k_tail<T><<<1, TAIL_THREADS, 0, st>>>(Prec<T>::ld(g), g->tail0, Lc, g->minv.p, g->coarse_mode, nu, nu,
+                                         symmetric ? 1 : 0);
+  for (int l = g->tail0 - 1; l >= 0; l--) {
+    DevLevel &d = g->lv[l];
+    DevLevel &dn = g->lv[l + 1];
+    k_prolong<T><<<nblk(d.t_ns, WPB), BLK, 0, st>>>(d.t_si.p, d.t_ns, d.t_cols.p, Prec<T>::tv(d),
+                                                     Prec<T>::x(dn), Prec<T>::x(d));
+    for (int it = 0; it < nu; it++)
+      for (int cc = 0; cc < d.ncol; cc++) gs_colour<T>(d, symmetric ? d.ncol - 1 - cc : cc, st);
+  }
+}
+
+static void launch_graph_or_issue(afe_amg *g, int prec, bool symmetric, int nu) {
+  hipStream_t st = g->ctx->stream;
+  hipGraphExec_t &ex = g->vc_graph[prec][symmetric ? 1 : 0];
+  if (ex && g->graph_nu[prec][symmetric ? 1 : 0] != nu) {
+    (void)hipGraphExecDestroy(ex);
+    ex = nullptr;
+  }
+  if (!ex) {
+    hipGraph_t graph;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    if (prec == 1)
+      issue_vcycle<float>(g, nu, symmetric, st);
+    else
+      issue_vcycle<double>(g, nu, symmetric, st);
+    HIPCHK(hipStreamEndCapture(st, &graph));
+    HIPCHK(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(graph));
+    g->graph_nu[prec][symmetric ? 1 : 0] = nu;
+  }
+  HIPCHK(hipGraphLaunch(ex, st));
+}
+
+static void residual_norms(afe_amg *g, const double *b, const double *x, double *r) {
+  DevLevel &d = g->lv[0];
+  hipStream_t st = g->ctx->stream;
+  k_residual_norms<<<RED_BLOCKS, BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, d.v64.p, d.diag64.p, b, x, r, g->part.p);
+  k_finalize<<<1, 256, 0, st>>>(g->part.p, RED_BLOCKS, 3, &g->sc.p->rr);
+}
+
+static void read_scalars(afe_amg *g) {
+  HIPCHK(hipMemcpyAsync(g->h_sc, g->sc.p, sizeof(CgScalars), hipMemcpyDeviceToHost, g->ctx->stream));
+  HIPCHK(hipStreamSynchronize(g->ctx->stream));
+}
+
+extern "C" int afe_amg_solve(afe_amg *g, double *d_x, const double *d_r, double tol, int64_t step, int32_t mode,
+                             double rtol, afe_solve_stats *st) {
+  API_BEGIN
+  if (!g || !d_x || !d_r || step < 0) return fail(AFE_E_ARG, "bad solve arguments");
+  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  HIPCHK(hipSetDevice(g->ctx->device));
+  afe_ctx *ctx = g->ctx;
+  hipStream_t s = ctx->stream;
+  DevLevel &d0 = g->lv[0];
+  const int64_t n = d0.n;
+  if (tol == 0.0) tol = 1.0e-12; // AMGSolver::tolerence() default
+  HIPCHK(hipEventRecord(ctx->ev0, s));
+  k_permute_in<double, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, d_r, d0.b64.p);
+  k_permute_in<double, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, d_x, d0.x64.p);
+  residual_norms(g, d0.b64.p, d0.x64.p, nullptr);
+  read_scalars(g);
+  const double init_l1 = g->h_sc->l1, bb = g->h_sc->bb;
+  double rr = g->h_sc->rr, l1 = init_l1;
+  int64_t it = 0;
+  bool conv = false;
+  while (true) {
+    launch_graph_or_issue(g, 0, false, g->prm.smooth);
+    residual_norms(g, d0.b64.p, d0.x64.p, nullptr);
+    read_scalars(g);
+    rr = g->h_sc->rr;
+    l1 = g->h_sc->l1;
+    it++;
+    if (rtol > 0 && rr <= rtol * rtol * bb) {
+      conv = true;
+      break;
+    }
+    if (it > step) break;
+    if (mode & 2) fprintf(stderr, "\r\tresidual = %g", l1);
+  }
+  k_permute_out<double, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, d0.x64.p, d_x);
+  HIPCHK(hipEventRecord(ctx->ev1, s));
+  HIPCHK(hipEventSynchronize(ctx->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (rtol <= 0) conv = l1 < tol * init_l1;
+  if ((mode & 1) == 0)
+    fprintf(stderr, "\r\t%s with residual %g at step %lld.\n", conv ? "converge" : "failed to converge", l1,
+            (long long)it);
+  if (st) {
+    st->iterations = (int32_t)it;
+    st->converged = conv ? 1 : 0;
+    st->rel_res_l2 = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+    st->res_l1 = l1;
+    st->rel_res_l1 = init_l1 > 0 ? l1 / init_l1 : 0.0;
+    st->seconds = ms * 1e-3;
+  }
+  API_END
+}
+
+extern "C" int afe_amg_precondition(afe_amg *g, double *d_x, const double *d_b, int32_t iterate_step) {
+  afe_solve_stats st;
+  return afe_amg_solve(g, d_x, d_b, 0.0, iterate_step, 1, 0.0, &st);
+}
+
+// z = M^{-1} r with one symmetric V-cycle from zero, permuted numbering
+template <class T> static void precondition_perm(afe_amg *g, const double *r, bool first) {
+  DevLevel &d0 = g->lv[0];
+  hipStream_t s = g->ctx->stream;
+  const int64_t n = d0.n;
+  k_prep_vcycle<T><<<nblk(n), 256, 0, s>>>(n, r, Prec<T>::b(d0), Prec<T>::x(d0));
+  launch_graph_or_issue(g, std::is_same<T, float>::value ? 1 : 0, true, g->prm.smooth);
+  if (first) k_fill<double><<<nblk(n), 256, 0, s>>>(n, g->wz.p, 0.0);
+  k_take_z<T><<<RED_BLOCKS, 256, 0, s>>>(n, Prec<T>::x(d0), r, g->wz.p, g->part.p);
+  k_finalize<<<1, 256, 0, s>>>(g->part.p, RED_BLOCKS, 2, &g->sc.p->rz_new);
+  k_cg_beta<<<1, 64, 0, s>>>(g->sc.p, first ? 1 : 0);
+  k_p_update<<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wz.p, g->wp.p);
+}
+
+// one CG iteration after the direction p is set: q = A p, alpha, x/r update,
+// rr to the host slot, then the next preconditioned direction
+template <class T> static void issue_cg_iteration(afe_amg *g) {
+  DevLevel &d0 = g->lv[0];
+  hipStream_t s = g->ctx->stream;
+  const int64_t n = d0.n;
+  k_spmv_dot<<<RED_BLOCKS, BLK, 0, s>>>(d0.si.p, d0.ns, d0.cols.p, d0.v64.p, d0.diag64.p, g->wp.p, g->wq.p,
+                                         g->part.p);
+  k_finalize<<<1, 256, 0, s>>>(g->part.p, RED_BLOCKS, 1, &g->sc.p->pq);
+  k_cg_update<T><<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wx.p, g->wr.p, g->wp.p, g->wq.p, Prec<T>::b(d0),
+                                             Prec<T>::x(d0), g->part.p);
+  k_finalize<<<1, 256, 0, s>>>(g->part.p, RED_BLOCKS, 1, &g->sc.p->rr);
+  HIPCHK(hipMemcpyAsync(g->h_sc, g->sc.p, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
+  const int prec = std::is_same<T, float>::value ? 1 : 0;
+  // V-cycle issued inline (nested graph capture is not used)
+  issue_vcycle<T>(g, g->prm.smooth, true, s);
+  k_take_z<T><<<RED_BLOCKS, 256, 0, s>>>(n, Prec<T>::x(d0), g->wr.p, g->wz.p, g->part.p);
+  k_finalize<<<1, 256, 0, s>>>(g->part.p, RED_BLOCKS, 2, &g->sc.p->rz_new);
+  k_cg_beta<<<1, 64, 0, s>>>(g->sc.p, 0);
+  k_p_update<<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wz.p, g->wp.p);
+  (void)prec;
+}
+
+template <class T> static void launch_cg_iteration(afe_amg *g) {
+  const int prec = std::is_same<T, float>::value ? 1 : 0;
+  hipStream_t st = g->ctx->stream;
+  hipGraphExec_t &ex = g->cg_graph[prec];
+  if (ex && g->cg_nu[prec] != g->prm.smooth) {
+    (void)hipGraphExecDestroy(ex);
+    ex = nullptr;
+  }
+  if (!ex) {
+    hipGraph_t graph;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    issue_cg_iteration<T>(g);
+    HIPCHK(hipStreamEndCapture(st, &graph));
+    HIPCHK(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(graph));
+    g->cg_nu[prec] = g->prm.smooth;
+  }
+  HIPCHK(hipGraphLaunch(ex, st));
+}
+
+template <class T>
+static int pcg_impl(afe_amg *g, double *d_x, const double *d_b, double rtol, int max_it, afe_solve_stats *st) {
+  afe_ctx *ctx = g->ctx;
+  hipStream_t s = ctx->stream;
+  DevLevel &d0 = g->lv[0];
+  const int64_t n = d0.n;
+  HIPCHK(hipEventRecord(ctx->ev0, s));
+  k_permute_in<double, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, d_b, g->wb.p);
+  k_permute_in<double, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, d_x, g->wx.p);
+  residual_norms(g, g->wb.p, g->wx.p, g->wr.p);
+  read_scalars(g);
+  const double bb = g->h_sc->bb;
+  const double init_l1 = g->h_sc->l1;
+  double rr = g->h_sc->rr, l1 = init_l1;
+  const double thr = rtol * rtol * bb;
+  int it = 0;
+  bool conv = rr <= thr;
+  int restarts = 0;
+  while (!conv && it < max_it) {
+    k_fill<double><<<nblk(n), 256, 0, s>>>(n, g->wp.p, 0.0);
+    precondition_perm<T>(g, g->wr.p, true);
+    while (it < max_it) {
+      launch_cg_iteration<T>(g);
+      HIPCHK(hipStreamSynchronize(s));
+      it++;
+      rr = g->h_sc->rr;
+      if (!(rr == rr)) return fail(AFE_E_NOTCONVERGED, "CG breakdown (NaN residual)");
+      if (rr <= thr) break;
+    }
+    // true residual check; restart from it when the recurrence drifted
+    residual_norms(g, g->wb.p, g->wx.p, g->wr.p);
+    read_scalars(g);
+    rr = g->h_sc->rr;
+    l1 = g->h_sc->l1;
+    conv = rr <= thr;
+    if (!conv && ++restarts > 20) break;
+  }
+  k_permute_out<double, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, g->wx.p, d_x);
+  HIPCHK(hipEventRecord(ctx->ev1, s));
+  HIPCHK(hipEventSynchronize(ctx->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (st) {
+    st->iterations = it;
+    st->converged = conv ? 1 : 0;
+    st->rel_res_l2 = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+    st->res_l1 = l1;
+    st->rel_res_l1 = init_l1 > 0 ? l1 / init_l1 : 0.0;
+    st->seconds = ms * 1e-3;
+  }
+  if (!conv) return fail(AFE_E_NOTCONVERGED, "PCG did not reach the tolerance");
+  return AFE_OK;
+}
+
+extern "C" int afe_amg_pcg(afe_amg *g, double *d_x, const double *d_b, double rtol, int32_t max_it, int32_t prec,
+                           afe_solve_stats *st) {
+  API_BEGIN
+  if (!g || !d_x || !d_b || rtol <= 0 || max_it < 1) return fail(AFE_E_ARG, "bad PCG arguments");
+  if (prec == AFE_PREC_FP32 && !g->has32) return fail(AFE_E_ARG, "hierarchy built without FP32 copies");
+  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  HIPCHK(hipSetDevice(g->ctx->device));
+  return prec == AFE_PREC_FP32 ? pcg_impl<float>(g, d_x, d_b, rtol, max_it, st)
+                               : pcg_impl<double>(g, d_x, d_b, rtol, max_it, st);
+  API_END
+}
+
+extern "C" int afe_amg_vcycle(afe_amg *g, double *d_z, const double *d_r, int32_t prec) {
+  API_BEGIN
+  if (!g || !d_z || !d_r) return fail(AFE_E_ARG, "null argument");
+  if (prec == AFE_PREC_FP32 && !g->has32) return fail(AFE_E_ARG, "hierarchy built without FP32 copies");
+  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  HIPCHK(hipSetDevice(g->ctx->device));
+  hipStream_t s = g->ctx->stream;
+  DevLevel &d0 = g->lv[0];
+  const int64_t n = d0.n;
+  k_permute_in<double, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, d_r, g->wr.p);
+  if (prec == AFE_PREC_FP32) {
+    k_prep_vcycle<float><<<nblk(n), 256, 0, s>>>(n, g->wr.p, d0.b32.p, d0.x32.p);
+    launch_graph_or_issue(g, 1, true, g->prm.smooth);
+    k_permute_out<float, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, d0.x32.p, d_z);
+  } else {
+    k_prep_vcycle<double><<<nblk(n), 256, 0, s>>>(n, g->wr.p, d0.b64.p, d0.x64.p);
+    launch_graph_or_issue(g, 0, true, g->prm.smooth);
+    k_permute_out<double, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, d0.x64.p, d_z);
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  API_END
+}

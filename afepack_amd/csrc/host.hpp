@@ -1,0 +1,127 @@
+// host.hpp -- host-side structures of the assemble-and-solve path: synthetic
+// meshes, FE space (DOF numbering), the CSR pattern with its element gather
+// map, the AMG hierarchy (coarsening stays on the host and is uploaded once),
+// multicolouring and the sliced-ELL device layout.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace afe {
+
+enum { P1_TRI = 1, P2_TRI = 2, P1_TET = 3 };
+inline int kind_tdim(int k) { return k == P1_TET ? 3 : 2; }
+inline int kind_nloc(int k) { return k == P1_TRI ? 3 : (k == P2_TRI ? 6 : 4); }
+inline int kind_nvtx(int k) { return k == P1_TET ? 4 : 3; }
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+// splitmix64: the seeded stream behind every synthetic input
+struct Rng {
+  uint64_t s;
+  explicit Rng(uint64_t seed) : s(seed) {}
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+void mesh_square(int n, double jitter, uint64_t seed, double *coords,
+                 int32_t *ev, int32_t *vmark);
+void mesh_cube(int n, double jitter, uint64_t seed, double *coords, int32_t *ev,
+               int32_t *vmark);
+
+// CSR with the reference's layout: square matrices keep the diagonal first in
+// each row, the remaining columns ascending; rectangular ones fully ascending.
+struct Csr {
+  int64_t m = 0, n = 0;
+  std::vector<int64_t> rp;
+  std::vector<int32_t> col;
+  std::vector<double> val;
+  int64_t nnz() const { return rp.empty() ? 0 : rp[m]; }
+};
+
+struct Space {
+  int kind = 0, tdim = 0, n_loc = 0, n_vtx = 0;
+  int64_t n_vert = 0, n_elem = 0, n_dof = 0;
+  std::vector<double> coords;
+  std::vector<int32_t> ev, vmark;
+  std::vector<int32_t> elem_dof, dof_mark;
+  std::vector<double> dof_point;
+  // CSR pattern (reference numbering)
+  std::vector<int64_t> rp;
+  std::vector<int32_t> col;
+  int max_row = 0;
+  // element -> CSR gather map: for row i, the adjacent elements in ascending
+  // order as e*8 + local index, and per entry n_loc slot offsets in row i
+  std::vector<int64_t> adj_ptr;
+  std::vector<int32_t> adj;
+  std::vector<uint8_t> slot;
+};
+
+// FEMSpace::buildDof first-touch numbering + pattern + gather map
+void space_build(Space &s, int kind, int64_t n_vert, const double *coords,
+                 int64_t n_elem, const int32_t *ev, const int32_t *vmark);
+
+struct AmgParams {
+  double alpha = 0.25;
+  unsigned nmpo = 50;
+  double nmpsd = 0.382;
+  int lazy = 0;
+};
+
+// host hierarchy: A[0] is a copy of the fine matrix, P[l]: n_{l+1} x n_l,
+// Pt[l] its transpose (the reference keeps it for lazy mode)
+struct Hierarchy {
+  AmgParams prm;
+  std::vector<Csr> A, P, Pt;
+  int full = 1;
+  std::vector<double> minv; // coarsest dense inverse, row-major
+};
+
+void amg_build(const Csr &A0, const AmgParams &prm, Hierarchy &h);
+void amg_lazy_refresh(Hierarchy &h, const Csr &A0);
+Csr galerkin(const Csr &P, const Csr &A, const Csr &Pt);
+Csr transpose(const Csr &a);
+int dense_inverse(int n, double *a);
+
+// ---- multicolour sliced-ELL (device layout) -------------------------------
+// Rows are permuted so that each colour is a contiguous range, sorted by
+// decreasing stored length inside the colour; slices of SELL_C consecutive
+// rows never straddle a colour.  Element (row0+lane, j) of a slice lives at
+// off + j*SELL_C + lane.  Padding uses the row itself as column with value 0.
+constexpr int SELL_C = 64;
+
+struct SliceInfo {
+  int32_t row0, nrow, width, off;
+};
+
+struct Sell {
+  int64_t n_rows = 0, n_cols = 0;
+  std::vector<SliceInfo> slices;
+  std::vector<int32_t> group_slice; // slices of group g: [group_slice[g], group_slice[g+1])
+  std::vector<int32_t> group_row;   // rows of group g
+  std::vector<int32_t> cols;
+  std::vector<double> vals;
+  std::vector<double> diag; // only for square matrices stored without diagonal
+  int64_t stored() const { return (int64_t)cols.size(); }
+};
+
+// greedy first-fit colouring (index order) of the symmetrised graph of the
+// nonzero off-diagonal entries
+int colour_rows(const Csr &a, std::vector<int32_t> &colour);
+
+// perm: new -> old row order; col_map: old column -> new column
+// groups: row-count per group in new order (slices never cross groups)
+void build_sell(const Csr &a, const std::vector<int32_t> &perm,
+                const std::vector<int32_t> &col_map,
+                const std::vector<int32_t> &group_rows, bool split_diag,
+                Sell &out);
+
+} // namespace afe

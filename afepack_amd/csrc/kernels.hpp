@@ -1,0 +1,824 @@
+// kernels.hpp -- gfx950 kernels of the assemble-and-solve path.
+//
+// Element assembly (FP64, VALU), the element -> CSR gather (LDS accumulation,
+// ascending element order, no atomics), right-hand side, Dirichlet
+// elimination, error functionals, the multicolour sliced-ELL Gauss-Seidel /
+// SpMV family used by the V-cycle (templated on FP64 / FP32), the
+// single-workgroup coarse-tail V-cycle and the FP64 CG vector kernels.
+// Wavefront = 64 lanes; sliced-ELL slices are 64 rows, one row per lane.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "host.hpp"
+
+namespace afe {
+
+constexpr int WAVE = 64;
+constexpr int WPB = 4;             // wavefronts per block in the SpMV family
+constexpr int BLK = WAVE * WPB;    // 256 threads
+constexpr int RED_BLOCKS = 1024;   // fixed grid of the reduction kernels
+constexpr int TAIL_THREADS = 1024; // single-workgroup coarse tail
+
+struct FieldDev {
+  int n_modes, dim;
+  const double *k, *phi, *w;
+  double scale;
+};
+
+__device__ inline double field_g(const FieldDev &f, const double *x, double *grad) {
+  double g = 0.0;
+  if (grad)
+    for (int d = 0; d < f.dim; d++) grad[d] = 0.0;
+  for (int m = 0; m < f.n_modes; m++) {
+    double arg = f.phi[m];
+    for (int d = 0; d < f.dim; d++) arg += 2.0 * M_PI * f.k[m * f.dim + d] * x[d];
+    double sn, cs;
+    sincos(arg, &sn, &cs);
+    g += f.w[m] * sn;
+    if (grad) {
+      const double c = f.w[m] * cs;
+      for (int d = 0; d < f.dim; d++) grad[d] += c * 2.0 * M_PI * f.k[m * f.dim + d];
+    }
+  }
+  if (grad)
+    for (int d = 0; d < f.dim; d++) grad[d] *= f.scale;
+  return f.scale * g;
+}
+
+// ---------------------------------------------------------------------------
+// element geometry: barycentric gradients of an affine simplex
+// ---------------------------------------------------------------------------
+// triangle: dl[i] = rot90(v_{i+1} - v_{i+2}) / (2|T|) (triangle.1.bas_fun.c:13-17);
+// jacobian = signed doubled area / reference doubled area (1)
+__device__ inline void tri_geom(const double v[3][2], double dl[3][2], double &jac) {
+  const double ar = (v[1][0] - v[0][0]) * (v[2][1] - v[0][1]) -
+                    (v[1][1] - v[0][1]) * (v[2][0] - v[0][0]);
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const int a = (i + 1) % 3, b = (i + 2) % 3;
+    dl[i][0] = (v[a][1] - v[b][1]) / ar;
+    dl[i][1] = (v[b][0] - v[a][0]) / ar;
+  }
+  jac = ar / 1.0;
+}
+
+// tetrahedron: grad lambda_m = -+ (v_b - v_a) x (v_c - v_a) / det
+// (tetrahedron.1.bas_fun.c:19-26,60-95); jacobian = (det/6) / (1/6)
+__device__ inline void tet_geom(const double v[4][3], double dl[4][3], double &jac) {
+  const double a0 = v[1][0] - v[0][0], a1 = v[1][1] - v[0][1], a2 = v[1][2] - v[0][2];
+  const double b0 = v[2][0] - v[0][0], b1 = v[2][1] - v[0][1], b2 = v[2][2] - v[0][2];
+  const double c0 = v[3][0] - v[0][0], c1 = v[3][1] - v[0][1], c2 = v[3][2] - v[0][2];
+  const double det = a0 * b1 * c2 + a1 * b2 * c0 + a2 * b0 * c1 - a0 * b2 * c1 -
+                     a1 * b0 * c2 - a2 * b1 * c0;
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const int ia = (m + 1) % 4, ib = (m + 2) % 4, ic = (m + 3) % 4;
+    const double sgn = (m % 2 == 0) ? -1. : 1.;
+#pragma unroll
+    for (int n = 0; n < 3; n++) {
+      const int n1 = (n + 1) % 3, n2 = (n + 2) % 3;
+      const double cof = sgn * ((v[ib][n1] - v[ia][n1]) * (v[ic][n2] - v[ia][n2]) -
+                                (v[ib][n2] - v[ia][n2]) * (v[ic][n1] - v[ia][n1]));
+      dl[m][n] = cof / det;
+    }
+  }
+  jac = (det / 6.) / (1. / 6.);
+}
+
+// basis values / gradients at barycentric coordinates lam (local order of
+// triangle.1 / triangle.2 / tetrahedron.1 .bas_fun)
+template <int KIND>
+__device__ inline void basis_val(const double *lam, double *val) {
+  if constexpr (KIND == P1_TRI) {
+    val[0] = lam[0]; val[1] = lam[1]; val[2] = lam[2];
+  } else if constexpr (KIND == P2_TRI) {
+    val[0] = lam[0] * (2 * lam[0] - 1.0);
+    val[1] = lam[1] * (2 * lam[1] - 1.0);
+    val[2] = lam[2] * (2 * lam[2] - 1.0);
+    val[3] = 4.0 * lam[1] * lam[2];
+    val[4] = 4.0 * lam[0] * lam[2];
+    val[5] = 4.0 * lam[0] * lam[1];
+  } else {
+    val[0] = lam[0]; val[1] = lam[1]; val[2] = lam[2]; val[3] = lam[3];
+  }
+}
+
+template <int KIND, int D>
+__device__ inline void basis_grad(const double *lam, const double dl[][D], double g[][D]) {
+  if constexpr (KIND == P2_TRI) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int d = 0; d < 2; d++) g[i][d] = (4.0 * lam[i] - 1.0) * dl[i][d];
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+      g[3][d] = 4.0 * (lam[2] * dl[1][d] + lam[1] * dl[2][d]);
+      g[4][d] = 4.0 * (lam[0] * dl[2][d] + lam[2] * dl[0][d]);
+      g[5][d] = 4.0 * (lam[1] * dl[0][d] + lam[0] * dl[1][d]);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < D + 1; i++)
+#pragma unroll
+      for (int d = 0; d < D; d++) g[i][d] = dl[i][d];
+  }
+}
+
+template <int KIND> struct KindInfo;
+template <> struct KindInfo<P1_TRI> { static constexpr int D = 2, NV = 3, NL = 3; };
+template <> struct KindInfo<P2_TRI> { static constexpr int D = 2, NV = 3, NL = 6; };
+template <> struct KindInfo<P1_TET> { static constexpr int D = 3, NV = 4, NL = 4; };
+
+template <int KIND>
+__device__ inline void load_elem(int64_t e, const double *__restrict__ coords,
+                                 const int32_t *__restrict__ ev, double v[][KindInfo<KIND>::D]) {
+  constexpr int D = KindInfo<KIND>::D, NV = KindInfo<KIND>::NV;
+#pragma unroll
+  for (int j = 0; j < NV; j++) {
+    const int64_t p = ev[e * NV + j];
+#pragma unroll
+    for (int d = 0; d < D; d++) v[j][d] = coords[p * D + d];
+  }
+}
+
+template <int KIND>
+__device__ inline double geom(const double v[][KindInfo<KIND>::D], double dl[][KindInfo<KIND>::D]) {
+  double jac;
+  if constexpr (KindInfo<KIND>::D == 2)
+    tri_geom(v, dl, jac);
+  else
+    tet_geom(v, dl, jac);
+  return jac;
+}
+
+// reference-simplex volume (triangle.volume.c, tetrahedron.volume.c)
+template <int D> __device__ constexpr double ref_volume() { return D == 2 ? 0.5 : 1. / 6.; }
+
+// ---------------------------------------------------------------------------
+// element matrices: StiffMatrix / MassMatrix / variable-coefficient stiffness
+// (BilinearOperator.templates.h:342-406, EllipticEquation.cpp:65-108):
+//   K(j,k) += w_l * jac * |ref| [* a(x_l)] * <g_j, g_k>   (or * v_j v_k)
+// one thread per element, K row-major per element (NL*NL doubles)
+// ---------------------------------------------------------------------------
+template <int KIND, int OP>
+__global__ __launch_bounds__(256) void k_element_matrices(
+    int64_t n_elem, const double *__restrict__ coords, const int32_t *__restrict__ ev,
+    const double *__restrict__ qd, int nq, FieldDev fld, double *__restrict__ Ke) {
+  constexpr int D = KindInfo<KIND>::D, NL = KindInfo<KIND>::NL;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_elem) return;
+  double v[D + 1][D], dl[D + 1][D];
+  load_elem<KIND>(e, coords, ev, v);
+  const double jac = geom<KIND>(v, dl);
+  double K[NL][NL];
+#pragma unroll
+  for (int j = 0; j < NL; j++)
+#pragma unroll
+    for (int k = 0; k < NL; k++) K[j][k] = 0.0;
+  for (int l = 0; l < nq; l++) {
+    const double *q = qd + l * (D + 1);
+    double lam[D + 1];
+    lam[0] = 1.0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      lam[d + 1] = q[d];
+      lam[0] -= q[d];
+    }
+    const double Jxw = q[D] * jac * ref_volume<D>();
+    if constexpr (OP == 1) { // mass
+      double val[NL];
+      basis_val<KIND>(lam, val);
+#pragma unroll
+      for (int j = 0; j < NL; j++)
+#pragma unroll
+        for (int k = j; k < NL; k++) K[j][k] += Jxw * val[j] * val[k];
+    } else {
+      double g[NL][D];
+      basis_grad<KIND, D>(lam, dl, g);
+      double coef = 1.0;
+      if constexpr (OP == 2) {
+        double x[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+          x[d] = 0.0;
+#pragma unroll
+          for (int j = 0; j <= D; j++) x[d] += lam[j] * v[j][d];
+        }
+        coef = exp(field_g(fld, x, nullptr));
+      }
+#pragma unroll
+      for (int j = 0; j < NL; j++)
+#pragma unroll
+        for (int k = j; k < NL; k++) {
+          double ip = 0;
+#pragma unroll
+          for (int d = 0; d < D; d++) ip += g[j][d] * g[k][d];
+          if constexpr (OP == 2)
+            K[j][k] += Jxw * coef * ip;
+          else
+            K[j][k] += Jxw * ip;
+        }
+    }
+  }
+  double *out = Ke + e * (NL * NL);
+#pragma unroll
+  for (int j = 0; j < NL; j++)
+#pragma unroll
+    for (int k = 0; k < NL; k++) out[j * NL + k] = (k >= j) ? K[j][k] : K[k][j];
+}
+
+// ---------------------------------------------------------------------------
+// element -> CSR gather: one thread per row, 256 rows per block; the block's
+// contiguous slice of the value array is accumulated in LDS (each slot in
+// ascending element order, the summation order of buildSparseMatrix) and
+// written back coalesced.  NL is the element DOF count.
+// ---------------------------------------------------------------------------
+template <int NL>
+__global__ __launch_bounds__(256) void k_gather(
+    int64_t n_rows, const int64_t *__restrict__ rp, const int64_t *__restrict__ adj_ptr,
+    const int32_t *__restrict__ adj, const uint8_t *__restrict__ slot,
+    const double *__restrict__ Ke, double *__restrict__ val) {
+  extern __shared__ double seg[];
+  const int64_t r0 = (int64_t)blockIdx.x * 256;
+  const int64_t r1 = r0 + 256 < n_rows ? r0 + 256 : n_rows;
+  const int64_t base = rp[r0];
+  const int64_t segn = rp[r1] - base;
+  for (int64_t k = threadIdx.x; k < segn; k += 256) seg[k] = 0.0;
+  __syncthreads();
+  const int64_t i = r0 + threadIdx.x;
+  if (i < r1) {
+    double *row = seg + (rp[i] - base);
+    for (int64_t p = adj_ptr[i]; p < adj_ptr[i + 1]; p++) {
+      const int32_t ea = adj[p];
+      const double *kr = Ke + ((int64_t)(ea >> 3) * NL + (ea & 7)) * NL;
+      const uint8_t *sl = slot + p * NL;
+#pragma unroll
+      for (int b = 0; b < NL; b++) row[sl[b]] += kr[b];
+    }
+  }
+  __syncthreads();
+  for (int64_t k = threadIdx.x; k < segn; k += 256) val[base + k] = seg[k];
+}
+
+// ---------------------------------------------------------------------------
+// right-hand side: element load vectors, then a per-DOF gather in ascending
+// element order (Operator::L2Discretize, Operator.discretize.templates.h:80-152)
+// ---------------------------------------------------------------------------
+template <int KIND>
+__global__ __launch_bounds__(256) void k_element_loads(
+    int64_t n_elem, const double *__restrict__ coords, const int32_t *__restrict__ ev,
+    const int32_t *__restrict__ edof, const double *__restrict__ qd, int nq, int fkind,
+    FieldDev fld, const double *__restrict__ nodal, double *__restrict__ Fe) {
+  constexpr int D = KindInfo<KIND>::D, NL = KindInfo<KIND>::NL;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_elem) return;
+  double v[D + 1][D], dl[D + 1][D];
+  load_elem<KIND>(e, coords, ev, v);
+  const double jac = geom<KIND>(v, dl);
+  double F[NL];
+#pragma unroll
+  for (int j = 0; j < NL; j++) F[j] = 0.0;
+  for (int l = 0; l < nq; l++) {
+    const double *q = qd + l * (D + 1);
+    double lam[D + 1];
+    lam[0] = 1.0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      lam[d + 1] = q[d];
+      lam[0] -= q[d];
+    }
+    double x[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      x[d] = 0.0;
+#pragma unroll
+      for (int j = 0; j <= D; j++) x[d] += lam[j] * v[j][d];
+    }
+    double val[NL];
+    basis_val<KIND>(lam, val);
+    double f;
+    if (fkind == 3) {
+      f = 1.0;
+    } else if (fkind == 2) {
+      f = 0.0;
+#pragma unroll
+      for (int j = 0; j < NL; j++) f += nodal[edof[e * NL + j]] * val[j];
+    } else {
+      double u = 1.0, gu[D];
+#pragma unroll
+      for (int d = 0; d < D; d++) u *= sin(M_PI * x[d]);
+      if (fkind == 0) {
+        f = D * M_PI * M_PI * u;
+      } else {
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+          double gd = M_PI * cos(M_PI * x[d]);
+#pragma unroll
+          for (int c = 0; c < D; c++)
+            if (c != d) gd *= sin(M_PI * x[c]);
+          gu[d] = gd;
+        }
+        double gg[3];
+        const double g = field_g(fld, x, gg);
+        double dot = 0;
+#pragma unroll
+        for (int d = 0; d < D; d++) dot += gg[d] * gu[d];
+        f = exp(g) * (D * M_PI * M_PI * u - dot);
+      }
+    }
+    const double Jxw = q[D] * jac * ref_volume<D>();
+#pragma unroll
+    for (int j = 0; j < NL; j++) F[j] += Jxw * f * val[j];
+  }
+#pragma unroll
+  for (int j = 0; j < NL; j++) Fe[e * NL + j] = F[j];
+}
+
+__global__ __launch_bounds__(256) void k_gather_vec(int64_t n, int nl, const int64_t *__restrict__ adj_ptr,
+                                                    const int32_t *__restrict__ adj,
+                                                    const double *__restrict__ Fe, double *__restrict__ f) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int64_t p = adj_ptr[i]; p < adj_ptr[i + 1]; p++) {
+    const int32_t ea = adj[p];
+    s += Fe[(int64_t)(ea >> 3) * nl + (ea & 7)];
+  }
+  f[i] = s;
+}
+
+// BoundaryConditionAdmin::apply with zero Dirichlet data
+// (FEMSpace.templates.h:1338-1373): a marked row keeps only its diagonal and
+// gets f_i = A_ii * 0; the eliminated column entries A_ki are zeroed (the
+// update f_k -= A_ki * 0 / A_ii leaves f_k unchanged).
+__global__ __launch_bounds__(256) void k_dirichlet_zero(int64_t n, const int64_t *__restrict__ rp,
+                                                        const int32_t *__restrict__ col,
+                                                        const int32_t *__restrict__ mark,
+                                                        double *__restrict__ val, double *__restrict__ f) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (mark[i] != 0) {
+    f[i] = val[rp[i]] * 0.0;
+    for (int64_t j = rp[i] + 1; j < rp[i + 1]; j++) val[j] = 0.0;
+  } else {
+    for (int64_t j = rp[i] + 1; j < rp[i + 1]; j++)
+      if (mark[col[j]] != 0) val[j] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// error functionals (Functional.templates.h:296-317,399-426): per-element
+// contributions, then a fixed-order reduction
+// ---------------------------------------------------------------------------
+template <int KIND>
+__global__ __launch_bounds__(256) void k_element_error(
+    int64_t n_elem, const double *__restrict__ coords, const int32_t *__restrict__ ev,
+    const int32_t *__restrict__ edof, const double *__restrict__ qd, int nq, int h1,
+    const double *__restrict__ uh, double *__restrict__ out) {
+  constexpr int D = KindInfo<KIND>::D, NL = KindInfo<KIND>::NL;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_elem) return;
+  double v[D + 1][D], dl[D + 1][D];
+  load_elem<KIND>(e, coords, ev, v);
+  const double jac = geom<KIND>(v, dl);
+  double u_loc[NL];
+#pragma unroll
+  for (int j = 0; j < NL; j++) u_loc[j] = uh[edof[e * NL + j]];
+  double err = 0.0;
+  for (int l = 0; l < nq; l++) {
+    const double *q = qd + l * (D + 1);
+    double lam[D + 1];
+    lam[0] = 1.0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      lam[d + 1] = q[d];
+      lam[0] -= q[d];
+    }
+    double x[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      x[d] = 0.0;
+#pragma unroll
+      for (int j = 0; j <= D; j++) x[d] += lam[j] * v[j][d];
+    }
+    const double Jxw = q[D] * jac * ref_volume<D>();
+    if (!h1) {
+      double val[NL];
+      basis_val<KIND>(lam, val);
+      double uhv = 0.0, ue = 1.0;
+#pragma unroll
+      for (int j = 0; j < NL; j++) uhv += u_loc[j] * val[j];
+#pragma unroll
+      for (int d = 0; d < D; d++) ue *= sin(M_PI * x[d]);
+      err += Jxw * (ue - uhv) * (ue - uhv);
+    } else {
+      double g[NL][D];
+      basis_grad<KIND, D>(lam, dl, g);
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        double gh = 0.0;
+#pragma unroll
+        for (int j = 0; j < NL; j++) gh += u_loc[j] * g[j][d];
+        double ge = M_PI * cos(M_PI * x[d]);
+#pragma unroll
+        for (int c = 0; c < D; c++)
+          if (c != d) ge *= sin(M_PI * x[c]);
+        err += Jxw * (gh - ge) * (gh - ge);
+      }
+    }
+  }
+  out[e] = err;
+}
+
+// ---------------------------------------------------------------------------
+// fixed-order reductions
+// ---------------------------------------------------------------------------
+__device__ inline double block_sum(double v, double *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// partial sums of x over a fixed grid (grid-stride), one per block
+__global__ __launch_bounds__(256) void k_sum_partial(int64_t n, const double *__restrict__ x,
+                                                     double *__restrict__ part) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += x[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// out[k] = sum of part[k*stride + 0 .. np) for k < nsum, one block
+__global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ part, int np, int nsum,
+                                                  double *__restrict__ out) {
+  __shared__ double sh[256];
+  for (int k = 0; k < nsum; k++) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[k * np + i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[k] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sliced-ELL kernels (one wavefront per slice, one lane per row)
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ inline T sell_dot(const SliceInfo &s, int lane, const int32_t *__restrict__ cols,
+                             const T *__restrict__ vals, const T *__restrict__ x) {
+  const int32_t *c = cols + s.off + lane;
+  const T *v = vals + s.off + lane;
+  T acc = 0;
+  int j = 0;
+  for (; j + 4 <= s.width; j += 4) {
+    const int32_t c0 = c[(j + 0) * WAVE], c1 = c[(j + 1) * WAVE];
+    const int32_t c2 = c[(j + 2) * WAVE], c3 = c[(j + 3) * WAVE];
+    const T v0 = v[(j + 0) * WAVE], v1 = v[(j + 1) * WAVE];
+    const T v2 = v[(j + 2) * WAVE], v3 = v[(j + 3) * WAVE];
+    acc += v0 * x[c0];
+    acc += v1 * x[c1];
+    acc += v2 * x[c2];
+    acc += v3 * x[c3];
+  }
+  for (; j < s.width; j++) acc += v[j * WAVE] * x[c[j * WAVE]];
+  return acc;
+}
+
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE); }
+
+// Gauss-Seidel over the slices [s0, s1) of one colour:
+// x_r = (b_r - sum_{j != r} a_rj x_j) / a_rr
+template <class T>
+__global__ __launch_bounds__(BLK) void k_gs(const SliceInfo *__restrict__ si, int s0, int s1,
+                                            const int32_t *__restrict__ cols, const T *__restrict__ vals,
+                                            const T *__restrict__ invd, const T *__restrict__ b,
+                                            T *__restrict__ x) {
+  const int s = s0 + blockIdx.x * WPB + wave_id();
+  if (s >= s1) return;
+  const SliceInfo q = si[s];
+  const int lane = threadIdx.x % WAVE;
+  if (lane >= q.nrow) return;
+  const int r = q.row0 + lane;
+  const T acc = sell_dot<T>(q, lane, cols, vals, x);
+  x[r] = (b[r] - acc) * invd[r];
+}
+
+// t = b - A x (A = diag + stored off-diagonals)
+template <class T>
+__global__ __launch_bounds__(BLK) void k_residual(const SliceInfo *__restrict__ si, int ns,
+                                                  const int32_t *__restrict__ cols, const T *__restrict__ vals,
+                                                  const T *__restrict__ diag, const T *__restrict__ b,
+                                                  const T *__restrict__ x, T *__restrict__ t) {
+  const int s = blockIdx.x * WPB + wave_id();
+  if (s >= ns) return;
+  const SliceInfo q = si[s];
+  const int lane = threadIdx.x % WAVE;
+  if (lane >= q.nrow) return;
+  const int r = q.row0 + lane;
+  const T acc = sell_dot<T>(q, lane, cols, vals, x);
+  t[r] = b[r] - diag[r] * x[r] - acc;
+}
+
+// restriction: bc = P t, and the coarse guess xc = 0
+template <class T>
+__global__ __launch_bounds__(BLK) void k_restrict(const SliceInfo *__restrict__ si, int ns,
+                                                  const int32_t *__restrict__ cols, const T *__restrict__ vals,
+                                                  const T *__restrict__ t, T *__restrict__ bc,
+                                                  T *__restrict__ xc) {
+  const int s = blockIdx.x * WPB + wave_id();
+  if (s >= ns) return;
+  const SliceInfo q = si[s];
+  const int lane = threadIdx.x % WAVE;
+  if (lane >= q.nrow) return;
+  const int r = q.row0 + lane;
+  bc[r] = sell_dot<T>(q, lane, cols, vals, t);
+  xc[r] = 0;
+}
+
+// prolongation: x += P^T xc (explicit transpose, no atomics)
+template <class T>
+__global__ __launch_bounds__(BLK) void k_prolong(const SliceInfo *__restrict__ si, int ns,
+                                                 const int32_t *__restrict__ cols, const T *__restrict__ vals,
+                                                 const T *__restrict__ xc, T *__restrict__ x) {
+  const int s = blockIdx.x * WPB + wave_id();
+  if (s >= ns) return;
+  const SliceInfo q = si[s];
+  const int lane = threadIdx.x % WAVE;
+  if (lane >= q.nrow) return;
+  const int r = q.row0 + lane;
+  x[r] += sell_dot<T>(q, lane, cols, vals, xc);
+}
+
+// ---------------------------------------------------------------------------
+// single-workgroup V-cycle over the small levels l0..lc
+// ---------------------------------------------------------------------------
+template <class T> struct LevelDev {
+  int n, ncol, ns, p_ns, t_ns;
+  const int32_t *col_slice; // ncol+1 slice bounds
+  const SliceInfo *si;
+  const int32_t *cols;
+  const T *vals, *invd, *diag;
+  const SliceInfo *p_si; // restriction to level+1
+  const int32_t *p_cols;
+  const T *p_vals;
+  const SliceInfo *t_si; // prolongation from level+1
+  const int32_t *t_cols;
+  const T *t_vals;
+  T *x, *b, *t;
+};
+
+template <class T>
+__device__ inline void tail_sweep(const LevelDev<T> &d, int c, int w, int nw, int lane) {
+  for (int s = d.col_slice[c] + w; s < d.col_slice[c + 1]; s += nw) {
+    const SliceInfo q = d.si[s];
+    if (lane < q.nrow) {
+      const int r = q.row0 + lane;
+      const T acc = sell_dot<T>(q, lane, d.cols, d.vals, d.x);
+      d.x[r] = (d.b[r] - acc) * d.invd[r];
+    }
+  }
+}
+
+// coarse_mode: 0 dense inverse (FP64), 1 diagonal scaling, 2 GS sweeps
+template <class T>
+__global__ __launch_bounds__(TAIL_THREADS) void k_tail(const LevelDev<T> *__restrict__ L, int l0, int lc,
+                                                       const double *__restrict__ minv, int coarse_mode,
+                                                       int nu_pre, int nu_post, int post_reverse) {
+  const int nw = blockDim.x / WAVE;
+  const int w = wave_id();
+  const int lane = threadIdx.x % WAVE;
+  for (int l = l0; l < lc; l++) {
+    const LevelDev<T> d = L[l];
+    for (int it = 0; it < nu_pre; it++)
+      for (int c = 0; c < d.ncol; c++) {
+        tail_sweep(d, c, w, nw, lane);
+        __syncthreads();
+      }
+    for (int s = w; s < d.ns; s += nw) {
+      const SliceInfo q = d.si[s];
+      if (lane < q.nrow) {
+        const int r = q.row0 + lane;
+        d.t[r] = d.b[r] - d.diag[r] * d.x[r] - sell_dot<T>(q, lane, d.cols, d.vals, d.x);
+      }
+    }
+    __syncthreads();
+    const LevelDev<T> dn = L[l + 1];
+    for (int s = w; s < d.p_ns; s += nw) {
+      const SliceInfo q = d.p_si[s];
+      if (lane < q.nrow) {
+        const int r = q.row0 + lane;
+        dn.b[r] = sell_dot<T>(q, lane, d.p_cols, d.p_vals, d.t);
+        dn.x[r] = 0;
+      }
+    }
+    __syncthreads();
+  }
+  {
+    const LevelDev<T> d = L[lc];
+    if (coarse_mode == 0) {
+      for (int i = threadIdx.x; i < d.n; i += blockDim.x) {
+        double s = 0.0;
+        for (int j = 0; j < d.n; j++) s += minv[(int64_t)i * d.n + j] * (double)d.b[j];
+        d.x[i] = (T)s;
+      }
+    } else if (coarse_mode == 1) {
+      for (int i = threadIdx.x; i < d.n; i += blockDim.x) d.x[i] = d.b[i] * d.invd[i];
+    } else {
+      for (int it = 0; it < nu_pre; it++)
+        for (int c = 0; c < d.ncol; c++) {
+          tail_sweep(d, c, w, nw, lane);
+          __syncthreads();
+        }
+    }
+    __syncthreads();
+  }
+  for (int l = lc - 1; l >= l0; l--) {
+    const LevelDev<T> d = L[l];
+    const LevelDev<T> dn = L[l + 1];
+    for (int s = w; s < d.t_ns; s += nw) {
+      const SliceInfo q = d.t_si[s];
+      if (lane < q.nrow) {
+        const int r = q.row0 + lane;
+        d.x[r] += sell_dot<T>(q, lane, d.t_cols, d.t_vals, dn.x);
+      }
+    }
+    __syncthreads();
+    for (int it = 0; it < nu_post; it++)
+      for (int cc = 0; cc < d.ncol; cc++) {
+        const int c = post_reverse ? d.ncol - 1 - cc : cc;
+        tail_sweep(d, c, w, nw, lane);
+        __syncthreads();
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// vector kernels (level-0 numbering after the colour permutation)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_fill(int64_t n, T *__restrict__ x, T v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+template <class S, class D>
+__global__ __launch_bounds__(256) void k_permute_in(int64_t n, const int32_t *__restrict__ perm,
+                                                    const S *__restrict__ src, D *__restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (D)src[perm[i]];
+}
+
+template <class S, class D>
+__global__ __launch_bounds__(256) void k_permute_out(int64_t n, const int32_t *__restrict__ perm,
+                                                     const S *__restrict__ src, D *__restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[perm[i]] = (D)src[i];
+}
+
+// y = A x (FP64, level 0) and the partial dots (x, y) per block; grid-stride
+// over slices with a fixed grid so the partial count is RED_BLOCKS
+__global__ __launch_bounds__(BLK) void k_spmv_dot(const SliceInfo *__restrict__ si, int ns,
+                                                  const int32_t *__restrict__ cols,
+                                                  const double *__restrict__ vals,
+                                                  const double *__restrict__ diag,
+                                                  const double *__restrict__ x, double *__restrict__ y,
+                                                  double *__restrict__ part) {
+  __shared__ double sh[BLK];
+  const int lane = threadIdx.x % WAVE;
+  double dot = 0.0;
+  for (int s = blockIdx.x * WPB + wave_id(); s < ns; s += gridDim.x * WPB) {
+    const SliceInfo q = si[s];
+    if (lane < q.nrow) {
+      const int r = q.row0 + lane;
+      const double yr = diag[r] * x[r] + sell_dot<double>(q, lane, cols, vals, x);
+      y[r] = yr;
+      dot += x[r] * yr;
+    }
+  }
+  dot = block_sum(dot, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = dot;
+}
+
+// r = b - A x and the partials of (r, r), (b, b), sum |r| (three arrays)
+__global__ __launch_bounds__(BLK) void k_residual_norms(const SliceInfo *__restrict__ si, int ns,
+                                                        const int32_t *__restrict__ cols,
+                                                        const double *__restrict__ vals,
+                                                        const double *__restrict__ diag,
+                                                        const double *__restrict__ b,
+                                                        const double *__restrict__ x, double *__restrict__ r,
+                                                        double *__restrict__ part) {
+  __shared__ double sh[BLK];
+  const int lane = threadIdx.x % WAVE;
+  double rr = 0.0, bb = 0.0, l1 = 0.0;
+  for (int s = blockIdx.x * WPB + wave_id(); s < ns; s += gridDim.x * WPB) {
+    const SliceInfo q = si[s];
+    if (lane < q.nrow) {
+      const int row = q.row0 + lane;
+      const double rv = b[row] - diag[row] * x[row] - sell_dot<double>(q, lane, cols, vals, x);
+      if (r) r[row] = rv;
+      rr += rv * rv;
+      bb += b[row] * b[row];
+      l1 += fabs(rv);
+    }
+  }
+  rr = block_sum(rr, sh);
+  bb = block_sum(bb, sh);
+  l1 = block_sum(l1, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = rr;
+    part[gridDim.x + blockIdx.x] = bb;
+    part[2 * gridDim.x + blockIdx.x] = l1;
+  }
+}
+
+// device scalars of the CG loop; k_finalize writes consecutive fields
+// (rz_new, rz_cross) and (rr, bb, l1) in one call
+struct CgScalars {
+  double rz, pq, rz_new, rz_cross, rr, bb, l1, beta;
+};
+
+// alpha = rz / pq; x += alpha p; r -= alpha q; (T) r -> b0, x0 = 0; partial (r, r)
+template <class T>
+__global__ __launch_bounds__(256) void k_cg_update(int64_t n, const CgScalars *__restrict__ sc,
+                                                   double *__restrict__ x, double *__restrict__ r,
+                                                   const double *__restrict__ p, const double *__restrict__ q,
+                                                   T *__restrict__ b0, T *__restrict__ x0,
+                                                   double *__restrict__ part) {
+  __shared__ double sh[256];
+  const double alpha = sc->rz / sc->pq;
+  double rr = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    b0[i] = (T)ri;
+    x0[i] = 0;
+    rr += ri * ri;
+  }
+  rr = block_sum(rr, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = rr;
+}
+
+// (T) r -> b0, x0 = 0 (start of a preconditioner application)
+template <class T>
+__global__ __launch_bounds__(256) void k_prep_vcycle(int64_t n, const double *__restrict__ r,
+                                                     T *__restrict__ b0, T *__restrict__ x0) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    b0[i] = (T)r[i];
+    x0[i] = 0;
+  }
+}
+
+// z_new = (double) x0; partials (r, z_new) and (r, z_old); z = z_new
+template <class T>
+__global__ __launch_bounds__(256) void k_take_z(int64_t n, const T *__restrict__ x0,
+                                                const double *__restrict__ r, double *__restrict__ z,
+                                                double *__restrict__ part) {
+  __shared__ double sh[256];
+  double a = 0.0, b = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double zn = (double)x0[i];
+    a += r[i] * zn;
+    b += r[i] * z[i];
+    z[i] = zn;
+  }
+  a = block_sum(a, sh);
+  b = block_sum(b, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[gridDim.x + blockIdx.x] = b;
+  }
+}
+
+// beta = ((r, z_new) - (r, z_old)) / rz: flexible (Polak-Ribiere) CG, robust
+// to the FP32 preconditioner; rz = (r, z_new).  first != 0: beta = 0 (p = z)
+__global__ void k_cg_beta(CgScalars *sc, int first) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    sc->beta = first ? 0.0 : (sc->rz_new - sc->rz_cross) / sc->rz;
+    sc->rz = sc->rz_new;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_p_update(int64_t n, const CgScalars *__restrict__ sc,
+                                                  const double *__restrict__ z, double *__restrict__ p) {
+  const double beta = sc->beta;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+}
+
+} // namespace afe

@@ -1,0 +1,234 @@
+/*
+ * afepack_amd.h -- C ABI of the MI355X (gfx950) assemble-and-solve path.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.
+ * Host arrays (meshes, patterns, fixtures) are borrowed for the duration of a
+ * call.  Vectors passed as `d_*` are device pointers (HBM) on the context's
+ * device; the library never keeps them after the call returns.  All device
+ * work of a context is issued on the context's stream; every call returns
+ * with that stream synchronised unless its comment says otherwise.
+ *
+ * Each entry point names the AFEPack interface it replaces (paths relative to
+ * the AFEPack source tree).  The C++ drop-in classes in
+ * afepack_amd/include/AFEPack/ (StiffMatrix, MassMatrix, AMGSolver, ...)
+ * are thin facades over these calls.
+ */
+#ifndef AFEPACK_AMD_H
+#define AFEPACK_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define AFE_OK 0
+#define AFE_E_ARG (-1)          /* invalid argument / size mismatch          */
+#define AFE_E_NOMEM (-2)        /* host or device allocation failed          */
+#define AFE_E_HIP (-3)          /* a HIP runtime call failed                 */
+#define AFE_E_SINGULAR (-4)     /* coarse matrix singular (dgetrf failure)    */
+#define AFE_E_NOTCONVERGED (-5) /* Krylov solve hit max_it above rtol        */
+#define AFE_E_PATTERN (-6)      /* lazyReinit with a different pattern        */
+#define AFE_E_BUDGET (-7)       /* request exceeds the HBM admission budget   */
+#define AFE_E_NODEVICE (-8)     /* no HIP device / device index out of range  */
+
+/* element kinds (template elements triangle.1, triangle.2, tetrahedron.1) */
+#define AFE_P1_TRI 1
+#define AFE_P2_TRI 2
+#define AFE_P1_TET 3
+
+/* bilinear operators (BilinearOperator.templates.h:342-406 and the
+ * variable-coefficient override of examples/elliptic_equation) */
+#define AFE_OP_STIFF 0
+#define AFE_OP_MASS 1
+#define AFE_OP_VARSTIFF 2
+
+/* right-hand sides for afe_rhs (Operator::L2Discretize) */
+#define AFE_F_POISSON 0 /* f = dim*pi^2*prod sin(pi x_d)                   */
+#define AFE_F_VARCOEF 1 /* f = -div(a grad u), u = prod sin(pi x_d)         */
+#define AFE_F_NODAL 2   /* f = interpolant of DOF values                    */
+#define AFE_F_ONE 3     /* f = 1                                            */
+
+/* solve modes for afe_amg_pcg */
+#define AFE_PREC_FP64 0 /* FP64 PCG, FP64 symmetric V-cycle preconditioner */
+#define AFE_PREC_FP32 1 /* FP64 PCG, FP32 symmetric V-cycle preconditioner */
+
+typedef struct afe_ctx afe_ctx;
+typedef struct afe_space afe_space;
+typedef struct afe_matrix afe_matrix;
+typedef struct afe_amg afe_amg;
+
+/* Seeded Fourier field g(x) = scale * sum_m w_m sin(2 pi k_m.x + phi_m);
+ * a(x) = exp(g(x)).  k is n_modes x dim, row-major. */
+typedef struct afe_field {
+  int32_t n_modes;
+  int32_t dim;
+  const double *k;
+  const double *phi;
+  const double *w;
+  double scale;
+} afe_field;
+
+typedef struct afe_amg_params {
+  double alpha;      /* strength threshold, AMGSolver.h:87-92 (0.25)        */
+  uint32_t nmpo;     /* max coarsest order (50)                            */
+  double nmpsd;      /* min coarsest density (0.382)                       */
+  int32_t lazy;      /* 1: lazyReinit/lazyProject hierarchy               */
+  int32_t smooth;    /* GS sweeps per side (AMGSolver::smoothStep, 3)      */
+  int32_t build_fp32;/* also keep FP32 copies for the mixed V-cycle        */
+} afe_amg_params;
+
+typedef struct afe_solve_stats {
+  int32_t iterations;     /* V-cycles (stationary) or PCG iterations       */
+  int32_t converged;      /* 1 if the tolerance was reached                */
+  double rel_res_l2;      /* ||b - A x||_2 / ||b||_2 at exit (true residual) */
+  double rel_res_l1;      /* ||A x - b||_1 / ||A x0 - b||_1 (AMGSolver.cpp:564,617) */
+  double res_l1;          /* ||A x - b||_1 at exit                         */
+  double seconds;         /* device wall time of the solve                 */
+} afe_solve_stats;
+
+/* ---- context ------------------------------------------------------------ */
+const char *afe_last_error(void);
+int afe_version(void);
+int afe_ctx_create(int device, afe_ctx **out);
+int afe_ctx_destroy(afe_ctx *ctx);
+int afe_ctx_synchronize(afe_ctx *ctx);
+/* bytes of device memory held by live objects of this context */
+int afe_ctx_device_bytes(afe_ctx *ctx, int64_t *bytes);
+
+/* ---- synthetic meshes (the benchmark inputs) ---------------------------- */
+/* 2D: n x n squares on the unit square, split (n00,n10,n11),(n00,n11,n01) as
+ * examples/FAST/src/poisson_lshape_stretched_afepack.cpp:262-286; interior
+ * nodes jittered by U(-jitter h, jitter h) per coordinate from seed.
+ * Sizes: (n+1)^2 vertices, 2 n^2 triangles. */
+int afe_mesh_square(int32_t n, double jitter, uint64_t seed, double *coords,
+                    int32_t *elem_vert, int32_t *vert_mark);
+/* 3D: n^3 cubes on the unit cube, 6 tetrahedra each as tools/tricube.cpp:67-90.
+ * Sizes: (n+1)^3 vertices, 6 n^3 tetrahedra. */
+int afe_mesh_cube(int32_t n, double jitter, uint64_t seed, double *coords,
+                  int32_t *elem_vert, int32_t *vert_mark);
+/* displace interior vertices by amp*h*sin(2 pi k x_d) (moving-mesh update) */
+int afe_mesh_displace(int32_t dim, int64_t n_vert, double *coords,
+                      const int32_t *vert_mark, double amp_h, int32_t k);
+/* seeded n_modes-mode field with |g| <= amplitude; k, phi, w are outputs of
+ * size n_modes*dim, n_modes, n_modes; *scale receives the scale factor. */
+int afe_field_random(int32_t n_modes, int32_t dim, double amplitude,
+                     uint64_t seed, double *k, double *phi, double *w,
+                     double *scale);
+
+/* ---- FE space: FEMSpace::buildDof + buildDofBoundaryMark --------------- */
+/* Replaces FEMSpace::buildElement/buildDof/buildDofBoundaryMark
+ * (include/AFEPack/FEMSpace.templates.h:183-258,689-880) and
+ * BilinearOperator::buildSparsityPattern (BilinearOperator.templates.h:64-172):
+ * first-touch DOF numbering, the deal.II-style CSR pattern (diagonal first,
+ * rest ascending) and the element-to-CSR gather map, built on the host and
+ * uploaded to HBM once. */
+/* ctx may be NULL: a host-only space (numbering and pattern only) */
+int afe_space_create(afe_ctx *ctx, int32_t kind, int64_t n_vert,
+                     const double *coords, int64_t n_elem,
+                     const int32_t *elem_vert, const int32_t *vert_mark,
+                     afe_space **out);
+int afe_space_destroy(afe_space *s);
+int afe_space_info(const afe_space *s, int64_t *n_dof, int64_t *nnz,
+                   int32_t *n_loc);
+/* copies out DOF data; any pointer may be NULL */
+int afe_space_dofs(const afe_space *s, int32_t *elem_dof, int32_t *dof_mark,
+                   double *dof_point);
+/* the pattern with the reference's size_t index width */
+int afe_space_pattern(const afe_space *s, int64_t *row_ptr, int64_t *col);
+/* moving mesh: new vertex coordinates, same connectivity (values-only
+ * re-assembly keeps pattern, gather map and lazy hierarchy) */
+int afe_space_update_coords(afe_space *s, const double *coords);
+
+/* ---- assembly: BilinearOperator::build ---------------------------------- */
+/* Replaces BilinearOperator::build / buildSparseMatrix / getElementMatrix
+ * (BilinearOperator.templates.h:55-60,176-255,342-406): element matrices in
+ * FP64 on the device, gathered into the CSR values in ascending element order
+ * (no atomics).  coef is used by AFE_OP_VARSTIFF only. */
+int afe_matrix_create(afe_space *s, afe_matrix **out);
+/* a square matrix given as the reference's CSR (SparseMatrix<double> over a
+ * compressed SparsityPattern: diagonal first in each row), for solvers on
+ * matrices assembled elsewhere; val may be NULL (zeros) */
+int afe_matrix_create_csr(afe_ctx *ctx, int64_t m, const int64_t *row_ptr,
+                          const int64_t *col, const double *val,
+                          afe_matrix **out);
+int afe_matrix_destroy(afe_matrix *a);
+int afe_assemble(afe_matrix *a, int32_t op, int32_t acc, const afe_field *coef);
+/* user-supplied element matrices (a getElementMatrix override evaluated by
+ * the caller): h_elem_mat is n_elem x n_loc x n_loc row-major on the host;
+ * the CSR gather runs on the device. */
+int afe_assemble_elements(afe_matrix *a, const double *h_elem_mat);
+int afe_matrix_values(const afe_matrix *a, double *h_val);     /* D2H copy */
+int afe_matrix_set_values(afe_matrix *a, const double *h_val); /* H2D copy */
+double *afe_matrix_device_values(afe_matrix *a);
+/* SparseMatrix::vmult (SparseMatrix.h:100-114): d_y = A d_x */
+int afe_matrix_vmult(afe_matrix *a, const double *d_x, double *d_y);
+
+/* ---- right-hand side and Dirichlet elimination -------------------------- */
+/* Operator::L2Discretize (Operator.discretize.templates.h:80-152) into d_f. */
+int afe_rhs(afe_space *s, int32_t acc, int32_t fkind, const afe_field *coef,
+            const double *d_nodal, double *d_f);
+/* BoundaryConditionAdmin::apply, preserve_symmetry = true, zero Dirichlet
+ * data on every DOF with a nonzero boundary mark
+ * (FEMSpace.templates.h:1338-1373).  d_u is not touched (as the reference). */
+int afe_apply_dirichlet(afe_matrix *a, double *d_u, double *d_f);
+
+/* ---- error functionals -------------------------------------------------- */
+/* Functional::L2Error / H1SemiError (Functional.templates.h:296-317,399-426)
+ * of the FE function d_uh against u = prod sin(pi x_d). */
+int afe_error_l2(afe_space *s, const double *d_uh, int32_t acc, double *err);
+int afe_error_h1(afe_space *s, const double *d_uh, int32_t acc, double *err);
+
+/* ---- AMG ---------------------------------------------------------------- */
+/* AMGSolver(M, tol, s, nmpo, nmpsd, alpha) / reinit (src/AMGSolver.cpp:384-438,
+ * 683-699) or lazyReinit on a fresh solver (:442-496): the hierarchy is built
+ * on the host (Project / lazyProject + getPMPt, identical patterns), coloured,
+ * converted to sliced-ELL and uploaded once. */
+int afe_amg_create(afe_matrix *a, const afe_amg_params *p, afe_amg **out);
+int afe_amg_destroy(afe_amg *g);
+/* host-only hierarchy of a CSR matrix (the coarsening half of
+ * AMGSolver::reinit / lazyInit, src/AMGSolver.cpp:384-496) without a device;
+ * inspect it with afe_amg_info / afe_amg_level_csr / afe_amg_coarse_inverse */
+int afe_amg_create_host(int64_t m, const int64_t *row_ptr, const int64_t *col,
+                        const double *val, const afe_amg_params *p,
+                        afe_amg **out);
+/* lazyReinit of a host-only lazy hierarchy with new values on the same
+ * pattern (src/AMGSolver.cpp:505-545) */
+int afe_amg_lazy_reinit_host(afe_amg *g, const double *val);
+/* the coarsest dense inverse, row-major n_c x n_c (DenseMatrix::gauss_jordan
+ * of AMGSolver::reinit, src/AMGSolver.cpp:423-435) */
+int afe_amg_coarse_inverse(const afe_amg *g, double *minv);
+/* AMGSolver::lazyReinit (src/AMGSolver.cpp:505-545): same pattern object
+ * required (AFE_E_PATTERN where the reference aborts); P is kept and the
+ * Galerkin products are recomputed. */
+int afe_amg_lazy_reinit(afe_amg *g, afe_matrix *a);
+int afe_amg_info(const afe_amg *g, int32_t *n_levels, int64_t *level_n,
+                 int64_t *level_nnz, int32_t *coarse_full);
+/* host copies of a level of the hierarchy (reference patterns, including the
+ * explicit zeros): which = 0 -> A_l, 1 -> P_l; NULL pointers skip arrays */
+int afe_amg_level_csr(const afe_amg *g, int32_t level, int32_t which,
+                      int64_t *m, int64_t *n, int64_t *nnz, int64_t *row_ptr,
+                      int64_t *col, double *val);
+/* AMGSolver::solve(x, r, tol, step, mode) (src/AMGSolver.cpp:549-639):
+ * exactly step+1 V(s,s) cycles from the initial guess d_x, FP64, multicolour
+ * Gauss-Seidel in forward colour order on both sides.  If rtol > 0 the
+ * cycles stop early once ||b-Ax||_2/||b||_2 <= rtol (benchmark mode). */
+int afe_amg_solve(afe_amg *g, double *d_x, const double *d_r, double tol,
+                  int64_t step, int32_t mode, double rtol,
+                  afe_solve_stats *st);
+/* AMGPreconditioner::vmult (AMGSolver.h:193-195) = solve(x, b, 0, is, 1) */
+int afe_amg_precondition(afe_amg *g, double *d_x, const double *d_b,
+                         int32_t iterate_step);
+/* FP64 preconditioned CG with one symmetric V-cycle (forward colours before,
+ * reverse colours after the coarse correction) per iteration, in FP64 or in
+ * FP32 (prec); stops at ||b-Ax||_2/||b||_2 <= rtol (true residual). */
+int afe_amg_pcg(afe_amg *g, double *d_x, const double *d_b, double rtol,
+                int32_t max_it, int32_t prec, afe_solve_stats *st);
+/* one symmetric V-cycle z = M^{-1} r from zero (prec as above) */
+int afe_amg_vcycle(afe_amg *g, double *d_z, const double *d_r, int32_t prec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

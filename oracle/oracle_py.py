@@ -1,0 +1,280 @@
+"""ctypes wrapper of the oracle library (oracle/liboracle_afe.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the afepack_amd package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_afe.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+P1_TRI, P2_TRI, P1_TET = 1, 2, 3
+OP_STIFF, OP_MASS, OP_VARSTIFF = 0, 1, 2
+F_POISSON, F_VARCOEF, F_NODAL, F_ONE = 0, 1, 2, 3
+U_SINPROD, U_ZERO = 0, 1
+
+P = C.c_void_p
+I32, I64, F64 = C.c_int32, C.c_int64, C.c_double
+
+
+class OrField(C.Structure):
+    _fields_ = [("n_modes", C.c_int), ("dim", C.c_int), ("k", C.POINTER(F64)),
+                ("phi", C.POINTER(F64)), ("w", C.POINTER(F64)), ("scale", F64)]
+
+
+_SIGS = {
+    "or_space_new": (P, [C.c_int, I64, P, I64, P, P]),
+    "or_space_free": (None, [P]),
+    "or_space_n_dof": (I64, [P]),
+    "or_space_n_loc": (C.c_int, [P]),
+    "or_space_dofs": (None, [P, P, P, P]),
+    "or_quad_n": (C.c_int, [C.c_int, C.c_int]),
+    "or_quad_get": (None, [C.c_int, C.c_int, P, P]),
+    "or_element_matrix": (None, [C.c_int, C.c_int, C.c_int, P, C.POINTER(OrField), P]),
+    "or_build": (P, [P, C.c_int, C.c_int, C.POINTER(OrField)]),
+    "or_mat_from_csr": (P, [I64, I64, P, P, P]),
+    "or_mat_share_pattern": (P, [P, P]),
+    "or_mat_free": (None, [P]),
+    "or_mat_m": (I64, [P]),
+    "or_mat_n": (I64, [P]),
+    "or_mat_nnz": (I64, [P]),
+    "or_mat_get": (None, [P, P, P, P]),
+    "or_mat_vmult": (None, [P, P, P]),
+    "or_mat_tvmult": (None, [P, P, P]),
+    "or_rhs": (None, [P, C.c_int, C.c_int, C.POINTER(OrField), P, P]),
+    "or_apply_dirichlet": (None, [P, P, P, P]),
+    "or_l2_error": (F64, [P, P, C.c_int, C.c_int]),
+    "or_h1_error": (F64, [P, P, C.c_int, C.c_int]),
+    "or_amg_new": (P, [P, C.c_int, F64, C.c_uint, F64]),
+    "or_amg_lazy_reinit": (C.c_int, [P, P]),
+    "or_amg_free": (None, [P]),
+    "or_amg_n_project": (C.c_int, [P]),
+    "or_amg_coarse_full": (C.c_int, [P]),
+    "or_amg_A": (P, [P, C.c_int]),
+    "or_amg_P": (P, [P, C.c_int]),
+    "or_amg_Pt": (P, [P, C.c_int]),
+    "or_amg_coarse_inverse": (None, [P, P]),
+    "or_amg_solve": (None, [P, P, P, C.c_uint, I64, C.c_int, P]),
+    "or_gauss_seidel": (None, [P, P, P, C.c_int]),
+    "or_dense_inverse": (C.c_int, [C.c_int, P]),
+}
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Field:
+    def __init__(self, k, phi, w, scale):
+        self.k = np.ascontiguousarray(k, dtype=np.float64)
+        self.phi = np.ascontiguousarray(phi, dtype=np.float64)
+        self.w = np.ascontiguousarray(w, dtype=np.float64)
+        self.scale = float(scale)
+        pf = C.POINTER(F64)
+        self.s = OrField(len(self.phi), self.k.size // len(self.phi), self.k.ctypes.data_as(pf),
+                         self.phi.ctypes.data_as(pf), self.w.ctypes.data_as(pf), self.scale)
+
+    def ptr(self):
+        return C.pointer(self.s)
+
+
+def _fp(f):
+    return None if f is None else f.ptr()
+
+
+class Mat:
+    def __init__(self, h, owner=None):
+        self.h = h
+        self.owner = owner  # keeps a borrowed matrix's owner alive
+
+    @classmethod
+    def from_csr(cls, m, n, rp, col, val=None):
+        rp = np.ascontiguousarray(rp, dtype=np.int64)
+        col = np.ascontiguousarray(col, dtype=np.int64)
+        v = None if val is None else np.ascontiguousarray(val, dtype=np.float64)
+        return cls(lib().or_mat_from_csr(m, n, _p(rp), _p(col), _p(v)))
+
+    def share_pattern(self, val):
+        v = np.ascontiguousarray(val, dtype=np.float64)
+        return Mat(lib().or_mat_share_pattern(self.h, _p(v)))
+
+    @property
+    def m(self):
+        return lib().or_mat_m(self.h)
+
+    @property
+    def n(self):
+        return lib().or_mat_n(self.h)
+
+    @property
+    def nnz(self):
+        return lib().or_mat_nnz(self.h)
+
+    def csr(self):
+        rp = np.empty(self.m + 1, dtype=np.int64)
+        col = np.empty(self.nnz, dtype=np.int64)
+        val = np.empty(self.nnz, dtype=np.float64)
+        lib().or_mat_get(self.h, _p(rp), _p(col), _p(val))
+        return rp, col, val
+
+    def dense(self):
+        rp, col, val = self.csr()
+        d = np.zeros((self.m, self.n))
+        for i in range(self.m):
+            for k in range(rp[i], rp[i + 1]):
+                d[i, col[k]] += val[k]
+        return d
+
+    def vmult(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.m)
+        lib().or_mat_vmult(self.h, _p(x), _p(y))
+        return y
+
+    def free(self):
+        if self.h and self.owner is None:
+            lib().or_mat_free(self.h)
+        self.h = None
+
+
+class Space:
+    def __init__(self, kind, coords, elem_vert, vert_mark):
+        self.kind = kind
+        self.dim = 3 if kind == P1_TET else 2
+        self.coords = np.ascontiguousarray(coords, dtype=np.float64)
+        self.ev = np.ascontiguousarray(elem_vert, dtype=np.int32)
+        self.vm = np.ascontiguousarray(vert_mark, dtype=np.int32)
+        nvtx = 4 if kind == P1_TET else 3
+        self.n_elem = len(self.ev) // nvtx
+        self.h = lib().or_space_new(kind, len(self.vm), _p(self.coords), self.n_elem, _p(self.ev),
+                                    _p(self.vm))
+        self.n_dof = lib().or_space_n_dof(self.h)
+        self.n_loc = lib().or_space_n_loc(self.h)
+
+    def dofs(self):
+        ed = np.empty(self.n_elem * self.n_loc, dtype=np.int32)
+        dm = np.empty(self.n_dof, dtype=np.int32)
+        dp = np.empty(self.n_dof * self.dim, dtype=np.float64)
+        lib().or_space_dofs(self.h, _p(ed), _p(dm), _p(dp))
+        return ed, dm, dp
+
+    def build(self, op, acc, field=None):
+        return Mat(lib().or_build(self.h, op, acc, _fp(field)))
+
+    def rhs(self, acc, fkind, field=None, nodal=None):
+        f = np.empty(self.n_dof)
+        nd = None if nodal is None else np.ascontiguousarray(nodal, dtype=np.float64)
+        lib().or_rhs(self.h, acc, fkind, _fp(field), _p(nd), _p(f))
+        return f
+
+    def apply_dirichlet(self, mat, f, u=None):
+        lib().or_apply_dirichlet(self.h, mat.h, _p(u), _p(f))
+
+    def l2_error(self, uh, acc, ukind=U_SINPROD):
+        return lib().or_l2_error(self.h, _p(np.ascontiguousarray(uh, dtype=np.float64)), acc, ukind)
+
+    def h1_error(self, uh, acc, ukind=U_SINPROD):
+        return lib().or_h1_error(self.h, _p(np.ascontiguousarray(uh, dtype=np.float64)), acc, ukind)
+
+    def free(self):
+        if self.h:
+            lib().or_space_free(self.h)
+            self.h = None
+
+
+def element_matrix(kind, op, acc, vert_coords, field=None):
+    nl = {P1_TRI: 3, P2_TRI: 6, P1_TET: 4}[kind]
+    v = np.ascontiguousarray(vert_coords, dtype=np.float64).ravel()
+    K = np.empty(nl * nl)
+    lib().or_element_matrix(kind, op, acc, _p(v), _fp(field), _p(K))
+    return K.reshape(nl, nl)
+
+
+def quad(tdim, acc):
+    n = lib().or_quad_n(tdim, acc)
+    pts = np.empty(n * tdim)
+    w = np.empty(n)
+    lib().or_quad_get(tdim, acc, _p(pts), _p(w))
+    return pts.reshape(n, tdim), w
+
+
+class Amg:
+    def __init__(self, mat, lazy=False, alpha=0.25, nmpo=50, nmpsd=0.382):
+        self.mat = mat
+        self.h = lib().or_amg_new(mat.h, int(lazy), alpha, nmpo, nmpsd)
+
+    @property
+    def n_project(self):
+        return lib().or_amg_n_project(self.h)
+
+    @property
+    def full(self):
+        return bool(lib().or_amg_coarse_full(self.h))
+
+    def A(self, l):
+        return Mat(lib().or_amg_A(self.h, l), owner=self)
+
+    def P(self, l):
+        return Mat(lib().or_amg_P(self.h, l), owner=self)
+
+    def Pt(self, l):
+        return Mat(lib().or_amg_Pt(self.h, l), owner=self)
+
+    def coarse_inverse(self):
+        n = self.A(self.n_project).m
+        out = np.empty(n * n)
+        lib().or_amg_coarse_inverse(self.h, _p(out))
+        return out.reshape(n, n)
+
+    def lazy_reinit(self, mat):
+        rc = lib().or_amg_lazy_reinit(self.h, mat.h)
+        if rc == 0:
+            self.mat = mat
+        return rc
+
+    def solve(self, x, r, smooth=3, step=20, exactly=True):
+        """exactly step+1 V-cycles in place on x; returns the L1 residual history"""
+        hist = np.empty(step + 2)
+        lib().or_amg_solve(self.h, _p(x), _p(np.ascontiguousarray(r, dtype=np.float64)), smooth,
+                           step, int(exactly), _p(hist))
+        return hist
+
+    def free(self):
+        if self.h:
+            lib().or_amg_free(self.h)
+            self.h = None
+
+
+def gauss_seidel(mat, x, r, s):
+    lib().or_gauss_seidel(mat.h, _p(x), _p(np.ascontiguousarray(r, dtype=np.float64)), s)
+
+
+def dense_inverse(a):
+    a = np.array(a, dtype=np.float64, order="C", copy=True)
+    rc = lib().or_dense_inverse(a.shape[0], _p(a))
+    return rc, a
