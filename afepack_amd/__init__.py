@@ -3,7 +3,7 @@
 The product is the C ABI in include/afepack_amd.h, implemented by the in-tree
 shared library libafepack_amd.so (HIP kernels + host C++).  This package only
 loads that library with ctypes and wraps it for Python callers; device vectors
-are torch tensors (PyTorch is used for device memory, nothing else).
+are DeviceVector buffers allocated through the library itself.
 
 Importing does not need a GPU.  Every compute entry point goes through the
 library; when the library is missing, load() raises -- there is no fallback.

@@ -36,6 +36,12 @@ SIGS = {
     "afe_ctx_destroy": (C.c_int, [P]),
     "afe_ctx_synchronize": (C.c_int, [P]),
     "afe_ctx_device_bytes": (C.c_int, [P, PI64]),
+    "afe_dev_alloc": (C.c_int, [P, I64, PP]),
+    "afe_dev_free": (C.c_int, [P, P]),
+    "afe_dev_upload": (C.c_int, [P, P, P, I64]),
+    "afe_dev_download": (C.c_int, [P, P, P, I64]),
+    "afe_dev_copy": (C.c_int, [P, P, P, I64]),
+    "afe_dev_zero": (C.c_int, [P, P, I64]),
     "afe_mesh_square": (C.c_int, [I32, F64, U64, P, P, P]),
     "afe_mesh_cube": (C.c_int, [I32, F64, U64, P, P, P]),
     "afe_mesh_displace": (C.c_int, [I32, I64, P, P, F64, I32]),
@@ -71,6 +77,9 @@ SIGS = {
     "afe_amg_precondition": (C.c_int, [P, P, P, I32]),
     "afe_amg_pcg": (C.c_int, [P, P, P, F64, I32, I32, C.POINTER(SolveStats)]),
     "afe_amg_vcycle": (C.c_int, [P, P, P, I32]),
+    "afe_amg_set_smooth": (C.c_int, [P, I32]),
+    "afe_amg_kernel_timing": (C.c_int, [P, I32, I32, PF64, PF64]),
+    "afe_amg_device_info": (C.c_int, [P, I32, PI64, PI32, PI64, PI32]),
 }
 
 

@@ -1,7 +1,8 @@
 """Python wrappers over the C ABI (include/afepack_amd.h).
 
-Host arrays are numpy; device vectors are torch float64 tensors on the
-context's device.  Every call goes through libafepack_amd.so.
+Host arrays are numpy; device vectors are DeviceVector buffers allocated
+through the library (afe_dev_alloc), so a process loads exactly one HIP
+runtime.  Every call goes through libafepack_amd.so.
 """
 import ctypes as C
 
@@ -13,7 +14,7 @@ __all__ = [
     "P1_TRI", "P2_TRI", "P1_TET", "OP_STIFF", "OP_MASS", "OP_VARSTIFF",
     "F_POISSON", "F_VARCOEF", "F_NODAL", "F_ONE", "PREC_FP64", "PREC_FP32",
     "mesh_square", "mesh_cube", "mesh_displace", "field_random", "Field",
-    "Context", "Space", "Matrix", "Amg", "HostAmg", "AmgParams",
+    "Context", "Space", "Matrix", "Amg", "HostAmg", "AmgParams", "DeviceVector",
 ]
 
 P1_TRI, P2_TRI, P1_TET = 1, 2, 3
@@ -35,8 +36,8 @@ def _chk(code):
 def _ptr(a):
     if a is None:
         return None
-    if hasattr(a, "data_ptr"):
-        return C.c_void_p(a.data_ptr())
+    if isinstance(a, DeviceVector):
+        return a.ptr
     return a.ctypes.data_as(C.c_void_p)
 
 
@@ -127,6 +128,60 @@ class Context:
             pass
 
 
+class DeviceVector:
+    """A float64 (or other dtype) buffer in HBM owned by a Context."""
+
+    def __init__(self, ctx, n, dtype=np.float64, zero=True):
+        self.ctx = ctx
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        _chk(_lib().afe_dev_alloc(ctx.h, self.nbytes, C.byref(p)))
+        self.ptr = p
+        if zero:
+            self.zero()
+
+    @classmethod
+    def from_numpy(cls, ctx, a):
+        a = np.ascontiguousarray(a)
+        v = cls(ctx, a.size, a.dtype, zero=False)
+        v.upload(a)
+        return v
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        _chk(_lib().afe_dev_upload(self.ctx.h, self.ptr, a.ctypes.data_as(C.c_void_p), self.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        _chk(_lib().afe_dev_download(self.ctx.h, out.ctypes.data_as(C.c_void_p), self.ptr,
+                                     self.nbytes))
+        return out
+
+    def zero(self):
+        _chk(_lib().afe_dev_zero(self.ctx.h, self.ptr, self.nbytes))
+
+    def copy_from(self, other):
+        assert other.nbytes == self.nbytes
+        _chk(_lib().afe_dev_copy(self.ctx.h, self.ptr, other.ptr, self.nbytes))
+
+    def __len__(self):
+        return self.n
+
+    def free(self):
+        if self.ptr and self.ctx is not None and self.ctx.h:
+            _lib().afe_dev_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Space:
     """FE space (DOF numbering + pattern); ctx=None builds it on the host only."""
 
@@ -166,9 +221,10 @@ class Space:
         _chk(_lib().afe_space_update_coords(self.h, _ptr(self.coords)))
 
     def rhs(self, acc, fkind, field=None, nodal=None, out=None):
-        import torch
         if out is None:
-            out = torch.empty(self.n_dof, dtype=torch.float64, device=f"cuda:{self.ctx.device}")
+            out = DeviceVector(self.ctx, self.n_dof, zero=False)
+        if nodal is not None and not isinstance(nodal, DeviceVector):
+            nodal = DeviceVector.from_numpy(self.ctx, np.asarray(nodal, dtype=np.float64))
         fp, _keep = _field_ptr(field)
         _chk(_lib().afe_rhs(self.h, acc, fkind, fp, _ptr(nodal), _ptr(out)))
         return out
@@ -338,3 +394,19 @@ class Amg(_AmgBase):
 
     def precondition(self, x, b, iterate_step=5):
         _chk(_lib().afe_amg_precondition(self.h, _ptr(x), _ptr(b), iterate_step))
+
+    def set_smooth(self, s):
+        _chk(_lib().afe_amg_set_smooth(self.h, s))
+
+    def kernel_timing(self, which, reps=20):
+        ms, by = C.c_double(), C.c_double()
+        _chk(_lib().afe_amg_kernel_timing(self.h, which, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+    def device_info(self):
+        out = []
+        for l in range(self.info()["levels"]):
+            r, c, s, t = C.c_int64(), C.c_int32(), C.c_int64(), C.c_int32()
+            _chk(_lib().afe_amg_device_info(self.h, l, C.byref(r), C.byref(c), C.byref(s), C.byref(t)))
+            out.append({"rows": r.value, "colours": c.value, "stored": s.value, "tail": t.value})
+        return out

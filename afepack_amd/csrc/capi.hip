@@ -6,7 +6,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -72,6 +74,7 @@ struct afe_ctx {
   std::recursive_mutex mu;
   int64_t bytes = 0;
   double *d_quad = nullptr; // all quadrature rules
+  std::map<void *, int64_t> user_bytes; // afe_dev_alloc buffers
 };
 
 template <class T> struct DevBuf {
@@ -147,6 +150,7 @@ extern "C" int afe_ctx_destroy(afe_ctx *c) {
   if (!c) return AFE_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  for (auto &kv : c->user_bytes) (void)hipFree(kv.first);
   (void)hipFree(c->d_quad);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
@@ -165,6 +169,74 @@ extern "C" int afe_ctx_device_bytes(afe_ctx *c, int64_t *bytes) {
   if (!c || !bytes) return fail(AFE_E_ARG, "null argument");
   *bytes = c->bytes;
   return AFE_OK;
+}
+
+extern "C" int afe_dev_alloc(afe_ctx *c, int64_t bytes, void **d_ptr) {
+  API_BEGIN
+  if (!c || !d_ptr || bytes < 0) return fail(AFE_E_ARG, "bad allocation arguments");
+  HIPCHK(hipSetDevice(c->device));
+  *d_ptr = nullptr;
+  if (bytes == 0) return AFE_OK;
+  if (hipMalloc(d_ptr, (size_t)bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *d_ptr = nullptr;
+    return fail(AFE_E_NOMEM, "hipMalloc failed");
+  }
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  c->bytes += bytes;
+  c->user_bytes[*d_ptr] = bytes;
+  API_END
+}
+
+extern "C" int afe_dev_free(afe_ctx *c, void *d_ptr) {
+  API_BEGIN
+  if (!c) return fail(AFE_E_ARG, "null context");
+  if (!d_ptr) return AFE_OK;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  auto it = c->user_bytes.find(d_ptr);
+  if (it == c->user_bytes.end()) return fail(AFE_E_ARG, "pointer not allocated by afe_dev_alloc");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipFree(d_ptr));
+  c->bytes -= it->second;
+  c->user_bytes.erase(it);
+  API_END
+}
+
+extern "C" int afe_dev_upload(afe_ctx *c, void *d_dst, const void *h_src, int64_t bytes) {
+  API_BEGIN
+  if (!c || (bytes > 0 && (!d_dst || !h_src))) return fail(AFE_E_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  if (bytes > 0) HIPCHK(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  API_END
+}
+
+extern "C" int afe_dev_download(afe_ctx *c, void *h_dst, const void *d_src, int64_t bytes) {
+  API_BEGIN
+  if (!c || (bytes > 0 && (!d_src || !h_dst))) return fail(AFE_E_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  if (bytes > 0) HIPCHK(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  API_END
+}
+
+extern "C" int afe_dev_copy(afe_ctx *c, void *d_dst, const void *d_src, int64_t bytes) {
+  API_BEGIN
+  if (!c || (bytes > 0 && (!d_src || !d_dst))) return fail(AFE_E_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  if (bytes > 0) HIPCHK(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  API_END
+}
+
+extern "C" int afe_dev_zero(afe_ctx *c, void *d_ptr, int64_t bytes) {
+  API_BEGIN
+  if (!c || (bytes > 0 && !d_ptr)) return fail(AFE_E_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  if (bytes > 0) HIPCHK(hipMemsetAsync(d_ptr, 0, bytes, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  API_END
 }
 
 // ===========================================================================
@@ -259,7 +331,7 @@ extern "C" int afe_space_create(afe_ctx *ctx, int32_t kind, int64_t n_vert, cons
     const int64_t r1 = std::min<int64_t>(r0 + 256, s->h.n_dof);
     s->seg_max = std::max(s->seg_max, s->h.rp[r1] - s->h.rp[r0]);
   }
-  if (s->seg_max * 8 > 160 * 1024) return fail(AFE_E_ARG, "rows too long for the LDS gather");
+  if (s->seg_max * 8 > 64 * 1024) return fail(AFE_E_ARG, "rows too long for the LDS gather");
   s->coords.upload(ctx, s->h.coords);
   s->ev.upload(ctx, s->h.ev);
   s->elem_dof.upload(ctx, s->h.elem_dof);
@@ -665,7 +737,7 @@ struct afe_amg {
   DevBuf<LevelDev<float>> ld32;
   DevBuf<int32_t> perm0;
   // CG workspace (level-0 permuted numbering)
-  DevBuf<double> wx, wr, wp, wq, wz, wb, part;
+  DevBuf<double> wx, wxl, wr, wp, wq, wz, wb, part;
   DevBuf<CgScalars> sc;
   CgScalars *h_sc = nullptr; // pinned
   bool has32 = false;
@@ -763,7 +835,9 @@ static void upload_hierarchy(afe_amg *g) {
     if (l + 1 < L) {
       Sell sp, st;
       build_sell(h.P[l], perm[l + 1], iperm[l], {(int32_t)h.P[l].m}, false, sp);
-      build_sell(h.Pt[l].m ? h.Pt[l] : transpose(h.P[l]), perm[l], iperm[l + 1], {(int32_t)h.A[l].m}, false, st);
+      // prolongation is P^T of the restriction (SparseMatrix::Tvmult of P in
+      // AMGSolver::solve, src/AMGSolver.cpp:608-613), stored explicitly
+      build_sell(transpose(h.P[l]), perm[l], iperm[l + 1], {(int32_t)h.A[l].m}, false, st);
       d.p_ns = (int)sp.slices.size();
       d.t_ns = (int)st.slices.size();
       d.p_si.upload(ctx, sp.slices);
@@ -819,6 +893,7 @@ static void upload_hierarchy(afe_amg *g) {
   if (g->has32) g->ld32.upload(ctx, d32);
   const int64_t n0 = g->lv[0].n;
   g->wx.alloc(ctx, n0);
+  g->wxl.alloc(ctx, n0);
   g->wr.alloc(ctx, n0);
   g->wp.alloc(ctx, n0);
   g->wq.alloc(ctx, n0);
@@ -1043,10 +1118,18 @@ static void launch_graph_or_issue(afe_amg *g, int prec, bool symmetric, int nu) 
   HIPCHK(hipGraphLaunch(ex, st));
 }
 
-static void residual_norms(afe_amg *g, const double *b, const double *x, double *r) {
+static bool verbose() {
+  static const bool v = getenv("AFE_VERBOSE") != nullptr;
+  return v;
+}
+
+// true residual r = b - A (x + xl) in compensated arithmetic, and its norms
+static void residual_norms(afe_amg *g, const double *b, const double *x, double *r,
+                           const double *xl = nullptr) {
   DevLevel &d = g->lv[0];
   hipStream_t st = g->ctx->stream;
-  k_residual_norms<<<RED_BLOCKS, BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, d.v64.p, d.diag64.p, b, x, r, g->part.p);
+  k_residual_norms_dd<<<RED_BLOCKS, BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, d.v64.p, d.diag64.p, b, x, xl, r,
+                                                   g->part.p);
   k_finalize<<<1, 256, 0, st>>>(g->part.p, RED_BLOCKS, 3, &g->sc.p->rr);
 }
 
@@ -1082,6 +1165,7 @@ extern "C" int afe_amg_solve(afe_amg *g, double *d_x, const double *d_r, double 
     rr = g->h_sc->rr;
     l1 = g->h_sc->l1;
     it++;
+    if (verbose()) fprintf(stderr, "cycle %lld rel res %.3e l1 %.3e\n", (long long)it, std::sqrt(rr / bb), l1);
     if (rtol > 0 && rr <= rtol * rtol * bb) {
       conv = true;
       break;
@@ -1137,8 +1221,8 @@ template <class T> static void issue_cg_iteration(afe_amg *g) {
   k_spmv_dot<<<RED_BLOCKS, BLK, 0, s>>>(d0.si.p, d0.ns, d0.cols.p, d0.v64.p, d0.diag64.p, g->wp.p, g->wq.p,
                                          g->part.p);
   k_finalize<<<1, 256, 0, s>>>(g->part.p, RED_BLOCKS, 1, &g->sc.p->pq);
-  k_cg_update<T><<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wx.p, g->wr.p, g->wp.p, g->wq.p, Prec<T>::b(d0),
-                                             Prec<T>::x(d0), g->part.p);
+  k_cg_update<T><<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wx.p, g->wxl.p, g->wr.p, g->wp.p, g->wq.p,
+                                             Prec<T>::b(d0), Prec<T>::x(d0), g->part.p);
   k_finalize<<<1, 256, 0, s>>>(g->part.p, RED_BLOCKS, 1, &g->sc.p->rr);
   HIPCHK(hipMemcpyAsync(g->h_sc, g->sc.p, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
   const int prec = std::is_same<T, float>::value ? 1 : 0;
@@ -1180,15 +1264,20 @@ static int pcg_impl(afe_amg *g, double *d_x, const double *d_b, double rtol, int
   HIPCHK(hipEventRecord(ctx->ev0, s));
   k_permute_in<double, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, d_b, g->wb.p);
   k_permute_in<double, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, d_x, g->wx.p);
-  residual_norms(g, g->wb.p, g->wx.p, g->wr.p);
+  k_fill<double><<<nblk(n), 256, 0, s>>>(n, g->wxl.p, 0.0);
+  residual_norms(g, g->wb.p, g->wx.p, g->wr.p, g->wxl.p);
   read_scalars(g);
   const double bb = g->h_sc->bb;
   const double init_l1 = g->h_sc->l1;
   double rr = g->h_sc->rr, l1 = init_l1;
   const double thr = rtol * rtol * bb;
+  // the pair x_hi + x_lo is driven to rtol/2 so that the single rounding to
+  // the delivered FP64 vector keeps its true residual below rtol
+  double pthr = 0.25 * thr;
   int it = 0;
-  bool conv = rr <= thr;
+  bool conv = rr <= pthr;
   int restarts = 0;
+  for (int outer = 0; outer < 4; outer++) {
   while (!conv && it < max_it) {
     k_fill<double><<<nblk(n), 256, 0, s>>>(n, g->wp.p, 0.0);
     precondition_perm<T>(g, g->wr.p, true);
@@ -1197,16 +1286,31 @@ static int pcg_impl(afe_amg *g, double *d_x, const double *d_b, double rtol, int
       HIPCHK(hipStreamSynchronize(s));
       it++;
       rr = g->h_sc->rr;
+      if (verbose())
+        fprintf(stderr, "pcg it %d rel res %.3e pq %.3e rz %.3e beta %.3e\n", it, std::sqrt(rr / bb),
+                g->h_sc->pq, g->h_sc->rz, g->h_sc->beta);
       if (!(rr == rr)) return fail(AFE_E_NOTCONVERGED, "CG breakdown (NaN residual)");
-      if (rr <= thr) break;
+      if (rr <= pthr) break;
     }
-    // true residual check; restart from it when the recurrence drifted
-    residual_norms(g, g->wb.p, g->wx.p, g->wr.p);
+    // true residual of the pair x_hi + x_lo; restart from it when the
+    // recurrence drifted
+    residual_norms(g, g->wb.p, g->wx.p, g->wr.p, g->wxl.p);
     read_scalars(g);
     rr = g->h_sc->rr;
-    l1 = g->h_sc->l1;
-    conv = rr <= thr;
+    conv = rr <= pthr;
     if (!conv && ++restarts > 20) break;
+  }
+  // deliver round(x_hi + x_lo) and check that vector's true residual
+  k_dd_round<<<nblk(n), 256, 0, s>>>(n, g->wx.p, g->wxl.p);
+  residual_norms(g, g->wb.p, g->wx.p, g->wr.p, nullptr);
+  read_scalars(g);
+  rr = g->h_sc->rr;
+  l1 = g->h_sc->l1;
+  if (verbose())
+    fprintf(stderr, "pcg delivered vector: rel res %.3e (pair converged %d)\n", std::sqrt(rr / bb), conv ? 1 : 0);
+  conv = rr <= thr;
+  if (conv || it >= max_it || restarts > 20) break;
+  pthr *= 0.25; // rounding floor reached: tighten the pair and continue
   }
   k_permute_out<double, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, g->wx.p, d_x);
   HIPCHK(hipEventRecord(ctx->ev1, s));
@@ -1258,4 +1362,79 @@ extern "C" int afe_amg_vcycle(afe_amg *g, double *d_z, const double *d_r, int32_
   }
   HIPCHK(hipStreamSynchronize(s));
   API_END
+}
+
+// ---------------------------------------------------------------------------
+// timing of the level-0 kernels as launched by the solvers (HIP events on the
+// context stream).  which: 0 FP64 SpMV with dot (CG q = A p), 1 one FP32
+// Gauss-Seidel sweep over all colours, 2 one FP64 sweep, 3 FP32 residual.
+// bytes: algorithmic bytes per launch (CSR formula of the reference pattern:
+// nnz*(sv+4) + (m+1)*4 + n*sv + m*sv, sweeps count x read and written and r
+// read: nnz*(sv+4) + (m+1)*4 + 3*m*sv).
+// ---------------------------------------------------------------------------
+extern "C" int afe_amg_kernel_timing(afe_amg *g, int32_t which, int32_t reps, double *avg_ms, double *bytes) {
+  API_BEGIN
+  if (!g || !g->ctx || reps < 1 || which < 0 || which > 3) return fail(AFE_E_ARG, "bad arguments");
+  if ((which == 1 || which == 3) && !g->has32) return fail(AFE_E_ARG, "no FP32 copies");
+  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  HIPCHK(hipSetDevice(g->ctx->device));
+  afe_ctx *ctx = g->ctx;
+  hipStream_t s = ctx->stream;
+  DevLevel &d = g->lv[0];
+  const int64_t n = d.n, nnz = g->h.A[0].nnz();
+  const double sv = (which == 1 || which == 3) ? 4.0 : 8.0;
+  if (which == 0)
+    *bytes = nnz * (sv + 4) + (n + 1) * 4.0 + 2.0 * n * sv;
+  else if (which == 3)
+    *bytes = nnz * (sv + 4) + (n + 1) * 4.0 + 3.0 * n * sv;
+  else
+    *bytes = nnz * (sv + 4) + (n + 1) * 4.0 + 3.0 * n * sv;
+  k_fill<double><<<nblk(n), 256, 0, s>>>(n, g->wp.p, 1.0);
+  k_fill<double><<<nblk(n), 256, 0, s>>>(n, d.x64.p, 0.0);
+  k_fill<double><<<nblk(n), 256, 0, s>>>(n, d.b64.p, 1.0);
+  if (g->has32) {
+    k_fill<float><<<nblk(n), 256, 0, s>>>(n, d.x32.p, 0.0f);
+    k_fill<float><<<nblk(n), 256, 0, s>>>(n, d.b32.p, 1.0f);
+  }
+  auto once = [&]() {
+    if (which == 0) {
+      k_spmv_dot<<<RED_BLOCKS, BLK, 0, s>>>(d.si.p, d.ns, d.cols.p, d.v64.p, d.diag64.p, g->wp.p, g->wq.p,
+                                             g->part.p);
+    } else if (which == 1) {
+      for (int c = 0; c < d.ncol; c++) gs_colour<float>(d, c, s);
+    } else if (which == 2) {
+      for (int c = 0; c < d.ncol; c++) gs_colour<double>(d, c, s);
+    } else {
+      k_residual<float><<<nblk(d.ns, WPB), BLK, 0, s>>>(d.si.p, d.ns, d.cols.p, d.v32.p, d.diag32.p, d.b32.p,
+                                                         d.x32.p, d.t32.p);
+    }
+  };
+  once();
+  HIPCHK(hipEventRecord(ctx->ev0, s));
+  for (int r = 0; r < reps; r++) once();
+  HIPCHK(hipEventRecord(ctx->ev1, s));
+  HIPCHK(hipEventSynchronize(ctx->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  *avg_ms = ms / reps;
+  API_END
+}
+
+// the solver's tunables after construction (smoothing sweeps per side)
+extern "C" int afe_amg_set_smooth(afe_amg *g, int32_t smooth) {
+  if (!g || smooth < 0 || smooth > 16) return fail(AFE_E_ARG, "bad smoothing count");
+  g->prm.smooth = smooth;
+  return AFE_OK;
+}
+
+// per-level device layout summary: rows, colours, stored sliced-ELL entries
+extern "C" int afe_amg_device_info(const afe_amg *g, int32_t level, int64_t *rows, int32_t *colours,
+                                   int64_t *stored, int32_t *tail_level) {
+  if (!g || !g->ctx || level < 0 || level >= (int)g->lv.size()) return fail(AFE_E_ARG, "bad level");
+  const DevLevel &d = g->lv[level];
+  if (rows) *rows = d.n;
+  if (colours) *colours = d.ncol;
+  if (stored) *stored = (int64_t)d.cols.n;
+  if (tail_level) *tail_level = g->tail0;
+  return AFE_OK;
 }

@@ -48,82 +48,27 @@ __device__ inline double field_g(const FieldDev &f, const double *x, double *gra
 }
 
 // ---------------------------------------------------------------------------
-// element geometry: barycentric gradients of an affine simplex
+// template-element plugins evaluated in the reference's floating-point order
 // ---------------------------------------------------------------------------
-// triangle: dl[i] = rot90(v_{i+1} - v_{i+2}) / (2|T|) (triangle.1.bas_fun.c:13-17);
-// jacobian = signed doubled area / reference doubled area (1)
-__device__ inline void tri_geom(const double v[3][2], double dl[3][2], double &jac) {
-  const double ar = (v[1][0] - v[0][0]) * (v[2][1] - v[0][1]) -
-                    (v[1][1] - v[0][1]) * (v[2][0] - v[0][0]);
-#pragma unroll
-  for (int i = 0; i < 3; i++) {
-    const int a = (i + 1) % 3, b = (i + 2) % 3;
-    dl[i][0] = (v[a][1] - v[b][1]) / ar;
-    dl[i][1] = (v[b][0] - v[a][0]) / ar;
-  }
-  jac = ar / 1.0;
+// The stiffness and mass kernels reproduce the reference's arithmetic step by
+// step -- quadrature points mapped through barycentric area ratios of the
+// reference simplex (triangle.crd_trs.c:15-27, tetrahedron.crd_trs.c:18-32),
+// basis functions evaluated at the physical point with the physical vertices
+// (triangle.1/2.bas_fun.c, tetrahedron.1.bas_fun.c), Jxw = w * jacobian *
+// volume and K(j,k) += Jxw * <g_j, g_k> -- with floating-point contraction
+// off, so each element matrix is bit-identical to the host restatement.
+#pragma clang fp contract(off)
+
+__device__ inline double area2(const double *a, const double *b, const double *c) {
+  return (b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0]);
 }
 
-// tetrahedron: grad lambda_m = -+ (v_b - v_a) x (v_c - v_a) / det
-// (tetrahedron.1.bas_fun.c:19-26,60-95); jacobian = (det/6) / (1/6)
-__device__ inline void tet_geom(const double v[4][3], double dl[4][3], double &jac) {
-  const double a0 = v[1][0] - v[0][0], a1 = v[1][1] - v[0][1], a2 = v[1][2] - v[0][2];
-  const double b0 = v[2][0] - v[0][0], b1 = v[2][1] - v[0][1], b2 = v[2][2] - v[0][2];
-  const double c0 = v[3][0] - v[0][0], c1 = v[3][1] - v[0][1], c2 = v[3][2] - v[0][2];
-  const double det = a0 * b1 * c2 + a1 * b2 * c0 + a2 * b0 * c1 - a0 * b2 * c1 -
-                     a1 * b0 * c2 - a2 * b1 * c0;
-#pragma unroll
-  for (int m = 0; m < 4; m++) {
-    const int ia = (m + 1) % 4, ib = (m + 2) % 4, ic = (m + 3) % 4;
-    const double sgn = (m % 2 == 0) ? -1. : 1.;
-#pragma unroll
-    for (int n = 0; n < 3; n++) {
-      const int n1 = (n + 1) % 3, n2 = (n + 2) % 3;
-      const double cof = sgn * ((v[ib][n1] - v[ia][n1]) * (v[ic][n2] - v[ia][n2]) -
-                                (v[ib][n2] - v[ia][n2]) * (v[ic][n1] - v[ia][n1]));
-      dl[m][n] = cof / det;
-    }
-  }
-  jac = (det / 6.) / (1. / 6.);
-}
-
-// basis values / gradients at barycentric coordinates lam (local order of
-// triangle.1 / triangle.2 / tetrahedron.1 .bas_fun)
-template <int KIND>
-__device__ inline void basis_val(const double *lam, double *val) {
-  if constexpr (KIND == P1_TRI) {
-    val[0] = lam[0]; val[1] = lam[1]; val[2] = lam[2];
-  } else if constexpr (KIND == P2_TRI) {
-    val[0] = lam[0] * (2 * lam[0] - 1.0);
-    val[1] = lam[1] * (2 * lam[1] - 1.0);
-    val[2] = lam[2] * (2 * lam[2] - 1.0);
-    val[3] = 4.0 * lam[1] * lam[2];
-    val[4] = 4.0 * lam[0] * lam[2];
-    val[5] = 4.0 * lam[0] * lam[1];
-  } else {
-    val[0] = lam[0]; val[1] = lam[1]; val[2] = lam[2]; val[3] = lam[3];
-  }
-}
-
-template <int KIND, int D>
-__device__ inline void basis_grad(const double *lam, const double dl[][D], double g[][D]) {
-  if constexpr (KIND == P2_TRI) {
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-      for (int d = 0; d < 2; d++) g[i][d] = (4.0 * lam[i] - 1.0) * dl[i][d];
-#pragma unroll
-    for (int d = 0; d < 2; d++) {
-      g[3][d] = 4.0 * (lam[2] * dl[1][d] + lam[1] * dl[2][d]);
-      g[4][d] = 4.0 * (lam[0] * dl[2][d] + lam[2] * dl[0][d]);
-      g[5][d] = 4.0 * (lam[1] * dl[0][d] + lam[0] * dl[1][d]);
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < D + 1; i++)
-#pragma unroll
-      for (int d = 0; d < D; d++) g[i][d] = dl[i][d];
-  }
+__device__ inline double det3(const double *v0, const double *v1, const double *v2,
+                              const double *v3) {
+  const double a0 = v1[0] - v0[0], a1 = v1[1] - v0[1], a2 = v1[2] - v0[2];
+  const double b0 = v2[0] - v0[0], b1 = v2[1] - v0[1], b2 = v2[2] - v0[2];
+  const double c0 = v3[0] - v0[0], c1 = v3[1] - v0[1], c2 = v3[2] - v0[2];
+  return a0 * b1 * c2 + a1 * b2 * c0 + a2 * b0 * c1 - a0 * b2 * c1 - a1 * b0 * c2 - a2 * b1 * c0;
 }
 
 template <int KIND> struct KindInfo;
@@ -143,24 +88,133 @@ __device__ inline void load_elem(int64_t e, const double *__restrict__ coords,
   }
 }
 
-template <int KIND>
-__device__ inline double geom(const double v[][KindInfo<KIND>::D], double dl[][KindInfo<KIND>::D]) {
-  double jac;
-  if constexpr (KindInfo<KIND>::D == 2)
-    tri_geom(v, dl, jac);
-  else
-    tet_geom(v, dl, jac);
-  return jac;
+// CoordTransform::local_to_global_jacobian
+template <int D> __device__ inline double l2g_jacobian(const double v[][D]) {
+  if constexpr (D == 2) {
+    const double r0[2] = {0.0, 0.0}, r1[2] = {1.0, 0.0}, r2[2] = {0.0, 1.0};
+    return area2(v[0], v[1], v[2]) / area2(r0, r1, r2);
+  } else {
+    const double r0[3] = {0.0, 0.0, 0.0}, r1[3] = {1.0, 0.0, 0.0}, r2[3] = {0.0, 1.0, 0.0},
+                 r3[3] = {0.0, 0.0, 1.0};
+    return (det3(v[0], v[1], v[2], v[3]) / 6.) / (det3(r0, r1, r2, r3) / 6.);
+  }
 }
 
-// reference-simplex volume (triangle.volume.c, tetrahedron.volume.c)
-template <int D> __device__ constexpr double ref_volume() { return D == 2 ? 0.5 : 1. / 6.; }
+// TemplateGeometry::volume of the reference simplex
+template <int D> __device__ inline double tmpl_volume() {
+  if constexpr (D == 2) {
+    return .5 * ((1.0 - 0.0) * (1.0 - 0.0) - (0.0 - 0.0) * (0.0 - 0.0));
+  } else {
+    const double r0[3] = {0.0, 0.0, 0.0}, r1[3] = {1.0, 0.0, 0.0}, r2[3] = {0.0, 1.0, 0.0},
+                 r3[3] = {0.0, 0.0, 1.0};
+    return det3(r0, r1, r2, r3) / 6.;
+  }
+}
+
+// CoordTransform::local_to_global of the reference point lp
+template <int D> __device__ inline void local_to_global(const double *lp, const double v[][D], double *x) {
+  if constexpr (D == 2) {
+    const double r0[2] = {0.0, 0.0}, r1[2] = {1.0, 0.0}, r2[2] = {0.0, 1.0};
+    const double ar = area2(r0, r1, r2);
+    const double l0 = area2(lp, r1, r2) / ar, l1 = area2(lp, r2, r0) / ar, l2 = area2(lp, r0, r1) / ar;
+    x[0] = l0 * v[0][0] + l1 * v[1][0] + l2 * v[2][0];
+    x[1] = l0 * v[0][1] + l1 * v[1][1] + l2 * v[2][1];
+  } else {
+    const double r0[3] = {0.0, 0.0, 0.0}, r1[3] = {1.0, 0.0, 0.0}, r2[3] = {0.0, 1.0, 0.0},
+                 r3[3] = {0.0, 0.0, 1.0};
+    const double vol = det3(r0, r1, r2, r3) / 6.;
+    const double l0 = det3(lp, r1, r2, r3) / 6. / vol, l1 = det3(r0, lp, r2, r3) / 6. / vol;
+    const double l2 = det3(r0, r1, lp, r3) / 6. / vol, l3 = det3(r0, r1, r2, lp) / 6. / vol;
+#pragma unroll
+    for (int d = 0; d < 3; d++) x[d] = l0 * v[0][d] + l1 * v[1][d] + l2 * v[2][d] + l3 * v[3][d];
+  }
+}
+
+// basis values at the physical point p (local order of the .bas_fun files)
+template <int KIND>
+__device__ inline void basis_val(const double *p, const double v[][KindInfo<KIND>::D], double *val) {
+  if constexpr (KIND == P1_TRI) {
+    const double ar = area2(v[0], v[1], v[2]);
+    val[0] = area2(p, v[1], v[2]) / ar;
+    val[1] = area2(v[0], p, v[2]) / ar;
+    val[2] = area2(v[0], v[1], p) / ar;
+  } else if constexpr (KIND == P2_TRI) {
+    const double ar = area2(v[0], v[1], v[2]);
+    const double l0 = area2(p, v[1], v[2]) / ar, l1 = area2(p, v[2], v[0]) / ar,
+                 l2 = area2(p, v[0], v[1]) / ar;
+    val[0] = l0 * (2 * l0 - 1.0);
+    val[1] = l1 * (2 * l1 - 1.0);
+    val[2] = l2 * (2 * l2 - 1.0);
+    val[3] = 4.0 * l1 * l2;
+    val[4] = 4.0 * l0 * l2;
+    val[5] = 4.0 * l0 * l1;
+  } else {
+    const double vol = det3(v[0], v[1], v[2], v[3]);
+    val[0] = det3(p, v[1], v[2], v[3]) / vol;
+    val[1] = det3(v[0], p, v[2], v[3]) / vol;
+    val[2] = det3(v[0], v[1], p, v[3]) / vol;
+    val[3] = det3(v[0], v[1], v[2], p) / vol;
+  }
+}
+
+// basis gradients at the physical point p
+template <int KIND>
+__device__ inline void basis_grad(const double *p, const double v[][KindInfo<KIND>::D],
+                                  double g[][KindInfo<KIND>::D]) {
+  if constexpr (KIND == P1_TRI || KIND == P2_TRI) {
+    const double ar = area2(v[0], v[1], v[2]);
+    double dl[3][2];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const int a = (i + 1) % 3, b = (i + 2) % 3;
+      dl[i][0] = (v[a][1] - v[b][1]) / ar;
+      dl[i][1] = (v[b][0] - v[a][0]) / ar;
+    }
+    if constexpr (KIND == P1_TRI) {
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        g[i][0] = dl[i][0];
+        g[i][1] = dl[i][1];
+      }
+    } else {
+      const double l0 = area2(p, v[1], v[2]) / ar, l1 = area2(p, v[2], v[0]) / ar,
+                   l2 = area2(p, v[0], v[1]) / ar;
+      const double lam[3] = {l0, l1, l2};
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int d = 0; d < 2; d++) g[i][d] = (4.0 * lam[i] - 1.0) * dl[i][d];
+#pragma unroll
+      for (int d = 0; d < 2; d++) {
+        g[3][d] = 4.0 * (l2 * dl[1][d] + l1 * dl[2][d]);
+        g[4][d] = 4.0 * (l0 * dl[2][d] + l2 * dl[0][d]);
+        g[5][d] = 4.0 * (l1 * dl[0][d] + l0 * dl[1][d]);
+      }
+    }
+  } else {
+    const double vol = det3(v[0], v[1], v[2], v[3]);
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const int ia = (m + 1) % 4, ib = (m + 2) % 4, ic = (m + 3) % 4;
+      const double sgn = (m % 2 == 0) ? -1. : 1.;
+#pragma unroll
+      for (int n = 0; n < 3; n++) {
+        const int n1 = (n + 1) % 3, n2 = (n + 2) % 3;
+        const double cof = sgn * ((v[ib][n1] - v[ia][n1]) * (v[ic][n2] - v[ia][n2]) -
+                                  (v[ib][n2] - v[ia][n2]) * (v[ic][n1] - v[ia][n1]));
+        g[m][n] = cof / vol;
+      }
+    }
+  }
+}
 
 // ---------------------------------------------------------------------------
 // element matrices: StiffMatrix / MassMatrix / variable-coefficient stiffness
 // (BilinearOperator.templates.h:342-406, EllipticEquation.cpp:65-108):
-//   K(j,k) += w_l * jac * |ref| [* a(x_l)] * <g_j, g_k>   (or * v_j v_k)
-// one thread per element, K row-major per element (NL*NL doubles)
+//   K(j,k) += w_l * jacobian * volume [* a(x_l)] * <g_j, g_k>  (or * v_j v_k)
+// one thread per element, K row-major per element (NL*NL doubles); for the
+// stiffness forms the lower triangle mirrors the upper one (<g_j,g_k> and
+// <g_k,g_j> are the same products), the mass form computes every entry
 // ---------------------------------------------------------------------------
 template <int KIND, int OP>
 __global__ __launch_bounds__(256) void k_element_matrices(
@@ -169,9 +223,10 @@ __global__ __launch_bounds__(256) void k_element_matrices(
   constexpr int D = KindInfo<KIND>::D, NL = KindInfo<KIND>::NL;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_elem) return;
-  double v[D + 1][D], dl[D + 1][D];
+  double v[D + 1][D];
   load_elem<KIND>(e, coords, ev, v);
-  const double jac = geom<KIND>(v, dl);
+  const double volume = tmpl_volume<D>();
+  const double jac = l2g_jacobian<D>(v);
   double K[NL][NL];
 #pragma unroll
   for (int j = 0; j < NL; j++)
@@ -179,35 +234,21 @@ __global__ __launch_bounds__(256) void k_element_matrices(
     for (int k = 0; k < NL; k++) K[j][k] = 0.0;
   for (int l = 0; l < nq; l++) {
     const double *q = qd + l * (D + 1);
-    double lam[D + 1];
-    lam[0] = 1.0;
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      lam[d + 1] = q[d];
-      lam[0] -= q[d];
-    }
-    const double Jxw = q[D] * jac * ref_volume<D>();
+    double x[D];
+    local_to_global<D>(q, v, x);
+    const double Jxw = q[D] * jac * volume;
     if constexpr (OP == 1) { // mass
       double val[NL];
-      basis_val<KIND>(lam, val);
+      basis_val<KIND>(x, v, val);
 #pragma unroll
       for (int j = 0; j < NL; j++)
 #pragma unroll
-        for (int k = j; k < NL; k++) K[j][k] += Jxw * val[j] * val[k];
+        for (int k = 0; k < NL; k++) K[j][k] += Jxw * val[j] * val[k];
     } else {
       double g[NL][D];
-      basis_grad<KIND, D>(lam, dl, g);
+      basis_grad<KIND>(x, v, g);
       double coef = 1.0;
-      if constexpr (OP == 2) {
-        double x[D];
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-          x[d] = 0.0;
-#pragma unroll
-          for (int j = 0; j <= D; j++) x[d] += lam[j] * v[j][d];
-        }
-        coef = exp(field_g(fld, x, nullptr));
-      }
+      if constexpr (OP == 2) coef = exp(field_g(fld, x, nullptr));
 #pragma unroll
       for (int j = 0; j < NL; j++)
 #pragma unroll
@@ -226,7 +267,7 @@ __global__ __launch_bounds__(256) void k_element_matrices(
 #pragma unroll
   for (int j = 0; j < NL; j++)
 #pragma unroll
-    for (int k = 0; k < NL; k++) out[j * NL + k] = (k >= j) ? K[j][k] : K[k][j];
+    for (int k = 0; k < NL; k++) out[j * NL + k] = (OP == 1 || k >= j) ? K[j][k] : K[k][j];
 }
 
 // ---------------------------------------------------------------------------
@@ -274,30 +315,19 @@ __global__ __launch_bounds__(256) void k_element_loads(
   constexpr int D = KindInfo<KIND>::D, NL = KindInfo<KIND>::NL;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_elem) return;
-  double v[D + 1][D], dl[D + 1][D];
+  double v[D + 1][D];
   load_elem<KIND>(e, coords, ev, v);
-  const double jac = geom<KIND>(v, dl);
+  const double volume = tmpl_volume<D>();
+  const double jac = l2g_jacobian<D>(v);
   double F[NL];
 #pragma unroll
   for (int j = 0; j < NL; j++) F[j] = 0.0;
   for (int l = 0; l < nq; l++) {
     const double *q = qd + l * (D + 1);
-    double lam[D + 1];
-    lam[0] = 1.0;
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      lam[d + 1] = q[d];
-      lam[0] -= q[d];
-    }
     double x[D];
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      x[d] = 0.0;
-#pragma unroll
-      for (int j = 0; j <= D; j++) x[d] += lam[j] * v[j][d];
-    }
+    local_to_global<D>(q, v, x);
     double val[NL];
-    basis_val<KIND>(lam, val);
+    basis_val<KIND>(x, v, val);
     double f;
     if (fkind == 3) {
       f = 1.0;
@@ -328,7 +358,7 @@ __global__ __launch_bounds__(256) void k_element_loads(
         f = exp(g) * (D * M_PI * M_PI * u - dot);
       }
     }
-    const double Jxw = q[D] * jac * ref_volume<D>();
+    const double Jxw = q[D] * jac * volume;
 #pragma unroll
     for (int j = 0; j < NL; j++) F[j] += Jxw * f * val[j];
   }
@@ -380,33 +410,22 @@ __global__ __launch_bounds__(256) void k_element_error(
   constexpr int D = KindInfo<KIND>::D, NL = KindInfo<KIND>::NL;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_elem) return;
-  double v[D + 1][D], dl[D + 1][D];
+  double v[D + 1][D];
   load_elem<KIND>(e, coords, ev, v);
-  const double jac = geom<KIND>(v, dl);
+  const double volume = tmpl_volume<D>();
+  const double jac = l2g_jacobian<D>(v);
   double u_loc[NL];
 #pragma unroll
   for (int j = 0; j < NL; j++) u_loc[j] = uh[edof[e * NL + j]];
   double err = 0.0;
   for (int l = 0; l < nq; l++) {
     const double *q = qd + l * (D + 1);
-    double lam[D + 1];
-    lam[0] = 1.0;
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      lam[d + 1] = q[d];
-      lam[0] -= q[d];
-    }
     double x[D];
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      x[d] = 0.0;
-#pragma unroll
-      for (int j = 0; j <= D; j++) x[d] += lam[j] * v[j][d];
-    }
-    const double Jxw = q[D] * jac * ref_volume<D>();
+    local_to_global<D>(q, v, x);
+    const double Jxw = q[D] * jac * volume;
     if (!h1) {
       double val[NL];
-      basis_val<KIND>(lam, val);
+      basis_val<KIND>(x, v, val);
       double uhv = 0.0, ue = 1.0;
 #pragma unroll
       for (int j = 0; j < NL; j++) uhv += u_loc[j] * val[j];
@@ -415,7 +434,7 @@ __global__ __launch_bounds__(256) void k_element_error(
       err += Jxw * (ue - uhv) * (ue - uhv);
     } else {
       double g[NL][D];
-      basis_grad<KIND, D>(lam, dl, g);
+      basis_grad<KIND>(x, v, g);
 #pragma unroll
       for (int d = 0; d < D; d++) {
         double gh = 0.0;
@@ -431,6 +450,8 @@ __global__ __launch_bounds__(256) void k_element_error(
   }
   out[e] = err;
 }
+
+#pragma clang fp contract(on)
 
 // ---------------------------------------------------------------------------
 // fixed-order reductions
@@ -742,16 +763,95 @@ __global__ __launch_bounds__(BLK) void k_residual_norms(const SliceInfo *__restr
   }
 }
 
+// error-free transformations (no contraction inside: fp contract is off in
+// this region; the product error uses an explicit fma)
+#pragma clang fp contract(off)
+__device__ inline void two_sum(double a, double b, double &s, double &e) {
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// r = b - A (x + xl) evaluated with compensated (error-free) products and
+// sums, and the partials of (r, r), (b, b), sum |r|; xl may be null
+__global__ __launch_bounds__(BLK) void k_residual_norms_dd(const SliceInfo *__restrict__ si, int ns,
+                                                           const int32_t *__restrict__ cols,
+                                                           const double *__restrict__ vals,
+                                                           const double *__restrict__ diag,
+                                                           const double *__restrict__ b,
+                                                           const double *__restrict__ x,
+                                                           const double *__restrict__ xl,
+                                                           double *__restrict__ r, double *__restrict__ part) {
+  __shared__ double sh[BLK];
+  const int lane = threadIdx.x % WAVE;
+  double rr = 0.0, bb = 0.0, l1 = 0.0;
+  for (int s = blockIdx.x * WPB + wave_id(); s < ns; s += gridDim.x * WPB) {
+    const SliceInfo q = si[s];
+    if (lane < q.nrow) {
+      const int row = q.row0 + lane;
+      double hi = b[row], lo = 0.0;
+      {
+        const double a = diag[row], xv = x[row];
+        const double pr = a * xv, pe = fma(a, xv, -pr);
+        double t, e;
+        two_sum(hi, -pr, t, e);
+        hi = t;
+        lo += e - pe;
+        if (xl) lo -= a * xl[row];
+      }
+      const int32_t *c = cols + q.off + lane;
+      const double *v = vals + q.off + lane;
+      for (int j = 0; j < q.width; j++) {
+        const int32_t cj = c[j * WAVE];
+        const double a = v[j * WAVE], xv = x[cj];
+        const double pr = a * xv, pe = fma(a, xv, -pr);
+        double t, e;
+        two_sum(hi, -pr, t, e);
+        hi = t;
+        lo += e - pe;
+        if (xl) lo -= a * xl[cj];
+      }
+      const double rv = hi + lo;
+      if (r) r[row] = rv;
+      rr += rv * rv;
+      bb += b[row] * b[row];
+      l1 += fabs(rv);
+    }
+  }
+  rr = block_sum(rr, sh);
+  bb = block_sum(bb, sh);
+  l1 = block_sum(l1, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = rr;
+    part[gridDim.x + blockIdx.x] = bb;
+    part[2 * gridDim.x + blockIdx.x] = l1;
+  }
+}
+
+// x = round(x + xl), xl = 0
+__global__ __launch_bounds__(256) void k_dd_round(int64_t n, double *__restrict__ x, double *__restrict__ xl) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    x[i] = x[i] + xl[i];
+    xl[i] = 0.0;
+  }
+}
+#pragma clang fp contract(on)
+
 // device scalars of the CG loop; k_finalize writes consecutive fields
 // (rz_new, rz_cross) and (rr, bb, l1) in one call
 struct CgScalars {
   double rz, pq, rz_new, rz_cross, rr, bb, l1, beta;
 };
 
-// alpha = rz / pq; x += alpha p; r -= alpha q; (T) r -> b0, x0 = 0; partial (r, r)
+#pragma clang fp contract(off)
+// alpha = rz / pq; x += alpha p kept as the unevaluated pair x_hi + x_lo
+// (the sum and the product errors go to x_lo); r -= alpha q;
+// (T) r -> b0, x0 = 0; partial (r, r)
 template <class T>
 __global__ __launch_bounds__(256) void k_cg_update(int64_t n, const CgScalars *__restrict__ sc,
-                                                   double *__restrict__ x, double *__restrict__ r,
+                                                   double *__restrict__ x, double *__restrict__ xl,
+                                                   double *__restrict__ r,
                                                    const double *__restrict__ p, const double *__restrict__ q,
                                                    T *__restrict__ b0, T *__restrict__ x0,
                                                    double *__restrict__ part) {
@@ -760,7 +860,12 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n, const CgScalars *_
   double rr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    x[i] += alpha * p[i];
+    const double ap = alpha * p[i];
+    const double ape = fma(alpha, p[i], -ap);
+    double s, e;
+    two_sum(x[i], ap, s, e);
+    x[i] = s;
+    xl[i] += e + ape;
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
     b0[i] = (T)ri;
@@ -770,6 +875,8 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n, const CgScalars *_
   rr = block_sum(rr, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = rr;
 }
+
+#pragma clang fp contract(on)
 
 // (T) r -> b0, x0 = 0 (start of a preconditioner application)
 template <class T>

@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""Solved DOF/s at 1e-10 relative residual on one MI355X.
+
+Workload (one "step" = one solve of this system from x0 = 0 with b resident
+in HBM): the 2D Poisson problem -Lap u = 2 pi^2 sin(pi x) sin(pi y), u = 0 on
+the boundary, on 1024 x 1024 squares split into 2 triangles each, interior
+nodes jittered by U(-0.2h, 0.2h) (seeded), P2 Lagrange elements (4,198,401
+DOFs), quadrature accuracy 6.  Assembly (FP64, device) and the host AMG setup
+are timed separately and are not part of a step.
+
+Forms measured:
+  mixed  -- FP64 preconditioned CG, one FP32 symmetric V(s,s)-cycle per
+            iteration (the reported `value`);
+  plain  -- the reference's stationary FP64 V(3,3)-cycle iteration
+            (AMGSolver::solve semantics) run until 1e-10;
+  pcg64  -- FP64 PCG with an FP64 V-cycle.
+All three use the lazy hierarchy (AMGSolver::lazyReinit, the reference's
+AMGPreconditioner path).  The CPU baseline is the oracle's restatement of
+AMGSolver::solve on the same matrix, timed on a bounded number of V-cycles.
+
+Prints one JSON line (rank 0).  --gpus N with N > 1 is not supported: the
+path runs on one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(rp, col, val, b, budget_s, n_dof):
+    """Oracle AMGSolver (lazy hierarchy, lexicographic V(3,3)) on the same
+    matrix: time a bounded number of cycles and scale by the cycle count the
+    oracle's own contraction needs for 1e-10 (L1, the reference's norm)."""
+    import numpy as np
+    from oracle import oracle_py as orc
+    t0 = time.time()
+    M = orc.Mat.from_csr(len(rp) - 1, len(rp) - 1, rp, col, val)
+    g = orc.Amg(M, lazy=True)
+    t_setup = time.time() - t0
+    x = np.zeros(n_dof)
+    # probe: one cycle to size the sample
+    t1 = time.perf_counter()
+    h1 = g.solve(x, b, smooth=3, step=0)
+    t_cyc = time.perf_counter() - t1
+    k = max(2, min(12, int(budget_s / max(t_cyc, 1e-3))))
+    x[:] = 0
+    t1 = time.perf_counter()
+    hist = g.solve(x, b, smooth=3, step=k - 1)
+    t_k = time.perf_counter() - t1
+    rho = (hist[k] / hist[1]) ** (1.0 / (k - 1)) if k > 1 and hist[1] > 0 else float("nan")
+    cycles = math.ceil(math.log(1e-10) / math.log(rho)) if 0 < rho < 1 else float("inf")
+    per_cycle = t_k / k
+    value = n_dof / (cycles * per_cycle) if math.isfinite(cycles) else 0.0
+    g.free()
+    return {
+        "value": value, "unit": "DOF/s", "cores": 1, "kind": "port",
+        "sample": (f"{k} stationary V(3,3) cycles of the oracle AMGSolver (lazy hierarchy, "
+                   f"lexicographic Gauss-Seidel) on the same {n_dof}-DOF system: "
+                   f"{per_cycle:.3f} s/cycle, L1 contraction {rho:.4f}/cycle -> "
+                   f"{cycles} cycles to 1e-10; hierarchy setup {t_setup:.1f} s untimed; "
+                   f"cpu {os.cpu_count()} visible, 1 used"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--seed", type=int, default=20261)
+    ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--smooth", type=int, default=0, help="V(s,s) for the PCG forms; 0 = autotune")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this path runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    import numpy as np
+    import afepack_amd as afe
+
+    n, acc = args.n, 6
+    ctx = afe.Context(0)
+    t = time.time()
+    coords, ev, vm = afe.mesh_square(n, 0.2, args.seed)
+    sp = afe.Space(ctx, afe.P2_TRI, coords, ev, vm)
+    t_space = time.time() - t
+    log(f"space: {sp.n_dof} DOFs, nnz {sp.nnz}, {t_space:.2f} s")
+    A = afe.Matrix(sp)
+    A.assemble(afe.OP_STIFF, acc)  # first call (module load, allocation)
+    asm = []
+    for _ in range(3):
+        t = time.perf_counter()
+        A.assemble(afe.OP_STIFF, acc)
+        asm.append(time.perf_counter() - t)
+    t_asm = min(asm)
+    b = sp.rhs(acc, afe.F_POISSON)
+    A.apply_dirichlet(b)
+    t = time.time()
+    g = afe.Amg(A, afe.AmgParams(lazy=1, smooth=3, build_fp32=1))
+    t_amg = time.time() - t
+    info = g.info()
+    dinfo = g.device_info()
+    log(f"AMG setup {t_amg:.2f} s: levels {info['levels']} n {info['n']} "
+        f"colours {[d['colours'] for d in dinfo]} tail from level {dinfo[0]['tail']}")
+    x = afe.DeviceVector(ctx, sp.n_dof)
+
+    def solve(prec, s):
+        g.set_smooth(s)
+        x.zero()
+        t0 = time.perf_counter()
+        st = g.pcg(x, b, rtol=args.rtol, max_it=1000, prec=prec)
+        return time.perf_counter() - t0, st
+
+    # autotune the PCG smoothing count on the mixed form
+    s_mixed = args.smooth
+    if s_mixed == 0:
+        best = None
+        for s in (1, 2):
+            solve(afe.PREC_FP32, s)
+            dt, st = solve(afe.PREC_FP32, s)
+            log(f"tune V({s},{s}): {dt*1e3:.2f} ms, {st.iterations} its")
+            if best is None or dt < best[0]:
+                best = (dt, s)
+        s_mixed = best[1]
+    for _ in range(args.warmup):
+        solve(afe.PREC_FP32, s_mixed)
+    lat, its = [], []
+    for _ in range(args.steps):
+        dt, st = solve(afe.PREC_FP32, s_mixed)
+        lat.append(dt)
+        its.append(st.iterations)
+        rel = st.rel_res_l2
+    xs = x.numpy()
+    ms = 1e3 * statistics.mean(lat)
+    value = sp.n_dof / statistics.mean(lat)
+    log(f"mixed: {ms:.2f} ms/solve, {its[-1]} its, rel res {rel:.2e}, {value:.3e} DOF/s")
+
+    # FP64 PCG with the same hierarchy and smoothing
+    solve(afe.PREC_FP64, s_mixed)
+    t64, st64 = solve(afe.PREC_FP64, s_mixed)
+    x64 = x.numpy()
+    log(f"pcg64: {t64*1e3:.2f} ms, {st64.iterations} its")
+
+    # the reference's stationary iteration in FP64 on the device
+    plain = None
+    if not args.no_plain:
+        g.set_smooth(3)
+        x.zero()
+        t0 = time.perf_counter()
+        stp = g.solve(x, b, step=1500, mode=1, rtol=args.rtol)
+        tp = time.perf_counter() - t0
+        plain = {"form": "stationary FP64 V(3,3) (AMGSolver::solve)", "seconds": tp,
+                 "cycles": stp.iterations, "converged": bool(stp.converged),
+                 "dof_per_s": (sp.n_dof / tp) if stp.converged else 0.0,
+                 "rel_res_at_exit": stp.rel_res_l2}
+        log(f"plain: {tp*1e3:.2f} ms, {stp.iterations} cycles, conv {stp.converged}")
+
+    # live roofline of the dominant kernels (HIP events on the solver stream)
+    g.set_smooth(s_mixed)
+    k_ms = {}
+    for which, name in ((0, "spmv_fp64"), (1, "gs_sweep_fp32"), (3, "residual_fp32")):
+        ms_k, by = g.kernel_timing(which, reps=20)
+        k_ms[name] = (ms_k, by)
+        log(f"{name}: {ms_k*1e3:.1f} us/launch, {by/ms_k/1e6:.0f} GB/s algorithmic")
+    dom_ms, dom_by = k_ms["spmv_fp64"]
+    achieved = dom_by / dom_ms / 1e6
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_spmv_fp64_c2.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+
+    # discretisation error and agreement of the two precisions
+    e2 = sp.error_l2(x, acc) if False else None
+    diff = float(np.linalg.norm(xs - x64) / np.linalg.norm(x64))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        rp, col = sp.pattern()
+        val = A.values()
+        bh = b.numpy()
+        log("cpu baseline ...")
+        cpu = cpu_baseline(rp, col, val, bh, args.cpu_seconds, sp.n_dof)
+        log(f"cpu: {cpu['value']:.3e} DOF/s")
+
+    lat_sorted = sorted(lat)
+    p50 = statistics.median(lat_sorted)
+    p99 = lat_sorted[min(len(lat_sorted) - 1, int(math.ceil(0.99 * len(lat_sorted))) - 1)]
+    out = {
+        "metric": "solved DOF/s at 1e-10 rel. residual, plain FP64 vs mixed AMG; p50/p99 latency",
+        "value": value,
+        "unit": "DOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (CG, residual, assembly, coarse solve) + f32 (V-cycle)",
+        "data": f"synthetic: seeded jittered mesh (seed {args.seed}), manufactured rhs",
+        "config": {
+            "workload": f"2D P2 Poisson, {n}x{n} squares x 2 triangles, jitter 0.2h, acc 6",
+            "n_dof": sp.n_dof, "nnz": sp.nnz, "rtol": args.rtol,
+            "hierarchy": "lazy (AMGSolver::lazyReinit)", "levels": info["levels"],
+            "level_n": info["n"], "colours": [d["colours"] for d in dinfo],
+            "pcg_smooth": s_mixed, "parallelism": "single GPU",
+        },
+        "mixed": {"iterations": its[-1], "rel_res": rel, "p50_ms": 1e3 * p50, "p99_ms": 1e3 * p99},
+        "pcg64": {"seconds": t64, "iterations": st64.iterations, "dof_per_s": sp.n_dof / t64},
+        "plain": plain,
+        "mixed_vs_plain_stationary": (value / plain["dof_per_s"]) if plain and plain["dof_per_s"] else None,
+        "mixed_vs_pcg64": value / (sp.n_dof / t64),
+        "mixed_vs_fp64_solution_rel_l2": diff,
+        "assembly_ms": 1e3 * t_asm,
+        "setup_s": {"space_pattern": t_space, "amg_host_and_upload": t_amg},
+        "kernels_us": {k: 1e3 * v[0] for k, v in k_ms.items()},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_spmv_dot (level-0 FP64 sliced-ELL SpMV, CG q = A p)"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -97,6 +97,15 @@ int afe_ctx_synchronize(afe_ctx *ctx);
 /* bytes of device memory held by live objects of this context */
 int afe_ctx_device_bytes(afe_ctx *ctx, int64_t *bytes);
 
+/* ---- device memory (plain HBM buffers for the d_* arguments) ------------- */
+int afe_dev_alloc(afe_ctx *ctx, int64_t bytes, void **d_ptr);
+int afe_dev_free(afe_ctx *ctx, void *d_ptr);
+int afe_dev_upload(afe_ctx *ctx, void *d_dst, const void *h_src, int64_t bytes);
+int afe_dev_download(afe_ctx *ctx, void *h_dst, const void *d_src,
+                     int64_t bytes);
+int afe_dev_copy(afe_ctx *ctx, void *d_dst, const void *d_src, int64_t bytes);
+int afe_dev_zero(afe_ctx *ctx, void *d_ptr, int64_t bytes);
+
 /* ---- synthetic meshes (the benchmark inputs) ---------------------------- */
 /* 2D: n x n squares on the unit square, split (n00,n10,n11),(n00,n11,n01) as
  * examples/FAST/src/poisson_lshape_stretched_afepack.cpp:262-286; interior
@@ -227,6 +236,20 @@ int afe_amg_pcg(afe_amg *g, double *d_x, const double *d_b, double rtol,
                 int32_t max_it, int32_t prec, afe_solve_stats *st);
 /* one symmetric V-cycle z = M^{-1} r from zero (prec as above) */
 int afe_amg_vcycle(afe_amg *g, double *d_z, const double *d_r, int32_t prec);
+/* AMGSolver::smoothStep() = s (AMGSolver.h:98-99) after construction */
+int afe_amg_set_smooth(afe_amg *g, int32_t smooth);
+
+/* ---- measurement hooks (no counterpart in the reference) ------------------ */
+/* average duration of `reps` back-to-back launches of a level-0 kernel as the
+ * solvers launch it, timed with HIP events on the context stream; which:
+ * 0 FP64 SpMV (CG q = A p), 1 FP32 Gauss-Seidel sweep (all colours),
+ * 2 FP64 sweep, 3 FP32 residual.  *bytes = algorithmic bytes per launch. */
+int afe_amg_kernel_timing(afe_amg *g, int32_t which, int32_t reps,
+                          double *avg_ms, double *bytes);
+/* device layout of a level: rows, colours, stored sliced-ELL entries and the
+ * first level handled by the single-workgroup coarse tail */
+int afe_amg_device_info(const afe_amg *g, int32_t level, int64_t *rows,
+                        int32_t *colours, int64_t *stored, int32_t *tail_level);
 
 #ifdef __cplusplus
 }
