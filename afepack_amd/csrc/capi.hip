@@ -709,6 +709,7 @@ extern "C" int afe_error_h1(afe_space *s, const double *d_uh, int32_t acc, doubl
 // AMG hierarchy on the device
 // ===========================================================================
 constexpr int64_t TAIL_MAX_ROWS = 4096;
+constexpr int64_t COARSE_TAIL_MAX = 256;
 
 struct DevLevel {
   int64_t n = 0;
@@ -732,6 +733,7 @@ struct afe_amg {
   std::vector<DevLevel> lv;
   int tail0 = 0;       // first level handled by the single-workgroup tail
   int coarse_mode = 0; // 0 dense inverse, 1 diagonal, 2 GS
+  bool big_coarse = false; // dense coarsest level solved by k_coarse_gemv
   DevBuf<double> minv;
   DevBuf<LevelDev<double>> ld64;
   DevBuf<LevelDev<float>> ld32;
@@ -862,11 +864,22 @@ static void upload_hierarchy(afe_amg *g) {
     d.perm = perm[l];
   }
   g->perm0.upload(ctx, perm[0]);
-  // first level of the coarse tail
-  g->tail0 = L - 1;
-  while (g->tail0 > 0 && g->lv[g->tail0 - 1].n <= TAIL_MAX_ROWS) g->tail0--;
+  // first level of the coarse tail; a large dense coarsest level is solved by
+  // the multi-block gemv instead of the single-workgroup tail
   g->coarse_mode = h.full ? 0 : 1;
-  if (h.full) g->minv.upload(ctx, h.minv);
+  g->big_coarse = h.full && g->lv[L - 1].n > COARSE_TAIL_MAX;
+  g->tail0 = L - 1;
+  if (!g->big_coarse)
+    while (g->tail0 > 0 && g->lv[g->tail0 - 1].n <= TAIL_MAX_ROWS) g->tail0--;
+  if (h.full) {
+    // the coarse vectors are in the level's colour order: permute the inverse
+    const std::vector<int32_t> &pc = perm[L - 1];
+    const int64_t nc = (int64_t)pc.size();
+    std::vector<double> mp((size_t)nc * nc);
+    for (int64_t i = 0; i < nc; i++)
+      for (int64_t j = 0; j < nc; j++) mp[(size_t)i * nc + j] = h.minv[(size_t)pc[i] * nc + pc[j]];
+    g->minv.upload(ctx, mp);
+  }
   // level descriptors for the tail kernel
   std::vector<LevelDev<double>> d64(L);
   std::vector<LevelDev<float>> d32(L);
@@ -1084,8 +1097,13 @@ static void issue_vcycle(afe_amg *g, int nu, bool symmetric, hipStream_t st) {
     k_restrict<T><<<nblk(d.p_ns, WPB), BLK, 0, st>>>(d.p_si.p, d.p_ns, d.p_cols.p, Prec<T>::pv(d),
                                                       Prec<T>::t(d), Prec<T>::b(dn), Prec<T>::x(dn));
   }
-  k_tail<T><<<1, TAIL_THREADS, 0, st>>>(Prec<T>::ld(g), g->tail0, Lc, g->minv.p, g->coarse_mode, nu, nu,
-                                         symmetric ? 1 : 0);
+  if (g->big_coarse) {
+    DevLevel &dc = g->lv[Lc];
+    k_coarse_gemv<T><<<nblk(dc.n, WPB), BLK, 0, st>>>((int)dc.n, g->minv.p, Prec<T>::b(dc), Prec<T>::x(dc));
+  } else {
+    k_tail<T><<<1, TAIL_THREADS, 0, st>>>(Prec<T>::ld(g), g->tail0, Lc, g->minv.p, g->coarse_mode, nu, nu,
+                                           symmetric ? 1 : 0);
+  }
   for (int l = g->tail0 - 1; l >= 0; l--) {
     DevLevel &d = g->lv[l];
     DevLevel &dn = g->lv[l + 1];
@@ -1096,8 +1114,22 @@ static void issue_vcycle(afe_amg *g, int nu, bool symmetric, hipStream_t st) {
   }
 }
 
+// AFE_NO_GRAPH=1 issues the captured sequences as plain launches (for tools
+// that do not follow graph launches)
+static bool use_graphs() {
+  static const bool v = getenv("AFE_NO_GRAPH") == nullptr;
+  return v;
+}
+
 static void launch_graph_or_issue(afe_amg *g, int prec, bool symmetric, int nu) {
   hipStream_t st = g->ctx->stream;
+  if (!use_graphs()) {
+    if (prec == 1)
+      issue_vcycle<float>(g, nu, symmetric, st);
+    else
+      issue_vcycle<double>(g, nu, symmetric, st);
+    return;
+  }
   hipGraphExec_t &ex = g->vc_graph[prec][symmetric ? 1 : 0];
   if (ex && g->graph_nu[prec][symmetric ? 1 : 0] != nu) {
     (void)hipGraphExecDestroy(ex);
@@ -1238,6 +1270,10 @@ template <class T> static void issue_cg_iteration(afe_amg *g) {
 template <class T> static void launch_cg_iteration(afe_amg *g) {
   const int prec = std::is_same<T, float>::value ? 1 : 0;
   hipStream_t st = g->ctx->stream;
+  if (!use_graphs()) {
+    issue_cg_iteration<T>(g);
+    return;
+  }
   hipGraphExec_t &ex = g->cg_graph[prec];
   if (ex && g->cg_nu[prec] != g->prm.smooth) {
     (void)hipGraphExecDestroy(ex);

@@ -585,13 +585,15 @@ int dense_inverse(int n, double *a) {
       a[c * n + j] /= piv;
       b[c * n + j] /= piv;
     }
+    // row updates are independent: parallel for large coarse levels
+#pragma omp parallel for schedule(static) if (n > 256)
     for (int r = 0; r < n; r++) {
       if (r == c) continue;
-      const double f = a[r * n + c];
+      const double f = a[(size_t)r * n + c];
       if (f == 0.0) continue;
       for (int j = 0; j < n; j++) {
-        a[r * n + j] -= f * a[c * n + j];
-        b[r * n + j] -= f * b[c * n + j];
+        a[(size_t)r * n + j] -= f * a[(size_t)c * n + j];
+        b[(size_t)r * n + j] -= f * b[(size_t)c * n + j];
       }
     }
   }

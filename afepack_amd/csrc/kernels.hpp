@@ -684,6 +684,32 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(const LevelDev<T> *__rest
   }
 }
 
+// dense coarse solve x = Minv b with the FP64 inverse (DenseMatrix::vmult of
+// the coarsest level, src/AMGSolver.cpp:591-593): one wavefront per row,
+// coalesced row reads, FP64 accumulation; for coarse levels too large for
+// the single-workgroup tail
+template <class T>
+__global__ __launch_bounds__(BLK) void k_coarse_gemv(int n, const double *__restrict__ minv,
+                                                     const T *__restrict__ b, T *__restrict__ x) {
+  __shared__ double sh[BLK];
+  const int row = blockIdx.x * WPB + wave_id();
+  const int lane = threadIdx.x % WAVE;
+  double s = 0.0;
+  if (row < n) {
+    const double *m = minv + (int64_t)row * n;
+    for (int j = lane; j < n; j += WAVE) s += m[j] * (double)b[j];
+  }
+  // wavefront reduction through LDS (fixed order)
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  const int base = threadIdx.x - lane;
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    if (lane < o) sh[base + lane] += sh[base + lane + o];
+    __syncthreads();
+  }
+  if (row < n && lane == 0) x[row] = (T)sh[base];
+}
+
 // ---------------------------------------------------------------------------
 // vector kernels (level-0 numbering after the colour permutation)
 // ---------------------------------------------------------------------------

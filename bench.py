@@ -83,6 +83,8 @@ def main():
     ap.add_argument("--seed", type=int, default=20261)
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--smooth", type=int, default=0, help="V(s,s) for the PCG forms; 0 = autotune")
+    ap.add_argument("--nmpo", type=int, default=0, help="coarsest order bound; 0 = autotune")
+    ap.add_argument("--nmpsd", type=float, default=0.382)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -111,33 +113,44 @@ def main():
     t_asm = min(asm)
     b = sp.rhs(acc, afe.F_POISSON)
     A.apply_dirichlet(b)
-    t = time.time()
-    g = afe.Amg(A, afe.AmgParams(lazy=1, smooth=3, build_fp32=1))
-    t_amg = time.time() - t
+    x = afe.DeviceVector(ctx, sp.n_dof)
+
+    def build(nmpo, nmpsd):
+        t = time.time()
+        gg = afe.Amg(A, afe.AmgParams(lazy=1, nmpo=nmpo, nmpsd=nmpsd, smooth=3, build_fp32=1))
+        return gg, time.time() - t
+
+    def solve_with(gg, prec, s):
+        gg.set_smooth(s)
+        x.zero()
+        t0 = time.perf_counter()
+        st = gg.pcg(x, b, rtol=args.rtol, max_it=1000, prec=prec)
+        return time.perf_counter() - t0, st
+
+    # autotune (mixed form): coarsest-level size through the reference's own
+    # AMGSolver arguments (nmpo, nmpsd) and the PCG smoothing count V(s,s)
+    choices = [(50, 0.382), (2048, 0.0)] if args.nmpo == 0 else [(args.nmpo, args.nmpsd)]
+    smooths = (1, 2) if args.smooth == 0 else (args.smooth,)
+    best = None
+    for nmpo, nmpsd in choices:
+        gg, t_b = build(nmpo, nmpsd)
+        for s in smooths:
+            solve_with(gg, afe.PREC_FP32, s)
+            dt, st = solve_with(gg, afe.PREC_FP32, s)
+            log(f"tune nmpo {nmpo} nmpsd {nmpsd} V({s},{s}): {dt*1e3:.2f} ms, {st.iterations} its "
+                f"(levels {gg.info()['n']}, setup {t_b:.2f} s)")
+            if best is None or dt < best[0]:
+                best = (dt, nmpo, nmpsd, s)
+        del gg
+    _, nmpo, nmpsd, s_mixed = best
+    g, t_amg = build(nmpo, nmpsd)
     info = g.info()
     dinfo = g.device_info()
     log(f"AMG setup {t_amg:.2f} s: levels {info['levels']} n {info['n']} "
         f"colours {[d['colours'] for d in dinfo]} tail from level {dinfo[0]['tail']}")
-    x = afe.DeviceVector(ctx, sp.n_dof)
 
     def solve(prec, s):
-        g.set_smooth(s)
-        x.zero()
-        t0 = time.perf_counter()
-        st = g.pcg(x, b, rtol=args.rtol, max_it=1000, prec=prec)
-        return time.perf_counter() - t0, st
-
-    # autotune the PCG smoothing count on the mixed form
-    s_mixed = args.smooth
-    if s_mixed == 0:
-        best = None
-        for s in (1, 2):
-            solve(afe.PREC_FP32, s)
-            dt, st = solve(afe.PREC_FP32, s)
-            log(f"tune V({s},{s}): {dt*1e3:.2f} ms, {st.iterations} its")
-            if best is None or dt < best[0]:
-                best = (dt, s)
-        s_mixed = best[1]
+        return solve_with(g, prec, s)
     for _ in range(args.warmup):
         solve(afe.PREC_FP32, s_mixed)
     lat, its = [], []
@@ -220,7 +233,8 @@ def main():
             "n_dof": sp.n_dof, "nnz": sp.nnz, "rtol": args.rtol,
             "hierarchy": "lazy (AMGSolver::lazyReinit)", "levels": info["levels"],
             "level_n": info["n"], "colours": [d["colours"] for d in dinfo],
-            "pcg_smooth": s_mixed, "parallelism": "single GPU",
+            "pcg_smooth": s_mixed, "amg_nmpo": nmpo, "amg_nmpsd": nmpsd,
+            "parallelism": "single GPU",
         },
         "mixed": {"iterations": its[-1], "rel_res": rel, "p50_ms": 1e3 * p50, "p99_ms": 1e3 * p99},
         "pcg64": {"seconds": t64, "iterations": st64.iterations, "dof_per_s": sp.n_dof / t64},
