@@ -13,7 +13,7 @@ from . import _sigs
 __all__ = [
     "P1_TRI", "P2_TRI", "P1_TET", "OP_STIFF", "OP_MASS", "OP_VARSTIFF",
     "F_POISSON", "F_VARCOEF", "F_NODAL", "F_ONE", "PREC_FP64", "PREC_FP32",
-    "mesh_square", "mesh_cube", "mesh_displace", "field_random", "Field",
+    "set_tuning", "mesh_square", "mesh_cube", "mesh_displace", "field_random", "Field",
     "Context", "Space", "Matrix", "Amg", "HostAmg", "AmgParams", "DeviceVector",
 ]
 
@@ -39,6 +39,11 @@ def _ptr(a):
     if isinstance(a, DeviceVector):
         return a.ptr
     return a.ctypes.data_as(C.c_void_p)
+
+
+def set_tuning(xcd=0, spmv_blocks=2048, tail_rows=4096):
+    """Launch tuning of the sliced-ELL kernels (process-wide)."""
+    _chk(_lib().afe_set_tuning(xcd, spmv_blocks, tail_rows))
 
 
 def mesh_square(n, jitter=0.0, seed=0):

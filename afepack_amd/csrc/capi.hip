@@ -74,6 +74,7 @@ struct afe_ctx {
   std::recursive_mutex mu;
   int64_t bytes = 0;
   double *d_quad = nullptr; // all quadrature rules
+  int64_t lds_bytes = 0;    // LDS granted to the coarse-tail workgroup
   std::map<void *, int64_t> user_bytes; // afe_dev_alloc buffers
 };
 
@@ -124,6 +125,22 @@ template <class T> struct DevBuf {
 
 static inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
+// launch tuning of the sliced-ELL family, chosen by the autotuner:
+// xcd: XCD-aware logical block order; spmv_blocks: grid of the grid-stride
+// SpMV / residual kernels (a multiple of 8)
+struct Tuning {
+  int xcd = 0;
+  int spmv_blocks = 2048;
+  int64_t tail_rows = 4096;          // levels at most this size run in the tail
+  int64_t stream_bytes = 128 << 20;  // matrices above this use streaming loads
+};
+static Tuning g_tune;
+
+static inline unsigned sblk(int64_t nslices) {
+  unsigned b = nblk(nslices, WPB);
+  return g_tune.xcd ? (b + 7) / 8 * 8 : b;
+}
+
 extern "C" int afe_ctx_create(int device, afe_ctx **out) {
   API_BEGIN
   if (!out) return fail(AFE_E_ARG, "null output");
@@ -139,6 +156,16 @@ extern "C" int afe_ctx_create(int device, afe_ctx **out) {
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&c->ev0));
   HIPCHK(hipEventCreate(&c->ev1));
+  {
+    int lds = 0;
+    HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    // leave room for the static shared memory of the kernels
+    c->lds_bytes = std::max(0, std::min(lds, 160 * 1024) - 4096);
+    HIPCHK(hipFuncSetAttribute((const void *)k_tail<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)c->lds_bytes));
+    HIPCHK(hipFuncSetAttribute((const void *)k_tail<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)c->lds_bytes));
+  }
   HIPCHK(hipMalloc((void **)&c->d_quad, sizeof(kQuadData)));
   c->bytes += sizeof(kQuadData);
   HIPCHK(hipMemcpy(c->d_quad, kQuadData, sizeof(kQuadData), hipMemcpyHostToDevice));
@@ -708,7 +735,6 @@ extern "C" int afe_error_h1(afe_space *s, const double *d_uh, int32_t acc, doubl
 // ===========================================================================
 // AMG hierarchy on the device
 // ===========================================================================
-constexpr int64_t TAIL_MAX_ROWS = 4096;
 constexpr int64_t COARSE_TAIL_MAX = 256;
 
 struct DevLevel {
@@ -723,6 +749,7 @@ struct DevLevel {
   DevBuf<double> x64, b64, t64;
   DevBuf<float> x32, b32, t32;
   std::vector<int32_t> perm; // new -> old
+  bool stream = false;       // matrix larger than the caches: non-temporal loads
 };
 
 struct afe_amg {
@@ -862,15 +889,16 @@ static void upload_hierarchy(afe_amg *g) {
       d.t32.alloc(ctx, d.n);
     }
     d.perm = perm[l];
+    // stream (non-temporal) when the level's FP32 matrix exceeds the cache
+    d.stream = (int64_t)sa.cols.size() * 8 > g_tune.stream_bytes;
   }
   g->perm0.upload(ctx, perm[0]);
   // first level of the coarse tail; a large dense coarsest level is solved by
-  // the multi-block gemv instead of the single-workgroup tail
+  // the multi-block gemv between the tail's down and up legs
   g->coarse_mode = h.full ? 0 : 1;
   g->big_coarse = h.full && g->lv[L - 1].n > COARSE_TAIL_MAX;
   g->tail0 = L - 1;
-  if (!g->big_coarse)
-    while (g->tail0 > 0 && g->lv[g->tail0 - 1].n <= TAIL_MAX_ROWS) g->tail0--;
+  while (g->tail0 > 0 && g->lv[g->tail0 - 1].n <= g_tune.tail_rows) g->tail0--;
   if (h.full) {
     // the coarse vectors are in the level's colour order: permute the inverse
     const std::vector<int32_t> &pc = perm[L - 1];
@@ -912,7 +940,7 @@ static void upload_hierarchy(afe_amg *g) {
   g->wq.alloc(ctx, n0);
   g->wz.alloc(ctx, n0);
   g->wb.alloc(ctx, n0);
-  g->part.alloc(ctx, 3 * RED_BLOCKS);
+  g->part.alloc(ctx, 3 * 8192);
   g->sc.alloc(ctx, 1);
   HIPCHK(hipStreamSynchronize(ctx->stream));
 }
@@ -1078,8 +1106,12 @@ template <class T>
 static void gs_colour(DevLevel &d, int c, hipStream_t st) {
   const int s0 = d.colour_slice[c], s1 = d.colour_slice[c + 1];
   if (s1 <= s0) return;
-  k_gs<T><<<nblk(s1 - s0, WPB), BLK, 0, st>>>(d.si.p, s0, s1, d.cols.p, Prec<T>::v(d), Prec<T>::invd(d),
-                                              Prec<T>::b(d), Prec<T>::x(d));
+  if (d.stream)
+    k_gs<T, true><<<sblk(s1 - s0), BLK, 0, st>>>(d.si.p, s0, s1, d.cols.p, Prec<T>::v(d), Prec<T>::invd(d),
+                                                 Prec<T>::b(d), Prec<T>::x(d), g_tune.xcd);
+  else
+    k_gs<T, false><<<sblk(s1 - s0), BLK, 0, st>>>(d.si.p, s0, s1, d.cols.p, Prec<T>::v(d), Prec<T>::invd(d),
+                                                  Prec<T>::b(d), Prec<T>::x(d), g_tune.xcd);
 }
 
 // one V(nu, nu) cycle on level-0 vectors (b, x) of precision T; the
@@ -1092,23 +1124,42 @@ static void issue_vcycle(afe_amg *g, int nu, bool symmetric, hipStream_t st) {
     DevLevel &dn = g->lv[l + 1];
     for (int it = 0; it < nu; it++)
       for (int c = 0; c < d.ncol; c++) gs_colour<T>(d, c, st);
-    k_residual<T><<<nblk(d.ns, WPB), BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, Prec<T>::v(d), Prec<T>::diag(d),
-                                                    Prec<T>::b(d), Prec<T>::x(d), Prec<T>::t(d));
-    k_restrict<T><<<nblk(d.p_ns, WPB), BLK, 0, st>>>(d.p_si.p, d.p_ns, d.p_cols.p, Prec<T>::pv(d),
-                                                      Prec<T>::t(d), Prec<T>::b(dn), Prec<T>::x(dn));
+    if (d.stream) {
+      k_residual<T, true><<<sblk(d.ns), BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, Prec<T>::v(d), Prec<T>::diag(d),
+                                                       Prec<T>::b(d), Prec<T>::x(d), Prec<T>::t(d), g_tune.xcd);
+      k_restrict<T, true><<<sblk(d.p_ns), BLK, 0, st>>>(d.p_si.p, d.p_ns, d.p_cols.p, Prec<T>::pv(d),
+                                                         Prec<T>::t(d), Prec<T>::b(dn), Prec<T>::x(dn), g_tune.xcd);
+    } else {
+      k_residual<T, false><<<sblk(d.ns), BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, Prec<T>::v(d), Prec<T>::diag(d),
+                                                        Prec<T>::b(d), Prec<T>::x(d), Prec<T>::t(d), g_tune.xcd);
+      k_restrict<T, false><<<sblk(d.p_ns), BLK, 0, st>>>(d.p_si.p, d.p_ns, d.p_cols.p, Prec<T>::pv(d),
+                                                          Prec<T>::t(d), Prec<T>::b(dn), Prec<T>::x(dn), g_tune.xcd);
+    }
   }
+  const int lds_rows = (int)(g->ctx->lds_bytes / (3 * sizeof(T)));
+  const size_t lds = (size_t)lds_rows * 3 * sizeof(T);
   if (g->big_coarse) {
     DevLevel &dc = g->lv[Lc];
+    if (g->tail0 < Lc)
+      k_tail<T><<<1, TAIL_THREADS, lds, st>>>(Prec<T>::ld(g), g->tail0, Lc, g->minv.p, g->coarse_mode, nu, nu,
+                                               symmetric ? 1 : 0, 1, lds_rows);
     k_coarse_gemv<T><<<nblk(dc.n, WPB), BLK, 0, st>>>((int)dc.n, g->minv.p, Prec<T>::b(dc), Prec<T>::x(dc));
+    if (g->tail0 < Lc)
+      k_tail<T><<<1, TAIL_THREADS, lds, st>>>(Prec<T>::ld(g), g->tail0, Lc, g->minv.p, g->coarse_mode, nu, nu,
+                                               symmetric ? 1 : 0, 4, lds_rows);
   } else {
-    k_tail<T><<<1, TAIL_THREADS, 0, st>>>(Prec<T>::ld(g), g->tail0, Lc, g->minv.p, g->coarse_mode, nu, nu,
-                                           symmetric ? 1 : 0);
+    k_tail<T><<<1, TAIL_THREADS, lds, st>>>(Prec<T>::ld(g), g->tail0, Lc, g->minv.p, g->coarse_mode, nu, nu,
+                                             symmetric ? 1 : 0, 7, lds_rows);
   }
   for (int l = g->tail0 - 1; l >= 0; l--) {
     DevLevel &d = g->lv[l];
     DevLevel &dn = g->lv[l + 1];
-    k_prolong<T><<<nblk(d.t_ns, WPB), BLK, 0, st>>>(d.t_si.p, d.t_ns, d.t_cols.p, Prec<T>::tv(d),
-                                                     Prec<T>::x(dn), Prec<T>::x(d));
+    if (d.stream)
+      k_prolong<T, true><<<sblk(d.t_ns), BLK, 0, st>>>(d.t_si.p, d.t_ns, d.t_cols.p, Prec<T>::tv(d),
+                                                        Prec<T>::x(dn), Prec<T>::x(d), g_tune.xcd);
+    else
+      k_prolong<T, false><<<sblk(d.t_ns), BLK, 0, st>>>(d.t_si.p, d.t_ns, d.t_cols.p, Prec<T>::tv(d),
+                                                         Prec<T>::x(dn), Prec<T>::x(d), g_tune.xcd);
     for (int it = 0; it < nu; it++)
       for (int cc = 0; cc < d.ncol; cc++) gs_colour<T>(d, symmetric ? d.ncol - 1 - cc : cc, st);
   }
@@ -1160,9 +1211,10 @@ static void residual_norms(afe_amg *g, const double *b, const double *x, double 
                            const double *xl = nullptr) {
   DevLevel &d = g->lv[0];
   hipStream_t st = g->ctx->stream;
-  k_residual_norms_dd<<<RED_BLOCKS, BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, d.v64.p, d.diag64.p, b, x, xl, r,
-                                                   g->part.p);
-  k_finalize<<<1, 256, 0, st>>>(g->part.p, RED_BLOCKS, 3, &g->sc.p->rr);
+  const int G = g_tune.spmv_blocks;
+  k_residual_norms_dd<<<G, BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, d.v64.p, d.diag64.p, b, x, xl, r, g->part.p,
+                                          g_tune.xcd);
+  k_finalize<<<1, 256, 0, st>>>(g->part.p, G, 3, &g->sc.p->rr);
 }
 
 static void read_scalars(afe_amg *g) {
@@ -1250,9 +1302,9 @@ template <class T> static void issue_cg_iteration(afe_amg *g) {
   DevLevel &d0 = g->lv[0];
   hipStream_t s = g->ctx->stream;
   const int64_t n = d0.n;
-  k_spmv_dot<<<RED_BLOCKS, BLK, 0, s>>>(d0.si.p, d0.ns, d0.cols.p, d0.v64.p, d0.diag64.p, g->wp.p, g->wq.p,
-                                         g->part.p);
-  k_finalize<<<1, 256, 0, s>>>(g->part.p, RED_BLOCKS, 1, &g->sc.p->pq);
+  k_spmv_dot<<<g_tune.spmv_blocks, BLK, 0, s>>>(d0.si.p, d0.ns, d0.cols.p, d0.v64.p, d0.diag64.p, g->wp.p,
+                                                 g->wq.p, g->part.p, g_tune.xcd);
+  k_finalize<<<1, 256, 0, s>>>(g->part.p, g_tune.spmv_blocks, 1, &g->sc.p->pq);
   k_cg_update<T><<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wx.p, g->wxl.p, g->wr.p, g->wp.p, g->wq.p,
                                              Prec<T>::b(d0), Prec<T>::x(d0), g->part.p);
   k_finalize<<<1, 256, 0, s>>>(g->part.p, RED_BLOCKS, 1, &g->sc.p->rr);
@@ -1434,15 +1486,15 @@ extern "C" int afe_amg_kernel_timing(afe_amg *g, int32_t which, int32_t reps, do
   }
   auto once = [&]() {
     if (which == 0) {
-      k_spmv_dot<<<RED_BLOCKS, BLK, 0, s>>>(d.si.p, d.ns, d.cols.p, d.v64.p, d.diag64.p, g->wp.p, g->wq.p,
-                                             g->part.p);
+      k_spmv_dot<<<g_tune.spmv_blocks, BLK, 0, s>>>(d.si.p, d.ns, d.cols.p, d.v64.p, d.diag64.p, g->wp.p,
+                                                     g->wq.p, g->part.p, g_tune.xcd);
     } else if (which == 1) {
       for (int c = 0; c < d.ncol; c++) gs_colour<float>(d, c, s);
     } else if (which == 2) {
       for (int c = 0; c < d.ncol; c++) gs_colour<double>(d, c, s);
     } else {
-      k_residual<float><<<nblk(d.ns, WPB), BLK, 0, s>>>(d.si.p, d.ns, d.cols.p, d.v32.p, d.diag32.p, d.b32.p,
-                                                         d.x32.p, d.t32.p);
+      k_residual<float, true><<<sblk(d.ns), BLK, 0, s>>>(d.si.p, d.ns, d.cols.p, d.v32.p, d.diag32.p, d.b32.p,
+                                                          d.x32.p, d.t32.p, g_tune.xcd);
     }
   };
   once();
@@ -1472,5 +1524,16 @@ extern "C" int afe_amg_device_info(const afe_amg *g, int32_t level, int64_t *row
   if (colours) *colours = d.ncol;
   if (stored) *stored = (int64_t)d.cols.n;
   if (tail_level) *tail_level = g->tail0;
+  return AFE_OK;
+}
+
+// launch tuning (process-wide): xcd = XCD-aware block order (0/1),
+// spmv_blocks = grid of the grid-stride level-0 kernels (8..8192, multiple of 8)
+extern "C" int afe_set_tuning(int32_t xcd, int32_t spmv_blocks, int64_t tail_rows) {
+  if ((xcd != 0 && xcd != 1) || spmv_blocks < 8 || spmv_blocks > 8192 || spmv_blocks % 8 || tail_rows < 0)
+    return fail(AFE_E_ARG, "bad tuning values");
+  g_tune.xcd = xcd;
+  g_tune.spmv_blocks = spmv_blocks;
+  g_tune.tail_rows = tail_rows;
   return AFE_OK;
 }

@@ -495,7 +495,20 @@ __global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ par
 // ---------------------------------------------------------------------------
 // sliced-ELL kernels (one wavefront per slice, one lane per row)
 // ---------------------------------------------------------------------------
-template <class T>
+// streamed matrix data: non-temporal loads keep the gathered vector in cache
+template <class U> __device__ inline U ld_stream(const U *p) { return __builtin_nontemporal_load(p); }
+
+template <bool NT, class U> __device__ inline U ld_mat(const U *p) {
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+
+// one row of a sliced-ELL matrix times x; NT selects non-temporal loads for
+// the matrix stream (the level-0 matrices, larger than the caches), plain
+// loads for the small levels whose matrices stay cached between cycles
+template <class T, bool NT = false>
 __device__ inline T sell_dot(const SliceInfo &s, int lane, const int32_t *__restrict__ cols,
                              const T *__restrict__ vals, const T *__restrict__ x) {
   const int32_t *c = cols + s.off + lane;
@@ -503,82 +516,91 @@ __device__ inline T sell_dot(const SliceInfo &s, int lane, const int32_t *__rest
   T acc = 0;
   int j = 0;
   for (; j + 4 <= s.width; j += 4) {
-    const int32_t c0 = c[(j + 0) * WAVE], c1 = c[(j + 1) * WAVE];
-    const int32_t c2 = c[(j + 2) * WAVE], c3 = c[(j + 3) * WAVE];
-    const T v0 = v[(j + 0) * WAVE], v1 = v[(j + 1) * WAVE];
-    const T v2 = v[(j + 2) * WAVE], v3 = v[(j + 3) * WAVE];
+    const int32_t c0 = ld_mat<NT>(c + (j + 0) * WAVE), c1 = ld_mat<NT>(c + (j + 1) * WAVE);
+    const int32_t c2 = ld_mat<NT>(c + (j + 2) * WAVE), c3 = ld_mat<NT>(c + (j + 3) * WAVE);
+    const T v0 = ld_mat<NT>(v + (j + 0) * WAVE), v1 = ld_mat<NT>(v + (j + 1) * WAVE);
+    const T v2 = ld_mat<NT>(v + (j + 2) * WAVE), v3 = ld_mat<NT>(v + (j + 3) * WAVE);
     acc += v0 * x[c0];
     acc += v1 * x[c1];
     acc += v2 * x[c2];
     acc += v3 * x[c3];
   }
-  for (; j < s.width; j++) acc += v[j * WAVE] * x[c[j * WAVE]];
+  for (; j < s.width; j++) acc += ld_mat<NT>(v + j * WAVE) * x[ld_mat<NT>(c + j * WAVE)];
   return acc;
 }
 
 __device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs; with xcd != 0 the
+// logical block index makes consecutive logical blocks share an XCD, so each
+// XCD's L2 serves a contiguous (spatially local) range of slices.  gridDim.x
+// is a multiple of 8 when xcd != 0.
+__device__ inline int logical_block(int xcd) {
+  const int b = blockIdx.x;
+  return xcd ? (b & 7) * (int)(gridDim.x >> 3) + (b >> 3) : b;
+}
+
 // Gauss-Seidel over the slices [s0, s1) of one colour:
 // x_r = (b_r - sum_{j != r} a_rj x_j) / a_rr
-template <class T>
+template <class T, bool NT>
 __global__ __launch_bounds__(BLK) void k_gs(const SliceInfo *__restrict__ si, int s0, int s1,
                                             const int32_t *__restrict__ cols, const T *__restrict__ vals,
                                             const T *__restrict__ invd, const T *__restrict__ b,
-                                            T *__restrict__ x) {
-  const int s = s0 + blockIdx.x * WPB + wave_id();
+                                            T *__restrict__ x, int xcd) {
+  const int s = s0 + logical_block(xcd) * WPB + wave_id();
   if (s >= s1) return;
   const SliceInfo q = si[s];
   const int lane = threadIdx.x % WAVE;
   if (lane >= q.nrow) return;
   const int r = q.row0 + lane;
-  const T acc = sell_dot<T>(q, lane, cols, vals, x);
+  const T acc = sell_dot<T, NT>(q, lane, cols, vals, x);
   x[r] = (b[r] - acc) * invd[r];
 }
 
 // t = b - A x (A = diag + stored off-diagonals)
-template <class T>
+template <class T, bool NT>
 __global__ __launch_bounds__(BLK) void k_residual(const SliceInfo *__restrict__ si, int ns,
                                                   const int32_t *__restrict__ cols, const T *__restrict__ vals,
                                                   const T *__restrict__ diag, const T *__restrict__ b,
-                                                  const T *__restrict__ x, T *__restrict__ t) {
-  const int s = blockIdx.x * WPB + wave_id();
+                                                  const T *__restrict__ x, T *__restrict__ t, int xcd) {
+  const int s = logical_block(xcd) * WPB + wave_id();
   if (s >= ns) return;
   const SliceInfo q = si[s];
   const int lane = threadIdx.x % WAVE;
   if (lane >= q.nrow) return;
   const int r = q.row0 + lane;
-  const T acc = sell_dot<T>(q, lane, cols, vals, x);
+  const T acc = sell_dot<T, NT>(q, lane, cols, vals, x);
   t[r] = b[r] - diag[r] * x[r] - acc;
 }
 
 // restriction: bc = P t, and the coarse guess xc = 0
-template <class T>
+template <class T, bool NT>
 __global__ __launch_bounds__(BLK) void k_restrict(const SliceInfo *__restrict__ si, int ns,
                                                   const int32_t *__restrict__ cols, const T *__restrict__ vals,
                                                   const T *__restrict__ t, T *__restrict__ bc,
-                                                  T *__restrict__ xc) {
-  const int s = blockIdx.x * WPB + wave_id();
+                                                  T *__restrict__ xc, int xcd) {
+  const int s = logical_block(xcd) * WPB + wave_id();
   if (s >= ns) return;
   const SliceInfo q = si[s];
   const int lane = threadIdx.x % WAVE;
   if (lane >= q.nrow) return;
   const int r = q.row0 + lane;
-  bc[r] = sell_dot<T>(q, lane, cols, vals, t);
+  bc[r] = sell_dot<T, NT>(q, lane, cols, vals, t);
   xc[r] = 0;
 }
 
 // prolongation: x += P^T xc (explicit transpose, no atomics)
-template <class T>
+template <class T, bool NT>
 __global__ __launch_bounds__(BLK) void k_prolong(const SliceInfo *__restrict__ si, int ns,
                                                  const int32_t *__restrict__ cols, const T *__restrict__ vals,
-                                                 const T *__restrict__ xc, T *__restrict__ x) {
-  const int s = blockIdx.x * WPB + wave_id();
+                                                 const T *__restrict__ xc, T *__restrict__ x, int xcd) {
+  const int s = logical_block(xcd) * WPB + wave_id();
   if (s >= ns) return;
   const SliceInfo q = si[s];
   const int lane = threadIdx.x % WAVE;
   if (lane >= q.nrow) return;
   const int r = q.row0 + lane;
-  x[r] += sell_dot<T>(q, lane, cols, vals, xc);
+  x[r] += sell_dot<T, NT>(q, lane, cols, vals, xc);
 }
 
 // ---------------------------------------------------------------------------
@@ -600,87 +622,131 @@ template <class T> struct LevelDev {
 };
 
 template <class T>
-__device__ inline void tail_sweep(const LevelDev<T> &d, int c, int w, int nw, int lane) {
+__device__ inline void tail_sweep(const LevelDev<T> &d, const T *b, T *x, int c, int w, int nw, int lane) {
   for (int s = d.col_slice[c] + w; s < d.col_slice[c + 1]; s += nw) {
     const SliceInfo q = d.si[s];
     if (lane < q.nrow) {
       const int r = q.row0 + lane;
-      const T acc = sell_dot<T>(q, lane, d.cols, d.vals, d.x);
-      d.x[r] = (d.b[r] - acc) * d.invd[r];
+      const T acc = sell_dot<T>(q, lane, d.cols, d.vals, x);
+      x[r] = (b[r] - acc) * d.invd[r];
     }
   }
 }
 
-// coarse_mode: 0 dense inverse (FP64), 1 diagonal scaling, 2 GS sweeps
+// Single-workgroup V-cycle over the small levels l0..lc.  phase bits:
+// 1 down leg (pre-smoothing, residual, restriction for l0..lc-1),
+// 2 coarse solve on level lc (dense FP64 inverse, diagonal or GS),
+// 4 up leg (prolongation, post-smoothing for lc-1..l0).
+// A level with n <= lds_rows keeps x, b and the residual in LDS while the
+// workgroup works on it (each colour step then waits on LDS, not on L2/HBM);
+// larger levels use the global vectors.  x of a level is written back to
+// global memory at the end of each leg.
 template <class T>
 __global__ __launch_bounds__(TAIL_THREADS) void k_tail(const LevelDev<T> *__restrict__ L, int l0, int lc,
                                                        const double *__restrict__ minv, int coarse_mode,
-                                                       int nu_pre, int nu_post, int post_reverse) {
+                                                       int nu_pre, int nu_post, int post_reverse, int phase,
+                                                       int lds_rows) {
+  extern __shared__ char tail_smem[];
+  T *sx = reinterpret_cast<T *>(tail_smem);
+  T *sb = sx + lds_rows;
+  T *st = sb + lds_rows;
   const int nw = blockDim.x / WAVE;
   const int w = wave_id();
   const int lane = threadIdx.x % WAVE;
-  for (int l = l0; l < lc; l++) {
-    const LevelDev<T> d = L[l];
-    for (int it = 0; it < nu_pre; it++)
-      for (int c = 0; c < d.ncol; c++) {
-        tail_sweep(d, c, w, nw, lane);
+  const int tid = threadIdx.x;
+  if (phase & 1) {
+    for (int l = l0; l < lc; l++) {
+      const LevelDev<T> d = L[l];
+      const bool lds = d.n <= lds_rows;
+      T *x = lds ? sx : d.x;
+      T *b = lds ? sb : d.b;
+      T *t = lds ? st : d.t;
+      if (lds) {
+        for (int i = tid; i < d.n; i += blockDim.x) {
+          sx[i] = d.x[i];
+          sb[i] = d.b[i];
+        }
         __syncthreads();
       }
-    for (int s = w; s < d.ns; s += nw) {
-      const SliceInfo q = d.si[s];
-      if (lane < q.nrow) {
-        const int r = q.row0 + lane;
-        d.t[r] = d.b[r] - d.diag[r] * d.x[r] - sell_dot<T>(q, lane, d.cols, d.vals, d.x);
+      for (int it = 0; it < nu_pre; it++)
+        for (int c = 0; c < d.ncol; c++) {
+          tail_sweep(d, b, x, c, w, nw, lane);
+          __syncthreads();
+        }
+      for (int s = w; s < d.ns; s += nw) {
+        const SliceInfo q = d.si[s];
+        if (lane < q.nrow) {
+          const int r = q.row0 + lane;
+          t[r] = b[r] - d.diag[r] * x[r] - sell_dot<T>(q, lane, d.cols, d.vals, x);
+        }
       }
-    }
-    __syncthreads();
-    const LevelDev<T> dn = L[l + 1];
-    for (int s = w; s < d.p_ns; s += nw) {
-      const SliceInfo q = d.p_si[s];
-      if (lane < q.nrow) {
-        const int r = q.row0 + lane;
-        dn.b[r] = sell_dot<T>(q, lane, d.p_cols, d.p_vals, d.t);
-        dn.x[r] = 0;
+      __syncthreads();
+      const LevelDev<T> dn = L[l + 1];
+      for (int s = w; s < d.p_ns; s += nw) {
+        const SliceInfo q = d.p_si[s];
+        if (lane < q.nrow) {
+          const int r = q.row0 + lane;
+          dn.b[r] = sell_dot<T>(q, lane, d.p_cols, d.p_vals, t);
+          dn.x[r] = 0;
+        }
       }
+      if (lds)
+        for (int i = tid; i < d.n; i += blockDim.x) d.x[i] = sx[i];
+      __syncthreads();
     }
-    __syncthreads();
   }
-  {
+  if (phase & 2) {
     const LevelDev<T> d = L[lc];
     if (coarse_mode == 0) {
-      for (int i = threadIdx.x; i < d.n; i += blockDim.x) {
+      for (int i = tid; i < d.n; i += blockDim.x) {
         double s = 0.0;
         for (int j = 0; j < d.n; j++) s += minv[(int64_t)i * d.n + j] * (double)d.b[j];
         d.x[i] = (T)s;
       }
     } else if (coarse_mode == 1) {
-      for (int i = threadIdx.x; i < d.n; i += blockDim.x) d.x[i] = d.b[i] * d.invd[i];
+      for (int i = tid; i < d.n; i += blockDim.x) d.x[i] = d.b[i] * d.invd[i];
     } else {
       for (int it = 0; it < nu_pre; it++)
         for (int c = 0; c < d.ncol; c++) {
-          tail_sweep(d, c, w, nw, lane);
+          tail_sweep(d, d.b, d.x, c, w, nw, lane);
           __syncthreads();
         }
     }
     __syncthreads();
   }
-  for (int l = lc - 1; l >= l0; l--) {
-    const LevelDev<T> d = L[l];
-    const LevelDev<T> dn = L[l + 1];
-    for (int s = w; s < d.t_ns; s += nw) {
-      const SliceInfo q = d.t_si[s];
-      if (lane < q.nrow) {
-        const int r = q.row0 + lane;
-        d.x[r] += sell_dot<T>(q, lane, d.t_cols, d.t_vals, dn.x);
-      }
-    }
-    __syncthreads();
-    for (int it = 0; it < nu_post; it++)
-      for (int cc = 0; cc < d.ncol; cc++) {
-        const int c = post_reverse ? d.ncol - 1 - cc : cc;
-        tail_sweep(d, c, w, nw, lane);
+  if (phase & 4) {
+    for (int l = lc - 1; l >= l0; l--) {
+      const LevelDev<T> d = L[l];
+      const LevelDev<T> dn = L[l + 1];
+      const bool lds = d.n <= lds_rows;
+      T *x = lds ? sx : d.x;
+      const T *b = lds ? sb : d.b;
+      if (lds) {
+        for (int i = tid; i < d.n; i += blockDim.x) {
+          sx[i] = d.x[i];
+          sb[i] = d.b[i];
+        }
         __syncthreads();
       }
+      for (int s = w; s < d.t_ns; s += nw) {
+        const SliceInfo q = d.t_si[s];
+        if (lane < q.nrow) {
+          const int r = q.row0 + lane;
+          x[r] += sell_dot<T>(q, lane, d.t_cols, d.t_vals, dn.x);
+        }
+      }
+      __syncthreads();
+      for (int it = 0; it < nu_post; it++)
+        for (int cc = 0; cc < d.ncol; cc++) {
+          const int c = post_reverse ? d.ncol - 1 - cc : cc;
+          tail_sweep(d, b, x, c, w, nw, lane);
+          __syncthreads();
+        }
+      if (lds) {
+        for (int i = tid; i < d.n; i += blockDim.x) d.x[i] = sx[i];
+        __syncthreads();
+      }
+    }
   }
 }
 
@@ -740,15 +806,18 @@ __global__ __launch_bounds__(BLK) void k_spmv_dot(const SliceInfo *__restrict__ 
                                                   const double *__restrict__ vals,
                                                   const double *__restrict__ diag,
                                                   const double *__restrict__ x, double *__restrict__ y,
-                                                  double *__restrict__ part) {
+                                                  double *__restrict__ part, int xcd) {
   __shared__ double sh[BLK];
   const int lane = threadIdx.x % WAVE;
   double dot = 0.0;
-  for (int s = blockIdx.x * WPB + wave_id(); s < ns; s += gridDim.x * WPB) {
+  const int spb = (ns + gridDim.x - 1) / gridDim.x;
+  const int lb = logical_block(xcd);
+  const int s_end = min(ns, (lb + 1) * spb);
+  for (int s = lb * spb + wave_id(); s < s_end; s += WPB) {
     const SliceInfo q = si[s];
     if (lane < q.nrow) {
       const int r = q.row0 + lane;
-      const double yr = diag[r] * x[r] + sell_dot<double>(q, lane, cols, vals, x);
+      const double yr = diag[r] * x[r] + sell_dot<double, true>(q, lane, cols, vals, x);
       y[r] = yr;
       dot += x[r] * yr;
     }
@@ -807,11 +876,15 @@ __global__ __launch_bounds__(BLK) void k_residual_norms_dd(const SliceInfo *__re
                                                            const double *__restrict__ b,
                                                            const double *__restrict__ x,
                                                            const double *__restrict__ xl,
-                                                           double *__restrict__ r, double *__restrict__ part) {
+                                                           double *__restrict__ r, double *__restrict__ part,
+                                                           int xcd) {
   __shared__ double sh[BLK];
   const int lane = threadIdx.x % WAVE;
   double rr = 0.0, bb = 0.0, l1 = 0.0;
-  for (int s = blockIdx.x * WPB + wave_id(); s < ns; s += gridDim.x * WPB) {
+  const int spb = (ns + gridDim.x - 1) / gridDim.x;
+  const int lb = logical_block(xcd);
+  const int s_end = min(ns, (lb + 1) * spb);
+  for (int s = lb * spb + wave_id(); s < s_end; s += WPB) {
     const SliceInfo q = si[s];
     if (lane < q.nrow) {
       const int row = q.row0 + lane;

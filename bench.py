@@ -127,22 +127,44 @@ def main():
         st = gg.pcg(x, b, rtol=args.rtol, max_it=1000, prec=prec)
         return time.perf_counter() - t0, st
 
-    # autotune (mixed form): coarsest-level size through the reference's own
+    # autotune 1: launch shape of the sliced-ELL kernels, by live HIP-event
+    # timing of the level-0 FP64 SpMV and FP32 Gauss-Seidel sweep
+    gg, _ = build(50, 0.382)
+    tune_log = []
+    best_t = None
+    for xcd in (0, 1):
+        for blocks in (1024, 2048, 4096):
+            afe.set_tuning(xcd, blocks, 4096)
+            t_sp, _ = gg.kernel_timing(0, reps=10)
+            t_gs, _ = gg.kernel_timing(1, reps=10)
+            tune_log.append({"xcd": xcd, "spmv_blocks": blocks, "spmv_us": 1e3 * t_sp, "gs_us": 1e3 * t_gs})
+            log(f"tune xcd {xcd} blocks {blocks}: spmv {1e3*t_sp:.1f} us, gs sweep {1e3*t_gs:.1f} us")
+            score = t_sp + 2 * t_gs
+            if best_t is None or score < best_t[0]:
+                best_t = (score, xcd, blocks)
+    afe.set_tuning(best_t[1], best_t[2], 4096)
+    del gg
+
+    # autotune 2 (mixed form): coarsest-level size through the reference's own
     # AMGSolver arguments (nmpo, nmpsd) and the PCG smoothing count V(s,s)
     choices = [(50, 0.382), (2048, 0.0)] if args.nmpo == 0 else [(args.nmpo, args.nmpsd)]
     smooths = (1, 2) if args.smooth == 0 else (args.smooth,)
+    tails = (4096, 12288)
     best = None
     for nmpo, nmpsd in choices:
-        gg, t_b = build(nmpo, nmpsd)
-        for s in smooths:
-            solve_with(gg, afe.PREC_FP32, s)
-            dt, st = solve_with(gg, afe.PREC_FP32, s)
-            log(f"tune nmpo {nmpo} nmpsd {nmpsd} V({s},{s}): {dt*1e3:.2f} ms, {st.iterations} its "
-                f"(levels {gg.info()['n']}, setup {t_b:.2f} s)")
-            if best is None or dt < best[0]:
-                best = (dt, nmpo, nmpsd, s)
-        del gg
-    _, nmpo, nmpsd, s_mixed = best
+        for tail in tails:
+            afe.set_tuning(best_t[1], best_t[2], tail)
+            gg, t_b = build(nmpo, nmpsd)
+            for s in smooths:
+                solve_with(gg, afe.PREC_FP32, s)
+                dt, st = solve_with(gg, afe.PREC_FP32, s)
+                log(f"tune nmpo {nmpo} nmpsd {nmpsd} tail {tail} V({s},{s}): {dt*1e3:.2f} ms, "
+                    f"{st.iterations} its (levels {gg.info()['n']}, setup {t_b:.2f} s)")
+                if best is None or dt < best[0]:
+                    best = (dt, nmpo, nmpsd, s, tail)
+            del gg
+    _, nmpo, nmpsd, s_mixed, tail_rows = best
+    afe.set_tuning(best_t[1], best_t[2], tail_rows)
     g, t_amg = build(nmpo, nmpsd)
     info = g.info()
     dinfo = g.device_info()
@@ -234,6 +256,7 @@ def main():
             "hierarchy": "lazy (AMGSolver::lazyReinit)", "levels": info["levels"],
             "level_n": info["n"], "colours": [d["colours"] for d in dinfo],
             "pcg_smooth": s_mixed, "amg_nmpo": nmpo, "amg_nmpsd": nmpsd,
+            "launch": {"xcd_order": best_t[1], "spmv_blocks": best_t[2], "tail_rows": tail_rows},
             "parallelism": "single GPU",
         },
         "mixed": {"iterations": its[-1], "rel_res": rel, "p50_ms": 1e3 * p50, "p99_ms": 1e3 * p99},

@@ -246,6 +246,13 @@ int afe_amg_set_smooth(afe_amg *g, int32_t smooth);
  * 2 FP64 sweep, 3 FP32 residual.  *bytes = algorithmic bytes per launch. */
 int afe_amg_kernel_timing(afe_amg *g, int32_t which, int32_t reps,
                           double *avg_ms, double *bytes);
+/* launch tuning of the sliced-ELL kernels (process-wide, set by the
+ * autotuner): xcd = 1 orders workgroups so that each XCD works on a
+ * contiguous slice range; spmv_blocks = grid of the grid-stride level-0
+ * kernels (multiple of 8, at most 8192); tail_rows = levels of at most this
+ * many rows run in the single-workgroup coarse tail.  Applies to
+ * hierarchies built afterwards. */
+int afe_set_tuning(int32_t xcd, int32_t spmv_blocks, int64_t tail_rows);
 /* device layout of a level: rows, colours, stored sliced-ELL entries and the
  * first level handled by the single-workgroup coarse tail */
 int afe_amg_device_info(const afe_amg *g, int32_t level, int64_t *rows,

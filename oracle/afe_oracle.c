@@ -309,6 +309,17 @@ static void basis_grad(int kind, const double *p, const double *v, double *g) {
 }
 
 static int kind_tdim(int kind) { return kind == OR_P1_TET ? 3 : 2; }
+
+void or_basis_eval(int kind, const double *p, const double *verts, double *val,
+                   double *grad) {
+  if (val) basis_value(kind, p, verts, val);
+  if (grad) basis_grad(kind, p, verts, grad);
+}
+void or_local_to_global(int tdim, const double *lp, const double *verts, double *gp) {
+  local_to_global(tdim, lp, verts, gp);
+}
+double or_l2g_jacobian(int tdim, const double *verts) { return l2g_jacobian(tdim, verts); }
+double or_template_volume(int tdim) { return tmpl_volume(tdim); }
 static int kind_nloc(int kind) { return kind == OR_P1_TRI ? 3 : kind == OR_P2_TRI ? 6 : 4; }
 static int kind_nvtx(int kind) { return kind == OR_P1_TET ? 4 : 3; }
 
@@ -994,7 +1005,7 @@ static or_mat *get_pmpt(const or_mat *P, const or_mat *M, const or_mat *Pt) {
  *     strong F neighbour l over the common C points (:174-219).
  *  5. A_c = getPMPt(P, A, P^T). */
 static void amg_project(const or_mat *M, double alpha, or_mat **Pout,
-                        or_mat **PMPt) {
+                        or_mat **PMPt, or_mat **Ptout) {
   int64_t n = M->sp->m;
   const int64_t *rp = M->sp->rp, *col = M->sp->col;
   const double *v = M->val;
@@ -1118,7 +1129,7 @@ static void amg_project(const or_mat *M, double alpha, or_mat **Pout,
   }
   *Pout = P;
   *PMPt = get_pmpt(P, M, Pt);
-  or_mat_free(Pt);
+  *Ptout = Pt; /* kept for inspection; the reference discards it here */
   free(dep);
   free(flag);
   free(beta);
@@ -1258,7 +1269,7 @@ or_amg *or_amg_new(const or_mat *M, int lazy, double alpha, unsigned nmpo,
     if (lazy)
       amg_lazy_project(g->A[g->n_project], &P, &PMPt, &Pt);
     else
-      amg_project(g->A[g->n_project], alpha, &P, &PMPt);
+      amg_project(g->A[g->n_project], alpha, &P, &PMPt, &Pt);
     m = (uint32_t)PMPt->sp->m;
     if (k == m || g->n_project >= OR_MAX_LEVELS) {
       g->full = 0;

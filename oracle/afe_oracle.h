@@ -66,6 +66,14 @@ void or_space_dofs(const or_space *s, int32_t *elem_dof, int32_t *dof_mark,
 int or_quad_n(int tdim, int acc);
 void or_quad_get(int tdim, int acc, double *pts, double *w);
 
+/* ---- template-element plugins (for pinning against the reference's .so) -- */
+void or_basis_eval(int kind, const double *p, const double *verts, double *val,
+                   double *grad);
+void or_local_to_global(int tdim, const double *lp, const double *verts,
+                        double *gp);
+double or_l2g_jacobian(int tdim, const double *verts);
+double or_template_volume(int tdim);
+
 /* ---- element matrices and assembly -------------------------------------- */
 void or_element_matrix(int kind, int op, int acc, const double *vert_coords,
                        const or_field *coef, double *K);

@@ -33,6 +33,10 @@ _SIGS = {
     "or_space_n_dof": (I64, [P]),
     "or_space_n_loc": (C.c_int, [P]),
     "or_space_dofs": (None, [P, P, P, P]),
+    "or_basis_eval": (None, [C.c_int, P, P, P, P]),
+    "or_local_to_global": (None, [C.c_int, P, P, P]),
+    "or_l2g_jacobian": (F64, [C.c_int, P]),
+    "or_template_volume": (F64, [C.c_int]),
     "or_quad_n": (C.c_int, [C.c_int, C.c_int]),
     "or_quad_get": (None, [C.c_int, C.c_int, P, P]),
     "or_element_matrix": (None, [C.c_int, C.c_int, C.c_int, P, C.POINTER(OrField), P]),
@@ -213,6 +217,33 @@ def element_matrix(kind, op, acc, vert_coords, field=None):
     K = np.empty(nl * nl)
     lib().or_element_matrix(kind, op, acc, _p(v), _fp(field), _p(K))
     return K.reshape(nl, nl)
+
+
+def basis_eval(kind, p, verts):
+    nl = {P1_TRI: 3, P2_TRI: 6, P1_TET: 4}[kind]
+    dim = 3 if kind == P1_TET else 2
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    v = np.ascontiguousarray(verts, dtype=np.float64).ravel()
+    val, grad = np.empty(nl), np.empty(nl * dim)
+    lib().or_basis_eval(kind, _p(p), _p(v), _p(val), _p(grad))
+    return val, grad.reshape(nl, dim)
+
+
+def local_to_global(tdim, lp, verts):
+    lp = np.ascontiguousarray(lp, dtype=np.float64)
+    v = np.ascontiguousarray(verts, dtype=np.float64).ravel()
+    gp = np.empty(tdim)
+    lib().or_local_to_global(tdim, _p(lp), _p(v), _p(gp))
+    return gp
+
+
+def l2g_jacobian(tdim, verts):
+    v = np.ascontiguousarray(verts, dtype=np.float64).ravel()
+    return lib().or_l2g_jacobian(tdim, _p(v))
+
+
+def template_volume(tdim):
+    return lib().or_template_volume(tdim)
 
 
 def quad(tdim, acc):

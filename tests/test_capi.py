@@ -1,0 +1,52 @@
+"""CPU: the C-ABI library loads, exports every entry point the public header
+declares, and reports errors without a device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import afepack_amd as afe
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(afe_[a-z0-9_]+)\s*\(", txt))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(afe.LIB_PATH)
+    names = _declared()
+    assert len(names) > 40
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_signatures_cover_the_header():
+    from afepack_amd import _sigs
+    assert _declared() <= set(_sigs.SIGS), sorted(_declared() - set(_sigs.SIGS))
+
+
+def test_errors_without_device():
+    lib = afe.load()
+    with pytest.raises(afe.AfeError) as e:
+        afe.mesh_square(0)
+    assert e.value.code == -1
+    assert b"bad mesh" in lib.afe_last_error()
+    if not os.path.exists("/dev/kfd"):
+        with pytest.raises(afe.AfeError) as e:
+            afe.Context(0)
+        assert e.value.code == -8
+
+
+def test_tuning_validation():
+    with pytest.raises(afe.AfeError):
+        afe.set_tuning(2, 1024, 4096)
+    afe.set_tuning(0, 2048, 4096)

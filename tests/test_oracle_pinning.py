@@ -1,0 +1,463 @@
+"""Pinning the oracle (CPU).  Each oracle function is checked against
+something that does not restate its own formula:
+
+  quadrature tables   the reference's .tmp_geo text (when the tree is present)
+                      and exact monomial integrals on the reference simplex
+  plugins             golden values produced by the reference's own C plugin
+                      sources (tests/golden/plugins.npz, make_golden.py)
+  DOF numbering       a brute-force first-touch numbering written here
+  CSR pattern         the reference's own SparsityPattern class
+                      (tests/golden/patterns.npz) and closed-form nnz counts
+  element matrices    closed forms: P1 |T| grad.grad * sum(w), P1 mass
+                      |T|/12 (1+delta), P2 stiffness by exact integration of
+                      barycentric monomials, tetrahedron |T| grad.grad
+  L2Discretize        exact integrals of interpolated polynomials
+  Dirichlet apply     the algebraic identity: the eliminated system has the
+                      solution of the constrained original system
+  Project / getPMPt   textbook 1-D result (C points every other node, weights
+                      1/2), dense P A P^T, unit C rows, P >= 0, P^T = P'
+  lazyProject         partition of unity of the interpolation columns
+  Gauss-Seidel        triangular solve (D+L)^-1 (r - U x) with scipy
+  V-cycle             two-level error-propagation operator built densely
+  dense inverse       A A^-1 = I
+  L2/H1 errors        analytic norms of u = sin(pi x) sin(pi y) (u_h = 0) and
+                      the P1/P2 convergence rates
+"""
+import math
+import os
+import re
+from math import factorial
+
+import numpy as np
+import pytest
+
+from oracle import oracle_py as orc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF = "/root/reference/include/AFEPack/template"
+
+
+def mesh(kind, n, jitter=0.0, seed=0):
+    import afepack_amd as afe
+    return afe.mesh_cube(n, jitter, seed) if kind == orc.P1_TET else afe.mesh_square(n, jitter, seed)
+
+
+# ---------------------------------------------------------------- quadrature
+def _parse_tmp_geo(path, tdim):
+    """quadrature rules of a .tmp_geo file: {acc: (points, weights)}.
+    Layout: library path, volume function, vertex list, then per dimension
+    0..tdim the sub-geometries (index, vertex list, boundary list), then the
+    number of rules and (accuracy, count, rows of coordinates + weight)."""
+    txt = open(path).read()
+    toks = " ".join(re.sub(r"#.*", "", l) for l in txt.splitlines()).split()
+    pos = 2  # library path, volume function name
+    npt = int(toks[pos]); pos += 1 + npt * tdim
+    for _ in range(tdim + 1):
+        ng = int(toks[pos]); pos += 1
+        for _ in range(ng):
+            pos += 1  # index
+            nv = int(toks[pos]); pos += 1 + nv
+            nb = int(toks[pos]); pos += 1 + nb
+    nrules = int(toks[pos]); pos += 1
+    out = {}
+    for _ in range(nrules):
+        acc, n = int(toks[pos]), int(toks[pos + 1])
+        pos += 2
+        a = np.array([float(x) for x in toks[pos:pos + n * (tdim + 1)]]).reshape(n, tdim + 1)
+        pos += n * (tdim + 1)
+        out[acc] = (a[:, :tdim], a[:, tdim])
+    return out
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference tree not present")
+@pytest.mark.parametrize("tdim,fname,accs", [(2, "triangle/triangle.tmp_geo", [1, 2, 3, 4, 5, 6, 7]),
+                                             (3, "tetrahedron/tetrahedron.tmp_geo", [1, 2, 3, 4, 6])])
+def test_quadrature_tables_equal_reference_text(tdim, fname, accs):
+    rules = _parse_tmp_geo(os.path.join(REF, fname), tdim)
+    for acc in accs:
+        pts, w = orc.quad(tdim, acc)
+        rp, rw = rules[acc]
+        assert np.array_equal(pts, rp) and np.array_equal(w, rw)
+
+
+@pytest.mark.parametrize("tdim,accs", [(2, [1, 2, 3, 4, 5, 6, 7]), (3, [1, 2, 3, 4, 6])])
+def test_quadrature_integrates_monomials_exactly(tdim, accs):
+    # int over the reference simplex of x^a y^b [z^c] = a! b! c! / (a+b+c+d)!
+    for acc in accs:
+        pts, w = orc.quad(tdim, acc)
+        vol = 0.5 if tdim == 2 else 1.0 / 6.0
+        for a in range(acc + 1):
+            for b in range(acc + 1 - a):
+                for c in (range(acc + 1 - a - b) if tdim == 3 else [0]):
+                    exact = factorial(a) * factorial(b) * factorial(c) / factorial(a + b + c + tdim)
+                    f = pts[:, 0] ** a * pts[:, 1] ** b * (pts[:, 2] ** c if tdim == 3 else 1)
+                    approx = vol * np.dot(w, f)
+                    # tables are 8-16 digit decimals
+                    tol = 5e-8 if tdim == 3 and acc <= 3 else 5e-14
+                    assert abs(approx - exact) <= tol * max(1.0, exact * 100), (tdim, acc, a, b, c)
+
+
+def test_quadrature_lookup_picks_next_accuracy():
+    assert orc.quad(3, 5)[0].shape[0] == orc.quad(3, 6)[0].shape[0] == 24
+
+
+# ------------------------------------------------------------------- plugins
+def test_plugins_bitwise_equal_reference_sources():
+    g = np.load(os.path.join(HERE, "golden", "plugins.npz"))
+    assert orc.template_volume(2) == g["tri_ref_volume"][0]
+    assert orc.template_volume(3) == g["tet_ref_volume"][0]
+    T, Q = g["tri_local"].shape[:2]
+    for t in range(T):
+        v = g["tri_vertices"][t]
+        assert orc.l2g_jacobian(2, v) == g["tri_jacobian"][t]
+        for q in range(Q):
+            gp = orc.local_to_global(2, g["tri_local"][t, q], v)
+            assert np.array_equal(gp, g["tri_global"][t, q])
+            val, grad = orc.basis_eval(orc.P1_TRI, gp, v)
+            assert np.array_equal(val, g["p1_values"][t, q]) and np.array_equal(grad, g["p1_gradients"][t, q])
+            val, grad = orc.basis_eval(orc.P2_TRI, gp, v)
+            assert np.array_equal(val, g["p2_values"][t, q]) and np.array_equal(grad, g["p2_gradients"][t, q])
+    for t in range(T):
+        v = g["tet_vertices"][t]
+        assert orc.l2g_jacobian(3, v) == g["tet_jacobian"][t]
+        for q in range(Q):
+            gp = orc.local_to_global(3, g["tet_local"][t, q], v)
+            assert np.array_equal(gp, g["tet_global"][t, q])
+            val, grad = orc.basis_eval(orc.P1_TET, gp, v)
+            assert np.array_equal(val, g["tet_values"][t, q]) and np.array_equal(grad, g["tet_gradients"][t, q])
+
+
+# ------------------------------------------------------------- DOF numbering
+def _brute_first_touch(kind, ev):
+    nv = 4 if kind == orc.P1_TET else 3
+    ev = ev.reshape(-1, nv)
+    ids = {}
+    out = []
+    for e in ev:
+        row = []
+        keys = [("v", int(p)) for p in e]
+        if kind == orc.P2_TRI:
+            keys += [("e", tuple(sorted((int(e[(k + 1) % 3]), int(e[(k + 2) % 3]))))) for k in range(3)]
+        for key in keys:
+            if key not in ids:
+                ids[key] = len(ids)
+            row.append(ids[key])
+        out.append(row)
+    return np.array(out, dtype=np.int32).ravel(), len(ids)
+
+
+@pytest.mark.parametrize("kind,n,jit", [(1, 6, 0.0), (2, 5, 0.2), (3, 3, 0.0)])
+def test_dof_numbering_first_touch(kind, n, jit):
+    c, ev, vm = mesh(kind, n, jit, 3)
+    s = orc.Space(kind, c, ev, vm)
+    ed, dm, dp = s.dofs()
+    bed, nd = _brute_first_touch(kind, ev)
+    assert s.n_dof == nd and np.array_equal(ed, bed)
+    # boundary marks: vertex marks, and edges lying on the square's boundary
+    dim = s.dim
+    pts = dp.reshape(-1, dim)
+    on_bnd = np.any((np.abs(pts) < 1e-12) | (np.abs(pts - 1) < 1e-12), axis=1)
+    assert np.array_equal(dm != 0, on_bnd)
+
+
+# --------------------------------------------------------------- CSR pattern
+@pytest.mark.parametrize("tag,kind,n", [("p1_8", 1, 8), ("p2_4", 2, 4), ("tet_3", 3, 3)])
+def test_pattern_equals_reference_sparsity_class(tag, kind, n):
+    g = np.load(os.path.join(HERE, "golden", "patterns.npz"))
+    c, ev, vm = mesh(kind, n)
+    s = orc.Space(kind, c, ev, vm)
+    ed, _, _ = s.dofs()
+    assert np.array_equal(ed, g[tag + "_elem_dof"].ravel())
+    rp, col, _ = s.build(orc.OP_STIFF, 2).csr()
+    assert np.array_equal(rp, g[tag + "_row_ptr"]) and np.array_equal(col, g[tag + "_col"])
+
+
+@pytest.mark.parametrize("kind,n,formula", [(1, 9, lambda n: 7 * n * n + 6 * n + 1),
+                                            (2, 7, lambda n: 46 * n * n + 16 * n + 1),
+                                            (3, 4, lambda n: 15 * n ** 3 + 21 * n * n + 9 * n + 1)])
+def test_pattern_closed_form_nnz_and_layout(kind, n, formula):
+    c, ev, vm = mesh(kind, n)
+    s = orc.Space(kind, c, ev, vm)
+    rp, col, _ = s.build(orc.OP_STIFF, 2).csr()
+    assert rp[-1] == formula(n)
+    for i in range(len(rp) - 1):
+        r = col[rp[i]:rp[i + 1]]
+        assert r[0] == i and np.all(np.diff(r[1:]) > 0)
+
+
+# ---------------------------------------------------------- element matrices
+def _tri_area(v):
+    return 0.5 * ((v[1, 0] - v[0, 0]) * (v[2, 1] - v[0, 1]) - (v[1, 1] - v[0, 1]) * (v[2, 0] - v[0, 0]))
+
+
+def _grad_lambda(v):
+    d = v.shape[1]
+    M = np.ones((d + 1, d + 1))
+    M[:, 1:] = v
+    return np.linalg.inv(M)[1:, :].T  # row i: grad lambda_i
+
+
+def test_p1_element_matrices_closed_form():
+    rng = np.random.default_rng(1)
+    for _ in range(10):
+        v = rng.uniform(-1, 1, (3, 2))
+        if _tri_area(v) < 0.1:
+            v = v[[0, 2, 1]]
+        if _tri_area(v) < 0.1:
+            continue
+        a = _tri_area(v)
+        G = _grad_lambda(v)
+        for acc, wsum in ((1, 1.0), (4, 0.999999999999999), (6, 1.000000000000002)):
+            K = orc.element_matrix(orc.P1_TRI, orc.OP_STIFF, acc, v)
+            assert np.allclose(K, wsum * a * G @ G.T, rtol=1e-13, atol=1e-13 * np.abs(K).max())
+            assert np.allclose(K.sum(axis=1), 0, atol=1e-13 * np.abs(K).max())
+        M = orc.element_matrix(orc.P1_TRI, orc.OP_MASS, 4, v)
+        assert np.allclose(M, a / 12 * (np.ones((3, 3)) + np.eye(3)), rtol=1e-13)
+
+
+def _bary_integral(area, powers):
+    # int_T l0^a l1^b l2^c = 2|T| a! b! c! / (a+b+c+2)!
+    a, b, c = powers
+    return 2 * area * factorial(a) * factorial(b) * factorial(c) / factorial(a + b + c + 2)
+
+
+def test_p2_stiffness_exact_integration():
+    rng = np.random.default_rng(2)
+    for _ in range(6):
+        v = rng.uniform(-1, 1, (3, 2))
+        if _tri_area(v) < 0.1:
+            v = v[[0, 2, 1]]
+        if _tri_area(v) < 0.1:
+            continue
+        area = _tri_area(v)
+        G = _grad_lambda(v)
+        # gradients as linear combinations sum_i c_i(lambda) G_i: represent each
+        # basis gradient as dict {monomial(lambda) : vector}
+        def grads():
+            out = []
+            for i in range(3):  # (4 l_i - 1) G_i
+                e = [0, 0, 0]
+                e[i] = 1
+                out.append({tuple(e): 4 * G[i], (0, 0, 0): -G[i]})
+            for (p, q) in ((1, 2), (0, 2), (0, 1)):  # 4 (l_q G_p + l_p G_q)
+                ep, eq = [0, 0, 0], [0, 0, 0]
+                ep[q] = 1
+                eq[p] = 1
+                out.append({tuple(ep): 4 * G[p], tuple(eq): 4 * G[q]})
+            return out
+        gr = grads()
+        Kx = np.zeros((6, 6))
+        for j in range(6):
+            for k in range(6):
+                s = 0.0
+                for m1, v1 in gr[j].items():
+                    for m2, v2 in gr[k].items():
+                        s += np.dot(v1, v2) * _bary_integral(area, tuple(x + y for x, y in zip(m1, m2)))
+                Kx[j, k] = s
+        K = orc.element_matrix(orc.P2_TRI, orc.OP_STIFF, 6, v)
+        # acc 6 weights sum to 1.000000000000002: relative 2e-15 plus rounding
+        assert np.abs(K - Kx).max() <= 2e-13 * np.abs(Kx).max()
+
+
+def test_tet_element_matrices_closed_form():
+    rng = np.random.default_rng(3)
+    for _ in range(6):
+        v = rng.uniform(-1, 1, (4, 3))
+        vol = np.linalg.det(np.array([v[1] - v[0], v[2] - v[0], v[3] - v[0]])) / 6
+        if vol < 0:
+            v = v[[0, 2, 1, 3]]
+            vol = -vol
+        if vol < 0.02:
+            continue
+        G = _grad_lambda(v)
+        K = orc.element_matrix(orc.P1_TET, orc.OP_STIFF, 2, v)
+        assert np.allclose(K, vol * G @ G.T, rtol=1e-12, atol=1e-12 * np.abs(K).max())
+        M = orc.element_matrix(orc.P1_TET, orc.OP_MASS, 2, v)
+        assert np.allclose(M, vol / 20 * (np.ones((4, 4)) + np.eye(4)), rtol=2e-7)
+
+
+# ------------------------------------------------------------ right-hand side
+def test_l2discretize_of_linear_interpolant_exact():
+    c, ev, vm = mesh(orc.P1_TRI, 5, 0.2, 9)
+    s = orc.Space(orc.P1_TRI, c, ev, vm)
+    ed, _, dp = s.dofs()
+    pts = dp.reshape(-1, 2)
+    nodal = 1.0 + 2.0 * pts[:, 0] - 3.0 * pts[:, 1]  # linear: interpolant is exact
+    f = s.rhs(4, orc.F_NODAL, nodal=nodal)
+    # exact: int (sum_k f_k l_k) l_j over each triangle = |T|/12 (f_j + sum f)
+    ref = np.zeros_like(f)
+    coords = c.reshape(-1, 2)
+    for e, tri in zip(ed.reshape(-1, 3), ev.reshape(-1, 3)):
+        a = abs(_tri_area(coords[tri]))
+        fv = nodal[e]
+        for j in range(3):
+            ref[e[j]] += a / 12 * (fv[j] + fv.sum())
+    assert np.abs(f - ref).max() <= 1e-14 * np.abs(ref).max() + 1e-15
+    one = s.rhs(4, orc.F_ONE)
+    assert abs(one.sum() - 0.999999999999999) <= 1e-13
+
+
+# ----------------------------------------------------------------- Dirichlet
+def test_dirichlet_elimination_solves_constrained_system():
+    c, ev, vm = mesh(orc.P2_TRI, 4, 0.2, 4)
+    s = orc.Space(orc.P2_TRI, c, ev, vm)
+    A = s.build(orc.OP_STIFF, 6)
+    f = s.rhs(6, orc.F_POISSON)
+    D0 = A.dense()
+    f0 = f.copy()
+    s.apply_dirichlet(A, f)
+    D1 = A.dense()
+    _, dm, _ = s.dofs()
+    B = dm != 0
+    I = ~B
+    x1 = np.linalg.solve(D1, f)
+    # constrained original system: u_B = 0, A_II u_I = f_I
+    xi = np.linalg.solve(D0[np.ix_(I, I)], f0[I])
+    assert np.allclose(x1[B], 0) and np.allclose(x1[I], xi, rtol=1e-10)
+    assert np.allclose(D1, D1.T)
+
+
+# ----------------------------------------------------------------------- AMG
+def _csr_from_dense(D):
+    n = D.shape[0]
+    rp, col, val = [0], [], []
+    for i in range(n):
+        nz = [j for j in range(n) if D[i, j] != 0 and j != i]
+        row = [i] + nz
+        col += row
+        val += [D[i, j] for j in row]
+        rp.append(len(col))
+    return orc.Mat.from_csr(n, n, np.array(rp), np.array(col), np.array(val))
+
+
+def test_project_1d_laplacian_textbook():
+    n = 9
+    D = 2 * np.eye(n) - np.eye(n, k=1) - np.eye(n, k=-1)
+    A = _csr_from_dense(D)
+    g = orc.Amg(A, nmpo=3, nmpsd=0.0)
+    P = g.P(0).dense()
+    # C points every other node, linear interpolation weights 1/2
+    assert P.shape == (5, 9)
+    Pexp = np.zeros((5, 9))
+    for c in range(5):
+        Pexp[c, 2 * c] = 1.0
+        if 2 * c - 1 >= 0:
+            Pexp[c, 2 * c - 1] = 0.5
+        if 2 * c + 1 < n:
+            Pexp[c, 2 * c + 1] = 0.5
+    assert np.array_equal(P, Pexp)
+    assert np.allclose(g.A(1).dense(), Pexp @ D @ Pexp.T)
+
+
+@pytest.mark.parametrize("lazy", [False, True])
+@pytest.mark.parametrize("kind,n,jit,acc", [(1, 24, 0.0, 4), (2, 8, 0.2, 6), (3, 4, 0.0, 2)])
+def test_hierarchy_invariants(lazy, kind, n, jit, acc):
+    c, ev, vm = mesh(kind, n, jit, 5)
+    s = orc.Space(kind, c, ev, vm)
+    A = s.build(orc.OP_STIFF, acc)
+    f = s.rhs(acc, orc.F_ONE)
+    s.apply_dirichlet(A, f)
+    g = orc.Amg(A, lazy=lazy)
+    L = g.n_project
+    for l in range(L):
+        Al, P, Pt = g.A(l).dense(), g.P(l).dense(), g.Pt(l).dense()
+        assert np.allclose(g.A(l + 1).dense(), P @ Al @ Pt, rtol=1e-12, atol=1e-14 * np.abs(Al).max())
+        assert np.array_equal(Pt, P.T) and np.all(P >= 0)
+        if not lazy:
+            # each coarse point is a fine point with a unit row entry
+            assert np.all(np.isclose(P.max(axis=1), 1.0))
+        else:
+            colsum = P.sum(axis=0)
+            assert np.allclose(colsum, 1.0, rtol=1e-15)
+    Ac = g.A(L)
+    if g.full:
+        m = Ac.m
+        assert m <= 50 and Ac.nnz / (m * m) >= 0.382
+        Minv = g.coarse_inverse()
+        assert np.allclose(Minv @ Ac.dense(), np.eye(m), atol=1e-10)
+
+
+def test_gauss_seidel_is_triangular_solve():
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as spl
+    c, ev, vm = mesh(orc.P1_TRI, 10, 0.2, 6)
+    s = orc.Space(orc.P1_TRI, c, ev, vm)
+    A = s.build(orc.OP_STIFF, 4)
+    f = s.rhs(4, orc.F_POISSON)
+    s.apply_dirichlet(A, f)
+    rp, col, val = A.csr()
+    S = sps.csr_matrix((val, col, rp), shape=(len(rp) - 1,) * 2)
+    x = np.random.default_rng(0).uniform(-1, 1, len(f))
+    x0 = x.copy()
+    orc.gauss_seidel(A, x, f, 2)
+    Lw = sps.tril(S, 0).tocsr()
+    U = sps.triu(S, 1).tocsr()
+    y = x0
+    for _ in range(2):
+        y = spl.spsolve_triangular(Lw, f - U @ y, lower=True)
+    assert np.allclose(x, y, rtol=1e-13, atol=1e-15)
+
+
+def test_vcycle_two_level_error_propagation():
+    c, ev, vm = mesh(orc.P1_TRI, 8, 0.0, 0)
+    s = orc.Space(orc.P1_TRI, c, ev, vm)
+    A = s.build(orc.OP_STIFF, 4)
+    f = s.rhs(4, orc.F_POISSON)
+    s.apply_dirichlet(A, f)
+    g = orc.Amg(A, nmpo=60, nmpsd=0.0)  # exactly two levels
+    assert g.n_project == 1 and g.full
+    D, P, Ac = A.dense(), g.P(0).dense(), g.A(1).dense()
+    n = D.shape[0]
+    Lw = np.tril(D)
+    S = np.eye(n) - np.linalg.solve(Lw, D)           # one forward GS sweep
+    S3 = np.linalg.matrix_power(S, 3)
+    E = S3 @ (np.eye(n) - P.T @ np.linalg.solve(Ac, P @ D)) @ S3
+    xs = np.linalg.solve(D, f)
+    x = np.random.default_rng(1).uniform(-1, 1, n)
+    pred = xs + E @ (x - xs)
+    g.solve(x, f, smooth=3, step=0)                  # exactly one cycle
+    assert np.allclose(x, pred, rtol=1e-10, atol=1e-12)
+
+
+def test_dense_inverse():
+    rng = np.random.default_rng(4)
+    M = rng.uniform(-1, 1, (30, 30)) + 30 * np.eye(30)
+    rc, Mi = orc.dense_inverse(M)
+    assert rc == 0 and np.allclose(Mi @ M, np.eye(30), atol=1e-13)
+    rc, _ = orc.dense_inverse(np.zeros((3, 3)))
+    assert rc == -1
+
+
+def test_lazy_reinit_requires_same_pattern_object():
+    c, ev, vm = mesh(orc.P1_TRI, 12)
+    s = orc.Space(orc.P1_TRI, c, ev, vm)
+    A = s.build(orc.OP_STIFF, 4)
+    g = orc.Amg(A, lazy=True)
+    rp, col, val = A.csr()
+    B = A.share_pattern(2.0 * val)
+    assert g.lazy_reinit(B) == 0
+    assert np.allclose(g.A(1).dense(), 2.0 * orc.Amg(A, lazy=True).A(1).dense())
+    other = orc.Mat.from_csr(len(rp) - 1, len(rp) - 1, rp, col, val)
+    assert g.lazy_reinit(other) == -1
+
+
+# ------------------------------------------------------------- error norms
+def test_error_functionals_analytic_and_rates():
+    c, ev, vm = mesh(orc.P1_TRI, 16)
+    s = orc.Space(orc.P1_TRI, c, ev, vm)
+    z = np.zeros(s.n_dof)
+    assert abs(s.l2_error(z, 6) - 0.5) < 1e-6
+    assert abs(s.h1_error(z, 6) - math.pi / math.sqrt(2)) < 1e-5
+    # interpolation errors converge at O(h^2)/O(h) for P1 and O(h^3)/O(h^2) for P2
+    for kind, pl2, ph1 in ((orc.P1_TRI, 2, 1), (orc.P2_TRI, 3, 2)):
+        errs = []
+        for n in (8, 16):
+            c, ev, vm = mesh(kind, n)
+            s = orc.Space(kind, c, ev, vm)
+            _, _, dp = s.dofs()
+            p = dp.reshape(-1, 2)
+            u = np.sin(np.pi * p[:, 0]) * np.sin(np.pi * p[:, 1])
+            errs.append((s.l2_error(u, 6), s.h1_error(u, 6)))
+        r2 = math.log2(errs[0][0] / errs[1][0])
+        r1 = math.log2(errs[0][1] / errs[1][1])
+        assert abs(r2 - pl2) < 0.15 and abs(r1 - ph1) < 0.15
