@@ -234,6 +234,30 @@ class Space:
         _chk(_lib().afe_rhs(self.h, acc, fkind, fp, _ptr(nodal), _ptr(out)))
         return out
 
+    def qpoints(self, acc):
+        """Physical quadrature points (n_elem, n_qp, dim) and Jxw (n_elem, n_qp)."""
+        nq = C.c_int64()
+        _chk(_lib().afe_space_qpoints(self.h, acc, C.byref(nq), None, None))
+        pts = np.empty(self.n_elem * nq.value * self.dim, dtype=np.float64)
+        jxw = np.empty(self.n_elem * nq.value, dtype=np.float64)
+        _chk(_lib().afe_space_qpoints(self.h, acc, C.byref(nq), _ptr(pts), _ptr(jxw)))
+        return pts.reshape(self.n_elem, nq.value, self.dim), jxw.reshape(self.n_elem, nq.value)
+
+    def rhs_qp(self, acc, fq, out=None):
+        """L2Discretize of f given at the quadrature points (n_elem, n_qp)."""
+        if out is None:
+            out = DeviceVector(self.ctx, self.n_dof, zero=False)
+        fq = np.ascontiguousarray(fq, dtype=np.float64)
+        _chk(_lib().afe_rhs_qp(self.h, acc, _ptr(fq), _ptr(out)))
+        return out
+
+    def error_qp(self, uh, acc, exact, h1=False):
+        """L2 error (h1=False, exact values) or H1 semi-error (exact gradients)."""
+        e = C.c_double()
+        ex = np.ascontiguousarray(exact, dtype=np.float64)
+        _chk(_lib().afe_error_qp(self.h, _ptr(uh), acc, 1 if h1 else 0, _ptr(ex), C.byref(e)))
+        return e.value
+
     def error_l2(self, uh, acc):
         e = C.c_double()
         _chk(_lib().afe_error_l2(self.h, _ptr(uh), acc, C.byref(e)))
@@ -292,6 +316,12 @@ class Matrix:
 
     def apply_dirichlet(self, f, u=None):
         _chk(_lib().afe_apply_dirichlet(self.h, _ptr(u), _ptr(f)))
+
+    def apply_dirichlet_values(self, marks, g, f, u=None):
+        """Dirichlet elimination on the DOFs whose mark is in marks, data g per DOF."""
+        m = np.ascontiguousarray(marks, dtype=np.int32)
+        gg = np.ascontiguousarray(g, dtype=np.float64)
+        _chk(_lib().afe_apply_dirichlet_values(self.h, _ptr(m), len(m), _ptr(gg), _ptr(u), _ptr(f)))
 
     def vmult(self, x, y):
         _chk(_lib().afe_matrix_vmult(self.h, _ptr(x), _ptr(y)))

@@ -691,6 +691,116 @@ extern "C" int afe_apply_dirichlet(afe_matrix *a, double *d_u, double *d_f) {
   API_END
 }
 
+extern "C" int afe_space_qpoints(afe_space *s, int32_t acc, int64_t *n_qp, double *h_pts, double *h_jxw) {
+  API_BEGIN
+  if (!s || !s->ctx) return fail(AFE_E_ARG, "space without device context");
+  const QuadRule *q = quad_find(s->h.tdim, acc);
+  if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
+  if (n_qp) *n_qp = q->n;
+  if (!h_pts && !h_jxw) return AFE_OK;
+  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  HIPCHK(hipSetDevice(s->ctx->device));
+  const int64_t ne = s->h.n_elem;
+  const int D = s->h.tdim;
+  DevBuf<double> pts, jxw;
+  pts.alloc(s->ctx, (size_t)ne * q->n * D);
+  jxw.alloc(s->ctx, (size_t)ne * q->n);
+  const double *qd = s->ctx->d_quad + q->off;
+  hipStream_t st = s->ctx->stream;
+  switch (s->h.kind) {
+  case P1_TRI: k_qpoints<P1_TRI><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, qd, q->n, pts.p, jxw.p); break;
+  case P2_TRI: k_qpoints<P2_TRI><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, qd, q->n, pts.p, jxw.p); break;
+  default: k_qpoints<P1_TET><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, qd, q->n, pts.p, jxw.p); break;
+  }
+  HIPCHK(hipGetLastError());
+  if (h_pts) HIPCHK(hipMemcpyAsync(h_pts, pts.p, pts.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (h_jxw) HIPCHK(hipMemcpyAsync(h_jxw, jxw.p, jxw.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  API_END
+}
+
+extern "C" int afe_rhs_qp(afe_space *s, int32_t acc, const double *h_fq, double *d_f) {
+  API_BEGIN
+  if (!s || !s->ctx || !h_fq || !d_f) return fail(AFE_E_ARG, "null argument");
+  const QuadRule *q = quad_find(s->h.tdim, acc);
+  if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
+  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  HIPCHK(hipSetDevice(s->ctx->device));
+  const int64_t ne = s->h.n_elem;
+  DevBuf<double> fq, Fe;
+  fq.upload(s->ctx, h_fq, (size_t)ne * q->n);
+  Fe.alloc(s->ctx, (size_t)ne * s->h.n_loc);
+  const double *qd = s->ctx->d_quad + q->off;
+  hipStream_t st = s->ctx->stream;
+  switch (s->h.kind) {
+  case P1_TRI: k_element_loads_qp<P1_TRI><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, qd, q->n, fq.p, Fe.p); break;
+  case P2_TRI: k_element_loads_qp<P2_TRI><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, qd, q->n, fq.p, Fe.p); break;
+  default: k_element_loads_qp<P1_TET><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, qd, q->n, fq.p, Fe.p); break;
+  }
+  k_gather_vec<<<nblk(s->h.n_dof), 256, 0, st>>>(s->h.n_dof, s->h.n_loc, s->adj_ptr.p, s->adj.p, Fe.p, d_f);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  API_END
+}
+
+extern "C" int afe_apply_dirichlet_values(afe_matrix *a, const int32_t *marks, int32_t n_marks, const double *h_g,
+                                          double *d_u, double *d_f) {
+  API_BEGIN
+  (void)d_u;
+  if (!a || !a->s || !d_f || !h_g || (n_marks > 0 && !marks)) return fail(AFE_E_ARG, "bad arguments");
+  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  HIPCHK(hipSetDevice(a->ctx->device));
+  afe_space *s = a->s;
+  const int64_t n = s->h.n_dof;
+  std::vector<uint8_t> flag(n, 0);
+  for (int64_t i = 0; i < n; i++) {
+    const int32_t m = s->h.dof_mark[i];
+    for (int32_t k = 0; k < n_marks; k++)
+      if (marks[k] == m) flag[i] = 1;
+  }
+  DevBuf<uint8_t> dflag;
+  DevBuf<double> dg, diag0;
+  dflag.upload(a->ctx, flag);
+  dg.upload(a->ctx, h_g, n);
+  diag0.alloc(a->ctx, n);
+  hipStream_t st = a->ctx->stream;
+  k_csr_diag<<<nblk(n), 256, 0, st>>>(n, s->rp.p, a->val.p, diag0.p);
+  k_dirichlet_values<<<nblk(n), 256, 0, st>>>(n, s->rp.p, s->col.p, dflag.p, diag0.p, dg.p, a->val.p, d_f);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  API_END
+}
+
+extern "C" int afe_error_qp(afe_space *s, const double *d_uh, int32_t acc, int32_t h1, const double *h_exact,
+                            double *err) {
+  API_BEGIN
+  if (!s || !s->ctx || !d_uh || !h_exact || !err) return fail(AFE_E_ARG, "null argument");
+  const QuadRule *q = quad_find(s->h.tdim, acc);
+  if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
+  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  HIPCHK(hipSetDevice(s->ctx->device));
+  const int64_t ne = s->h.n_elem;
+  DevBuf<double> ex, pe, part, tot;
+  ex.upload(s->ctx, h_exact, (size_t)ne * q->n * (h1 ? s->h.tdim : 1));
+  pe.alloc(s->ctx, ne);
+  part.alloc(s->ctx, RED_BLOCKS);
+  tot.alloc(s->ctx, 1);
+  const double *qd = s->ctx->d_quad + q->off;
+  hipStream_t st = s->ctx->stream;
+  switch (s->h.kind) {
+  case P1_TRI: k_element_error_qp<P1_TRI><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, s->elem_dof.p, qd, q->n, h1, d_uh, ex.p, pe.p); break;
+  case P2_TRI: k_element_error_qp<P2_TRI><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, s->elem_dof.p, qd, q->n, h1, d_uh, ex.p, pe.p); break;
+  default: k_element_error_qp<P1_TET><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, s->elem_dof.p, qd, q->n, h1, d_uh, ex.p, pe.p); break;
+  }
+  k_sum_partial<<<RED_BLOCKS, 256, 0, st>>>(ne, pe.p, part.p);
+  k_finalize<<<1, 256, 0, st>>>(part.p, RED_BLOCKS, 1, tot.p);
+  double h = 0;
+  HIPCHK(hipMemcpyAsync(&h, tot.p, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *err = std::sqrt(std::fabs(h));
+  API_END
+}
+
 template <int KIND>
 static void launch_error(afe_space *s, const QuadRule *q, int h1, const double *uh, double *out) {
   const int64_t ne = s->h.n_elem;

@@ -9,7 +9,10 @@
 
 namespace afe {
 
+#ifndef AFE_KIND_ENUM
+#define AFE_KIND_ENUM
 enum { P1_TRI = 1, P2_TRI = 2, P1_TET = 3 };
+#endif
 inline int kind_tdim(int k) { return k == P1_TET ? 3 : 2; }
 inline int kind_nloc(int k) { return k == P1_TRI ? 3 : (k == P2_TRI ? 6 : 4); }
 inline int kind_nvtx(int k) { return k == P1_TET ? 4 : 3; }

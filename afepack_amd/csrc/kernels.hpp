@@ -11,6 +11,7 @@
 
 #include <cstdint>
 
+#include "element_geom.hpp"
 #include "host.hpp"
 
 namespace afe {
@@ -47,166 +48,11 @@ __device__ inline double field_g(const FieldDev &f, const double *x, double *gra
   return f.scale * g;
 }
 
-// ---------------------------------------------------------------------------
-// template-element plugins evaluated in the reference's floating-point order
-// ---------------------------------------------------------------------------
-// The stiffness and mass kernels reproduce the reference's arithmetic step by
-// step -- quadrature points mapped through barycentric area ratios of the
-// reference simplex (triangle.crd_trs.c:15-27, tetrahedron.crd_trs.c:18-32),
-// basis functions evaluated at the physical point with the physical vertices
-// (triangle.1/2.bas_fun.c, tetrahedron.1.bas_fun.c), Jxw = w * jacobian *
-// volume and K(j,k) += Jxw * <g_j, g_k> -- with floating-point contraction
-// off, so each element matrix is bit-identical to the host restatement.
+// The element kernels below follow the reference's arithmetic step by step
+// (the plugins of element_geom.hpp, Jxw = w * jacobian * volume, K(j,k) +=
+// Jxw * <g_j, g_k>) with floating-point contraction off, so each element
+// matrix is bit-identical to the host restatement.
 #pragma clang fp contract(off)
-
-__device__ inline double area2(const double *a, const double *b, const double *c) {
-  return (b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0]);
-}
-
-__device__ inline double det3(const double *v0, const double *v1, const double *v2,
-                              const double *v3) {
-  const double a0 = v1[0] - v0[0], a1 = v1[1] - v0[1], a2 = v1[2] - v0[2];
-  const double b0 = v2[0] - v0[0], b1 = v2[1] - v0[1], b2 = v2[2] - v0[2];
-  const double c0 = v3[0] - v0[0], c1 = v3[1] - v0[1], c2 = v3[2] - v0[2];
-  return a0 * b1 * c2 + a1 * b2 * c0 + a2 * b0 * c1 - a0 * b2 * c1 - a1 * b0 * c2 - a2 * b1 * c0;
-}
-
-template <int KIND> struct KindInfo;
-template <> struct KindInfo<P1_TRI> { static constexpr int D = 2, NV = 3, NL = 3; };
-template <> struct KindInfo<P2_TRI> { static constexpr int D = 2, NV = 3, NL = 6; };
-template <> struct KindInfo<P1_TET> { static constexpr int D = 3, NV = 4, NL = 4; };
-
-template <int KIND>
-__device__ inline void load_elem(int64_t e, const double *__restrict__ coords,
-                                 const int32_t *__restrict__ ev, double v[][KindInfo<KIND>::D]) {
-  constexpr int D = KindInfo<KIND>::D, NV = KindInfo<KIND>::NV;
-#pragma unroll
-  for (int j = 0; j < NV; j++) {
-    const int64_t p = ev[e * NV + j];
-#pragma unroll
-    for (int d = 0; d < D; d++) v[j][d] = coords[p * D + d];
-  }
-}
-
-// CoordTransform::local_to_global_jacobian
-template <int D> __device__ inline double l2g_jacobian(const double v[][D]) {
-  if constexpr (D == 2) {
-    const double r0[2] = {0.0, 0.0}, r1[2] = {1.0, 0.0}, r2[2] = {0.0, 1.0};
-    return area2(v[0], v[1], v[2]) / area2(r0, r1, r2);
-  } else {
-    const double r0[3] = {0.0, 0.0, 0.0}, r1[3] = {1.0, 0.0, 0.0}, r2[3] = {0.0, 1.0, 0.0},
-                 r3[3] = {0.0, 0.0, 1.0};
-    return (det3(v[0], v[1], v[2], v[3]) / 6.) / (det3(r0, r1, r2, r3) / 6.);
-  }
-}
-
-// TemplateGeometry::volume of the reference simplex
-template <int D> __device__ inline double tmpl_volume() {
-  if constexpr (D == 2) {
-    return .5 * ((1.0 - 0.0) * (1.0 - 0.0) - (0.0 - 0.0) * (0.0 - 0.0));
-  } else {
-    const double r0[3] = {0.0, 0.0, 0.0}, r1[3] = {1.0, 0.0, 0.0}, r2[3] = {0.0, 1.0, 0.0},
-                 r3[3] = {0.0, 0.0, 1.0};
-    return det3(r0, r1, r2, r3) / 6.;
-  }
-}
-
-// CoordTransform::local_to_global of the reference point lp
-template <int D> __device__ inline void local_to_global(const double *lp, const double v[][D], double *x) {
-  if constexpr (D == 2) {
-    const double r0[2] = {0.0, 0.0}, r1[2] = {1.0, 0.0}, r2[2] = {0.0, 1.0};
-    const double ar = area2(r0, r1, r2);
-    const double l0 = area2(lp, r1, r2) / ar, l1 = area2(lp, r2, r0) / ar, l2 = area2(lp, r0, r1) / ar;
-    x[0] = l0 * v[0][0] + l1 * v[1][0] + l2 * v[2][0];
-    x[1] = l0 * v[0][1] + l1 * v[1][1] + l2 * v[2][1];
-  } else {
-    const double r0[3] = {0.0, 0.0, 0.0}, r1[3] = {1.0, 0.0, 0.0}, r2[3] = {0.0, 1.0, 0.0},
-                 r3[3] = {0.0, 0.0, 1.0};
-    const double vol = det3(r0, r1, r2, r3) / 6.;
-    const double l0 = det3(lp, r1, r2, r3) / 6. / vol, l1 = det3(r0, lp, r2, r3) / 6. / vol;
-    const double l2 = det3(r0, r1, lp, r3) / 6. / vol, l3 = det3(r0, r1, r2, lp) / 6. / vol;
-#pragma unroll
-    for (int d = 0; d < 3; d++) x[d] = l0 * v[0][d] + l1 * v[1][d] + l2 * v[2][d] + l3 * v[3][d];
-  }
-}
-
-// basis values at the physical point p (local order of the .bas_fun files)
-template <int KIND>
-__device__ inline void basis_val(const double *p, const double v[][KindInfo<KIND>::D], double *val) {
-  if constexpr (KIND == P1_TRI) {
-    const double ar = area2(v[0], v[1], v[2]);
-    val[0] = area2(p, v[1], v[2]) / ar;
-    val[1] = area2(v[0], p, v[2]) / ar;
-    val[2] = area2(v[0], v[1], p) / ar;
-  } else if constexpr (KIND == P2_TRI) {
-    const double ar = area2(v[0], v[1], v[2]);
-    const double l0 = area2(p, v[1], v[2]) / ar, l1 = area2(p, v[2], v[0]) / ar,
-                 l2 = area2(p, v[0], v[1]) / ar;
-    val[0] = l0 * (2 * l0 - 1.0);
-    val[1] = l1 * (2 * l1 - 1.0);
-    val[2] = l2 * (2 * l2 - 1.0);
-    val[3] = 4.0 * l1 * l2;
-    val[4] = 4.0 * l0 * l2;
-    val[5] = 4.0 * l0 * l1;
-  } else {
-    const double vol = det3(v[0], v[1], v[2], v[3]);
-    val[0] = det3(p, v[1], v[2], v[3]) / vol;
-    val[1] = det3(v[0], p, v[2], v[3]) / vol;
-    val[2] = det3(v[0], v[1], p, v[3]) / vol;
-    val[3] = det3(v[0], v[1], v[2], p) / vol;
-  }
-}
-
-// basis gradients at the physical point p
-template <int KIND>
-__device__ inline void basis_grad(const double *p, const double v[][KindInfo<KIND>::D],
-                                  double g[][KindInfo<KIND>::D]) {
-  if constexpr (KIND == P1_TRI || KIND == P2_TRI) {
-    const double ar = area2(v[0], v[1], v[2]);
-    double dl[3][2];
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-      const int a = (i + 1) % 3, b = (i + 2) % 3;
-      dl[i][0] = (v[a][1] - v[b][1]) / ar;
-      dl[i][1] = (v[b][0] - v[a][0]) / ar;
-    }
-    if constexpr (KIND == P1_TRI) {
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        g[i][0] = dl[i][0];
-        g[i][1] = dl[i][1];
-      }
-    } else {
-      const double l0 = area2(p, v[1], v[2]) / ar, l1 = area2(p, v[2], v[0]) / ar,
-                   l2 = area2(p, v[0], v[1]) / ar;
-      const double lam[3] = {l0, l1, l2};
-#pragma unroll
-      for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int d = 0; d < 2; d++) g[i][d] = (4.0 * lam[i] - 1.0) * dl[i][d];
-#pragma unroll
-      for (int d = 0; d < 2; d++) {
-        g[3][d] = 4.0 * (l2 * dl[1][d] + l1 * dl[2][d]);
-        g[4][d] = 4.0 * (l0 * dl[2][d] + l2 * dl[0][d]);
-        g[5][d] = 4.0 * (l1 * dl[0][d] + l0 * dl[1][d]);
-      }
-    }
-  } else {
-    const double vol = det3(v[0], v[1], v[2], v[3]);
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
-      const int ia = (m + 1) % 4, ib = (m + 2) % 4, ic = (m + 3) % 4;
-      const double sgn = (m % 2 == 0) ? -1. : 1.;
-#pragma unroll
-      for (int n = 0; n < 3; n++) {
-        const int n1 = (n + 1) % 3, n2 = (n + 2) % 3;
-        const double cof = sgn * ((v[ib][n1] - v[ia][n1]) * (v[ic][n2] - v[ia][n2]) -
-                                  (v[ib][n2] - v[ia][n2]) * (v[ic][n1] - v[ia][n1]));
-        g[m][n] = cof / vol;
-      }
-    }
-  }
-}
 
 // ---------------------------------------------------------------------------
 // element matrices: StiffMatrix / MassMatrix / variable-coefficient stiffness
@@ -366,6 +212,111 @@ __global__ __launch_bounds__(256) void k_element_loads(
   for (int j = 0; j < NL; j++) Fe[e * NL + j] = F[j];
 }
 
+// load vectors from f given at the physical quadrature points, fq[e*nq + l]
+// (Operator::L2Discretize with a host function, Operator.discretize.templates.h:121-152)
+template <int KIND>
+__global__ __launch_bounds__(256) void k_element_loads_qp(int64_t n_elem, const double *__restrict__ coords,
+                                                          const int32_t *__restrict__ ev,
+                                                          const double *__restrict__ qd, int nq,
+                                                          const double *__restrict__ fq, double *__restrict__ Fe) {
+  constexpr int D = KindInfo<KIND>::D, NL = KindInfo<KIND>::NL;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_elem) return;
+  double v[D + 1][D];
+  load_elem<KIND>(e, coords, ev, v);
+  const double volume = tmpl_volume<D>();
+  const double jac = l2g_jacobian<D>(v);
+  double F[NL];
+#pragma unroll
+  for (int j = 0; j < NL; j++) F[j] = 0.0;
+  for (int l = 0; l < nq; l++) {
+    const double *q = qd + l * (D + 1);
+    double x[D];
+    local_to_global<D>(q, v, x);
+    double val[NL];
+    basis_val<KIND>(x, v, val);
+    const double f = fq[e * nq + l];
+    const double Jxw = q[D] * jac * volume;
+#pragma unroll
+    for (int j = 0; j < NL; j++) F[j] += Jxw * f * val[j];
+  }
+#pragma unroll
+  for (int j = 0; j < NL; j++) Fe[e * NL + j] = F[j];
+}
+
+// physical quadrature points and Jxw of every element (the host evaluates a
+// user function there)
+template <int KIND>
+__global__ __launch_bounds__(256) void k_qpoints(int64_t n_elem, const double *__restrict__ coords,
+                                                 const int32_t *__restrict__ ev, const double *__restrict__ qd,
+                                                 int nq, double *__restrict__ pts, double *__restrict__ jxw) {
+  constexpr int D = KindInfo<KIND>::D;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_elem) return;
+  double v[D + 1][D];
+  load_elem<KIND>(e, coords, ev, v);
+  const double volume = tmpl_volume<D>();
+  const double jac = l2g_jacobian<D>(v);
+  for (int l = 0; l < nq; l++) {
+    const double *q = qd + l * (D + 1);
+    double x[D];
+    local_to_global<D>(q, v, x);
+#pragma unroll
+    for (int d = 0; d < D; d++) pts[(e * nq + l) * D + d] = x[d];
+    if (jxw) jxw[e * nq + l] = q[D] * jac * volume;
+  }
+}
+
+// errors against an exact solution given at the quadrature points:
+// h1 = 0: uq[e*nq+l]; h1 = 1: gq[(e*nq+l)*D + d]
+template <int KIND>
+__global__ __launch_bounds__(256) void k_element_error_qp(int64_t n_elem, const double *__restrict__ coords,
+                                                          const int32_t *__restrict__ ev,
+                                                          const int32_t *__restrict__ edof,
+                                                          const double *__restrict__ qd, int nq, int h1,
+                                                          const double *__restrict__ uh,
+                                                          const double *__restrict__ exact,
+                                                          double *__restrict__ out) {
+  constexpr int D = KindInfo<KIND>::D, NL = KindInfo<KIND>::NL;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_elem) return;
+  double v[D + 1][D];
+  load_elem<KIND>(e, coords, ev, v);
+  const double volume = tmpl_volume<D>();
+  const double jac = l2g_jacobian<D>(v);
+  double u_loc[NL];
+#pragma unroll
+  for (int j = 0; j < NL; j++) u_loc[j] = uh[edof[e * NL + j]];
+  double err = 0.0;
+  for (int l = 0; l < nq; l++) {
+    const double *q = qd + l * (D + 1);
+    double x[D];
+    local_to_global<D>(q, v, x);
+    const double Jxw = q[D] * jac * volume;
+    if (!h1) {
+      double val[NL];
+      basis_val<KIND>(x, v, val);
+      double uhv = 0.0;
+#pragma unroll
+      for (int j = 0; j < NL; j++) uhv += u_loc[j] * val[j];
+      const double df = exact[e * nq + l] - uhv;
+      err += Jxw * df * df;
+    } else {
+      double g[NL][D];
+      basis_grad<KIND>(x, v, g);
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        double gh = 0.0;
+#pragma unroll
+        for (int j = 0; j < NL; j++) gh += u_loc[j] * g[j][d];
+        const double df = gh - exact[(e * nq + l) * D + d];
+        err += Jxw * df * df;
+      }
+    }
+  }
+  out[e] = err;
+}
+
 __global__ __launch_bounds__(256) void k_gather_vec(int64_t n, int nl, const int64_t *__restrict__ adj_ptr,
                                                     const int32_t *__restrict__ adj,
                                                     const double *__restrict__ Fe, double *__restrict__ f) {
@@ -396,6 +347,44 @@ __global__ __launch_bounds__(256) void k_dirichlet_zero(int64_t n, const int64_t
     for (int64_t j = rp[i] + 1; j < rp[i + 1]; j++)
       if (mark[col[j]] != 0) val[j] = 0.0;
   }
+}
+
+// BoundaryConditionAdmin::apply with Dirichlet data g on the flagged DOFs
+// (FEMSpace.templates.h:1338-1373): a flagged row gets f_i = A_ii g_i and
+// keeps only its diagonal; every other row k subtracts, column by column in
+// ascending order (the order in which the reference visits the boundary
+// DOFs), A_ki * (A_ii g_i) / A_ii and zeroes A_ki.  Flagged rows read the
+// pre-elimination values of their own row only.
+__global__ __launch_bounds__(256) void k_dirichlet_values(int64_t n, const int64_t *__restrict__ rp,
+                                                          const int32_t *__restrict__ col,
+                                                          const uint8_t *__restrict__ flag,
+                                                          const double *__restrict__ diag0,
+                                                          const double *__restrict__ g,
+                                                          double *__restrict__ val, double *__restrict__ f) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (flag[i]) {
+    f[i] = val[rp[i]] * g[i];
+    for (int64_t j = rp[i] + 1; j < rp[i + 1]; j++) val[j] = 0.0;
+  } else {
+    double fi = f[i];
+    for (int64_t j = rp[i] + 1; j < rp[i + 1]; j++) {
+      const int32_t c = col[j];
+      if (flag[c]) {
+        const double fc = diag0[c] * g[c];
+        fi -= val[j] * fc / diag0[c];
+        val[j] = 0.0;
+      }
+    }
+    f[i] = fi;
+  }
+}
+
+// diagonal of a CSR matrix whose rows start with the diagonal
+__global__ __launch_bounds__(256) void k_csr_diag(int64_t n, const int64_t *__restrict__ rp,
+                                                  const double *__restrict__ val, double *__restrict__ d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = val[rp[i]];
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,163 @@
+// AMGSolver.h -- drop-in AMGSolver / AMGPreconditioner (include/AFEPack/
+// AMGSolver.h:66-196, src/AMGSolver.cpp:384-699 of the reference).
+//
+// The hierarchy is built by the library from the matrix's values (the
+// reference's coarsening, on the host, uploaded once); solve() and vmult()
+// run the V-cycles on the device.  Host vectors are copied to HBM and back
+// around each call; callers holding device vectors use the C ABI directly
+// (afe_amg_solve / afe_amg_pcg).
+#pragma once
+#include "FEMSpace.h"
+
+namespace AFEPack {
+
+class AMGSolver {
+public:
+  typedef SparseMatrix<double> Matrix;
+  typedef unsigned int u_int;
+
+  AMGSolver() = default;
+  AMGSolver(const Matrix &M, double tol = 1.0e-12, u_int s = 3, u_int nmpo = 50, double nmpsd = 0.382,
+            double alp = 0.25)
+      : toler_(tol), smooth_(s), nmpo_(nmpo), nmpsd_(nmpsd), alpha_(alp) {
+    reinit(M);
+  }
+  virtual ~AMGSolver() { clear(); }
+  AMGSolver(const AMGSolver &) = delete;
+  AMGSolver &operator=(const AMGSolver &) = delete;
+
+  void clear() {
+    if (g_) afe_amg_destroy(g_);
+    if (own_) afe_matrix_destroy(own_);
+    g_ = nullptr;
+    own_ = nullptr;
+    src_ = nullptr;
+  }
+  double tolerence() const { return toler_; }
+  double &tolerence() { return toler_; }
+  u_int smoothStep() const { return smooth_; }
+  u_int &smoothStep() { return smooth_; }
+  bool &isSolveMostProjectExactly() { return exact_coarse_; }
+  bool isSolveMostProjectExactly() const { return exact_coarse_; }
+
+  // AMGSolver::reinit: full coarsening (src/AMGSolver.cpp:384-438)
+  void reinit(const Matrix &M) { build(M, 0); }
+  // AMGSolver::lazyReinit (src/AMGSolver.cpp:442-545): the first call builds
+  // the lazy hierarchy, later calls with a matrix of the same pattern keep P
+  // and recompute the Galerkin products
+  void lazyReinit(const Matrix &M) {
+    if (g_ && lazy_ && src_ == &M) {
+      M.push_to_device();
+      SparseMatrix<double>::check(afe_amg_lazy_reinit(g_, device_of(M)));
+      return;
+    }
+    build(M, 1);
+  }
+  // AMGSolver::solve (src/AMGSolver.cpp:549-639): exactly step+1 V(s,s)
+  // cycles, mode 0 reports the final residual on stderr
+  void solve(Vector<double> &x, const Vector<double> &r, double tol = 0.0, size_t step = 20, int mode = 0) const {
+    prepare(x, r);
+    detail::DeviceVector dx(x.size()), dr(r.size());
+    dx.upload(x.data());
+    dr.upload(r.data());
+    afe_solve_stats st;
+    detail::check(afe_amg_solve(g_, dx.p, dr.p, tol == 0.0 ? toler_ : tol, (int64_t)step, mode, 0.0, &st));
+    dx.download(x.data());
+  }
+  // FP64 conjugate gradients preconditioned by one symmetric V-cycle (FP32
+  // or FP64) per iteration; returns the iteration count (no counterpart in
+  // the reference, which pairs AMGPreconditioner with an external CG)
+  int pcg(Vector<double> &x, const Vector<double> &b, double rtol, int max_it, bool fp32 = true) const {
+    prepare(x, b);
+    detail::DeviceVector dx(x.size()), db(b.size());
+    dx.upload(x.data());
+    db.upload(b.data());
+    afe_solve_stats st;
+    detail::check(afe_amg_pcg(g_, dx.p, db.p, rtol, max_it, fp32 ? AFE_PREC_FP32 : AFE_PREC_FP64, &st));
+    dx.download(x.data());
+    return st.iterations;
+  }
+  afe_amg *handle() const { return g_; }
+
+protected:
+  void prepare(const Vector<double> &x, const Vector<double> &r) const {
+    if (!g_) throw std::runtime_error("AMGSolver used before reinit");
+    if (!exact_coarse_) throw std::runtime_error("only the exact coarsest solve is supported on this path");
+    if (x.size() != r.size() || x.size() != n_) throw std::runtime_error("AMGSolver: size mismatch");
+    if (smooth_ != smooth_set_) {
+      detail::check(afe_amg_set_smooth(g_, (int32_t)smooth_));
+      smooth_set_ = smooth_;
+    }
+  }
+  afe_matrix *device_of(const Matrix &M) {
+    if (M.device_matrix()) return M.device_matrix();
+    // a host-assembled matrix: upload its CSR
+    const auto &pat = M.get_sparsity_pattern();
+    std::vector<int64_t> rp(pat.row_ptr().begin(), pat.row_ptr().end()),
+        col(pat.col_indices().begin(), pat.col_indices().end());
+    if (own_) afe_matrix_destroy(own_);
+    own_ = nullptr;
+    detail::check(afe_matrix_create_csr(detail::device(), (int64_t)M.m(), rp.data(), col.data(), M.values().data(), &own_));
+    return own_;
+  }
+  void build(const Matrix &M, int lazy) {
+    clear();
+    M.push_to_device();
+    afe_matrix *a = device_of(M);
+    afe_amg_params p;
+    p.alpha = alpha_;
+    p.nmpo = nmpo_;
+    p.nmpsd = nmpsd_;
+    p.lazy = lazy;
+    p.smooth = (int32_t)smooth_;
+    p.build_fp32 = 1;
+    detail::check(afe_amg_create(a, &p, &g_));
+    lazy_ = lazy;
+    src_ = &M;
+    n_ = M.m();
+    smooth_set_ = smooth_;
+  }
+
+  afe_amg *g_ = nullptr;
+  afe_matrix *own_ = nullptr;
+  const Matrix *src_ = nullptr;
+  size_t n_ = 0;
+  int lazy_ = 0;
+  double toler_ = 1.0e-12;
+  u_int smooth_ = 3, nmpo_ = 50;
+  mutable u_int smooth_set_ = 3;
+  double nmpsd_ = 0.382, alpha_ = 0.25;
+  bool exact_coarse_ = true;
+};
+
+// AMGSolver used as a preconditioner: vmult = iterate_step cycles from the
+// given x in precondition mode (AMGSolver.h:171-196)
+class AMGPreconditioner : public AMGSolver {
+public:
+  AMGPreconditioner() = default;
+  AMGPreconditioner(const Matrix &M, u_int is = 5, u_int s = 3) : iterate_step_(is) {
+    smoothStep() = s;
+    lazyReinit(M);
+  }
+  void reinit(const Matrix &M, u_int is = 5) {
+    AMGSolver::lazyReinit(M);
+    iterate_step_ = is;
+  }
+  u_int &iterateStep() { return iterate_step_; }
+  const u_int &iterateStep() const { return iterate_step_; }
+  void vmult(Vector<double> &x, const Vector<double> &b) const {
+    prepare(x, b);
+    detail::DeviceVector dx(x.size()), db(b.size());
+    dx.upload(x.data());
+    db.upload(b.data());
+    detail::check(afe_amg_precondition(g_, dx.p, db.p, (int32_t)iterate_step_));
+    dx.download(x.data());
+  }
+
+private:
+  u_int iterate_step_ = 5;
+};
+
+} // namespace AFEPack
+
+using namespace AFEPack;

@@ -1,0 +1,175 @@
+// BilinearOperator.h -- drop-in BilinearOperator / StiffMatrix / MassMatrix
+// (include/AFEPack/BilinearOperator.h:76-224 and
+// BilinearOperator.templates.h:55-255,300-406 of the reference).
+//
+// build() = buildSparsityPattern() + buildSparseMatrix().  The pattern is the
+// one afe_space_create computed (first-touch numbering, diagonal first, rest
+// ascending).  buildSparseMatrix() assembles on the device:
+//   * StiffMatrix and MassMatrix objects run the device element kernels
+//     (afe_assemble, op STIFF / MASS);
+//   * a derived class -- the reference's extension point, which overrides
+//     getElementMatrix (e.g. a variable-coefficient stiffness) -- is
+//     evaluated on the host element by element, as the reference does, and
+//     the element matrices are gathered into the CSR values on the device
+//     (afe_assemble_elements).
+// The matrix keeps a device copy; the host values are copied back lazily on
+// the first host access.
+#pragma once
+#include <typeinfo>
+
+#include "FEMSpace.h"
+
+namespace AFEPack {
+
+template <int DIM, class value_type0, class value_type1, int DOW = DIM, int TDIM0 = DIM, int TDIM1 = DIM>
+class BilinearOperator : public SparseMatrix<double> {
+public:
+  using space0_t = FEMSpace<value_type0, DIM, DOW, TDIM0>;
+  using space1_t = FEMSpace<value_type1, DIM, DOW, TDIM1>;
+  using element0_t = Element<value_type0, DIM, DOW, TDIM0>;
+  using element1_t = Element<value_type1, DIM, DOW, TDIM1>;
+
+  BilinearOperator() = default;
+  BilinearOperator(space0_t &s0, space1_t &s1) { reinit(s0, s1); }
+  virtual ~BilinearOperator() = default;
+  void reinit(space0_t &s0, space1_t &s1) {
+    if ((void *)&s0 != (void *)&s1)
+      throw std::runtime_error("BilinearOperator: two different FE spaces are not supported on this path");
+    sp0_ = &s0;
+    sp1_ = &s1;
+  }
+  const space0_t &FEMSpace0() const { return *sp0_; }
+  const space1_t &FEMSpace1() const { return *sp1_; }
+  space0_t *FEMSpace0() { return sp0_; }
+  space1_t *FEMSpace1() { return sp1_; }
+  const SparsityPattern &getSparsityPattern() const { return pattern_store_; }
+  SparsityPattern &getSparsityPattern() { return pattern_store_; }
+
+  const DenseMatrix<double> &elementMatrix() const { return element_matrix_; }
+  DenseMatrix<double> &elementMatrix() { return element_matrix_; }
+  double elementMatrix(int i, int j) const { return element_matrix_(i, j); }
+  double &elementMatrix(int i, int j) { return element_matrix_(i, j); }
+  const int &algebricAccuracy() const { return acc_; }
+  int &algebricAccuracy() { return acc_; }
+
+  virtual void getElementMatrix(const element0_t &, const element1_t &,
+                                const typename ActiveElementPairIterator<DIM>::State state =
+                                    ActiveElementPairIterator<DIM>::EQUAL) = 0;
+  virtual void build() {
+    buildSparsityPattern();
+    buildSparseMatrix();
+  }
+  // the CSR pattern of the space (BilinearOperator.templates.h:64-172)
+  virtual void buildSparsityPattern() {
+    afe_space *h = sp0_->handle();
+    if (!h) throw std::runtime_error("buildDof() must run before build()");
+    int64_t nd, nnz;
+    int32_t nl;
+    check(afe_space_info(h, &nd, &nnz, &nl));
+    std::vector<int64_t> rp(nd + 1), col(nnz);
+    check(afe_space_pattern(h, rp.data(), col.data()));
+    pattern_store_.assign_compressed((size_t)nd, (size_t)nd, std::vector<size_t>(rp.begin(), rp.end()),
+                                     std::vector<size_t>(col.begin(), col.end()));
+  }
+  // values on the device (BilinearOperator.templates.h:176-255)
+  virtual void buildSparseMatrix() {
+    SparseMatrix<double>::reinit(pattern_store_);
+    afe_matrix *a = nullptr;
+    check(afe_matrix_create(sp0_->handle(), &a));
+    bind_device(a, true);
+    const int n_elem = sp0_->n_element();
+    // the device element kernels serve exactly StiffMatrix / MassMatrix;
+    // a derived class (which may override getElementMatrix) is evaluated on
+    // the host element by element
+    const int op = device_operator();
+    const bool builtin = op >= 0;
+    if (builtin) {
+      check(afe_assemble(a, op, acc_, nullptr));
+    } else {
+      const int nl = n_elem ? (int)sp0_->element(0).dof().size() : 0;
+      std::vector<double> em((size_t)n_elem * nl * nl);
+      for (int e = 0; e < n_elem; e++) {
+        element_matrix_.reinit(nl, nl);
+        getElementMatrix(sp0_->element(e), sp1_->element(e));
+        std::copy(element_matrix_.data(), element_matrix_.data() + (size_t)nl * nl, em.begin() + (size_t)e * nl * nl);
+      }
+      check(afe_assemble_elements(a, em.data()));
+    }
+    mark_device_newer();
+  }
+
+protected:
+  // AFE_OP_* of a built-in element matrix, -1 for user operators
+  virtual int device_operator() const { return -1; }
+  space0_t *sp0_ = nullptr;
+  space1_t *sp1_ = nullptr;
+
+private:
+  SparsityPattern pattern_store_;
+  DenseMatrix<double> element_matrix_;
+  int acc_ = 3;
+};
+
+template <int DIM, class value_type, int DOW = DIM, int TDIM = DIM>
+class MassMatrix : public BilinearOperator<DIM, value_type, value_type, DOW, TDIM, TDIM> {
+  using base = BilinearOperator<DIM, value_type, value_type, DOW, TDIM, TDIM>;
+
+public:
+  MassMatrix() = default;
+  explicit MassMatrix(FEMSpace<value_type, DIM, DOW, TDIM> &sp) : base(sp, sp) {}
+  void reinit(FEMSpace<value_type, DIM, DOW, TDIM> &sp) { base::reinit(sp, sp); }
+  // host evaluation (BilinearOperator.templates.h:300-330)
+  virtual void getElementMatrix(const Element<value_type, DIM, DOW, TDIM> &e0,
+                                const Element<value_type, DIM, DOW, TDIM> &e1,
+                                const typename ActiveElementPairIterator<DIM>::State =
+                                    ActiveElementPairIterator<DIM>::EQUAL) {
+    const int n0 = (int)e0.dof().size(), n1 = (int)e1.dof().size();
+    const double volume = e0.templateElement().volume();
+    const auto qi = e0.findQuadratureInfo(this->algebricAccuracy());
+    const auto jac = e0.local_to_global_jacobian(qi.quadraturePoint());
+    const auto q = e0.local_to_global(qi.quadraturePoint());
+    const auto bv = e0.basis_function_value(q);
+    for (int l = 0; l < qi.n_quadraturePoint(); l++) {
+      const double Jxw = qi.weight(l) * jac[l] * volume;
+      for (int j = 0; j < n0; j++)
+        for (int k = 0; k < n1; k++) this->elementMatrix(j, k) += Jxw * bv[j][l] * bv[k][l];
+    }
+  }
+
+protected:
+  int device_operator() const override { return typeid(*this) == typeid(MassMatrix) ? AFE_OP_MASS : -1; }
+};
+
+template <int DIM, class value_type, int DOW = DIM, int TDIM = DIM>
+class StiffMatrix : public BilinearOperator<DIM, value_type, value_type, DOW, TDIM, TDIM> {
+  using base = BilinearOperator<DIM, value_type, value_type, DOW, TDIM, TDIM>;
+
+public:
+  StiffMatrix() = default;
+  explicit StiffMatrix(FEMSpace<value_type, DIM, DOW, TDIM> &sp) : base(sp, sp) {}
+  void reinit(FEMSpace<value_type, DIM, DOW, TDIM> &sp) { base::reinit(sp, sp); }
+  // host evaluation (BilinearOperator.templates.h:374-403)
+  virtual void getElementMatrix(const Element<value_type, DIM, DOW, TDIM> &e0,
+                                const Element<value_type, DIM, DOW, TDIM> &e1,
+                                const typename ActiveElementPairIterator<DIM>::State =
+                                    ActiveElementPairIterator<DIM>::EQUAL) {
+    const int n0 = (int)e0.dof().size(), n1 = (int)e1.dof().size();
+    const double volume = e0.templateElement().volume();
+    const auto qi = e0.findQuadratureInfo(this->algebricAccuracy());
+    const auto jac = e0.local_to_global_jacobian(qi.quadraturePoint());
+    const auto q = e0.local_to_global(qi.quadraturePoint());
+    const auto bg = e0.basis_function_gradient(q);
+    for (int l = 0; l < qi.n_quadraturePoint(); l++) {
+      const double Jxw = qi.weight(l) * jac[l] * volume;
+      for (int j = 0; j < n0; j++)
+        for (int k = 0; k < n1; k++) this->elementMatrix(j, k) += Jxw * innerProduct(bg[j][l], bg[k][l]);
+    }
+  }
+
+protected:
+  int device_operator() const override { return typeid(*this) == typeid(StiffMatrix) ? AFE_OP_STIFF : -1; }
+};
+
+} // namespace AFEPack
+
+using namespace AFEPack;

@@ -183,6 +183,27 @@ int afe_rhs(afe_space *s, int32_t acc, int32_t fkind, const afe_field *coef,
  * (FEMSpace.templates.h:1338-1373).  d_u is not touched (as the reference). */
 int afe_apply_dirichlet(afe_matrix *a, double *d_u, double *d_f);
 
+/* Callbacks of the reference (f, boundary data, exact solutions) are host C
+ * functions; these entry points take their values and keep the assembly on
+ * the device.  afe_space_qpoints: physical quadrature points (n_elem x n_qp x
+ * dim) and Jxw (n_elem x n_qp) of rule acc; either output may be NULL, *n_qp
+ * receives the rule size. */
+int afe_space_qpoints(afe_space *s, int32_t acc, int64_t *n_qp, double *h_pts,
+                      double *h_jxw);
+/* Operator::L2Discretize(f, space, rhs, acc) (Operator.discretize.templates.h:
+ * 121-152) with f given at the quadrature points (n_elem x n_qp) */
+int afe_rhs_qp(afe_space *s, int32_t acc, const double *h_fq, double *d_f);
+/* BoundaryConditionAdmin::apply(A, u, f) (FEMSpace.templates.h:1338-1373) for
+ * the DOFs whose boundary mark is in marks[0..n_marks), Dirichlet data h_g
+ * (one value per DOF, used on those DOFs); same elimination arithmetic */
+int afe_apply_dirichlet_values(afe_matrix *a, const int32_t *marks,
+                               int32_t n_marks, const double *h_g, double *d_u,
+                               double *d_f);
+/* Functional::L2Error (h1 = 0, exact values n_elem x n_qp) and H1SemiError
+ * (h1 = 1, exact gradients n_elem x n_qp x dim) */
+int afe_error_qp(afe_space *s, const double *d_uh, int32_t acc, int32_t h1,
+                 const double *h_exact, double *err);
+
 /* ---- error functionals -------------------------------------------------- */
 /* Functional::L2Error / H1SemiError (Functional.templates.h:296-317,399-426)
  * of the FE function d_uh against u = prod sin(pi x_d). */

@@ -851,6 +851,31 @@ void or_apply_dirichlet(const or_space *s, or_mat *a, double *u, double *f) {
   }
 }
 
+/* BoundaryConditionAdmin::apply (FEMSpace.templates.h:1338-1373), preserve_
+ * symmetry = true, for the DOFs whose mark is one of marks[0..n_marks) with
+ * Dirichlet data g (one value per DOF): f_i = A_ii g_i, then the same
+ * elimination as above.  u is not touched. */
+void or_apply_dirichlet_values(const or_space *s, or_mat *a, const int32_t *marks, int n_marks,
+                               const double *g, double *f) {
+  const int64_t *rp = a->sp->rp, *col = a->sp->col;
+  for (int64_t i = 0; i < s->n_dof; i++) {
+    int hit = 0;
+    for (int k = 0; k < n_marks; k++) hit |= (marks[k] == s->dof_mark[i]);
+    if (!hit) continue;
+    f[i] = a->val[rp[i]] * g[i];
+    for (int64_t j = rp[i] + 1; j < rp[i + 1]; j++) a->val[j] = 0.0;
+    for (int64_t j = rp[i] + 1; j < rp[i + 1]; j++) {
+      int64_t k = col[j];
+      for (int64_t p = rp[k] + 1; p < rp[k + 1]; p++)
+        if (col[p] == i) {
+          f[k] -= a->val[p] * f[i] / a->val[rp[i]];
+          a->val[p] = 0.;
+          break;
+        }
+    }
+  }
+}
+
 /* Functional::L2Error (Functional.templates.h:296-317) and
  * Functional::H1SemiError (:399-426) against u = prod sin(pi x_d) (or 0). */
 static double fe_error(const or_space *s, const double *uh, int acc, int ukind,

@@ -317,6 +317,51 @@ def test_dirichlet_elimination_solves_constrained_system():
     assert np.allclose(D1, D1.T)
 
 
+@pytest.mark.parametrize("kind", [orc.P1_TRI, orc.P2_TRI, orc.P1_TET])
+def test_dirichlet_values_reproduce_linear_harmonic(kind):
+    # Laplace problem with linear Dirichlet data: the discrete solution is the
+    # linear function itself at every DOF (P1/P2 contain it, A of a linear
+    # function is zero row-wise on interior DOFs)
+    n = 3 if kind == orc.P1_TET else 5
+    c, ev, vm = mesh(kind, n, 0.05 if kind == orc.P1_TET else 0.2, 11)
+    s = orc.Space(kind, c, ev, vm)
+    A = s.build(orc.OP_STIFF, 4)
+    _, dm, dp = s.dofs()
+    dim = 3 if kind == orc.P1_TET else 2
+    P = dp.reshape(-1, dim)
+    coef = np.array([1.5, -3.0, 0.75][:dim])
+    g = 0.25 + P @ coef
+    f = np.zeros(s.n_dof)
+    s.apply_dirichlet_values(A, [1], g, f)
+    D1 = A.dense()
+    x = np.linalg.solve(D1, f)
+    assert np.abs(x - g).max() < 1e-11
+    assert np.allclose(D1, D1.T)
+
+
+def test_dirichlet_values_partial_marks_constrained_system():
+    # only the DOFs marked 2 (on x = 0) are constrained, to nonzero data; the
+    # result must be the constrained original system u_B = g_B,
+    # A_II u_I = f_I - A_IB g_B
+    c, ev, vm = mesh(orc.P2_TRI, 4, 0.2, 5)
+    cc = c.reshape(-1, 2)
+    vm = vm.copy()
+    vm[(vm != 0) & (cc[:, 0] == 0.0)] = 2
+    s = orc.Space(orc.P2_TRI, c, ev, vm)
+    A = s.build(orc.OP_STIFF, 6)
+    f = s.rhs(6, orc.F_POISSON)
+    D0, f0 = A.dense(), f.copy()
+    _, dm, dp = s.dofs()
+    g = np.sin(3.0 * dp.reshape(-1, 2)[:, 1]) + 2.0
+    s.apply_dirichlet_values(A, [2], g, f)
+    B = dm == 2
+    I = ~B
+    assert B.sum() > 0 and (dm == 1).sum() > 0
+    x1 = np.linalg.solve(A.dense(), f)
+    xi = np.linalg.solve(D0[np.ix_(I, I)], f0[I] - D0[np.ix_(I, B)] @ g[B])
+    assert np.allclose(x1[B], g[B], rtol=1e-12) and np.allclose(x1[I], xi, rtol=1e-10)
+
+
 # ----------------------------------------------------------------------- AMG
 def _csr_from_dense(D):
     n = D.shape[0]
