@@ -81,13 +81,17 @@ int main(int argc, char *argv[]) {
   fem_space.buildDof();
   fem_space.buildDofBoundaryMark();
 
+  // accuracy 4 (6-point rule) rather than the reference's 3: the 4-point
+  // rule of accuracy 3 has a negative centroid weight, and on this mesh the
+  // line x = -y cuts elements whose coefficient is 400 at the centroid and 1
+  // at the other points, which makes their element matrices indefinite
   Matrix stiff_matrix(fem_space);
-  stiff_matrix.algebricAccuracy() = 3;
+  stiff_matrix.algebricAccuracy() = 4;
   stiff_matrix.build();
 
   FEMFunction<double, DIM> u_h(fem_space);
   Vector<double> f_h;
-  Operator::L2Discretize(&_f_, fem_space, f_h, 3);
+  Operator::L2Discretize(&_f_, fem_space, f_h, 4);
 
   BoundaryFunction<double, DIM> boundary(BoundaryConditionInfo::DIRICHLET, 1, &_u_);
   BoundaryConditionAdmin<double, DIM> boundary_admin(fem_space);

@@ -62,8 +62,9 @@ def test_qp_entry_points_match_oracle(ctx, kind, n, jitter):
     pts, jxw = sp.qpoints(acc)
     _, qw = orc.quad(sp.dim, acc)
     assert pts.shape == (sp.n_elem, len(qw), sp.dim)
-    # Jxw integrates 1 over the unit square / cube
-    assert np.all(jxw > 0) and abs(jxw.sum() - 1.0) <= 1e-12
+    # Jxw integrates 1 over the unit square / cube (the 11-point tetrahedron
+    # rule has a negative centroid weight, so the sign is not checked)
+    assert abs(jxw.sum() - 1.0) <= 1e-12
     # f at the points, as a host callback would give it: F_POISSON's form
     fq = sp.dim * np.pi ** 2 * _sinprod(pts)
     f = sp.rhs_qp(acc, fq).numpy()
