@@ -84,6 +84,7 @@ SIGS = {
     "afe_amg_set_smooth": (C.c_int, [P, I32]),
     "afe_set_tuning": (C.c_int, [I32, I32, I64]),
     "afe_amg_kernel_timing": (C.c_int, [P, I32, I32, PF64, PF64]),
+    "afe_copy_kernel_timing": (C.c_int, [P, I64, I32, PF64]),
     "afe_amg_device_info": (C.c_int, [P, I32, PI64, PI32, PI64, PI32]),
 }
 

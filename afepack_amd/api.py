@@ -123,6 +123,12 @@ class Context:
         _chk(_lib().afe_ctx_device_bytes(self.h, C.byref(b)))
         return b.value
 
+    def copy_kernel_timing(self, nbytes, reps=20):
+        """Average ms of a plain streaming copy of nbytes (read + write)."""
+        ms = C.c_double()
+        _chk(_lib().afe_copy_kernel_timing(self.h, int(nbytes), reps, C.byref(ms)))
+        return ms.value
+
     def synchronize(self):
         _chk(_lib().afe_ctx_synchronize(self.h))
 

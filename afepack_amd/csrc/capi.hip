@@ -1618,6 +1618,31 @@ extern "C" int afe_amg_kernel_timing(afe_amg *g, int32_t which, int32_t reps, do
   API_END
 }
 
+// plain copy of `bytes` (rounded down to 16 B) between two scratch buffers,
+// `reps` timed launches on the context stream
+extern "C" int afe_copy_kernel_timing(afe_ctx *ctx, int64_t bytes, int32_t reps, double *avg_ms) {
+  API_BEGIN
+  if (!ctx || bytes < 16 || reps < 1 || !avg_ms) return fail(AFE_E_ARG, "bad arguments");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  const int64_t n2 = bytes / 16;
+  DevBuf<double> a, b;
+  a.alloc(ctx, (size_t)n2 * 2);
+  b.alloc(ctx, (size_t)n2 * 2);
+  hipStream_t s = ctx->stream;
+  k_fill<double><<<nblk(n2 * 2), 256, 0, s>>>(n2 * 2, a.p, 1.0);
+  const int grid = 8192;
+  k_stream_copy<<<grid, 256, 0, s>>>(n2, (const dvec2 *)a.p, (dvec2 *)b.p);
+  HIPCHK(hipEventRecord(ctx->ev0, s));
+  for (int r = 0; r < reps; r++) k_stream_copy<<<grid, 256, 0, s>>>(n2, (const dvec2 *)a.p, (dvec2 *)b.p);
+  HIPCHK(hipEventRecord(ctx->ev1, s));
+  HIPCHK(hipEventSynchronize(ctx->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  *avg_ms = ms / reps;
+  API_END
+}
+
 // the solver's tunables after construction (smoothing sweeps per side)
 extern "C" int afe_amg_set_smooth(afe_amg *g, int32_t smooth) {
   if (!g || smooth < 0 || smooth > 16) return fail(AFE_E_ARG, "bad smoothing count");

@@ -768,6 +768,16 @@ __global__ __launch_bounds__(BLK) void k_coarse_gemv(int n, const double *__rest
 // ---------------------------------------------------------------------------
 // vector kernels (level-0 numbering after the colour permutation)
 // ---------------------------------------------------------------------------
+// plain streaming copy (16-byte vectors, grid-stride): the calibration
+// kernel of the byte counters and of the achievable HBM bandwidth
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(256) void k_stream_copy(int64_t n4, const dvec2 *__restrict__ src,
+                                                     dvec2 *__restrict__ dst) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
+    dst[i] = __builtin_nontemporal_load(&src[i]);
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_fill(int64_t n, T *__restrict__ x, T v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

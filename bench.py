@@ -88,6 +88,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="build the C2 system and hierarchy, then only time the level-0 kernels and a "
+                         "plain copy (for profiler runs); prints one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this path runs on one GPU; --gpus must be 1"}))
@@ -126,6 +129,18 @@ def main():
         t0 = time.perf_counter()
         st = gg.pcg(x, b, rtol=args.rtol, max_it=1000, prec=prec)
         return time.perf_counter() - t0, st
+
+    if args.kernels_only:
+        gg, t_b = build(args.nmpo or 2048, args.nmpsd if args.nmpo else 0.0)
+        out = {"workload": f"2D P2 Poisson {n}x{n}, {sp.n_dof} DOFs", "kernels": {}}
+        for which, name in ((0, "spmv_fp64"), (1, "gs_sweep_fp32"), (3, "residual_fp32")):
+            ms_k, by = gg.kernel_timing(which, reps=10)
+            out["kernels"][name] = {"us": 1e3 * ms_k, "algorithmic_bytes": by, "gbs": by / ms_k / 1e6}
+        cb = 1 << 30
+        ms_c = ctx.copy_kernel_timing(cb, reps=10)
+        out["copy"] = {"bytes_read": cb, "bytes_written": cb, "us": 1e3 * ms_c, "gbs": 2 * cb / ms_c / 1e6}
+        print(json.dumps(out), flush=True)
+        return
 
     # autotune 1: launch shape of the sliced-ELL kernels, by live HIP-event
     # timing of the level-0 FP64 SpMV and FP32 Gauss-Seidel sweep
@@ -221,8 +236,12 @@ def main():
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
 
-    # discretisation error and agreement of the two precisions
-    e2 = sp.error_l2(x, acc) if False else None
+    # achievable streaming bandwidth next to it (plain 1 GiB copy)
+    ms_c = ctx.copy_kernel_timing(1 << 30, reps=10)
+    copy_gbs = 2.0 * (1 << 30) / ms_c / 1e6
+    log(f"copy kernel: {copy_gbs:.0f} GB/s")
+
+    # agreement of the two precisions
     diff = float(np.linalg.norm(xs - x64) / np.linalg.norm(x64))
 
     cpu = None
@@ -268,6 +287,8 @@ def main():
         "assembly_ms": 1e3 * t_asm,
         "setup_s": {"space_pattern": t_space, "amg_host_and_upload": t_amg},
         "kernels_us": {k: 1e3 * v[0] for k, v in k_ms.items()},
+        "kernels_algorithmic_gbs": {k: v[1] / v[0] / 1e6 for k, v in k_ms.items()},
+        "copy_kernel_gbs": copy_gbs,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_spmv_dot (level-0 FP64 sliced-ELL SpMV, CG q = A p)"},

@@ -267,6 +267,11 @@ int afe_amg_set_smooth(afe_amg *g, int32_t smooth);
  * 2 FP64 sweep, 3 FP32 residual.  *bytes = algorithmic bytes per launch. */
 int afe_amg_kernel_timing(afe_amg *g, int32_t which, int32_t reps,
                           double *avg_ms, double *bytes);
+/* average duration of `reps` launches of a plain streaming copy of `bytes`
+ * (read + write, 16-byte vectors): calibrates the HBM byte counters and
+ * gives the achievable copy bandwidth next to the solver kernels */
+int afe_copy_kernel_timing(afe_ctx *ctx, int64_t bytes, int32_t reps,
+                           double *avg_ms);
 /* launch tuning of the sliced-ELL kernels (process-wide, set by the
  * autotuner): xcd = 1 orders workgroups so that each XCD works on a
  * contiguous slice range; spmv_blocks = grid of the grid-stride level-0
