@@ -135,10 +135,12 @@ struct Tuning {
   int64_t stream_bytes = 128 << 20;  // matrices above this use streaming loads
 };
 static Tuning g_tune;
+static constexpr int kMaxGridBlocks = 16384; // bound of spmv_blocks (partial-sum buffer)
+static constexpr int64_t NAT_WINDOW = 256;    // length-sort window of the natural-order copy
 
 static inline unsigned sblk(int64_t nslices) {
   unsigned b = nblk(nslices, WPB);
-  return g_tune.xcd ? (b + 7) / 8 * 8 : b;
+  return g_tune.xcd == 1 ? (b + 7) / 8 * 8 : b;
 }
 
 extern "C" int afe_ctx_create(int device, afe_ctx **out) {
@@ -874,8 +876,15 @@ struct afe_amg {
   DevBuf<double> minv;
   DevBuf<LevelDev<double>> ld64;
   DevBuf<LevelDev<float>> ld32;
-  DevBuf<int32_t> perm0;
-  // CG workspace (level-0 permuted numbering)
+  DevBuf<int32_t> perm0, iperm0; // colour index -> natural row, and back
+  // level-0 FP64 matrix in the natural numbering (the CG SpMV and the true
+  // residual): rows sorted by stored length inside windows of NAT_WINDOW
+  // rows, nat_rowmap = stored row -> natural row
+  int nat_ns = 0;
+  DevBuf<SliceInfo> nat_si;
+  DevBuf<int32_t> nat_cols, nat_rowmap;
+  DevBuf<double> nat_v64, nat_diag;
+  // CG workspace (level-0 natural numbering)
   DevBuf<double> wx, wxl, wr, wp, wq, wz, wb, part;
   DevBuf<CgScalars> sc;
   CgScalars *h_sc = nullptr; // pinned
@@ -1003,6 +1012,27 @@ static void upload_hierarchy(afe_amg *g) {
     d.stream = (int64_t)sa.cols.size() * 8 > g_tune.stream_bytes;
   }
   g->perm0.upload(ctx, perm[0]);
+  g->iperm0.upload(ctx, iperm[0]);
+  {
+    const Csr &A = h.A[0];
+    std::vector<int32_t> len(A.m, 0), nat(A.m), ident(A.m);
+    for (int64_t i = 0; i < A.m; i++)
+      for (int64_t k = A.rp[i]; k < A.rp[i + 1]; k++)
+        if (A.col[k] != i && A.val[k] != 0.0) len[i]++;
+    std::iota(nat.begin(), nat.end(), 0);
+    std::iota(ident.begin(), ident.end(), 0);
+    for (int64_t w = 0; w < A.m; w += NAT_WINDOW)
+      std::stable_sort(nat.begin() + w, nat.begin() + std::min<int64_t>(A.m, w + NAT_WINDOW),
+                       [&](int32_t a, int32_t b) { return len[a] > len[b]; });
+    Sell sn;
+    build_sell(A, nat, ident, {(int32_t)A.m}, true, sn);
+    g->nat_ns = (int)sn.slices.size();
+    g->nat_si.upload(ctx, sn.slices);
+    g->nat_cols.upload(ctx, sn.cols);
+    g->nat_v64.upload(ctx, sn.vals);
+    g->nat_diag.upload(ctx, sn.diag);
+    g->nat_rowmap.upload(ctx, nat);
+  }
   // first level of the coarse tail; a large dense coarsest level is solved by
   // the multi-block gemv between the tail's down and up legs
   g->coarse_mode = h.full ? 0 : 1;
@@ -1050,7 +1080,7 @@ static void upload_hierarchy(afe_amg *g) {
   g->wq.alloc(ctx, n0);
   g->wz.alloc(ctx, n0);
   g->wb.alloc(ctx, n0);
-  g->part.alloc(ctx, 3 * 8192);
+  g->part.alloc(ctx, 3 * (size_t)kMaxGridBlocks);
   g->sc.alloc(ctx, 1);
   HIPCHK(hipStreamSynchronize(ctx->stream));
 }
@@ -1317,13 +1347,19 @@ static bool verbose() {
 }
 
 // true residual r = b - A (x + xl) in compensated arithmetic, and its norms
-static void residual_norms(afe_amg *g, const double *b, const double *x, double *r,
+// natural = true: vectors in the natural numbering (the CG), else in the
+// level-0 colour order (the stationary solver)
+static void residual_norms(afe_amg *g, bool natural, const double *b, const double *x, double *r,
                            const double *xl = nullptr) {
   DevLevel &d = g->lv[0];
   hipStream_t st = g->ctx->stream;
   const int G = g_tune.spmv_blocks;
-  k_residual_norms_dd<<<G, BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, d.v64.p, d.diag64.p, b, x, xl, r, g->part.p,
-                                          g_tune.xcd);
+  if (natural)
+    k_residual_norms_dd<<<G, BLK, 0, st>>>(g->nat_si.p, g->nat_ns, g->nat_cols.p, g->nat_v64.p, g->nat_diag.p,
+                                            g->nat_rowmap.p, b, x, xl, r, g->part.p, g_tune.xcd);
+  else
+    k_residual_norms_dd<<<G, BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, d.v64.p, d.diag64.p, nullptr, b, x, xl, r,
+                                            g->part.p, g_tune.xcd);
   k_finalize<<<1, 256, 0, st>>>(g->part.p, G, 3, &g->sc.p->rr);
 }
 
@@ -1346,7 +1382,7 @@ extern "C" int afe_amg_solve(afe_amg *g, double *d_x, const double *d_r, double 
   HIPCHK(hipEventRecord(ctx->ev0, s));
   k_permute_in<double, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, d_r, d0.b64.p);
   k_permute_in<double, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, d_x, d0.x64.p);
-  residual_norms(g, d0.b64.p, d0.x64.p, nullptr);
+  residual_norms(g, false, d0.b64.p, d0.x64.p, nullptr);
   read_scalars(g);
   const double init_l1 = g->h_sc->l1, bb = g->h_sc->bb;
   double rr = g->h_sc->rr, l1 = init_l1;
@@ -1354,7 +1390,7 @@ extern "C" int afe_amg_solve(afe_amg *g, double *d_x, const double *d_r, double 
   bool conv = false;
   while (true) {
     launch_graph_or_issue(g, 0, false, g->prm.smooth);
-    residual_norms(g, d0.b64.p, d0.x64.p, nullptr);
+    residual_norms(g, false, d0.b64.p, d0.x64.p, nullptr);
     read_scalars(g);
     rr = g->h_sc->rr;
     l1 = g->h_sc->l1;
@@ -1392,15 +1428,16 @@ extern "C" int afe_amg_precondition(afe_amg *g, double *d_x, const double *d_b, 
   return afe_amg_solve(g, d_x, d_b, 0.0, iterate_step, 1, 0.0, &st);
 }
 
-// z = M^{-1} r with one symmetric V-cycle from zero, permuted numbering
+// z = M^{-1} r with one symmetric V-cycle from zero; r and z in the natural
+// numbering, the V-cycle in the level-0 colour order
 template <class T> static void precondition_perm(afe_amg *g, const double *r, bool first) {
   DevLevel &d0 = g->lv[0];
   hipStream_t s = g->ctx->stream;
   const int64_t n = d0.n;
-  k_prep_vcycle<T><<<nblk(n), 256, 0, s>>>(n, r, Prec<T>::b(d0), Prec<T>::x(d0));
+  k_prep_vcycle<T><<<nblk(n), 256, 0, s>>>(n, r, Prec<T>::b(d0), Prec<T>::x(d0), g->iperm0.p);
   launch_graph_or_issue(g, std::is_same<T, float>::value ? 1 : 0, true, g->prm.smooth);
   if (first) k_fill<double><<<nblk(n), 256, 0, s>>>(n, g->wz.p, 0.0);
-  k_take_z<T><<<RED_BLOCKS, 256, 0, s>>>(n, Prec<T>::x(d0), r, g->wz.p, g->part.p);
+  k_take_z<T><<<RED_BLOCKS, 256, 0, s>>>(n, Prec<T>::x(d0), r, g->wz.p, g->iperm0.p, g->part.p);
   k_finalize<<<1, 256, 0, s>>>(g->part.p, RED_BLOCKS, 2, &g->sc.p->rz_new);
   k_cg_beta<<<1, 64, 0, s>>>(g->sc.p, first ? 1 : 0);
   k_p_update<<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wz.p, g->wp.p);
@@ -1412,17 +1449,18 @@ template <class T> static void issue_cg_iteration(afe_amg *g) {
   DevLevel &d0 = g->lv[0];
   hipStream_t s = g->ctx->stream;
   const int64_t n = d0.n;
-  k_spmv_dot<<<g_tune.spmv_blocks, BLK, 0, s>>>(d0.si.p, d0.ns, d0.cols.p, d0.v64.p, d0.diag64.p, g->wp.p,
-                                                 g->wq.p, g->part.p, g_tune.xcd);
+  k_spmv_dot<<<g_tune.spmv_blocks, BLK, 0, s>>>(g->nat_si.p, g->nat_ns, g->nat_cols.p, g->nat_v64.p,
+                                                 g->nat_diag.p, g->nat_rowmap.p, g->wp.p, g->wq.p, g->part.p,
+                                                 g_tune.xcd);
   k_finalize<<<1, 256, 0, s>>>(g->part.p, g_tune.spmv_blocks, 1, &g->sc.p->pq);
   k_cg_update<T><<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wx.p, g->wxl.p, g->wr.p, g->wp.p, g->wq.p,
-                                             Prec<T>::b(d0), Prec<T>::x(d0), g->part.p);
+                                             Prec<T>::b(d0), Prec<T>::x(d0), g->iperm0.p, g->part.p);
   k_finalize<<<1, 256, 0, s>>>(g->part.p, RED_BLOCKS, 1, &g->sc.p->rr);
   HIPCHK(hipMemcpyAsync(g->h_sc, g->sc.p, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
   const int prec = std::is_same<T, float>::value ? 1 : 0;
   // V-cycle issued inline (nested graph capture is not used)
   issue_vcycle<T>(g, g->prm.smooth, true, s);
-  k_take_z<T><<<RED_BLOCKS, 256, 0, s>>>(n, Prec<T>::x(d0), g->wr.p, g->wz.p, g->part.p);
+  k_take_z<T><<<RED_BLOCKS, 256, 0, s>>>(n, Prec<T>::x(d0), g->wr.p, g->wz.p, g->iperm0.p, g->part.p);
   k_finalize<<<1, 256, 0, s>>>(g->part.p, RED_BLOCKS, 2, &g->sc.p->rz_new);
   k_cg_beta<<<1, 64, 0, s>>>(g->sc.p, 0);
   k_p_update<<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wz.p, g->wp.p);
@@ -1460,10 +1498,10 @@ static int pcg_impl(afe_amg *g, double *d_x, const double *d_b, double rtol, int
   DevLevel &d0 = g->lv[0];
   const int64_t n = d0.n;
   HIPCHK(hipEventRecord(ctx->ev0, s));
-  k_permute_in<double, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, d_b, g->wb.p);
-  k_permute_in<double, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, d_x, g->wx.p);
+  HIPCHK(hipMemcpyAsync(g->wb.p, d_b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(g->wx.p, d_x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   k_fill<double><<<nblk(n), 256, 0, s>>>(n, g->wxl.p, 0.0);
-  residual_norms(g, g->wb.p, g->wx.p, g->wr.p, g->wxl.p);
+  residual_norms(g, true, g->wb.p, g->wx.p, g->wr.p, g->wxl.p);
   read_scalars(g);
   const double bb = g->h_sc->bb;
   const double init_l1 = g->h_sc->l1;
@@ -1492,7 +1530,7 @@ static int pcg_impl(afe_amg *g, double *d_x, const double *d_b, double rtol, int
     }
     // true residual of the pair x_hi + x_lo; restart from it when the
     // recurrence drifted
-    residual_norms(g, g->wb.p, g->wx.p, g->wr.p, g->wxl.p);
+    residual_norms(g, true, g->wb.p, g->wx.p, g->wr.p, g->wxl.p);
     read_scalars(g);
     rr = g->h_sc->rr;
     conv = rr <= pthr;
@@ -1500,7 +1538,7 @@ static int pcg_impl(afe_amg *g, double *d_x, const double *d_b, double rtol, int
   }
   // deliver round(x_hi + x_lo) and check that vector's true residual
   k_dd_round<<<nblk(n), 256, 0, s>>>(n, g->wx.p, g->wxl.p);
-  residual_norms(g, g->wb.p, g->wx.p, g->wr.p, nullptr);
+  residual_norms(g, true, g->wb.p, g->wx.p, g->wr.p, nullptr);
   read_scalars(g);
   rr = g->h_sc->rr;
   l1 = g->h_sc->l1;
@@ -1510,7 +1548,7 @@ static int pcg_impl(afe_amg *g, double *d_x, const double *d_b, double rtol, int
   if (conv || it >= max_it || restarts > 20) break;
   pthr *= 0.25; // rounding floor reached: tighten the pair and continue
   }
-  k_permute_out<double, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, g->wx.p, d_x);
+  HIPCHK(hipMemcpyAsync(d_x, g->wx.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   HIPCHK(hipEventRecord(ctx->ev1, s));
   HIPCHK(hipEventSynchronize(ctx->ev1));
   float ms = 0;
@@ -1550,11 +1588,11 @@ extern "C" int afe_amg_vcycle(afe_amg *g, double *d_z, const double *d_r, int32_
   const int64_t n = d0.n;
   k_permute_in<double, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, d_r, g->wr.p);
   if (prec == AFE_PREC_FP32) {
-    k_prep_vcycle<float><<<nblk(n), 256, 0, s>>>(n, g->wr.p, d0.b32.p, d0.x32.p);
+    k_prep_vcycle<float><<<nblk(n), 256, 0, s>>>(n, g->wr.p, d0.b32.p, d0.x32.p, nullptr);
     launch_graph_or_issue(g, 1, true, g->prm.smooth);
     k_permute_out<float, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, d0.x32.p, d_z);
   } else {
-    k_prep_vcycle<double><<<nblk(n), 256, 0, s>>>(n, g->wr.p, d0.b64.p, d0.x64.p);
+    k_prep_vcycle<double><<<nblk(n), 256, 0, s>>>(n, g->wr.p, d0.b64.p, d0.x64.p, nullptr);
     launch_graph_or_issue(g, 0, true, g->prm.smooth);
     k_permute_out<double, double><<<nblk(n), 256, 0, s>>>(n, g->perm0.p, d0.x64.p, d_z);
   }
@@ -1596,8 +1634,9 @@ extern "C" int afe_amg_kernel_timing(afe_amg *g, int32_t which, int32_t reps, do
   }
   auto once = [&]() {
     if (which == 0) {
-      k_spmv_dot<<<g_tune.spmv_blocks, BLK, 0, s>>>(d.si.p, d.ns, d.cols.p, d.v64.p, d.diag64.p, g->wp.p,
-                                                     g->wq.p, g->part.p, g_tune.xcd);
+      k_spmv_dot<<<g_tune.spmv_blocks, BLK, 0, s>>>(g->nat_si.p, g->nat_ns, g->nat_cols.p, g->nat_v64.p,
+                                                     g->nat_diag.p, g->nat_rowmap.p, g->wp.p, g->wq.p, g->part.p,
+                                                     g_tune.xcd);
     } else if (which == 1) {
       for (int c = 0; c < d.ncol; c++) gs_colour<float>(d, c, s);
     } else if (which == 2) {
@@ -1631,7 +1670,7 @@ extern "C" int afe_copy_kernel_timing(afe_ctx *ctx, int64_t bytes, int32_t reps,
   b.alloc(ctx, (size_t)n2 * 2);
   hipStream_t s = ctx->stream;
   k_fill<double><<<nblk(n2 * 2), 256, 0, s>>>(n2 * 2, a.p, 1.0);
-  const int grid = 8192;
+  const int grid = 1024; // fastest measured shape of this copy on MI355X
   k_stream_copy<<<grid, 256, 0, s>>>(n2, (const dvec2 *)a.p, (dvec2 *)b.p);
   HIPCHK(hipEventRecord(ctx->ev0, s));
   for (int r = 0; r < reps; r++) k_stream_copy<<<grid, 256, 0, s>>>(n2, (const dvec2 *)a.p, (dvec2 *)b.p);
@@ -1663,9 +1702,9 @@ extern "C" int afe_amg_device_info(const afe_amg *g, int32_t level, int64_t *row
 }
 
 // launch tuning (process-wide): xcd = XCD-aware block order (0/1),
-// spmv_blocks = grid of the grid-stride level-0 kernels (8..8192, multiple of 8)
+// spmv_blocks = grid of the grid-stride level-0 kernels (8..16384, multiple of 8)
 extern "C" int afe_set_tuning(int32_t xcd, int32_t spmv_blocks, int64_t tail_rows) {
-  if ((xcd != 0 && xcd != 1) || spmv_blocks < 8 || spmv_blocks > 8192 || spmv_blocks % 8 || tail_rows < 0)
+  if (xcd < 0 || xcd > 2 || spmv_blocks < 8 || spmv_blocks > kMaxGridBlocks || spmv_blocks % 8 || tail_rows < 0)
     return fail(AFE_E_ARG, "bad tuning values");
   g_tune.xcd = xcd;
   g_tune.spmv_blocks = spmv_blocks;

@@ -526,7 +526,24 @@ __device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadId
 // is a multiple of 8 when xcd != 0.
 __device__ inline int logical_block(int xcd) {
   const int b = blockIdx.x;
-  return xcd ? (b & 7) * (int)(gridDim.x >> 3) + (b >> 3) : b;
+  return xcd == 1 ? (b & 7) * (int)(gridDim.x >> 3) + (b >> 3) : b;
+}
+
+// slice schedule of the grid-stride level-0 kernels: sched 0 = contiguous
+// chunk per block, 1 = contiguous chunk per XCD, 2 = interleaved (the blocks
+// running at one time read neighbouring slices)
+__device__ inline void slice_range(int ns, int sched, int &s, int &end, int &step) {
+  if (sched == 2) {
+    s = blockIdx.x * WPB + wave_id();
+    end = ns;
+    step = gridDim.x * WPB;
+  } else {
+    const int spb = (ns + gridDim.x - 1) / gridDim.x;
+    const int lb = logical_block(sched);
+    s = lb * spb + wave_id();
+    end = min(ns, (lb + 1) * spb);
+    step = WPB;
+  }
 }
 
 // Gauss-Seidel over the slices [s0, s1) of one colour:
@@ -775,7 +792,7 @@ __global__ __launch_bounds__(256) void k_stream_copy(int64_t n4, const dvec2 *__
                                                      dvec2 *__restrict__ dst) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
-    dst[i] = __builtin_nontemporal_load(&src[i]);
+    __builtin_nontemporal_store(__builtin_nontemporal_load(&src[i]), &dst[i]);
 }
 
 template <class T>
@@ -804,21 +821,23 @@ __global__ __launch_bounds__(BLK) void k_spmv_dot(const SliceInfo *__restrict__ 
                                                   const int32_t *__restrict__ cols,
                                                   const double *__restrict__ vals,
                                                   const double *__restrict__ diag,
+                                                  const int32_t *__restrict__ rowmap,
                                                   const double *__restrict__ x, double *__restrict__ y,
                                                   double *__restrict__ part, int xcd) {
   __shared__ double sh[BLK];
   const int lane = threadIdx.x % WAVE;
   double dot = 0.0;
-  const int spb = (ns + gridDim.x - 1) / gridDim.x;
-  const int lb = logical_block(xcd);
-  const int s_end = min(ns, (lb + 1) * spb);
-  for (int s = lb * spb + wave_id(); s < s_end; s += WPB) {
+  int s0, s_end, step;
+  slice_range(ns, xcd, s0, s_end, step);
+  for (int s = s0; s < s_end; s += step) {
     const SliceInfo q = si[s];
     if (lane < q.nrow) {
-      const int r = q.row0 + lane;
-      const double yr = diag[r] * x[r] + sell_dot<double, true>(q, lane, cols, vals, x);
+      const int pos = q.row0 + lane;
+      const int r = rowmap ? rowmap[pos] : pos;
+      const double xr = x[r];
+      const double yr = diag[pos] * xr + sell_dot<double, true>(q, lane, cols, vals, x);
       y[r] = yr;
-      dot += x[r] * yr;
+      dot += xr * yr;
     }
   }
   dot = block_sum(dot, sh);
@@ -872,6 +891,7 @@ __global__ __launch_bounds__(BLK) void k_residual_norms_dd(const SliceInfo *__re
                                                            const int32_t *__restrict__ cols,
                                                            const double *__restrict__ vals,
                                                            const double *__restrict__ diag,
+                                                           const int32_t *__restrict__ rowmap,
                                                            const double *__restrict__ b,
                                                            const double *__restrict__ x,
                                                            const double *__restrict__ xl,
@@ -880,16 +900,16 @@ __global__ __launch_bounds__(BLK) void k_residual_norms_dd(const SliceInfo *__re
   __shared__ double sh[BLK];
   const int lane = threadIdx.x % WAVE;
   double rr = 0.0, bb = 0.0, l1 = 0.0;
-  const int spb = (ns + gridDim.x - 1) / gridDim.x;
-  const int lb = logical_block(xcd);
-  const int s_end = min(ns, (lb + 1) * spb);
-  for (int s = lb * spb + wave_id(); s < s_end; s += WPB) {
+  int s0, s_end, step;
+  slice_range(ns, xcd, s0, s_end, step);
+  for (int s = s0; s < s_end; s += step) {
     const SliceInfo q = si[s];
     if (lane < q.nrow) {
-      const int row = q.row0 + lane;
+      const int pos = q.row0 + lane;
+      const int row = rowmap ? rowmap[pos] : pos;
       double hi = b[row], lo = 0.0;
       {
-        const double a = diag[row], xv = x[row];
+        const double a = diag[pos], xv = x[row];
         const double pr = a * xv, pe = fma(a, xv, -pr);
         double t, e;
         two_sum(hi, -pr, t, e);
@@ -952,6 +972,7 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n, const CgScalars *_
                                                    double *__restrict__ r,
                                                    const double *__restrict__ p, const double *__restrict__ q,
                                                    T *__restrict__ b0, T *__restrict__ x0,
+                                                   const int32_t *__restrict__ iperm,
                                                    double *__restrict__ part) {
   __shared__ double sh[256];
   const double alpha = sc->rz / sc->pq;
@@ -966,7 +987,7 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n, const CgScalars *_
     xl[i] += e + ape;
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
-    b0[i] = (T)ri;
+    b0[iperm ? iperm[i] : i] = (T)ri;
     x0[i] = 0;
     rr += ri * ri;
   }
@@ -976,27 +997,31 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n, const CgScalars *_
 
 #pragma clang fp contract(on)
 
-// (T) r -> b0, x0 = 0 (start of a preconditioner application)
+// (T) r -> b0 (colour order through iperm when given), x0 = 0 (start of a
+// preconditioner application)
 template <class T>
 __global__ __launch_bounds__(256) void k_prep_vcycle(int64_t n, const double *__restrict__ r,
-                                                     T *__restrict__ b0, T *__restrict__ x0) {
+                                                     T *__restrict__ b0, T *__restrict__ x0,
+                                                     const int32_t *__restrict__ iperm) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    b0[i] = (T)r[i];
+    b0[iperm ? iperm[i] : i] = (T)r[i];
     x0[i] = 0;
   }
 }
 
-// z_new = (double) x0; partials (r, z_new) and (r, z_old); z = z_new
+// z_new = (double) x0 (gathered from colour order through iperm when given);
+// partials (r, z_new) and (r, z_old); z = z_new
 template <class T>
 __global__ __launch_bounds__(256) void k_take_z(int64_t n, const T *__restrict__ x0,
                                                 const double *__restrict__ r, double *__restrict__ z,
+                                                const int32_t *__restrict__ iperm,
                                                 double *__restrict__ part) {
   __shared__ double sh[256];
   double a = 0.0, b = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const double zn = (double)x0[i];
+    const double zn = (double)x0[iperm ? iperm[i] : i];
     a += r[i] * zn;
     b += r[i] * z[i];
     z[i] = zn;

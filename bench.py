@@ -147,8 +147,8 @@ def main():
     gg, _ = build(50, 0.382)
     tune_log = []
     best_t = None
-    for xcd in (0, 1):
-        for blocks in (1024, 2048, 4096):
+    for xcd in (0, 1, 2):
+        for blocks in (1024, 2048, 4096, 8192, 16384):
             afe.set_tuning(xcd, blocks, 4096)
             t_sp, _ = gg.kernel_timing(0, reps=10)
             t_gs, _ = gg.kernel_timing(1, reps=10)

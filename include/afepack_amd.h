@@ -273,11 +273,12 @@ int afe_amg_kernel_timing(afe_amg *g, int32_t which, int32_t reps,
 int afe_copy_kernel_timing(afe_ctx *ctx, int64_t bytes, int32_t reps,
                            double *avg_ms);
 /* launch tuning of the sliced-ELL kernels (process-wide, set by the
- * autotuner): xcd = 1 orders workgroups so that each XCD works on a
- * contiguous slice range; spmv_blocks = grid of the grid-stride level-0
- * kernels (multiple of 8, at most 8192); tail_rows = levels of at most this
- * many rows run in the single-workgroup coarse tail.  Applies to
- * hierarchies built afterwards. */
+ * autotuner): xcd = 0 plain workgroup order; 1 orders workgroups so that
+ * each XCD works on a contiguous slice range; 2 interleaves the slices of the
+ * grid-stride level-0 kernels (neighbouring slices read at the same time);
+ * spmv_blocks = grid of the grid-stride level-0 kernels (multiple of 8, at
+ * most 16384); tail_rows = levels of at most this many rows run in the
+ * single-workgroup coarse tail.  Applies to hierarchies built afterwards. */
 int afe_set_tuning(int32_t xcd, int32_t spmv_blocks, int64_t tail_rows);
 /* device layout of a level: rows, colours, stored sliced-ELL entries and the
  * first level handled by the single-workgroup coarse tail */

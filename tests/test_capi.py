@@ -47,6 +47,8 @@ def test_errors_without_device():
 
 
 def test_tuning_validation():
-    with pytest.raises(afe.AfeError):
-        afe.set_tuning(2, 1024, 4096)
+    for bad in ((3, 1024, 4096), (0, 1001, 4096), (0, 32768, 4096), (0, 1024, -1)):
+        with pytest.raises(afe.AfeError):
+            afe.set_tuning(*bad)
+    afe.set_tuning(2, 16384, 4096)
     afe.set_tuning(0, 2048, 4096)
