@@ -85,6 +85,13 @@ SIGS = {
     "afe_set_tuning": (C.c_int, [I32, I32, I64]),
     "afe_amg_kernel_timing": (C.c_int, [P, I32, I32, PF64, PF64]),
     "afe_copy_kernel_timing": (C.c_int, [P, I64, I32, PF64]),
+    "afe_batch_create": (C.c_int, [P, I32, I32, PP]),
+    "afe_batch_destroy": (C.c_int, [P]),
+    "afe_batch_add": (C.c_int, [P, P, P, P, F64, I32, PI32]),
+    "afe_batch_step": (C.c_int, [P, PI32]),
+    "afe_batch_run": (C.c_int, [P]),
+    "afe_batch_result": (C.c_int, [P, I32, PI32, P]),
+    "afe_batch_remove": (C.c_int, [P, I32]),
     "afe_amg_device_info": (C.c_int, [P, I32, PI64, PI32, PI64, PI32]),
 }
 

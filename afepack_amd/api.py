@@ -14,7 +14,7 @@ __all__ = [
     "P1_TRI", "P2_TRI", "P1_TET", "OP_STIFF", "OP_MASS", "OP_VARSTIFF",
     "F_POISSON", "F_VARCOEF", "F_NODAL", "F_ONE", "PREC_FP64", "PREC_FP32",
     "set_tuning", "mesh_square", "mesh_cube", "mesh_displace", "field_random", "Field",
-    "Context", "Space", "Matrix", "Amg", "HostAmg", "AmgParams", "DeviceVector",
+    "Context", "Space", "Matrix", "Amg", "HostAmg", "AmgParams", "DeviceVector", "Batch",
 ]
 
 P1_TRI, P2_TRI, P1_TET = 1, 2, 3
@@ -451,3 +451,54 @@ class Amg(_AmgBase):
             _chk(_lib().afe_amg_device_info(self.h, l, C.byref(r), C.byref(c), C.byref(s), C.byref(t)))
             out.append({"rows": r.value, "colours": c.value, "stored": s.value, "tail": t.value})
         return out
+
+
+class Batch:
+    """Continuous batching of FP64 CG solves preconditioned by one V-cycle
+    (FP32 or FP64) per iteration; every step runs one iteration of every
+    active member with shared launches."""
+
+    def __init__(self, ctx, prec=PREC_FP32, smooth=1):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _chk(_lib().afe_batch_create(ctx.h, prec, smooth, C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def add(self, amg, x, b, rtol=1e-10, max_it=1000):
+        slot = C.c_int32()
+        _chk(_lib().afe_batch_add(self.h, amg.h, _ptr(x), _ptr(b), rtol, max_it, C.byref(slot)))
+        self._keep[slot.value] = (amg, x, b)
+        return slot.value
+
+    def step(self):
+        n = C.c_int32()
+        _chk(_lib().afe_batch_step(self.h, C.byref(n)))
+        return n.value
+
+    def run(self):
+        _chk(_lib().afe_batch_run(self.h))
+
+    def result(self, slot):
+        """(done, stats, status code)"""
+        done = C.c_int32()
+        st = _sigs.SolveStats()
+        code = _lib().afe_batch_result(self.h, slot, C.byref(done), C.byref(st))
+        if code not in (0, -5):
+            _chk(code)
+        return bool(done.value), st, code
+
+    def remove(self, slot):
+        _chk(_lib().afe_batch_remove(self.h, slot))
+        self._keep.pop(slot, None)
+
+    def close(self):
+        if self.h:
+            _lib().afe_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

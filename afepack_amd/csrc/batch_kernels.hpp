@@ -1,0 +1,252 @@
+// batch_kernels.hpp -- one launch for the same step of many independent
+// solves (continuous batching of the AMG-preconditioned CG).
+//
+// A batched launch covers a list of segments; a segment is one member's share
+// of the launch: a contiguous run of blocks working on a contiguous range of
+// that member's slices (or rows).  Blocks never mix members, and a member's
+// slices are visited by its own blocks in the same way whatever else is in
+// the batch, so every member's result -- reductions included, which are
+// summed per member in block order -- is bitwise the same alone and in any
+// mix.
+#pragma once
+#include "kernels.hpp"
+
+namespace afe {
+
+struct Seg {
+  int32_t block0; // first block of the segment in the grid
+  int32_t member; // index into the launch's member tables
+  int32_t s0, s1; // slice (or row) range of the member
+};
+
+// the segment of this block (segments sorted by block0, block0[0] = 0)
+__device__ inline Seg find_seg(const Seg *__restrict__ segs, int nseg) {
+  int lo = 0, hi = nseg - 1;
+  const int b = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return segs[lo];
+}
+
+// per-member data of the CG in the natural numbering and the level-0 vectors
+// of the V-cycle (colour order)
+template <class T> struct MemberDev {
+  int64_t n;
+  const SliceInfo *nat_si;
+  const int32_t *nat_cols, *nat_rowmap, *iperm0;
+  const double *nat_v, *nat_diag;
+  double *x, *xl, *r, *p, *q, *z;
+  T *b0, *x0;
+  CgScalars *sc;
+};
+
+// ---- V-cycle level kernels: one wavefront per slice ----------------------
+template <class T>
+__global__ __launch_bounds__(BLK) void kb_gs(const Seg *__restrict__ segs, int nseg,
+                                             const LevelDev<T> *__restrict__ lv) {
+  const Seg g = find_seg(segs, nseg);
+  const int s = g.s0 + (blockIdx.x - g.block0) * WPB + wave_id();
+  if (s >= g.s1) return;
+  const LevelDev<T> &d = lv[g.member];
+  const SliceInfo q = d.si[s];
+  const int lane = threadIdx.x % WAVE;
+  if (lane >= q.nrow) return;
+  const int r = q.row0 + lane;
+  const T acc = sell_dot<T>(q, lane, d.cols, d.vals, d.x);
+  d.x[r] = (d.b[r] - acc) * d.invd[r];
+}
+
+template <class T>
+__global__ __launch_bounds__(BLK) void kb_residual(const Seg *__restrict__ segs, int nseg,
+                                                   const LevelDev<T> *__restrict__ lv) {
+  const Seg g = find_seg(segs, nseg);
+  const int s = g.s0 + (blockIdx.x - g.block0) * WPB + wave_id();
+  if (s >= g.s1) return;
+  const LevelDev<T> &d = lv[g.member];
+  const SliceInfo q = d.si[s];
+  const int lane = threadIdx.x % WAVE;
+  if (lane >= q.nrow) return;
+  const int r = q.row0 + lane;
+  const T acc = sell_dot<T>(q, lane, d.cols, d.vals, d.x);
+  d.t[r] = d.b[r] - d.diag[r] * d.x[r] - acc;
+}
+
+// b_{l+1} = P t_l, x_{l+1} = 0 (lv: level l, lvn: level l+1 of each member)
+template <class T>
+__global__ __launch_bounds__(BLK) void kb_restrict(const Seg *__restrict__ segs, int nseg,
+                                                   const LevelDev<T> *__restrict__ lv,
+                                                   const LevelDev<T> *__restrict__ lvn) {
+  const Seg g = find_seg(segs, nseg);
+  const int s = g.s0 + (blockIdx.x - g.block0) * WPB + wave_id();
+  if (s >= g.s1) return;
+  const LevelDev<T> &d = lv[g.member];
+  const LevelDev<T> &dn = lvn[g.member];
+  const SliceInfo q = d.p_si[s];
+  const int lane = threadIdx.x % WAVE;
+  if (lane >= q.nrow) return;
+  const int r = q.row0 + lane;
+  dn.b[r] = sell_dot<T>(q, lane, d.p_cols, d.p_vals, d.t);
+  dn.x[r] = 0;
+}
+
+// x_l += P^T x_{l+1}
+template <class T>
+__global__ __launch_bounds__(BLK) void kb_prolong(const Seg *__restrict__ segs, int nseg,
+                                                  const LevelDev<T> *__restrict__ lv,
+                                                  const LevelDev<T> *__restrict__ lvn) {
+  const Seg g = find_seg(segs, nseg);
+  const int s = g.s0 + (blockIdx.x - g.block0) * WPB + wave_id();
+  if (s >= g.s1) return;
+  const LevelDev<T> &d = lv[g.member];
+  const LevelDev<T> &dn = lvn[g.member];
+  const SliceInfo q = d.t_si[s];
+  const int lane = threadIdx.x % WAVE;
+  if (lane >= q.nrow) return;
+  const int r = q.row0 + lane;
+  d.x[r] += sell_dot<T>(q, lane, d.t_cols, d.t_vals, dn.x);
+}
+
+// the coarse tail of each member: one workgroup per member
+template <class T> struct TailJob {
+  const LevelDev<T> *L;
+  const double *minv;
+  int l0, lc, coarse_mode;
+};
+
+template <class T>
+__global__ __launch_bounds__(TAIL_THREADS) void kb_tail(const TailJob<T> *__restrict__ jobs, int nu,
+                                                        int lds_rows) {
+  const TailJob<T> j = jobs[blockIdx.x];
+  tail_body<T>(j.L, j.l0, j.lc, j.minv, j.coarse_mode, nu, nu, 1, 7, lds_rows);
+}
+
+// ---- CG kernels (natural numbering) ---------------------------------------
+// q = A p and the block partial of (p, q); segments over natural slices
+template <class T>
+__global__ __launch_bounds__(BLK) void kb_spmv_dot(const Seg *__restrict__ segs, int nseg,
+                                                   const MemberDev<T> *__restrict__ md,
+                                                   double *__restrict__ part) {
+  __shared__ double sh[BLK];
+  const Seg g = find_seg(segs, nseg);
+  const int s = g.s0 + (blockIdx.x - g.block0) * WPB + wave_id();
+  const int lane = threadIdx.x % WAVE;
+  double dot = 0.0;
+  if (s < g.s1) {
+    const MemberDev<T> &m = md[g.member];
+    const SliceInfo q = m.nat_si[s];
+    if (lane < q.nrow) {
+      const int pos = q.row0 + lane;
+      const int r = m.nat_rowmap[pos];
+      const double xr = m.p[r];
+      const double yr = m.nat_diag[pos] * xr + sell_dot<double>(q, lane, m.nat_cols, m.nat_v, m.p);
+      m.q[r] = yr;
+      dot = xr * yr;
+    }
+  }
+  dot = block_sum(dot, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = dot;
+}
+
+// sum of each member's block partials, in block order, into a scalar slot:
+// one block per member; first[m], count[m] = the member's block range
+__global__ __launch_bounds__(256) void kb_finalize(const int32_t *__restrict__ first,
+                                                   const int32_t *__restrict__ count, int nsum, int stride,
+                                                   const double *__restrict__ part,
+                                                   CgScalars *const *__restrict__ sc, int field) {
+  __shared__ double sh[256];
+  const int m = blockIdx.x;
+  for (int k = 0; k < nsum; k++) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < count[m]; i += blockDim.x) s += part[k * stride + first[m] + i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) (&sc[m]->rz)[field + k] = s;
+  }
+}
+
+#pragma clang fp contract(off)
+// x += alpha p (double-double pair), r -= alpha q, (T) r -> b0 (colour
+// order), x0 = 0, block partial of (r, r); segments over rows, 256 per block
+template <class T>
+__global__ __launch_bounds__(256) void kb_cg_update(const Seg *__restrict__ segs, int nseg,
+                                                    const MemberDev<T> *__restrict__ md,
+                                                    double *__restrict__ part) {
+  __shared__ double sh[256];
+  const Seg g = find_seg(segs, nseg);
+  const int64_t i = (int64_t)g.s0 + (int64_t)(blockIdx.x - g.block0) * 256 + threadIdx.x;
+  double rr = 0.0;
+  if (i < g.s1) {
+    const MemberDev<T> &m = md[g.member];
+    const double alpha = m.sc->rz / m.sc->pq;
+    const double ap = alpha * m.p[i];
+    const double ape = fma(alpha, m.p[i], -ap);
+    double s, e;
+    two_sum(m.x[i], ap, s, e);
+    m.x[i] = s;
+    m.xl[i] += e + ape;
+    const double ri = m.r[i] - alpha * m.q[i];
+    m.r[i] = ri;
+    m.b0[m.iperm0[i]] = (T)ri;
+    m.x0[i] = 0;
+    rr = ri * ri;
+  }
+  rr = block_sum(rr, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = rr;
+}
+#pragma clang fp contract(on)
+
+// z_new = (double) x0 gathered from colour order; partials (r, z_new) and
+// (r, z_old) at part[b] and part[stride + b]; z = z_new
+template <class T>
+__global__ __launch_bounds__(256) void kb_take_z(const Seg *__restrict__ segs, int nseg,
+                                                 const MemberDev<T> *__restrict__ md, int stride,
+                                                 double *__restrict__ part) {
+  __shared__ double sh[256];
+  const Seg g = find_seg(segs, nseg);
+  const int64_t i = (int64_t)g.s0 + (int64_t)(blockIdx.x - g.block0) * 256 + threadIdx.x;
+  double a = 0.0, b = 0.0;
+  if (i < g.s1) {
+    const MemberDev<T> &m = md[g.member];
+    const double zn = (double)m.x0[m.iperm0[i]];
+    a = m.r[i] * zn;
+    b = m.r[i] * m.z[i];
+    m.z[i] = zn;
+  }
+  a = block_sum(a, sh);
+  b = block_sum(b, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[stride + blockIdx.x] = b;
+  }
+}
+
+// flexible CG beta of every member (one thread per member)
+__global__ void kb_cg_beta(CgScalars *const *__restrict__ sc, int nm) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < nm) {
+    CgScalars *c = sc[m];
+    c->beta = (c->rz_new - c->rz_cross) / c->rz;
+    c->rz = c->rz_new;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void kb_p_update(const Seg *__restrict__ segs, int nseg,
+                                                   const MemberDev<T> *__restrict__ md) {
+  const Seg g = find_seg(segs, nseg);
+  const int64_t i = (int64_t)g.s0 + (int64_t)(blockIdx.x - g.block0) * 256 + threadIdx.x;
+  if (i < g.s1) {
+    const MemberDev<T> &m = md[g.member];
+    m.p[i] = m.z[i] + m.sc->beta * m.p[i];
+  }
+}
+
+// copy every member's scalar block into one array (one D2H copy per step)
+__global__ void kb_gather_scalars(CgScalars *const *__restrict__ sc, int nm, CgScalars *__restrict__ out) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < nm) out[m] = *sc[m];
+}
+
+} // namespace afe

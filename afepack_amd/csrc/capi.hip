@@ -19,6 +19,7 @@
 #include "../../include/afepack_amd.h"
 #include "host.hpp"
 #include "kernels.hpp"
+#include "batch_kernels.hpp"
 #include "quadrature.hpp"
 
 using namespace afe;
@@ -137,6 +138,7 @@ struct Tuning {
 static Tuning g_tune;
 static constexpr int kMaxGridBlocks = 16384; // bound of spmv_blocks (partial-sum buffer)
 static constexpr int64_t NAT_WINDOW = 256;    // length-sort window of the natural-order copy
+static constexpr int32_t COLOUR_WINDOW = 1024; // length-sort window inside a colour
 
 static inline unsigned sblk(int64_t nslices) {
   unsigned b = nblk(nslices, WPB);
@@ -166,6 +168,10 @@ extern "C" int afe_ctx_create(int device, afe_ctx **out) {
     HIPCHK(hipFuncSetAttribute((const void *)k_tail<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)c->lds_bytes));
     HIPCHK(hipFuncSetAttribute((const void *)k_tail<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)c->lds_bytes));
+    HIPCHK(hipFuncSetAttribute((const void *)kb_tail<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)c->lds_bytes));
+    HIPCHK(hipFuncSetAttribute((const void *)kb_tail<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)c->lds_bytes));
   }
   HIPCHK(hipMalloc((void **)&c->d_quad, sizeof(kQuadData)));
@@ -876,6 +882,8 @@ struct afe_amg {
   DevBuf<double> minv;
   DevBuf<LevelDev<double>> ld64;
   DevBuf<LevelDev<float>> ld32;
+  std::vector<LevelDev<double>> h_ld64; // host copies of the level descriptors
+  std::vector<LevelDev<float>> h_ld32;
   DevBuf<int32_t> perm0, iperm0; // colour index -> natural row, and back
   // level-0 FP64 matrix in the natural numbering (the CG SpMV and the true
   // residual): rows sorted by stored length inside windows of NAT_WINDOW
@@ -950,10 +958,16 @@ static void upload_hierarchy(afe_amg *g) {
     for (int64_t i = 0; i < A.m; i++)
       for (int64_t k = A.rp[i]; k < A.rp[i + 1]; k++)
         if (A.col[k] != i && A.val[k] != 0.0) len[i]++;
+    // rows of one colour keep their index order in windows of COLOUR_WINDOW
+    // rows (x locality of the gathers) and are sorted by stored length inside
+    // a window (little sliced-ELL padding)
+    std::vector<int32_t> win(A.m), seen(ncol, 0);
+    for (int64_t i = 0; i < A.m; i++) win[i] = seen[colour[i]]++ / COLOUR_WINDOW;
     perm[l].resize(A.m);
     std::iota(perm[l].begin(), perm[l].end(), 0);
     std::stable_sort(perm[l].begin(), perm[l].end(), [&](int32_t a, int32_t b) {
       if (colour[a] != colour[b]) return colour[a] < colour[b];
+      if (win[a] != win[b]) return win[a] < win[b];
       return len[a] > len[b];
     });
     iperm[l].resize(A.m);
@@ -1072,6 +1086,8 @@ static void upload_hierarchy(afe_amg *g) {
   }
   g->ld64.upload(ctx, d64);
   if (g->has32) g->ld32.upload(ctx, d32);
+  g->h_ld64 = d64;
+  g->h_ld32 = d32;
   const int64_t n0 = g->lv[0].n;
   g->wx.alloc(ctx, n0);
   g->wxl.alloc(ctx, n0);
@@ -1711,3 +1727,5 @@ extern "C" int afe_set_tuning(int32_t xcd, int32_t spmv_blocks, int64_t tail_row
   g_tune.tail_rows = tail_rows;
   return AFE_OK;
 }
+
+#include "batch_impl.hpp"

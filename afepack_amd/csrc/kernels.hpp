@@ -648,10 +648,9 @@ __device__ inline void tail_sweep(const LevelDev<T> &d, const T *b, T *x, int c,
 // larger levels use the global vectors.  x of a level is written back to
 // global memory at the end of each leg.
 template <class T>
-__global__ __launch_bounds__(TAIL_THREADS) void k_tail(const LevelDev<T> *__restrict__ L, int l0, int lc,
-                                                       const double *__restrict__ minv, int coarse_mode,
-                                                       int nu_pre, int nu_post, int post_reverse, int phase,
-                                                       int lds_rows) {
+__device__ inline void tail_body(const LevelDev<T> *__restrict__ L, int l0, int lc,
+                                 const double *__restrict__ minv, int coarse_mode, int nu_pre, int nu_post,
+                                 int post_reverse, int phase, int lds_rows) {
   extern __shared__ char tail_smem[];
   T *sx = reinterpret_cast<T *>(tail_smem);
   T *sb = sx + lds_rows;
@@ -754,6 +753,14 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(const LevelDev<T> *__rest
       }
     }
   }
+}
+
+template <class T>
+__global__ __launch_bounds__(TAIL_THREADS) void k_tail(const LevelDev<T> *__restrict__ L, int l0, int lc,
+                                                       const double *__restrict__ minv, int coarse_mode,
+                                                       int nu_pre, int nu_post, int post_reverse, int phase,
+                                                       int lds_rows) {
+  tail_body<T>(L, l0, lc, minv, coarse_mode, nu_pre, nu_post, post_reverse, phase, lds_rows);
 }
 
 // dense coarse solve x = Minv b with the FP64 inverse (DenseMatrix::vmult of

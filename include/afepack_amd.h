@@ -260,6 +260,32 @@ int afe_amg_vcycle(afe_amg *g, double *d_z, const double *d_r, int32_t prec);
 /* AMGSolver::smoothStep() = s (AMGSolver.h:98-99) after construction */
 int afe_amg_set_smooth(afe_amg *g, int32_t smooth);
 
+/* ---- batched solves (no counterpart in the reference) --------------------
+ * Continuous batching of afe_amg_pcg solves on hierarchies of one context:
+ * every afe_batch_step runs one CG iteration of every active member with one
+ * launch per step kind and level/colour covering all members.  A member's
+ * arithmetic does not depend on the other members: its result is bitwise
+ * the same alone and in any mix.  Members may be added between steps; a
+ * member finishes when its delivered vector reaches rtol (as afe_amg_pcg) or
+ * max_it; its solution is then in its d_x.  The V-cycle uses `smooth`
+ * sweeps per side for every member (the member hierarchy's smoothing count
+ * is set to it).  One active solve per hierarchy at a time. */
+typedef struct afe_batch afe_batch;
+int afe_batch_create(afe_ctx *ctx, int32_t prec, int32_t smooth,
+                     afe_batch **out);
+int afe_batch_destroy(afe_batch *b);
+int afe_batch_add(afe_batch *b, afe_amg *g, double *d_x, const double *d_b,
+                  double rtol, int32_t max_it, int32_t *slot);
+/* one batched CG iteration; *n_active = members still iterating */
+int afe_batch_step(afe_batch *b, int32_t *n_active);
+/* steps until no member is active */
+int afe_batch_run(afe_batch *b);
+/* *done, stats (seconds = time from afe_batch_add to completion); returns
+ * the member's status once done (AFE_E_NOTCONVERGED if it missed rtol) */
+int afe_batch_result(afe_batch *b, int32_t slot, int32_t *done,
+                     afe_solve_stats *st);
+int afe_batch_remove(afe_batch *b, int32_t slot);
+
 /* ---- measurement hooks (no counterpart in the reference) ------------------ */
 /* average duration of `reps` back-to-back launches of a level-0 kernel as the
  * solvers launch it, timed with HIP events on the context stream; which:

@@ -1,0 +1,115 @@
+"""GPU tests of the batched solver (afe_batch_*).
+
+Bars: every member's solution is bitwise identical alone, in a full batch
+and when admitted while other members are mid-solve; batched solutions are
+within 1e-9 relative L2 of a direct solve of the oracle's system (rtol 1e-12).
+"""
+import numpy as np
+import pytest
+
+from tests.conftest import make_mesh
+
+pytestmark = pytest.mark.gpu
+
+# kind, n, jitter, op, rhs
+PROBLEMS = [
+    (1, 24, 0.2, 0, "poisson"),
+    (2, 12, 0.2, 0, "poisson"),
+    (1, 40, 0.1, 2, "nodal"),
+    (3, 5, 0.05, 2, "poisson"),
+    (2, 20, 0.0, 2, "nodal"),
+    (1, 8, 0.0, 0, "poisson"),
+]
+
+
+def _problem(ctx, kind, n, jitter, op, rhs, seed):
+    import afepack_amd as afe
+    c, ev, vm = make_mesh(kind, n, jitter, seed)
+    sp = afe.Space(ctx, kind, c, ev, vm)
+    field = afe.field_random(8, sp.dim, 0.5, seed) if op == 2 else None
+    A = afe.Matrix(sp)
+    A.assemble(op, 4, field)
+    if rhs == "nodal":
+        rng = np.random.default_rng(seed)
+        b = sp.rhs(4, afe.F_NODAL, nodal=rng.uniform(-1, 1, sp.n_dof))
+    else:
+        b = sp.rhs(4, afe.F_POISSON)
+    A.apply_dirichlet(b)
+    g = afe.Amg(A, afe.AmgParams(lazy=1, smooth=1, build_fp32=1))
+    return {"sp": sp, "A": A, "b": b, "g": g, "field": field}
+
+
+@pytest.fixture(scope="module")
+def problems(ctx):
+    return [_problem(ctx, *p, seed=31 + i) for i, p in enumerate(PROBLEMS)]
+
+
+def _solve_batch(ctx, probs, order, rtol, stagger=0):
+    import afepack_amd as afe
+    bt = afe.Batch(ctx, afe.PREC_FP32, smooth=1)
+    xs, slots = {}, {}
+    first = order[: len(order) - stagger] if stagger else order
+    for i in first:
+        xs[i] = afe.DeviceVector(ctx, probs[i]["sp"].n_dof)
+        slots[i] = bt.add(probs[i]["g"], xs[i], probs[i]["b"], rtol=rtol, max_it=400)
+    if stagger:
+        for _ in range(3):
+            bt.step()
+        for i in order[len(order) - stagger:]:
+            xs[i] = afe.DeviceVector(ctx, probs[i]["sp"].n_dof)
+            slots[i] = bt.add(probs[i]["g"], xs[i], probs[i]["b"], rtol=rtol, max_it=400)
+    bt.run()
+    out = {}
+    for i in order:
+        done, st, code = bt.result(slots[i])
+        assert done and code == 0 and st.converged == 1, f"member {i}: code {code}, res {st.rel_res_l2}"
+        assert st.rel_res_l2 <= rtol
+        out[i] = (xs[i].numpy(), st.iterations)
+    bt.close()
+    return out
+
+
+def test_batch_alone_equals_mixed_bitwise(ctx, problems):
+    print("case batch alone vs mixed", flush=True)
+    idx = list(range(len(problems)))
+    alone = {}
+    for i in idx:
+        alone.update(_solve_batch(ctx, problems, [i], 1e-10))
+    mixed = _solve_batch(ctx, problems, idx, 1e-10)
+    late = _solve_batch(ctx, problems, idx[::-1], 1e-10, stagger=3)
+    for i in idx:
+        assert np.array_equal(alone[i][0], mixed[i][0]), f"member {i} differs in the mix"
+        assert np.array_equal(alone[i][0], late[i][0]), f"member {i} differs when admitted late"
+        assert alone[i][1] == mixed[i][1] == late[i][1]
+
+
+def test_batch_solutions_match_direct(ctx, problems):
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as spl
+    print("case batch vs direct", flush=True)
+    idx = list(range(len(problems)))
+    res = _solve_batch(ctx, problems, idx, 1e-12)
+    for i in idx:
+        p = problems[i]
+        rp, col = p["sp"].pattern()
+        A = sps.csr_matrix((p["A"].values(), col, rp), shape=(p["sp"].n_dof,) * 2)
+        xd = spl.spsolve(A.tocsc(), p["b"].numpy())
+        x = res[i][0]
+        assert np.linalg.norm(x - xd) <= 1e-9 * np.linalg.norm(xd), f"member {i}"
+
+
+def test_batch_matches_unbatched_iterations(ctx, problems):
+    import afepack_amd as afe
+    print("case batch vs afe_amg_pcg", flush=True)
+    idx = list(range(len(problems)))
+    res = _solve_batch(ctx, problems, idx, 1e-10)
+    for i in idx:
+        p = problems[i]
+        x = afe.DeviceVector(ctx, p["sp"].n_dof)
+        p["g"].set_smooth(1)
+        st = p["g"].pcg(x, p["b"], rtol=1e-10, max_it=400, prec=afe.PREC_FP32)
+        # same algorithm, different reduction blocking: same iteration count
+        # up to one, solutions agree to the solve tolerance
+        assert abs(st.iterations - res[i][1]) <= 1
+        xr = x.numpy()
+        assert np.linalg.norm(xr - res[i][0]) <= 1e-6 * np.linalg.norm(xr)
