@@ -85,6 +85,7 @@ SIGS = {
     "afe_set_tuning": (C.c_int, [I32, I32, I64]),
     "afe_amg_kernel_timing": (C.c_int, [P, I32, I32, PF64, PF64]),
     "afe_copy_kernel_timing": (C.c_int, [P, I64, I32, PF64]),
+    "afe_amg_set_tail_rows": (C.c_int, [P, I64]),
     "afe_batch_create": (C.c_int, [P, I32, I32, PP]),
     "afe_batch_destroy": (C.c_int, [P]),
     "afe_batch_add": (C.c_int, [P, P, P, P, F64, I32, PI32]),

@@ -439,6 +439,9 @@ class Amg(_AmgBase):
     def set_smooth(self, s):
         _chk(_lib().afe_amg_set_smooth(self.h, s))
 
+    def set_tail_rows(self, rows):
+        _chk(_lib().afe_amg_set_tail_rows(self.h, int(rows)))
+
     def kernel_timing(self, which, reps=20):
         ms, by = C.c_double(), C.c_double()
         _chk(_lib().afe_amg_kernel_timing(self.h, which, reps, C.byref(ms), C.byref(by)))

@@ -1705,6 +1705,17 @@ extern "C" int afe_amg_set_smooth(afe_amg *g, int32_t smooth) {
   return AFE_OK;
 }
 
+// levels of at most `rows` rows run in the single-workgroup tail from now on
+extern "C" int afe_amg_set_tail_rows(afe_amg *g, int64_t rows) {
+  if (!g || !g->ctx || rows < 0) return fail(AFE_E_ARG, "bad arguments");
+  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  const int L = (int)g->lv.size();
+  drop_graphs(g);
+  g->tail0 = L - 1;
+  while (g->tail0 > 0 && g->lv[g->tail0 - 1].n <= rows) g->tail0--;
+  return AFE_OK;
+}
+
 // per-level device layout summary: rows, colours, stored sliced-ELL entries
 extern "C" int afe_amg_device_info(const afe_amg *g, int32_t level, int64_t *rows, int32_t *colours,
                                    int64_t *stored, int32_t *tail_level) {

@@ -306,6 +306,10 @@ int afe_copy_kernel_timing(afe_ctx *ctx, int64_t bytes, int32_t reps,
  * most 16384); tail_rows = levels of at most this many rows run in the
  * single-workgroup coarse tail.  Applies to hierarchies built afterwards. */
 int afe_set_tuning(int32_t xcd, int32_t spmv_blocks, int64_t tail_rows);
+/* move the boundary of the single-workgroup coarse tail of one hierarchy:
+ * levels of at most `rows` rows run in the tail (batches of small problems
+ * run one tail workgroup per member in parallel) */
+int afe_amg_set_tail_rows(afe_amg *g, int64_t rows);
 /* device layout of a level: rows, colours, stored sliced-ELL entries and the
  * first level handled by the single-workgroup coarse tail */
 int afe_amg_device_info(const afe_amg *g, int32_t level, int64_t *rows,
