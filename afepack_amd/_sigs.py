@@ -27,6 +27,12 @@ class SolveStats(C.Structure):
                 ("rel_res_l1", F64), ("res_l1", F64), ("seconds", F64)]
 
 
+class Request(C.Structure):
+    _fields_ = [("mesh_key", I64), ("kind", I32), ("n", I32), ("jitter", F64), ("seed", U64),
+                ("move_amp", F64), ("move_k", I32), ("acc", I32), ("op", I32), ("fkind", I32),
+                ("coef", Field), ("rtol", F64), ("max_it", I32), ("keep_x", I32)]
+
+
 PField = C.POINTER(Field)
 
 SIGS = {
@@ -86,6 +92,13 @@ SIGS = {
     "afe_amg_kernel_timing": (C.c_int, [P, I32, I32, PF64, PF64]),
     "afe_copy_kernel_timing": (C.c_int, [P, I64, I32, PF64]),
     "afe_amg_set_tail_rows": (C.c_int, [P, I64]),
+    "afe_server_create": (C.c_int, [I32, I64, I32, I32, I32, PP]),
+    "afe_server_destroy": (C.c_int, [P]),
+    "afe_server_submit": (C.c_int, [P, C.POINTER(Request), PI64]),
+    "afe_server_poll": (C.c_int, [P, I64, PI32]),
+    "afe_server_wait": (C.c_int, [P, I64, P, PI64, C.POINTER(SolveStats), PF64]),
+    "afe_server_release": (C.c_int, [P, I64]),
+    "afe_server_info": (C.c_int, [P, PI64, PI64, PI64, PI64]),
     "afe_batch_create": (C.c_int, [P, I32, I32, PP]),
     "afe_batch_destroy": (C.c_int, [P]),
     "afe_batch_add": (C.c_int, [P, P, P, P, F64, I32, PI32]),

@@ -14,7 +14,7 @@ __all__ = [
     "P1_TRI", "P2_TRI", "P1_TET", "OP_STIFF", "OP_MASS", "OP_VARSTIFF",
     "F_POISSON", "F_VARCOEF", "F_NODAL", "F_ONE", "PREC_FP64", "PREC_FP32",
     "set_tuning", "mesh_square", "mesh_cube", "mesh_displace", "field_random", "Field",
-    "Context", "Space", "Matrix", "Amg", "HostAmg", "AmgParams", "DeviceVector", "Batch",
+    "Context", "Space", "Matrix", "Amg", "HostAmg", "AmgParams", "DeviceVector", "Batch", "Server",
 ]
 
 P1_TRI, P2_TRI, P1_TET = 1, 2, 3
@@ -498,6 +498,63 @@ class Batch:
     def close(self):
         if self.h:
             _lib().afe_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Solve server: set-up thread pool, HBM mesh/hierarchy cache, one
+    continuously running batch (FP64 CG + FP32 V-cycle)."""
+
+    def __init__(self, device=0, hbm_budget=200 << 30, max_slots=256, setup_threads=8, smooth=1):
+        h = C.c_void_p()
+        _chk(_lib().afe_server_create(device, int(hbm_budget), max_slots, setup_threads, smooth, C.byref(h)))
+        self.h = h
+
+    def submit(self, mesh_key, kind, n, jitter=0.0, seed=0, move_amp=0.0, move_k=1, acc=4,
+               op=OP_STIFF, fkind=F_POISSON, field=None, rtol=1e-10, max_it=1000, keep_x=False):
+        r = _sigs.Request()
+        r.mesh_key, r.kind, r.n, r.jitter, r.seed = mesh_key, kind, n, jitter, seed
+        r.move_amp, r.move_k, r.acc, r.op, r.fkind = move_amp, move_k, acc, op, fkind
+        r.rtol, r.max_it, r.keep_x = rtol, max_it, 1 if keep_x else 0
+        if field is not None:
+            r.coef = field.cstruct()  # the field's arrays are copied by the call
+        rid = C.c_int64()
+        _chk(_lib().afe_server_submit(self.h, C.byref(r), C.byref(rid)))
+        return rid.value
+
+    def poll(self, rid):
+        d = C.c_int32()
+        _chk(_lib().afe_server_poll(self.h, rid, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, rid, n_dof=None):
+        """(solution or None, stats, latency seconds, status)"""
+        nd = C.c_int64()
+        st = _sigs.SolveStats()
+        lat = C.c_double()
+        x = np.empty(n_dof) if n_dof else None
+        code = _lib().afe_server_wait(self.h, rid, _ptr(x), C.byref(nd), C.byref(st), C.byref(lat))
+        if code not in (0, -5):
+            _chk(code)
+        return x, st, lat.value, code
+
+    def release(self, rid):
+        _chk(_lib().afe_server_release(self.h, rid))
+
+    def info(self):
+        b, c, d, e = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        _chk(_lib().afe_server_info(self.h, C.byref(b), C.byref(c), C.byref(d), C.byref(e)))
+        return {"device_bytes": b.value, "cached": c.value, "completed": d.value, "evictions": e.value}
+
+    def close(self):
+        if self.h:
+            _lib().afe_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

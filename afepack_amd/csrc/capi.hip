@@ -4,6 +4,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -942,16 +945,28 @@ static std::vector<double> inv_of(const std::vector<double> &d) {
 }
 
 // colour, permute and convert every level of the host hierarchy; upload
-static void upload_hierarchy(afe_amg *g) {
-  afe_ctx *ctx = g->ctx;
-  const Hierarchy &h = g->h;
+// device layout of a hierarchy, computed on the host without touching the
+// device (so that it can be prepared while other work uses the context)
+struct HostLevel {
+  int ncol = 0;
+  int64_t n = 0;
+  Sell sa, sp, st;
+  std::vector<int32_t> perm, iperm;
+};
+struct HostLayout {
+  std::vector<HostLevel> lv;
+  Sell nat;
+  std::vector<int32_t> nat_rowmap;
+  std::vector<double> minv_perm;
+};
+
+static void build_layout(const Hierarchy &h, HostLayout &lay) {
   const int L = (int)h.A.size();
-  drop_graphs(g);
-  g->lv.clear();
-  g->lv.resize(L);
-  std::vector<std::vector<int32_t>> perm(L), iperm(L), groups(L);
+  lay.lv.assign(L, HostLevel());
+  std::vector<std::vector<int32_t>> groups(L);
   for (int l = 0; l < L; l++) {
     const Csr &A = h.A[l];
+    HostLevel &hl = lay.lv[l];
     std::vector<int32_t> colour;
     const int ncol = colour_rows(A, colour);
     std::vector<int32_t> len(A.m, 0);
@@ -963,24 +978,69 @@ static void upload_hierarchy(afe_amg *g) {
     // a window (little sliced-ELL padding)
     std::vector<int32_t> win(A.m), seen(ncol, 0);
     for (int64_t i = 0; i < A.m; i++) win[i] = seen[colour[i]]++ / COLOUR_WINDOW;
-    perm[l].resize(A.m);
-    std::iota(perm[l].begin(), perm[l].end(), 0);
-    std::stable_sort(perm[l].begin(), perm[l].end(), [&](int32_t a, int32_t b) {
+    hl.perm.resize(A.m);
+    std::iota(hl.perm.begin(), hl.perm.end(), 0);
+    std::stable_sort(hl.perm.begin(), hl.perm.end(), [&](int32_t a, int32_t b) {
       if (colour[a] != colour[b]) return colour[a] < colour[b];
       if (win[a] != win[b]) return win[a] < win[b];
       return len[a] > len[b];
     });
-    iperm[l].resize(A.m);
-    for (int64_t i = 0; i < A.m; i++) iperm[l][perm[l][i]] = (int32_t)i;
+    hl.iperm.resize(A.m);
+    for (int64_t i = 0; i < A.m; i++) hl.iperm[hl.perm[i]] = (int32_t)i;
     groups[l].assign(ncol, 0);
     for (int64_t i = 0; i < A.m; i++) groups[l][colour[i]]++;
-    g->lv[l].ncol = ncol;
-    g->lv[l].n = A.m;
+    hl.ncol = ncol;
+    hl.n = A.m;
   }
   for (int l = 0; l < L; l++) {
+    HostLevel &hl = lay.lv[l];
+    build_sell(h.A[l], hl.perm, hl.iperm, groups[l], true, hl.sa);
+    if (l + 1 < L) {
+      build_sell(h.P[l], lay.lv[l + 1].perm, hl.iperm, {(int32_t)h.P[l].m}, false, hl.sp);
+      // prolongation is P^T of the restriction (SparseMatrix::Tvmult of P in
+      // AMGSolver::solve, src/AMGSolver.cpp:608-613), stored explicitly
+      build_sell(transpose(h.P[l]), hl.perm, lay.lv[l + 1].iperm, {(int32_t)h.A[l].m}, false, hl.st);
+    }
+  }
+  {
+    // level 0 in the natural numbering for the CG
+    const Csr &A = h.A[0];
+    std::vector<int32_t> len(A.m, 0), ident(A.m);
+    for (int64_t i = 0; i < A.m; i++)
+      for (int64_t k = A.rp[i]; k < A.rp[i + 1]; k++)
+        if (A.col[k] != i && A.val[k] != 0.0) len[i]++;
+    lay.nat_rowmap.resize(A.m);
+    std::iota(lay.nat_rowmap.begin(), lay.nat_rowmap.end(), 0);
+    std::iota(ident.begin(), ident.end(), 0);
+    for (int64_t w = 0; w < A.m; w += NAT_WINDOW)
+      std::stable_sort(lay.nat_rowmap.begin() + w, lay.nat_rowmap.begin() + std::min<int64_t>(A.m, w + NAT_WINDOW),
+                       [&](int32_t a, int32_t b) { return len[a] > len[b]; });
+    build_sell(A, lay.nat_rowmap, ident, {(int32_t)A.m}, true, lay.nat);
+  }
+  if (h.full) {
+    // the coarse vectors are in the level's colour order: permute the inverse
+    const std::vector<int32_t> &pc = lay.lv[L - 1].perm;
+    const int64_t nc = (int64_t)pc.size();
+    lay.minv_perm.resize((size_t)nc * nc);
+    for (int64_t i = 0; i < nc; i++)
+      for (int64_t j = 0; j < nc; j++) lay.minv_perm[(size_t)i * nc + j] = h.minv[(size_t)pc[i] * nc + pc[j]];
+  }
+}
+
+// uploads a prepared layout (caller holds the context lock)
+static void upload_layout(afe_amg *g, HostLayout &lay) {
+  afe_ctx *ctx = g->ctx;
+  const Hierarchy &h = g->h;
+  const int L = (int)h.A.size();
+  drop_graphs(g);
+  g->lv.clear();
+  g->lv.resize(L);
+  for (int l = 0; l < L; l++) {
     DevLevel &d = g->lv[l];
-    Sell sa;
-    build_sell(h.A[l], perm[l], iperm[l], groups[l], true, sa);
+    HostLevel &hl = lay.lv[l];
+    d.ncol = hl.ncol;
+    d.n = hl.n;
+    Sell &sa = hl.sa;
     d.ns = (int)sa.slices.size();
     d.colour_slice = sa.group_slice;
     d.si.upload(ctx, sa.slices);
@@ -995,11 +1055,7 @@ static void upload_hierarchy(afe_amg *g) {
       d.invd32.upload(ctx, to_prec<float>(inv_of(sa.diag)));
     }
     if (l + 1 < L) {
-      Sell sp, st;
-      build_sell(h.P[l], perm[l + 1], iperm[l], {(int32_t)h.P[l].m}, false, sp);
-      // prolongation is P^T of the restriction (SparseMatrix::Tvmult of P in
-      // AMGSolver::solve, src/AMGSolver.cpp:608-613), stored explicitly
-      build_sell(transpose(h.P[l]), perm[l], iperm[l + 1], {(int32_t)h.A[l].m}, false, st);
+      Sell &sp = hl.sp, &st = hl.st;
       d.p_ns = (int)sp.slices.size();
       d.t_ns = (int)st.slices.size();
       d.p_si.upload(ctx, sp.slices);
@@ -1021,47 +1077,25 @@ static void upload_hierarchy(afe_amg *g) {
       d.b32.alloc(ctx, d.n);
       d.t32.alloc(ctx, d.n);
     }
-    d.perm = perm[l];
+    d.perm = hl.perm;
     // stream (non-temporal) when the level's FP32 matrix exceeds the cache
     d.stream = (int64_t)sa.cols.size() * 8 > g_tune.stream_bytes;
   }
-  g->perm0.upload(ctx, perm[0]);
-  g->iperm0.upload(ctx, iperm[0]);
-  {
-    const Csr &A = h.A[0];
-    std::vector<int32_t> len(A.m, 0), nat(A.m), ident(A.m);
-    for (int64_t i = 0; i < A.m; i++)
-      for (int64_t k = A.rp[i]; k < A.rp[i + 1]; k++)
-        if (A.col[k] != i && A.val[k] != 0.0) len[i]++;
-    std::iota(nat.begin(), nat.end(), 0);
-    std::iota(ident.begin(), ident.end(), 0);
-    for (int64_t w = 0; w < A.m; w += NAT_WINDOW)
-      std::stable_sort(nat.begin() + w, nat.begin() + std::min<int64_t>(A.m, w + NAT_WINDOW),
-                       [&](int32_t a, int32_t b) { return len[a] > len[b]; });
-    Sell sn;
-    build_sell(A, nat, ident, {(int32_t)A.m}, true, sn);
-    g->nat_ns = (int)sn.slices.size();
-    g->nat_si.upload(ctx, sn.slices);
-    g->nat_cols.upload(ctx, sn.cols);
-    g->nat_v64.upload(ctx, sn.vals);
-    g->nat_diag.upload(ctx, sn.diag);
-    g->nat_rowmap.upload(ctx, nat);
-  }
+  g->perm0.upload(ctx, lay.lv[0].perm);
+  g->iperm0.upload(ctx, lay.lv[0].iperm);
+  g->nat_ns = (int)lay.nat.slices.size();
+  g->nat_si.upload(ctx, lay.nat.slices);
+  g->nat_cols.upload(ctx, lay.nat.cols);
+  g->nat_v64.upload(ctx, lay.nat.vals);
+  g->nat_diag.upload(ctx, lay.nat.diag);
+  g->nat_rowmap.upload(ctx, lay.nat_rowmap);
   // first level of the coarse tail; a large dense coarsest level is solved by
   // the multi-block gemv between the tail's down and up legs
   g->coarse_mode = h.full ? 0 : 1;
   g->big_coarse = h.full && g->lv[L - 1].n > COARSE_TAIL_MAX;
   g->tail0 = L - 1;
   while (g->tail0 > 0 && g->lv[g->tail0 - 1].n <= g_tune.tail_rows) g->tail0--;
-  if (h.full) {
-    // the coarse vectors are in the level's colour order: permute the inverse
-    const std::vector<int32_t> &pc = perm[L - 1];
-    const int64_t nc = (int64_t)pc.size();
-    std::vector<double> mp((size_t)nc * nc);
-    for (int64_t i = 0; i < nc; i++)
-      for (int64_t j = 0; j < nc; j++) mp[(size_t)i * nc + j] = h.minv[(size_t)pc[i] * nc + pc[j]];
-    g->minv.upload(ctx, mp);
-  }
+  if (h.full) g->minv.upload(ctx, lay.minv_perm);
   // level descriptors for the tail kernel
   std::vector<LevelDev<double>> d64(L);
   std::vector<LevelDev<float>> d32(L);
@@ -1101,11 +1135,15 @@ static void upload_hierarchy(afe_amg *g) {
   HIPCHK(hipStreamSynchronize(ctx->stream));
 }
 
+static void upload_hierarchy(afe_amg *g) {
+  HostLayout lay;
+  build_layout(g->h, lay);
+  upload_layout(g, lay);
+}
+
 extern "C" int afe_amg_create(afe_matrix *a, const afe_amg_params *p, afe_amg **out) {
   API_BEGIN
   if (!a || !out) return fail(AFE_E_ARG, "null argument");
-  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
-  HIPCHK(hipSetDevice(a->ctx->device));
   auto g = std::make_unique<afe_amg>();
   g->ctx = a->ctx;
   g->A0 = a;
@@ -1119,10 +1157,21 @@ extern "C" int afe_amg_create(afe_matrix *a, const afe_amg_params *p, afe_amg **
   hp.nmpo = prm.nmpo;
   hp.nmpsd = prm.nmpsd;
   hp.lazy = prm.lazy;
-  Csr A0 = fine_csr(a);
+  Csr A0;
+  {
+    std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+    HIPCHK(hipSetDevice(a->ctx->device));
+    A0 = fine_csr(a);
+  }
+  // the coarsening and the device layout are host work: the context stays
+  // free for other callers meanwhile
   amg_build(A0, hp, g->h);
+  HostLayout lay;
+  build_layout(g->h, lay);
+  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  HIPCHK(hipSetDevice(a->ctx->device));
   HIPCHK(hipHostMalloc((void **)&g->h_sc, sizeof(CgScalars), hipHostMallocDefault));
-  upload_hierarchy(g.get());
+  upload_layout(g.get(), lay);
   *out = g.release();
   API_END
 }
@@ -1192,12 +1241,21 @@ extern "C" int afe_amg_lazy_reinit(afe_amg *g, afe_matrix *a) {
   if (!g->prm.lazy) return fail(AFE_E_ARG, "hierarchy was not built in lazy mode");
   if (a->s != g->A0->s || (a->s == nullptr && a != g->A0))
     return fail(AFE_E_PATTERN, "The solver is not initialized with the same sparsity pattern.");
+  Csr A0;
+  {
+    std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+    HIPCHK(hipSetDevice(g->ctx->device));
+    A0 = fine_csr(a);
+  }
+  // host Galerkin products with the kept P and the new layout, unlocked (the
+  // caller must not use this hierarchy concurrently)
+  amg_lazy_refresh(g->h, A0);
+  HostLayout lay;
+  build_layout(g->h, lay);
   std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
   HIPCHK(hipSetDevice(g->ctx->device));
   g->A0 = a;
-  Csr A0 = fine_csr(a);
-  amg_lazy_refresh(g->h, A0);
-  upload_hierarchy(g);
+  upload_layout(g, lay);
   API_END
 }
 
@@ -1740,3 +1798,4 @@ extern "C" int afe_set_tuning(int32_t xcd, int32_t spmv_blocks, int64_t tail_row
 }
 
 #include "batch_impl.hpp"
+#include "server_impl.hpp"

@@ -1,0 +1,420 @@
+// server_impl.hpp -- a solve server on one device (part of capi.hip).
+//
+// Requests (structured meshes, optionally a moving-mesh update of a cached
+// mesh) are set up by a pool of host threads -- mesh, FE space, device
+// assembly, right-hand side, Dirichlet elimination and the lazy AMG
+// hierarchy, or for a cached mesh only the new coordinates, the values and
+// AMGSolver::lazyReinit -- and then admitted into one continuously running
+// batch (afe_batch_*): requests that arrive while others are solving join at
+// the next step, finished ones leave, and every step advances all of them
+// with shared launches.  Meshes, matrices and hierarchies stay cached in HBM
+// under a byte budget (least recently used idle entries are evicted).
+
+namespace {
+
+struct SrvEntry {
+  int64_t key = 0;
+  int32_t kind = 0, n = 0;
+  double jitter = 0;
+  uint64_t seed = 0;
+  afe_space *sp = nullptr;
+  afe_matrix *A = nullptr;
+  afe_amg *g = nullptr;
+  double *d_b = nullptr, *d_x = nullptr;
+  std::vector<double> base_coords;
+  std::vector<int32_t> vmark;
+  int64_t n_dof = 0, est_bytes = 0;
+  bool busy = false;
+  uint64_t last_use = 0;
+};
+
+struct SrvRequest {
+  int64_t id = 0;
+  afe_request r{};
+  std::vector<double> fk, fphi, fw; // owned copies of the coefficient field
+  int state = 0;                    // 0 pending, 1 setup, 2 ready, 3 solving, 4 done
+  int status = AFE_OK;
+  SrvEntry *e = nullptr;
+  int32_t slot = -1;
+  std::vector<double> x; // solution (when keep_x)
+  afe_solve_stats st{};
+  std::chrono::steady_clock::time_point t_submit, t_done;
+};
+
+// structured-mesh sizes
+static void mesh_sizes(int kind, int n, int64_t &nv, int64_t &ne, int64_t &ndof, int64_t &nnz) {
+  const int64_t m = n;
+  if (kind == AFE_P1_TET) {
+    nv = (m + 1) * (m + 1) * (m + 1);
+    ne = 6 * m * m * m;
+    ndof = nv;
+    nnz = 15 * m * m * m + 21 * m * m + 9 * m + 1;
+  } else {
+    nv = (m + 1) * (m + 1);
+    ne = 2 * m * m;
+    ndof = kind == AFE_P2_TRI ? (2 * m + 1) * (2 * m + 1) : nv;
+    nnz = kind == AFE_P2_TRI ? 46 * m * m + 16 * m + 1 : 7 * m * m + 6 * m + 1;
+  }
+}
+
+// HBM of one cached problem: pattern (int64), values, gather map, the
+// hierarchy's sliced-ELL copies (FP64, FP32, int32 columns, natural copy),
+// transfer operators and about thirty vectors
+static int64_t entry_bytes(int kind, int n) {
+  int64_t nv, ne, ndof, nnz;
+  mesh_sizes(kind, n, nv, ne, ndof, nnz);
+  return 64 * nnz + 320 * ndof + 64 * ne;
+}
+
+} // namespace
+
+struct afe_server {
+  afe_ctx *ctx = nullptr;
+  afe_batch *batch = nullptr;
+  int64_t budget = 0, reserved = 0;
+  int max_slots = 0, active = 0;
+  std::map<int64_t, std::unique_ptr<SrvEntry>> cache;
+  std::map<int64_t, std::unique_ptr<SrvRequest>> reqs;
+  std::deque<SrvRequest *> pending, ready;
+  std::mutex m;
+  std::condition_variable cv_work, cv_done;
+  std::vector<std::thread> workers;
+  std::thread engine;
+  bool stop = false;
+  int64_t next_id = 1, completed = 0, evictions = 0;
+  uint64_t clock = 0;
+  std::string engine_error;
+};
+
+static void srv_free_entry(afe_server *s, SrvEntry *e) {
+  if (e->g) afe_amg_destroy(e->g);
+  if (e->A) afe_matrix_destroy(e->A);
+  if (e->sp) afe_space_destroy(e->sp);
+  if (e->d_b) afe_dev_free(s->ctx, e->d_b);
+  if (e->d_x) afe_dev_free(s->ctx, e->d_x);
+  e->g = nullptr; e->A = nullptr; e->sp = nullptr; e->d_b = e->d_x = nullptr;
+}
+
+// bytes held by the cache (estimated) + in-flight setups
+static int64_t srv_bytes(afe_server *s) {
+  int64_t b = s->reserved;
+  for (auto &kv : s->cache)
+    if (kv.second->g) b += kv.second->est_bytes;
+  return b;
+}
+
+// make room for `need` bytes: evict idle entries, least recently used first
+// (caller holds s->m); false when only busy entries are left
+static bool srv_make_room(afe_server *s, int64_t need) {
+  while (srv_bytes(s) + need > s->budget) {
+    SrvEntry *victim = nullptr;
+    for (auto &kv : s->cache) {
+      SrvEntry *e = kv.second.get();
+      if (e->busy || !e->g) continue;
+      if (!victim || e->last_use < victim->last_use) victim = e;
+    }
+    if (!victim) return false;
+    srv_free_entry(s, victim);
+    s->evictions++;
+    s->cache.erase(victim->key);
+  }
+  return true;
+}
+
+static int srv_setup(afe_server *s, SrvRequest *q) {
+  SrvEntry *e = q->e;
+  const afe_request &r = q->r;
+  afe_field coef = r.coef;
+  coef.k = q->fk.empty() ? nullptr : q->fk.data();
+  coef.phi = q->fphi.empty() ? nullptr : q->fphi.data();
+  coef.w = q->fw.empty() ? nullptr : q->fw.data();
+  const afe_field *cf = coef.n_modes > 0 ? &coef : nullptr;
+  const int dim = r.kind == AFE_P1_TET ? 3 : 2;
+  int rc;
+  if (!e->g) {
+    int64_t nv, ne, ndof, nnz;
+    mesh_sizes(r.kind, r.n, nv, ne, ndof, nnz);
+    e->base_coords.resize((size_t)nv * dim);
+    e->vmark.resize(nv);
+    std::vector<int32_t> ev((size_t)ne * (dim + 1));
+    rc = r.kind == AFE_P1_TET ? afe_mesh_cube(r.n, r.jitter, r.seed, e->base_coords.data(), ev.data(), e->vmark.data())
+                              : afe_mesh_square(r.n, r.jitter, r.seed, e->base_coords.data(), ev.data(), e->vmark.data());
+    if (rc) return rc;
+    std::vector<double> c = e->base_coords;
+    if (r.move_amp != 0) afe_mesh_displace(dim, nv, c.data(), e->vmark.data(), r.move_amp / r.n, r.move_k);
+    if ((rc = afe_space_create(s->ctx, r.kind, nv, c.data(), ne, ev.data(), e->vmark.data(), &e->sp))) return rc;
+    if ((rc = afe_matrix_create(e->sp, &e->A))) return rc;
+    int32_t nl;
+    int64_t nz;
+    afe_space_info(e->sp, &e->n_dof, &nz, &nl);
+    if ((rc = afe_dev_alloc(s->ctx, e->n_dof * 8, (void **)&e->d_b))) return rc;
+    if ((rc = afe_dev_alloc(s->ctx, e->n_dof * 8, (void **)&e->d_x))) return rc;
+    if ((rc = afe_assemble(e->A, r.op, r.acc, cf))) return rc;
+    if ((rc = afe_rhs(e->sp, r.acc, r.fkind, cf, nullptr, e->d_b))) return rc;
+    if ((rc = afe_apply_dirichlet(e->A, nullptr, e->d_b))) return rc;
+    afe_amg_params p{0.25, 50u, 0.382, 1, 1, 1};
+    if ((rc = afe_amg_create(e->A, &p, &e->g))) return rc;
+    afe_amg_set_tail_rows(e->g, g_tune.tail_rows);
+  } else {
+    // moving-mesh re-solve: new coordinates, same connectivity and pattern;
+    // values-only assembly and the lazy hierarchy refresh
+    std::vector<double> c = e->base_coords;
+    const int64_t nv = (int64_t)e->vmark.size();
+    if (r.move_amp != 0) afe_mesh_displace(dim, nv, c.data(), e->vmark.data(), r.move_amp / r.n, r.move_k);
+    if ((rc = afe_space_update_coords(e->sp, c.data()))) return rc;
+    if ((rc = afe_assemble(e->A, r.op, r.acc, cf))) return rc;
+    if ((rc = afe_rhs(e->sp, r.acc, r.fkind, cf, nullptr, e->d_b))) return rc;
+    if ((rc = afe_apply_dirichlet(e->A, nullptr, e->d_b))) return rc;
+    if ((rc = afe_amg_lazy_reinit(e->g, e->A))) return rc;
+  }
+  return afe_dev_zero(s->ctx, e->d_x, e->n_dof * 8);
+}
+
+static void srv_worker(afe_server *s) {
+  std::unique_lock<std::mutex> lk(s->m);
+  while (!s->stop) {
+    // first pending request whose mesh is not being solved or set up
+    SrvRequest *q = nullptr;
+    for (auto it = s->pending.begin(); it != s->pending.end(); ++it) {
+      auto ce = s->cache.find((*it)->r.mesh_key);
+      if (ce != s->cache.end() && ce->second->busy) continue;
+      q = *it;
+      s->pending.erase(it);
+      break;
+    }
+    if (!q) {
+      s->cv_work.wait(lk);
+      continue;
+    }
+    q->state = 1;
+    auto ce = s->cache.find(q->r.mesh_key);
+    SrvEntry *e;
+    bool fresh = false;
+    if (ce != s->cache.end() && ce->second->g && ce->second->kind == q->r.kind && ce->second->n == q->r.n &&
+        ce->second->jitter == q->r.jitter && ce->second->seed == q->r.seed) {
+      e = ce->second.get();
+    } else {
+      if (ce != s->cache.end()) { // same key, different mesh: replace
+        srv_free_entry(s, ce->second.get());
+        s->cache.erase(ce);
+      }
+      std::unique_ptr<SrvEntry> ne(new SrvEntry);
+      ne->key = q->r.mesh_key;
+      ne->kind = q->r.kind;
+      ne->n = q->r.n;
+      ne->jitter = q->r.jitter;
+      ne->seed = q->r.seed;
+      ne->est_bytes = entry_bytes(q->r.kind, q->r.n);
+      e = ne.get();
+      s->cache[ne->key] = std::move(ne);
+      fresh = true;
+    }
+    e->busy = true;
+    q->e = e;
+    int64_t need = fresh ? e->est_bytes : 0;
+    // HBM admission: wait until the budget has room (finished requests free
+    // their entries for eviction)
+    while (fresh && !s->stop && !srv_make_room(s, need)) s->cv_done.wait(lk);
+    if (s->stop) break;
+    s->reserved += need;
+    lk.unlock();
+    const int rc = srv_setup(s, q);
+    lk.lock();
+    s->reserved -= need;
+    if (rc != AFE_OK) {
+      q->status = rc;
+      q->state = 4;
+      q->t_done = std::chrono::steady_clock::now();
+      srv_free_entry(s, e);
+      s->cache.erase(e->key);
+      s->completed++;
+      s->cv_done.notify_all();
+      continue;
+    }
+    q->state = 2;
+    s->ready.push_back(q);
+    s->cv_work.notify_all();
+  }
+}
+
+static void srv_engine(afe_server *s) {
+  std::vector<SrvRequest *> solving;
+  std::unique_lock<std::mutex> lk(s->m);
+  while (!s->stop) {
+    // admit every ready request while slots are free
+    while (!s->ready.empty() && (int)solving.size() < s->max_slots) {
+      SrvRequest *q = s->ready.front();
+      s->ready.pop_front();
+      const int rc = afe_batch_add(s->batch, q->e->g, q->e->d_x, q->e->d_b, q->r.rtol, q->r.max_it, &q->slot);
+      if (rc != AFE_OK) {
+        q->status = rc;
+        q->state = 4;
+        q->t_done = std::chrono::steady_clock::now();
+        q->e->busy = false;
+        s->completed++;
+        s->cv_done.notify_all();
+        continue;
+      }
+      q->state = 3;
+      solving.push_back(q);
+    }
+    if (solving.empty()) {
+      s->cv_work.wait(lk);
+      continue;
+    }
+    lk.unlock();
+    int32_t na = 0;
+    const int rc = afe_batch_step(s->batch, &na);
+    lk.lock();
+    if (rc != AFE_OK) s->engine_error = afe_last_error();
+    for (size_t k = 0; k < solving.size();) {
+      SrvRequest *q = solving[k];
+      int32_t done = 0;
+      const int st = afe_batch_result(s->batch, q->slot, &done, &q->st);
+      if (!done && rc == AFE_OK) {
+        k++;
+        continue;
+      }
+      q->status = rc != AFE_OK ? rc : st;
+      if (q->r.keep_x) {
+        q->x.resize(q->e->n_dof);
+        afe_dev_download(s->ctx, q->x.data(), q->e->d_x, q->e->n_dof * 8);
+      }
+      afe_batch_remove(s->batch, q->slot);
+      q->slot = -1;
+      q->state = 4;
+      q->t_done = std::chrono::steady_clock::now();
+      q->e->busy = false;
+      q->e->last_use = ++s->clock;
+      s->completed++;
+      solving[k] = solving.back();
+      solving.pop_back();
+      s->cv_done.notify_all();
+      s->cv_work.notify_all();
+    }
+  }
+}
+
+extern "C" int afe_server_create(int32_t device, int64_t hbm_budget, int32_t max_slots, int32_t setup_threads,
+                                 int32_t smooth, afe_server **out) {
+  API_BEGIN
+  if (!out || hbm_budget <= 0 || max_slots < 1 || setup_threads < 1 || setup_threads > 64 || smooth < 1)
+    return fail(AFE_E_ARG, "bad server arguments");
+  std::unique_ptr<afe_server> s(new afe_server);
+  int rc = afe_ctx_create(device, &s->ctx);
+  if (rc) return rc;
+  if ((rc = afe_batch_create(s->ctx, AFE_PREC_FP32, smooth, &s->batch))) {
+    afe_ctx_destroy(s->ctx);
+    return rc;
+  }
+  s->budget = hbm_budget;
+  s->max_slots = max_slots;
+  afe_server *sp = s.get();
+  s->engine = std::thread(srv_engine, sp);
+  for (int i = 0; i < setup_threads; i++) s->workers.emplace_back(srv_worker, sp);
+  *out = s.release();
+  API_END
+}
+
+extern "C" int afe_server_destroy(afe_server *s) {
+  if (!s) return AFE_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->m);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  s->cv_done.notify_all();
+  for (auto &t : s->workers) t.join();
+  s->engine.join();
+  afe_batch_destroy(s->batch);
+  for (auto &kv : s->cache) srv_free_entry(s, kv.second.get());
+  s->cache.clear();
+  afe_ctx_destroy(s->ctx);
+  delete s;
+  return AFE_OK;
+}
+
+extern "C" int afe_server_submit(afe_server *s, const afe_request *r, int64_t *id) {
+  API_BEGIN
+  if (!s || !r || !id) return fail(AFE_E_ARG, "null argument");
+  if (r->kind < AFE_P1_TRI || r->kind > AFE_P1_TET || r->n < 1 || r->rtol <= 0 || r->max_it < 1 ||
+      (r->op != AFE_OP_STIFF && r->op != AFE_OP_VARSTIFF) ||
+      (r->fkind != AFE_F_POISSON && r->fkind != AFE_F_VARCOEF && r->fkind != AFE_F_ONE))
+    return fail(AFE_E_ARG, "bad request");
+  if ((r->op == AFE_OP_VARSTIFF || r->fkind == AFE_F_VARCOEF) &&
+      (r->coef.n_modes < 1 || !r->coef.k || !r->coef.phi || !r->coef.w))
+    return fail(AFE_E_ARG, "variable coefficient without a field");
+  const int64_t est = entry_bytes(r->kind, r->n);
+  if (est > s->budget) return fail(AFE_E_BUDGET, "request larger than the HBM budget");
+  std::unique_ptr<SrvRequest> q(new SrvRequest);
+  q->r = *r;
+  if (r->coef.n_modes > 0) {
+    const int nm = r->coef.n_modes, dm = r->coef.dim;
+    q->fk.assign(r->coef.k, r->coef.k + (size_t)nm * dm);
+    q->fphi.assign(r->coef.phi, r->coef.phi + nm);
+    q->fw.assign(r->coef.w, r->coef.w + nm);
+  }
+  q->t_submit = std::chrono::steady_clock::now();
+  std::lock_guard<std::mutex> lk(s->m);
+  q->id = s->next_id++;
+  *id = q->id;
+  s->pending.push_back(q.get());
+  s->reqs[q->id] = std::move(q);
+  s->cv_work.notify_all();
+  API_END
+}
+
+extern "C" int afe_server_poll(afe_server *s, int64_t id, int32_t *done) {
+  if (!s || !done) return fail(AFE_E_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(s->m);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return fail(AFE_E_ARG, "unknown request");
+  *done = it->second->state == 4 ? 1 : 0;
+  return AFE_OK;
+}
+
+extern "C" int afe_server_wait(afe_server *s, int64_t id, double *h_x, int64_t *n_dof, afe_solve_stats *st,
+                               double *latency_s) {
+  if (!s) return fail(AFE_E_ARG, "null argument");
+  std::unique_lock<std::mutex> lk(s->m);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return fail(AFE_E_ARG, "unknown request");
+  SrvRequest *q = it->second.get();
+  while (q->state != 4 && !s->stop) s->cv_done.wait(lk);
+  if (q->state != 4) return fail(AFE_E_ARG, "server stopped");
+  if (n_dof) *n_dof = q->e ? q->e->n_dof : 0;
+  if (h_x && !q->x.empty()) std::memcpy(h_x, q->x.data(), q->x.size() * sizeof(double));
+  if (st) *st = q->st;
+  if (latency_s) *latency_s = seconds_between(q->t_submit, q->t_done);
+  if (q->status != AFE_OK) {
+    if (q->status == AFE_E_NOTCONVERGED) return fail(AFE_E_NOTCONVERGED, "request did not reach its tolerance");
+    return fail(q->status, "request failed");
+  }
+  return AFE_OK;
+}
+
+extern "C" int afe_server_release(afe_server *s, int64_t id) {
+  if (!s) return fail(AFE_E_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(s->m);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return fail(AFE_E_ARG, "unknown request");
+  if (it->second->state != 4) return fail(AFE_E_ARG, "request still running");
+  s->reqs.erase(it);
+  return AFE_OK;
+}
+
+extern "C" int afe_server_info(afe_server *s, int64_t *device_bytes, int64_t *cached, int64_t *completed,
+                               int64_t *evictions) {
+  if (!s) return fail(AFE_E_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(s->m);
+  if (device_bytes) {
+    std::lock_guard<std::recursive_mutex> lc(s->ctx->mu);
+    int64_t u = 0;
+    for (auto &kv : s->ctx->user_bytes) u += kv.second;
+    *device_bytes = s->ctx->bytes + u;
+  }
+  if (cached) *cached = (int64_t)s->cache.size();
+  if (completed) *completed = s->completed;
+  if (evictions) *evictions = s->evictions;
+  return AFE_OK;
+}

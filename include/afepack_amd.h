@@ -117,7 +117,8 @@ int afe_mesh_square(int32_t n, double jitter, uint64_t seed, double *coords,
  * Sizes: (n+1)^3 vertices, 6 n^3 tetrahedra. */
 int afe_mesh_cube(int32_t n, double jitter, uint64_t seed, double *coords,
                   int32_t *elem_vert, int32_t *vert_mark);
-/* displace interior vertices by amp*h*sin(2 pi k x_d) (moving-mesh update) */
+/* displace interior vertices by amp_h*sin(2 pi k x_d) per coordinate
+ * (moving-mesh update; amp_h is absolute, e.g. 0.1 h) */
 int afe_mesh_displace(int32_t dim, int64_t n_vert, double *coords,
                       const int32_t *vert_mark, double amp_h, int32_t k);
 /* seeded n_modes-mode field with |g| <= amplitude; k, phi, w are outputs of
@@ -285,6 +286,50 @@ int afe_batch_run(afe_batch *b);
 int afe_batch_result(afe_batch *b, int32_t slot, int32_t *done,
                      afe_solve_stats *st);
 int afe_batch_remove(afe_batch *b, int32_t slot);
+
+/* ---- solve server (no counterpart in the reference) ----------------------
+ * One device, a pool of host set-up threads and one continuously running
+ * batch.  A request names a structured mesh (n x n squares split in two
+ * triangles, or n^3 cubes split in six tetrahedra, interior jitter and
+ * seed) by a cache key; a later request with the same key re-solves that
+ * mesh after a moving-mesh update (interior vertices displaced by
+ * move_amp * h * sin(2 pi move_k x_d) from the base mesh): the pattern,
+ * gather map and lazy hierarchy are kept, only values are reassembled and
+ * AMGSolver::lazyReinit refreshes the Galerkin products.  Requests are set
+ * up concurrently, admitted into the batch as slots free up, and solved
+ * with FP64 CG + FP32 V-cycle to rtol; cached meshes stay in HBM within
+ * hbm_budget bytes (idle ones evicted, least recently used first).  A
+ * request's result does not depend on the requests solved with it. */
+typedef struct afe_request {
+  int64_t mesh_key;  /* cache key of the mesh                              */
+  int32_t kind;      /* AFE_P1_TRI, AFE_P2_TRI, AFE_P1_TET                  */
+  int32_t n;         /* squares (cubes) per side                            */
+  double jitter;     /* interior jitter of the base mesh, fraction of h    */
+  uint64_t seed;     /* base-mesh seed                                     */
+  double move_amp;   /* moving-mesh displacement amplitude / h (0: none)   */
+  int32_t move_k;    /* its wave number                                    */
+  int32_t acc;       /* quadrature accuracy                                */
+  int32_t op;        /* AFE_OP_STIFF or AFE_OP_VARSTIFF                     */
+  int32_t fkind;     /* AFE_F_POISSON, AFE_F_VARCOEF or AFE_F_ONE           */
+  afe_field coef;    /* coefficient field (copied at submit)               */
+  double rtol;       /* relative residual target                           */
+  int32_t max_it;    /* CG iteration limit                                 */
+  int32_t keep_x;    /* 1: keep the solution on the host for afe_server_wait */
+} afe_request;
+typedef struct afe_server afe_server;
+int afe_server_create(int32_t device, int64_t hbm_budget, int32_t max_slots,
+                      int32_t setup_threads, int32_t smooth,
+                      afe_server **out);
+int afe_server_destroy(afe_server *s);
+int afe_server_submit(afe_server *s, const afe_request *r, int64_t *id);
+int afe_server_poll(afe_server *s, int64_t id, int32_t *done);
+/* blocks until the request is done; latency = completion - submission;
+ * h_x (n_dof doubles, may be NULL) receives the solution if keep_x */
+int afe_server_wait(afe_server *s, int64_t id, double *h_x, int64_t *n_dof,
+                    afe_solve_stats *st, double *latency_s);
+int afe_server_release(afe_server *s, int64_t id);
+int afe_server_info(afe_server *s, int64_t *device_bytes, int64_t *cached,
+                    int64_t *completed, int64_t *evictions);
 
 /* ---- measurement hooks (no counterpart in the reference) ------------------ */
 /* average duration of `reps` back-to-back launches of a level-0 kernel as the
