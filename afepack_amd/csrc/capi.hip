@@ -871,6 +871,12 @@ struct DevLevel {
   DevBuf<float> x32, b32, t32;
   std::vector<int32_t> perm; // new -> old
   bool stream = false;       // matrix larger than the caches: non-temporal loads
+  // value-only refresh of a lazy hierarchy: the level's CSR (levels >= 1),
+  // the restriction P and its transpose to the next level, and the source
+  // maps CSR entry -> layout position / stored row -> CSR diagonal
+  DevBuf<int64_t> c_rp, P_rp, Pt_rp;
+  DevBuf<int32_t> c_col, P_col, Pt_col, smap, dsrc;
+  DevBuf<double> c_val, P_val, Pt_val;
 };
 
 struct afe_amg {
@@ -893,8 +899,10 @@ struct afe_amg {
   // rows, nat_rowmap = stored row -> natural row
   int nat_ns = 0;
   DevBuf<SliceInfo> nat_si;
-  DevBuf<int32_t> nat_cols, nat_rowmap;
+  DevBuf<int32_t> nat_cols, nat_rowmap, nat_smap, nat_dsrc;
   DevBuf<double> nat_v64, nat_diag;
+  bool dev_refresh = false;      // device lazyReinit available (source maps uploaded)
+  bool host_vals_stale = false;  // host level values older than the device's
   // CG workspace (level-0 natural numbering)
   DevBuf<double> wx, wxl, wr, wp, wq, wz, wb, part;
   DevBuf<CgScalars> sc;
@@ -952,16 +960,19 @@ struct HostLevel {
   int64_t n = 0;
   Sell sa, sp, st;
   std::vector<int32_t> perm, iperm;
+  std::vector<int32_t> smap, dsrc; // CSR entry -> layout position, row -> CSR diagonal
 };
 struct HostLayout {
+  bool maps = false; // source maps for the value-only refresh
   std::vector<HostLevel> lv;
   Sell nat;
-  std::vector<int32_t> nat_rowmap;
+  std::vector<int32_t> nat_rowmap, nat_smap, nat_dsrc;
   std::vector<double> minv_perm;
 };
 
-static void build_layout(const Hierarchy &h, HostLayout &lay) {
+static void build_layout(const Hierarchy &h, HostLayout &lay, bool maps) {
   const int L = (int)h.A.size();
+  lay.maps = maps && h.A[0].nnz() < INT32_MAX;
   lay.lv.assign(L, HostLevel());
   std::vector<std::vector<int32_t>> groups(L);
   for (int l = 0; l < L; l++) {
@@ -972,7 +983,7 @@ static void build_layout(const Hierarchy &h, HostLayout &lay) {
     std::vector<int32_t> len(A.m, 0);
     for (int64_t i = 0; i < A.m; i++)
       for (int64_t k = A.rp[i]; k < A.rp[i + 1]; k++)
-        if (A.col[k] != i && A.val[k] != 0.0) len[i]++;
+        if (A.col[k] != i) len[i]++;
     // rows of one colour keep their index order in windows of COLOUR_WINDOW
     // rows (x locality of the gathers) and are sorted by stored length inside
     // a window (little sliced-ELL padding)
@@ -994,7 +1005,8 @@ static void build_layout(const Hierarchy &h, HostLayout &lay) {
   }
   for (int l = 0; l < L; l++) {
     HostLevel &hl = lay.lv[l];
-    build_sell(h.A[l], hl.perm, hl.iperm, groups[l], true, hl.sa);
+    build_sell(h.A[l], hl.perm, hl.iperm, groups[l], true, hl.sa, lay.maps ? &hl.smap : nullptr,
+               lay.maps ? &hl.dsrc : nullptr);
     if (l + 1 < L) {
       build_sell(h.P[l], lay.lv[l + 1].perm, hl.iperm, {(int32_t)h.P[l].m}, false, hl.sp);
       // prolongation is P^T of the restriction (SparseMatrix::Tvmult of P in
@@ -1008,14 +1020,15 @@ static void build_layout(const Hierarchy &h, HostLayout &lay) {
     std::vector<int32_t> len(A.m, 0), ident(A.m);
     for (int64_t i = 0; i < A.m; i++)
       for (int64_t k = A.rp[i]; k < A.rp[i + 1]; k++)
-        if (A.col[k] != i && A.val[k] != 0.0) len[i]++;
+        if (A.col[k] != i) len[i]++;
     lay.nat_rowmap.resize(A.m);
     std::iota(lay.nat_rowmap.begin(), lay.nat_rowmap.end(), 0);
     std::iota(ident.begin(), ident.end(), 0);
     for (int64_t w = 0; w < A.m; w += NAT_WINDOW)
       std::stable_sort(lay.nat_rowmap.begin() + w, lay.nat_rowmap.begin() + std::min<int64_t>(A.m, w + NAT_WINDOW),
                        [&](int32_t a, int32_t b) { return len[a] > len[b]; });
-    build_sell(A, lay.nat_rowmap, ident, {(int32_t)A.m}, true, lay.nat);
+    build_sell(A, lay.nat_rowmap, ident, {(int32_t)A.m}, true, lay.nat, lay.maps ? &lay.nat_smap : nullptr,
+               lay.maps ? &lay.nat_dsrc : nullptr);
   }
   if (h.full) {
     // the coarse vectors are in the level's colour order: permute the inverse
@@ -1089,6 +1102,30 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
   g->nat_v64.upload(ctx, lay.nat.vals);
   g->nat_diag.upload(ctx, lay.nat.diag);
   g->nat_rowmap.upload(ctx, lay.nat_rowmap);
+  g->dev_refresh = lay.maps;
+  g->host_vals_stale = false;
+  if (lay.maps) {
+    g->nat_smap.upload(ctx, lay.nat_smap);
+    g->nat_dsrc.upload(ctx, lay.nat_dsrc);
+    for (int l = 0; l < L; l++) {
+      DevLevel &d = g->lv[l];
+      d.smap.upload(ctx, lay.lv[l].smap);
+      d.dsrc.upload(ctx, lay.lv[l].dsrc);
+      if (l > 0) {
+        d.c_rp.upload(ctx, h.A[l].rp);
+        d.c_col.upload(ctx, h.A[l].col);
+        d.c_val.upload(ctx, h.A[l].val);
+      }
+      if (l + 1 < L) {
+        d.P_rp.upload(ctx, h.P[l].rp);
+        d.P_col.upload(ctx, h.P[l].col);
+        d.P_val.upload(ctx, h.P[l].val);
+        d.Pt_rp.upload(ctx, h.Pt[l].rp);
+        d.Pt_col.upload(ctx, h.Pt[l].col);
+        d.Pt_val.upload(ctx, h.Pt[l].val);
+      }
+    }
+  }
   // first level of the coarse tail; a large dense coarsest level is solved by
   // the multi-block gemv between the tail's down and up legs
   g->coarse_mode = h.full ? 0 : 1;
@@ -1137,7 +1174,7 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
 
 static void upload_hierarchy(afe_amg *g) {
   HostLayout lay;
-  build_layout(g->h, lay);
+  build_layout(g->h, lay, g->prm.lazy != 0);
   upload_layout(g, lay);
 }
 
@@ -1167,7 +1204,7 @@ extern "C" int afe_amg_create(afe_matrix *a, const afe_amg_params *p, afe_amg **
   // free for other callers meanwhile
   amg_build(A0, hp, g->h);
   HostLayout lay;
-  build_layout(g->h, lay);
+  build_layout(g->h, lay, prm.lazy != 0);
   std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
   HIPCHK(hipHostMalloc((void **)&g->h_sc, sizeof(CgScalars), hipHostMallocDefault));
@@ -1235,12 +1272,79 @@ extern "C" int afe_amg_destroy(afe_amg *g) {
   return AFE_OK;
 }
 
+// AMGSolver::lazyReinit on the device: Galerkin products level by level with
+// the kept P (bit-identical to the host product), the layouts' values through
+// their source maps, and the coarsest inverse on the host from the new
+// coarsest values.  Layouts, launch tables and captured graphs stay valid.
+static void device_lazy_refresh(afe_amg *g, afe_matrix *a) {
+  afe_ctx *ctx = g->ctx;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  Hierarchy &h = g->h;
+  const int L = (int)g->lv.size();
+  auto lvals = [&](int l) -> const double * { return l == 0 ? a->val.p : g->lv[l].c_val.p; };
+  for (int l = 0; l + 1 < L; l++) {
+    DevLevel &d = g->lv[l];
+    DevLevel &dn = g->lv[l + 1];
+    const int64_t *arp = l == 0 ? a->rp() : d.c_rp.p;
+    const int32_t *acol = l == 0 ? a->col() : d.c_col.p;
+    const int64_t nc = dn.n;
+    k_galerkin_rows<<<nblk(nc), 256, 0, s>>>(nc, d.P_rp.p, d.P_col.p, d.P_val.p, arp, acol, lvals(l), d.Pt_rp.p,
+                                              d.Pt_col.p, d.Pt_val.p, dn.c_rp.p, dn.c_col.p, dn.c_val.p);
+  }
+  for (int l = 0; l < L; l++) {
+    DevLevel &d = g->lv[l];
+    const int64_t nnz = h.A[l].nnz();
+    k_refresh_vals<<<nblk(nnz), 256, 0, s>>>(nnz, lvals(l), d.smap.p, d.v64.p, g->has32 ? d.v32.p : nullptr);
+    k_refresh_diag<<<nblk(d.n), 256, 0, s>>>(d.n, lvals(l), d.dsrc.p, d.diag64.p, d.invd64.p,
+                                             g->has32 ? d.diag32.p : nullptr, g->has32 ? d.invd32.p : nullptr);
+  }
+  const int64_t n0 = g->lv[0].n, nnz0 = h.A[0].nnz();
+  k_refresh_vals<<<nblk(nnz0), 256, 0, s>>>(nnz0, a->val.p, g->nat_smap.p, g->nat_v64.p, nullptr);
+  k_refresh_diag<<<nblk(n0), 256, 0, s>>>(n0, a->val.p, g->nat_dsrc.p, g->nat_diag.p, nullptr, nullptr, nullptr);
+  HIPCHK(hipGetLastError());
+  if (h.full) {
+    Csr &C = h.A[L - 1];
+    HIPCHK(hipMemcpyAsync(C.val.data(), lvals(L - 1), C.nnz() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    coarse_inverse(h);
+    const std::vector<int32_t> &pc = g->lv[L - 1].perm;
+    const int64_t nc = (int64_t)pc.size();
+    std::vector<double> mp((size_t)nc * nc);
+    for (int64_t i = 0; i < nc; i++)
+      for (int64_t j = 0; j < nc; j++) mp[(size_t)i * nc + j] = h.minv[(size_t)pc[i] * nc + pc[j]];
+    g->minv.upload(ctx, mp);
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  g->A0 = a;
+  g->host_vals_stale = true;
+}
+
+// host copies of the level values after device refreshes
+static void sync_host_values(afe_amg *g) {
+  if (!g->host_vals_stale || !g->ctx) return;
+  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  hipStream_t s = g->ctx->stream;
+  for (size_t l = 0; l < g->lv.size(); l++) {
+    Csr &A = g->h.A[l];
+    const double *src = l == 0 ? g->A0->val.p : g->lv[l].c_val.p;
+    HIPCHK(hipMemcpyAsync(A.val.data(), src, A.nnz() * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  g->host_vals_stale = false;
+}
+
 extern "C" int afe_amg_lazy_reinit(afe_amg *g, afe_matrix *a) {
   API_BEGIN
   if (!g || !a) return fail(AFE_E_ARG, "null argument");
   if (!g->prm.lazy) return fail(AFE_E_ARG, "hierarchy was not built in lazy mode");
   if (a->s != g->A0->s || (a->s == nullptr && a != g->A0))
     return fail(AFE_E_PATTERN, "The solver is not initialized with the same sparsity pattern.");
+  if (g->dev_refresh) {
+    device_lazy_refresh(g, a);
+    return AFE_OK;
+  }
   Csr A0;
   {
     std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
@@ -1251,7 +1355,7 @@ extern "C" int afe_amg_lazy_reinit(afe_amg *g, afe_matrix *a) {
   // caller must not use this hierarchy concurrently)
   amg_lazy_refresh(g->h, A0);
   HostLayout lay;
-  build_layout(g->h, lay);
+  build_layout(g->h, lay, true);
   std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
   HIPCHK(hipSetDevice(g->ctx->device));
   g->A0 = a;
@@ -1278,6 +1382,13 @@ extern "C" int afe_amg_level_csr(const afe_amg *g, int32_t level, int32_t which,
   const int L = (int)g->h.A.size();
   if (level < 0 || level >= L || (which == 1 && level >= L - 1) || which < 0 || which > 1)
     return fail(AFE_E_ARG, "no such level matrix");
+  if (val && which == 0) {
+    try {
+      sync_host_values(const_cast<afe_amg *>(g));
+    } catch (const HipFail &h) {
+      return fail(AFE_E_HIP, std::string(h.what) + ": " + hipGetErrorString(h.e));
+    }
+  }
   const Csr &c = which == 0 ? g->h.A[level] : g->h.P[level];
   if (m) *m = c.m;
   if (n) *n = c.n;

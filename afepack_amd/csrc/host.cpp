@@ -601,7 +601,7 @@ int dense_inverse(int n, double *a) {
   return 0;
 }
 
-static void coarse_inverse(Hierarchy &h) {
+void coarse_inverse(Hierarchy &h) {
   const Csr &Ac = h.A.back();
   const int n = (int)Ac.m;
   h.minv.assign((size_t)n * n, 0.0);
@@ -656,11 +656,12 @@ void amg_lazy_refresh(Hierarchy &h, const Csr &A0) {
 // ===========================================================================
 int colour_rows(const Csr &a, std::vector<int32_t> &colour) {
   const int64_t n = a.m;
-  // symmetrised adjacency of the nonzero off-diagonal entries
+  // symmetrised adjacency of the off-diagonal pattern entries (explicit
+  // zeros included: the colouring stays valid for any values on the pattern)
   std::vector<int64_t> deg(n + 1, 0);
   for (int64_t i = 0; i < n; i++)
     for (int64_t k = a.rp[i]; k < a.rp[i + 1]; k++)
-      if (a.col[k] != i && a.val[k] != 0.0) {
+      if (a.col[k] != i) {
         deg[i + 1]++;
         deg[a.col[k] + 1]++;
       }
@@ -670,7 +671,7 @@ int colour_rows(const Csr &a, std::vector<int32_t> &colour) {
     std::vector<int64_t> fill(deg.begin(), deg.end() - 1);
     for (int64_t i = 0; i < n; i++)
       for (int64_t k = a.rp[i]; k < a.rp[i + 1]; k++)
-        if (a.col[k] != i && a.val[k] != 0.0) {
+        if (a.col[k] != i) {
           adj[fill[i]++] = a.col[k];
           adj[fill[a.col[k]]++] = (int32_t)i;
         }
@@ -698,15 +699,17 @@ int colour_rows(const Csr &a, std::vector<int32_t> &colour) {
 void build_sell(const Csr &a, const std::vector<int32_t> &perm,
                 const std::vector<int32_t> &col_map,
                 const std::vector<int32_t> &group_rows, bool split_diag,
-                Sell &out) {
+                Sell &out, std::vector<int32_t> *src_map,
+                std::vector<int32_t> *diag_src) {
   out = Sell();
   out.n_rows = (int64_t)perm.size();
   out.n_cols = a.n;
   if (split_diag) out.diag.assign(out.n_rows, 0.0);
+  // every pattern entry is stored (explicit zeros too), so that the layout
+  // holds any values on the pattern
   auto stored_len = [&](int64_t o) {
     int64_t c = 0;
     for (int64_t k = a.rp[o]; k < a.rp[o + 1]; k++) {
-      if (a.val[k] == 0.0) continue;
       if (split_diag && a.col[k] == o) continue;
       c++;
     }
@@ -731,6 +734,8 @@ void build_sell(const Csr &a, const std::vector<int32_t> &perm,
   if (off > INT32_MAX) throw Error{-1, "sliced-ELL storage exceeds 2^31 entries"};
   out.cols.assign(off, 0);
   out.vals.assign(off, 0.0);
+  if (src_map) src_map->assign(a.nnz(), -1);
+  if (diag_src) diag_src->assign(out.n_rows, -1);
 #pragma omp parallel for schedule(static, 64)
   for (int64_t s = 0; s < (int64_t)out.slices.size(); s++) {
     const SliceInfo si = out.slices[s];
@@ -741,11 +746,13 @@ void build_sell(const Csr &a, const std::vector<int32_t> &perm,
       for (int64_t k = a.rp[o]; k < a.rp[o + 1]; k++) {
         if (split_diag && a.col[k] == o) {
           out.diag[nr] = a.val[k];
+          if (diag_src) (*diag_src)[nr] = (int32_t)k;
           continue;
         }
-        if (a.val[k] == 0.0) continue;
-        out.cols[si.off + j * SELL_C + l] = col_map[a.col[k]];
-        out.vals[si.off + j * SELL_C + l] = a.val[k];
+        const int64_t pos = si.off + j * SELL_C + l;
+        out.cols[pos] = col_map[a.col[k]];
+        out.vals[pos] = a.val[k];
+        if (src_map) (*src_map)[k] = (int32_t)pos;
         j++;
       }
       for (; j < si.width; j++) {

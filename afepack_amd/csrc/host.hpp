@@ -90,6 +90,8 @@ struct Hierarchy {
 
 void amg_build(const Csr &A0, const AmgParams &prm, Hierarchy &h);
 void amg_lazy_refresh(Hierarchy &h, const Csr &A0);
+// the coarsest level's dense inverse from its current values (h.minv)
+void coarse_inverse(Hierarchy &h);
 Csr galerkin(const Csr &P, const Csr &A, const Csr &Pt);
 Csr transpose(const Csr &a);
 int dense_inverse(int n, double *a);
@@ -122,9 +124,12 @@ int colour_rows(const Csr &a, std::vector<int32_t> &colour);
 
 // perm: new -> old row order; col_map: old column -> new column
 // groups: row-count per group in new order (slices never cross groups)
+// src_map (optional): position of every CSR entry in the layout, -1 for a
+// split diagonal; diag_src (optional): CSR index of each stored row's diagonal
 void build_sell(const Csr &a, const std::vector<int32_t> &perm,
                 const std::vector<int32_t> &col_map,
                 const std::vector<int32_t> &group_rows, bool split_diag,
-                Sell &out);
+                Sell &out, std::vector<int32_t> *src_map = nullptr,
+                std::vector<int32_t> *diag_src = nullptr);
 
 } // namespace afe

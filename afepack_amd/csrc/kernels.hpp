@@ -883,6 +883,82 @@ __global__ __launch_bounds__(BLK) void k_residual_norms(const SliceInfo *__restr
   }
 }
 
+// ---------------------------------------------------------------------------
+// value-only refresh of a lazy hierarchy (AMGSolver::lazyReinit,
+// src/AMGSolver.cpp:505-545): the Galerkin products with the kept P on the
+// fixed coarse patterns, then the layouts' values from the CSR values
+// ---------------------------------------------------------------------------
+#pragma clang fp contract(off)
+// one coarse row per thread: C(i, :) = sum_j P(i, a) A(a, b) P^T(b, :), the
+// terms accumulated in the order of the host product (P row, A row, P^T row)
+// so that every coarse value is bit-identical to the host's; the coarse row
+// holds the diagonal first and the other columns ascending
+__global__ __launch_bounds__(256) void k_galerkin_rows(
+    int64_t nc, const int64_t *__restrict__ Prp, const int32_t *__restrict__ Pcol,
+    const double *__restrict__ Pval, const int64_t *__restrict__ Arp, const int32_t *__restrict__ Acol,
+    const double *__restrict__ Aval, const int64_t *__restrict__ Ptrp, const int32_t *__restrict__ Ptcol,
+    const double *__restrict__ Ptval, const int64_t *__restrict__ Crp, const int32_t *__restrict__ Ccol,
+    double *__restrict__ Cval) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  const int64_t c0 = Crp[i], c1 = Crp[i + 1];
+  for (int64_t k = c0; k < c1; k++) Cval[k] = 0.0;
+  for (int64_t j = Prp[i]; j < Prp[i + 1]; j++) {
+    const int64_t a = Pcol[j];
+    const double pa = Pval[j];
+    for (int64_t k = Arp[a]; k < Arp[a + 1]; k++) {
+      const int64_t b = Acol[k];
+      const double pab = pa * Aval[k];
+      for (int64_t l = Ptrp[b]; l < Ptrp[b + 1]; l++) {
+        const int32_t cc = Ptcol[l];
+        int64_t pos = -1;
+        if (cc == i) {
+          pos = c0;
+        } else {
+          int64_t lo = c0 + 1, hi = c1 - 1;
+          while (lo <= hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            const int32_t v = Ccol[mid];
+            if (v == cc) {
+              pos = mid;
+              break;
+            }
+            if (v < cc) lo = mid + 1;
+            else hi = mid - 1;
+          }
+        }
+        if (pos >= 0) Cval[pos] += pab * Ptval[l];
+      }
+    }
+  }
+}
+
+// layout values from CSR values through the source map (-1: split diagonal)
+__global__ __launch_bounds__(256) void k_refresh_vals(int64_t nnz, const double *__restrict__ val,
+                                                      const int32_t *__restrict__ smap, double *__restrict__ v64,
+                                                      float *__restrict__ v32) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const int32_t p = smap[k];
+  if (p < 0) return;
+  v64[p] = val[k];
+  if (v32) v32[p] = (float)val[k];
+}
+
+__global__ __launch_bounds__(256) void k_refresh_diag(int64_t n, const double *__restrict__ val,
+                                                      const int32_t *__restrict__ dsrc, double *__restrict__ diag64,
+                                                      double *__restrict__ invd64, float *__restrict__ diag32,
+                                                      float *__restrict__ invd32) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const double d = val[dsrc[r]];
+  diag64[r] = d;
+  if (invd64) invd64[r] = 1.0 / d;
+  if (diag32) diag32[r] = (float)d;
+  if (invd32) invd32[r] = (float)(1.0 / d);
+}
+#pragma clang fp contract(on)
+
 // error-free transformations (no contraction inside: fp contract is off in
 // this region; the product error uses an explicit fma)
 #pragma clang fp contract(off)

@@ -95,12 +95,17 @@ def main():
                     help="build the C2 system and hierarchy, then only time the level-0 kernels and a "
                          "plain copy (for profiler runs); prints one JSON line")
     ap.add_argument("--problems", type=int, default=256, help="c4: number of problems")
+    ap.add_argument("--rate", type=float, default=200.0, help="c5: arrivals per second")
+    ap.add_argument("--duration", type=float, default=60.0, help="c5: seconds of arrivals")
+    ap.add_argument("--setup-threads", type=int, default=12, help="c5: host set-up threads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this path runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
     if args.workload == "c4":
         return run_c4(args)
+    if args.workload == "c5":
+        return run_c5(args)
     return run_c2(args)
 
 
@@ -405,6 +410,118 @@ def run_c4(args):
         "sequential_dof_per_s": total / t_seq,
         "batched_vs_sequential": t_seq / statistics.mean(dts),
         "setup_s": t_setup,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def c5_request(rng, cached_keys, next_key):
+    """One request of the serving stream: 70 % 2D (P1 or P2), 30 % 3D P1,
+    DOFs log-uniform in [1e3, 2e6] rounded to the nearest structured mesh;
+    40 % re-solve a cached mesh after a moving-mesh update
+    0.1 h sin(2 pi k x), k in {1, 2, 3}."""
+    if cached_keys and rng.uniform() < 0.4:
+        key, spec = cached_keys[int(rng.integers(len(cached_keys)))]
+        return key, dict(spec, move_amp=0.1, move_k=int(rng.integers(1, 4))), True
+    u = rng.uniform()
+    N = math.exp(rng.uniform(math.log(1e3), math.log(2e6)))
+    if u < 0.3:
+        kind, n, acc = 3, max(2, round(N ** (1.0 / 3.0) - 1)), 2
+    elif u < 0.65:
+        kind, n, acc = 1, max(2, round(math.sqrt(N) - 1)), 4
+    else:
+        kind, n, acc = 2, max(2, round((math.sqrt(N) - 1) / 2)), 6
+    spec = dict(kind=kind, n=int(n), acc=acc, seed=next_key)
+    return next_key, spec, False
+
+
+def run_c5(args):
+    """Serving stream: Poisson arrivals, solves set up by a host thread pool
+    and continuously batched on the device, meshes and hierarchies cached in
+    HBM under a 200 GB budget.  Latency = completion - arrival."""
+    import threading
+    import numpy as np
+    import afepack_amd as afe
+
+    rng = np.random.default_rng(args.seed)
+    srv = afe.Server(0, hbm_budget=200 << 30, max_slots=256, setup_threads=args.setup_threads, smooth=1)
+    # warm the device and the library (first launches, graph code paths)
+    w = srv.submit(0, kind=1, n=32, rtol=args.rtol)
+    srv.wait(w)
+    submitted = []
+    results = {}
+    done_evt = threading.Event()
+
+    def waiter():
+        i = 0
+        while True:
+            if i >= len(submitted):
+                if done_evt.is_set() and i >= len(submitted):
+                    return
+                time.sleep(0.001)
+                continue
+            rid, ndof, t_arr, moved = submitted[i]
+            _, st, lat, code = srv.wait(rid)
+            results[rid] = (ndof, lat, st.iterations, st.rel_res_l2, code, moved, time.perf_counter())
+            srv.release(rid)
+            i += 1
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    cached, next_key = [], 1
+    t0 = time.perf_counter()
+    t_next = t0
+    n_req = 0
+    while True:
+        t_next += rng.exponential(1.0 / args.rate)
+        if t_next - t0 > args.duration:
+            break
+        now = time.perf_counter()
+        if t_next > now:
+            time.sleep(t_next - now)
+        key, spec, moved = c5_request(rng, cached, next_key)
+        if not moved:
+            next_key += 1
+            cached.append((key, spec))
+            cached = cached[-64:]
+        kind, n = spec["kind"], spec["n"]
+        ndof = (n + 1) ** 3 if kind == 3 else ((2 * n + 1) ** 2 if kind == 2 else (n + 1) ** 2)
+        rid = srv.submit(key, rtol=args.rtol, max_it=1000, **spec)
+        submitted.append((rid, ndof, time.perf_counter(), moved))
+        n_req += 1
+        if n_req % 200 == 0:
+            log(f"c5: {n_req} submitted at t={time.perf_counter() - t0:.1f} s, {len(results)} done")
+    t_sub_end = time.perf_counter()
+    done_evt.set()
+    th.join()
+    t_end = max(r[6] for r in results.values())
+    info = srv.info()
+    srv.close()
+    lat = sorted(1e3 * r[1] for r in results.values())
+    ok = [r for r in results.values() if r[4] == 0 and r[3] <= args.rtol]
+    total_dof = sum(r[0] for r in results.values())
+    p50 = statistics.median(lat)
+    p99 = lat[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)]
+    moved = [1e3 * r[1] for r in results.values() if r[5]]
+    fresh = [1e3 * r[1] for r in results.values() if not r[5]]
+    value = total_dof / (t_end - t0)
+    log(f"c5: {len(results)} requests, {total_dof} DOFs in {t_end - t0:.1f} s -> {value:.3e} DOF/s, "
+        f"p50 {p50:.1f} ms p99 {p99:.1f} ms, {len(ok)} converged")
+    out = {
+        "metric": "solved DOF/s at 1e-10 rel. residual, plain FP64 vs mixed AMG; p50/p99 latency",
+        "value": value, "unit": "DOF/s", "n_gpus": 1, "steps": len(results), "warmup": 1,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (CG, residual, assembly, coarse solve) + f32 (V-cycle)",
+        "data": f"synthetic request stream (seed {args.seed})",
+        "config": {"workload": (f"serving stream: Poisson arrivals {args.rate:g}/s for {args.duration:g} s; "
+                                "70 % 2D P1/P2, 30 % 3D P1; DOFs log-uniform [1e3, 2e6]; 40 % moving-mesh "
+                                "re-solves of cached meshes (0.1 h sin(2 pi k x), k in 1..3); HBM budget 200 GB"),
+                   "setup_threads": args.setup_threads, "rtol": args.rtol, "parallelism": "single GPU"},
+        "requests": len(results), "converged": len(ok),
+        "p50_ms": p50, "p99_ms": p99,
+        "p50_ms_moving_mesh": statistics.median(moved) if moved else None,
+        "p50_ms_new_mesh": statistics.median(fresh) if fresh else None,
+        "offered_dof_per_s": total_dof / (t_sub_end - t0),
+        "server": info,
     }
     print(json.dumps(out), flush=True)
 

@@ -189,13 +189,16 @@ def test_vcycle_matches_dense_model(ctx, nmpo, nmpsd):
     z = _zeros(ctx, sp.n_dof)
     g.vcycle(z, f, afe.PREC_FP64)
     L = g.info()["levels"]
-    As, Ps = [], []
+    As, Ps, Gs = [], [], []
     for l in range(L):
         m, n, rp, col, val = g.level_csr(l, 0)
         D = np.zeros((m, n))
+        S = np.zeros((m, n), dtype=bool)
         for i in range(m):
             D[i, col[rp[i]:rp[i + 1]]] += val[rp[i]:rp[i + 1]]
+            S[i, col[rp[i]:rp[i + 1]]] = True
         As.append(D)
+        Gs.append(S)
         if l < L - 1:
             m, n, rp, col, val = g.level_csr(l, 1)
             D = np.zeros((m, n))
@@ -203,10 +206,11 @@ def test_vcycle_matches_dense_model(ctx, nmpo, nmpsd):
                 D[i, col[rp[i]:rp[i + 1]]] += val[rp[i]:rp[i + 1]]
             Ps.append(D)
 
-    def colours(Am):
-        # greedy first-fit in index order on the nonzero off-diagonal graph
-        n = Am.shape[0]
-        G = (Am != 0) | (Am.T != 0)
+    def colours(S):
+        # greedy first-fit in index order on the symmetrised off-diagonal
+        # pattern (explicit zeros included)
+        n = S.shape[0]
+        G = (S | S.T) & ~np.eye(n, dtype=bool)
         c = -np.ones(n, int)
         for i in range(n):
             used = set(c[np.nonzero(G[i])[0]]) - {-1}
@@ -223,7 +227,7 @@ def test_vcycle_matches_dense_model(ctx, nmpo, nmpsd):
     def vc(l, b):
         if l == L - 1:
             return np.linalg.solve(As[l], b)
-        c = colours(As[l])
+        c = colours(Gs[l])
         fwd = np.concatenate([np.nonzero(c == k)[0] for k in range(c.max() + 1)])
         x = np.zeros_like(b)
         sweep(As[l], x, b, fwd)
