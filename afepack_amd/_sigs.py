@@ -99,6 +99,7 @@ SIGS = {
     "afe_server_wait": (C.c_int, [P, I64, P, PI64, C.POINTER(SolveStats), PF64]),
     "afe_server_release": (C.c_int, [P, I64]),
     "afe_server_info": (C.c_int, [P, PI64, PI64, PI64, PI64]),
+    "afe_server_timing": (C.c_int, [P, PF64]),
     "afe_batch_create": (C.c_int, [P, I32, I32, PP]),
     "afe_batch_destroy": (C.c_int, [P]),
     "afe_batch_add": (C.c_int, [P, P, P, P, F64, I32, PI32]),

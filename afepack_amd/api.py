@@ -552,6 +552,12 @@ class Server:
         _chk(_lib().afe_server_info(self.h, C.byref(b), C.byref(c), C.byref(d), C.byref(e)))
         return {"device_bytes": b.value, "cached": c.value, "completed": d.value, "evictions": e.value}
 
+    def timing(self):
+        t = (C.c_double * 7)()
+        _chk(_lib().afe_server_timing(self.h, t))
+        keys = ["setup_new_s", "setup_moving_s", "wait_setup_s", "wait_slot_s", "solve_s", "n_new", "n_moving"]
+        return dict(zip(keys, list(t)))
+
     def close(self):
         if self.h:
             _lib().afe_server_destroy(self.h)

@@ -82,6 +82,10 @@ struct afe_ctx {
   std::map<void *, int64_t> user_bytes; // afe_dev_alloc buffers
 };
 
+// device bytes allocated minus freed by the calling thread (the server
+// measures what one request's set-up keeps)
+static thread_local int64_t t_dev_bytes = 0;
+
 template <class T> struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
@@ -103,6 +107,7 @@ template <class T> struct DevBuf {
     if (p) {
       (void)hipFree(p);
       if (ctx) ctx->bytes -= (int64_t)(n * sizeof(T));
+      t_dev_bytes -= (int64_t)(n * sizeof(T));
     }
     p = nullptr;
     n = 0;
@@ -119,6 +124,7 @@ template <class T> struct DevBuf {
     }
     n = count;
     ctx->bytes += (int64_t)(n * sizeof(T));
+    t_dev_bytes += (int64_t)(n * sizeof(T));
   }
   void upload(afe_ctx *c, const T *h, size_t count) {
     alloc(c, count);
@@ -222,6 +228,7 @@ extern "C" int afe_dev_alloc(afe_ctx *c, int64_t bytes, void **d_ptr) {
   }
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   c->bytes += bytes;
+  t_dev_bytes += bytes;
   c->user_bytes[*d_ptr] = bytes;
   API_END
 }
@@ -237,6 +244,7 @@ extern "C" int afe_dev_free(afe_ctx *c, void *d_ptr) {
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipFree(d_ptr));
   c->bytes -= it->second;
+  t_dev_bytes -= it->second;
   c->user_bytes.erase(it);
   API_END
 }

@@ -7,6 +7,8 @@
 // independent rows instead of std::list scans.
 #include "host.hpp"
 
+#include <omp.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -762,5 +764,8 @@ void build_sell(const Csr &a, const std::vector<int32_t> &perm,
     }
   }
 }
+
+// host threads of the parallel loops started by the calling thread
+void set_host_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 
 } // namespace afe

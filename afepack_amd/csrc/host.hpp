@@ -132,4 +132,7 @@ void build_sell(const Csr &a, const std::vector<int32_t> &perm,
                 Sell &out, std::vector<int32_t> *src_map = nullptr,
                 std::vector<int32_t> *diag_src = nullptr);
 
+// host threads of the parallel loops started by the calling thread
+void set_host_threads(int n);
+
 } // namespace afe

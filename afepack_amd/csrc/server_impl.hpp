@@ -30,6 +30,7 @@ struct SrvEntry {
 
 struct SrvRequest {
   int64_t id = 0;
+  bool moving = false; // set up as a moving-mesh re-solve of a cached mesh
   afe_request r{};
   std::vector<double> fk, fphi, fw; // owned copies of the coefficient field
   int state = 0;                    // 0 pending, 1 setup, 2 ready, 3 solving, 4 done
@@ -38,7 +39,7 @@ struct SrvRequest {
   int32_t slot = -1;
   std::vector<double> x; // solution (when keep_x)
   afe_solve_stats st{};
-  std::chrono::steady_clock::time_point t_submit, t_done;
+  std::chrono::steady_clock::time_point t_submit, t_setup, t_ready, t_admit, t_done;
 };
 
 // structured-mesh sizes
@@ -57,13 +58,14 @@ static void mesh_sizes(int kind, int n, int64_t &nv, int64_t &ne, int64_t &ndof,
   }
 }
 
-// HBM of one cached problem: pattern (int64), values, gather map, the
-// hierarchy's sliced-ELL copies (FP64, FP32, int32 columns, natural copy),
-// transfer operators and about thirty vectors
+// HBM of one cached problem before it is built (reserved for admission; the
+// measured size replaces it once built): pattern (int64), values, gather
+// map, the hierarchy's sliced-ELL copies (FP64, FP32, int32 columns, natural
+// copy), refresh maps, transfer operators and about forty vectors
 static int64_t entry_bytes(int kind, int n) {
   int64_t nv, ne, ndof, nnz;
   mesh_sizes(kind, n, nv, ne, ndof, nnz);
-  return 64 * nnz + 320 * ndof + 64 * ne;
+  return 96 * nnz + 480 * ndof + 96 * ne;
 }
 
 } // namespace
@@ -84,6 +86,10 @@ struct afe_server {
   int64_t next_id = 1, completed = 0, evictions = 0;
   uint64_t clock = 0;
   std::string engine_error;
+  // sums over finished requests: set-up of new / moved meshes, waiting for a
+  // set-up thread, waiting for a batch slot, solving; counts of new / moved
+  double t_setup_new = 0, t_setup_moving = 0, t_wait_setup = 0, t_wait_slot = 0, t_solve = 0;
+  int64_t n_new = 0, n_moving = 0;
 };
 
 static void srv_free_entry(afe_server *s, SrvEntry *e) {
@@ -171,6 +177,8 @@ static int srv_setup(afe_server *s, SrvRequest *q) {
 }
 
 static void srv_worker(afe_server *s) {
+  // the pool works on many requests at once: one host thread per set-up
+  set_host_threads(1);
   std::unique_lock<std::mutex> lk(s->m);
   while (!s->stop) {
     // first pending request whose mesh is not being solved or set up
@@ -187,6 +195,7 @@ static void srv_worker(afe_server *s) {
       continue;
     }
     q->state = 1;
+    q->t_setup = std::chrono::steady_clock::now();
     auto ce = s->cache.find(q->r.mesh_key);
     SrvEntry *e;
     bool fresh = false;
@@ -211,6 +220,7 @@ static void srv_worker(afe_server *s) {
     }
     e->busy = true;
     q->e = e;
+    q->moving = !fresh;
     int64_t need = fresh ? e->est_bytes : 0;
     // HBM admission: wait until the budget has room (finished requests free
     // their entries for eviction)
@@ -218,9 +228,12 @@ static void srv_worker(afe_server *s) {
     if (s->stop) break;
     s->reserved += need;
     lk.unlock();
+    t_dev_bytes = 0;
     const int rc = srv_setup(s, q);
+    const int64_t kept = t_dev_bytes;
     lk.lock();
     s->reserved -= need;
+    if (fresh && rc == AFE_OK) e->est_bytes = kept; // measured from now on
     if (rc != AFE_OK) {
       q->status = rc;
       q->state = 4;
@@ -232,6 +245,7 @@ static void srv_worker(afe_server *s) {
       continue;
     }
     q->state = 2;
+    q->t_ready = std::chrono::steady_clock::now();
     s->ready.push_back(q);
     s->cv_work.notify_all();
   }
@@ -256,6 +270,7 @@ static void srv_engine(afe_server *s) {
         continue;
       }
       q->state = 3;
+      q->t_admit = std::chrono::steady_clock::now();
       solving.push_back(q);
     }
     if (solving.empty()) {
@@ -284,6 +299,11 @@ static void srv_engine(afe_server *s) {
       q->slot = -1;
       q->state = 4;
       q->t_done = std::chrono::steady_clock::now();
+      (q->moving ? s->t_setup_moving : s->t_setup_new) += seconds_between(q->t_setup, q->t_ready);
+      (q->moving ? s->n_moving : s->n_new)++;
+      s->t_wait_setup += seconds_between(q->t_submit, q->t_setup);
+      s->t_wait_slot += seconds_between(q->t_ready, q->t_admit);
+      s->t_solve += seconds_between(q->t_admit, q->t_done);
       q->e->busy = false;
       q->e->last_use = ++s->clock;
       s->completed++;
@@ -416,5 +436,22 @@ extern "C" int afe_server_info(afe_server *s, int64_t *device_bytes, int64_t *ca
   if (cached) *cached = (int64_t)s->cache.size();
   if (completed) *completed = s->completed;
   if (evictions) *evictions = s->evictions;
+  return AFE_OK;
+}
+
+// sums over finished requests (seconds): out[0] set-up of new meshes,
+// out[1] set-up of moving-mesh re-solves, out[2] waiting for a set-up
+// thread, out[3] waiting for a batch slot, out[4] solving; out[5], out[6]
+// the counts of new and moving-mesh requests
+extern "C" int afe_server_timing(afe_server *s, double *out) {
+  if (!s || !out) return fail(AFE_E_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(s->m);
+  out[0] = s->t_setup_new;
+  out[1] = s->t_setup_moving;
+  out[2] = s->t_wait_setup;
+  out[3] = s->t_wait_slot;
+  out[4] = s->t_solve;
+  out[5] = (double)s->n_new;
+  out[6] = (double)s->n_moving;
   return AFE_OK;
 }

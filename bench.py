@@ -495,6 +495,7 @@ def run_c5(args):
     th.join()
     t_end = max(r[6] for r in results.values())
     info = srv.info()
+    tm = srv.timing()
     srv.close()
     lat = sorted(1e3 * r[1] for r in results.values())
     ok = [r for r in results.values() if r[4] == 0 and r[3] <= args.rtol]
@@ -522,6 +523,13 @@ def run_c5(args):
         "p50_ms_new_mesh": statistics.median(fresh) if fresh else None,
         "offered_dof_per_s": total_dof / (t_sub_end - t0),
         "server": info,
+        "server_mean_s": {
+            "setup_new_mesh": tm["setup_new_s"] / max(1, tm["n_new"]),
+            "setup_moving_mesh": tm["setup_moving_s"] / max(1, tm["n_moving"]),
+            "wait_for_setup_thread": tm["wait_setup_s"] / max(1, tm["n_new"] + tm["n_moving"]),
+            "wait_for_slot": tm["wait_slot_s"] / max(1, tm["n_new"] + tm["n_moving"]),
+            "solve": tm["solve_s"] / max(1, tm["n_new"] + tm["n_moving"]),
+        },
     }
     print(json.dumps(out), flush=True)
 

@@ -330,6 +330,11 @@ int afe_server_wait(afe_server *s, int64_t id, double *h_x, int64_t *n_dof,
 int afe_server_release(afe_server *s, int64_t id);
 int afe_server_info(afe_server *s, int64_t *device_bytes, int64_t *cached,
                     int64_t *completed, int64_t *evictions);
+/* sums over finished requests (seconds): out[0] set-up of new meshes,
+ * out[1] set-up of moving-mesh re-solves, out[2] waiting for a set-up
+ * thread, out[3] waiting for a batch slot, out[4] solving; out[5], out[6]
+ * the counts of new and moving-mesh requests */
+int afe_server_timing(afe_server *s, double *out);
 
 /* ---- measurement hooks (no counterpart in the reference) ------------------ */
 /* average duration of `reps` back-to-back launches of a level-0 kernel as the

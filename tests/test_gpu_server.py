@@ -65,7 +65,7 @@ def test_server_alone_mix_moving_mesh():
 def test_server_budget_evicts_idle_meshes():
     import afepack_amd as afe
     print("case server budget", flush=True)
-    srv = afe.Server(0, hbm_budget=20 << 20, max_slots=8, setup_threads=3, smooth=1)
+    srv = afe.Server(0, hbm_budget=32 << 20, max_slots=8, setup_threads=3, smooth=1)
     try:
         ids = [srv.submit(k, kind=P2 if k % 2 else P1, n=20 + 4 * k, seed=k) for k in range(10)]
         for rid in ids:
