@@ -145,6 +145,12 @@ struct Tuning {
   int64_t stream_bytes = 128 << 20;  // matrices above this use streaming loads
 };
 static Tuning g_tune;
+
+// AFE_VERBOSE=1 prints solver progress and set-up phase times to stderr
+static bool verbose() {
+  static const bool v = getenv("AFE_VERBOSE") != nullptr;
+  return v;
+}
 static constexpr int kMaxGridBlocks = 16384; // bound of spmv_blocks (partial-sum buffer)
 static constexpr int64_t NAT_WINDOW = 256;    // length-sort window of the natural-order copy
 static constexpr int32_t COLOUR_WINDOW = 1024; // length-sort window inside a colour
@@ -1210,13 +1216,22 @@ extern "C" int afe_amg_create(afe_matrix *a, const afe_amg_params *p, afe_amg **
   }
   // the coarsening and the device layout are host work: the context stays
   // free for other callers meanwhile
+  const auto t0 = std::chrono::steady_clock::now();
   amg_build(A0, hp, g->h);
+  const auto t1 = std::chrono::steady_clock::now();
   HostLayout lay;
   build_layout(g->h, lay, prm.lazy != 0);
+  const auto t2 = std::chrono::steady_clock::now();
   std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
   HIPCHK(hipHostMalloc((void **)&g->h_sc, sizeof(CgScalars), hipHostMallocDefault));
   upload_layout(g.get(), lay);
+  if (verbose()) {
+    const auto t3 = std::chrono::steady_clock::now();
+    fprintf(stderr, "amg_create n %lld: coarsening %.3f s, layout %.3f s, upload %.3f s\n", (long long)A0.m,
+            std::chrono::duration<double>(t1 - t0).count(), std::chrono::duration<double>(t2 - t1).count(),
+            std::chrono::duration<double>(t3 - t2).count());
+  }
   *out = g.release();
   API_END
 }
@@ -1534,10 +1549,6 @@ static void launch_graph_or_issue(afe_amg *g, int prec, bool symmetric, int nu) 
   HIPCHK(hipGraphLaunch(ex, st));
 }
 
-static bool verbose() {
-  static const bool v = getenv("AFE_VERBOSE") != nullptr;
-  return v;
-}
 
 // true residual r = b - A (x + xl) in compensated arithmetic, and its norms
 // natural = true: vectors in the natural numbering (the CG), else in the

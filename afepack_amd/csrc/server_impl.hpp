@@ -101,12 +101,18 @@ static void srv_free_entry(afe_server *s, SrvEntry *e) {
   e->g = nullptr; e->A = nullptr; e->sp = nullptr; e->d_b = e->d_x = nullptr;
 }
 
-// bytes held by the cache (estimated) + in-flight setups
+// device bytes held: the larger of the cache's accounted sizes and the
+// context's live allocations, plus the reservations of in-flight set-ups
 static int64_t srv_bytes(afe_server *s) {
-  int64_t b = s->reserved;
+  int64_t b = 0;
   for (auto &kv : s->cache)
     if (kv.second->g) b += kv.second->est_bytes;
-  return b;
+  int64_t live;
+  {
+    std::lock_guard<std::recursive_mutex> lc(s->ctx->mu);
+    live = s->ctx->bytes;
+  }
+  return std::max(b, live) + s->reserved;
 }
 
 // make room for `need` bytes: evict idle entries, least recently used first
@@ -429,9 +435,7 @@ extern "C" int afe_server_info(afe_server *s, int64_t *device_bytes, int64_t *ca
   std::lock_guard<std::mutex> lk(s->m);
   if (device_bytes) {
     std::lock_guard<std::recursive_mutex> lc(s->ctx->mu);
-    int64_t u = 0;
-    for (auto &kv : s->ctx->user_bytes) u += kv.second;
-    *device_bytes = s->ctx->bytes + u;
+    *device_bytes = s->ctx->bytes; // includes the afe_dev_alloc buffers
   }
   if (cached) *cached = (int64_t)s->cache.size();
   if (completed) *completed = s->completed;

@@ -238,3 +238,39 @@ def test_vcycle_matches_dense_model(ctx, nmpo, nmpsd):
 
     ref = vc(0, f.numpy())
     assert np.linalg.norm(z.numpy() - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("kind,n,jitter,acc,modes,amp", [
+    (3, 8, 0.05, 2, 16, 1.0),   # the 3D variable-coefficient configuration, small
+    (2, 12, 0.0, 6, 8, 0.5),    # a batch member: P2, 8-mode field
+])
+def test_variable_coefficient_solution_matches_oracle(ctx, kind, n, jitter, acc, modes, amp):
+    """-div(a grad u) = f, a = exp(g): assembly, rhs (f from the manufactured
+    u), Dirichlet, mixed FP32-V-cycle / FP64 PCG to 1e-12 against the oracle's
+    AMGSolver solution (1e-9), L2/H1 errors to 1e-6 relative."""
+    import afepack_amd as afe
+    from oracle import oracle_py as orc
+    print(f"varcoef kind={kind} n={n}", flush=True)
+    sp, osp, _ = _spaces(ctx, kind, n, jitter)
+    field = afe.field_random(modes, sp.dim, amp, 23)
+    ofield = orc.Field(field.k, field.phi, field.w, field.scale)
+    A = afe.Matrix(sp)
+    A.assemble(afe.OP_VARSTIFF, acc, field)
+    f = sp.rhs(acc, afe.F_VARCOEF, field)
+    A.apply_dirichlet(f)
+    OA = osp.build(orc.OP_VARSTIFF, acc, ofield)
+    of = osp.rhs(acc, orc.F_VARCOEF, ofield)
+    osp.apply_dirichlet(OA, of)
+    rp, col, oval = OA.csr()
+    assert rel_err_rows(A.values(), oval, rp) <= 1e-13
+    assert np.abs(f.numpy() - of).max() <= 1e-13 * np.abs(of).max()
+    ox, _, _ = _oracle_solution(OA, of)
+    g = afe.Amg(A, afe.AmgParams(lazy=1, smooth=1, build_fp32=1))
+    x = _zeros(ctx, sp.n_dof)
+    st = g.pcg(x, f, rtol=1e-12, max_it=400, prec=afe.PREC_FP32)
+    assert st.converged == 1 and st.rel_res_l2 <= 1e-12
+    xs = x.numpy()
+    assert np.linalg.norm(xs - ox) <= 1e-9 * np.linalg.norm(ox)
+    e2, e1 = sp.error_l2(x, acc), sp.error_h1(x, acc)
+    oe2, oe1 = osp.l2_error(ox, acc), osp.h1_error(ox, acc)
+    assert abs(e2 - oe2) <= 1e-6 * oe2 and abs(e1 - oe1) <= 1e-6 * oe1
