@@ -818,7 +818,7 @@ extern "C" int afe_error_qp(afe_space *s, const double *d_uh, int32_t acc, int32
   default: k_element_error_qp<P1_TET><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, s->elem_dof.p, qd, q->n, h1, d_uh, ex.p, pe.p); break;
   }
   k_sum_partial<<<RED_BLOCKS, 256, 0, st>>>(ne, pe.p, part.p);
-  k_finalize<<<1, 256, 0, st>>>(part.p, RED_BLOCKS, 1, tot.p);
+  k_finalize<<<1, FIN_THREADS, 0, st>>>(part.p, RED_BLOCKS, 1, tot.p);
   double h = 0;
   HIPCHK(hipMemcpyAsync(&h, tot.p, sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -852,7 +852,7 @@ static int fe_error(afe_space *s, const double *d_uh, int acc, int h1, double *e
   default: launch_error<P1_TET>(s, q, h1, d_uh, pe.p); break;
   }
   k_sum_partial<<<RED_BLOCKS, 256, 0, s->ctx->stream>>>(s->h.n_elem, pe.p, part.p);
-  k_finalize<<<1, 256, 0, s->ctx->stream>>>(part.p, RED_BLOCKS, 1, tot.p);
+  k_finalize<<<1, FIN_THREADS, 0, s->ctx->stream>>>(part.p, RED_BLOCKS, 1, tot.p);
   double h = 0;
   HIPCHK(hipMemcpyAsync(&h, tot.p, sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
@@ -1564,7 +1564,7 @@ static void residual_norms(afe_amg *g, bool natural, const double *b, const doub
   else
     k_residual_norms_dd<<<G, BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, d.v64.p, d.diag64.p, nullptr, b, x, xl, r,
                                             g->part.p, g_tune.xcd);
-  k_finalize<<<1, 256, 0, st>>>(g->part.p, G, 3, &g->sc.p->rr);
+  k_finalize<<<1, FIN_THREADS, 0, st>>>(g->part.p, G, 3, &g->sc.p->rr);
 }
 
 static void read_scalars(afe_amg *g) {
@@ -1642,7 +1642,7 @@ template <class T> static void precondition_perm(afe_amg *g, const double *r, bo
   launch_graph_or_issue(g, std::is_same<T, float>::value ? 1 : 0, true, g->prm.smooth);
   if (first) k_fill<double><<<nblk(n), 256, 0, s>>>(n, g->wz.p, 0.0);
   k_take_z<T><<<RED_BLOCKS, 256, 0, s>>>(n, Prec<T>::x(d0), r, g->wz.p, g->iperm0.p, g->part.p);
-  k_finalize<<<1, 256, 0, s>>>(g->part.p, RED_BLOCKS, 2, &g->sc.p->rz_new);
+  k_finalize<<<1, FIN_THREADS, 0, s>>>(g->part.p, RED_BLOCKS, 2, &g->sc.p->rz_new);
   k_cg_beta<<<1, 64, 0, s>>>(g->sc.p, first ? 1 : 0);
   k_p_update<<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wz.p, g->wp.p);
 }
@@ -1656,16 +1656,16 @@ template <class T> static void issue_cg_iteration(afe_amg *g) {
   k_spmv_dot<<<g_tune.spmv_blocks, BLK, 0, s>>>(g->nat_si.p, g->nat_ns, g->nat_cols.p, g->nat_v64.p,
                                                  g->nat_diag.p, g->nat_rowmap.p, g->wp.p, g->wq.p, g->part.p,
                                                  g_tune.xcd);
-  k_finalize<<<1, 256, 0, s>>>(g->part.p, g_tune.spmv_blocks, 1, &g->sc.p->pq);
+  k_finalize<<<1, FIN_THREADS, 0, s>>>(g->part.p, g_tune.spmv_blocks, 1, &g->sc.p->pq);
   k_cg_update<T><<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wx.p, g->wxl.p, g->wr.p, g->wp.p, g->wq.p,
                                              Prec<T>::b(d0), Prec<T>::x(d0), g->iperm0.p, g->part.p);
-  k_finalize<<<1, 256, 0, s>>>(g->part.p, RED_BLOCKS, 1, &g->sc.p->rr);
+  k_finalize<<<1, FIN_THREADS, 0, s>>>(g->part.p, RED_BLOCKS, 1, &g->sc.p->rr);
   HIPCHK(hipMemcpyAsync(g->h_sc, g->sc.p, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
   const int prec = std::is_same<T, float>::value ? 1 : 0;
   // V-cycle issued inline (nested graph capture is not used)
   issue_vcycle<T>(g, g->prm.smooth, true, s);
   k_take_z<T><<<RED_BLOCKS, 256, 0, s>>>(n, Prec<T>::x(d0), g->wr.p, g->wz.p, g->iperm0.p, g->part.p);
-  k_finalize<<<1, 256, 0, s>>>(g->part.p, RED_BLOCKS, 2, &g->sc.p->rz_new);
+  k_finalize<<<1, FIN_THREADS, 0, s>>>(g->part.p, RED_BLOCKS, 2, &g->sc.p->rz_new);
   k_cg_beta<<<1, 64, 0, s>>>(g->sc.p, 0);
   k_p_update<<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wz.p, g->wp.p);
   (void)prec;

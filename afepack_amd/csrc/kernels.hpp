@@ -470,9 +470,12 @@ __global__ __launch_bounds__(256) void k_sum_partial(int64_t n, const double *__
 }
 
 // out[k] = sum of part[k*stride + 0 .. np) for k < nsum, one block
-__global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ part, int np, int nsum,
-                                                  double *__restrict__ out) {
-  __shared__ double sh[256];
+// one workgroup of FIN_THREADS sums np block partials per scalar (fixed
+// order: strided per-thread sums, then a tree)
+constexpr int FIN_THREADS = 1024;
+__global__ __launch_bounds__(FIN_THREADS) void k_finalize(const double *__restrict__ part, int np, int nsum,
+                                                          double *__restrict__ out) {
+  __shared__ double sh[FIN_THREADS];
   for (int k = 0; k < nsum; k++) {
     double s = 0.0;
     for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[k * np + i];

@@ -91,6 +91,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--xcd", type=int, default=-1,
+                    help="slice schedule of the level-0 grid-stride kernels (0, 1, 2); -1 = autotune")
+    ap.add_argument("--spmv-blocks", type=int, default=16384, help="grid of those kernels when --xcd is given")
     ap.add_argument("--kernels-only", action="store_true",
                     help="build the C2 system and hierarchy, then only time the level-0 kernels and a "
                          "plain copy (for profiler runs); prints one JSON line")
@@ -147,6 +150,7 @@ def run_c2(args):
         return time.perf_counter() - t0, st
 
     if args.kernels_only:
+        afe.set_tuning(args.xcd if args.xcd >= 0 else 2, args.spmv_blocks, 4096)
         gg, t_b = build(args.nmpo or 2048, args.nmpsd if args.nmpo else 0.0)
         out = {"workload": f"2D P2 Poisson {n}x{n}, {sp.n_dof} DOFs", "kernels": {}}
         for which, name in ((0, "spmv_fp64"), (1, "gs_sweep_fp32"), (3, "residual_fp32")):
@@ -163,16 +167,18 @@ def run_c2(args):
     gg, _ = build(50, 0.382)
     tune_log = []
     best_t = None
-    for xcd in (0, 1, 2):
-        for blocks in (1024, 2048, 4096, 8192, 16384):
-            afe.set_tuning(xcd, blocks, 4096)
-            t_sp, _ = gg.kernel_timing(0, reps=10)
-            t_gs, _ = gg.kernel_timing(1, reps=10)
-            tune_log.append({"xcd": xcd, "spmv_blocks": blocks, "spmv_us": 1e3 * t_sp, "gs_us": 1e3 * t_gs})
-            log(f"tune xcd {xcd} blocks {blocks}: spmv {1e3*t_sp:.1f} us, gs sweep {1e3*t_gs:.1f} us")
-            score = t_sp + 2 * t_gs
-            if best_t is None or score < best_t[0]:
-                best_t = (score, xcd, blocks)
+    shapes = [(x, bl) for x in (0, 1, 2) for bl in (1024, 2048, 4096, 8192, 16384)]
+    if args.xcd >= 0:
+        shapes = [(args.xcd, args.spmv_blocks)]
+    for xcd, blocks in shapes:
+        afe.set_tuning(xcd, blocks, 4096)
+        t_sp, _ = gg.kernel_timing(0, reps=10)
+        t_gs, _ = gg.kernel_timing(1, reps=10)
+        tune_log.append({"xcd": xcd, "spmv_blocks": blocks, "spmv_us": 1e3 * t_sp, "gs_us": 1e3 * t_gs})
+        log(f"tune xcd {xcd} blocks {blocks}: spmv {1e3*t_sp:.1f} us, gs sweep {1e3*t_gs:.1f} us")
+        score = t_sp + 2 * t_gs
+        if best_t is None or score < best_t[0]:
+            best_t = (score, xcd, blocks)
     afe.set_tuning(best_t[1], best_t[2], 4096)
     del gg
 
