@@ -2,8 +2,10 @@
 // C++ classes and writes every intermediate result as raw little-endian
 // arrays, so that they can be compared with an independent computation.
 //
-// usage: facade_check <kind 1|2|3> <n> <outdir>
-//   kind 1 = triangle.1 (P1), 2 = triangle.2 (P2), 3 = tetrahedron.1
+// usage: facade_check <kind 1|2|3> <n> <outdir> [easymesh-base]
+//   kind 1 = triangle.1 (P1), 2 = triangle.2 (P2), 3 = tetrahedron.1; with an
+//   EasyMesh base name (2D) the mesh is read from <base>.n/.s/.e instead of
+//   generated
 // files: rp.i64 col.i64 (pattern), K.f64 (StiffMatrix), KH.f64 (the same
 // form evaluated on the host by a derived class), K2.f64 (an override that
 // doubles it), M.f64 (MassMatrix), f.f64 (L2Discretize), Ka.f64 fa.f64 ua.f64
@@ -73,13 +75,22 @@ template <int DIM> struct DoubledStiff : public StiffMatrix<DIM, double> {
   }
 };
 
-template <int DIM> static int run(int kind, int n, const std::string &out) {
+template <int DIM> static int run(int kind, int n, const std::string &out, const std::string &easymesh) {
   g_dim = DIM;
   Mesh<DIM, DIM> mesh;
-  if constexpr (DIM == 2)
-    buildSquareMesh(mesh, n, 0.2, 7);
-  else
+  if constexpr (DIM == 2) {
+    if (!easymesh.empty()) {
+      EasyMesh em;
+      em.readData(easymesh);
+      mesh.coords = em.coords;
+      mesh.elem = em.elem;
+      mesh.vmark = em.vmark;
+    } else {
+      buildSquareMesh(mesh, n, 0.2, 7);
+    }
+  } else {
     buildCubeMesh(mesh, n, 0.05, 7);
+  }
 
   TemplateGeometry<DIM> geo;
   CoordTransform<DIM, DIM> crd;
@@ -172,7 +183,8 @@ int main(int argc, char **argv) {
   }
   const int kind = atoi(argv[1]), n = atoi(argv[2]);
   try {
-    return kind == AFE_P1_TET ? run<3>(kind, n, argv[3]) : run<2>(kind, n, argv[3]);
+    const std::string em = argc > 4 ? argv[4] : "";
+    return kind == AFE_P1_TET ? run<3>(kind, n, argv[3], em) : run<2>(kind, n, argv[3], em);
   } catch (const std::exception &e) {
     std::fprintf(stderr, "facade_check failed: %s\n", e.what());
     return 3;

@@ -164,3 +164,65 @@ def test_cpp_examples_run(tmp_path, prog, limit):
     err = float(line.split("=")[1])
     assert 0 < err < limit
     assert os.path.getsize(os.path.join(tmp_path, "u.dx")) > 0
+
+
+def _write_easymesh(base, c, ev, vm, flip):
+    """EasyMesh .n/.s/.e files of a triangulation; elements listed in flip
+    are written clockwise (the reader turns them back)"""
+    C = c.reshape(-1, 2)
+    E = ev.reshape(-1, 3).copy()
+    E[flip] = E[flip][:, [0, 2, 1]]
+    sides = {}
+    for e in E:
+        for k in range(3):
+            a, b = sorted((int(e[(k + 1) % 3]), int(e[(k + 2) % 3])))
+            sides.setdefault((a, b), 0)
+            sides[(a, b)] += 1
+    side_list = list(sides)
+    sid = {s: i for i, s in enumerate(side_list)}
+    with open(base + ".n", "w") as f:
+        f.write(f"{len(C)} {len(E)} {len(side_list)} **(Nnd, Nee, Nsd)**\n")
+        for i, p in enumerate(C):
+            f.write(f"{i} {float(p[0])!r} {float(p[1])!r} {int(vm[i])}\n")
+    with open(base + ".s", "w") as f:
+        f.write(f"{len(side_list)}\n")
+        for i, (a, b) in enumerate(side_list):
+            f.write(f"{i} {a} {b} -1 -1 {1 if sides[(a, b)] == 1 else 0}\n")
+    with open(base + ".e", "w") as f:
+        f.write(f"{len(E)} {len(C)} {len(side_list)} **(Nee, Nnd, Nsd)**\n")
+        for i, e in enumerate(E):
+            s = [sid[tuple(sorted((int(e[(k + 1) % 3]), int(e[(k + 2) % 3]))))] for k in range(3)]
+            f.write(f"{i} {e[0]} {e[1]} {e[2]} -1 -1 -1 {s[0]} {s[1]} {s[2]}\n")
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_cpp_facade_easymesh_input(tmp_path, kind):
+    """EasyMesh ingestion (EasyMesh.templates.h:16-94): elements in a
+    shuffled order, some written clockwise; the facade's pattern, matrix and
+    right-hand side equal the oracle's on the re-oriented mesh bit for bit."""
+    import afepack_amd as afe
+    from oracle import oracle_py as orc
+    exe = os.path.join(BIN, "facade_check")
+    if not os.path.exists(exe):
+        pytest.fail("afepack_amd/examples/bin/facade_check is not built")
+    print(f"case easymesh kind={kind}", flush=True)
+    c, ev, vm = afe.mesh_square(10, 0.2, 41)
+    rng = np.random.default_rng(5)
+    E = ev.reshape(-1, 3)[rng.permutation(len(ev) // 3)]
+    flip = rng.uniform(size=len(E)) < 0.3
+    base = str(tmp_path / "mesh")
+    _write_easymesh(base, c, E.ravel(), vm, flip)
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([exe, str(kind), "0", str(out), base], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    osp = orc.Space(kind, c, np.ascontiguousarray(E.ravel()), vm)
+    OK = osp.build(orc.OP_STIFF, 4)
+    rp, col, kval = OK.csr()
+    assert np.array_equal(_read(out, "rp.i64", np.int64), rp)
+    assert np.array_equal(_read(out, "col.i64", np.int64), col)
+    assert np.array_equal(_read(out, "K.f64"), kval)
+    of = osp.rhs(4, orc.F_POISSON)
+    f = _read(out, "f.f64")
+    assert np.abs(f - of).max() <= 1e-13 * np.abs(of).max()
