@@ -289,7 +289,8 @@ def run_c2(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (CG, residual, assembly, coarse solve) + f32 (V-cycle)",
+        "dtype": "f64+f32",
+        "precision": "FP64 assembly, CG, residual and coarse solve; FP32 V-cycle",
         "data": f"synthetic: seeded jittered mesh (seed {args.seed}), manufactured rhs",
         "config": {
             "workload": f"2D P2 Poisson, {n}x{n} squares x 2 triangles, jitter 0.2h, acc 6",
@@ -395,7 +396,8 @@ def run_c3(args):
         "metric": "solved DOF/s at 1e-10 rel. residual, plain FP64 vs mixed AMG; p50/p99 latency",
         "value": value, "unit": "DOF/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 (CG, residual, assembly, coarse solve) + f32 (V-cycle)",
+        "dtype": "f64+f32",
+        "precision": "FP64 assembly, CG, residual and coarse solve; FP32 V-cycle",
         "data": f"synthetic: seeded coefficient field (seed {args.seed}), manufactured solution",
         "config": {"workload": ("3D -div(a grad u) = f, a = exp(16-mode field, amplitude 1), 128^3 cubes x 6 "
                                 "tetrahedra, P1, accuracy 2"),
@@ -497,7 +499,8 @@ def run_c4(args):
         "metric": "solved DOF/s at 1e-10 rel. residual, plain FP64 vs mixed AMG; p50/p99 latency",
         "value": value, "unit": "DOF/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 (CG, residual, assembly, coarse solve) + f32 (V-cycle)",
+        "dtype": "f64+f32",
+        "precision": "FP64 assembly, CG, residual and coarse solve; FP32 V-cycle",
         "data": f"synthetic: seeded sizes, fields and right-hand sides (seed {args.seed})",
         "config": {"workload": (f"{K} 2D elliptic problems in one batch, n log-uniform in [32, 512], "
                                 "even P1 acc 4 / odd P2 acc 6, a = exp(8-mode field, amplitude 0.5), "
@@ -611,7 +614,8 @@ def run_c5(args):
         "metric": "solved DOF/s at 1e-10 rel. residual, plain FP64 vs mixed AMG; p50/p99 latency",
         "value": value, "unit": "DOF/s", "n_gpus": 1, "steps": len(results), "warmup": 1,
         "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 (CG, residual, assembly, coarse solve) + f32 (V-cycle)",
+        "dtype": "f64+f32",
+        "precision": "FP64 assembly, CG, residual and coarse solve; FP32 V-cycle",
         "data": f"synthetic request stream (seed {args.seed})",
         "config": {"workload": (f"serving stream: Poisson arrivals {args.rate:g}/s for {args.duration:g} s; "
                                 "70 % 2D P1/P2, 30 % 3D P1; DOFs log-uniform [1e3, 2e6]; 40 % moving-mesh "

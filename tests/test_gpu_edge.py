@@ -1,0 +1,66 @@
+"""GPU edge cases: the smallest meshes (one square / one cube), single-level
+hierarchies, and the batch with members of one level next to deep ones."""
+import numpy as np
+import pytest
+
+from tests.conftest import make_mesh
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("kind,n", [(1, 1), (2, 1), (3, 1), (1, 2), (2, 2), (3, 2)])
+def test_tiny_meshes_match_oracle(ctx, kind, n):
+    import afepack_amd as afe
+    from oracle import oracle_py as orc
+    print(f"case tiny kind={kind} n={n}", flush=True)
+    c, ev, vm = make_mesh(kind, n, 0.0, 0)
+    sp = afe.Space(ctx, kind, c, ev, vm)
+    osp = orc.Space(kind, c, ev, vm)
+    acc = 2 if kind == 3 else 4
+    A = afe.Matrix(sp)
+    A.assemble(afe.OP_STIFF, acc)
+    OA = osp.build(orc.OP_STIFF, acc)
+    rp, col, oval = OA.csr()
+    prp, pcol = sp.pattern()
+    assert np.array_equal(prp, rp) and np.array_equal(pcol, col)
+    assert np.array_equal(A.values(), oval)
+    f = sp.rhs(acc, afe.F_ONE)
+    of = osp.rhs(acc, orc.F_ONE)
+    A.apply_dirichlet(f)
+    osp.apply_dirichlet(OA, of)
+    xd = np.linalg.solve(OA.dense(), of)
+    for lazy in (0, 1):
+        g = afe.Amg(A, afe.AmgParams(lazy=lazy, smooth=1, build_fp32=1))
+        x = afe.DeviceVector(ctx, sp.n_dof)
+        st = g.pcg(x, f, rtol=1e-12, max_it=50, prec=afe.PREC_FP32)
+        assert st.converged == 1
+        assert np.linalg.norm(x.numpy() - xd) <= 1e-10 * max(1.0, np.linalg.norm(xd))
+        x.zero()
+        g.solve(x, f, step=20, rtol=1e-13)
+        assert np.linalg.norm(x.numpy() - xd) <= 1e-10 * max(1.0, np.linalg.norm(xd))
+
+
+def test_batch_with_single_level_members(ctx):
+    import afepack_amd as afe
+    print("case batch tiny+deep", flush=True)
+    probs = []
+    for kind, n in ((1, 2), (2, 1), (1, 48), (3, 2), (2, 24)):
+        c, ev, vm = make_mesh(kind, n, 0.0, 0)
+        sp = afe.Space(ctx, kind, c, ev, vm)
+        A = afe.Matrix(sp)
+        acc = 2 if kind == 3 else 4
+        A.assemble(afe.OP_STIFF, acc)
+        b = sp.rhs(acc, afe.F_POISSON)
+        A.apply_dirichlet(b)
+        g = afe.Amg(A, afe.AmgParams(lazy=1, smooth=1, build_fp32=1))
+        probs.append((sp, A, b, g))
+    assert any(p[3].info()["levels"] == 1 for p in probs)
+    assert any(p[3].info()["levels"] > 2 for p in probs)
+    bt = afe.Batch(ctx, afe.PREC_FP32, smooth=1)
+    xs = [afe.DeviceVector(ctx, p[0].n_dof) for p in probs]
+    slots = [bt.add(p[3], x, p[2], rtol=1e-10, max_it=200) for p, x in zip(probs, xs)]
+    bt.run()
+    for k, (p, x) in enumerate(zip(probs, xs)):
+        done, st, code = bt.result(slots[k])
+        assert done and code == 0 and st.rel_res_l2 <= 1e-10
+    bt.close()
