@@ -52,3 +52,20 @@ def test_tuning_validation():
             afe.set_tuning(*bad)
     afe.set_tuning(2, 16384, 4096)
     afe.set_tuning(0, 2048, 4096)
+
+
+def test_cpp_dropin_headers_compile(tmp_path):
+    """The C++ drop-in headers compile the example programs (no device needed
+    to build them; they run in the GPU tests)."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no host C++ compiler")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    inc = [f"-I{root}/afepack_amd/include", f"-I{root}/include"]
+    for prog in ("poisson_equation", "elliptic_equation", "facade_check"):
+        src = os.path.join(root, "afepack_amd", "examples", prog + ".cpp")
+        r = subprocess.run(["g++", "-std=c++17", "-ffp-contract=off", "-Wall", "-fsyntax-only", *inc, src],
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        assert "warning" not in r.stderr, r.stderr[-2000:]
