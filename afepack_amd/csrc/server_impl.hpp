@@ -35,6 +35,7 @@ struct SrvRequest {
   std::vector<double> fk, fphi, fw; // owned copies of the coefficient field
   int state = 0;                    // 0 pending, 1 setup, 2 ready, 3 solving, 4 done
   int status = AFE_OK;
+  std::string err; // message of a failed set-up or solve
   SrvEntry *e = nullptr;
   int32_t slot = -1;
   std::vector<double> x; // solution (when keep_x)
@@ -74,7 +75,7 @@ struct afe_server {
   afe_ctx *ctx = nullptr;
   afe_batch *batch = nullptr;
   int64_t budget = 0, reserved = 0;
-  int max_slots = 0, active = 0;
+  int max_slots = 0;
   std::map<int64_t, std::unique_ptr<SrvEntry>> cache;
   std::map<int64_t, std::unique_ptr<SrvRequest>> reqs;
   std::deque<SrvRequest *> pending, ready;
@@ -242,6 +243,7 @@ static void srv_worker(afe_server *s) {
     if (fresh && rc == AFE_OK) e->est_bytes = kept; // measured from now on
     if (rc != AFE_OK) {
       q->status = rc;
+      q->err = afe_last_error();
       q->state = 4;
       q->t_done = std::chrono::steady_clock::now();
       srv_free_entry(s, e);
@@ -268,6 +270,7 @@ static void srv_engine(afe_server *s) {
       const int rc = afe_batch_add(s->batch, q->e->g, q->e->d_x, q->e->d_b, q->r.rtol, q->r.max_it, &q->slot);
       if (rc != AFE_OK) {
         q->status = rc;
+        q->err = afe_last_error();
         q->state = 4;
         q->t_done = std::chrono::steady_clock::now();
         q->e->busy = false;
@@ -414,7 +417,8 @@ extern "C" int afe_server_wait(afe_server *s, int64_t id, double *h_x, int64_t *
   if (latency_s) *latency_s = seconds_between(q->t_submit, q->t_done);
   if (q->status != AFE_OK) {
     if (q->status == AFE_E_NOTCONVERGED) return fail(AFE_E_NOTCONVERGED, "request did not reach its tolerance");
-    return fail(q->status, "request failed");
+    const std::string why = !q->err.empty() ? q->err : s->engine_error;
+    return fail(q->status, "request failed" + (why.empty() ? std::string() : ": " + why));
   }
   return AFE_OK;
 }
