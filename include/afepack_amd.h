@@ -215,7 +215,9 @@ int afe_error_h1(afe_space *s, const double *d_uh, int32_t acc, double *err);
 /* AMGSolver(M, tol, s, nmpo, nmpsd, alpha) / reinit (src/AMGSolver.cpp:384-438,
  * 683-699) or lazyReinit on a fresh solver (:442-496): the hierarchy is built
  * on the host (Project / lazyProject + getPMPt, identical patterns), coloured,
- * converted to sliced-ELL and uploaded once. */
+ * converted to sliced-ELL and uploaded once.  As in the reference, the fine
+ * matrix is borrowed (projected_matrix[0] = &M): it must outlive the
+ * hierarchy or its next afe_amg_lazy_reinit. */
 int afe_amg_create(afe_matrix *a, const afe_amg_params *p, afe_amg **out);
 int afe_amg_destroy(afe_amg *g);
 /* host-only hierarchy of a CSR matrix (the coarsening half of
@@ -271,6 +273,8 @@ int afe_amg_set_smooth(afe_amg *g, int32_t smooth);
  * max_it; its solution is then in its d_x.  The V-cycle uses `smooth`
  * sweeps per side for every member (the member hierarchy's smoothing count
  * is set to it).  One active solve per hierarchy at a time. */
+/* A member's hierarchy and vectors must stay alive until the member is
+ * removed or the batch destroyed. */
 typedef struct afe_batch afe_batch;
 int afe_batch_create(afe_ctx *ctx, int32_t prec, int32_t smooth,
                      afe_batch **out);
