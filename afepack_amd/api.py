@@ -439,6 +439,13 @@ class Amg(_AmgBase):
     def set_smooth(self, s):
         _chk(_lib().afe_amg_set_smooth(self.h, s))
 
+    def live_timing(self, enable):
+        """Start (True) or stop (False) live timing of the CG's level-0 SpMV;
+        returns (average ms so far, launches)."""
+        ms, n = C.c_double(), C.c_int64()
+        _chk(_lib().afe_amg_live_timing(self.h, 1 if enable else 0, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def set_tail_rows(self, rows):
         _chk(_lib().afe_amg_set_tail_rows(self.h, int(rows)))
 

@@ -896,6 +896,12 @@ struct DevLevel {
 struct afe_amg {
   afe_ctx *ctx = nullptr;
   afe_matrix *A0 = nullptr;
+  // live timing of the CG's level-0 SpMV inside the solves (events recorded
+  // around it in every iteration, summed after each iteration's sync)
+  bool live = false;
+  hipEvent_t live0 = nullptr, live1 = nullptr;
+  double live_ms = 0;
+  int64_t live_n = 0;
   afe_amg_params prm{};
   Hierarchy h;
   std::vector<DevLevel> lv;
@@ -1291,6 +1297,8 @@ extern "C" int afe_amg_destroy(afe_amg *g) {
   (void)hipStreamSynchronize(g->ctx->stream);
   drop_graphs(g);
   if (g->h_sc) (void)hipHostFree(g->h_sc);
+  if (g->live0) (void)hipEventDestroy(g->live0);
+  if (g->live1) (void)hipEventDestroy(g->live1);
   delete g;
   return AFE_OK;
 }
@@ -1653,9 +1661,11 @@ template <class T> static void issue_cg_iteration(afe_amg *g) {
   DevLevel &d0 = g->lv[0];
   hipStream_t s = g->ctx->stream;
   const int64_t n = d0.n;
+  if (g->live) HIPCHK(hipEventRecord(g->live0, s));
   k_spmv_dot<<<g_tune.spmv_blocks, BLK, 0, s>>>(g->nat_si.p, g->nat_ns, g->nat_cols.p, g->nat_v64.p,
                                                  g->nat_diag.p, g->nat_rowmap.p, g->wp.p, g->wq.p, g->part.p,
                                                  g_tune.xcd);
+  if (g->live) HIPCHK(hipEventRecord(g->live1, s));
   k_finalize<<<1, FIN_THREADS, 0, s>>>(g->part.p, g_tune.spmv_blocks, 1, &g->sc.p->pq);
   k_cg_update<T><<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wx.p, g->wxl.p, g->wr.p, g->wp.p, g->wq.p,
                                              Prec<T>::b(d0), Prec<T>::x(d0), g->iperm0.p, g->part.p);
@@ -1724,6 +1734,12 @@ static int pcg_impl(afe_amg *g, double *d_x, const double *d_b, double rtol, int
     while (it < max_it) {
       launch_cg_iteration<T>(g);
       HIPCHK(hipStreamSynchronize(s));
+      if (g->live) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, g->live0, g->live1));
+        g->live_ms += ms;
+        g->live_n++;
+      }
       it++;
       rr = g->h_sc->rr;
       if (verbose())
@@ -1891,6 +1907,30 @@ extern "C" int afe_amg_set_smooth(afe_amg *g, int32_t smooth) {
   if (!g || smooth < 0 || smooth > 16) return fail(AFE_E_ARG, "bad smoothing count");
   g->prm.smooth = smooth;
   return AFE_OK;
+}
+
+// live timing of the level-0 SpMV inside afe_amg_pcg: enable = 1 resets and
+// starts recording, 0 stops; the average so far is returned either way
+extern "C" int afe_amg_live_timing(afe_amg *g, int32_t enable, double *avg_ms, int64_t *launches) {
+  API_BEGIN
+  if (!g || !g->ctx) return fail(AFE_E_ARG, "bad arguments");
+  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  HIPCHK(hipSetDevice(g->ctx->device));
+  if (avg_ms) *avg_ms = g->live_n ? g->live_ms / g->live_n : 0.0;
+  if (launches) *launches = g->live_n;
+  if (enable && !g->live0) {
+    HIPCHK(hipEventCreate(&g->live0));
+    HIPCHK(hipEventCreate(&g->live1));
+  }
+  if (enable) {
+    g->live_ms = 0;
+    g->live_n = 0;
+  }
+  if ((enable != 0) != g->live) {
+    g->live = enable != 0;
+    drop_graphs(g); // the captured iteration gains or loses the event records
+  }
+  API_END
 }
 
 // levels of at most `rows` rows run in the single-workgroup tail from now on
