@@ -1657,15 +1657,18 @@ template <class T> static void precondition_perm(afe_amg *g, const double *r, bo
 
 // one CG iteration after the direction p is set: q = A p, alpha, x/r update,
 // rr to the host slot, then the next preconditioned direction
-template <class T> static void issue_cg_iteration(afe_amg *g) {
-  DevLevel &d0 = g->lv[0];
-  hipStream_t s = g->ctx->stream;
-  const int64_t n = d0.n;
-  if (g->live) HIPCHK(hipEventRecord(g->live0, s));
+// q = A p and (p, q)
+static void issue_cg_spmv(afe_amg *g, hipStream_t s) {
   k_spmv_dot<<<g_tune.spmv_blocks, BLK, 0, s>>>(g->nat_si.p, g->nat_ns, g->nat_cols.p, g->nat_v64.p,
                                                  g->nat_diag.p, g->nat_rowmap.p, g->wp.p, g->wq.p, g->part.p,
                                                  g_tune.xcd);
-  if (g->live) HIPCHK(hipEventRecord(g->live1, s));
+}
+
+// the rest of a CG iteration after the SpMV
+template <class T> static void issue_cg_rest(afe_amg *g) {
+  DevLevel &d0 = g->lv[0];
+  hipStream_t s = g->ctx->stream;
+  const int64_t n = d0.n;
   k_finalize<<<1, FIN_THREADS, 0, s>>>(g->part.p, g_tune.spmv_blocks, 1, &g->sc.p->pq);
   k_cg_update<T><<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wx.p, g->wxl.p, g->wr.p, g->wp.p, g->wq.p,
                                              Prec<T>::b(d0), Prec<T>::x(d0), g->iperm0.p, g->part.p);
@@ -1681,11 +1684,22 @@ template <class T> static void issue_cg_iteration(afe_amg *g) {
   (void)prec;
 }
 
+// one CG iteration: captured as one graph; with live timing the SpMV is
+// launched directly between two HIP events and the rest is the graph
 template <class T> static void launch_cg_iteration(afe_amg *g) {
   const int prec = std::is_same<T, float>::value ? 1 : 0;
   hipStream_t st = g->ctx->stream;
+  if (g->live) {
+    HIPCHK(hipEventRecord(g->live0, st));
+    issue_cg_spmv(g, st);
+    HIPCHK(hipEventRecord(g->live1, st));
+  }
+  auto issue = [&]() {
+    if (!g->live) issue_cg_spmv(g, st);
+    issue_cg_rest<T>(g);
+  };
   if (!use_graphs()) {
-    issue_cg_iteration<T>(g);
+    issue();
     return;
   }
   hipGraphExec_t &ex = g->cg_graph[prec];
@@ -1696,7 +1710,7 @@ template <class T> static void launch_cg_iteration(afe_amg *g) {
   if (!ex) {
     hipGraph_t graph;
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    issue_cg_iteration<T>(g);
+    issue();
     HIPCHK(hipStreamEndCapture(st, &graph));
     HIPCHK(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
     HIPCHK(hipGraphDestroy(graph));
