@@ -347,9 +347,10 @@ int afe_server_timing(afe_server *s, double *out);
  * 2 FP64 sweep, 3 FP32 residual.  *bytes = algorithmic bytes per launch. */
 int afe_amg_kernel_timing(afe_amg *g, int32_t which, int32_t reps,
                           double *avg_ms, double *bytes);
-/* live timing of the level-0 FP64 SpMV inside afe_amg_pcg (HIP events on
- * the solver stream around that kernel in every CG iteration): enable = 1
- * resets and starts, 0 stops; *avg_ms / *launches = the average so far */
+/* live timing of the level-0 FP64 SpMV inside afe_amg_pcg (a start/stop
+ * HIP event pair attached to that kernel's dispatch on the solver stream in
+ * every CG iteration; the iterate is unchanged): enable = 1 resets and
+ * starts, 0 stops; *avg_ms / *launches = the average so far */
 int afe_amg_live_timing(afe_amg *g, int32_t enable, double *avg_ms,
                         int64_t *launches);
 /* average duration of `reps` launches of a plain streaming copy of `bytes`

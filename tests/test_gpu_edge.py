@@ -64,3 +64,32 @@ def test_batch_with_single_level_members(ctx):
         done, st, code = bt.result(slots[k])
         assert done and code == 0 and st.rel_res_l2 <= 1e-10
     bt.close()
+
+
+def test_live_timing_keeps_results(ctx):
+    """The SpMV launched with dispatch-timed events gives the same iterate as
+    the captured iteration, and one timing per CG iteration."""
+    import afepack_amd as afe
+    print("case live timing P2 n=64", flush=True)
+    c, ev, vm = make_mesh(2, 64, 0.2, 3)
+    sp = afe.Space(ctx, 2, c, ev, vm)
+    A = afe.Matrix(sp)
+    A.assemble(afe.OP_STIFF, 4)
+    b = sp.rhs(4, afe.F_POISSON)
+    A.apply_dirichlet(b)
+    g = afe.Amg(A, afe.AmgParams(lazy=1, smooth=1, build_fp32=1))
+    x0 = afe.DeviceVector(ctx, sp.n_dof)
+    st0 = g.pcg(x0, b, rtol=1e-10, max_it=200, prec=afe.PREC_FP32)
+    g.live_timing(True)
+    x1 = afe.DeviceVector(ctx, sp.n_dof)
+    st1 = g.pcg(x1, b, rtol=1e-10, max_it=200, prec=afe.PREC_FP32)
+    ms, n = g.live_timing(False)
+    assert st0.converged == 1 and st1.converged == 1
+    assert st1.iterations == st0.iterations
+    assert np.array_equal(x1.numpy(), x0.numpy())
+    assert n == st1.iterations and ms > 0.0
+    # stopped: another solve adds no launches
+    x2 = afe.DeviceVector(ctx, sp.n_dof)
+    g.pcg(x2, b, rtol=1e-10, max_it=200, prec=afe.PREC_FP32)
+    ms2, n2 = g.live_timing(False)
+    assert n2 == n and ms2 == ms
