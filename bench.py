@@ -69,7 +69,8 @@ def cpu_baseline(rp, col, val, b, budget_s, n_dof):
         "sample": (f"{k} stationary V(3,3) cycles of the oracle AMGSolver (lazy hierarchy, "
                    f"lexicographic Gauss-Seidel) on the same {n_dof}-DOF system: "
                    f"{per_cycle:.3f} s/cycle, L1 contraction {rho:.4f}/cycle -> "
-                   f"{cycles} cycles to 1e-10; hierarchy setup {t_setup:.1f} s untimed; "
+                   f"{cycles} cycles to 1e-10 (an upper bound on its rate: its FP64 iterate "
+                   f"stops near the rounding floor of this system); hierarchy setup {t_setup:.1f} s untimed; "
                    f"cpu {os.cpu_count()} visible, 1 used"),
     }
 
@@ -183,15 +184,16 @@ def run_c2(args):
     del gg
 
     # autotune 2 (mixed form): coarsest-level size through the reference's own
-    # AMGSolver arguments (nmpo, nmpsd) and the PCG smoothing count V(s,s)
+    # AMGSolver arguments (nmpo, nmpsd), the PCG smoothing count V(s,s) and
+    # the first level of the single-workgroup tail
     choices = [(50, 0.382), (2048, 0.0)] if args.nmpo == 0 else [(args.nmpo, args.nmpsd)]
     smooths = (1, 2) if args.smooth == 0 else (args.smooth,)
     tails = (4096, 12288)
     best = None
     for nmpo, nmpsd in choices:
+        gg, t_b = build(nmpo, nmpsd)
         for tail in tails:
-            afe.set_tuning(best_t[1], best_t[2], tail)
-            gg, t_b = build(nmpo, nmpsd)
+            gg.set_tail_rows(tail)
             for s in smooths:
                 solve_with(gg, afe.PREC_FP32, s)
                 dt, st = solve_with(gg, afe.PREC_FP32, s)
@@ -199,7 +201,7 @@ def run_c2(args):
                     f"{st.iterations} its (levels {gg.info()['n']}, setup {t_b:.2f} s)")
                 if best is None or dt < best[0]:
                     best = (dt, nmpo, nmpsd, s, tail)
-            del gg
+        del gg
     _, nmpo, nmpsd, s_mixed, tail_rows = best
     afe.set_tuning(best_t[1], best_t[2], tail_rows)
     g, t_amg = build(nmpo, nmpsd)
@@ -242,8 +244,27 @@ def run_c2(args):
         plain = {"form": "stationary FP64 V(3,3) (AMGSolver::solve)", "seconds": tp,
                  "cycles": stp.iterations, "converged": bool(stp.converged),
                  "dof_per_s": (sp.n_dof / tp) if stp.converged else 0.0,
-                 "rel_res_at_exit": stp.rel_res_l2}
+                 "rel_res_at_exit": stp.rel_res_l2,
+                 "note": ("the iterate is one FP64 vector: its true residual stops near the FP64 "
+                          "rounding floor of this system, so 1e-10 may be out of reach; the "
+                          "comparison at 1e-9 below is reachable by both forms")}
         log(f"plain: {tp*1e3:.2f} ms, {stp.iterations} cycles, conv {stp.converged}")
+        # both forms at a tolerance the FP64 iterate can reach
+        x.zero()
+        t0 = time.perf_counter()
+        stp9 = g.solve(x, b, step=1500, mode=1, rtol=1e-9)
+        tp9 = time.perf_counter() - t0
+        g.set_smooth(s_mixed)
+        x.zero()
+        t0 = time.perf_counter()
+        stm9 = g.pcg(x, b, rtol=1e-9, max_it=1000, prec=afe.PREC_FP32)
+        tm9 = time.perf_counter() - t0
+        plain["at_1e-9"] = {"plain_seconds": tp9, "plain_cycles": stp9.iterations,
+                            "plain_converged": bool(stp9.converged), "mixed_seconds": tm9,
+                            "mixed_iterations": stm9.iterations,
+                            "mixed_vs_plain": (tp9 / tm9) if stp9.converged else None}
+        log(f"at 1e-9: plain {tp9*1e3:.1f} ms ({stp9.iterations} cycles, conv {stp9.converged}), "
+            f"mixed {tm9*1e3:.1f} ms ({stm9.iterations} its)")
 
     # live roofline of the dominant kernels (HIP events on the solver stream)
     g.set_smooth(s_mixed)
@@ -309,6 +330,7 @@ def run_c2(args):
         "pcg64": {"seconds": t64, "iterations": st64.iterations, "dof_per_s": sp.n_dof / t64},
         "plain": plain,
         "mixed_vs_plain_stationary": (value / plain["dof_per_s"]) if plain and plain["dof_per_s"] else None,
+        "mixed_vs_plain_at_1e-9": plain["at_1e-9"]["mixed_vs_plain"] if plain else None,
         "mixed_vs_pcg64": value / (sp.n_dof / t64),
         "mixed_vs_fp64_solution_rel_l2": diff,
         "assembly_ms": 1e3 * t_asm,
