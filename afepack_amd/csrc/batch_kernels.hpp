@@ -55,8 +55,9 @@ __global__ __launch_bounds__(BLK) void kb_gs(const Seg *__restrict__ segs, int n
   const int lane = threadIdx.x % WAVE;
   if (lane >= q.nrow) return;
   const int r = q.row0 + lane;
+  const T br = d.b[r], ir = d.invd[r];
   const T acc = sell_dot<T>(q, lane, d.cols, d.vals, d.x);
-  d.x[r] = (d.b[r] - acc) * d.invd[r];
+  d.x[r] = (br - acc) * ir;
 }
 
 template <class T>
@@ -70,8 +71,9 @@ __global__ __launch_bounds__(BLK) void kb_residual(const Seg *__restrict__ segs,
   const int lane = threadIdx.x % WAVE;
   if (lane >= q.nrow) return;
   const int r = q.row0 + lane;
+  const T br = d.b[r], dr = d.diag[r], xr = d.x[r];
   const T acc = sell_dot<T>(q, lane, d.cols, d.vals, d.x);
-  d.t[r] = d.b[r] - d.diag[r] * d.x[r] - acc;
+  d.t[r] = br - dr * xr - acc;
 }
 
 // b_{l+1} = P t_l, x_{l+1} = 0 (lv: level l, lvn: level l+1 of each member)
@@ -106,7 +108,8 @@ __global__ __launch_bounds__(BLK) void kb_prolong(const Seg *__restrict__ segs, 
   const int lane = threadIdx.x % WAVE;
   if (lane >= q.nrow) return;
   const int r = q.row0 + lane;
-  d.x[r] += sell_dot<T>(q, lane, d.t_cols, d.t_vals, dn.x);
+  const T xr = d.x[r];
+  d.x[r] = xr + sell_dot<T>(q, lane, d.t_cols, d.t_vals, dn.x);
 }
 
 // the coarse tail of each member: one workgroup per member
@@ -141,7 +144,7 @@ __global__ __launch_bounds__(BLK) void kb_spmv_dot(const Seg *__restrict__ segs,
       const int pos = q.row0 + lane;
       const int r = m.nat_rowmap[pos];
       const double xr = m.p[r];
-      const double yr = m.nat_diag[pos] * xr + sell_dot<double>(q, lane, m.nat_cols, m.nat_v, m.p);
+      const double yr = m.nat_diag[pos] * xr + sell_dot<double, false, SELL_CHUNK_SPMV>(q, lane, m.nat_cols, m.nat_v, m.p);
       m.q[r] = yr;
       dot = xr * yr;
     }
