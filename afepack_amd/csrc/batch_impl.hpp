@@ -314,7 +314,8 @@ template <class T> static void batch_build_epoch(afe_batch *b) {
 }
 
 // one CG iteration of every active member (the batched form of
-// issue_cg_spmv + issue_cg_rest), then the scalars of all members gathered
+// issue_cg_spmv, issue_cg_update and issue_cg_precondition), then the
+// scalars of all members gathered
 template <class T> static void batch_issue_step(afe_batch *b, hipStream_t s) {
   const int K = (int)b->act.size();
   const Seg *S = b->segs.p;

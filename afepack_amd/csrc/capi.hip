@@ -877,6 +877,7 @@ struct DevLevel {
   int64_t n = 0;
   int ncol = 0;
   std::vector<int32_t> colour_slice; // host copy
+  int64_t rows_c0 = 0;               // rows of colour 0 (first in the colour order)
   int ns = 0, p_ns = 0, t_ns = 0;
   DevBuf<SliceInfo> si, p_si, t_si;
   DevBuf<int32_t> cols, p_cols, t_cols, col_slice;
@@ -931,9 +932,13 @@ struct afe_amg {
   bool has32 = false;
   // captured V-cycles: [precision][symmetric]
   hipGraphExec_t vc_graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-  hipGraphExec_t cg_graph[2] = {nullptr, nullptr};
+  // captured CG iteration in two parts: [part][precision]; part 0 ends with
+  // the copy of the scalars to the host, part 1 is the preconditioner and the
+  // direction update
+  hipGraphExec_t cg_graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   int graph_nu[2][2] = {{-1, -1}, {-1, -1}};
-  int cg_nu[2] = {-1, -1};
+  int cg_nu[2][2] = {{-1, -1}, {-1, -1}};
+  hipEvent_t it_ev = nullptr; // recorded after part 0 of each CG iteration
 };
 
 static void drop_graphs(afe_amg *g) {
@@ -943,11 +948,12 @@ static void drop_graphs(afe_amg *g) {
         (void)hipGraphExecDestroy(e);
         e = nullptr;
       }
-  for (auto &e : g->cg_graph)
-    if (e) {
-      (void)hipGraphExecDestroy(e);
-      e = nullptr;
-    }
+  for (auto &row : g->cg_graph)
+    for (auto &e : row)
+      if (e) {
+        (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
 }
 
 // host CSR of the fine matrix (values from the device)
@@ -1077,6 +1083,7 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
     Sell &sa = hl.sa;
     d.ns = (int)sa.slices.size();
     d.colour_slice = sa.group_slice;
+    d.rows_c0 = sa.group_row.size() > 1 ? sa.group_row[1] : hl.n;
     d.si.upload(ctx, sa.slices);
     d.cols.upload(ctx, sa.cols);
     d.col_slice.upload(ctx, sa.group_slice);
@@ -1300,6 +1307,7 @@ extern "C" int afe_amg_destroy(afe_amg *g) {
   if (g->h_sc) (void)hipHostFree(g->h_sc);
   if (g->live0) (void)hipEventDestroy(g->live0);
   if (g->live1) (void)hipEventDestroy(g->live1);
+  if (g->it_ev) (void)hipEventDestroy(g->it_ev);
   delete g;
   return AFE_OK;
 }
@@ -1472,15 +1480,24 @@ static void gs_colour(DevLevel &d, int c, hipStream_t st) {
 }
 
 // one V(nu, nu) cycle on level-0 vectors (b, x) of precision T; the
-// post-smoother runs the colours in reverse order when symmetric
+// post-smoother runs the colours in reverse order when symmetric.  x0_zero:
+// the cycle starts from x = 0 on level 0, and the first colour kernel writes
+// all of level 0's x (the caller need not clear it).
 template <class T>
-static void issue_vcycle(afe_amg *g, int nu, bool symmetric, hipStream_t st) {
+static void issue_vcycle(afe_amg *g, int nu, bool symmetric, hipStream_t st, bool x0_zero) {
   const int Lc = (int)g->lv.size() - 1;
   for (int l = 0; l < g->tail0; l++) {
     DevLevel &d = g->lv[l];
     DevLevel &dn = g->lv[l + 1];
+    if (l == 0 && x0_zero && nu == 0) k_fill<T><<<nblk(d.n), 256, 0, st>>>(d.n, Prec<T>::x(d), (T)0);
     for (int it = 0; it < nu; it++)
-      for (int c = 0; c < d.ncol; c++) gs_colour<T>(d, c, st);
+      for (int c = 0; c < d.ncol; c++) {
+        if (l == 0 && it == 0 && c == 0 && x0_zero) {
+          k_gs_first<T><<<nblk(d.n), 256, 0, st>>>(d.n, d.rows_c0, Prec<T>::b(d), Prec<T>::invd(d), Prec<T>::x(d));
+          continue;
+        }
+        gs_colour<T>(d, c, st);
+      }
     if (d.stream) {
       k_residual<T, true><<<sblk(d.ns), BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, Prec<T>::v(d), Prec<T>::diag(d),
                                                        Prec<T>::b(d), Prec<T>::x(d), Prec<T>::t(d), g_tune.xcd);
@@ -1533,9 +1550,9 @@ static void launch_graph_or_issue(afe_amg *g, int prec, bool symmetric, int nu) 
   hipStream_t st = g->ctx->stream;
   if (!use_graphs()) {
     if (prec == 1)
-      issue_vcycle<float>(g, nu, symmetric, st);
+      issue_vcycle<float>(g, nu, symmetric, st, symmetric);
     else
-      issue_vcycle<double>(g, nu, symmetric, st);
+      issue_vcycle<double>(g, nu, symmetric, st, symmetric);
     return;
   }
   hipGraphExec_t &ex = g->vc_graph[prec][symmetric ? 1 : 0];
@@ -1547,9 +1564,9 @@ static void launch_graph_or_issue(afe_amg *g, int prec, bool symmetric, int nu) 
     hipGraph_t graph;
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     if (prec == 1)
-      issue_vcycle<float>(g, nu, symmetric, st);
+      issue_vcycle<float>(g, nu, symmetric, st, symmetric);
     else
-      issue_vcycle<double>(g, nu, symmetric, st);
+      issue_vcycle<double>(g, nu, symmetric, st, symmetric);
     HIPCHK(hipStreamEndCapture(st, &graph));
     HIPCHK(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
     HIPCHK(hipGraphDestroy(graph));
@@ -1674,42 +1691,46 @@ static void issue_cg_spmv(afe_amg *g, hipStream_t s, hipEvent_t start = nullptr,
                                                  g_tune.xcd);
 }
 
-// the rest of a CG iteration after the SpMV
-template <class T> static void issue_cg_rest(afe_amg *g) {
+// part 0 of a CG iteration after the SpMV: alpha, the x / r update with the
+// FP32 (or FP64) right-hand side of the V-cycle, (r, r), and the scalars to
+// the host
+template <class T> static void issue_cg_update(afe_amg *g) {
   DevLevel &d0 = g->lv[0];
   hipStream_t s = g->ctx->stream;
   const int64_t n = d0.n;
   k_finalize<<<1, FIN_THREADS, 0, s>>>(g->part.p, g_tune.spmv_blocks, 1, &g->sc.p->pq);
+  // level 0 runs as launches (tail0 > 0): the V-cycle's first colour kernel
+  // sets x0, so the update does not clear it
   k_cg_update<T><<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wx.p, g->wxl.p, g->wr.p, g->wp.p, g->wq.p,
-                                             Prec<T>::b(d0), Prec<T>::x(d0), g->iperm0.p, g->part.p);
+                                             Prec<T>::b(d0), g->tail0 > 0 ? nullptr : Prec<T>::x(d0),
+                                             g->iperm0.p, g->part.p);
   k_finalize<<<1, FIN_THREADS, 0, s>>>(g->part.p, RED_BLOCKS, 1, &g->sc.p->rr);
   HIPCHK(hipMemcpyAsync(g->h_sc, g->sc.p, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
-  const int prec = std::is_same<T, float>::value ? 1 : 0;
-  // V-cycle issued inline (nested graph capture is not used)
-  issue_vcycle<T>(g, g->prm.smooth, true, s);
+}
+
+// part 1: z = M r (the V-cycle, issued inline: nested capture is not used),
+// the flexible beta and p = z + beta p
+template <class T> static void issue_cg_precondition(afe_amg *g) {
+  DevLevel &d0 = g->lv[0];
+  hipStream_t s = g->ctx->stream;
+  const int64_t n = d0.n;
+  issue_vcycle<T>(g, g->prm.smooth, true, s, true);
   k_take_z<T><<<RED_BLOCKS, 256, 0, s>>>(n, Prec<T>::x(d0), g->wr.p, g->wz.p, g->iperm0.p, g->part.p);
   k_finalize<<<1, FIN_THREADS, 0, s>>>(g->part.p, RED_BLOCKS, 2, &g->sc.p->rz_new);
   k_cg_beta<<<1, 64, 0, s>>>(g->sc.p, 0);
   k_p_update<<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wz.p, g->wp.p);
-  (void)prec;
 }
 
-// one CG iteration: captured as one graph; with live timing the SpMV is
-// launched directly with its dispatch-timed events and the rest is the graph
-template <class T> static void launch_cg_iteration(afe_amg *g) {
-  const int prec = std::is_same<T, float>::value ? 1 : 0;
+// launches one captured part (capturing it on first use or after a change
+// of the smoothing count)
+template <class F> static void launch_part(afe_amg *g, int part, int prec, F issue) {
   hipStream_t st = g->ctx->stream;
-  if (g->live) issue_cg_spmv(g, st, g->live0, g->live1);
-  auto issue = [&]() {
-    if (!g->live) issue_cg_spmv(g, st);
-    issue_cg_rest<T>(g);
-  };
   if (!use_graphs()) {
     issue();
     return;
   }
-  hipGraphExec_t &ex = g->cg_graph[prec];
-  if (ex && g->cg_nu[prec] != g->prm.smooth) {
+  hipGraphExec_t &ex = g->cg_graph[part][prec];
+  if (ex && g->cg_nu[part][prec] != g->prm.smooth) {
     (void)hipGraphExecDestroy(ex);
     ex = nullptr;
   }
@@ -1720,9 +1741,27 @@ template <class T> static void launch_cg_iteration(afe_amg *g) {
     HIPCHK(hipStreamEndCapture(st, &graph));
     HIPCHK(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
     HIPCHK(hipGraphDestroy(graph));
-    g->cg_nu[prec] = g->prm.smooth;
+    g->cg_nu[part][prec] = g->prm.smooth;
   }
   HIPCHK(hipGraphLaunch(ex, st));
+}
+
+// one CG iteration: part 0 (with the SpMV; with live timing the SpMV is
+// launched directly with its dispatch-timed events), the event it_ev, part 1.
+// The host waits for it_ev only: it reads (r, r) and queues the next
+// iteration while part 1 still runs, so the device does not idle between
+// iterations.
+template <class T> static void launch_cg_iteration(afe_amg *g) {
+  const int prec = std::is_same<T, float>::value ? 1 : 0;
+  hipStream_t st = g->ctx->stream;
+  if (!g->it_ev) HIPCHK(hipEventCreateWithFlags(&g->it_ev, hipEventDisableTiming));
+  if (g->live) issue_cg_spmv(g, st, g->live0, g->live1);
+  launch_part(g, 0, prec, [&]() {
+    if (!g->live) issue_cg_spmv(g, st);
+    issue_cg_update<T>(g);
+  });
+  HIPCHK(hipEventRecord(g->it_ev, st));
+  launch_part(g, 1, prec, [&]() { issue_cg_precondition<T>(g); });
 }
 
 template <class T>
@@ -1753,7 +1792,7 @@ static int pcg_impl(afe_amg *g, double *d_x, const double *d_b, double rtol, int
     precondition_perm<T>(g, g->wr.p, true);
     while (it < max_it) {
       launch_cg_iteration<T>(g);
-      HIPCHK(hipStreamSynchronize(s));
+      HIPCHK(hipEventSynchronize(g->it_ev));
       if (g->live) {
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, g->live0, g->live1));
@@ -1765,7 +1804,10 @@ static int pcg_impl(afe_amg *g, double *d_x, const double *d_b, double rtol, int
       if (verbose())
         fprintf(stderr, "pcg it %d rel res %.3e pq %.3e rz %.3e beta %.3e\n", it, std::sqrt(rr / bb),
                 g->h_sc->pq, g->h_sc->rz, g->h_sc->beta);
-      if (!(rr == rr)) return fail(AFE_E_NOTCONVERGED, "CG breakdown (NaN residual)");
+      if (!(rr == rr)) {
+        HIPCHK(hipStreamSynchronize(s)); // part 1 of the iteration is still queued
+        return fail(AFE_E_NOTCONVERGED, "CG breakdown (NaN residual)");
+      }
       if (rr <= pthr) break;
     }
     // true residual of the pair x_hi + x_lo; restart from it when the

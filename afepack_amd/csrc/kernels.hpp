@@ -582,6 +582,18 @@ __global__ __launch_bounds__(BLK) void k_gs(const SliceInfo *__restrict__ si, in
   x[r] = (br - acc) * ir;
 }
 
+// the first colour of a sweep that starts from x = 0 (level 0 of the
+// preconditioner): every neighbour of a colour-0 row is still zero, so the
+// row takes (b - 0) * invd = b * invd, bit for bit what k_gs computes; the
+// rows of the other colours get the zero the sweep starts from.  Colour-0
+// rows come first in the colour order (rows [0, n0)).
+template <class T>
+__global__ __launch_bounds__(256) void k_gs_first(int64_t n, int64_t n0, const T *__restrict__ b,
+                                                  const T *__restrict__ invd, T *__restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = i < n0 ? b[i] * invd[i] : (T)0;
+}
+
 // t = b - A x (A = diag + stored off-diagonals)
 template <class T, bool NT>
 __global__ __launch_bounds__(BLK) void k_residual(const SliceInfo *__restrict__ si, int ns,
@@ -1096,7 +1108,7 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n, const CgScalars *_
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
     b0[iperm ? iperm[i] : i] = (T)ri;
-    x0[i] = 0;
+    if (x0) x0[i] = 0; // null: the V-cycle's first colour kernel sets x0
     rr += ri * ri;
   }
   rr = block_sum(rr, sh);

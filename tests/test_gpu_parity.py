@@ -177,17 +177,9 @@ def test_reference_solve_semantics(ctx):
     assert st.rel_res_l1 < 100 * hist[8] / hist[0]
 
 
-@pytest.mark.parametrize("nmpo,nmpsd", [(50, 0.382), (400, 0.0)])
-def test_vcycle_matches_dense_model(ctx, nmpo, nmpsd):
-    """One symmetric FP64 V(1,1)-cycle from zero on the device against a dense
-    numpy model of the same cycle: multicolour Gauss-Seidel in the device's
-    colour order (forward before, reverse after the coarse correction),
-    restriction P, prolongation P^T, exact coarse solve."""
-    import afepack_amd as afe
-    sp, osp, A, OA, f, of = _system(ctx, 1, 32, 0.0, 4)
-    g = afe.Amg(A, afe.AmgParams(lazy=1, nmpo=nmpo, nmpsd=nmpsd, smooth=1))
-    z = _zeros(ctx, sp.n_dof)
-    g.vcycle(z, f, afe.PREC_FP64)
+def _dense_levels(g):
+    """Dense copies of the device hierarchy: level matrices, restrictions P
+    and the stored patterns (explicit zeros included)."""
     L = g.info()["levels"]
     As, Ps, Gs = [], [], []
     for l in range(L):
@@ -205,20 +197,35 @@ def test_vcycle_matches_dense_model(ctx, nmpo, nmpsd):
             for i in range(m):
                 D[i, col[rp[i]:rp[i + 1]]] += val[rp[i]:rp[i + 1]]
             Ps.append(D)
+    return As, Ps, Gs
 
-    def colours(S):
-        # greedy first-fit in index order on the symmetrised off-diagonal
-        # pattern (explicit zeros included)
-        n = S.shape[0]
-        G = (S | S.T) & ~np.eye(n, dtype=bool)
-        c = -np.ones(n, int)
-        for i in range(n):
-            used = set(c[np.nonzero(G[i])[0]]) - {-1}
-            k = 0
-            while k in used:
-                k += 1
-            c[i] = k
-        return c
+
+def _model_colours(S):
+    # greedy first-fit in index order on the symmetrised off-diagonal
+    # pattern (explicit zeros included)
+    n = S.shape[0]
+    G = (S | S.T) & ~np.eye(n, dtype=bool)
+    c = -np.ones(n, int)
+    for i in range(n):
+        used = set(c[np.nonzero(G[i])[0]]) - {-1}
+        k = 0
+        while k in used:
+            k += 1
+        c[i] = k
+    return c
+
+
+def _model_vcycle(As, Ps, Gs, nu):
+    """z = M b for the symmetric V(nu, nu)-cycle from zero: multicolour
+    Gauss-Seidel in the device's colour order (forward before, reverse after
+    the coarse correction), restriction P, prolongation P^T, exact coarse
+    solve."""
+    L = len(As)
+    orders = []
+    for l in range(L - 1):
+        c = _model_colours(Gs[l])
+        orders.append((np.concatenate([np.nonzero(c == k)[0] for k in range(c.max() + 1)]),
+                       np.concatenate([np.nonzero(c == k)[0] for k in range(c.max(), -1, -1)])))
 
     def sweep(Am, x, b, order):
         for i in order:
@@ -227,17 +234,67 @@ def test_vcycle_matches_dense_model(ctx, nmpo, nmpsd):
     def vc(l, b):
         if l == L - 1:
             return np.linalg.solve(As[l], b)
-        c = colours(Gs[l])
-        fwd = np.concatenate([np.nonzero(c == k)[0] for k in range(c.max() + 1)])
         x = np.zeros_like(b)
-        sweep(As[l], x, b, fwd)
+        for _ in range(nu):
+            sweep(As[l], x, b, orders[l][0])
         x += Ps[l].T @ vc(l + 1, Ps[l] @ (b - As[l] @ x))
-        rev = np.concatenate([np.nonzero(c == k)[0] for k in range(c.max(), -1, -1)])
-        sweep(As[l], x, b, rev)
+        for _ in range(nu):
+            sweep(As[l], x, b, orders[l][1])
         return x
 
-    ref = vc(0, f.numpy())
+    return lambda b: vc(0, b)
+
+
+@pytest.mark.parametrize("nmpo,nmpsd", [(50, 0.382), (400, 0.0)])
+def test_vcycle_matches_dense_model(ctx, nmpo, nmpsd):
+    """One symmetric FP64 V(1,1)-cycle from zero on the device against the
+    dense numpy model of the same cycle."""
+    import afepack_amd as afe
+    sp, osp, A, OA, f, of = _system(ctx, 1, 32, 0.0, 4)
+    g = afe.Amg(A, afe.AmgParams(lazy=1, nmpo=nmpo, nmpsd=nmpsd, smooth=1))
+    z = _zeros(ctx, sp.n_dof)
+    g.vcycle(z, f, afe.PREC_FP64)
+    As, Ps, Gs = _dense_levels(g)
+    ref = _model_vcycle(As, Ps, Gs, 1)(f.numpy())
     assert np.linalg.norm(z.numpy() - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("smooth", [0, 1, 2])
+def test_pcg_iterates_match_model(ctx, smooth):
+    """Three iterations of the FP64 flexible PCG on the device (several
+    levels run as launches, the first level-0 colour from the zero guess, and
+    no smoothing at all) against numpy flexible PCG with the dense V-cycle
+    model: x_{k+1} = x_k + alpha p, alpha = (r, z) / (p, A p),
+    beta = ((r, z_new) - (r, z_old)) / (r_old, z_old)."""
+    import afepack_amd as afe
+    print(f"case pcg model smooth={smooth}", flush=True)
+    sp, osp, A, OA, f, of = _system(ctx, 1, 24, 0.1, 4)
+    g = afe.Amg(A, afe.AmgParams(lazy=1, nmpo=50, nmpsd=0.382, smooth=max(smooth, 1), build_fp32=1))
+    g.set_tail_rows(0)  # every level but the coarsest as separate launches
+    g.set_smooth(smooth)
+    As, Ps, Gs = _dense_levels(g)
+    M = _model_vcycle(As, Ps, Gs, smooth)
+    A0, b = As[0], f.numpy()
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = M(r)
+    p = z.copy()
+    rz = r @ z
+    its = 3
+    for _ in range(its):
+        q = A0 @ p
+        alpha = rz / (p @ q)
+        x = x + alpha * p
+        r = r - alpha * q
+        zn = M(r)
+        beta = (r @ zn - r @ z) / rz
+        rz = r @ zn
+        z = zn
+        p = z + beta * p
+    xd = _zeros(ctx, sp.n_dof)
+    st = g.pcg(xd, f, rtol=1e-15, max_it=its, prec=afe.PREC_FP64, check_conv=False)
+    assert st.iterations == its
+    assert np.linalg.norm(xd.numpy() - x) <= 1e-11 * np.linalg.norm(x)
 
 
 @pytest.mark.parametrize("kind,n,jitter,acc,modes,amp", [
