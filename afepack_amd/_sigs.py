@@ -83,6 +83,7 @@ SIGS = {
     "afe_amg_lazy_reinit": (C.c_int, [P, P]),
     "afe_amg_info": (C.c_int, [P, PI32, P, P, PI32]),
     "afe_amg_level_csr": (C.c_int, [P, I32, I32, PI64, PI64, PI64, P, P, P]),
+    "afe_amg_level_colours": (C.c_int, [P, I32, P]),
     "afe_amg_solve": (C.c_int, [P, P, P, F64, I64, I32, F64, C.POINTER(SolveStats)]),
     "afe_amg_precondition": (C.c_int, [P, P, P, I32]),
     "afe_amg_pcg": (C.c_int, [P, P, P, F64, I32, I32, C.POINTER(SolveStats)]),

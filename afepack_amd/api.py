@@ -369,6 +369,13 @@ class _AmgBase:
                                       _ptr(rp), _ptr(col), _ptr(val)))
         return m.value, n.value, rp, col, val
 
+    def level_colours(self, level):
+        """Multicolour Gauss-Seidel colour of every row of a device level."""
+        n = self.info()["n"][level]
+        out = np.empty(n, dtype=np.int32)
+        _chk(_lib().afe_amg_level_colours(self.h, level, _ptr(out)))
+        return out
+
     def coarse_inverse(self):
         info = self.info()
         nc = info["n"][-1]

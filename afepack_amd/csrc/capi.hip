@@ -878,6 +878,7 @@ struct DevLevel {
   int ncol = 0;
   std::vector<int32_t> colour_slice; // host copy
   int64_t rows_c0 = 0;               // rows of colour 0 (first in the colour order)
+  std::vector<int32_t> colour_row;   // host copy: rows of colour c are [colour_row[c], colour_row[c+1])
   int ns = 0, p_ns = 0, t_ns = 0;
   DevBuf<SliceInfo> si, p_si, t_si;
   DevBuf<int32_t> cols, p_cols, t_cols, col_slice;
@@ -1084,6 +1085,7 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
     d.ns = (int)sa.slices.size();
     d.colour_slice = sa.group_slice;
     d.rows_c0 = sa.group_row.size() > 1 ? sa.group_row[1] : hl.n;
+    d.colour_row = sa.group_row;
     d.si.upload(ctx, sa.slices);
     d.cols.upload(ctx, sa.cols);
     d.col_slice.upload(ctx, sa.group_slice);
@@ -1413,6 +1415,16 @@ extern "C" int afe_amg_info(const afe_amg *g, int32_t *n_levels, int64_t *level_
     if (level_nnz) level_nnz[l] = g->h.A[l].nnz();
   }
   if (coarse_full) *coarse_full = g->h.full;
+  return AFE_OK;
+}
+
+// colour of every row of a device level, in the level's own row numbering
+extern "C" int afe_amg_level_colours(const afe_amg *g, int32_t level, int32_t *colour) {
+  if (!g || !colour) return fail(AFE_E_ARG, "null argument");
+  if (level < 0 || level >= (int)g->lv.size()) return fail(AFE_E_ARG, "no such level");
+  const DevLevel &d = g->lv[level];
+  for (int c = 0; c + 1 < (int)d.colour_row.size(); c++)
+    for (int32_t r = d.colour_row[c]; r < d.colour_row[c + 1]; r++) colour[d.perm[r]] = c;
   return AFE_OK;
 }
 

@@ -13,6 +13,8 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <queue>
+#include <tuple>
 #include <stdexcept>
 
 namespace afe {
@@ -679,6 +681,43 @@ int colour_rows(const Csr &a, std::vector<int32_t> &colour) {
         }
   }
   colour.assign(n, -1);
+  if (n <= DSATUR_MAX_ROWS) {
+    // DSatur (Brelaz 1979): colour next the uncoloured row whose neighbours
+    // already use the most distinct colours (ties: more neighbours, then the
+    // lower index), with the lowest colour its neighbours do not use.  On
+    // the coarse levels it needs one or two colours fewer than first-fit in
+    // index order, and every colour is one launch per sweep.
+    std::vector<uint64_t> used(n, 0); // neighbour colours (at most 64 here)
+    std::vector<int32_t> sat(n, 0);
+    typedef std::tuple<int32_t, int64_t, int64_t> Key; // (saturation, degree, -row)
+    std::priority_queue<Key> heap;
+    for (int64_t i = 0; i < n; i++) heap.emplace(0, deg[i + 1] - deg[i], -i);
+    int ncol = 0;
+    bool overflow = false;
+    while (!heap.empty()) {
+      const Key top = heap.top();
+      heap.pop();
+      const int64_t i = -std::get<2>(top);
+      if (colour[i] >= 0 || std::get<0>(top) != sat[i]) continue; // stale entry
+      int c = 0;
+      while (c < 64 && (used[i] >> c & 1)) c++;
+      if (c == 64) {
+        overflow = true;
+        break;
+      }
+      colour[i] = c;
+      ncol = std::max(ncol, c + 1);
+      for (int64_t k = deg[i]; k < deg[i + 1]; k++) {
+        const int64_t j = adj[k];
+        if (colour[j] >= 0 || (used[j] >> c & 1)) continue;
+        used[j] |= uint64_t(1) << c;
+        sat[j]++;
+        heap.emplace(sat[j], deg[j + 1] - deg[j], -j);
+      }
+    }
+    if (!overflow) return ncol;
+    colour.assign(n, -1); // more than 64 colours: first-fit below
+  }
   std::vector<int64_t> stamp(64, -1);
   int ncol = 0;
   for (int64_t i = 0; i < n; i++) {

@@ -118,8 +118,11 @@ struct Sell {
   int64_t stored() const { return (int64_t)cols.size(); }
 };
 
-// greedy first-fit colouring (index order) of the symmetrised graph of the
-// nonzero off-diagonal entries
+// colouring of the symmetrised graph of the stored off-diagonal entries
+// (explicit zeros included): DSatur for levels of at most DSATUR_MAX_ROWS
+// rows, greedy first-fit in index order above (linear time; the fine P1/P2
+// levels get the same colour count either way)
+constexpr int64_t DSATUR_MAX_ROWS = 1 << 18;
 int colour_rows(const Csr &a, std::vector<int32_t> &colour);
 
 // perm: new -> old row order; col_map: old column -> new column

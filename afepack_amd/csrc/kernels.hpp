@@ -511,6 +511,12 @@ template <bool NT, class U> __device__ inline U ld_mat(const U *p) {
 // longer chunk cost more occupancy than they save in latency
 constexpr int SELL_CHUNK = 16;
 constexpr int SELL_CHUNK_SPMV = 8;
+// per-level launches: the level-0 streams (NT) use SELL_CHUNK; the small
+// FP32 levels take a whole row in one chunk (their rows are short, and the
+// launches are few waves long, so registers are free)
+template <class T, bool NT> struct LaunchChunk {
+  static constexpr int v = (!NT && sizeof(T) == 4) ? 24 : SELL_CHUNK;
+};
 template <class T, bool NT = false, int CH = SELL_CHUNK>
 __device__ inline T sell_dot(const SliceInfo &s, int lane, const int32_t *__restrict__ cols,
                              const T *__restrict__ vals, const T *__restrict__ x) {
@@ -578,7 +584,7 @@ __global__ __launch_bounds__(BLK) void k_gs(const SliceInfo *__restrict__ si, in
   if (lane >= q.nrow) return;
   const int r = q.row0 + lane;
   const T br = b[r], ir = invd[r]; // issued before the row's dependent loads
-  const T acc = sell_dot<T, NT>(q, lane, cols, vals, x);
+  const T acc = sell_dot<T, NT, LaunchChunk<T, NT>::v>(q, lane, cols, vals, x);
   x[r] = (br - acc) * ir;
 }
 
@@ -607,7 +613,7 @@ __global__ __launch_bounds__(BLK) void k_residual(const SliceInfo *__restrict__ 
   if (lane >= q.nrow) return;
   const int r = q.row0 + lane;
   const T br = b[r], dr = diag[r], xr = x[r];
-  const T acc = sell_dot<T, NT>(q, lane, cols, vals, x);
+  const T acc = sell_dot<T, NT, LaunchChunk<T, NT>::v>(q, lane, cols, vals, x);
   t[r] = br - dr * xr - acc;
 }
 
@@ -623,7 +629,7 @@ __global__ __launch_bounds__(BLK) void k_restrict(const SliceInfo *__restrict__ 
   const int lane = threadIdx.x % WAVE;
   if (lane >= q.nrow) return;
   const int r = q.row0 + lane;
-  bc[r] = sell_dot<T, NT>(q, lane, cols, vals, t);
+  bc[r] = sell_dot<T, NT, LaunchChunk<T, NT>::v>(q, lane, cols, vals, t);
   xc[r] = 0;
 }
 
@@ -639,7 +645,7 @@ __global__ __launch_bounds__(BLK) void k_prolong(const SliceInfo *__restrict__ s
   if (lane >= q.nrow) return;
   const int r = q.row0 + lane;
   const T xr = x[r];
-  x[r] = xr + sell_dot<T, NT>(q, lane, cols, vals, xc);
+  x[r] = xr + sell_dot<T, NT, LaunchChunk<T, NT>::v>(q, lane, cols, vals, xc);
 }
 
 // ---------------------------------------------------------------------------

@@ -243,6 +243,13 @@ int afe_amg_info(const afe_amg *g, int32_t *n_levels, int64_t *level_n,
 int afe_amg_level_csr(const afe_amg *g, int32_t level, int32_t which,
                       int64_t *m, int64_t *n, int64_t *nnz, int64_t *row_ptr,
                       int64_t *col, double *val);
+/* the multicolour Gauss-Seidel colouring of a device level: colour[i] of
+ * every row i of A_l (n_l entries).  The sweeps visit the colours in
+ * increasing order (the symmetric cycle's post-smoother in decreasing
+ * order); no two rows coupled by a stored entry share a colour.  The
+ * colouring replaces the row order of AMGSolver::GaussSidel
+ * (src/AMGSolver.cpp:362-381); no counterpart in the reference's interface. */
+int afe_amg_level_colours(const afe_amg *g, int32_t level, int32_t *colour);
 /* AMGSolver::solve(x, r, tol, step, mode) (src/AMGSolver.cpp:549-639):
  * exactly step+1 V(s,s) cycles from the initial guess d_x, FP64, multicolour
  * Gauss-Seidel in forward colour order on both sides.  If rtol > 0 the

@@ -200,30 +200,29 @@ def _dense_levels(g):
     return As, Ps, Gs
 
 
-def _model_colours(S):
-    # greedy first-fit in index order on the symmetrised off-diagonal
-    # pattern (explicit zeros included)
-    n = S.shape[0]
-    G = (S | S.T) & ~np.eye(n, dtype=bool)
-    c = -np.ones(n, int)
-    for i in range(n):
-        used = set(c[np.nonzero(G[i])[0]]) - {-1}
-        k = 0
-        while k in used:
-            k += 1
-        c[i] = k
-    return c
+def _device_colours(g, Gs):
+    """The device's colouring of each level, checked to be a colouring of
+    the symmetrised stored pattern (no two coupled rows share a colour)."""
+    out = []
+    for l, S in enumerate(Gs):
+        c = g.level_colours(l)
+        G = (S | S.T) & ~np.eye(S.shape[0], dtype=bool)
+        i, j = np.nonzero(G)
+        assert not np.any(c[i] == c[j]), f"level {l}: coupled rows share a colour"
+        assert c.min() == 0 and set(np.unique(c)) == set(range(c.max() + 1))
+        out.append(c)
+    return out
 
 
-def _model_vcycle(As, Ps, Gs, nu):
+def _model_vcycle(As, Ps, colours, nu):
     """z = M b for the symmetric V(nu, nu)-cycle from zero: multicolour
-    Gauss-Seidel in the device's colour order (forward before, reverse after
+    Gauss-Seidel in the given colour order (forward before, reverse after
     the coarse correction), restriction P, prolongation P^T, exact coarse
     solve."""
     L = len(As)
     orders = []
     for l in range(L - 1):
-        c = _model_colours(Gs[l])
+        c = colours[l]
         orders.append((np.concatenate([np.nonzero(c == k)[0] for k in range(c.max() + 1)]),
                        np.concatenate([np.nonzero(c == k)[0] for k in range(c.max(), -1, -1)])))
 
@@ -255,7 +254,7 @@ def test_vcycle_matches_dense_model(ctx, nmpo, nmpsd):
     z = _zeros(ctx, sp.n_dof)
     g.vcycle(z, f, afe.PREC_FP64)
     As, Ps, Gs = _dense_levels(g)
-    ref = _model_vcycle(As, Ps, Gs, 1)(f.numpy())
+    ref = _model_vcycle(As, Ps, _device_colours(g, Gs), 1)(f.numpy())
     assert np.linalg.norm(z.numpy() - ref) <= 1e-12 * np.linalg.norm(ref)
 
 
@@ -273,7 +272,7 @@ def test_pcg_iterates_match_model(ctx, smooth):
     g.set_tail_rows(0)  # every level but the coarsest as separate launches
     g.set_smooth(smooth)
     As, Ps, Gs = _dense_levels(g)
-    M = _model_vcycle(As, Ps, Gs, smooth)
+    M = _model_vcycle(As, Ps, _device_colours(g, Gs), smooth)
     A0, b = As[0], f.numpy()
     x = np.zeros_like(b)
     r = b.copy()
