@@ -510,8 +510,9 @@ extern "C" int afe_batch_run(afe_batch *b) {
 }
 
 extern "C" int afe_batch_result(afe_batch *b, int32_t slot, int32_t *done, afe_solve_stats *st) {
-  if (!b || slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot]) return fail(AFE_E_ARG, "bad slot");
+  if (!b) return fail(AFE_E_ARG, "null batch");
   std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+  if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot]) return fail(AFE_E_ARG, "bad slot");
   const BatchMember &m = *b->slots[slot];
   if (done) *done = m.done ? 1 : 0;
   if (st) {
@@ -528,8 +529,9 @@ extern "C" int afe_batch_result(afe_batch *b, int32_t slot, int32_t *done, afe_s
 
 // forget a finished (or abandon an active) member; its slot is reused
 extern "C" int afe_batch_remove(afe_batch *b, int32_t slot) {
-  if (!b || slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot]) return fail(AFE_E_ARG, "bad slot");
+  if (!b) return fail(AFE_E_ARG, "null batch");
   std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+  if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot]) return fail(AFE_E_ARG, "bad slot");
   if (b->slots[slot]->active) b->dirty = true;
   b->slots[slot].reset();
   return AFE_OK;

@@ -36,7 +36,8 @@ struct SrvRequest {
   int state = 0;                    // 0 pending, 1 setup, 2 ready, 3 solving, 4 done
   int status = AFE_OK;
   std::string err; // message of a failed set-up or solve
-  SrvEntry *e = nullptr;
+  SrvEntry *e = nullptr; // the cache entry while the request runs (null once done: it may be evicted)
+  int64_t n_dof = 0;      // DOFs of the request's space, kept when it finishes
   int32_t slot = -1;
   std::vector<double> x; // solution (when keep_x)
   afe_solve_stats st{};
@@ -246,6 +247,8 @@ static void srv_worker(afe_server *s) {
       q->err = afe_last_error();
       q->state = 4;
       q->t_done = std::chrono::steady_clock::now();
+      q->n_dof = e->n_dof;
+      q->e = nullptr;
       srv_free_entry(s, e);
       s->cache.erase(e->key);
       s->completed++;
@@ -273,7 +276,9 @@ static void srv_engine(afe_server *s) {
         q->err = afe_last_error();
         q->state = 4;
         q->t_done = std::chrono::steady_clock::now();
+        q->n_dof = q->e->n_dof;
         q->e->busy = false;
+        q->e = nullptr;
         s->completed++;
         s->cv_done.notify_all();
         continue;
@@ -302,7 +307,10 @@ static void srv_engine(afe_server *s) {
       q->status = rc != AFE_OK ? rc : st;
       if (q->r.keep_x) {
         q->x.resize(q->e->n_dof);
-        afe_dev_download(s->ctx, q->x.data(), q->e->d_x, q->e->n_dof * 8);
+        if (afe_dev_download(s->ctx, q->x.data(), q->e->d_x, q->e->n_dof * 8) != AFE_OK && q->status == AFE_OK) {
+          q->status = AFE_E_HIP;
+          q->err = afe_last_error();
+        }
       }
       afe_batch_remove(s->batch, q->slot);
       q->slot = -1;
@@ -313,8 +321,10 @@ static void srv_engine(afe_server *s) {
       s->t_wait_setup += seconds_between(q->t_submit, q->t_setup);
       s->t_wait_slot += seconds_between(q->t_ready, q->t_admit);
       s->t_solve += seconds_between(q->t_admit, q->t_done);
+      q->n_dof = q->e->n_dof;
       q->e->busy = false;
       q->e->last_use = ++s->clock;
+      q->e = nullptr;
       s->completed++;
       solving[k] = solving.back();
       solving.pop_back();
@@ -411,7 +421,7 @@ extern "C" int afe_server_wait(afe_server *s, int64_t id, double *h_x, int64_t *
   SrvRequest *q = it->second.get();
   while (q->state != 4 && !s->stop) s->cv_done.wait(lk);
   if (q->state != 4) return fail(AFE_E_ARG, "server stopped");
-  if (n_dof) *n_dof = q->e ? q->e->n_dof : 0;
+  if (n_dof) *n_dof = q->n_dof;
   if (h_x && !q->x.empty()) std::memcpy(h_x, q->x.data(), q->x.size() * sizeof(double));
   if (st) *st = q->st;
   if (latency_s) *latency_s = seconds_between(q->t_submit, q->t_done);

@@ -281,7 +281,9 @@ int afe_amg_set_smooth(afe_amg *g, int32_t smooth);
  * sweeps per side for every member (the member hierarchy's smoothing count
  * is set to it).  One active solve per hierarchy at a time. */
 /* A member's hierarchy and vectors must stay alive until the member is
- * removed or the batch destroyed. */
+ * removed or the batch destroyed.  The member's CG state lives in its
+ * hierarchy's work vectors: no afe_amg_pcg / _solve / _vcycle /
+ * _precondition on that hierarchy while it is an active member. */
 typedef struct afe_batch afe_batch;
 int afe_batch_create(afe_ctx *ctx, int32_t prec, int32_t smooth,
                      afe_batch **out);
