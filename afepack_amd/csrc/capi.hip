@@ -212,12 +212,15 @@ extern "C" int afe_ctx_destroy(afe_ctx *c) {
 
 extern "C" int afe_ctx_synchronize(afe_ctx *c) {
   API_BEGIN
+  if (!c) return fail(AFE_E_ARG, "null context");
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
   HIPCHK(hipStreamSynchronize(c->stream));
   API_END
 }
 
 extern "C" int afe_ctx_device_bytes(afe_ctx *c, int64_t *bytes) {
   if (!c || !bytes) return fail(AFE_E_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
   *bytes = c->bytes;
   return AFE_OK;
 }
@@ -259,6 +262,7 @@ extern "C" int afe_dev_free(afe_ctx *c, void *d_ptr) {
 extern "C" int afe_dev_upload(afe_ctx *c, void *d_dst, const void *h_src, int64_t bytes) {
   API_BEGIN
   if (!c || (bytes > 0 && (!d_dst || !h_src))) return fail(AFE_E_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(c->mu); // stream work: never inside another thread's capture
   HIPCHK(hipSetDevice(c->device));
   if (bytes > 0) HIPCHK(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -268,6 +272,7 @@ extern "C" int afe_dev_upload(afe_ctx *c, void *d_dst, const void *h_src, int64_
 extern "C" int afe_dev_download(afe_ctx *c, void *h_dst, const void *d_src, int64_t bytes) {
   API_BEGIN
   if (!c || (bytes > 0 && (!d_src || !h_dst))) return fail(AFE_E_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(c->mu); // stream work: never inside another thread's capture
   HIPCHK(hipSetDevice(c->device));
   if (bytes > 0) HIPCHK(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -277,6 +282,7 @@ extern "C" int afe_dev_download(afe_ctx *c, void *h_dst, const void *d_src, int6
 extern "C" int afe_dev_copy(afe_ctx *c, void *d_dst, const void *d_src, int64_t bytes) {
   API_BEGIN
   if (!c || (bytes > 0 && (!d_src || !d_dst))) return fail(AFE_E_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(c->mu); // stream work: never inside another thread's capture
   HIPCHK(hipSetDevice(c->device));
   if (bytes > 0) HIPCHK(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -286,6 +292,7 @@ extern "C" int afe_dev_copy(afe_ctx *c, void *d_dst, const void *d_src, int64_t 
 extern "C" int afe_dev_zero(afe_ctx *c, void *d_ptr, int64_t bytes) {
   API_BEGIN
   if (!c || (bytes > 0 && !d_ptr)) return fail(AFE_E_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(c->mu); // stream work: never inside another thread's capture
   HIPCHK(hipSetDevice(c->device));
   if (bytes > 0) HIPCHK(hipMemsetAsync(d_ptr, 0, bytes, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -623,6 +630,7 @@ extern "C" int afe_assemble_elements(afe_matrix *a, const double *h_elem_mat) {
 extern "C" int afe_matrix_values(const afe_matrix *a, double *h_val) {
   API_BEGIN
   if (!a || !h_val) return fail(AFE_E_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
   HIPCHK(hipMemcpyAsync(h_val, a->val.p, a->nnz * sizeof(double), hipMemcpyDeviceToHost, a->ctx->stream));
   HIPCHK(hipStreamSynchronize(a->ctx->stream));
@@ -632,6 +640,7 @@ extern "C" int afe_matrix_values(const afe_matrix *a, double *h_val) {
 extern "C" int afe_matrix_set_values(afe_matrix *a, const double *h_val) {
   API_BEGIN
   if (!a || !h_val) return fail(AFE_E_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
   HIPCHK(hipMemcpyAsync(a->val.p, h_val, a->nnz * sizeof(double), hipMemcpyHostToDevice, a->ctx->stream));
   HIPCHK(hipStreamSynchronize(a->ctx->stream));
@@ -1434,7 +1443,8 @@ extern "C" int afe_amg_level_csr(const afe_amg *g, int32_t level, int32_t which,
   const int L = (int)g->h.A.size();
   if (level < 0 || level >= L || (which == 1 && level >= L - 1) || which < 0 || which > 1)
     return fail(AFE_E_ARG, "no such level matrix");
-  if (val && which == 0) {
+  if (val && which == 0 && g->ctx) {
+    std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
     try {
       sync_host_values(const_cast<afe_amg *>(g));
     } catch (const HipFail &h) {
