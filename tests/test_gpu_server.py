@@ -76,3 +76,26 @@ def test_server_budget_evicts_idle_meshes():
         assert info["evictions"] > 0 and info["cached"] < 10
     finally:
         srv.close()
+
+
+def test_server_stream_of_small_requests():
+    """Set-ups running on several host threads while the engine steps the
+    batch (and captures its graphs): every request of a stream of new and
+    moving-mesh requests succeeds."""
+    import afepack_amd as afe
+    print("case server stream", flush=True)
+    srv = afe.Server(0, hbm_budget=2 << 30, max_slots=12, setup_threads=6, smooth=1)
+    try:
+        ids = []
+        for k in range(80):
+            key = k % 25  # keys repeat: moving-mesh re-solves of cached meshes
+            kind = (P1, P2, TET)[key % 3]
+            n = (4 + key % 7) if kind == TET else (8 + 5 * (key % 9))
+            ids.append((srv.submit(1000 + key, kind=kind, n=n, seed=key, move_amp=0.05 * (k // 25),
+                                   move_k=1 + k % 3), kind, n))
+        for rid, kind, n in ids:
+            _, st, lat, code = srv.wait(rid)
+            assert code == 0 and st.converged == 1 and st.rel_res_l2 <= 1e-10, (rid, code)
+        assert srv.info()["completed"] == len(ids)
+    finally:
+        srv.close()
