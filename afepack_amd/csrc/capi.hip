@@ -1012,43 +1012,61 @@ static void build_layout(const Hierarchy &h, HostLayout &lay, bool maps) {
   lay.maps = maps && h.A[0].nnz() < INT32_MAX;
   lay.lv.assign(L, HostLevel());
   std::vector<std::vector<int32_t>> groups(L);
+  // phase times for AFE_VERBOSE: colouring, row order, level sliced ELL,
+  // transfer operators, natural copy
+  double tph[5] = {0, 0, 0, 0, 0};
+  auto tick = [t = std::chrono::steady_clock::now()](double &acc) mutable {
+    const auto now = std::chrono::steady_clock::now();
+    acc += std::chrono::duration<double>(now - t).count();
+    t = now;
+  };
+  double tdrop = 0;
+  tick(tdrop);
   for (int l = 0; l < L; l++) {
     const Csr &A = h.A[l];
     HostLevel &hl = lay.lv[l];
     std::vector<int32_t> colour;
     const int ncol = colour_rows(A, colour);
+    tick(tph[0]);
     std::vector<int32_t> len(A.m, 0);
     for (int64_t i = 0; i < A.m; i++)
       for (int64_t k = A.rp[i]; k < A.rp[i + 1]; k++)
         if (A.col[k] != i) len[i]++;
     // rows of one colour keep their index order in windows of COLOUR_WINDOW
     // rows (x locality of the gathers) and are sorted by stored length inside
-    // a window (little sliced-ELL padding)
-    std::vector<int32_t> win(A.m), seen(ncol, 0);
-    for (int64_t i = 0; i < A.m; i++) win[i] = seen[colour[i]]++ / COLOUR_WINDOW;
+    // a window (little sliced-ELL padding); ties keep the index order.  A
+    // counting sort by colour, then a stable sort of each window.
+    std::vector<int64_t> cstart(ncol + 1, 0);
+    for (int64_t i = 0; i < A.m; i++) cstart[colour[i] + 1]++;
+    for (int c = 0; c < ncol; c++) cstart[c + 1] += cstart[c];
     hl.perm.resize(A.m);
-    std::iota(hl.perm.begin(), hl.perm.end(), 0);
-    std::stable_sort(hl.perm.begin(), hl.perm.end(), [&](int32_t a, int32_t b) {
-      if (colour[a] != colour[b]) return colour[a] < colour[b];
-      if (win[a] != win[b]) return win[a] < win[b];
-      return len[a] > len[b];
-    });
+    {
+      std::vector<int64_t> fill(cstart.begin(), cstart.end() - 1);
+      for (int64_t i = 0; i < A.m; i++) hl.perm[fill[colour[i]]++] = (int32_t)i;
+    }
+    for (int c = 0; c < ncol; c++)
+      for (int64_t w = cstart[c]; w < cstart[c + 1]; w += COLOUR_WINDOW)
+        std::stable_sort(hl.perm.begin() + w, hl.perm.begin() + std::min<int64_t>(cstart[c + 1], w + COLOUR_WINDOW),
+                         [&](int32_t a, int32_t b) { return len[a] > len[b]; });
     hl.iperm.resize(A.m);
     for (int64_t i = 0; i < A.m; i++) hl.iperm[hl.perm[i]] = (int32_t)i;
     groups[l].assign(ncol, 0);
     for (int64_t i = 0; i < A.m; i++) groups[l][colour[i]]++;
     hl.ncol = ncol;
     hl.n = A.m;
+    tick(tph[1]);
   }
   for (int l = 0; l < L; l++) {
     HostLevel &hl = lay.lv[l];
     build_sell(h.A[l], hl.perm, hl.iperm, groups[l], true, hl.sa, lay.maps ? &hl.smap : nullptr,
                lay.maps ? &hl.dsrc : nullptr);
+    tick(tph[2]);
     if (l + 1 < L) {
       build_sell(h.P[l], lay.lv[l + 1].perm, hl.iperm, {(int32_t)h.P[l].m}, false, hl.sp);
       // prolongation is P^T of the restriction (SparseMatrix::Tvmult of P in
       // AMGSolver::solve, src/AMGSolver.cpp:608-613), stored explicitly
       build_sell(transpose(h.P[l]), hl.perm, lay.lv[l + 1].iperm, {(int32_t)h.A[l].m}, false, hl.st);
+      tick(tph[3]);
     }
   }
   {
@@ -1066,7 +1084,11 @@ static void build_layout(const Hierarchy &h, HostLayout &lay, bool maps) {
                        [&](int32_t a, int32_t b) { return len[a] > len[b]; });
     build_sell(A, lay.nat_rowmap, ident, {(int32_t)A.m}, true, lay.nat, lay.maps ? &lay.nat_smap : nullptr,
                lay.maps ? &lay.nat_dsrc : nullptr);
+    tick(tph[4]);
   }
+  if (verbose())
+    fprintf(stderr, "layout: colouring %.3f s, row order %.3f s, level ELL %.3f s, transfer %.3f s, natural %.3f s\n",
+            tph[0], tph[1], tph[2], tph[3], tph[4]);
   if (h.full) {
     // the coarse vectors are in the level's colour order: permute the inverse
     const std::vector<int32_t> &pc = lay.lv[L - 1].perm;

@@ -689,16 +689,23 @@ int colour_rows(const Csr &a, std::vector<int32_t> &colour) {
     // index order, and every colour is one launch per sweep.
     std::vector<uint64_t> used(n, 0); // neighbour colours (at most 64 here)
     std::vector<int32_t> sat(n, 0);
-    typedef std::tuple<int32_t, int64_t, int64_t> Key; // (saturation, degree, -row)
-    std::priority_queue<Key> heap;
-    for (int64_t i = 0; i < n; i++) heap.emplace(0, deg[i + 1] - deg[i], -i);
+    // heap key: saturation (bits 56..63), degree (32..55), n-1-row (0..31),
+    // so the largest key is the most saturated, then the most connected,
+    // then the lowest row
+    auto key = [&](int64_t i) {
+      const uint64_t d = std::min<int64_t>(deg[i + 1] - deg[i], (1 << 24) - 1);
+      return (uint64_t)sat[i] << 56 | d << 32 | (uint64_t)(n - 1 - i);
+    };
+    std::vector<uint64_t> init(n);
+    for (int64_t i = 0; i < n; i++) init[i] = key(i);
+    std::priority_queue<uint64_t> heap(std::less<uint64_t>(), std::move(init));
     int ncol = 0;
     bool overflow = false;
     while (!heap.empty()) {
-      const Key top = heap.top();
+      const uint64_t top = heap.top();
       heap.pop();
-      const int64_t i = -std::get<2>(top);
-      if (colour[i] >= 0 || std::get<0>(top) != sat[i]) continue; // stale entry
+      const int64_t i = n - 1 - (int64_t)(top & 0xffffffffu);
+      if (colour[i] >= 0 || (int32_t)(top >> 56) != sat[i]) continue; // stale entry
       int c = 0;
       while (c < 64 && (used[i] >> c & 1)) c++;
       if (c == 64) {
@@ -712,7 +719,7 @@ int colour_rows(const Csr &a, std::vector<int32_t> &colour) {
         if (colour[j] >= 0 || (used[j] >> c & 1)) continue;
         used[j] |= uint64_t(1) << c;
         sat[j]++;
-        heap.emplace(sat[j], deg[j + 1] - deg[j], -j);
+        heap.push(key(j));
       }
     }
     if (!overflow) return ncol;
