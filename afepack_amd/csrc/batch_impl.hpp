@@ -166,6 +166,8 @@ template <class T> static void batch_build_epoch(afe_batch *b) {
     m.n = g->lv[0].n;
     m.nat_si = g->nat_si.p;
     m.nat_cols = g->nat_cols.p;
+    m.nat_c16 = g->nat_c16.p;
+    m.nat_cbase = g->nat_cbase.p;
     m.nat_rowmap = g->nat_rowmap.p;
     m.iperm0 = g->iperm0.p;
     m.nat_v = g->nat_v64.p;

@@ -37,6 +37,8 @@ template <class T> struct MemberDev {
   int64_t n;
   const SliceInfo *nat_si;
   const int32_t *nat_cols, *nat_rowmap, *iperm0;
+  const uint16_t *nat_c16; // 16-bit columns (offsets from nat_cbase per slice) when not null
+  const int32_t *nat_cbase;
   const double *nat_v, *nat_diag;
   double *x, *xl, *r, *p, *q, *z;
   T *b0, *x0;
@@ -144,7 +146,10 @@ __global__ __launch_bounds__(BLK) void kb_spmv_dot(const Seg *__restrict__ segs,
       const int pos = q.row0 + lane;
       const int r = m.nat_rowmap[pos];
       const double xr = m.p[r];
-      const double yr = m.nat_diag[pos] * xr + sell_dot<double, false, SELL_CHUNK_SPMV>(q, lane, m.nat_cols, m.nat_v, m.p);
+      const double acc = m.nat_c16 ? sell_dot<double, false, SELL_CHUNK_SPMV, uint16_t>(q, lane, m.nat_c16, m.nat_v,
+                                                                                       m.p, m.nat_cbase[s])
+                                   : sell_dot<double, false, SELL_CHUNK_SPMV>(q, lane, m.nat_cols, m.nat_v, m.p);
+      const double yr = m.nat_diag[pos] * xr + acc;
       m.q[r] = yr;
       dot = xr * yr;
     }

@@ -932,6 +932,8 @@ struct afe_amg {
   int nat_ns = 0;
   DevBuf<SliceInfo> nat_si;
   DevBuf<int32_t> nat_cols, nat_rowmap, nat_smap, nat_dsrc;
+  DevBuf<uint16_t> nat_c16; // 16-bit columns (with nat_cbase per slice) in place of nat_cols
+  DevBuf<int32_t> nat_cbase;
   DevBuf<double> nat_v64, nat_diag;
   bool dev_refresh = false;      // device lazyReinit available (source maps uploaded)
   bool host_vals_stale = false;  // host level values older than the device's
@@ -1004,6 +1006,10 @@ struct HostLayout {
   std::vector<HostLevel> lv;
   Sell nat;
   std::vector<int32_t> nat_rowmap, nat_smap, nat_dsrc;
+  // 16-bit columns of the natural copy: offsets from a base column per
+  // slice, when every slice's columns span at most 65535 (else empty)
+  std::vector<uint16_t> nat_c16;
+  std::vector<int32_t> nat_cbase;
   std::vector<double> minv_perm;
 };
 
@@ -1084,6 +1090,41 @@ static void build_layout(const Hierarchy &h, HostLayout &lay, bool maps) {
                        [&](int32_t a, int32_t b) { return len[a] > len[b]; });
     build_sell(A, lay.nat_rowmap, ident, {(int32_t)A.m}, true, lay.nat, lay.maps ? &lay.nat_smap : nullptr,
                lay.maps ? &lay.nat_dsrc : nullptr);
+    // rows of a slice lie in one window of NAT_WINDOW rows, so their columns
+    // span little more than the matrix bandwidth: 16-bit offsets from the
+    // slice's smallest column when that fits
+    // (AFE_NO_C16=1 keeps the 32-bit columns: both forms are tested against
+    // each other)
+    const int64_t ns = (int64_t)lay.nat.slices.size();
+    std::vector<int32_t> base(ns);
+    bool fits = getenv("AFE_NO_C16") == nullptr;
+    // (the lanes of a partial last slice hold no row: they stay 0 and are
+    // never read)
+    for (int64_t q = 0; q < ns && fits; q++) {
+      const SliceInfo &si = lay.nat.slices[q];
+      int32_t lo = INT32_MAX, hi = INT32_MIN;
+      for (int64_t j = 0; j < si.width; j++)
+        for (int64_t l = 0; l < si.nrow; l++) {
+          const int32_t c = lay.nat.cols[si.off + j * SELL_C + l];
+          lo = std::min(lo, c);
+          hi = std::max(hi, c);
+        }
+      if (si.width == 0 || si.nrow == 0) lo = hi = 0;
+      base[q] = lo;
+      fits = (int64_t)hi - lo <= 65535;
+    }
+    if (fits) {
+      lay.nat_c16.assign(lay.nat.cols.size(), 0);
+      for (int64_t q = 0; q < ns; q++) {
+        const SliceInfo &si = lay.nat.slices[q];
+        for (int64_t j = 0; j < si.width; j++)
+          for (int64_t l = 0; l < si.nrow; l++) {
+            const int64_t e = si.off + j * SELL_C + l;
+            lay.nat_c16[e] = (uint16_t)(lay.nat.cols[e] - base[q]);
+          }
+      }
+      lay.nat_cbase = std::move(base);
+    }
     tick(tph[4]);
   }
   if (verbose())
@@ -1159,7 +1200,15 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
   g->iperm0.upload(ctx, lay.lv[0].iperm);
   g->nat_ns = (int)lay.nat.slices.size();
   g->nat_si.upload(ctx, lay.nat.slices);
-  g->nat_cols.upload(ctx, lay.nat.cols);
+  if (!lay.nat_c16.empty()) {
+    g->nat_c16.upload(ctx, lay.nat_c16);
+    g->nat_cbase.upload(ctx, lay.nat_cbase);
+    g->nat_cols.release();
+  } else {
+    g->nat_cols.upload(ctx, lay.nat.cols);
+    g->nat_c16.release();
+    g->nat_cbase.release();
+  }
   g->nat_v64.upload(ctx, lay.nat.vals);
   g->nat_diag.upload(ctx, lay.nat.diag);
   g->nat_rowmap.upload(ctx, lay.nat_rowmap);
@@ -1629,11 +1678,12 @@ static void residual_norms(afe_amg *g, bool natural, const double *b, const doub
   hipStream_t st = g->ctx->stream;
   const int G = g_tune.spmv_blocks;
   if (natural)
-    k_residual_norms_dd<<<G, BLK, 0, st>>>(g->nat_si.p, g->nat_ns, g->nat_cols.p, g->nat_v64.p, g->nat_diag.p,
-                                            g->nat_rowmap.p, b, x, xl, r, g->part.p, g_tune.xcd);
+    k_residual_norms_dd<<<G, BLK, 0, st>>>(g->nat_si.p, g->nat_ns, g->nat_cols.p, g->nat_c16.p, g->nat_cbase.p,
+                                            g->nat_v64.p, g->nat_diag.p, g->nat_rowmap.p, b, x, xl, r, g->part.p,
+                                            g_tune.xcd);
   else
-    k_residual_norms_dd<<<G, BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, d.v64.p, d.diag64.p, nullptr, b, x, xl, r,
-                                            g->part.p, g_tune.xcd);
+    k_residual_norms_dd<<<G, BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, nullptr, nullptr, d.v64.p, d.diag64.p, nullptr,
+                                            b, x, xl, r, g->part.p, g_tune.xcd);
   k_finalize<<<1, FIN_THREADS, 0, st>>>(g->part.p, G, 3, &g->sc.p->rr);
 }
 
@@ -1725,14 +1775,14 @@ template <class T> static void precondition_perm(afe_amg *g, const double *r, bo
 static void issue_cg_spmv(afe_amg *g, hipStream_t s, hipEvent_t start = nullptr, hipEvent_t stop = nullptr) {
   if (start) {
     hipExtLaunchKernelGGL(k_spmv_dot, dim3(g_tune.spmv_blocks), dim3(BLK), 0, s, start, stop, 0,
-                          g->nat_si.p, g->nat_ns, g->nat_cols.p, g->nat_v64.p, g->nat_diag.p, g->nat_rowmap.p,
-                          g->wp.p, g->wq.p, g->part.p, g_tune.xcd);
+                          g->nat_si.p, g->nat_ns, g->nat_cols.p, g->nat_c16.p, g->nat_cbase.p, g->nat_v64.p,
+                          g->nat_diag.p, g->nat_rowmap.p, g->wp.p, g->wq.p, g->part.p, g_tune.xcd);
     HIPCHK(hipGetLastError());
     return;
   }
-  k_spmv_dot<<<g_tune.spmv_blocks, BLK, 0, s>>>(g->nat_si.p, g->nat_ns, g->nat_cols.p, g->nat_v64.p,
-                                                 g->nat_diag.p, g->nat_rowmap.p, g->wp.p, g->wq.p, g->part.p,
-                                                 g_tune.xcd);
+  k_spmv_dot<<<g_tune.spmv_blocks, BLK, 0, s>>>(g->nat_si.p, g->nat_ns, g->nat_cols.p, g->nat_c16.p,
+                                                 g->nat_cbase.p, g->nat_v64.p, g->nat_diag.p, g->nat_rowmap.p,
+                                                 g->wp.p, g->wq.p, g->part.p, g_tune.xcd);
 }
 
 // part 0 of a CG iteration after the SpMV: alpha, the x / r update with the
@@ -1946,7 +1996,8 @@ extern "C" int afe_amg_kernel_timing(afe_amg *g, int32_t which, int32_t reps, do
   const int64_t n = d.n, nnz = g->h.A[0].nnz();
   const double sv = (which == 1 || which == 3) ? 4.0 : 8.0;
   if (which == 0)
-    *bytes = nnz * (sv + 4) + (n + 1) * 4.0 + 2.0 * n * sv;
+    *bytes = g->nat_c16.p ? nnz * (sv + 2) + (double)g->nat_ns * 4 + (n + 1) * 4.0 + 2.0 * n * sv
+                          : nnz * (sv + 4) + (n + 1) * 4.0 + 2.0 * n * sv;
   else if (which == 3)
     *bytes = nnz * (sv + 4) + (n + 1) * 4.0 + 3.0 * n * sv;
   else
@@ -1960,9 +2011,9 @@ extern "C" int afe_amg_kernel_timing(afe_amg *g, int32_t which, int32_t reps, do
   }
   auto once = [&]() {
     if (which == 0) {
-      k_spmv_dot<<<g_tune.spmv_blocks, BLK, 0, s>>>(g->nat_si.p, g->nat_ns, g->nat_cols.p, g->nat_v64.p,
-                                                     g->nat_diag.p, g->nat_rowmap.p, g->wp.p, g->wq.p, g->part.p,
-                                                     g_tune.xcd);
+      k_spmv_dot<<<g_tune.spmv_blocks, BLK, 0, s>>>(g->nat_si.p, g->nat_ns, g->nat_cols.p, g->nat_c16.p,
+                                                     g->nat_cbase.p, g->nat_v64.p, g->nat_diag.p, g->nat_rowmap.p,
+                                                     g->wp.p, g->wq.p, g->part.p, g_tune.xcd);
     } else if (which == 1) {
       for (int c = 0; c < d.ncol; c++) gs_colour<float>(d, c, s);
     } else if (which == 2) {

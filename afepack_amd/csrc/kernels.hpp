@@ -517,14 +517,16 @@ constexpr int SELL_CHUNK_SPMV = 8;
 template <class T, bool NT> struct LaunchChunk {
   static constexpr int v = (!NT && sizeof(T) == 4) ? 24 : SELL_CHUNK;
 };
-template <class T, bool NT = false, int CH = SELL_CHUNK>
-__device__ inline T sell_dot(const SliceInfo &s, int lane, const int32_t *__restrict__ cols,
-                             const T *__restrict__ vals, const T *__restrict__ x) {
-  const int32_t *c = cols + s.off + lane;
+// Columns are int32 indices, or (CI = uint16_t) offsets from the slice's
+// base column cbase.
+template <class T, bool NT = false, int CH = SELL_CHUNK, class CI = int32_t>
+__device__ inline T sell_dot(const SliceInfo &s, int lane, const CI *__restrict__ cols,
+                             const T *__restrict__ vals, const T *__restrict__ x, int32_t cbase = 0) {
+  const CI *c = cols + s.off + lane;
   const T *v = vals + s.off + lane;
   T acc = 0;
   for (int j = 0; j < s.width; j += CH) {
-    int32_t cc[CH];
+    int32_t cc[CH]; // the base is added at the gather, after every load of the chunk is issued
     T vv[CH], xx[CH];
 #pragma unroll
     for (int k = 0; k < CH; k++)
@@ -534,7 +536,7 @@ __device__ inline T sell_dot(const SliceInfo &s, int lane, const int32_t *__rest
       }
 #pragma unroll
     for (int k = 0; k < CH; k++)
-      if (j + k < s.width) xx[k] = x[cc[k]];
+      if (j + k < s.width) xx[k] = x[cbase + cc[k]];
 #pragma unroll
     for (int k = 0; k < CH; k++)
       if (j + k < s.width) acc += vv[k] * xx[k];
@@ -869,6 +871,8 @@ __global__ __launch_bounds__(256) void k_permute_out(int64_t n, const int32_t *_
 // over slices with a fixed grid so the partial count is RED_BLOCKS
 __global__ __launch_bounds__(BLK) void k_spmv_dot(const SliceInfo *__restrict__ si, int ns,
                                                   const int32_t *__restrict__ cols,
+                                                  const uint16_t *__restrict__ c16,
+                                                  const int32_t *__restrict__ cbase,
                                                   const double *__restrict__ vals,
                                                   const double *__restrict__ diag,
                                                   const int32_t *__restrict__ rowmap,
@@ -885,7 +889,9 @@ __global__ __launch_bounds__(BLK) void k_spmv_dot(const SliceInfo *__restrict__ 
       const int pos = q.row0 + lane;
       const int r = rowmap ? rowmap[pos] : pos;
       const double xr = x[r];
-      const double yr = diag[pos] * xr + sell_dot<double, true, SELL_CHUNK_SPMV>(q, lane, cols, vals, x);
+      const double acc = c16 ? sell_dot<double, true, SELL_CHUNK_SPMV, uint16_t>(q, lane, c16, vals, x, cbase[s])
+                             : sell_dot<double, true, SELL_CHUNK_SPMV>(q, lane, cols, vals, x);
+      const double yr = diag[pos] * xr + acc;
       y[r] = yr;
       dot += xr * yr;
     }
@@ -1015,6 +1021,8 @@ __device__ inline void two_sum(double a, double b, double &s, double &e) {
 // sums, and the partials of (r, r), (b, b), sum |r|; xl may be null
 __global__ __launch_bounds__(BLK) void k_residual_norms_dd(const SliceInfo *__restrict__ si, int ns,
                                                            const int32_t *__restrict__ cols,
+                                                           const uint16_t *__restrict__ c16,
+                                                           const int32_t *__restrict__ cbase,
                                                            const double *__restrict__ vals,
                                                            const double *__restrict__ diag,
                                                            const int32_t *__restrict__ rowmap,
@@ -1043,10 +1051,12 @@ __global__ __launch_bounds__(BLK) void k_residual_norms_dd(const SliceInfo *__re
         lo += e - pe;
         if (xl) lo -= a * xl[row];
       }
-      const int32_t *c = cols + q.off + lane;
+      const int32_t *c = c16 ? nullptr : cols + q.off + lane;
+      const uint16_t *c2 = c16 ? c16 + q.off + lane : nullptr;
+      const int32_t cb = c16 ? cbase[s] : 0;
       const double *v = vals + q.off + lane;
       for (int j = 0; j < q.width; j++) {
-        const int32_t cj = c[j * WAVE];
+        const int32_t cj = c16 ? cb + (int32_t)c2[j * WAVE] : c[j * WAVE];
         const double a = v[j * WAVE], xv = x[cj];
         const double pr = a * xv, pe = fma(a, xv, -pr);
         double t, e;

@@ -93,3 +93,38 @@ def test_live_timing_keeps_results(ctx):
     g.pcg(x2, b, rtol=1e-10, max_it=200, prec=afe.PREC_FP32)
     ms2, n2 = g.live_timing(False)
     assert n2 == n and ms2 == ms
+
+
+@pytest.mark.parametrize("kind,n", [(2, 48), (3, 10)])
+def test_natural_columns_16_and_32_bit_agree(ctx, kind, n):
+    """The natural-order FP64 copy of level 0 stores 16-bit column offsets
+    from a per-slice base when the columns of every slice span at most
+    65535; with AFE_NO_C16 set it keeps 32-bit columns.  Both decode to the
+    same columns, so the PCG iterates are bitwise equal."""
+    import os
+    import afepack_amd as afe
+    print(f"case c16 kind={kind} n={n}", flush=True)
+    c, ev, vm = make_mesh(kind, n, 0.05 if kind == 3 else 0.2, 4)
+    sp = afe.Space(ctx, kind, c, ev, vm)
+    A = afe.Matrix(sp)
+    acc = 2 if kind == 3 else 4
+    A.assemble(afe.OP_STIFF, acc)
+    b = sp.rhs(acc, afe.F_POISSON)
+    A.apply_dirichlet(b)
+    xs = []
+    for no16 in (False, True):
+        if no16:
+            os.environ["AFE_NO_C16"] = "1"
+        try:
+            g = afe.Amg(A, afe.AmgParams(lazy=1, smooth=1, build_fp32=1))
+        finally:
+            os.environ.pop("AFE_NO_C16", None)
+        x = afe.DeviceVector(ctx, sp.n_dof)
+        st = g.pcg(x, b, rtol=1e-10, max_it=300, prec=afe.PREC_FP32)
+        ms, by = g.kernel_timing(0, reps=2)
+        xs.append((st.iterations, x.numpy(), by))
+        g.close()
+    assert xs[0][0] == xs[1][0]
+    assert np.array_equal(xs[0][1], xs[1][1])
+    nnz = sp.nnz
+    assert xs[1][2] - xs[0][2] >= 2 * nnz - 4 * (sp.n_dof // 64 + 1) - 1  # 2 bytes per entry saved
