@@ -147,6 +147,15 @@ struct Tuning {
 };
 static Tuning g_tune;
 
+// level matrices larger than this many bytes use the streamed kernel forms
+// (non-temporal loads, colour-coded columns); AFE_STREAM_BYTES overrides the
+// threshold for hierarchies built afterwards (tests set 0 to run the
+// streamed forms on small systems)
+static int64_t stream_bytes() {
+  const char *e = getenv("AFE_STREAM_BYTES");
+  return e ? std::strtoll(e, nullptr, 10) : g_tune.stream_bytes;
+}
+
 // AFE_VERBOSE=1 prints solver progress and set-up phase times to stderr
 static bool verbose() {
   static const bool v = getenv("AFE_VERBOSE") != nullptr;
@@ -891,6 +900,9 @@ struct DevLevel {
   int ns = 0, p_ns = 0, t_ns = 0;
   DevBuf<SliceInfo> si, p_si, t_si;
   DevBuf<int32_t> cols, p_cols, t_cols, col_slice;
+  DevBuf<uint16_t> c16; // colour-coded 16-bit columns (streamed level 0), used by k_gs / k_residual
+  DevBuf<int32_t> cb16;
+  int cshift = 0;
   DevBuf<double> v64, invd64, diag64, p_v64, t_v64;
   DevBuf<float> v32, invd32, diag32, p_v32, t_v32;
   DevBuf<double> x64, b64, t64;
@@ -998,6 +1010,10 @@ struct HostLevel {
   int ncol = 0;
   int64_t n = 0;
   Sell sa, sp, st;
+  // colour-coded 16-bit columns of a streamed level (empty when they do not fit)
+  std::vector<uint16_t> c16;
+  std::vector<int32_t> cb16;
+  int cshift = 0;
   std::vector<int32_t> perm, iperm;
   std::vector<int32_t> smap, dsrc; // CSR entry -> layout position, row -> CSR diagonal
 };
@@ -1012,6 +1028,51 @@ struct HostLayout {
   std::vector<int32_t> nat_cbase;
   std::vector<double> minv_perm;
 };
+
+// colour-coded 16-bit columns of a level's layout (see sell_dot_cc): per
+// slice and colour the smallest position of that colour among the slice's
+// columns is the base; kept only when every offset fits in 16 - colour bits
+static void colour_code(HostLevel &hl) {
+  const Sell &sa = hl.sa;
+  const int cbits = hl.ncol <= 2 ? 1 : (hl.ncol <= 4 ? 2 : 3);
+  const int cshift = 16 - cbits;
+  const int32_t maxoff = (1 << cshift) - 1;
+  std::vector<uint8_t> pc(sa.n_rows);
+  for (int c = 0; c < hl.ncol; c++)
+    for (int32_t p = sa.group_row[c]; p < sa.group_row[c + 1]; p++) pc[p] = (uint8_t)c;
+  const int64_t ns = (int64_t)sa.slices.size();
+  std::vector<int32_t> cb((size_t)ns * CC_BASES, 0);
+  std::vector<uint16_t> c16(sa.cols.size(), 0);
+  for (int64_t q = 0; q < ns; q++) {
+    const SliceInfo &si = sa.slices[q];
+    int32_t lo[CC_BASES], hi[CC_BASES];
+    for (int c = 0; c < CC_BASES; c++) {
+      lo[c] = INT32_MAX;
+      hi[c] = INT32_MIN;
+    }
+    for (int64_t j = 0; j < si.width; j++)
+      for (int64_t l = 0; l < si.nrow; l++) {
+        const int32_t col = sa.cols[si.off + j * SELL_C + l];
+        const int c = pc[col];
+        lo[c] = std::min(lo[c], col);
+        hi[c] = std::max(hi[c], col);
+      }
+    for (int c = 0; c < CC_BASES; c++) {
+      if (hi[c] < lo[c]) continue;
+      if (hi[c] - lo[c] > maxoff) return; // does not fit: keep the int32 columns only
+      cb[(size_t)q * CC_BASES + c] = lo[c];
+    }
+    for (int64_t j = 0; j < si.width; j++)
+      for (int64_t l = 0; l < si.nrow; l++) {
+        const int64_t e = si.off + j * SELL_C + l;
+        const int c = pc[sa.cols[e]];
+        c16[e] = (uint16_t)(c << cshift | (sa.cols[e] - cb[(size_t)q * CC_BASES + c]));
+      }
+  }
+  hl.c16 = std::move(c16);
+  hl.cb16 = std::move(cb);
+  hl.cshift = cshift;
+}
 
 static void build_layout(const Hierarchy &h, HostLayout &lay, bool maps) {
   const int L = (int)h.A.size();
@@ -1066,6 +1127,8 @@ static void build_layout(const Hierarchy &h, HostLayout &lay, bool maps) {
     HostLevel &hl = lay.lv[l];
     build_sell(h.A[l], hl.perm, hl.iperm, groups[l], true, hl.sa, lay.maps ? &hl.smap : nullptr,
                lay.maps ? &hl.dsrc : nullptr);
+    if ((int64_t)hl.sa.cols.size() * 8 > stream_bytes() && hl.ncol <= CC_BASES && getenv("AFE_NO_C16") == nullptr)
+      colour_code(hl);
     tick(tph[2]);
     if (l + 1 < L) {
       build_sell(h.P[l], lay.lv[l + 1].perm, hl.iperm, {(int32_t)h.P[l].m}, false, hl.sp);
@@ -1161,6 +1224,11 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
     d.si.upload(ctx, sa.slices);
     d.cols.upload(ctx, sa.cols);
     d.col_slice.upload(ctx, sa.group_slice);
+    if (!hl.c16.empty()) {
+      d.c16.upload(ctx, hl.c16);
+      d.cb16.upload(ctx, hl.cb16);
+      d.cshift = hl.cshift;
+    }
     d.v64.upload(ctx, sa.vals);
     d.diag64.upload(ctx, sa.diag);
     d.invd64.upload(ctx, inv_of(sa.diag));
@@ -1194,7 +1262,7 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
     }
     d.perm = hl.perm;
     // stream (non-temporal) when the level's FP32 matrix exceeds the cache
-    d.stream = (int64_t)sa.cols.size() * 8 > g_tune.stream_bytes;
+    d.stream = (int64_t)sa.cols.size() * 8 > stream_bytes();
   }
   g->perm0.upload(ctx, lay.lv[0].perm);
   g->iperm0.upload(ctx, lay.lv[0].iperm);
@@ -1565,10 +1633,12 @@ static void gs_colour(DevLevel &d, int c, hipStream_t st) {
   const int s0 = d.colour_slice[c], s1 = d.colour_slice[c + 1];
   if (s1 <= s0) return;
   if (d.stream)
-    k_gs<T, true><<<sblk(s1 - s0), BLK, 0, st>>>(d.si.p, s0, s1, d.cols.p, Prec<T>::v(d), Prec<T>::invd(d),
+    k_gs<T, true><<<sblk(s1 - s0), BLK, 0, st>>>(d.si.p, s0, s1, d.cols.p, d.c16.p, d.cb16.p, d.cshift,
+                                                 Prec<T>::v(d), Prec<T>::invd(d),
                                                  Prec<T>::b(d), Prec<T>::x(d), g_tune.xcd);
   else
-    k_gs<T, false><<<sblk(s1 - s0), BLK, 0, st>>>(d.si.p, s0, s1, d.cols.p, Prec<T>::v(d), Prec<T>::invd(d),
+    k_gs<T, false><<<sblk(s1 - s0), BLK, 0, st>>>(d.si.p, s0, s1, d.cols.p, nullptr, nullptr, 0,
+                                                  Prec<T>::v(d), Prec<T>::invd(d),
                                                   Prec<T>::b(d), Prec<T>::x(d), g_tune.xcd);
 }
 
@@ -1592,12 +1662,14 @@ static void issue_vcycle(afe_amg *g, int nu, bool symmetric, hipStream_t st, boo
         gs_colour<T>(d, c, st);
       }
     if (d.stream) {
-      k_residual<T, true><<<sblk(d.ns), BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, Prec<T>::v(d), Prec<T>::diag(d),
+      k_residual<T, true><<<sblk(d.ns), BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, d.c16.p, d.cb16.p, d.cshift,
+                                                       Prec<T>::v(d), Prec<T>::diag(d),
                                                        Prec<T>::b(d), Prec<T>::x(d), Prec<T>::t(d), g_tune.xcd);
       k_restrict<T, true><<<sblk(d.p_ns), BLK, 0, st>>>(d.p_si.p, d.p_ns, d.p_cols.p, Prec<T>::pv(d),
                                                          Prec<T>::t(d), Prec<T>::b(dn), Prec<T>::x(dn), g_tune.xcd);
     } else {
-      k_residual<T, false><<<sblk(d.ns), BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, Prec<T>::v(d), Prec<T>::diag(d),
+      k_residual<T, false><<<sblk(d.ns), BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, nullptr, nullptr, 0,
+                                                        Prec<T>::v(d), Prec<T>::diag(d),
                                                         Prec<T>::b(d), Prec<T>::x(d), Prec<T>::t(d), g_tune.xcd);
       k_restrict<T, false><<<sblk(d.p_ns), BLK, 0, st>>>(d.p_si.p, d.p_ns, d.p_cols.p, Prec<T>::pv(d),
                                                           Prec<T>::t(d), Prec<T>::b(dn), Prec<T>::x(dn), g_tune.xcd);
@@ -1998,8 +2070,8 @@ extern "C" int afe_amg_kernel_timing(afe_amg *g, int32_t which, int32_t reps, do
   if (which == 0)
     *bytes = g->nat_c16.p ? nnz * (sv + 2) + (double)g->nat_ns * 4 + (n + 1) * 4.0 + 2.0 * n * sv
                           : nnz * (sv + 4) + (n + 1) * 4.0 + 2.0 * n * sv;
-  else if (which == 3)
-    *bytes = nnz * (sv + 4) + (n + 1) * 4.0 + 3.0 * n * sv;
+  else if (d.c16.p) // colour-coded columns: 2 bytes per entry and 8 bases per slice
+    *bytes = nnz * (sv + 2) + (double)d.ns * CC_BASES * 4 + (n + 1) * 4.0 + 3.0 * n * sv;
   else
     *bytes = nnz * (sv + 4) + (n + 1) * 4.0 + 3.0 * n * sv;
   k_fill<double><<<nblk(n), 256, 0, s>>>(n, g->wp.p, 1.0);
@@ -2019,7 +2091,8 @@ extern "C" int afe_amg_kernel_timing(afe_amg *g, int32_t which, int32_t reps, do
     } else if (which == 2) {
       for (int c = 0; c < d.ncol; c++) gs_colour<double>(d, c, s);
     } else {
-      k_residual<float, true><<<sblk(d.ns), BLK, 0, s>>>(d.si.p, d.ns, d.cols.p, d.v32.p, d.diag32.p, d.b32.p,
+      k_residual<float, true><<<sblk(d.ns), BLK, 0, s>>>(d.si.p, d.ns, d.cols.p, d.c16.p, d.cb16.p, d.cshift,
+                                                          d.v32.p, d.diag32.p, d.b32.p,
                                                           d.x32.p, d.t32.p, g_tune.xcd);
     }
   };

@@ -544,6 +544,49 @@ __device__ inline T sell_dot(const SliceInfo &s, int lane, const CI *__restrict_
   return acc;
 }
 
+// Colour-coded 16-bit columns (the streamed level-0 colour layout): entry
+// e = (colour << cshift) | offset, column = cb[colour] + offset, where cb
+// holds the slice's base position in each colour's range (8 per slice).
+// The neighbours of a slice's rows sit in a narrow band of every other
+// colour's range, so the offsets fit in 13 bits on the P2 and tetrahedral
+// systems.  Same chunked loads and the same summation order as sell_dot.
+constexpr int CC_BASES = 8;
+template <class T, bool NT = true, int CH = SELL_CHUNK>
+__device__ inline T sell_dot_cc(const SliceInfo &s, int lane, const uint16_t *__restrict__ cols,
+                                const int32_t *__restrict__ cb, int cshift, const T *__restrict__ vals,
+                                const T *__restrict__ x) {
+  int32_t b[CC_BASES];
+#pragma unroll
+  for (int k = 0; k < CC_BASES; k++) b[k] = cb[k];
+  const int32_t mask = (1 << cshift) - 1;
+  const uint16_t *c = cols + s.off + lane;
+  const T *v = vals + s.off + lane;
+  T acc = 0;
+  for (int j = 0; j < s.width; j += CH) {
+    int32_t cc[CH];
+    T vv[CH], xx[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++)
+      if (j + k < s.width) {
+        cc[k] = ld_mat<NT>(c + (j + k) * WAVE);
+        vv[k] = ld_mat<NT>(v + (j + k) * WAVE);
+      }
+#pragma unroll
+    for (int k = 0; k < CH; k++)
+      if (j + k < s.width) {
+        const int ci = cc[k] >> cshift;
+        int32_t base = b[0];
+#pragma unroll
+        for (int m = 1; m < CC_BASES; m++) base = ci == m ? b[m] : base;
+        xx[k] = x[base + (cc[k] & mask)];
+      }
+#pragma unroll
+    for (int k = 0; k < CH; k++)
+      if (j + k < s.width) acc += vv[k] * xx[k];
+  }
+  return acc;
+}
+
 __device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE); }
 
 // Workgroups are dispatched round-robin over the 8 XCDs; with xcd != 0 the
@@ -574,11 +617,14 @@ __device__ inline void slice_range(int ns, int sched, int &s, int &end, int &ste
 
 // Gauss-Seidel over the slices [s0, s1) of one colour:
 // x_r = (b_r - sum_{j != r} a_rj x_j) / a_rr
+// (c16 != null: colour-coded 16-bit columns c16 / cb16 / cshift, see
+// sell_dot_cc; otherwise the int32 columns)
 template <class T, bool NT>
 __global__ __launch_bounds__(BLK) void k_gs(const SliceInfo *__restrict__ si, int s0, int s1,
-                                            const int32_t *__restrict__ cols, const T *__restrict__ vals,
-                                            const T *__restrict__ invd, const T *__restrict__ b,
-                                            T *__restrict__ x, int xcd) {
+                                            const int32_t *__restrict__ cols, const uint16_t *__restrict__ c16,
+                                            const int32_t *__restrict__ cb16, int cshift,
+                                            const T *__restrict__ vals, const T *__restrict__ invd,
+                                            const T *__restrict__ b, T *__restrict__ x, int xcd) {
   const int s = s0 + logical_block(xcd) * WPB + wave_id();
   if (s >= s1) return;
   const SliceInfo q = si[s];
@@ -586,7 +632,9 @@ __global__ __launch_bounds__(BLK) void k_gs(const SliceInfo *__restrict__ si, in
   if (lane >= q.nrow) return;
   const int r = q.row0 + lane;
   const T br = b[r], ir = invd[r]; // issued before the row's dependent loads
-  const T acc = sell_dot<T, NT, LaunchChunk<T, NT>::v>(q, lane, cols, vals, x);
+  const T acc = c16 ? sell_dot_cc<T, NT, LaunchChunk<T, NT>::v>(q, lane, c16, cb16 + (int64_t)s * CC_BASES, cshift,
+                                                                vals, x)
+                    : sell_dot<T, NT, LaunchChunk<T, NT>::v>(q, lane, cols, vals, x);
   x[r] = (br - acc) * ir;
 }
 
@@ -605,9 +653,11 @@ __global__ __launch_bounds__(256) void k_gs_first(int64_t n, int64_t n0, const T
 // t = b - A x (A = diag + stored off-diagonals)
 template <class T, bool NT>
 __global__ __launch_bounds__(BLK) void k_residual(const SliceInfo *__restrict__ si, int ns,
-                                                  const int32_t *__restrict__ cols, const T *__restrict__ vals,
-                                                  const T *__restrict__ diag, const T *__restrict__ b,
-                                                  const T *__restrict__ x, T *__restrict__ t, int xcd) {
+                                                  const int32_t *__restrict__ cols, const uint16_t *__restrict__ c16,
+                                                  const int32_t *__restrict__ cb16, int cshift,
+                                                  const T *__restrict__ vals, const T *__restrict__ diag,
+                                                  const T *__restrict__ b, const T *__restrict__ x,
+                                                  T *__restrict__ t, int xcd) {
   const int s = logical_block(xcd) * WPB + wave_id();
   if (s >= ns) return;
   const SliceInfo q = si[s];
@@ -615,7 +665,9 @@ __global__ __launch_bounds__(BLK) void k_residual(const SliceInfo *__restrict__ 
   if (lane >= q.nrow) return;
   const int r = q.row0 + lane;
   const T br = b[r], dr = diag[r], xr = x[r];
-  const T acc = sell_dot<T, NT, LaunchChunk<T, NT>::v>(q, lane, cols, vals, x);
+  const T acc = c16 ? sell_dot_cc<T, NT, LaunchChunk<T, NT>::v>(q, lane, c16, cb16 + (int64_t)s * CC_BASES, cshift,
+                                                                vals, x)
+                    : sell_dot<T, NT, LaunchChunk<T, NT>::v>(q, lane, cols, vals, x);
   t[r] = br - dr * xr - acc;
 }
 

@@ -95,15 +95,19 @@ def test_live_timing_keeps_results(ctx):
     assert n2 == n and ms2 == ms
 
 
-@pytest.mark.parametrize("kind,n", [(2, 48), (3, 10)])
-def test_natural_columns_16_and_32_bit_agree(ctx, kind, n):
+@pytest.mark.parametrize("kind,n,stream", [(2, 48, None), (3, 10, None), (2, 48, "0"), (3, 10, "0")])
+def test_natural_columns_16_and_32_bit_agree(ctx, kind, n, stream):
     """The natural-order FP64 copy of level 0 stores 16-bit column offsets
     from a per-slice base when the columns of every slice span at most
-    65535; with AFE_NO_C16 set it keeps 32-bit columns.  Both decode to the
+    65535, and streamed levels (all of them with AFE_STREAM_BYTES=0) store
+    colour-coded 16-bit columns for the Gauss-Seidel and residual kernels;
+    with AFE_NO_C16 set both keep 32-bit columns.  Both forms decode to the
     same columns, so the PCG iterates are bitwise equal."""
     import os
     import afepack_amd as afe
-    print(f"case c16 kind={kind} n={n}", flush=True)
+    print(f"case c16 kind={kind} n={n} stream={stream}", flush=True)
+    if stream is not None:
+        os.environ["AFE_STREAM_BYTES"] = stream
     c, ev, vm = make_mesh(kind, n, 0.05 if kind == 3 else 0.2, 4)
     sp = afe.Space(ctx, kind, c, ev, vm)
     A = afe.Matrix(sp)
@@ -121,10 +125,19 @@ def test_natural_columns_16_and_32_bit_agree(ctx, kind, n):
             os.environ.pop("AFE_NO_C16", None)
         x = afe.DeviceVector(ctx, sp.n_dof)
         st = g.pcg(x, b, rtol=1e-10, max_it=300, prec=afe.PREC_FP32)
+        x64 = afe.DeviceVector(ctx, sp.n_dof)
+        g.pcg(x64, b, rtol=1e-10, max_it=300, prec=afe.PREC_FP64)
+        xst = afe.DeviceVector(ctx, sp.n_dof)
+        g.solve(xst, b, step=4, mode=1)
         ms, by = g.kernel_timing(0, reps=2)
-        xs.append((st.iterations, x.numpy(), by))
+        ms1, by1 = g.kernel_timing(1, reps=1)
+        xs.append((st.iterations, x.numpy(), by, x64.numpy(), xst.numpy(), by1))
         g.close()
+    os.environ.pop("AFE_STREAM_BYTES", None)
     assert xs[0][0] == xs[1][0]
+    assert np.array_equal(xs[0][3], xs[1][3]) and np.array_equal(xs[0][4], xs[1][4])
+    if stream is not None:  # level 0 colour-coded: 2 bytes per entry less in the sweep
+        assert xs[1][5] > xs[0][5]
     assert np.array_equal(xs[0][1], xs[1][1])
     nnz = sp.nnz
     assert xs[1][2] - xs[0][2] >= 2 * nnz - 4 * (sp.n_dof // 64 + 1) - 1  # 2 bytes per entry saved
