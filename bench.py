@@ -77,7 +77,7 @@ def cpu_baseline(rp, col, val, b, budget_s, n_dof):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
                     help="c2 (default): 4.2M-DOF 2D P2 Poisson; c3: 3D variable-coefficient P1; "
                          "c4: batch of 256 uneven 2D problems; c5: serving stream")
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,7 +112,89 @@ def main():
         return run_c5(args)
     if args.workload == "c3":
         return run_c3(args)
+    if args.workload == "c1":
+        return run_c1(args)
     return run_c2(args)
+
+
+def run_c1(args):
+    """The small configuration: -Laplace u = 2 pi^2 sin(pi x) sin(pi y), u = 0 on
+    the boundary, 64x64 squares x 2 triangles, P1, accuracy 4, AMGSolver with
+    the reference's default parameters (value-based coarsening, V(3,3)) to a
+    relative residual of 1e-10.  Reports cycles, solve time and L2 error for
+    the device's stationary iteration (the reference's AMGSolver::solve) and
+    the mixed PCG, and the oracle AMGSolver on one host core."""
+    import numpy as np
+    import afepack_amd as afe
+    from oracle import oracle_py as orc
+    n, acc, rtol = 64, 4, 1e-10
+    ctx = afe.Context(0)
+    c, ev, vm = afe.mesh_square(n, 0.0, 0)
+    sp = afe.Space(ctx, afe.P1_TRI, c, ev, vm)
+    A = afe.Matrix(sp)
+    A.assemble(afe.OP_STIFF, acc)
+    b = sp.rhs(acc, afe.F_POISSON)
+    A.apply_dirichlet(b)
+    t0 = time.perf_counter()
+    g = afe.Amg(A, afe.AmgParams(lazy=0, smooth=3, build_fp32=1))  # AMGSolver(A): reinit
+    t_setup = time.perf_counter() - t0
+    x = afe.DeviceVector(ctx, sp.n_dof)
+    g.solve(x, b, step=200, mode=1, rtol=rtol)  # warm-up (graph capture)
+    x.zero()
+    t0 = time.perf_counter()
+    st = g.solve(x, b, step=200, mode=1, rtol=rtol)
+    t_plain = time.perf_counter() - t0
+    e_plain = sp.error_l2(x, acc)
+    g.set_smooth(1)
+    g.pcg(x, b, rtol=rtol, max_it=200, prec=afe.PREC_FP32)
+    x.zero()
+    t0 = time.perf_counter()
+    stm = g.pcg(x, b, rtol=rtol, max_it=200, prec=afe.PREC_FP32)
+    t_mixed = time.perf_counter() - t0
+    e_mixed = sp.error_l2(x, acc)
+    log(f"c1 device: stationary V(3,3) {st.iterations} cycles {1e3*t_plain:.2f} ms (rel res {st.rel_res_l2:.2e}), "
+        f"mixed PCG {stm.iterations} its {1e3*t_mixed:.2f} ms; L2 errors {e_plain:.6e} / {e_mixed:.6e}")
+    # the oracle AMGSolver on the host: cycles to 1e-10 of the initial L1 residual
+    osp = orc.Space(afe.P1_TRI, c, ev, vm)
+    OA = osp.build(orc.OP_STIFF, acc)
+    of = osp.rhs(acc, orc.F_POISSON)
+    osp.apply_dirichlet(OA, of)
+    t0 = time.perf_counter()
+    og = orc.Amg(OA, lazy=False)
+    t_osetup = time.perf_counter() - t0
+    ox = np.zeros(len(of))
+    t0 = time.perf_counter()
+    hist = og.solve(ox, of, smooth=3, step=199)
+    t_cpu = time.perf_counter() - t0
+    hist = np.asarray(hist)
+    reach = np.nonzero(hist[1:] <= rtol * hist[0])[0]
+    cycles_cpu = int(reach[0]) + 1 if len(reach) else None
+    t_cpu_to = t_cpu * (cycles_cpu / (len(hist) - 1)) if cycles_cpu else None
+    ox2 = np.zeros(len(of))
+    if cycles_cpu:
+        og.solve(ox2, of, smooth=3, step=cycles_cpu - 1)
+    e_cpu = osp.l2_error(ox2, acc)
+    log(f"c1 oracle (1 core): {cycles_cpu} cycles to 1e-10 (L1), about {1e3*(t_cpu_to or 0):.2f} ms; L2 error {e_cpu:.6e}")
+    out = {
+        "metric": "solved DOF/s at 1e-10 rel. residual, plain FP64 vs mixed AMG; p50/p99 latency",
+        "value": sp.n_dof / t_mixed, "unit": "DOF/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": 1e3 * t_mixed, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+f32", "data": "synthetic: structured mesh, manufactured rhs",
+        "config": {"workload": "2D P1 Poisson, 64x64 squares x 2 triangles, acc 4", "n_dof": sp.n_dof,
+                   "rtol": rtol, "hierarchy": "AMGSolver(A) (value-based coarsening)",
+                   "levels": g.info()["levels"], "parallelism": "single GPU"},
+        "plain": {"form": "stationary FP64 V(3,3) (AMGSolver::solve)", "cycles": st.iterations,
+                  "seconds": t_plain, "rel_res_l2": st.rel_res_l2, "l2_error": e_plain},
+        "mixed": {"iterations": stm.iterations, "seconds": t_mixed, "rel_res_l2": stm.rel_res_l2,
+                  "l2_error": e_mixed},
+        "setup_s": t_setup,
+        "cpu_baseline": {"value": (sp.n_dof / t_cpu_to) if t_cpu_to else 0.0, "unit": "DOF/s", "cores": 1,
+                         "kind": "port", "cycles": cycles_cpu, "seconds": t_cpu_to, "l2_error": e_cpu,
+                         "setup_s": t_osetup,
+                         "sample": "oracle AMGSolver (value-based coarsening, lexicographic V(3,3)), 200 cycles "
+                                   "timed, time to the first cycle at 1e-10 of the initial L1 residual"},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def run_c2(args):
