@@ -952,38 +952,6 @@ __global__ __launch_bounds__(BLK) void k_spmv_dot(const SliceInfo *__restrict__ 
   if (threadIdx.x == 0) part[blockIdx.x] = dot;
 }
 
-// r = b - A x and the partials of (r, r), (b, b), sum |r| (three arrays)
-__global__ __launch_bounds__(BLK) void k_residual_norms(const SliceInfo *__restrict__ si, int ns,
-                                                        const int32_t *__restrict__ cols,
-                                                        const double *__restrict__ vals,
-                                                        const double *__restrict__ diag,
-                                                        const double *__restrict__ b,
-                                                        const double *__restrict__ x, double *__restrict__ r,
-                                                        double *__restrict__ part) {
-  __shared__ double sh[BLK];
-  const int lane = threadIdx.x % WAVE;
-  double rr = 0.0, bb = 0.0, l1 = 0.0;
-  for (int s = blockIdx.x * WPB + wave_id(); s < ns; s += gridDim.x * WPB) {
-    const SliceInfo q = si[s];
-    if (lane < q.nrow) {
-      const int row = q.row0 + lane;
-      const double rv = b[row] - diag[row] * x[row] - sell_dot<double, false, SELL_CHUNK_SPMV>(q, lane, cols, vals, x);
-      if (r) r[row] = rv;
-      rr += rv * rv;
-      bb += b[row] * b[row];
-      l1 += fabs(rv);
-    }
-  }
-  rr = block_sum(rr, sh);
-  bb = block_sum(bb, sh);
-  l1 = block_sum(l1, sh);
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = rr;
-    part[gridDim.x + blockIdx.x] = bb;
-    part[2 * gridDim.x + blockIdx.x] = l1;
-  }
-}
-
 // ---------------------------------------------------------------------------
 // value-only refresh of a lazy hierarchy (AMGSolver::lazyReinit,
 // src/AMGSolver.cpp:505-545): the Galerkin products with the kept P on the
