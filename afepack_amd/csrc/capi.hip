@@ -1350,12 +1350,6 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
   HIPCHK(hipStreamSynchronize(ctx->stream));
 }
 
-static void upload_hierarchy(afe_amg *g) {
-  HostLayout lay;
-  build_layout(g->h, lay, g->prm.lazy != 0);
-  upload_layout(g, lay);
-}
-
 extern "C" int afe_amg_create(afe_matrix *a, const afe_amg_params *p, afe_amg **out) {
   API_BEGIN
   if (!a || !out) return fail(AFE_E_ARG, "null argument");
