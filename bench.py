@@ -479,6 +479,9 @@ def run_c3(args):
     t = time.time()
     g = afe.Amg(A, afe.AmgParams(lazy=1, nmpo=nmpo, nmpsd=nmpsd, smooth=s, build_fp32=1))
     t_amg = time.time() - t
+    dinfo = g.device_info()
+    log(f"c3 AMG: levels {g.info()['n']}, colours {[d['colours'] for d in dinfo]}, "
+        f"stored entries {[d['stored'] for d in dinfo]}")
     lat, its = [], []
     for k in range(args.warmup + args.steps):
         x.zero()
@@ -512,7 +515,8 @@ def run_c3(args):
         "config": {"workload": ("3D -div(a grad u) = f, a = exp(16-mode field, amplitude 1), 128^3 cubes x 6 "
                                 "tetrahedra, P1, accuracy 2"),
                    "n_dof": sp.n_dof, "nnz": sp.nnz, "rtol": args.rtol, "amg_nmpo": nmpo, "amg_nmpsd": nmpsd,
-                   "pcg_smooth": s, "levels": g.info()["n"], "parallelism": "single GPU"},
+                   "pcg_smooth": s, "levels": g.info()["n"], "colours": [d["colours"] for d in dinfo],
+                   "parallelism": "single GPU"},
         "iterations": its[-1], "pcg64": {"seconds": t64, "iterations": st64.iterations,
                                           "dof_per_s": sp.n_dof / t64},
         "mixed_vs_pcg64": t64 / statistics.mean(lat),
