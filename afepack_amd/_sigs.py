@@ -109,7 +109,7 @@ SIGS = {
     "afe_batch_run": (C.c_int, [P]),
     "afe_batch_result": (C.c_int, [P, I32, PI32, P]),
     "afe_batch_remove": (C.c_int, [P, I32]),
-    "afe_amg_device_info": (C.c_int, [P, I32, PI64, PI32, PI64, PI32]),
+    "afe_amg_device_info": (C.c_int, [P, I32, PI64, PI32, PI64, PI32, PI32]),
 }
 
 

@@ -464,9 +464,11 @@ class Amg(_AmgBase):
     def device_info(self):
         out = []
         for l in range(self.info()["levels"]):
-            r, c, s, t = C.c_int64(), C.c_int32(), C.c_int64(), C.c_int32()
-            _chk(_lib().afe_amg_device_info(self.h, l, C.byref(r), C.byref(c), C.byref(s), C.byref(t)))
-            out.append({"rows": r.value, "colours": c.value, "stored": s.value, "tail": t.value})
+            r, c, s, t, g = C.c_int64(), C.c_int32(), C.c_int64(), C.c_int32(), C.c_int32()
+            _chk(_lib().afe_amg_device_info(self.h, l, C.byref(r), C.byref(c), C.byref(s), C.byref(t),
+                                            C.byref(g)))
+            out.append({"rows": r.value, "colours": c.value, "stored": s.value, "tail": t.value,
+                        "row_lanes": g.value})
         return out
 
 

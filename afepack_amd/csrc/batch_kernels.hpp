@@ -45,7 +45,7 @@ template <class T> struct MemberDev {
   CgScalars *sc;
 };
 
-// ---- V-cycle level kernels: one wavefront per slice ----------------------
+// ---- V-cycle level kernels: one wavefront per slice, 2^gs lanes per row ---
 template <class T>
 __global__ __launch_bounds__(BLK) void kb_gs(const Seg *__restrict__ segs, int nseg,
                                              const LevelDev<T> *__restrict__ lv) {
@@ -55,11 +55,12 @@ __global__ __launch_bounds__(BLK) void kb_gs(const Seg *__restrict__ segs, int n
   const LevelDev<T> &d = lv[g.member];
   const SliceInfo q = d.si[s];
   const int lane = threadIdx.x % WAVE;
-  if (lane >= q.nrow) return;
-  const int r = q.row0 + lane;
+  const int gs = d.gs;
+  if ((lane >> gs) >= q.nrow) return;
+  const int r = q.row0 + (lane >> gs);
   const T br = d.b[r], ir = d.invd[r];
-  const T acc = sell_dot<T>(q, lane, d.cols, d.vals, d.x);
-  d.x[r] = (br - acc) * ir;
+  const T acc = row_group_sum(sell_dot<T>(q, lane, d.cols, d.vals, d.x), gs);
+  if ((lane & ((1 << gs) - 1)) == 0) d.x[r] = (br - acc) * ir;
 }
 
 template <class T>
@@ -71,11 +72,12 @@ __global__ __launch_bounds__(BLK) void kb_residual(const Seg *__restrict__ segs,
   const LevelDev<T> &d = lv[g.member];
   const SliceInfo q = d.si[s];
   const int lane = threadIdx.x % WAVE;
-  if (lane >= q.nrow) return;
-  const int r = q.row0 + lane;
+  const int gs = d.gs;
+  if ((lane >> gs) >= q.nrow) return;
+  const int r = q.row0 + (lane >> gs);
   const T br = d.b[r], dr = d.diag[r], xr = d.x[r];
-  const T acc = sell_dot<T>(q, lane, d.cols, d.vals, d.x);
-  d.t[r] = br - dr * xr - acc;
+  const T acc = row_group_sum(sell_dot<T>(q, lane, d.cols, d.vals, d.x), gs);
+  if ((lane & ((1 << gs) - 1)) == 0) d.t[r] = br - dr * xr - acc;
 }
 
 // b_{l+1} = P t_l, x_{l+1} = 0 (lv: level l, lvn: level l+1 of each member)
@@ -90,10 +92,14 @@ __global__ __launch_bounds__(BLK) void kb_restrict(const Seg *__restrict__ segs,
   const LevelDev<T> &dn = lvn[g.member];
   const SliceInfo q = d.p_si[s];
   const int lane = threadIdx.x % WAVE;
-  if (lane >= q.nrow) return;
-  const int r = q.row0 + lane;
-  dn.b[r] = sell_dot<T>(q, lane, d.p_cols, d.p_vals, d.t);
-  dn.x[r] = 0;
+  const int gs = d.p_gs;
+  if ((lane >> gs) >= q.nrow) return;
+  const int r = q.row0 + (lane >> gs);
+  const T acc = row_group_sum(sell_dot<T>(q, lane, d.p_cols, d.p_vals, d.t), gs);
+  if ((lane & ((1 << gs) - 1)) == 0) {
+    dn.b[r] = acc;
+    dn.x[r] = 0;
+  }
 }
 
 // x_l += P^T x_{l+1}

@@ -898,6 +898,7 @@ struct DevLevel {
   int64_t rows_c0 = 0;               // rows of colour 0 (first in the colour order)
   std::vector<int32_t> colour_row;   // host copy: rows of colour c are [colour_row[c], colour_row[c+1])
   int ns = 0, p_ns = 0, t_ns = 0;
+  int gs = 0, p_gs = 0; // lanes per row (shift) of the level matrix and of the restriction
   DevBuf<SliceInfo> si, p_si, t_si;
   DevBuf<int32_t> cols, p_cols, t_cols, col_slice;
   DevBuf<uint16_t> c16; // colour-coded 16-bit columns (streamed level 0), used by k_gs / k_residual
@@ -1050,8 +1051,9 @@ static void colour_code(HostLevel &hl) {
       lo[c] = INT32_MAX;
       hi[c] = INT32_MIN;
     }
+    const int64_t lanes = (int64_t)si.nrow << sa.gshift;
     for (int64_t j = 0; j < si.width; j++)
-      for (int64_t l = 0; l < si.nrow; l++) {
+      for (int64_t l = 0; l < lanes; l++) {
         const int32_t col = sa.cols[si.off + j * SELL_C + l];
         const int c = pc[col];
         lo[c] = std::min(lo[c], col);
@@ -1063,7 +1065,7 @@ static void colour_code(HostLevel &hl) {
       cb[(size_t)q * CC_BASES + c] = lo[c];
     }
     for (int64_t j = 0; j < si.width; j++)
-      for (int64_t l = 0; l < si.nrow; l++) {
+      for (int64_t l = 0; l < lanes; l++) {
         const int64_t e = si.off + j * SELL_C + l;
         const int c = pc[sa.cols[e]];
         c16[e] = (uint16_t)(c << cshift | (sa.cols[e] - cb[(size_t)q * CC_BASES + c]));
@@ -1123,15 +1125,27 @@ static void build_layout(const Hierarchy &h, HostLayout &lay, bool maps) {
     hl.n = A.m;
     tick(tph[1]);
   }
+  // lanes per row: level 0 keeps one lane per row (its colours fill the
+  // chip); AFE_ROW_LANES=G caps the lanes per row at G (1: one lane per row
+  // everywhere, the control of the tests)
+  int max_shift = MAX_ROW_LANE_SHIFT;
+  if (const char *e = getenv("AFE_ROW_LANES")) {
+    const int v = atoi(e);
+    max_shift = 0;
+    while (max_shift < MAX_ROW_LANE_SHIFT && (2 << max_shift) <= v) max_shift++;
+  }
   for (int l = 0; l < L; l++) {
     HostLevel &hl = lay.lv[l];
-    build_sell(h.A[l], hl.perm, hl.iperm, groups[l], true, hl.sa, lay.maps ? &hl.smap : nullptr,
-               lay.maps ? &hl.dsrc : nullptr);
+    const Csr &A = h.A[l];
+    const int gs = l == 0 ? 0 : row_lane_shift(A.m, hl.ncol, A.nnz() - A.m, max_shift);
+    build_sell(A, hl.perm, hl.iperm, groups[l], true, hl.sa, lay.maps ? &hl.smap : nullptr,
+               lay.maps ? &hl.dsrc : nullptr, gs);
     if ((int64_t)hl.sa.cols.size() * 8 > stream_bytes() && hl.ncol <= CC_BASES && getenv("AFE_NO_C16") == nullptr)
       colour_code(hl);
     tick(tph[2]);
     if (l + 1 < L) {
-      build_sell(h.P[l], lay.lv[l + 1].perm, hl.iperm, {(int32_t)h.P[l].m}, false, hl.sp);
+      build_sell(h.P[l], lay.lv[l + 1].perm, hl.iperm, {(int32_t)h.P[l].m}, false, hl.sp, nullptr, nullptr,
+                 l == 0 ? 0 : row_lane_shift(h.P[l].m, 1, h.P[l].nnz(), max_shift));
       // prolongation is P^T of the restriction (SparseMatrix::Tvmult of P in
       // AMGSolver::solve, src/AMGSolver.cpp:608-613), stored explicitly
       build_sell(transpose(h.P[l]), hl.perm, lay.lv[l + 1].iperm, {(int32_t)h.A[l].m}, false, hl.st);
@@ -1218,6 +1232,7 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
     d.n = hl.n;
     Sell &sa = hl.sa;
     d.ns = (int)sa.slices.size();
+    d.gs = sa.gshift;
     d.colour_slice = sa.group_slice;
     d.rows_c0 = sa.group_row.size() > 1 ? sa.group_row[1] : hl.n;
     d.colour_row = sa.group_row;
@@ -1240,6 +1255,7 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
     if (l + 1 < L) {
       Sell &sp = hl.sp, &st = hl.st;
       d.p_ns = (int)sp.slices.size();
+      d.p_gs = sp.gshift;
       d.t_ns = (int)st.slices.size();
       d.p_si.upload(ctx, sp.slices);
       d.p_cols.upload(ctx, sp.cols);
@@ -1318,6 +1334,7 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
     DevLevel &d = g->lv[l];
     LevelDev<double> a{};
     a.n = (int)d.n; a.ncol = d.ncol; a.ns = d.ns; a.p_ns = d.p_ns; a.t_ns = d.t_ns;
+    a.gs = d.gs; a.p_gs = d.p_gs;
     a.col_slice = d.col_slice.p; a.si = d.si.p; a.cols = d.cols.p;
     a.vals = d.v64.p; a.invd = d.invd64.p; a.diag = d.diag64.p;
     a.p_si = d.p_si.p; a.p_cols = d.p_cols.p; a.p_vals = d.p_v64.p;
@@ -1326,6 +1343,7 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
     d64[l] = a;
     LevelDev<float> b{};
     b.n = a.n; b.ncol = a.ncol; b.ns = a.ns; b.p_ns = a.p_ns; b.t_ns = a.t_ns;
+    b.gs = a.gs; b.p_gs = a.p_gs;
     b.col_slice = a.col_slice; b.si = a.si; b.cols = a.cols;
     b.vals = d.v32.p; b.invd = d.invd32.p; b.diag = d.diag32.p;
     b.p_si = a.p_si; b.p_cols = a.p_cols; b.p_vals = d.p_v32.p;
@@ -1629,11 +1647,11 @@ static void gs_colour(DevLevel &d, int c, hipStream_t st) {
   if (d.stream)
     k_gs<T, true><<<sblk(s1 - s0), BLK, 0, st>>>(d.si.p, s0, s1, d.cols.p, d.c16.p, d.cb16.p, d.cshift,
                                                  Prec<T>::v(d), Prec<T>::invd(d),
-                                                 Prec<T>::b(d), Prec<T>::x(d), g_tune.xcd);
+                                                 Prec<T>::b(d), Prec<T>::x(d), d.gs, g_tune.xcd);
   else
     k_gs<T, false><<<sblk(s1 - s0), BLK, 0, st>>>(d.si.p, s0, s1, d.cols.p, nullptr, nullptr, 0,
                                                   Prec<T>::v(d), Prec<T>::invd(d),
-                                                  Prec<T>::b(d), Prec<T>::x(d), g_tune.xcd);
+                                                  Prec<T>::b(d), Prec<T>::x(d), d.gs, g_tune.xcd);
 }
 
 // one V(nu, nu) cycle on level-0 vectors (b, x) of precision T; the
@@ -1658,15 +1676,19 @@ static void issue_vcycle(afe_amg *g, int nu, bool symmetric, hipStream_t st, boo
     if (d.stream) {
       k_residual<T, true><<<sblk(d.ns), BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, d.c16.p, d.cb16.p, d.cshift,
                                                        Prec<T>::v(d), Prec<T>::diag(d),
-                                                       Prec<T>::b(d), Prec<T>::x(d), Prec<T>::t(d), g_tune.xcd);
+                                                       Prec<T>::b(d), Prec<T>::x(d), Prec<T>::t(d), d.gs,
+                                                       g_tune.xcd);
       k_restrict<T, true><<<sblk(d.p_ns), BLK, 0, st>>>(d.p_si.p, d.p_ns, d.p_cols.p, Prec<T>::pv(d),
-                                                         Prec<T>::t(d), Prec<T>::b(dn), Prec<T>::x(dn), g_tune.xcd);
+                                                         Prec<T>::t(d), Prec<T>::b(dn), Prec<T>::x(dn), d.p_gs,
+                                                         g_tune.xcd);
     } else {
       k_residual<T, false><<<sblk(d.ns), BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, nullptr, nullptr, 0,
                                                         Prec<T>::v(d), Prec<T>::diag(d),
-                                                        Prec<T>::b(d), Prec<T>::x(d), Prec<T>::t(d), g_tune.xcd);
+                                                        Prec<T>::b(d), Prec<T>::x(d), Prec<T>::t(d), d.gs,
+                                                        g_tune.xcd);
       k_restrict<T, false><<<sblk(d.p_ns), BLK, 0, st>>>(d.p_si.p, d.p_ns, d.p_cols.p, Prec<T>::pv(d),
-                                                          Prec<T>::t(d), Prec<T>::b(dn), Prec<T>::x(dn), g_tune.xcd);
+                                                          Prec<T>::t(d), Prec<T>::b(dn), Prec<T>::x(dn), d.p_gs,
+                                                          g_tune.xcd);
     }
   }
   const int lds_rows = (int)(g->ctx->lds_bytes / (3 * sizeof(T)));
@@ -2087,7 +2109,7 @@ extern "C" int afe_amg_kernel_timing(afe_amg *g, int32_t which, int32_t reps, do
     } else {
       k_residual<float, true><<<sblk(d.ns), BLK, 0, s>>>(d.si.p, d.ns, d.cols.p, d.c16.p, d.cb16.p, d.cshift,
                                                           d.v32.p, d.diag32.p, d.b32.p,
-                                                          d.x32.p, d.t32.p, g_tune.xcd);
+                                                          d.x32.p, d.t32.p, d.gs, g_tune.xcd);
     }
   };
   once();
@@ -2170,13 +2192,14 @@ extern "C" int afe_amg_set_tail_rows(afe_amg *g, int64_t rows) {
 
 // per-level device layout summary: rows, colours, stored sliced-ELL entries
 extern "C" int afe_amg_device_info(const afe_amg *g, int32_t level, int64_t *rows, int32_t *colours,
-                                   int64_t *stored, int32_t *tail_level) {
+                                   int64_t *stored, int32_t *tail_level, int32_t *row_lanes) {
   if (!g || !g->ctx || level < 0 || level >= (int)g->lv.size()) return fail(AFE_E_ARG, "bad level");
   const DevLevel &d = g->lv[level];
   if (rows) *rows = d.n;
   if (colours) *colours = d.ncol;
   if (stored) *stored = (int64_t)d.cols.n;
   if (tail_level) *tail_level = g->tail0;
+  if (row_lanes) *row_lanes = 1 << d.gs;
   return AFE_OK;
 }
 

@@ -744,12 +744,25 @@ int colour_rows(const Csr &a, std::vector<int32_t> &colour) {
   return ncol;
 }
 
+int row_lane_shift(int64_t rows, int64_t groups, int64_t entries, int max_shift) {
+  if (rows <= 0 || groups <= 0) return 0;
+  const int64_t per_group = (rows + groups - 1) / groups;
+  int s = 0;
+  while (s < std::min(max_shift, MAX_ROW_LANE_SHIFT) && (int64_t(2) << s) * 4 * rows <= entries &&
+         (per_group << s) < (int64_t(1) << 16))
+    s++;
+  return s;
+}
+
 void build_sell(const Csr &a, const std::vector<int32_t> &perm,
                 const std::vector<int32_t> &col_map,
                 const std::vector<int32_t> &group_rows, bool split_diag,
                 Sell &out, std::vector<int32_t> *src_map,
-                std::vector<int32_t> *diag_src) {
+                std::vector<int32_t> *diag_src, int gshift) {
+  if (gshift < 0 || gshift > MAX_ROW_LANE_SHIFT) throw Error{-1, "lanes per row out of range"};
   out = Sell();
+  out.gshift = gshift;
+  const int G = 1 << gshift, R = SELL_C >> gshift; // lanes per row, rows per slice
   out.n_rows = (int64_t)perm.size();
   out.n_cols = a.n;
   if (split_diag) out.diag.assign(out.n_rows, 0.0);
@@ -768,10 +781,11 @@ void build_sell(const Csr &a, const std::vector<int32_t> &perm,
   out.group_row.push_back(0);
   for (int32_t gr : group_rows) {
     const int64_t rend = r + gr;
-    for (int64_t r0 = r; r0 < rend; r0 += SELL_C) {
-      const int32_t nrow = (int32_t)std::min<int64_t>(SELL_C, rend - r0);
-      int64_t width = 0;
-      for (int32_t l = 0; l < nrow; l++) width = std::max(width, stored_len(perm[r0 + l]));
+    for (int64_t r0 = r; r0 < rend; r0 += R) {
+      const int32_t nrow = (int32_t)std::min<int64_t>(R, rend - r0);
+      int64_t len = 0;
+      for (int32_t l = 0; l < nrow; l++) len = std::max(len, stored_len(perm[r0 + l]));
+      const int64_t width = (len + G - 1) >> gshift; // steps of SELL_C entries
       out.slices.push_back({(int32_t)r0, nrow, (int32_t)width, (int32_t)off});
       off += width * SELL_C;
     }
@@ -797,15 +811,16 @@ void build_sell(const Csr &a, const std::vector<int32_t> &perm,
           if (diag_src) (*diag_src)[nr] = (int32_t)k;
           continue;
         }
-        const int64_t pos = si.off + j * SELL_C + l;
+        const int64_t pos = si.off + (j >> gshift) * SELL_C + (l << gshift) + (j & (G - 1));
         out.cols[pos] = col_map[a.col[k]];
         out.vals[pos] = a.val[k];
         if (src_map) (*src_map)[k] = (int32_t)pos;
         j++;
       }
-      for (; j < si.width; j++) {
-        out.cols[si.off + j * SELL_C + l] = pad_col;
-        out.vals[si.off + j * SELL_C + l] = 0.0;
+      for (; j < (int64_t)si.width << gshift; j++) {
+        const int64_t pos = si.off + (j >> gshift) * SELL_C + (l << gshift) + (j & (G - 1));
+        out.cols[pos] = pad_col;
+        out.vals[pos] = 0.0;
       }
     }
   }

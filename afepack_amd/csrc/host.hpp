@@ -98,10 +98,15 @@ int dense_inverse(int n, double *a);
 
 // ---- multicolour sliced-ELL (device layout) -------------------------------
 // Rows are permuted so that each colour is a contiguous range, sorted by
-// decreasing stored length inside the colour; slices of SELL_C consecutive
-// rows never straddle a colour.  Element (row0+lane, j) of a slice lives at
-// off + j*SELL_C + lane.  Padding uses the row itself as column with value 0.
+// decreasing stored length inside the colour; slices never straddle a colour.
+// A slice is one wavefront: with G = 2^gshift lanes per row it holds
+// SELL_C / G consecutive rows, and stored entry j of its row l lives at
+// off + (j / G)*SELL_C + l*G + j % G, so lane l*G + k walks entries k, k+G,
+// k+2G, ... of row l at off + step*SELL_C + lane (width = steps).  G = 1 is
+// the plain one-lane-per-row sliced ELL.  Padding uses the row itself as
+// column with value 0.
 constexpr int SELL_C = 64;
+constexpr int MAX_ROW_LANE_SHIFT = 4; // at most 16 lanes per row
 
 struct SliceInfo {
   int32_t row0, nrow, width, off;
@@ -115,8 +120,17 @@ struct Sell {
   std::vector<int32_t> cols;
   std::vector<double> vals;
   std::vector<double> diag; // only for square matrices stored without diagonal
+  int gshift = 0;           // 2^gshift lanes per row
   int64_t stored() const { return (int64_t)cols.size(); }
 };
+
+// lanes per row (as a shift) of a matrix whose rows are launched in groups
+// (colours) of rows/groups rows on average with `entries` stored entries in
+// all: rows are split over more lanes while every lane keeps at least 4
+// entries and a group stays under 2^16 lanes (the small levels of the
+// V-cycle, whose colours are a few thousand rows of long rows, are latency
+// bound with one lane per row); max_shift caps it
+int row_lane_shift(int64_t rows, int64_t groups, int64_t entries, int max_shift);
 
 // colouring of the symmetrised graph of the stored off-diagonal entries
 // (explicit zeros included): DSatur for levels of at most DSATUR_MAX_ROWS
@@ -129,11 +143,12 @@ int colour_rows(const Csr &a, std::vector<int32_t> &colour);
 // groups: row-count per group in new order (slices never cross groups)
 // src_map (optional): position of every CSR entry in the layout, -1 for a
 // split diagonal; diag_src (optional): CSR index of each stored row's diagonal
+// gshift: 2^gshift lanes per row
 void build_sell(const Csr &a, const std::vector<int32_t> &perm,
                 const std::vector<int32_t> &col_map,
                 const std::vector<int32_t> &group_rows, bool split_diag,
                 Sell &out, std::vector<int32_t> *src_map = nullptr,
-                std::vector<int32_t> *diag_src = nullptr);
+                std::vector<int32_t> *diag_src = nullptr, int gshift = 0);
 
 // host threads of the parallel loops started by the calling thread
 void set_host_threads(int n);

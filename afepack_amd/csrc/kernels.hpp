@@ -589,6 +589,19 @@ __device__ inline T sell_dot_cc(const SliceInfo &s, int lane, const uint16_t *__
 
 __device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE); }
 
+// A row of a slice with 2^gs lanes per row: lane = row_in_slice * 2^gs + k,
+// and sell_dot of lane gives the partial sum of the row's entries k, k + 2^gs,
+// ... in stored order.  The row's sum adds the partials by an xor butterfly
+// over the group's lanes (distance 1, 2, 4, ...); a + b and b + a are the
+// same bits, so every lane of the group ends with the same sum.  gs = 0 is
+// the one-lane-per-row sum unchanged.  The groups of a wavefront are whole
+// (a slice's rows are aligned groups), so every lane read by a shuffle is
+// active.
+template <class T> __device__ inline T row_group_sum(T v, int gs) {
+  for (int m = 1; m < (1 << gs); m <<= 1) v += __shfl_xor(v, m, WAVE);
+  return v;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs; with xcd != 0 the
 // logical block index makes consecutive logical blocks share an XCD, so each
 // XCD's L2 serves a contiguous (spatially local) range of slices.  gridDim.x
@@ -618,24 +631,27 @@ __device__ inline void slice_range(int ns, int sched, int &s, int &end, int &ste
 // Gauss-Seidel over the slices [s0, s1) of one colour:
 // x_r = (b_r - sum_{j != r} a_rj x_j) / a_rr
 // (c16 != null: colour-coded 16-bit columns c16 / cb16 / cshift, see
-// sell_dot_cc; otherwise the int32 columns)
+// sell_dot_cc; otherwise the int32 columns; 2^gs lanes per row).  No row of
+// the colour reads another's x, so the group's leader writes x_r once every
+// lane of the group has read (the butterfly orders it after those reads).
 template <class T, bool NT>
 __global__ __launch_bounds__(BLK) void k_gs(const SliceInfo *__restrict__ si, int s0, int s1,
                                             const int32_t *__restrict__ cols, const uint16_t *__restrict__ c16,
                                             const int32_t *__restrict__ cb16, int cshift,
                                             const T *__restrict__ vals, const T *__restrict__ invd,
-                                            const T *__restrict__ b, T *__restrict__ x, int xcd) {
+                                            const T *__restrict__ b, T *__restrict__ x, int gs, int xcd) {
   const int s = s0 + logical_block(xcd) * WPB + wave_id();
   if (s >= s1) return;
   const SliceInfo q = si[s];
   const int lane = threadIdx.x % WAVE;
-  if (lane >= q.nrow) return;
-  const int r = q.row0 + lane;
+  if ((lane >> gs) >= q.nrow) return;
+  const int r = q.row0 + (lane >> gs);
   const T br = b[r], ir = invd[r]; // issued before the row's dependent loads
-  const T acc = c16 ? sell_dot_cc<T, NT, LaunchChunk<T, NT>::v>(q, lane, c16, cb16 + (int64_t)s * CC_BASES, cshift,
-                                                                vals, x)
-                    : sell_dot<T, NT, LaunchChunk<T, NT>::v>(q, lane, cols, vals, x);
-  x[r] = (br - acc) * ir;
+  T acc = c16 ? sell_dot_cc<T, NT, LaunchChunk<T, NT>::v>(q, lane, c16, cb16 + (int64_t)s * CC_BASES, cshift,
+                                                          vals, x)
+              : sell_dot<T, NT, LaunchChunk<T, NT>::v>(q, lane, cols, vals, x);
+  acc = row_group_sum(acc, gs);
+  if ((lane & ((1 << gs) - 1)) == 0) x[r] = (br - acc) * ir;
 }
 
 // the first colour of a sweep that starts from x = 0 (level 0 of the
@@ -657,18 +673,19 @@ __global__ __launch_bounds__(BLK) void k_residual(const SliceInfo *__restrict__ 
                                                   const int32_t *__restrict__ cb16, int cshift,
                                                   const T *__restrict__ vals, const T *__restrict__ diag,
                                                   const T *__restrict__ b, const T *__restrict__ x,
-                                                  T *__restrict__ t, int xcd) {
+                                                  T *__restrict__ t, int gs, int xcd) {
   const int s = logical_block(xcd) * WPB + wave_id();
   if (s >= ns) return;
   const SliceInfo q = si[s];
   const int lane = threadIdx.x % WAVE;
-  if (lane >= q.nrow) return;
-  const int r = q.row0 + lane;
+  if ((lane >> gs) >= q.nrow) return;
+  const int r = q.row0 + (lane >> gs);
   const T br = b[r], dr = diag[r], xr = x[r];
-  const T acc = c16 ? sell_dot_cc<T, NT, LaunchChunk<T, NT>::v>(q, lane, c16, cb16 + (int64_t)s * CC_BASES, cshift,
-                                                                vals, x)
-                    : sell_dot<T, NT, LaunchChunk<T, NT>::v>(q, lane, cols, vals, x);
-  t[r] = br - dr * xr - acc;
+  T acc = c16 ? sell_dot_cc<T, NT, LaunchChunk<T, NT>::v>(q, lane, c16, cb16 + (int64_t)s * CC_BASES, cshift,
+                                                          vals, x)
+              : sell_dot<T, NT, LaunchChunk<T, NT>::v>(q, lane, cols, vals, x);
+  acc = row_group_sum(acc, gs);
+  if ((lane & ((1 << gs) - 1)) == 0) t[r] = br - dr * xr - acc;
 }
 
 // restriction: bc = P t, and the coarse guess xc = 0
@@ -676,15 +693,18 @@ template <class T, bool NT>
 __global__ __launch_bounds__(BLK) void k_restrict(const SliceInfo *__restrict__ si, int ns,
                                                   const int32_t *__restrict__ cols, const T *__restrict__ vals,
                                                   const T *__restrict__ t, T *__restrict__ bc,
-                                                  T *__restrict__ xc, int xcd) {
+                                                  T *__restrict__ xc, int gs, int xcd) {
   const int s = logical_block(xcd) * WPB + wave_id();
   if (s >= ns) return;
   const SliceInfo q = si[s];
   const int lane = threadIdx.x % WAVE;
-  if (lane >= q.nrow) return;
-  const int r = q.row0 + lane;
-  bc[r] = sell_dot<T, NT, LaunchChunk<T, NT>::v>(q, lane, cols, vals, t);
-  xc[r] = 0;
+  if ((lane >> gs) >= q.nrow) return;
+  const int r = q.row0 + (lane >> gs);
+  const T acc = row_group_sum(sell_dot<T, NT, LaunchChunk<T, NT>::v>(q, lane, cols, vals, t), gs);
+  if ((lane & ((1 << gs) - 1)) == 0) {
+    bc[r] = acc;
+    xc[r] = 0;
+  }
 }
 
 // prolongation: x += P^T xc (explicit transpose, no atomics)
@@ -707,6 +727,7 @@ __global__ __launch_bounds__(BLK) void k_prolong(const SliceInfo *__restrict__ s
 // ---------------------------------------------------------------------------
 template <class T> struct LevelDev {
   int n, ncol, ns, p_ns, t_ns;
+  int gs, p_gs; // lanes per row (shift) of the level matrix and of the restriction
   const int32_t *col_slice; // ncol+1 slice bounds
   const SliceInfo *si;
   const int32_t *cols;
@@ -722,13 +743,14 @@ template <class T> struct LevelDev {
 
 template <class T>
 __device__ inline void tail_sweep(const LevelDev<T> &d, const T *b, T *x, int c, int w, int nw, int lane) {
+  const int gs = d.gs;
   for (int s = d.col_slice[c] + w; s < d.col_slice[c + 1]; s += nw) {
     const SliceInfo q = d.si[s];
-    if (lane < q.nrow) {
-      const int r = q.row0 + lane;
+    if ((lane >> gs) < q.nrow) {
+      const int r = q.row0 + (lane >> gs);
       const T br = b[r], ir = d.invd[r];
-      const T acc = sell_dot<T>(q, lane, d.cols, d.vals, x);
-      x[r] = (br - acc) * ir;
+      const T acc = row_group_sum(sell_dot<T>(q, lane, d.cols, d.vals, x), gs);
+      if ((lane & ((1 << gs) - 1)) == 0) x[r] = (br - acc) * ir;
     }
   }
 }
@@ -774,21 +796,24 @@ __device__ inline void tail_body(const LevelDev<T> *__restrict__ L, int l0, int 
         }
       for (int s = w; s < d.ns; s += nw) {
         const SliceInfo q = d.si[s];
-        if (lane < q.nrow) {
-          const int r = q.row0 + lane;
+        if ((lane >> d.gs) < q.nrow) {
+          const int r = q.row0 + (lane >> d.gs);
           const T br = b[r], dr = d.diag[r], xr = x[r];
-          const T acc = sell_dot<T>(q, lane, d.cols, d.vals, x);
-          t[r] = br - dr * xr - acc;
+          const T acc = row_group_sum(sell_dot<T>(q, lane, d.cols, d.vals, x), d.gs);
+          if ((lane & ((1 << d.gs) - 1)) == 0) t[r] = br - dr * xr - acc;
         }
       }
       __syncthreads();
       const LevelDev<T> dn = L[l + 1];
       for (int s = w; s < d.p_ns; s += nw) {
         const SliceInfo q = d.p_si[s];
-        if (lane < q.nrow) {
-          const int r = q.row0 + lane;
-          dn.b[r] = sell_dot<T>(q, lane, d.p_cols, d.p_vals, t);
-          dn.x[r] = 0;
+        if ((lane >> d.p_gs) < q.nrow) {
+          const int r = q.row0 + (lane >> d.p_gs);
+          const T acc = row_group_sum(sell_dot<T>(q, lane, d.p_cols, d.p_vals, t), d.p_gs);
+          if ((lane & ((1 << d.p_gs) - 1)) == 0) {
+            dn.b[r] = acc;
+            dn.x[r] = 0;
+          }
         }
       }
       if (lds)

@@ -379,10 +379,13 @@ int afe_set_tuning(int32_t xcd, int32_t spmv_blocks, int64_t tail_rows);
  * levels of at most `rows` rows run in the tail (batches of small problems
  * run one tail workgroup per member in parallel) */
 int afe_amg_set_tail_rows(afe_amg *g, int64_t rows);
-/* device layout of a level: rows, colours, stored sliced-ELL entries and the
- * first level handled by the single-workgroup coarse tail */
+/* device layout of a level: rows, colours, stored sliced-ELL entries, the
+ * first level handled by the single-workgroup coarse tail, and the lanes per
+ * row of the level matrix's kernels (1 on level 0; up to 16 on small levels
+ * of long rows, whose row sums then add per-lane partials) */
 int afe_amg_device_info(const afe_amg *g, int32_t level, int64_t *rows,
-                        int32_t *colours, int64_t *stored, int32_t *tail_level);
+                        int32_t *colours, int64_t *stored, int32_t *tail_level,
+                        int32_t *row_lanes);
 
 #ifdef __cplusplus
 }

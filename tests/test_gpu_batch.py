@@ -83,12 +83,22 @@ def test_batch_alone_equals_mixed_bitwise(ctx, problems):
         assert alone[i][1] == mixed[i][1] == late[i][1]
 
 
-def test_batch_solutions_match_direct(ctx, problems):
+@pytest.mark.parametrize("tail", [4096, 0])
+def test_batch_solutions_match_direct(ctx, problems, tail):
+    """tail 0: every level of every member but its coarsest runs in the
+    shared per-level launches (several lanes per row on the 3D member's
+    coarse levels), else the small levels run in the per-member tails."""
     import scipy.sparse as sps
     import scipy.sparse.linalg as spl
-    print("case batch vs direct", flush=True)
+    print(f"case batch vs direct tail={tail}", flush=True)
     idx = list(range(len(problems)))
-    res = _solve_batch(ctx, problems, idx, 1e-12)
+    try:
+        for p in problems:
+            p["g"].set_tail_rows(tail)
+        res = _solve_batch(ctx, problems, idx, 1e-12)
+    finally:
+        for p in problems:
+            p["g"].set_tail_rows(4096)
     for i in idx:
         p = problems[i]
         rp, col = p["sp"].pattern()

@@ -141,3 +141,46 @@ def test_natural_columns_16_and_32_bit_agree(ctx, kind, n, stream):
     assert np.array_equal(xs[0][1], xs[1][1])
     nnz = sp.nnz
     assert xs[1][2] - xs[0][2] >= 2 * nnz - 4 * (sp.n_dof // 64 + 1) - 1  # 2 bytes per entry saved
+
+
+@pytest.mark.parametrize("tail", [4096, 0])
+def test_row_lanes_agree_with_one_lane_per_row(ctx, tail):
+    """Small levels of long rows split each row over 2, 4, 8 or 16 lanes
+    (per-lane partial sums added by a butterfly); with AFE_ROW_LANES=1 every
+    level keeps one lane per row.  The two layouts hold the same matrices and
+    differ only in the order of each row's sum: the V-cycles agree to FP64
+    rounding and the PCG solves to the solve tolerance."""
+    import os
+    import afepack_amd as afe
+    print(f"case row lanes tail={tail}", flush=True)
+    c, ev, vm = make_mesh(3, 12, 0.05, 5)
+    sp = afe.Space(ctx, 3, c, ev, vm)
+    A = afe.Matrix(sp)
+    A.assemble(afe.OP_VARSTIFF, 2, afe.field_random(8, 3, 0.5, 5))
+    b = sp.rhs(2, afe.F_POISSON)
+    A.apply_dirichlet(b)
+    out = []
+    for lanes in (None, "1"):
+        if lanes:
+            os.environ["AFE_ROW_LANES"] = lanes
+        try:
+            g = afe.Amg(A, afe.AmgParams(lazy=1, smooth=1, build_fp32=1))
+        finally:
+            os.environ.pop("AFE_ROW_LANES", None)
+        g.set_tail_rows(tail)
+        rl = [d["row_lanes"] for d in g.device_info()]
+        z = afe.DeviceVector(ctx, sp.n_dof)
+        g.vcycle(z, b, afe.PREC_FP64)
+        x64 = afe.DeviceVector(ctx, sp.n_dof)
+        s64 = g.pcg(x64, b, rtol=1e-11, max_it=300, prec=afe.PREC_FP64)
+        x32 = afe.DeviceVector(ctx, sp.n_dof)
+        s32 = g.pcg(x32, b, rtol=1e-11, max_it=300, prec=afe.PREC_FP32)
+        out.append((rl, z.numpy(), s64, x64.numpy(), s32, x32.numpy()))
+        g.close()
+    (rl, z, s64, x64, s32, x32), (rl1, z1, t64, y64, t32, y32) = out
+    assert rl[0] == 1 and max(rl) > 1 and set(rl1) == {1}
+    assert np.linalg.norm(z - z1) <= 1e-13 * np.linalg.norm(z1)
+    assert s64.converged == t64.converged == 1 and abs(s64.iterations - t64.iterations) <= 1
+    assert s32.converged == t32.converged == 1 and abs(s32.iterations - t32.iterations) <= 1
+    assert np.linalg.norm(x64 - y64) <= 1e-9 * np.linalg.norm(y64)
+    assert np.linalg.norm(x32 - y32) <= 1e-9 * np.linalg.norm(y32)

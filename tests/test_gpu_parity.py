@@ -244,13 +244,23 @@ def _model_vcycle(As, Ps, colours, nu):
     return lambda b: vc(0, b)
 
 
-@pytest.mark.parametrize("nmpo,nmpsd", [(50, 0.382), (400, 0.0)])
-def test_vcycle_matches_dense_model(ctx, nmpo, nmpsd):
+@pytest.mark.parametrize("kind,n,jitter,acc,nmpo,nmpsd", [
+    (1, 32, 0.0, 4, 50, 0.382),
+    (1, 32, 0.0, 4, 400, 0.0),
+    (3, 8, 0.05, 2, 50, 0.382),   # coarse levels with several lanes per row
+])
+@pytest.mark.parametrize("tail", [4096, 0])
+def test_vcycle_matches_dense_model(ctx, kind, n, jitter, acc, nmpo, nmpsd, tail):
     """One symmetric FP64 V(1,1)-cycle from zero on the device against the
-    dense numpy model of the same cycle."""
+    dense numpy model of the same cycle, with the small levels in the
+    single-workgroup tail (tail 4096) and as per-level launches (tail 0)."""
     import afepack_amd as afe
-    sp, osp, A, OA, f, of = _system(ctx, 1, 32, 0.0, 4)
+    print(f"case vcycle model kind={kind} n={n} nmpo={nmpo} tail={tail}", flush=True)
+    sp, osp, A, OA, f, of = _system(ctx, kind, n, jitter, acc)
     g = afe.Amg(A, afe.AmgParams(lazy=1, nmpo=nmpo, nmpsd=nmpsd, smooth=1))
+    g.set_tail_rows(tail)
+    if kind == 3:
+        assert max(d["row_lanes"] for d in g.device_info()) > 1
     z = _zeros(ctx, sp.n_dof)
     g.vcycle(z, f, afe.PREC_FP64)
     As, Ps, Gs = _dense_levels(g)
@@ -258,16 +268,17 @@ def test_vcycle_matches_dense_model(ctx, nmpo, nmpsd):
     assert np.linalg.norm(z.numpy() - ref) <= 1e-12 * np.linalg.norm(ref)
 
 
-@pytest.mark.parametrize("smooth", [0, 1, 2])
-def test_pcg_iterates_match_model(ctx, smooth):
+@pytest.mark.parametrize("kind,n,jitter,acc,smooth", [
+    (1, 24, 0.1, 4, 0), (1, 24, 0.1, 4, 1), (1, 24, 0.1, 4, 2), (3, 8, 0.05, 2, 1)])
+def test_pcg_iterates_match_model(ctx, kind, n, jitter, acc, smooth):
     """Three iterations of the FP64 flexible PCG on the device (several
     levels run as launches, the first level-0 colour from the zero guess, and
     no smoothing at all) against numpy flexible PCG with the dense V-cycle
     model: x_{k+1} = x_k + alpha p, alpha = (r, z) / (p, A p),
     beta = ((r, z_new) - (r, z_old)) / (r_old, z_old)."""
     import afepack_amd as afe
-    print(f"case pcg model smooth={smooth}", flush=True)
-    sp, osp, A, OA, f, of = _system(ctx, 1, 24, 0.1, 4)
+    print(f"case pcg model kind={kind} smooth={smooth}", flush=True)
+    sp, osp, A, OA, f, of = _system(ctx, kind, n, jitter, acc)
     g = afe.Amg(A, afe.AmgParams(lazy=1, nmpo=50, nmpsd=0.382, smooth=max(smooth, 1), build_fp32=1))
     g.set_tail_rows(0)  # every level but the coarsest as separate launches
     g.set_smooth(smooth)
