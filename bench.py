@@ -290,7 +290,8 @@ def run_c2(args):
     info = g.info()
     dinfo = g.device_info()
     log(f"AMG setup {t_amg:.2f} s: levels {info['levels']} n {info['n']} "
-        f"colours {[d['colours'] for d in dinfo]} tail from level {dinfo[0]['tail']}")
+        f"colours {[d['colours'] for d in dinfo]} lanes per row {[d['row_lanes'] for d in dinfo]} "
+        f"tail from level {dinfo[0]['tail']}")
 
     def solve(prec, s):
         return solve_with(g, prec, s)
@@ -404,6 +405,7 @@ def run_c2(args):
             "n_dof": sp.n_dof, "nnz": sp.nnz, "rtol": args.rtol,
             "hierarchy": "lazy (AMGSolver::lazyReinit)", "levels": info["levels"],
             "level_n": info["n"], "colours": [d["colours"] for d in dinfo],
+            "row_lanes": [d["row_lanes"] for d in dinfo],
             "pcg_smooth": s_mixed, "amg_nmpo": nmpo, "amg_nmpsd": nmpsd,
             "launch": {"xcd_order": best_t[1], "spmv_blocks": best_t[2], "tail_rows": tail_rows},
             "parallelism": "single GPU",
@@ -481,7 +483,7 @@ def run_c3(args):
     t_amg = time.time() - t
     dinfo = g.device_info()
     log(f"c3 AMG: levels {g.info()['n']}, colours {[d['colours'] for d in dinfo]}, "
-        f"stored entries {[d['stored'] for d in dinfo]}")
+        f"lanes per row {[d['row_lanes'] for d in dinfo]}, stored entries {[d['stored'] for d in dinfo]}")
     lat, its = [], []
     for k in range(args.warmup + args.steps):
         x.zero()
@@ -516,6 +518,7 @@ def run_c3(args):
                                 "tetrahedra, P1, accuracy 2"),
                    "n_dof": sp.n_dof, "nnz": sp.nnz, "rtol": args.rtol, "amg_nmpo": nmpo, "amg_nmpsd": nmpsd,
                    "pcg_smooth": s, "levels": g.info()["n"], "colours": [d["colours"] for d in dinfo],
+                   "row_lanes": [d["row_lanes"] for d in dinfo],
                    "parallelism": "single GPU"},
         "iterations": its[-1], "pcg64": {"seconds": t64, "iterations": st64.iterations,
                                           "dof_per_s": sp.n_dof / t64},
