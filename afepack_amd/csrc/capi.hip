@@ -1126,18 +1126,22 @@ static void build_layout(const Hierarchy &h, HostLayout &lay, bool maps) {
     tick(tph[1]);
   }
   // lanes per row: level 0 keeps one lane per row (its colours fill the
-  // chip); AFE_ROW_LANES=G caps the lanes per row at G (1: one lane per row
-  // everywhere, the control of the tests)
-  int max_shift = MAX_ROW_LANE_SHIFT;
+  // chip), the other levels and their restrictions follow row_lane_shift.
+  // AFE_ROW_LANES=G (1, 2, 4, 8 or 16) sets G on every level >= 1 instead
+  // (G = 1: one lane per row everywhere; the tests compare the forms)
+  int forced = -1;
   if (const char *e = getenv("AFE_ROW_LANES")) {
     const int v = atoi(e);
-    max_shift = 0;
-    while (max_shift < MAX_ROW_LANE_SHIFT && (2 << max_shift) <= v) max_shift++;
+    forced = 0;
+    while (forced < MAX_ROW_LANE_SHIFT && (2 << forced) <= v) forced++;
   }
+  auto lane_shift = [&](int64_t rows, int64_t groups, int64_t entries) {
+    return forced >= 0 ? forced : row_lane_shift(rows, groups, entries, MAX_ROW_LANE_SHIFT);
+  };
   for (int l = 0; l < L; l++) {
     HostLevel &hl = lay.lv[l];
     const Csr &A = h.A[l];
-    const int gs = l == 0 ? 0 : row_lane_shift(A.m, hl.ncol, A.nnz() - A.m, max_shift);
+    const int gs = l == 0 ? 0 : lane_shift(A.m, hl.ncol, A.nnz() - A.m);
     build_sell(A, hl.perm, hl.iperm, groups[l], true, hl.sa, lay.maps ? &hl.smap : nullptr,
                lay.maps ? &hl.dsrc : nullptr, gs);
     if ((int64_t)hl.sa.cols.size() * 8 > stream_bytes() && hl.ncol <= CC_BASES && getenv("AFE_NO_C16") == nullptr)
@@ -1145,7 +1149,7 @@ static void build_layout(const Hierarchy &h, HostLayout &lay, bool maps) {
     tick(tph[2]);
     if (l + 1 < L) {
       build_sell(h.P[l], lay.lv[l + 1].perm, hl.iperm, {(int32_t)h.P[l].m}, false, hl.sp, nullptr, nullptr,
-                 l == 0 ? 0 : row_lane_shift(h.P[l].m, 1, h.P[l].nnz(), max_shift));
+                 l == 0 ? 0 : lane_shift(h.P[l].m, 1, h.P[l].nnz()));
       // prolongation is P^T of the restriction (SparseMatrix::Tvmult of P in
       // AMGSolver::solve, src/AMGSolver.cpp:608-613), stored explicitly
       build_sell(transpose(h.P[l]), hl.perm, lay.lv[l + 1].iperm, {(int32_t)h.A[l].m}, false, hl.st);

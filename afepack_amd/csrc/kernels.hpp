@@ -589,16 +589,34 @@ __device__ inline T sell_dot_cc(const SliceInfo &s, int lane, const uint16_t *__
 
 __device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE); }
 
+// DPP lane moves inside rows of 16 lanes (VALU, no LDS round trip):
+// quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror, row_mirror
+constexpr int DPP_QUAD_XOR1 = 0xB1, DPP_QUAD_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140;
+template <int CTRL> __device__ inline float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL> __device__ inline double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 // A row of a slice with 2^gs lanes per row: lane = row_in_slice * 2^gs + k,
 // and sell_dot of lane gives the partial sum of the row's entries k, k + 2^gs,
-// ... in stored order.  The row's sum adds the partials by an xor butterfly
-// over the group's lanes (distance 1, 2, 4, ...); a + b and b + a are the
+// ... in stored order.  The row's sum adds the partials pairwise over lane
+// distance 1, 2, 4, 8 (an xor butterfly): lane i adds the value of lane
+// i ^ 1, then of i ^ 2 (quad permutations); at distance 4 and 8 every quad
+// (then every half row) already holds one value, so the mirror moves read
+// the same bits as lane i ^ 4 (i ^ 8) would hold.  a + b and b + a are the
 // same bits, so every lane of the group ends with the same sum.  gs = 0 is
 // the one-lane-per-row sum unchanged.  The groups of a wavefront are whole
-// (a slice's rows are aligned groups), so every lane read by a shuffle is
-// active.
+// (a slice's rows are aligned groups), so every lane read is active.
 template <class T> __device__ inline T row_group_sum(T v, int gs) {
-  for (int m = 1; m < (1 << gs); m <<= 1) v += __shfl_xor(v, m, WAVE);
+  if (gs >= 1) v += dpp_move<DPP_QUAD_XOR1>(v);
+  if (gs >= 2) v += dpp_move<DPP_QUAD_XOR2>(v);
+  if (gs >= 3) v += dpp_move<DPP_HALF_MIRROR>(v);
+  if (gs >= 4) v += dpp_move<DPP_MIRROR>(v);
   return v;
 }
 

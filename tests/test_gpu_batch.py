@@ -19,10 +19,12 @@ PROBLEMS = [
     (3, 5, 0.05, 2, "poisson"),
     (2, 20, 0.0, 2, "nodal"),
     (1, 8, 0.0, 0, "poisson"),
+    (3, 6, 0.05, 2, "nodal", "16"),  # 16 lanes per row on its coarse levels
 ]
 
 
-def _problem(ctx, kind, n, jitter, op, rhs, seed):
+def _problem(ctx, kind, n, jitter, op, rhs, lanes=None, seed=0):
+    import os
     import afepack_amd as afe
     c, ev, vm = make_mesh(kind, n, jitter, seed)
     sp = afe.Space(ctx, kind, c, ev, vm)
@@ -35,7 +37,14 @@ def _problem(ctx, kind, n, jitter, op, rhs, seed):
     else:
         b = sp.rhs(4, afe.F_POISSON)
     A.apply_dirichlet(b)
-    g = afe.Amg(A, afe.AmgParams(lazy=1, smooth=1, build_fp32=1))
+    if lanes:
+        os.environ["AFE_ROW_LANES"] = lanes
+    try:
+        g = afe.Amg(A, afe.AmgParams(lazy=1, smooth=1, build_fp32=1))
+    finally:
+        os.environ.pop("AFE_ROW_LANES", None)
+    if lanes:
+        assert [d["row_lanes"] for d in g.device_info()][1:] == [int(lanes)] * (g.info()["levels"] - 1)
     return {"sp": sp, "A": A, "b": b, "g": g, "field": field}
 
 

@@ -244,23 +244,34 @@ def _model_vcycle(As, Ps, colours, nu):
     return lambda b: vc(0, b)
 
 
-@pytest.mark.parametrize("kind,n,jitter,acc,nmpo,nmpsd", [
-    (1, 32, 0.0, 4, 50, 0.382),
-    (1, 32, 0.0, 4, 400, 0.0),
-    (3, 8, 0.05, 2, 50, 0.382),   # coarse levels with several lanes per row
+@pytest.mark.parametrize("kind,n,jitter,acc,nmpo,nmpsd,lanes", [
+    (1, 32, 0.0, 4, 50, 0.382, None),
+    (1, 32, 0.0, 4, 400, 0.0, None),
+    (3, 8, 0.05, 2, 50, 0.382, None),  # coarse levels with several lanes per row
+    (3, 8, 0.05, 2, 50, 0.382, "8"),   # 8 and 16 lanes on every coarse level
+    (3, 8, 0.05, 2, 50, 0.382, "16"),
 ])
 @pytest.mark.parametrize("tail", [4096, 0])
-def test_vcycle_matches_dense_model(ctx, kind, n, jitter, acc, nmpo, nmpsd, tail):
+def test_vcycle_matches_dense_model(ctx, kind, n, jitter, acc, nmpo, nmpsd, lanes, tail):
     """One symmetric FP64 V(1,1)-cycle from zero on the device against the
     dense numpy model of the same cycle, with the small levels in the
     single-workgroup tail (tail 4096) and as per-level launches (tail 0)."""
+    import os
     import afepack_amd as afe
-    print(f"case vcycle model kind={kind} n={n} nmpo={nmpo} tail={tail}", flush=True)
+    print(f"case vcycle model kind={kind} n={n} nmpo={nmpo} lanes={lanes} tail={tail}", flush=True)
     sp, osp, A, OA, f, of = _system(ctx, kind, n, jitter, acc)
-    g = afe.Amg(A, afe.AmgParams(lazy=1, nmpo=nmpo, nmpsd=nmpsd, smooth=1))
+    if lanes:
+        os.environ["AFE_ROW_LANES"] = lanes
+    try:
+        g = afe.Amg(A, afe.AmgParams(lazy=1, nmpo=nmpo, nmpsd=nmpsd, smooth=1))
+    finally:
+        os.environ.pop("AFE_ROW_LANES", None)
     g.set_tail_rows(tail)
-    if kind == 3:
-        assert max(d["row_lanes"] for d in g.device_info()) > 1
+    rl = [d["row_lanes"] for d in g.device_info()]
+    if lanes:
+        assert rl[0] == 1 and set(rl[1:]) == {int(lanes)}
+    elif kind == 3:
+        assert max(rl) > 1
     z = _zeros(ctx, sp.n_dof)
     g.vcycle(z, f, afe.PREC_FP64)
     As, Ps, Gs = _dense_levels(g)
