@@ -346,8 +346,8 @@ template <class T> static void batch_issue_step(afe_batch *b, hipStream_t s) {
     const BLaunch &bp = b->down_restrict[l];
     if (bp.grid) kb_restrict<T><<<bp.grid, BLK, 0, s>>>(S + bp.seg_off, bp.nseg, LV + bp.lv_off, LV + bp.lvn_off);
   }
-  const int lds_rows = (int)(b->ctx->lds_bytes / (3 * sizeof(T)));
-  kb_tail<T><<<K, TAIL_THREADS, (size_t)lds_rows * 3 * sizeof(T), s>>>(tj_buf<T>(b).p, b->nu, lds_rows);
+  const int lds_cap = (int)b->ctx->lds_bytes;
+  kb_tail<T><<<K, TAIL_THREADS, (size_t)lds_cap, s>>>(tj_buf<T>(b).p, b->nu, lds_cap);
   for (int l = L - 1; l >= 0; l--) {
     const BLaunch &bt = b->up_prolong[l];
     if (bt.grid) kb_prolong<T><<<bt.grid, BLK, 0, s>>>(S + bt.seg_off, bt.nseg, LV + bt.lv_off, LV + bt.lvn_off);

@@ -129,9 +129,9 @@ template <class T> struct TailJob {
 
 template <class T>
 __global__ __launch_bounds__(TAIL_THREADS) void kb_tail(const TailJob<T> *__restrict__ jobs, int nu,
-                                                        int lds_rows) {
+                                                        int lds_cap) {
   const TailJob<T> j = jobs[blockIdx.x];
-  tail_body<T>(j.L, j.l0, j.lc, j.minv, j.coarse_mode, nu, nu, 1, 7, lds_rows);
+  tail_body<T>(j.L, j.l0, j.lc, j.minv, j.coarse_mode, nu, nu, 1, 7, lds_cap);
 }
 
 // ---- CG kernels (natural numbering) ---------------------------------------

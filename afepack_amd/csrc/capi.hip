@@ -1695,20 +1695,20 @@ static void issue_vcycle(afe_amg *g, int nu, bool symmetric, hipStream_t st, boo
                                                           g_tune.xcd);
     }
   }
-  const int lds_rows = (int)(g->ctx->lds_bytes / (3 * sizeof(T)));
-  const size_t lds = (size_t)lds_rows * 3 * sizeof(T);
+  const int lds_cap = (int)g->ctx->lds_bytes; // dynamic LDS of the tail workgroup
+  const size_t lds = (size_t)lds_cap;
   if (g->big_coarse) {
     DevLevel &dc = g->lv[Lc];
     if (g->tail0 < Lc)
       k_tail<T><<<1, TAIL_THREADS, lds, st>>>(Prec<T>::ld(g), g->tail0, Lc, g->minv.p, g->coarse_mode, nu, nu,
-                                               symmetric ? 1 : 0, 1, lds_rows);
+                                               symmetric ? 1 : 0, 1, lds_cap);
     k_coarse_gemv<T><<<nblk(dc.n, WPB), BLK, 0, st>>>((int)dc.n, g->minv.p, Prec<T>::b(dc), Prec<T>::x(dc));
     if (g->tail0 < Lc)
       k_tail<T><<<1, TAIL_THREADS, lds, st>>>(Prec<T>::ld(g), g->tail0, Lc, g->minv.p, g->coarse_mode, nu, nu,
-                                               symmetric ? 1 : 0, 4, lds_rows);
+                                               symmetric ? 1 : 0, 4, lds_cap);
   } else {
     k_tail<T><<<1, TAIL_THREADS, lds, st>>>(Prec<T>::ld(g), g->tail0, Lc, g->minv.p, g->coarse_mode, nu, nu,
-                                             symmetric ? 1 : 0, 7, lds_rows);
+                                             symmetric ? 1 : 0, 7, lds_cap);
   }
   for (int l = g->tail0 - 1; l >= 0; l--) {
     DevLevel &d = g->lv[l];

@@ -759,138 +759,291 @@ template <class T> struct LevelDev {
   T *x, *b, *t;
 };
 
-template <class T>
-__device__ inline void tail_sweep(const LevelDev<T> &d, const T *b, T *x, int c, int w, int nw, int lane) {
+// The tail's operands carry their address space in the type, so that the
+// compiler issues LDS (ds_*) and global (global_*) instructions rather than
+// flat ones: global loads count only in the vector-memory counter, so loads
+// issued before a barrier stay in flight across it (the barrier waits for
+// the LDS counter), and LDS accesses take the short LDS path.
+template <class T> using GPtr = __attribute__((address_space(1))) T *;
+template <class T> using LPtr = __attribute__((address_space(3))) T *;
+template <class T> __device__ __forceinline__ GPtr<T> gptr(T *p) { return (GPtr<T>)p; }
+template <class T> __device__ __forceinline__ GPtr<const T> gptr(const T *p) { return (GPtr<const T>)p; }
+
+// a slice descriptor read through an address-space-qualified pointer
+// (field by field: the implicit copy works on generic references only)
+template <class SP> __device__ __forceinline__ SliceInfo load_si(SP p, int s) {
+  SliceInfo q;
+  q.row0 = p[s].row0;
+  q.nrow = p[s].nrow;
+  q.width = p[s].width;
+  q.off = p[s].off;
+  return q;
+}
+
+// One wavefront's share of a colour step: the slice's first chunk of column
+// and value loads, 1/diag and b of its row, issued one step ahead (none of
+// them depends on x).
+template <class T> constexpr int TailChunk = sizeof(T) == 8 ? 4 : 8; // keeps the tail within 128 VGPRs
+template <class T> struct TailRow {
+  SliceInfo q;
+  int32_t cc[TailChunk<T>];
+  T vv[TailChunk<T>];
+  T br, ir;
+  bool on; // this lane has a row
+};
+
+template <class T, class SP, class VP>
+__device__ __forceinline__ void tail_issue(TailRow<T> &p, const LevelDev<T> &d, SP si, VP b, int s, int lane) {
+  p.q = load_si(si, s);
   const int gs = d.gs;
-  for (int s = d.col_slice[c] + w; s < d.col_slice[c + 1]; s += nw) {
-    const SliceInfo q = d.si[s];
+  p.on = (lane >> gs) < p.q.nrow;
+  if (!p.on) return;
+  const int r = p.q.row0 + (lane >> gs);
+  p.br = b[r];
+  p.ir = gptr(d.invd)[r];
+  const GPtr<const int32_t> c = gptr(d.cols) + p.q.off + lane;
+  const GPtr<const T> v = gptr(d.vals) + p.q.off + lane;
+#pragma unroll
+  for (int k = 0; k < TailChunk<T>; k++)
+    if (k < p.q.width) {
+      p.cc[k] = c[k * WAVE];
+      p.vv[k] = v[k * WAVE];
+    }
+}
+
+// the rest of sell_dot's row product after the first chunk (same loads,
+// same summation order: chunk by chunk, entry by entry)
+template <class T, class XP>
+__device__ __forceinline__ T tail_dot(const SliceInfo &q, int lane, GPtr<const int32_t> cols, GPtr<const T> vals,
+                                      XP x, int j0, T acc) {
+  const GPtr<const int32_t> c = cols + q.off + lane;
+  const GPtr<const T> v = vals + q.off + lane;
+  constexpr int CH = TailChunk<T>;
+  for (int j = j0; j < q.width; j += CH) {
+    int32_t cc[CH];
+    T vv[CH], xx[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++)
+      if (j + k < q.width) {
+        cc[k] = c[(j + k) * WAVE];
+        vv[k] = v[(j + k) * WAVE];
+      }
+#pragma unroll
+    for (int k = 0; k < CH; k++)
+      if (j + k < q.width) xx[k] = x[cc[k]];
+#pragma unroll
+    for (int k = 0; k < CH; k++)
+      if (j + k < q.width) acc += vv[k] * xx[k];
+  }
+  return acc;
+}
+
+// x_r = (b_r - sum_{j != r} a_rj x_j) / a_rr for the row of an issued slice
+template <class T, class XP>
+__device__ __forceinline__ void tail_finish(const TailRow<T> &p, const LevelDev<T> &d, XP x, int lane) {
+  if (!p.on) return;
+  const int gs = d.gs;
+  T acc = 0;
+  {
+    T xx[TailChunk<T>];
+#pragma unroll
+    for (int k = 0; k < TailChunk<T>; k++)
+      if (k < p.q.width) xx[k] = x[p.cc[k]];
+#pragma unroll
+    for (int k = 0; k < TailChunk<T>; k++)
+      if (k < p.q.width) acc += p.vv[k] * xx[k];
+  }
+  acc = tail_dot<T>(p.q, lane, gptr(d.cols), gptr(d.vals), x, TailChunk<T>, acc);
+  acc = row_group_sum(acc, gs);
+  if ((lane & ((1 << gs) - 1)) == 0) x[p.q.row0 + (lane >> gs)] = (p.br - acc) * p.ir;
+}
+
+// nu sweeps over the colours of a level (forward, or reverse when rev),
+// wavefront w taking slices w, w + nw, ... of each colour.  The loads of a
+// wavefront's first slice of the next colour are issued before the barrier
+// that ends the current colour.
+template <class T, class SP, class CP, class VP>
+__device__ __forceinline__ void tail_sweeps(const LevelDev<T> &d, SP si, CP cs, VP b, VP x, int nu, bool rev,
+                                            int w, int nw, int lane) {
+  const int nc = d.ncol, steps = nu * nc;
+  auto colour = [&](int i) {
+    const int c = i % nc;
+    return rev ? nc - 1 - c : c;
+  };
+  TailRow<T> p;
+  p.on = false;
+  if (steps > 0 && cs[colour(0)] + w < cs[colour(0) + 1]) tail_issue(p, d, si, b, cs[colour(0)] + w, lane);
+  for (int i = 0; i < steps; i++) {
+    const int c = colour(i), s1 = cs[c + 1];
+    if (cs[c] + w < s1) tail_finish(p, d, x, lane);
+    for (int s = cs[c] + w + nw; s < s1; s += nw) {
+      TailRow<T> e;
+      tail_issue(e, d, si, b, s, lane);
+      tail_finish(e, d, x, lane);
+    }
+    p.on = false;
+    if (i + 1 < steps) {
+      const int cn = colour(i + 1);
+      if (cs[cn] + w < cs[cn + 1]) tail_issue(p, d, si, b, cs[cn] + w, lane);
+    }
+    __syncthreads();
+  }
+}
+
+// y_r = sum_j m_rj v_j over the slices of a matrix (restriction and
+// prolongation): one lane group per row, y written by the group's leader
+// as f(r, sum)
+template <class T, class SP, class XP, class F>
+__device__ __forceinline__ void tail_rows(SP si, int ns, const int32_t *cols, const T *vals, int gs, XP v, int w,
+                                          int nw, int lane, F f) {
+  for (int s = w; s < ns; s += nw) {
+    const SliceInfo q = load_si(si, s);
     if ((lane >> gs) < q.nrow) {
-      const int r = q.row0 + (lane >> gs);
-      const T br = b[r], ir = d.invd[r];
-      const T acc = row_group_sum(sell_dot<T>(q, lane, d.cols, d.vals, x), gs);
-      if ((lane & ((1 << gs) - 1)) == 0) x[r] = (br - acc) * ir;
+      const T acc = row_group_sum(tail_dot<T>(q, lane, gptr(cols), gptr(vals), v, 0, T(0)), gs);
+      if ((lane & ((1 << gs) - 1)) == 0) f(q.row0 + (lane >> gs), acc);
     }
   }
+}
+
+// LDS layout of one level while the tail works on it: [x | b | t] when the
+// vectors fit (n T each), then the slice table and the colour bounds when
+// they fit behind them; otherwise those stay in global memory.
+struct TailStage {
+  bool vec, tab;
+  int tab_off; // bytes
+};
+template <class T> __device__ __forceinline__ TailStage tail_stage_plan(const LevelDev<T> &d, int lds_cap) {
+  TailStage t;
+  const int64_t vb = 3 * (int64_t)d.n * sizeof(T);
+  t.vec = vb <= lds_cap;
+  t.tab_off = t.vec ? (int)((vb + 15) & ~int64_t(15)) : 0;
+  t.tab = t.tab_off + (int64_t)d.ns * sizeof(SliceInfo) + (int64_t)(d.ncol + 1) * 4 <= lds_cap;
+  return t;
 }
 
 // Single-workgroup V-cycle over the small levels l0..lc.  phase bits:
 // 1 down leg (pre-smoothing, residual, restriction for l0..lc-1),
 // 2 coarse solve on level lc (dense FP64 inverse, diagonal or GS),
 // 4 up leg (prolongation, post-smoothing for lc-1..l0).
-// A level with n <= lds_rows keeps x, b and the residual in LDS while the
-// workgroup works on it (each colour step then waits on LDS, not on L2/HBM);
-// larger levels use the global vectors.  x of a level is written back to
-// global memory at the end of each leg.
+// lds_cap bytes of dynamic LDS hold the working level's vectors and tables
+// (tail_stage_plan); x of a level is written back to global memory at the
+// end of each leg.
+template <class T, class SP, class CP, class VP>
+__device__ __forceinline__ void tail_down(const LevelDev<T> &d, const LevelDev<T> &dn, SP si, CP cs, VP x, VP b,
+                                          VP t, int nu, int w, int nw, int lane) {
+  tail_sweeps<T>(d, si, cs, b, x, nu, false, w, nw, lane);
+  // t = b - A x
+  for (int s = w; s < d.ns; s += nw) {
+    const SliceInfo q = load_si(si, s);
+    const int gs = d.gs;
+    if ((lane >> gs) < q.nrow) {
+      const int r = q.row0 + (lane >> gs);
+      const T br = b[r], dr = gptr(d.diag)[r], xr = x[r];
+      const T acc = row_group_sum(tail_dot<T>(q, lane, gptr(d.cols), gptr(d.vals), x, 0, T(0)), gs);
+      if ((lane & ((1 << gs) - 1)) == 0) t[r] = br - dr * xr - acc;
+    }
+  }
+  __syncthreads();
+  // b_{l+1} = P t, x_{l+1} = 0
+  const GPtr<T> nb = gptr(dn.b), nx = gptr(dn.x);
+  tail_rows<T>(gptr(d.p_si), d.p_ns, d.p_cols, d.p_vals, d.p_gs, t, w, nw, lane, [&](int r, T v) {
+    nb[r] = v;
+    nx[r] = 0;
+  });
+}
+
+template <class T, class SP, class CP, class VP>
+__device__ __forceinline__ void tail_up(const LevelDev<T> &d, const LevelDev<T> &dn, SP si, CP cs, VP x, VP b,
+                                        int nu, bool rev, int w, int nw, int lane) {
+  // x += P^T x_{l+1}
+  const GPtr<const T> nx = gptr((const T *)dn.x);
+  tail_rows<T>(gptr(d.t_si), d.t_ns, d.t_cols, d.t_vals, 0, nx, w, nw, lane, [&](int r, T v) { x[r] = x[r] + v; });
+  __syncthreads();
+  tail_sweeps<T>(d, si, cs, b, x, nu, rev, w, nw, lane);
+}
+
+// runs f(si, cs, x, b, t) with the level's tables and vectors in LDS or in
+// global memory as planned (staging copies made and synchronised first)
+template <class T, class F>
+__device__ __forceinline__ void tail_with_level(const LevelDev<T> &d, char *smem, int lds_cap, F f) {
+  const TailStage pl = tail_stage_plan(d, lds_cap);
+  const int tid = threadIdx.x;
+  LPtr<T> sx = (LPtr<T>)(T *)smem;
+  LPtr<T> sb = sx + d.n, st = sb + d.n;
+  LPtr<SliceInfo> ssi = (LPtr<SliceInfo>)(SliceInfo *)(smem + pl.tab_off);
+  LPtr<int32_t> scs = (LPtr<int32_t>)(int32_t *)(smem + pl.tab_off + (int64_t)d.ns * sizeof(SliceInfo));
+  if (pl.vec)
+    for (int i = tid; i < d.n; i += blockDim.x) {
+      sx[i] = gptr(d.x)[i];
+      sb[i] = gptr(d.b)[i];
+    }
+  if (pl.tab) {
+    for (int i = tid; i < d.ns; i += blockDim.x) {
+      const SliceInfo q = d.si[i];
+      ssi[i].row0 = q.row0;
+      ssi[i].nrow = q.nrow;
+      ssi[i].width = q.width;
+      ssi[i].off = q.off;
+    }
+    for (int i = tid; i <= d.ncol; i += blockDim.x) scs[i] = gptr(d.col_slice)[i];
+  }
+  __syncthreads();
+  if (pl.vec) {
+    if (pl.tab)
+      f(ssi, scs, sx, sb, st);
+    else
+      f(gptr(d.si), gptr(d.col_slice), sx, sb, st);
+    for (int i = tid; i < d.n; i += blockDim.x) gptr(d.x)[i] = sx[i];
+  } else if (pl.tab) {
+    f(ssi, scs, gptr(d.x), gptr(d.b), gptr(d.t));
+  } else {
+    f(gptr(d.si), gptr(d.col_slice), gptr(d.x), gptr(d.b), gptr(d.t));
+  }
+  __syncthreads();
+}
+
 template <class T>
-__device__ inline void tail_body(const LevelDev<T> *__restrict__ L, int l0, int lc,
-                                 const double *__restrict__ minv, int coarse_mode, int nu_pre, int nu_post,
-                                 int post_reverse, int phase, int lds_rows) {
-  extern __shared__ char tail_smem[];
-  T *sx = reinterpret_cast<T *>(tail_smem);
-  T *sb = sx + lds_rows;
-  T *st = sb + lds_rows;
+__device__ __forceinline__ void tail_body(const LevelDev<T> *__restrict__ L, int l0, int lc,
+                                          const double *__restrict__ minv, int coarse_mode, int nu_pre,
+                                          int nu_post, int post_reverse, int phase, int lds_cap) {
+  extern __shared__ __align__(16) char tail_smem[];
   const int nw = blockDim.x / WAVE;
   const int w = wave_id();
   const int lane = threadIdx.x % WAVE;
   const int tid = threadIdx.x;
   if (phase & 1) {
     for (int l = l0; l < lc; l++) {
-      const LevelDev<T> d = L[l];
-      const bool lds = d.n <= lds_rows;
-      T *x = lds ? sx : d.x;
-      T *b = lds ? sb : d.b;
-      T *t = lds ? st : d.t;
-      if (lds) {
-        for (int i = tid; i < d.n; i += blockDim.x) {
-          sx[i] = d.x[i];
-          sb[i] = d.b[i];
-        }
-        __syncthreads();
-      }
-      for (int it = 0; it < nu_pre; it++)
-        for (int c = 0; c < d.ncol; c++) {
-          tail_sweep(d, b, x, c, w, nw, lane);
-          __syncthreads();
-        }
-      for (int s = w; s < d.ns; s += nw) {
-        const SliceInfo q = d.si[s];
-        if ((lane >> d.gs) < q.nrow) {
-          const int r = q.row0 + (lane >> d.gs);
-          const T br = b[r], dr = d.diag[r], xr = x[r];
-          const T acc = row_group_sum(sell_dot<T>(q, lane, d.cols, d.vals, x), d.gs);
-          if ((lane & ((1 << d.gs) - 1)) == 0) t[r] = br - dr * xr - acc;
-        }
-      }
-      __syncthreads();
-      const LevelDev<T> dn = L[l + 1];
-      for (int s = w; s < d.p_ns; s += nw) {
-        const SliceInfo q = d.p_si[s];
-        if ((lane >> d.p_gs) < q.nrow) {
-          const int r = q.row0 + (lane >> d.p_gs);
-          const T acc = row_group_sum(sell_dot<T>(q, lane, d.p_cols, d.p_vals, t), d.p_gs);
-          if ((lane & ((1 << d.p_gs) - 1)) == 0) {
-            dn.b[r] = acc;
-            dn.x[r] = 0;
-          }
-        }
-      }
-      if (lds)
-        for (int i = tid; i < d.n; i += blockDim.x) d.x[i] = sx[i];
-      __syncthreads();
+      const LevelDev<T> &d = L[l];
+      const LevelDev<T> &dn = L[l + 1];
+      tail_with_level<T>(d, tail_smem, lds_cap, [&](auto si, auto cs, auto x, auto b, auto t) {
+        tail_down<T>(d, dn, si, cs, x, b, t, nu_pre, w, nw, lane);
+      });
     }
   }
   if (phase & 2) {
-    const LevelDev<T> d = L[lc];
+    const LevelDev<T> &d = L[lc];
     if (coarse_mode == 0) {
+      const GPtr<const T> b = gptr((const T *)d.b);
+      const GPtr<T> x = gptr(d.x);
       for (int i = tid; i < d.n; i += blockDim.x) {
         double s = 0.0;
-        for (int j = 0; j < d.n; j++) s += minv[(int64_t)i * d.n + j] * (double)d.b[j];
-        d.x[i] = (T)s;
+        for (int j = 0; j < d.n; j++) s += gptr(minv)[(int64_t)i * d.n + j] * (double)b[j];
+        x[i] = (T)s;
       }
     } else if (coarse_mode == 1) {
-      for (int i = tid; i < d.n; i += blockDim.x) d.x[i] = d.b[i] * d.invd[i];
+      for (int i = tid; i < d.n; i += blockDim.x) gptr(d.x)[i] = gptr(d.b)[i] * gptr(d.invd)[i];
     } else {
-      for (int it = 0; it < nu_pre; it++)
-        for (int c = 0; c < d.ncol; c++) {
-          tail_sweep(d, d.b, d.x, c, w, nw, lane);
-          __syncthreads();
-        }
+      tail_sweeps<T>(d, gptr(d.si), gptr(d.col_slice), gptr(d.b), gptr(d.x), nu_pre, false, w, nw, lane);
     }
     __syncthreads();
   }
   if (phase & 4) {
     for (int l = lc - 1; l >= l0; l--) {
-      const LevelDev<T> d = L[l];
-      const LevelDev<T> dn = L[l + 1];
-      const bool lds = d.n <= lds_rows;
-      T *x = lds ? sx : d.x;
-      const T *b = lds ? sb : d.b;
-      if (lds) {
-        for (int i = tid; i < d.n; i += blockDim.x) {
-          sx[i] = d.x[i];
-          sb[i] = d.b[i];
-        }
-        __syncthreads();
-      }
-      for (int s = w; s < d.t_ns; s += nw) {
-        const SliceInfo q = d.t_si[s];
-        if (lane < q.nrow) {
-          const int r = q.row0 + lane;
-          const T xr = x[r];
-          x[r] = xr + sell_dot<T>(q, lane, d.t_cols, d.t_vals, dn.x);
-        }
-      }
-      __syncthreads();
-      for (int it = 0; it < nu_post; it++)
-        for (int cc = 0; cc < d.ncol; cc++) {
-          const int c = post_reverse ? d.ncol - 1 - cc : cc;
-          tail_sweep(d, b, x, c, w, nw, lane);
-          __syncthreads();
-        }
-      if (lds) {
-        for (int i = tid; i < d.n; i += blockDim.x) d.x[i] = sx[i];
-        __syncthreads();
-      }
+      const LevelDev<T> &d = L[l];
+      const LevelDev<T> &dn = L[l + 1];
+      tail_with_level<T>(d, tail_smem, lds_cap, [&](auto si, auto cs, auto x, auto b, auto) {
+        tail_up<T>(d, dn, si, cs, x, b, nu_post, post_reverse != 0, w, nw, lane);
+      });
     }
   }
 }
@@ -899,8 +1052,8 @@ template <class T>
 __global__ __launch_bounds__(TAIL_THREADS) void k_tail(const LevelDev<T> *__restrict__ L, int l0, int lc,
                                                        const double *__restrict__ minv, int coarse_mode,
                                                        int nu_pre, int nu_post, int post_reverse, int phase,
-                                                       int lds_rows) {
-  tail_body<T>(L, l0, lc, minv, coarse_mode, nu_pre, nu_post, post_reverse, phase, lds_rows);
+                                                       int lds_cap) {
+  tail_body<T>(L, l0, lc, minv, coarse_mode, nu_pre, nu_post, post_reverse, phase, lds_cap);
 }
 
 // dense coarse solve x = Minv b with the FP64 inverse (DenseMatrix::vmult of

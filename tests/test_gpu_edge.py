@@ -184,3 +184,41 @@ def test_row_lanes_agree_with_one_lane_per_row(ctx, tail):
     assert s32.converged == t32.converged == 1 and abs(s32.iterations - t32.iterations) <= 1
     assert np.linalg.norm(x64 - y64) <= 1e-9 * np.linalg.norm(y64)
     assert np.linalg.norm(x32 - y32) <= 1e-9 * np.linalg.norm(y32)
+
+
+@pytest.mark.parametrize("kind,n", [(1, 128), (3, 24)])
+def test_tail_with_levels_beyond_lds(ctx, kind, n):
+    """The single-workgroup tail keeps a level's vectors (and slice tables)
+    in LDS when they fit and works on global memory otherwise: with every
+    level in the tail (level 0 of 16641 / 15625 rows does not fit), the
+    V-cycle agrees with per-level launches to rounding and the solves agree
+    to the tolerance, in FP32 and FP64."""
+    import afepack_amd as afe
+    print(f"case big tail kind={kind} n={n}", flush=True)
+    c, ev, vm = make_mesh(kind, n, 0.05 if kind == 3 else 0.2, 9)
+    sp = afe.Space(ctx, kind, c, ev, vm)
+    A = afe.Matrix(sp)
+    acc = 2 if kind == 3 else 4
+    A.assemble(afe.OP_STIFF, acc)
+    b = sp.rhs(acc, afe.F_POISSON)
+    A.apply_dirichlet(b)
+    g = afe.Amg(A, afe.AmgParams(lazy=1, smooth=1, build_fp32=1))
+    out = {}
+    for tail in (1 << 22, 0):
+        g.set_tail_rows(tail)
+        assert g.device_info()[0]["tail"] == (0 if tail else g.info()["levels"] - 1)
+        z = afe.DeviceVector(ctx, sp.n_dof)
+        g.vcycle(z, b, afe.PREC_FP64)
+        res = [z.numpy()]
+        for prec in (afe.PREC_FP64, afe.PREC_FP32):
+            x = afe.DeviceVector(ctx, sp.n_dof)
+            st = g.pcg(x, b, rtol=1e-11, max_it=300, prec=prec)
+            assert st.converged == 1
+            res += [x.numpy(), st.iterations]
+        out[tail] = res
+    big, lau = out[1 << 22], out[0]
+    assert np.linalg.norm(big[0] - lau[0]) <= 1e-12 * np.linalg.norm(lau[0])
+    for k in (1, 3):
+        assert abs(big[k + 1] - lau[k + 1]) <= 1
+        assert np.linalg.norm(big[k] - lau[k]) <= 1e-9 * np.linalg.norm(lau[k])
+    g.close()
