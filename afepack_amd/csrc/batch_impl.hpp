@@ -19,6 +19,11 @@ struct BatchMember {
   double rtol = 0, bb = 0, thr = 0, pthr = 0, rr = 0, l1 = 0, init_l1 = 0;
   int max_it = 0, it = 0, restarts = 0, outer = 0;
   bool active = false, done = false, conv = false;
+  // the next step starts the member's CG from its residual r (p = z = 0,
+  // rz = 0: alpha and beta are 0, so the step sets z = M r and p = z without
+  // moving x); it is not counted as an iteration.  thr is set from the first
+  // step's gathered (b, b) when it is still negative.
+  bool fresh = false;
   int status = AFE_OK;
   std::chrono::steady_clock::time_point t_add, t_done;
 };
@@ -59,6 +64,11 @@ struct afe_batch {
   hipGraphExec_t graph = nullptr;
   int nu = 1;
   std::recursive_mutex mu;
+  // host time of the epochs (tables, capture and instantiation) and the
+  // steps, printed at destruction with AFE_VERBOSE
+  int n_epochs = 0;
+  int64_t n_steps = 0;
+  double t_epoch = 0, t_graph = 0, t_step = 0;
   ~afe_batch() {
     if (graph) (void)hipGraphExecDestroy(graph);
     if (h_gath) (void)hipHostFree(h_gath);
@@ -69,11 +79,16 @@ static double seconds_between(std::chrono::steady_clock::time_point a, std::chro
   return std::chrono::duration<double>(b - a).count();
 }
 
-template <class T> static void batch_restart(BatchMember &m) {
+// restart the member's CG from the residual in its r: the next batched step
+// preconditions it (see BatchMember::fresh)
+static void batch_restart(BatchMember &m) {
   afe_amg *g = m.g;
   const int64_t n = g->lv[0].n;
-  k_fill<double><<<nblk(n), 256, 0, g->ctx->stream>>>(n, g->wp.p, 0.0);
-  precondition_perm<T>(g, g->wr.p, true);
+  hipStream_t s = g->ctx->stream;
+  k_fill<double><<<nblk(n), 256, 0, s>>>(n, g->wp.p, 0.0);
+  k_fill<double><<<nblk(n), 256, 0, s>>>(n, g->wz.p, 0.0);
+  k_fill<double><<<1, 64, 0, s>>>(1, &g->sc.p->rz, 0.0);
+  m.fresh = true;
 }
 
 // the member's pair reached its target (or the iteration limit): check the
@@ -87,7 +102,7 @@ template <class T> static bool batch_check(afe_batch *b, BatchMember &m) {
   m.rr = g->h_sc->rr;
   bool pair_conv = m.rr <= m.pthr;
   if (!pair_conv && ++m.restarts <= 20 && m.it < m.max_it) {
-    batch_restart<T>(m);
+    batch_restart(m);
     return false;
   }
   for (;;) {
@@ -102,7 +117,7 @@ template <class T> static bool batch_check(afe_batch *b, BatchMember &m) {
     m.outer++;
     m.pthr *= 0.25;
     if (m.rr > m.pthr) {
-      batch_restart<T>(m);
+      batch_restart(m);
       return false;
     }
   }
@@ -370,12 +385,19 @@ template <class T> static void batch_issue_step(afe_batch *b, hipStream_t s) {
 template <class T> static void batch_step_impl(afe_batch *b, int *n_active) {
   afe_ctx *ctx = b->ctx;
   hipStream_t s = ctx->stream;
-  if (b->dirty) batch_build_epoch<T>(b);
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
+  if (b->dirty) {
+    batch_build_epoch<T>(b);
+    b->n_epochs++;
+  }
   const int K = (int)b->act.size();
   if (K == 0) {
     *n_active = 0;
     return;
   }
+  const auto t1 = clk::now();
+  b->t_epoch += std::chrono::duration<double>(t1 - t0).count();
   if (use_graphs()) {
     if (!b->graph) {
       hipGraph_t graph;
@@ -384,6 +406,7 @@ template <class T> static void batch_step_impl(afe_batch *b, int *n_active) {
       HIPCHK(hipStreamEndCapture(s, &graph));
       HIPCHK(hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0));
       HIPCHK(hipGraphDestroy(graph));
+      b->t_graph += std::chrono::duration<double>(clk::now() - t1).count();
     }
     HIPCHK(hipGraphLaunch(b->graph, s));
   } else {
@@ -391,10 +414,22 @@ template <class T> static void batch_step_impl(afe_batch *b, int *n_active) {
   }
   HIPCHK(hipMemcpyAsync(b->h_gath, b->gath.p, sizeof(CgScalars) * K, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  b->n_steps++;
+  b->t_step += std::chrono::duration<double>(clk::now() - t1).count();
   const std::vector<int> act = b->act;
   for (int k = 0; k < K; k++) {
     BatchMember &m = *b->slots[act[k]];
-    m.it++;
+    if (m.fresh) {
+      m.fresh = false;
+      if (m.thr < 0) {
+        m.bb = b->h_gath[k].bb;
+        m.init_l1 = m.l1 = b->h_gath[k].l1;
+        m.thr = m.rtol * m.rtol * m.bb;
+        m.pthr = 0.25 * m.thr;
+      }
+    } else {
+      m.it++;
+    }
     m.rr = b->h_gath[k].rr;
     if (!(m.rr == m.rr)) {
       m.active = false;
@@ -426,6 +461,9 @@ extern "C" int afe_batch_create(afe_ctx *ctx, int32_t prec, int32_t smooth, afe_
 
 extern "C" int afe_batch_destroy(afe_batch *b) {
   if (!b) return AFE_OK;
+  if (verbose())
+    fprintf(stderr, "batch: %lld steps, %d epochs; host time: epochs %.1f ms, graphs %.1f ms, steps %.1f ms\n",
+            (long long)b->n_steps, b->n_epochs, 1e3 * b->t_epoch, 1e3 * b->t_graph, 1e3 * b->t_step);
   {
     std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
     (void)hipSetDevice(b->ctx->device);
@@ -458,13 +496,11 @@ extern "C" int afe_batch_add(afe_batch *b, afe_amg *g, double *d_x, const double
   HIPCHK(hipMemcpyAsync(g->wb.p, d_b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   HIPCHK(hipMemcpyAsync(g->wx.p, d_x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   k_fill<double><<<nblk(n), 256, 0, s>>>(n, g->wxl.p, 0.0);
+  // r = b - A x and (r, r), (b, b), |r|_1 into the member's scalars; the
+  // first batched step preconditions r (no V-cycle of its own here, and the
+  // thresholds come from that step's gathered scalars)
   residual_norms(g, true, g->wb.p, g->wx.p, g->wr.p, g->wxl.p);
-  read_scalars(g);
-  m->bb = g->h_sc->bb;
-  m->rr = g->h_sc->rr;
-  m->init_l1 = m->l1 = g->h_sc->l1;
-  m->thr = rtol * rtol * m->bb;
-  m->pthr = 0.25 * m->thr;
+  m->thr = m->pthr = -1.0;
   m->active = true;
   int idx = -1;
   for (size_t k = 0; k < b->slots.size(); k++)
@@ -479,13 +515,8 @@ extern "C" int afe_batch_add(afe_batch *b, afe_amg *g, double *d_x, const double
   b->slots[idx] = std::move(m);
   BatchMember &mm = *b->slots[idx];
   b->dirty = true;
-  if (mm.rr <= mm.pthr) {
-    if (b->prec == AFE_PREC_FP32) batch_check<float>(b, mm);
-    else batch_check<double>(b, mm);
-  } else {
-    if (b->prec == AFE_PREC_FP32) batch_restart<float>(mm);
-    else batch_restart<double>(mm);
-  }
+  batch_restart(mm);
+  HIPCHK(hipStreamSynchronize(s));
   *slot = idx;
   API_END
 }

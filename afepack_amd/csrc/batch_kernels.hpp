@@ -53,14 +53,14 @@ __global__ __launch_bounds__(BLK) void kb_gs(const Seg *__restrict__ segs, int n
   const int s = g.s0 + (blockIdx.x - g.block0) * WPB + wave_id();
   if (s >= g.s1) return;
   const LevelDev<T> &d = lv[g.member];
-  const SliceInfo q = d.si[s];
+  const SliceInfo q = load_si(gptr(d.si), s);
   const int lane = threadIdx.x % WAVE;
   const int gs = d.gs;
   if ((lane >> gs) >= q.nrow) return;
   const int r = q.row0 + (lane >> gs);
-  const T br = d.b[r], ir = d.invd[r];
-  const T acc = row_group_sum(sell_dot<T>(q, lane, d.cols, d.vals, d.x), gs);
-  if ((lane & ((1 << gs) - 1)) == 0) d.x[r] = (br - acc) * ir;
+  const T br = gptr(d.b)[r], ir = gptr(d.invd)[r];
+  const T acc = row_group_sum(sell_dot<T>(q, lane, gptr(d.cols), gptr(d.vals), gptr((const T *)d.x)), gs);
+  if ((lane & ((1 << gs) - 1)) == 0) gptr(d.x)[r] = (br - acc) * ir;
 }
 
 template <class T>
@@ -70,14 +70,14 @@ __global__ __launch_bounds__(BLK) void kb_residual(const Seg *__restrict__ segs,
   const int s = g.s0 + (blockIdx.x - g.block0) * WPB + wave_id();
   if (s >= g.s1) return;
   const LevelDev<T> &d = lv[g.member];
-  const SliceInfo q = d.si[s];
+  const SliceInfo q = load_si(gptr(d.si), s);
   const int lane = threadIdx.x % WAVE;
   const int gs = d.gs;
   if ((lane >> gs) >= q.nrow) return;
   const int r = q.row0 + (lane >> gs);
-  const T br = d.b[r], dr = d.diag[r], xr = d.x[r];
-  const T acc = row_group_sum(sell_dot<T>(q, lane, d.cols, d.vals, d.x), gs);
-  if ((lane & ((1 << gs) - 1)) == 0) d.t[r] = br - dr * xr - acc;
+  const T br = gptr(d.b)[r], dr = gptr(d.diag)[r], xr = gptr(d.x)[r];
+  const T acc = row_group_sum(sell_dot<T>(q, lane, gptr(d.cols), gptr(d.vals), gptr((const T *)d.x)), gs);
+  if ((lane & ((1 << gs) - 1)) == 0) gptr(d.t)[r] = br - dr * xr - acc;
 }
 
 // b_{l+1} = P t_l, x_{l+1} = 0 (lv: level l, lvn: level l+1 of each member)
@@ -90,15 +90,15 @@ __global__ __launch_bounds__(BLK) void kb_restrict(const Seg *__restrict__ segs,
   if (s >= g.s1) return;
   const LevelDev<T> &d = lv[g.member];
   const LevelDev<T> &dn = lvn[g.member];
-  const SliceInfo q = d.p_si[s];
+  const SliceInfo q = load_si(gptr(d.p_si), s);
   const int lane = threadIdx.x % WAVE;
   const int gs = d.p_gs;
   if ((lane >> gs) >= q.nrow) return;
   const int r = q.row0 + (lane >> gs);
-  const T acc = row_group_sum(sell_dot<T>(q, lane, d.p_cols, d.p_vals, d.t), gs);
+  const T acc = row_group_sum(sell_dot<T>(q, lane, gptr(d.p_cols), gptr(d.p_vals), gptr((const T *)d.t)), gs);
   if ((lane & ((1 << gs) - 1)) == 0) {
-    dn.b[r] = acc;
-    dn.x[r] = 0;
+    gptr(dn.b)[r] = acc;
+    gptr(dn.x)[r] = 0;
   }
 }
 
@@ -112,12 +112,12 @@ __global__ __launch_bounds__(BLK) void kb_prolong(const Seg *__restrict__ segs, 
   if (s >= g.s1) return;
   const LevelDev<T> &d = lv[g.member];
   const LevelDev<T> &dn = lvn[g.member];
-  const SliceInfo q = d.t_si[s];
+  const SliceInfo q = load_si(gptr(d.t_si), s);
   const int lane = threadIdx.x % WAVE;
   if (lane >= q.nrow) return;
   const int r = q.row0 + lane;
-  const T xr = d.x[r];
-  d.x[r] = xr + sell_dot<T>(q, lane, d.t_cols, d.t_vals, dn.x);
+  const T xr = gptr(d.x)[r];
+  gptr(d.x)[r] = xr + sell_dot<T>(q, lane, gptr(d.t_cols), gptr(d.t_vals), gptr((const T *)dn.x));
 }
 
 // the coarse tail of each member: one workgroup per member
@@ -147,16 +147,18 @@ __global__ __launch_bounds__(BLK) void kb_spmv_dot(const Seg *__restrict__ segs,
   double dot = 0.0;
   if (s < g.s1) {
     const MemberDev<T> &m = md[g.member];
-    const SliceInfo q = m.nat_si[s];
+    const SliceInfo q = load_si(gptr(m.nat_si), s);
     if (lane < q.nrow) {
       const int pos = q.row0 + lane;
-      const int r = m.nat_rowmap[pos];
-      const double xr = m.p[r];
-      const double acc = m.nat_c16 ? sell_dot<double, false, SELL_CHUNK_SPMV, uint16_t>(q, lane, m.nat_c16, m.nat_v,
-                                                                                       m.p, m.nat_cbase[s])
-                                   : sell_dot<double, false, SELL_CHUNK_SPMV>(q, lane, m.nat_cols, m.nat_v, m.p);
-      const double yr = m.nat_diag[pos] * xr + acc;
-      m.q[r] = yr;
+      const int r = gptr(m.nat_rowmap)[pos];
+      const GPtr<const double> p = gptr((const double *)m.p);
+      const double xr = p[r];
+      const double acc =
+          m.nat_c16 ? sell_dot<double, false, SELL_CHUNK_SPMV>(q, lane, gptr(m.nat_c16), gptr(m.nat_v), p,
+                                                               gptr(m.nat_cbase)[s])
+                    : sell_dot<double, false, SELL_CHUNK_SPMV>(q, lane, gptr(m.nat_cols), gptr(m.nat_v), p);
+      const double yr = gptr(m.nat_diag)[pos] * xr + acc;
+      gptr(m.q)[r] = yr;
       dot = xr * yr;
     }
   }
@@ -193,17 +195,21 @@ __global__ __launch_bounds__(256) void kb_cg_update(const Seg *__restrict__ segs
   double rr = 0.0;
   if (i < g.s1) {
     const MemberDev<T> &m = md[g.member];
-    const double alpha = m.sc->rz / m.sc->pq;
-    const double ap = alpha * m.p[i];
-    const double ape = fma(alpha, m.p[i], -ap);
+    // p = 0 (a member's first step after its start or a restart): pq = 0
+    // and alpha = 0, so x and r stay
+    const double pq = m.sc->pq;
+    const double alpha = pq != 0.0 ? m.sc->rz / pq : 0.0;
+    const double pi = gptr(m.p)[i];
+    const double ap = alpha * pi;
+    const double ape = fma(alpha, pi, -ap);
     double s, e;
-    two_sum(m.x[i], ap, s, e);
-    m.x[i] = s;
-    m.xl[i] += e + ape;
-    const double ri = m.r[i] - alpha * m.q[i];
-    m.r[i] = ri;
-    m.b0[m.iperm0[i]] = (T)ri;
-    m.x0[i] = 0;
+    two_sum(gptr(m.x)[i], ap, s, e);
+    gptr(m.x)[i] = s;
+    gptr(m.xl)[i] += e + ape;
+    const double ri = gptr(m.r)[i] - alpha * gptr(m.q)[i];
+    gptr(m.r)[i] = ri;
+    gptr(m.b0)[gptr(m.iperm0)[i]] = (T)ri;
+    gptr(m.x0)[i] = 0;
     rr = ri * ri;
   }
   rr = block_sum(rr, sh);
@@ -223,10 +229,11 @@ __global__ __launch_bounds__(256) void kb_take_z(const Seg *__restrict__ segs, i
   double a = 0.0, b = 0.0;
   if (i < g.s1) {
     const MemberDev<T> &m = md[g.member];
-    const double zn = (double)m.x0[m.iperm0[i]];
-    a = m.r[i] * zn;
-    b = m.r[i] * m.z[i];
-    m.z[i] = zn;
+    const double zn = (double)gptr(m.x0)[gptr(m.iperm0)[i]];
+    const double ri = gptr(m.r)[i];
+    a = ri * zn;
+    b = ri * gptr(m.z)[i];
+    gptr(m.z)[i] = zn;
   }
   a = block_sum(a, sh);
   b = block_sum(b, sh);
@@ -241,7 +248,8 @@ __global__ void kb_cg_beta(CgScalars *const *__restrict__ sc, int nm) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m < nm) {
     CgScalars *c = sc[m];
-    c->beta = (c->rz_new - c->rz_cross) / c->rz;
+    // rz = 0 on a member's first step: beta = 0 and p = z
+    c->beta = c->rz != 0.0 ? (c->rz_new - c->rz_cross) / c->rz : 0.0;
     c->rz = c->rz_new;
   }
 }
@@ -253,7 +261,7 @@ __global__ __launch_bounds__(256) void kb_p_update(const Seg *__restrict__ segs,
   const int64_t i = (int64_t)g.s0 + (int64_t)(blockIdx.x - g.block0) * 256 + threadIdx.x;
   if (i < g.s1) {
     const MemberDev<T> &m = md[g.member];
-    m.p[i] = m.z[i] + m.sc->beta * m.p[i];
+    gptr(m.p)[i] = gptr(m.z)[i] + m.sc->beta * gptr(m.p)[i];
   }
 }
 

@@ -487,10 +487,31 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(const double *__restri
 // ---------------------------------------------------------------------------
 // sliced-ELL kernels (one wavefront per slice, one lane per row)
 // ---------------------------------------------------------------------------
+// Pointers with their address space in the type: loads and stores through
+// them compile to global_* (or ds_*) instructions.  Pointers read from
+// memory (tables of levels or members) are otherwise generic, and their
+// accesses compile to flat instructions, which count in the LDS counter as
+// well and are waited for more coarsely.
+template <class T> using GPtr = __attribute__((address_space(1))) T *;
+template <class T> using LPtr = __attribute__((address_space(3))) T *;
+template <class T> __device__ __forceinline__ GPtr<T> gptr(T *p) { return (GPtr<T>)p; }
+template <class T> __device__ __forceinline__ GPtr<const T> gptr(const T *p) { return (GPtr<const T>)p; }
+
+// a slice descriptor read through an address-space-qualified pointer
+// (field by field: the implicit copy works on generic references only)
+template <class SP> __device__ __forceinline__ SliceInfo load_si(SP p, int s) {
+  SliceInfo q;
+  q.row0 = p[s].row0;
+  q.nrow = p[s].nrow;
+  q.width = p[s].width;
+  q.off = p[s].off;
+  return q;
+}
+
 // streamed matrix data: non-temporal loads keep the gathered vector in cache
 template <class U> __device__ inline U ld_stream(const U *p) { return __builtin_nontemporal_load(p); }
 
-template <bool NT, class U> __device__ inline U ld_mat(const U *p) {
+template <bool NT, class P> __device__ inline auto ld_mat(P p) {
   if constexpr (NT)
     return __builtin_nontemporal_load(p);
   else
@@ -518,12 +539,14 @@ template <class T, bool NT> struct LaunchChunk {
   static constexpr int v = (!NT && sizeof(T) == 4) ? 24 : SELL_CHUNK;
 };
 // Columns are int32 indices, or (CI = uint16_t) offsets from the slice's
-// base column cbase.
-template <class T, bool NT = false, int CH = SELL_CHUNK, class CI = int32_t>
-__device__ inline T sell_dot(const SliceInfo &s, int lane, const CI *__restrict__ cols,
-                             const T *__restrict__ vals, const T *__restrict__ x, int32_t cbase = 0) {
-  const CI *c = cols + s.off + lane;
-  const T *v = vals + s.off + lane;
+// base column cbase.  The pointers may carry an address space (the batch
+// kernels pass global-typed pointers read from their tables).
+template <class T, bool NT = false, int CH = SELL_CHUNK, class CI = int32_t, class CP = const CI *,
+          class VP = const T *, class XP = const T *>
+__device__ inline T sell_dot(const SliceInfo &s, int lane, CP __restrict__ cols, VP __restrict__ vals,
+                             XP __restrict__ x, int32_t cbase = 0) {
+  const auto c = cols + s.off + lane;
+  const auto v = vals + s.off + lane;
   T acc = 0;
   for (int j = 0; j < s.width; j += CH) {
     int32_t cc[CH]; // the base is added at the gather, after every load of the chunk is issued
@@ -764,21 +787,6 @@ template <class T> struct LevelDev {
 // flat ones: global loads count only in the vector-memory counter, so loads
 // issued before a barrier stay in flight across it (the barrier waits for
 // the LDS counter), and LDS accesses take the short LDS path.
-template <class T> using GPtr = __attribute__((address_space(1))) T *;
-template <class T> using LPtr = __attribute__((address_space(3))) T *;
-template <class T> __device__ __forceinline__ GPtr<T> gptr(T *p) { return (GPtr<T>)p; }
-template <class T> __device__ __forceinline__ GPtr<const T> gptr(const T *p) { return (GPtr<const T>)p; }
-
-// a slice descriptor read through an address-space-qualified pointer
-// (field by field: the implicit copy works on generic references only)
-template <class SP> __device__ __forceinline__ SliceInfo load_si(SP p, int s) {
-  SliceInfo q;
-  q.row0 = p[s].row0;
-  q.nrow = p[s].nrow;
-  q.width = p[s].width;
-  q.off = p[s].off;
-  return q;
-}
 
 // One wavefront's share of a colour step: the slice's first chunk of column
 // and value loads, 1/diag and b of its row, issued one step ahead (none of
