@@ -91,44 +91,68 @@ static void batch_restart(BatchMember &m) {
   m.fresh = true;
 }
 
-// the member's pair reached its target (or the iteration limit): check the
-// pair's true residual, restart, or deliver round(x_hi + x_lo)
-template <class T> static bool batch_check(afe_batch *b, BatchMember &m) {
-  afe_amg *g = m.g;
-  const int64_t n = g->lv[0].n;
-  hipStream_t s = g->ctx->stream;
-  residual_norms(g, true, g->wb.p, g->wx.p, g->wr.p, g->wxl.p);
-  read_scalars(g);
-  m.rr = g->h_sc->rr;
-  bool pair_conv = m.rr <= m.pthr;
-  if (!pair_conv && ++m.restarts <= 20 && m.it < m.max_it) {
-    batch_restart(m);
-    return false;
+// members whose pair reached its target (or the iteration limit): check each
+// pair's true residual, restart, or deliver round(x_hi + x_lo).  The members
+// checked in one step share each host round trip (their kernels are issued
+// together, then one synchronisation and the reads of their scalars).
+static void batch_check_all(afe_batch *b, const std::vector<BatchMember *> &list) {
+  if (list.empty()) return;
+  hipStream_t s = b->ctx->stream;
+  auto fetch = [&](const std::vector<BatchMember *> &ms) {
+    for (BatchMember *m : ms)
+      HIPCHK(hipMemcpyAsync(m->g->h_sc, m->g->sc.p, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  };
+  for (BatchMember *m : list) residual_norms(m->g, true, m->g->wb.p, m->g->wx.p, m->g->wr.p, m->g->wxl.p);
+  fetch(list);
+  std::vector<BatchMember *> round, deliver;
+  for (BatchMember *m : list) {
+    m->rr = m->g->h_sc->rr;
+    const bool pair_conv = m->rr <= m->pthr;
+    if (!pair_conv && ++m->restarts <= 20 && m->it < m->max_it)
+      batch_restart(*m);
+    else
+      round.push_back(m);
   }
-  for (;;) {
-    k_dd_round<<<nblk(n), 256, 0, s>>>(n, g->wx.p, g->wxl.p);
-    residual_norms(g, true, g->wb.p, g->wx.p, g->wr.p, nullptr);
-    read_scalars(g);
-    m.rr = g->h_sc->rr;
-    m.l1 = g->h_sc->l1;
-    m.conv = m.rr <= m.thr;
-    if (m.conv || m.it >= m.max_it || m.restarts > 20 || m.outer >= 3) break;
-    // rounding floor reached: tighten the pair and continue iterating
-    m.outer++;
-    m.pthr *= 0.25;
-    if (m.rr > m.pthr) {
-      batch_restart(m);
-      return false;
+  while (!round.empty()) {
+    for (BatchMember *m : round) {
+      afe_amg *g = m->g;
+      const int64_t n = g->lv[0].n;
+      k_dd_round<<<nblk(n), 256, 0, s>>>(n, g->wx.p, g->wxl.p);
+      residual_norms(g, true, g->wb.p, g->wx.p, g->wr.p, nullptr);
     }
+    fetch(round);
+    std::vector<BatchMember *> again;
+    for (BatchMember *m : round) {
+      m->rr = m->g->h_sc->rr;
+      m->l1 = m->g->h_sc->l1;
+      m->conv = m->rr <= m->thr;
+      if (m->conv || m->it >= m->max_it || m->restarts > 20 || m->outer >= 3) {
+        deliver.push_back(m);
+        continue;
+      }
+      // rounding floor reached: tighten the pair and continue iterating
+      m->outer++;
+      m->pthr *= 0.25;
+      if (m->rr > m->pthr)
+        batch_restart(*m);
+      else
+        again.push_back(m);
+    }
+    round.swap(again);
   }
-  HIPCHK(hipMemcpyAsync(m.d_x, g->wx.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  if (deliver.empty()) return;
+  for (BatchMember *m : deliver)
+    HIPCHK(hipMemcpyAsync(m->d_x, m->g->wx.p, m->g->lv[0].n * sizeof(double), hipMemcpyDeviceToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
-  m.active = false;
-  m.done = true;
-  m.status = m.conv ? AFE_OK : AFE_E_NOTCONVERGED;
-  m.t_done = std::chrono::steady_clock::now();
+  const auto now = std::chrono::steady_clock::now();
+  for (BatchMember *m : deliver) {
+    m->active = false;
+    m->done = true;
+    m->status = m->conv ? AFE_OK : AFE_E_NOTCONVERGED;
+    m->t_done = now;
+  }
   b->dirty = true;
-  return true;
 }
 
 template <class T> static void member_tables(afe_amg *g, MemberDev<T> &md, std::vector<LevelDev<T>> **hl);
@@ -417,6 +441,7 @@ template <class T> static void batch_step_impl(afe_batch *b, int *n_active) {
   b->n_steps++;
   b->t_step += std::chrono::duration<double>(clk::now() - t1).count();
   const std::vector<int> act = b->act;
+  std::vector<BatchMember *> check;
   for (int k = 0; k < K; k++) {
     BatchMember &m = *b->slots[act[k]];
     if (m.fresh) {
@@ -439,8 +464,9 @@ template <class T> static void batch_step_impl(afe_batch *b, int *n_active) {
       b->dirty = true;
       continue;
     }
-    if (m.rr <= m.pthr || m.it >= m.max_it) batch_check<T>(b, m);
+    if (m.rr <= m.pthr || m.it >= m.max_it) check.push_back(&m);
   }
+  batch_check_all(b, check);
   int na = 0;
   for (auto &p : b->slots)
     if (p && p->active) na++;
