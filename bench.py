@@ -461,11 +461,13 @@ def run_c3(args):
     log(f"c3 space: {sp.n_dof} DOFs, nnz {sp.nnz}, {t_space:.1f} s, assembly {1e3*min(asm):.2f} ms")
     x = afe.DeviceVector(ctx, sp.n_dof)
     best = None
-    for nmpo, nmpsd in ((50, 0.382), (2048, 0.0)):
+    choices = ((50, 0.382), (2048, 0.0)) if args.nmpo == 0 else ((args.nmpo, args.nmpsd),)
+    smooths = (1, 2) if args.smooth == 0 else (args.smooth,)
+    for nmpo, nmpsd in choices:
         t = time.time()
         g = afe.Amg(A, afe.AmgParams(lazy=1, nmpo=nmpo, nmpsd=nmpsd, smooth=1, build_fp32=1))
         t_amg = time.time() - t
-        for s in (1, 2):
+        for s in smooths:
             g.set_smooth(s)
             x.zero()
             g.pcg(x, b, rtol=args.rtol, max_it=1000, prec=afe.PREC_FP32)
