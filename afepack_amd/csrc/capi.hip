@@ -1095,7 +1095,7 @@ static void build_layout(const Hierarchy &h, HostLayout &lay, bool maps) {
     const Csr &A = h.A[l];
     HostLevel &hl = lay.lv[l];
     std::vector<int32_t> colour;
-    const int ncol = colour_rows(A, colour);
+    const int ncol = colour_rows(A, colour, l == 0 ? DSATUR_MAX_ROWS : DSATUR_MAX_ROWS_COARSE);
     tick(tph[0]);
     std::vector<int32_t> len(A.m, 0);
     for (int64_t i = 0; i < A.m; i++)

@@ -658,7 +658,7 @@ void amg_lazy_refresh(Hierarchy &h, const Csr &A0) {
 // ===========================================================================
 // colouring and sliced ELL
 // ===========================================================================
-int colour_rows(const Csr &a, std::vector<int32_t> &colour) {
+int colour_rows(const Csr &a, std::vector<int32_t> &colour, int64_t dsatur_max_rows) {
   const int64_t n = a.m;
   // symmetrised adjacency of the off-diagonal pattern entries (explicit
   // zeros included: the colouring stays valid for any values on the pattern)
@@ -681,12 +681,12 @@ int colour_rows(const Csr &a, std::vector<int32_t> &colour) {
         }
   }
   colour.assign(n, -1);
-  if (n <= DSATUR_MAX_ROWS) {
+  if (n <= dsatur_max_rows) {
     // DSatur (Brelaz 1979): colour next the uncoloured row whose neighbours
     // already use the most distinct colours (ties: more neighbours, then the
     // lower index), with the lowest colour its neighbours do not use.  On
-    // the coarse levels it needs one or two colours fewer than first-fit in
-    // index order, and every colour is one launch per sweep.
+    // the coarse levels it needs fewer colours than first-fit in index
+    // order, and every colour is one launch per sweep.
     std::vector<uint64_t> used(n, 0); // neighbour colours (at most 64 here)
     std::vector<int32_t> sat(n, 0);
     // heap key: saturation (bits 56..63), degree (32..55), n-1-row (0..31),
