@@ -75,6 +75,10 @@ __global__ __launch_bounds__(BLK) void kb_residual(const Seg *__restrict__ segs,
   const int gs = d.gs;
   if ((lane >> gs) >= q.nrow) return;
   const int r = q.row0 + (lane >> gs);
+  if (s >= d.zs) { // the last colour after the forward sweep (the batch smooths at least once)
+    if ((lane & ((1 << gs) - 1)) == 0) gptr(d.t)[r] = 0;
+    return;
+  }
   const T br = gptr(d.b)[r], dr = gptr(d.diag)[r], xr = gptr(d.x)[r];
   const T acc = row_group_sum(sell_dot<T>(q, lane, gptr(d.cols), gptr(d.vals), gptr((const T *)d.x)), gs);
   if ((lane & ((1 << gs) - 1)) == 0) gptr(d.t)[r] = br - dr * xr - acc;

@@ -899,6 +899,7 @@ struct DevLevel {
   std::vector<int32_t> colour_row;   // host copy: rows of colour c are [colour_row[c], colour_row[c+1])
   int ns = 0, p_ns = 0, t_ns = 0;
   int gs = 0, p_gs = 0; // lanes per row (shift) of the level matrix and of the restriction
+  int zs = 0;           // first slice of the last colour
   DevBuf<SliceInfo> si, p_si, t_si;
   DevBuf<int32_t> cols, p_cols, t_cols, col_slice;
   DevBuf<uint16_t> c16; // colour-coded 16-bit columns (streamed level 0), used by k_gs / k_residual
@@ -1238,6 +1239,7 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
     d.ns = (int)sa.slices.size();
     d.gs = sa.gshift;
     d.colour_slice = sa.group_slice;
+    d.zs = d.ncol > 0 ? sa.group_slice[d.ncol - 1] : d.ns;
     d.rows_c0 = sa.group_row.size() > 1 ? sa.group_row[1] : hl.n;
     d.colour_row = sa.group_row;
     d.si.upload(ctx, sa.slices);
@@ -1338,7 +1340,7 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
     DevLevel &d = g->lv[l];
     LevelDev<double> a{};
     a.n = (int)d.n; a.ncol = d.ncol; a.ns = d.ns; a.p_ns = d.p_ns; a.t_ns = d.t_ns;
-    a.gs = d.gs; a.p_gs = d.p_gs;
+    a.gs = d.gs; a.p_gs = d.p_gs; a.zs = d.zs;
     a.col_slice = d.col_slice.p; a.si = d.si.p; a.cols = d.cols.p;
     a.vals = d.v64.p; a.invd = d.invd64.p; a.diag = d.diag64.p;
     a.p_si = d.p_si.p; a.p_cols = d.p_cols.p; a.p_vals = d.p_v64.p;
@@ -1347,7 +1349,7 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
     d64[l] = a;
     LevelDev<float> b{};
     b.n = a.n; b.ncol = a.ncol; b.ns = a.ns; b.p_ns = a.p_ns; b.t_ns = a.t_ns;
-    b.gs = a.gs; b.p_gs = a.p_gs;
+    b.gs = a.gs; b.p_gs = a.p_gs; b.zs = a.zs;
     b.col_slice = a.col_slice; b.si = a.si; b.cols = a.cols;
     b.vals = d.v32.p; b.invd = d.invd32.p; b.diag = d.diag32.p;
     b.p_si = a.p_si; b.p_cols = a.p_cols; b.p_vals = d.p_v32.p;
@@ -1678,7 +1680,8 @@ static void issue_vcycle(afe_amg *g, int nu, bool symmetric, hipStream_t st, boo
         gs_colour<T>(d, c, st);
       }
     if (d.stream) {
-      k_residual<T, true><<<sblk(d.ns), BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, d.c16.p, d.cb16.p, d.cshift,
+      k_residual<T, true><<<sblk(d.ns), BLK, 0, st>>>(d.si.p, d.ns, nu > 0 ? d.zs : d.ns, d.cols.p, d.c16.p,
+                                                       d.cb16.p, d.cshift,
                                                        Prec<T>::v(d), Prec<T>::diag(d),
                                                        Prec<T>::b(d), Prec<T>::x(d), Prec<T>::t(d), d.gs,
                                                        g_tune.xcd);
@@ -1686,7 +1689,8 @@ static void issue_vcycle(afe_amg *g, int nu, bool symmetric, hipStream_t st, boo
                                                          Prec<T>::t(d), Prec<T>::b(dn), Prec<T>::x(dn), d.p_gs,
                                                          g_tune.xcd);
     } else {
-      k_residual<T, false><<<sblk(d.ns), BLK, 0, st>>>(d.si.p, d.ns, d.cols.p, nullptr, nullptr, 0,
+      k_residual<T, false><<<sblk(d.ns), BLK, 0, st>>>(d.si.p, d.ns, nu > 0 ? d.zs : d.ns, d.cols.p, nullptr,
+                                                        nullptr, 0,
                                                         Prec<T>::v(d), Prec<T>::diag(d),
                                                         Prec<T>::b(d), Prec<T>::x(d), Prec<T>::t(d), d.gs,
                                                         g_tune.xcd);
@@ -2111,7 +2115,7 @@ extern "C" int afe_amg_kernel_timing(afe_amg *g, int32_t which, int32_t reps, do
     } else if (which == 2) {
       for (int c = 0; c < d.ncol; c++) gs_colour<double>(d, c, s);
     } else {
-      k_residual<float, true><<<sblk(d.ns), BLK, 0, s>>>(d.si.p, d.ns, d.cols.p, d.c16.p, d.cb16.p, d.cshift,
+      k_residual<float, true><<<sblk(d.ns), BLK, 0, s>>>(d.si.p, d.ns, d.ns, d.cols.p, d.c16.p, d.cb16.p, d.cshift,
                                                           d.v32.p, d.diag32.p, d.b32.p,
                                                           d.x32.p, d.t32.p, d.gs, g_tune.xcd);
     }

@@ -707,9 +707,13 @@ __global__ __launch_bounds__(256) void k_gs_first(int64_t n, int64_t n0, const T
   if (i < n) x[i] = i < n0 ? b[i] * invd[i] : (T)0;
 }
 
-// t = b - A x (A = diag + stored off-diagonals)
+// t = b - A x (A = diag + stored off-diagonals).  Slices from s_zero on
+// hold the rows of the colour a forward sweep updated last: none of their
+// neighbours changed after them, so their residual is exactly zero in exact
+// arithmetic and t = 0 is written without reading the matrix (s_zero = ns
+// computes every row).
 template <class T, bool NT>
-__global__ __launch_bounds__(BLK) void k_residual(const SliceInfo *__restrict__ si, int ns,
+__global__ __launch_bounds__(BLK) void k_residual(const SliceInfo *__restrict__ si, int ns, int s_zero,
                                                   const int32_t *__restrict__ cols, const uint16_t *__restrict__ c16,
                                                   const int32_t *__restrict__ cb16, int cshift,
                                                   const T *__restrict__ vals, const T *__restrict__ diag,
@@ -721,6 +725,10 @@ __global__ __launch_bounds__(BLK) void k_residual(const SliceInfo *__restrict__ 
   const int lane = threadIdx.x % WAVE;
   if ((lane >> gs) >= q.nrow) return;
   const int r = q.row0 + (lane >> gs);
+  if (s >= s_zero) {
+    if ((lane & ((1 << gs) - 1)) == 0) t[r] = 0;
+    return;
+  }
   const T br = b[r], dr = diag[r], xr = x[r];
   T acc = c16 ? sell_dot_cc<T, NT, LaunchChunk<T, NT>::v>(q, lane, c16, cb16 + (int64_t)s * CC_BASES, cshift,
                                                           vals, x)
@@ -769,6 +777,7 @@ __global__ __launch_bounds__(BLK) void k_prolong(const SliceInfo *__restrict__ s
 template <class T> struct LevelDev {
   int n, ncol, ns, p_ns, t_ns;
   int gs, p_gs; // lanes per row (shift) of the level matrix and of the restriction
+  int zs;       // first slice of the last colour (zero residual after a forward sweep)
   const int32_t *col_slice; // ncol+1 slice bounds
   const SliceInfo *si;
   const int32_t *cols;
@@ -940,12 +949,17 @@ template <class T, class SP, class CP, class VP>
 __device__ __forceinline__ void tail_down(const LevelDev<T> &d, const LevelDev<T> &dn, SP si, CP cs, VP x, VP b,
                                           VP t, int nu, int w, int nw, int lane) {
   tail_sweeps<T>(d, si, cs, b, x, nu, false, w, nw, lane);
-  // t = b - A x
+  // t = b - A x (zero on the last colour after a sweep, as k_residual)
+  const int s_zero = nu > 0 ? d.zs : d.ns;
   for (int s = w; s < d.ns; s += nw) {
     const SliceInfo q = load_si(si, s);
     const int gs = d.gs;
     if ((lane >> gs) < q.nrow) {
       const int r = q.row0 + (lane >> gs);
+      if (s >= s_zero) {
+        if ((lane & ((1 << gs) - 1)) == 0) t[r] = 0;
+        continue;
+      }
       const T br = b[r], dr = gptr(d.diag)[r], xr = x[r];
       const T acc = row_group_sum(tail_dot<T>(q, lane, gptr(d.cols), gptr(d.vals), x, 0, T(0)), gs);
       if ((lane & ((1 << gs) - 1)) == 0) t[r] = br - dr * xr - acc;
