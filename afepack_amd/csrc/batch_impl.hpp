@@ -376,6 +376,8 @@ template <class T> static void batch_issue_step(afe_batch *b, hipStream_t s) {
     size_t g0 = gi;
     for (int it = 0; it < b->nu; it++) {
       for (gi = g0; gi < b->down_gs.size() && b->down_gs_level[gi] == l; gi++) {
+        // a level below 0 starts from the first colour step that kb_restrict took
+        if (l > 0 && it == 0 && gi == g0) continue;
         const BLaunch &bl = b->down_gs[gi];
         if (bl.grid) kb_gs<T><<<bl.grid, BLK, 0, s>>>(S + bl.seg_off, bl.nseg, LV + bl.lv_off);
       }

@@ -101,8 +101,10 @@ __global__ __launch_bounds__(BLK) void kb_restrict(const Seg *__restrict__ segs,
   const int r = q.row0 + (lane >> gs);
   const T acc = row_group_sum(sell_dot<T>(q, lane, gptr(d.p_cols), gptr(d.p_vals), gptr((const T *)d.t)), gs);
   if ((lane & ((1 << gs) - 1)) == 0) {
+    // the next level's first colour from zero (as k_restrict); a coarse
+    // tail recomputes the same values when it sweeps that colour
     gptr(dn.b)[r] = acc;
-    gptr(dn.x)[r] = 0;
+    gptr(dn.x)[r] = r < dn.rows_c0 ? acc * gptr(dn.invd)[r] : (T)0;
   }
 }
 

@@ -1340,7 +1340,7 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
     DevLevel &d = g->lv[l];
     LevelDev<double> a{};
     a.n = (int)d.n; a.ncol = d.ncol; a.ns = d.ns; a.p_ns = d.p_ns; a.t_ns = d.t_ns;
-    a.gs = d.gs; a.p_gs = d.p_gs; a.zs = d.zs;
+    a.gs = d.gs; a.p_gs = d.p_gs; a.zs = d.zs; a.rows_c0 = (int)d.rows_c0;
     a.col_slice = d.col_slice.p; a.si = d.si.p; a.cols = d.cols.p;
     a.vals = d.v64.p; a.invd = d.invd64.p; a.diag = d.diag64.p;
     a.p_si = d.p_si.p; a.p_cols = d.p_cols.p; a.p_vals = d.p_v64.p;
@@ -1349,7 +1349,7 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
     d64[l] = a;
     LevelDev<float> b{};
     b.n = a.n; b.ncol = a.ncol; b.ns = a.ns; b.p_ns = a.p_ns; b.t_ns = a.t_ns;
-    b.gs = a.gs; b.p_gs = a.p_gs; b.zs = a.zs;
+    b.gs = a.gs; b.p_gs = a.p_gs; b.zs = a.zs; b.rows_c0 = a.rows_c0;
     b.col_slice = a.col_slice; b.si = a.si; b.cols = a.cols;
     b.vals = d.v32.p; b.invd = d.invd32.p; b.diag = d.diag32.p;
     b.p_si = a.p_si; b.p_cols = a.p_cols; b.p_vals = d.p_v32.p;
@@ -1671,12 +1671,16 @@ static void issue_vcycle(afe_amg *g, int nu, bool symmetric, hipStream_t st, boo
     DevLevel &d = g->lv[l];
     DevLevel &dn = g->lv[l + 1];
     if (l == 0 && x0_zero && nu == 0) k_fill<T><<<nblk(d.n), 256, 0, st>>>(d.n, Prec<T>::x(d), (T)0);
+    // the next level is smoothed by launches from zero: the restriction
+    // takes its first colour step
+    const bool first_next = nu > 0 && l + 1 < g->tail0;
     for (int it = 0; it < nu; it++)
       for (int c = 0; c < d.ncol; c++) {
         if (l == 0 && it == 0 && c == 0 && x0_zero) {
           k_gs_first<T><<<nblk(d.n), 256, 0, st>>>(d.n, d.rows_c0, Prec<T>::b(d), Prec<T>::invd(d), Prec<T>::x(d));
           continue;
         }
+        if (l > 0 && it == 0 && c == 0) continue; // done by the restriction from level l - 1
         gs_colour<T>(d, c, st);
       }
     if (d.stream) {
@@ -1686,8 +1690,9 @@ static void issue_vcycle(afe_amg *g, int nu, bool symmetric, hipStream_t st, boo
                                                        Prec<T>::b(d), Prec<T>::x(d), Prec<T>::t(d), d.gs,
                                                        g_tune.xcd);
       k_restrict<T, true><<<sblk(d.p_ns), BLK, 0, st>>>(d.p_si.p, d.p_ns, d.p_cols.p, Prec<T>::pv(d),
-                                                         Prec<T>::t(d), Prec<T>::b(dn), Prec<T>::x(dn), d.p_gs,
-                                                         g_tune.xcd);
+                                                         Prec<T>::t(d), Prec<T>::b(dn), Prec<T>::x(dn),
+                                                         first_next ? Prec<T>::invd(dn) : nullptr, (int)dn.rows_c0,
+                                                         d.p_gs, g_tune.xcd);
     } else {
       k_residual<T, false><<<sblk(d.ns), BLK, 0, st>>>(d.si.p, d.ns, nu > 0 ? d.zs : d.ns, d.cols.p, nullptr,
                                                         nullptr, 0,
@@ -1695,8 +1700,9 @@ static void issue_vcycle(afe_amg *g, int nu, bool symmetric, hipStream_t st, boo
                                                         Prec<T>::b(d), Prec<T>::x(d), Prec<T>::t(d), d.gs,
                                                         g_tune.xcd);
       k_restrict<T, false><<<sblk(d.p_ns), BLK, 0, st>>>(d.p_si.p, d.p_ns, d.p_cols.p, Prec<T>::pv(d),
-                                                          Prec<T>::t(d), Prec<T>::b(dn), Prec<T>::x(dn), d.p_gs,
-                                                          g_tune.xcd);
+                                                          Prec<T>::t(d), Prec<T>::b(dn), Prec<T>::x(dn),
+                                                          first_next ? Prec<T>::invd(dn) : nullptr,
+                                                          (int)dn.rows_c0, d.p_gs, g_tune.xcd);
     }
   }
   const int lds_cap = (int)g->ctx->lds_bytes; // dynamic LDS of the tail workgroup

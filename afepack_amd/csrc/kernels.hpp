@@ -737,12 +737,17 @@ __global__ __launch_bounds__(BLK) void k_residual(const SliceInfo *__restrict__ 
   if ((lane & ((1 << gs) - 1)) == 0) t[r] = br - dr * xr - acc;
 }
 
-// restriction: bc = P t, and the coarse guess xc = 0
+// restriction: bc = P t, and the coarse guess xc = 0.  With invd_c (the
+// coarse level is smoothed next, from zero), the first colour's rows
+// [0, rows_c0) get xc = bc / diag at once: their neighbours are all zero, so
+// this is the first colour step of the coarse sweep (as k_gs_first), and
+// that launch is left out.
 template <class T, bool NT>
 __global__ __launch_bounds__(BLK) void k_restrict(const SliceInfo *__restrict__ si, int ns,
                                                   const int32_t *__restrict__ cols, const T *__restrict__ vals,
                                                   const T *__restrict__ t, T *__restrict__ bc,
-                                                  T *__restrict__ xc, int gs, int xcd) {
+                                                  T *__restrict__ xc, const T *__restrict__ invd_c, int rows_c0,
+                                                  int gs, int xcd) {
   const int s = logical_block(xcd) * WPB + wave_id();
   if (s >= ns) return;
   const SliceInfo q = si[s];
@@ -752,7 +757,7 @@ __global__ __launch_bounds__(BLK) void k_restrict(const SliceInfo *__restrict__ 
   const T acc = row_group_sum(sell_dot<T, NT, LaunchChunk<T, NT>::v>(q, lane, cols, vals, t), gs);
   if ((lane & ((1 << gs) - 1)) == 0) {
     bc[r] = acc;
-    xc[r] = 0;
+    xc[r] = invd_c && r < rows_c0 ? acc * invd_c[r] : (T)0;
   }
 }
 
@@ -778,6 +783,7 @@ template <class T> struct LevelDev {
   int n, ncol, ns, p_ns, t_ns;
   int gs, p_gs; // lanes per row (shift) of the level matrix and of the restriction
   int zs;       // first slice of the last colour (zero residual after a forward sweep)
+  int rows_c0;  // rows of the first colour (rows [0, rows_c0))
   const int32_t *col_slice; // ncol+1 slice bounds
   const SliceInfo *si;
   const int32_t *cols;
