@@ -122,6 +122,7 @@ __global__ __launch_bounds__(BLK) void kb_prolong(const Seg *__restrict__ segs, 
   const int lane = threadIdx.x % WAVE;
   if (lane >= q.nrow) return;
   const int r = q.row0 + lane;
+  if (r >= d.rows_lc) return; // the last colour: the reverse post-sweep overwrites it unread
   const T xr = gptr(d.x)[r];
   gptr(d.x)[r] = xr + sell_dot<T>(q, lane, gptr(d.t_cols), gptr(d.t_vals), gptr((const T *)dn.x));
 }

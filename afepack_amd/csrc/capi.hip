@@ -1341,6 +1341,7 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
     LevelDev<double> a{};
     a.n = (int)d.n; a.ncol = d.ncol; a.ns = d.ns; a.p_ns = d.p_ns; a.t_ns = d.t_ns;
     a.gs = d.gs; a.p_gs = d.p_gs; a.zs = d.zs; a.rows_c0 = (int)d.rows_c0;
+    a.rows_lc = d.ncol > 0 ? d.colour_row[d.ncol - 1] : (int)d.n;
     a.col_slice = d.col_slice.p; a.si = d.si.p; a.cols = d.cols.p;
     a.vals = d.v64.p; a.invd = d.invd64.p; a.diag = d.diag64.p;
     a.p_si = d.p_si.p; a.p_cols = d.p_cols.p; a.p_vals = d.p_v64.p;
@@ -1349,7 +1350,7 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
     d64[l] = a;
     LevelDev<float> b{};
     b.n = a.n; b.ncol = a.ncol; b.ns = a.ns; b.p_ns = a.p_ns; b.t_ns = a.t_ns;
-    b.gs = a.gs; b.p_gs = a.p_gs; b.zs = a.zs; b.rows_c0 = a.rows_c0;
+    b.gs = a.gs; b.p_gs = a.p_gs; b.zs = a.zs; b.rows_c0 = a.rows_c0; b.rows_lc = a.rows_lc;
     b.col_slice = a.col_slice; b.si = a.si; b.cols = a.cols;
     b.vals = d.v32.p; b.invd = d.invd32.p; b.diag = d.diag32.p;
     b.p_si = a.p_si; b.p_cols = a.p_cols; b.p_vals = d.p_v32.p;
@@ -1723,12 +1724,16 @@ static void issue_vcycle(afe_amg *g, int nu, bool symmetric, hipStream_t st, boo
   for (int l = g->tail0 - 1; l >= 0; l--) {
     DevLevel &d = g->lv[l];
     DevLevel &dn = g->lv[l + 1];
+    // the colour the post-smoother sweeps first is left to it
+    const int lc0 = d.ncol > 0 ? d.colour_row[d.ncol - 1] : (int)d.n;
+    const int skip0 = nu == 0 ? 0 : (symmetric ? lc0 : 0);
+    const int skip1 = nu == 0 ? 0 : (symmetric ? (int)d.n : (int)d.rows_c0);
     if (d.stream)
       k_prolong<T, true><<<sblk(d.t_ns), BLK, 0, st>>>(d.t_si.p, d.t_ns, d.t_cols.p, Prec<T>::tv(d),
-                                                        Prec<T>::x(dn), Prec<T>::x(d), g_tune.xcd);
+                                                        Prec<T>::x(dn), Prec<T>::x(d), skip0, skip1, g_tune.xcd);
     else
       k_prolong<T, false><<<sblk(d.t_ns), BLK, 0, st>>>(d.t_si.p, d.t_ns, d.t_cols.p, Prec<T>::tv(d),
-                                                         Prec<T>::x(dn), Prec<T>::x(d), g_tune.xcd);
+                                                         Prec<T>::x(dn), Prec<T>::x(d), skip0, skip1, g_tune.xcd);
     for (int it = 0; it < nu; it++)
       for (int cc = 0; cc < d.ncol; cc++) gs_colour<T>(d, symmetric ? d.ncol - 1 - cc : cc, st);
   }

@@ -761,17 +761,22 @@ __global__ __launch_bounds__(BLK) void k_restrict(const SliceInfo *__restrict__ 
   }
 }
 
-// prolongation: x += P^T xc (explicit transpose, no atomics)
+// prolongation: x += P^T xc (explicit transpose, no atomics).  Rows in
+// [skip0, skip1) are left alone: they form the colour the post-smoother
+// updates first, which overwrites them without reading them (a row's own x
+// is not among its stored off-diagonal entries).
 template <class T, bool NT>
 __global__ __launch_bounds__(BLK) void k_prolong(const SliceInfo *__restrict__ si, int ns,
                                                  const int32_t *__restrict__ cols, const T *__restrict__ vals,
-                                                 const T *__restrict__ xc, T *__restrict__ x, int xcd) {
+                                                 const T *__restrict__ xc, T *__restrict__ x, int skip0, int skip1,
+                                                 int xcd) {
   const int s = logical_block(xcd) * WPB + wave_id();
   if (s >= ns) return;
   const SliceInfo q = si[s];
   const int lane = threadIdx.x % WAVE;
   if (lane >= q.nrow) return;
   const int r = q.row0 + lane;
+  if (r >= skip0 && r < skip1) return;
   const T xr = x[r];
   x[r] = xr + sell_dot<T, NT, LaunchChunk<T, NT>::v>(q, lane, cols, vals, xc);
 }
@@ -784,6 +789,7 @@ template <class T> struct LevelDev {
   int gs, p_gs; // lanes per row (shift) of the level matrix and of the restriction
   int zs;       // first slice of the last colour (zero residual after a forward sweep)
   int rows_c0;  // rows of the first colour (rows [0, rows_c0))
+  int rows_lc;  // first row of the last colour
   const int32_t *col_slice; // ncol+1 slice bounds
   const SliceInfo *si;
   const int32_t *cols;
@@ -918,12 +924,13 @@ __device__ __forceinline__ void tail_sweeps(const LevelDev<T> &d, SP si, CP cs, 
 // as f(r, sum)
 template <class T, class SP, class XP, class F>
 __device__ __forceinline__ void tail_rows(SP si, int ns, const int32_t *cols, const T *vals, int gs, XP v, int w,
-                                          int nw, int lane, F f) {
+                                          int nw, int lane, int skip0, int skip1, F f) {
   for (int s = w; s < ns; s += nw) {
     const SliceInfo q = load_si(si, s);
-    if ((lane >> gs) < q.nrow) {
+    const int r = q.row0 + (lane >> gs);
+    if ((lane >> gs) < q.nrow && !(r >= skip0 && r < skip1)) {
       const T acc = row_group_sum(tail_dot<T>(q, lane, gptr(cols), gptr(vals), v, 0, T(0)), gs);
-      if ((lane & ((1 << gs) - 1)) == 0) f(q.row0 + (lane >> gs), acc);
+      if ((lane & ((1 << gs) - 1)) == 0) f(r, acc);
     }
   }
 }
@@ -974,7 +981,7 @@ __device__ __forceinline__ void tail_down(const LevelDev<T> &d, const LevelDev<T
   __syncthreads();
   // b_{l+1} = P t, x_{l+1} = 0
   const GPtr<T> nb = gptr(dn.b), nx = gptr(dn.x);
-  tail_rows<T>(gptr(d.p_si), d.p_ns, d.p_cols, d.p_vals, d.p_gs, t, w, nw, lane, [&](int r, T v) {
+  tail_rows<T>(gptr(d.p_si), d.p_ns, d.p_cols, d.p_vals, d.p_gs, t, w, nw, lane, 0, 0, [&](int r, T v) {
     nb[r] = v;
     nx[r] = 0;
   });
@@ -983,9 +990,12 @@ __device__ __forceinline__ void tail_down(const LevelDev<T> &d, const LevelDev<T
 template <class T, class SP, class CP, class VP>
 __device__ __forceinline__ void tail_up(const LevelDev<T> &d, const LevelDev<T> &dn, SP si, CP cs, VP x, VP b,
                                         int nu, bool rev, int w, int nw, int lane) {
-  // x += P^T x_{l+1}
+  // x += P^T x_{l+1}, except on the colour the post-smoother sweeps first
+  // (overwritten unread, as in k_prolong)
   const GPtr<const T> nx = gptr((const T *)dn.x);
-  tail_rows<T>(gptr(d.t_si), d.t_ns, d.t_cols, d.t_vals, 0, nx, w, nw, lane, [&](int r, T v) { x[r] = x[r] + v; });
+  const int skip0 = nu == 0 ? 0 : (rev ? d.rows_lc : 0), skip1 = nu == 0 ? 0 : (rev ? d.n : d.rows_c0);
+  tail_rows<T>(gptr(d.t_si), d.t_ns, d.t_cols, d.t_vals, 0, nx, w, nw, lane, skip0, skip1,
+               [&](int r, T v) { x[r] = x[r] + v; });
   __syncthreads();
   tail_sweeps<T>(d, si, cs, b, x, nu, rev, w, nw, lane);
 }
