@@ -402,8 +402,7 @@ template <class T> static void batch_issue_step(afe_batch *b, hipStream_t s) {
   }
   // z, (r, z) and the flexible beta, new direction
   kb_take_z<T><<<b->rows.grid, 256, 0, s>>>(S + b->rows.seg_off, b->rows.nseg, md, b->rows.grid, part);
-  kb_finalize<<<K, 256, 0, s>>>(b->row_first.p, b->row_count.p, 2, b->rows.grid, part, b->scp.p, 2);
-  kb_cg_beta<<<(K + 63) / 64, 64, 0, s>>>(b->scp.p, K);
+  kb_finalize<<<K, 256, 0, s>>>(b->row_first.p, b->row_count.p, 2, b->rows.grid, part, b->scp.p, 2, 1);
   kb_p_update<T><<<b->rows.grid, 256, 0, s>>>(S + b->rows.seg_off, b->rows.nseg, md);
   kb_gather_scalars<<<(K + 63) / 64, 64, 0, s>>>(b->scp.p, K, b->gath.p);
 }

@@ -175,10 +175,13 @@ __global__ __launch_bounds__(BLK) void kb_spmv_dot(const Seg *__restrict__ segs,
 
 // sum of each member's block partials, in block order, into a scalar slot:
 // one block per member; first[m], count[m] = the member's block range
+// with beta set, the (rz_new, rz_cross) sums are followed (same thread, in
+// order) by the member's flexible beta and rz = rz_new (rz = 0 on a member's
+// first step gives beta = 0)
 __global__ __launch_bounds__(256) void kb_finalize(const int32_t *__restrict__ first,
                                                    const int32_t *__restrict__ count, int nsum, int stride,
                                                    const double *__restrict__ part,
-                                                   CgScalars *const *__restrict__ sc, int field) {
+                                                   CgScalars *const *__restrict__ sc, int field, int beta = 0) {
   __shared__ double sh[256];
   const int m = blockIdx.x;
   for (int k = 0; k < nsum; k++) {
@@ -186,6 +189,11 @@ __global__ __launch_bounds__(256) void kb_finalize(const int32_t *__restrict__ f
     for (int i = threadIdx.x; i < count[m]; i += blockDim.x) s += part[k * stride + first[m] + i];
     s = block_sum(s, sh);
     if (threadIdx.x == 0) (&sc[m]->rz)[field + k] = s;
+  }
+  if (beta && threadIdx.x == 0) {
+    CgScalars *c = sc[m];
+    c->beta = c->rz != 0.0 ? (c->rz_new - c->rz_cross) / c->rz : 0.0;
+    c->rz = c->rz_new;
   }
 }
 
@@ -247,17 +255,6 @@ __global__ __launch_bounds__(256) void kb_take_z(const Seg *__restrict__ segs, i
   if (threadIdx.x == 0) {
     part[blockIdx.x] = a;
     part[stride + blockIdx.x] = b;
-  }
-}
-
-// flexible CG beta of every member (one thread per member)
-__global__ void kb_cg_beta(CgScalars *const *__restrict__ sc, int nm) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m < nm) {
-    CgScalars *c = sc[m];
-    // rz = 0 on a member's first step: beta = 0 and p = z
-    c->beta = c->rz != 0.0 ? (c->rz_new - c->rz_cross) / c->rz : 0.0;
-    c->rz = c->rz_new;
   }
 }
 

@@ -1869,8 +1869,7 @@ template <class T> static void precondition_perm(afe_amg *g, const double *r, bo
   launch_graph_or_issue(g, std::is_same<T, float>::value ? 1 : 0, true, g->prm.smooth);
   if (first) k_fill<double><<<nblk(n), 256, 0, s>>>(n, g->wz.p, 0.0);
   k_take_z<T><<<RED_BLOCKS, 256, 0, s>>>(n, Prec<T>::x(d0), r, g->wz.p, g->iperm0.p, g->part.p);
-  k_finalize<<<1, FIN_THREADS, 0, s>>>(g->part.p, RED_BLOCKS, 2, &g->sc.p->rz_new);
-  k_cg_beta<<<1, 64, 0, s>>>(g->sc.p, first ? 1 : 0);
+  k_finalize_beta<<<1, FIN_THREADS, 0, s>>>(g->part.p, RED_BLOCKS, g->sc.p, first ? 1 : 0);
   k_p_update<<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wz.p, g->wp.p);
 }
 
@@ -1917,8 +1916,7 @@ template <class T> static void issue_cg_precondition(afe_amg *g) {
   const int64_t n = d0.n;
   issue_vcycle<T>(g, g->prm.smooth, true, s, true);
   k_take_z<T><<<RED_BLOCKS, 256, 0, s>>>(n, Prec<T>::x(d0), g->wr.p, g->wz.p, g->iperm0.p, g->part.p);
-  k_finalize<<<1, FIN_THREADS, 0, s>>>(g->part.p, RED_BLOCKS, 2, &g->sc.p->rz_new);
-  k_cg_beta<<<1, 64, 0, s>>>(g->sc.p, 0);
+  k_finalize_beta<<<1, FIN_THREADS, 0, s>>>(g->part.p, RED_BLOCKS, g->sc.p, 0);
   k_p_update<<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wz.p, g->wp.p);
 }
 

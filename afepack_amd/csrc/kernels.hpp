@@ -1426,10 +1426,23 @@ __global__ __launch_bounds__(256) void k_take_z(int64_t n, const T *__restrict__
 
 // beta = ((r, z_new) - (r, z_old)) / rz: flexible (Polak-Ribiere) CG, robust
 // to the FP32 preconditioner; rz = (r, z_new).  first != 0: beta = 0 (p = z)
-__global__ void k_cg_beta(CgScalars *sc, int first) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    sc->beta = first ? 0.0 : (sc->rz_new - sc->rz_cross) / sc->rz;
-    sc->rz = sc->rz_new;
+// k_finalize of the (r, z_new) and (r, z_old) partials into rz_new and
+// rz_cross, then (the same thread, in order) the flexible beta =
+// (rz_new - rz_cross) / rz (0 on the first direction) and rz = rz_new
+__global__ __launch_bounds__(FIN_THREADS) void k_finalize_beta(const double *__restrict__ part, int np,
+                                                               CgScalars *__restrict__ sc, int first) {
+  __shared__ double sh[FIN_THREADS];
+  double v[2];
+  for (int k = 0; k < 2; k++) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[k * np + i];
+    v[k] = block_sum(s, sh);
+  }
+  if (threadIdx.x == 0) {
+    sc->rz_new = v[0];
+    sc->rz_cross = v[1];
+    sc->beta = first ? 0.0 : (v[0] - v[1]) / sc->rz;
+    sc->rz = v[0];
   }
 }
 
