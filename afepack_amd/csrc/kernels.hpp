@@ -812,7 +812,7 @@ template <class T> struct LevelDev {
 // One wavefront's share of a colour step: the slice's first chunk of column
 // and value loads, 1/diag and b of its row, issued one step ahead (none of
 // them depends on x).
-template <class T> constexpr int TailChunk = sizeof(T) == 8 ? 4 : 8; // keeps the tail within 128 VGPRs
+template <class T> constexpr int TailChunk = sizeof(T) == 8 ? 4 : 6; // keeps the tail within 128 VGPRs
 template <class T> struct TailRow {
   SliceInfo q;
   int32_t cc[TailChunk<T>];
@@ -889,8 +889,8 @@ __device__ __forceinline__ void tail_finish(const TailRow<T> &p, const LevelDev<
 
 // nu sweeps over the colours of a level (forward, or reverse when rev),
 // wavefront w taking slices w, w + nw, ... of each colour.  The loads of a
-// wavefront's first slice of the next colour are issued before the barrier
-// that ends the current colour.
+// wavefront's first slice of a colour are issued two colour steps ahead,
+// before the barriers that end the steps in between.
 template <class T, class SP, class CP, class VP>
 __device__ __forceinline__ void tail_sweeps(const LevelDev<T> &d, SP si, CP cs, VP b, VP x, int nu, bool rev,
                                             int w, int nw, int lane) {
@@ -899,10 +899,17 @@ __device__ __forceinline__ void tail_sweeps(const LevelDev<T> &d, SP si, CP cs, 
     const int c = i % nc;
     return rev ? nc - 1 - c : c;
   };
-  TailRow<T> p;
-  p.on = false;
-  if (steps > 0 && cs[colour(0)] + w < cs[colour(0) + 1]) tail_issue(p, d, si, b, cs[colour(0)] + w, lane);
-  for (int i = 0; i < steps; i++) {
+  // two steps in flight: the loads for step i + 2 are issued at the end of
+  // step i into the buffer step i consumed (two named buffers, the loop
+  // unrolled by two, so that they stay in registers)
+  auto issue_step = [&](TailRow<T> &p, int i) {
+    p.on = false;
+    if (i < steps) {
+      const int c = colour(i);
+      if (cs[c] + w < cs[c + 1]) tail_issue(p, d, si, b, cs[c] + w, lane);
+    }
+  };
+  auto run_step = [&](TailRow<T> &p, int i) {
     const int c = colour(i), s1 = cs[c + 1];
     if (cs[c] + w < s1) tail_finish(p, d, x, lane);
     for (int s = cs[c] + w + nw; s < s1; s += nw) {
@@ -910,12 +917,15 @@ __device__ __forceinline__ void tail_sweeps(const LevelDev<T> &d, SP si, CP cs, 
       tail_issue(e, d, si, b, s, lane);
       tail_finish(e, d, x, lane);
     }
-    p.on = false;
-    if (i + 1 < steps) {
-      const int cn = colour(i + 1);
-      if (cs[cn] + w < cs[cn + 1]) tail_issue(p, d, si, b, cs[cn] + w, lane);
-    }
+    issue_step(p, i + 2);
     __syncthreads();
+  };
+  TailRow<T> pa, pb;
+  issue_step(pa, 0);
+  issue_step(pb, 1);
+  for (int i = 0; i < steps; i += 2) {
+    run_step(pa, i);
+    if (i + 1 < steps) run_step(pb, i + 1);
   }
 }
 
