@@ -288,6 +288,9 @@ typedef struct afe_batch afe_batch;
 int afe_batch_create(afe_ctx *ctx, int32_t prec, int32_t smooth,
                      afe_batch **out);
 int afe_batch_destroy(afe_batch *b);
+/* add a member: its b and x are copied and its residual issued; its first
+ * preconditioning runs inside the next afe_batch_step (not counted as an
+ * iteration), together with the other members */
 int afe_batch_add(afe_batch *b, afe_amg *g, double *d_x, const double *d_b,
                   double rtol, int32_t max_it, int32_t *slot);
 /* one batched CG iteration; *n_active = members still iterating */
