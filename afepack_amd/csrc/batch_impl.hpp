@@ -55,6 +55,8 @@ struct afe_batch {
   DevBuf<MemberDev<double>> md64;
   DevBuf<TailJob<float>> tj32;
   DevBuf<TailJob<double>> tj64;
+  DevBuf<LevelDev<float>> tl32; // every member's level descriptors, one after another
+  DevBuf<LevelDev<double>> tl64;
   DevBuf<CgScalars *> scp;
   DevBuf<int32_t> spmv_first, spmv_count, row_first, row_count;
   DevBuf<double> part;
@@ -177,6 +179,9 @@ template <> DevBuf<MemberDev<double>> &md_buf<double>(afe_batch *b) { return b->
 template <class T> static DevBuf<TailJob<T>> &tj_buf(afe_batch *b);
 template <> DevBuf<TailJob<float>> &tj_buf<float>(afe_batch *b) { return b->tj32; }
 template <> DevBuf<TailJob<double>> &tj_buf<double>(afe_batch *b) { return b->tj64; }
+template <class T> static DevBuf<LevelDev<T>> &tl_buf(afe_batch *b);
+template <> DevBuf<LevelDev<float>> &tl_buf<float>(afe_batch *b) { return b->tl32; }
+template <> DevBuf<LevelDev<double>> &tl_buf<double>(afe_batch *b) { return b->tl64; }
 
 // launch tables of the active members
 template <class T> static void batch_build_epoch(afe_batch *b) {
@@ -195,6 +200,7 @@ template <class T> static void batch_build_epoch(afe_batch *b) {
   std::vector<LevelDev<T>> lvs;
   std::vector<MemberDev<T>> mds(K);
   std::vector<TailJob<T>> tjs(K);
+  std::vector<LevelDev<T>> tls;
   std::vector<CgScalars *> scs(K);
   std::vector<int32_t> sf(K), sc(K), rf(K), rcnt(K);
   std::vector<std::vector<LevelDev<T>> *> hl(K);
@@ -216,7 +222,8 @@ template <class T> static void batch_build_epoch(afe_batch *b) {
     member_tables<T>(g, m, &hl[k]);
     scs[k] = g->sc.p;
     const int L = (int)g->lv.size();
-    tjs[k].L = Prec<T>::ld(g);
+    tjs[k].loff = (int)tls.size();
+    tls.insert(tls.end(), hl[k]->begin(), hl[k]->end());
     tjs[k].minv = g->minv.p;
     tjs[k].l0 = g->tail0;
     tjs[k].lc = L - 1;
@@ -337,6 +344,7 @@ template <class T> static void batch_build_epoch(afe_batch *b) {
   lv_buf<T>(b).upload(ctx, lvs);
   md_buf<T>(b).upload(ctx, mds);
   tj_buf<T>(b).upload(ctx, tjs);
+  tl_buf<T>(b).upload(ctx, tls);
   b->scp.upload(ctx, scs);
   b->spmv_first.upload(ctx, sf);
   b->spmv_count.upload(ctx, sc);
@@ -388,7 +396,7 @@ template <class T> static void batch_issue_step(afe_batch *b, hipStream_t s) {
     if (bp.grid) kb_restrict<T><<<bp.grid, BLK, 0, s>>>(S + bp.seg_off, bp.nseg, LV + bp.lv_off, LV + bp.lvn_off);
   }
   const int lds_cap = (int)b->ctx->lds_bytes;
-  kb_tail<T><<<K, TAIL_THREADS, (size_t)lds_cap, s>>>(tj_buf<T>(b).p, b->nu, lds_cap);
+  kb_tail<T><<<K, TAIL_THREADS, (size_t)lds_cap, s>>>(tj_buf<T>(b).p, tl_buf<T>(b).p, b->nu, lds_cap);
   for (int l = L - 1; l >= 0; l--) {
     const BLaunch &bt = b->up_prolong[l];
     if (bt.grid) kb_prolong<T><<<bt.grid, BLK, 0, s>>>(S + bt.seg_off, bt.nseg, LV + bt.lv_off, LV + bt.lvn_off);

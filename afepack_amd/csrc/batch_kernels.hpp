@@ -128,17 +128,20 @@ __global__ __launch_bounds__(BLK) void kb_prolong(const Seg *__restrict__ segs, 
 }
 
 // the coarse tail of each member: one workgroup per member
+// a member's level descriptors are at tl + loff in one table of the batch:
+// reached from a kernel argument, their fields load as scalars (a table
+// pointer read from memory would make every field a flat vector load)
 template <class T> struct TailJob {
-  const LevelDev<T> *L;
   const double *minv;
-  int l0, lc, coarse_mode;
+  int loff, l0, lc, coarse_mode;
 };
 
 template <class T>
-__global__ __launch_bounds__(TAIL_THREADS) void kb_tail(const TailJob<T> *__restrict__ jobs, int nu,
+__global__ __launch_bounds__(TAIL_THREADS) void kb_tail(const TailJob<T> *__restrict__ jobs,
+                                                        const LevelDev<T> *__restrict__ tl, int nu,
                                                         int lds_cap) {
   const TailJob<T> j = jobs[blockIdx.x];
-  tail_body<T>(j.L, j.l0, j.lc, j.minv, j.coarse_mode, nu, nu, 1, 7, lds_cap);
+  tail_body<T>(tl + j.loff, j.l0, j.lc, j.minv, j.coarse_mode, nu, nu, 1, 7, lds_cap);
 }
 
 // ---- CG kernels (natural numbering) ---------------------------------------

@@ -812,7 +812,11 @@ template <class T> struct LevelDev {
 // One wavefront's share of a colour step: the slice's first chunk of column
 // and value loads, 1/diag and b of its row, issued one step ahead (none of
 // them depends on x).
-template <class T> constexpr int TailChunk = sizeof(T) == 8 ? 4 : 6; // keeps the tail within 128 VGPRs
+// entries per lane loaded ahead for a colour step (most tail rows fit in
+// one) and the chunk of the other row products; together they keep the tail
+// within 128 VGPRs
+template <class T> constexpr int TailChunk = sizeof(T) == 8 ? 6 : 12;
+template <class T> constexpr int TailDotChunk = sizeof(T) == 8 ? 4 : 6;
 template <class T> struct TailRow {
   SliceInfo q;
   int32_t cc[TailChunk<T>];
@@ -847,7 +851,7 @@ __device__ __forceinline__ T tail_dot(const SliceInfo &q, int lane, GPtr<const i
                                       XP x, int j0, T acc) {
   const GPtr<const int32_t> c = cols + q.off + lane;
   const GPtr<const T> v = vals + q.off + lane;
-  constexpr int CH = TailChunk<T>;
+  constexpr int CH = TailDotChunk<T>;
   for (int j = j0; j < q.width; j += CH) {
     int32_t cc[CH];
     T vv[CH], xx[CH];
@@ -889,8 +893,9 @@ __device__ __forceinline__ void tail_finish(const TailRow<T> &p, const LevelDev<
 
 // nu sweeps over the colours of a level (forward, or reverse when rev),
 // wavefront w taking slices w, w + nw, ... of each colour.  The loads of a
-// wavefront's first slice of a colour are issued two colour steps ahead,
-// before the barriers that end the steps in between.
+// wavefront's first slice of the next colour are issued before the barrier
+// that ends the current colour; the chunk holds a whole row's share of most
+// tail levels (up to 12 entries per lane in FP32).
 template <class T, class SP, class CP, class VP>
 __device__ __forceinline__ void tail_sweeps(const LevelDev<T> &d, SP si, CP cs, VP b, VP x, int nu, bool rev,
                                             int w, int nw, int lane) {
@@ -899,9 +904,6 @@ __device__ __forceinline__ void tail_sweeps(const LevelDev<T> &d, SP si, CP cs, 
     const int c = i % nc;
     return rev ? nc - 1 - c : c;
   };
-  // two steps in flight: the loads for step i + 2 are issued at the end of
-  // step i into the buffer step i consumed (two named buffers, the loop
-  // unrolled by two, so that they stay in registers)
   auto issue_step = [&](TailRow<T> &p, int i) {
     p.on = false;
     if (i < steps) {
@@ -909,7 +911,9 @@ __device__ __forceinline__ void tail_sweeps(const LevelDev<T> &d, SP si, CP cs, 
       if (cs[c] + w < cs[c + 1]) tail_issue(p, d, si, b, cs[c] + w, lane);
     }
   };
-  auto run_step = [&](TailRow<T> &p, int i) {
+  TailRow<T> p;
+  issue_step(p, 0);
+  for (int i = 0; i < steps; i++) {
     const int c = colour(i), s1 = cs[c + 1];
     if (cs[c] + w < s1) tail_finish(p, d, x, lane);
     for (int s = cs[c] + w + nw; s < s1; s += nw) {
@@ -917,15 +921,8 @@ __device__ __forceinline__ void tail_sweeps(const LevelDev<T> &d, SP si, CP cs, 
       tail_issue(e, d, si, b, s, lane);
       tail_finish(e, d, x, lane);
     }
-    issue_step(p, i + 2);
+    issue_step(p, i + 1);
     __syncthreads();
-  };
-  TailRow<T> pa, pb;
-  issue_step(pa, 0);
-  issue_step(pb, 1);
-  for (int i = 0; i < steps; i += 2) {
-    run_step(pa, i);
-    if (i + 1 < steps) run_step(pb, i + 1);
   }
 }
 
