@@ -1342,6 +1342,7 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
     a.n = (int)d.n; a.ncol = d.ncol; a.ns = d.ns; a.p_ns = d.p_ns; a.t_ns = d.t_ns;
     a.gs = d.gs; a.p_gs = d.p_gs; a.zs = d.zs; a.rows_c0 = (int)d.rows_c0;
     a.rows_lc = d.ncol > 0 ? d.colour_row[d.ncol - 1] : (int)d.n;
+    a.nst = (int)d.cols.n;
     a.col_slice = d.col_slice.p; a.si = d.si.p; a.cols = d.cols.p;
     a.vals = d.v64.p; a.invd = d.invd64.p; a.diag = d.diag64.p;
     a.p_si = d.p_si.p; a.p_cols = d.p_cols.p; a.p_vals = d.p_v64.p;
@@ -1350,7 +1351,7 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
     d64[l] = a;
     LevelDev<float> b{};
     b.n = a.n; b.ncol = a.ncol; b.ns = a.ns; b.p_ns = a.p_ns; b.t_ns = a.t_ns;
-    b.gs = a.gs; b.p_gs = a.p_gs; b.zs = a.zs; b.rows_c0 = a.rows_c0; b.rows_lc = a.rows_lc;
+    b.gs = a.gs; b.p_gs = a.p_gs; b.zs = a.zs; b.rows_c0 = a.rows_c0; b.rows_lc = a.rows_lc; b.nst = a.nst;
     b.col_slice = a.col_slice; b.si = a.si; b.cols = a.cols;
     b.vals = d.v32.p; b.invd = d.invd32.p; b.diag = d.diag32.p;
     b.p_si = a.p_si; b.p_cols = a.p_cols; b.p_vals = d.p_v32.p;

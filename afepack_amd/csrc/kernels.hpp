@@ -790,6 +790,7 @@ template <class T> struct LevelDev {
   int zs;       // first slice of the last colour (zero residual after a forward sweep)
   int rows_c0;  // rows of the first colour (rows [0, rows_c0))
   int rows_lc;  // first row of the last colour
+  int nst;      // stored entries of the level matrix
   const int32_t *col_slice; // ncol+1 slice bounds
   const SliceInfo *si;
   const int32_t *cols;
@@ -825,8 +826,9 @@ template <class T> struct TailRow {
   bool on; // this lane has a row
 };
 
-template <class T, class SP, class VP>
-__device__ __forceinline__ void tail_issue(TailRow<T> &p, const LevelDev<T> &d, SP si, VP b, int s, int lane) {
+template <class T, class SP, class VP, class MC, class MV>
+__device__ __forceinline__ void tail_issue(TailRow<T> &p, const LevelDev<T> &d, SP si, VP b, int s, int lane, MC mc,
+                                           MV mv) {
   p.q = load_si(si, s);
   const int gs = d.gs;
   p.on = (lane >> gs) < p.q.nrow;
@@ -834,8 +836,8 @@ __device__ __forceinline__ void tail_issue(TailRow<T> &p, const LevelDev<T> &d, 
   const int r = p.q.row0 + (lane >> gs);
   p.br = b[r];
   p.ir = gptr(d.invd)[r];
-  const GPtr<const int32_t> c = gptr(d.cols) + p.q.off + lane;
-  const GPtr<const T> v = gptr(d.vals) + p.q.off + lane;
+  const auto c = mc + p.q.off + lane;
+  const auto v = mv + p.q.off + lane;
 #pragma unroll
   for (int k = 0; k < TailChunk<T>; k++)
     if (k < p.q.width) {
@@ -846,11 +848,10 @@ __device__ __forceinline__ void tail_issue(TailRow<T> &p, const LevelDev<T> &d, 
 
 // the rest of sell_dot's row product after the first chunk (same loads,
 // same summation order: chunk by chunk, entry by entry)
-template <class T, class XP>
-__device__ __forceinline__ T tail_dot(const SliceInfo &q, int lane, GPtr<const int32_t> cols, GPtr<const T> vals,
-                                      XP x, int j0, T acc) {
-  const GPtr<const int32_t> c = cols + q.off + lane;
-  const GPtr<const T> v = vals + q.off + lane;
+template <class T, class XP, class MC, class MV>
+__device__ __forceinline__ T tail_dot(const SliceInfo &q, int lane, MC cols, MV vals, XP x, int j0, T acc) {
+  const auto c = cols + q.off + lane;
+  const auto v = vals + q.off + lane;
   constexpr int CH = TailDotChunk<T>;
   for (int j = j0; j < q.width; j += CH) {
     int32_t cc[CH];
@@ -872,8 +873,8 @@ __device__ __forceinline__ T tail_dot(const SliceInfo &q, int lane, GPtr<const i
 }
 
 // x_r = (b_r - sum_{j != r} a_rj x_j) / a_rr for the row of an issued slice
-template <class T, class XP>
-__device__ __forceinline__ void tail_finish(const TailRow<T> &p, const LevelDev<T> &d, XP x, int lane) {
+template <class T, class XP, class MC, class MV>
+__device__ __forceinline__ void tail_finish(const TailRow<T> &p, const LevelDev<T> &d, XP x, int lane, MC mc, MV mv) {
   if (!p.on) return;
   const int gs = d.gs;
   T acc = 0;
@@ -886,7 +887,7 @@ __device__ __forceinline__ void tail_finish(const TailRow<T> &p, const LevelDev<
     for (int k = 0; k < TailChunk<T>; k++)
       if (k < p.q.width) acc += p.vv[k] * xx[k];
   }
-  acc = tail_dot<T>(p.q, lane, gptr(d.cols), gptr(d.vals), x, TailChunk<T>, acc);
+  acc = tail_dot<T>(p.q, lane, mc, mv, x, TailChunk<T>, acc);
   acc = row_group_sum(acc, gs);
   if ((lane & ((1 << gs) - 1)) == 0) x[p.q.row0 + (lane >> gs)] = (p.br - acc) * p.ir;
 }
@@ -896,9 +897,9 @@ __device__ __forceinline__ void tail_finish(const TailRow<T> &p, const LevelDev<
 // wavefront's first slice of the next colour are issued before the barrier
 // that ends the current colour; the chunk holds a whole row's share of most
 // tail levels (up to 12 entries per lane in FP32).
-template <class T, class SP, class CP, class VP>
+template <class T, class SP, class CP, class VP, class MC, class MV>
 __device__ __forceinline__ void tail_sweeps(const LevelDev<T> &d, SP si, CP cs, VP b, VP x, int nu, bool rev,
-                                            int w, int nw, int lane) {
+                                            int w, int nw, int lane, MC mc, MV mv) {
   const int nc = d.ncol, steps = nu * nc;
   auto colour = [&](int i) {
     const int c = i % nc;
@@ -908,18 +909,18 @@ __device__ __forceinline__ void tail_sweeps(const LevelDev<T> &d, SP si, CP cs, 
     p.on = false;
     if (i < steps) {
       const int c = colour(i);
-      if (cs[c] + w < cs[c + 1]) tail_issue(p, d, si, b, cs[c] + w, lane);
+      if (cs[c] + w < cs[c + 1]) tail_issue(p, d, si, b, cs[c] + w, lane, mc, mv);
     }
   };
   TailRow<T> p;
   issue_step(p, 0);
   for (int i = 0; i < steps; i++) {
     const int c = colour(i), s1 = cs[c + 1];
-    if (cs[c] + w < s1) tail_finish(p, d, x, lane);
+    if (cs[c] + w < s1) tail_finish(p, d, x, lane, mc, mv);
     for (int s = cs[c] + w + nw; s < s1; s += nw) {
       TailRow<T> e;
-      tail_issue(e, d, si, b, s, lane);
-      tail_finish(e, d, x, lane);
+      tail_issue(e, d, si, b, s, lane, mc, mv);
+      tail_finish(e, d, x, lane, mc, mv);
     }
     issue_step(p, i + 1);
     __syncthreads();
@@ -944,17 +945,23 @@ __device__ __forceinline__ void tail_rows(SP si, int ns, const int32_t *cols, co
 
 // LDS layout of one level while the tail works on it: [x | b | t] when the
 // vectors fit (n T each), then the slice table and the colour bounds when
-// they fit behind them; otherwise those stay in global memory.
+// they fit behind them, then the level matrix (values, int32 columns) when
+// all of it fits too; otherwise those stay in global memory.  A level whose
+// matrix is staged runs its colour steps on LDS alone.
 struct TailStage {
-  bool vec, tab;
-  int tab_off; // bytes
+  bool vec, tab, mat;
+  int tab_off, mat_off; // bytes
 };
 template <class T> __device__ __forceinline__ TailStage tail_stage_plan(const LevelDev<T> &d, int lds_cap) {
   TailStage t;
   const int64_t vb = 3 * (int64_t)d.n * sizeof(T);
   t.vec = vb <= lds_cap;
   t.tab_off = t.vec ? (int)((vb + 15) & ~int64_t(15)) : 0;
-  t.tab = t.tab_off + (int64_t)d.ns * sizeof(SliceInfo) + (int64_t)(d.ncol + 1) * 4 <= lds_cap;
+  const int64_t te = t.tab_off + (int64_t)d.ns * sizeof(SliceInfo) + (int64_t)(d.ncol + 1) * 4;
+  t.tab = te <= lds_cap;
+  // the level matrix itself (values, then int32 columns) when everything fits
+  t.mat_off = (int)((te + 15) & ~int64_t(15));
+  t.mat = t.vec && t.tab && t.mat_off + (int64_t)d.nst * (sizeof(T) + 4) <= lds_cap;
   return t;
 }
 
@@ -965,10 +972,10 @@ template <class T> __device__ __forceinline__ TailStage tail_stage_plan(const Le
 // lds_cap bytes of dynamic LDS hold the working level's vectors and tables
 // (tail_stage_plan); x of a level is written back to global memory at the
 // end of each leg.
-template <class T, class SP, class CP, class VP>
+template <class T, class SP, class CP, class VP, class MC, class MV>
 __device__ __forceinline__ void tail_down(const LevelDev<T> &d, const LevelDev<T> &dn, SP si, CP cs, VP x, VP b,
-                                          VP t, int nu, int w, int nw, int lane) {
-  tail_sweeps<T>(d, si, cs, b, x, nu, false, w, nw, lane);
+                                          VP t, int nu, int w, int nw, int lane, MC mc, MV mv) {
+  tail_sweeps<T>(d, si, cs, b, x, nu, false, w, nw, lane, mc, mv);
   // t = b - A x (zero on the last colour after a sweep, as k_residual)
   const int s_zero = nu > 0 ? d.zs : d.ns;
   for (int s = w; s < d.ns; s += nw) {
@@ -981,7 +988,7 @@ __device__ __forceinline__ void tail_down(const LevelDev<T> &d, const LevelDev<T
         continue;
       }
       const T br = b[r], dr = gptr(d.diag)[r], xr = x[r];
-      const T acc = row_group_sum(tail_dot<T>(q, lane, gptr(d.cols), gptr(d.vals), x, 0, T(0)), gs);
+      const T acc = row_group_sum(tail_dot<T>(q, lane, mc, mv, x, 0, T(0)), gs);
       if ((lane & ((1 << gs) - 1)) == 0) t[r] = br - dr * xr - acc;
     }
   }
@@ -994,9 +1001,9 @@ __device__ __forceinline__ void tail_down(const LevelDev<T> &d, const LevelDev<T
   });
 }
 
-template <class T, class SP, class CP, class VP>
+template <class T, class SP, class CP, class VP, class MC, class MV>
 __device__ __forceinline__ void tail_up(const LevelDev<T> &d, const LevelDev<T> &dn, SP si, CP cs, VP x, VP b,
-                                        int nu, bool rev, int w, int nw, int lane) {
+                                        int nu, bool rev, int w, int nw, int lane, MC mc, MV mv) {
   // x += P^T x_{l+1}, except on the colour the post-smoother sweeps first
   // (overwritten unread, as in k_prolong)
   const GPtr<const T> nx = gptr((const T *)dn.x);
@@ -1004,7 +1011,7 @@ __device__ __forceinline__ void tail_up(const LevelDev<T> &d, const LevelDev<T> 
   tail_rows<T>(gptr(d.t_si), d.t_ns, d.t_cols, d.t_vals, 0, nx, w, nw, lane, skip0, skip1,
                [&](int r, T v) { x[r] = x[r] + v; });
   __syncthreads();
-  tail_sweeps<T>(d, si, cs, b, x, nu, rev, w, nw, lane);
+  tail_sweeps<T>(d, si, cs, b, x, nu, rev, w, nw, lane, mc, mv);
 }
 
 // runs f(si, cs, x, b, t) with the level's tables and vectors in LDS or in
@@ -1017,6 +1024,13 @@ __device__ __forceinline__ void tail_with_level(const LevelDev<T> &d, char *smem
   LPtr<T> sb = sx + d.n, st = sb + d.n;
   LPtr<SliceInfo> ssi = (LPtr<SliceInfo>)(SliceInfo *)(smem + pl.tab_off);
   LPtr<int32_t> scs = (LPtr<int32_t>)(int32_t *)(smem + pl.tab_off + (int64_t)d.ns * sizeof(SliceInfo));
+  LPtr<T> smv = (LPtr<T>)(T *)(smem + pl.mat_off);
+  LPtr<int32_t> smc = (LPtr<int32_t>)(int32_t *)(smem + pl.mat_off + (int64_t)d.nst * sizeof(T));
+  if (pl.mat)
+    for (int i = tid; i < d.nst; i += blockDim.x) {
+      smv[i] = gptr(d.vals)[i];
+      smc[i] = gptr(d.cols)[i];
+    }
   if (pl.vec)
     for (int i = tid; i < d.n; i += blockDim.x) {
       sx[i] = gptr(d.x)[i];
@@ -1033,16 +1047,20 @@ __device__ __forceinline__ void tail_with_level(const LevelDev<T> &d, char *smem
     for (int i = tid; i <= d.ncol; i += blockDim.x) scs[i] = gptr(d.col_slice)[i];
   }
   __syncthreads();
+  const GPtr<const int32_t> gc = gptr(d.cols);
+  const GPtr<const T> gv = gptr(d.vals);
   if (pl.vec) {
-    if (pl.tab)
-      f(ssi, scs, sx, sb, st);
+    if (pl.mat)
+      f(ssi, scs, sx, sb, st, (LPtr<const int32_t>)smc, (LPtr<const T>)smv);
+    else if (pl.tab)
+      f(ssi, scs, sx, sb, st, gc, gv);
     else
-      f(gptr(d.si), gptr(d.col_slice), sx, sb, st);
+      f(gptr(d.si), gptr(d.col_slice), sx, sb, st, gc, gv);
     for (int i = tid; i < d.n; i += blockDim.x) gptr(d.x)[i] = sx[i];
   } else if (pl.tab) {
-    f(ssi, scs, gptr(d.x), gptr(d.b), gptr(d.t));
+    f(ssi, scs, gptr(d.x), gptr(d.b), gptr(d.t), gc, gv);
   } else {
-    f(gptr(d.si), gptr(d.col_slice), gptr(d.x), gptr(d.b), gptr(d.t));
+    f(gptr(d.si), gptr(d.col_slice), gptr(d.x), gptr(d.b), gptr(d.t), gc, gv);
   }
   __syncthreads();
 }
@@ -1060,8 +1078,8 @@ __device__ __forceinline__ void tail_body(const LevelDev<T> *__restrict__ L, int
     for (int l = l0; l < lc; l++) {
       const LevelDev<T> &d = L[l];
       const LevelDev<T> &dn = L[l + 1];
-      tail_with_level<T>(d, tail_smem, lds_cap, [&](auto si, auto cs, auto x, auto b, auto t) {
-        tail_down<T>(d, dn, si, cs, x, b, t, nu_pre, w, nw, lane);
+      tail_with_level<T>(d, tail_smem, lds_cap, [&](auto si, auto cs, auto x, auto b, auto t, auto mc, auto mv) {
+        tail_down<T>(d, dn, si, cs, x, b, t, nu_pre, w, nw, lane, mc, mv);
       });
     }
   }
@@ -1078,7 +1096,8 @@ __device__ __forceinline__ void tail_body(const LevelDev<T> *__restrict__ L, int
     } else if (coarse_mode == 1) {
       for (int i = tid; i < d.n; i += blockDim.x) gptr(d.x)[i] = gptr(d.b)[i] * gptr(d.invd)[i];
     } else {
-      tail_sweeps<T>(d, gptr(d.si), gptr(d.col_slice), gptr(d.b), gptr(d.x), nu_pre, false, w, nw, lane);
+      tail_sweeps<T>(d, gptr(d.si), gptr(d.col_slice), gptr(d.b), gptr(d.x), nu_pre, false, w, nw, lane,
+                     gptr(d.cols), gptr(d.vals));
     }
     __syncthreads();
   }
@@ -1086,8 +1105,8 @@ __device__ __forceinline__ void tail_body(const LevelDev<T> *__restrict__ L, int
     for (int l = lc - 1; l >= l0; l--) {
       const LevelDev<T> &d = L[l];
       const LevelDev<T> &dn = L[l + 1];
-      tail_with_level<T>(d, tail_smem, lds_cap, [&](auto si, auto cs, auto x, auto b, auto) {
-        tail_up<T>(d, dn, si, cs, x, b, nu_post, post_reverse != 0, w, nw, lane);
+      tail_with_level<T>(d, tail_smem, lds_cap, [&](auto si, auto cs, auto x, auto b, auto, auto mc, auto mv) {
+        tail_up<T>(d, dn, si, cs, x, b, nu_post, post_reverse != 0, w, nw, lane, mc, mv);
       });
     }
   }
