@@ -222,3 +222,27 @@ def test_tail_with_levels_beyond_lds(ctx, kind, n):
         assert abs(big[k + 1] - lau[k + 1]) <= 1
         assert np.linalg.norm(big[k] - lau[k]) <= 1e-9 * np.linalg.norm(lau[k])
     g.close()
+
+
+def test_stationary_cycles_back_to_back_match_checked_cycles(ctx):
+    """With rtol = 0 (the reference's AMGSolver::solve: step + 1 cycles, no
+    early exit) the cycles run back to back and the residual is taken once;
+    with a positive rtol that is never met the residual is checked after
+    every cycle.  Both run the same cycles: the iterates are bitwise equal."""
+    import afepack_amd as afe
+    print("case stationary back to back", flush=True)
+    c, ev, vm = make_mesh(2, 12, 0.2, 3)
+    sp = afe.Space(ctx, 2, c, ev, vm)
+    A = afe.Matrix(sp)
+    A.assemble(afe.OP_STIFF, 6)
+    b = sp.rhs(6, afe.F_POISSON)
+    A.apply_dirichlet(b)
+    g = afe.Amg(A)
+    xs, sts = [], []
+    for rtol in (0.0, 1e-300):
+        x = afe.DeviceVector(ctx, sp.n_dof)
+        sts.append(g.solve(x, b, step=5, mode=1, rtol=rtol))
+        xs.append(x.numpy())
+    assert sts[0].iterations == sts[1].iterations == 6
+    assert np.array_equal(xs[0], xs[1])
+    assert sts[0].res_l1 == sts[1].res_l1
