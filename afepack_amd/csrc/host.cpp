@@ -747,10 +747,13 @@ int colour_rows(const Csr &a, std::vector<int32_t> &colour, int64_t dsatur_max_r
 int row_lane_shift(int64_t rows, int64_t groups, int64_t entries, int max_shift) {
   if (rows <= 0 || groups <= 0) return 0;
   const int64_t per_group = (rows + groups - 1) / groups;
+  const int smax = std::min(max_shift, MAX_ROW_LANE_SHIFT);
   int s = 0;
-  while (s < std::min(max_shift, MAX_ROW_LANE_SHIFT) && (int64_t(2) << s) * 4 * rows <= entries &&
-         (per_group << s) < (int64_t(1) << 16))
-    s++;
+  while (s < smax && (int64_t(2) << s) * 4 * rows <= entries && (per_group << s) < (int64_t(1) << 16)) s++;
+  // a group of a few long rows (the small coarse levels: 2-10 rows per
+  // colour) fills a wavefront's 64 lanes with more lanes per row: fewer
+  // slices and less padding than 64/G rows per slice would leave
+  while (s < smax && (per_group << s) < 64 && (int64_t(1) << s) * 2 * rows <= entries) s++;
   return s;
 }
 
