@@ -751,8 +751,9 @@ int row_lane_shift(int64_t rows, int64_t groups, int64_t entries, int max_shift)
   int s = 0;
   while (s < smax && (int64_t(2) << s) * 4 * rows <= entries && (per_group << s) < (int64_t(1) << 16)) s++;
   // a group of a few long rows (the small coarse levels: 2-10 rows per
-  // colour) fills a wavefront's 64 lanes with more lanes per row: fewer
-  // slices and less padding than 64/G rows per slice would leave
+  // colour) fills a wavefront's 64 lanes with more lanes per row (while
+  // every lane keeps at least one entry on average): fewer slices and less
+  // padding than 64/G rows per slice would leave
   while (s < smax && (per_group << s) < 64 && (int64_t(1) << s) * 2 * rows <= entries) s++;
   return s;
 }

@@ -129,8 +129,9 @@ struct Sell {
 // all: rows are split over more lanes while every lane keeps at least 4
 // entries and a group stays under 2^16 lanes (the small levels of the
 // V-cycle, whose colours are a few thousand rows of long rows, are latency
-// bound with one lane per row), and further, down to 2 entries per lane,
-// while a group's rows fill less than one wavefront; max_shift caps it
+// bound with one lane per row), and further, while every lane keeps an
+// entry, as long as a group's rows fill less than one wavefront; max_shift
+// caps it
 int row_lane_shift(int64_t rows, int64_t groups, int64_t entries, int max_shift);
 
 // colouring of the symmetrised graph of the stored off-diagonal entries
