@@ -1089,8 +1089,21 @@ __device__ __forceinline__ void tail_body(const LevelDev<T> *__restrict__ L, int
       const GPtr<const T> b = gptr((const T *)d.b);
       const GPtr<T> x = gptr(d.x);
       for (int i = tid; i < d.n; i += blockDim.x) {
+        const GPtr<const double> m = gptr(minv) + (int64_t)i * d.n;
         double s = 0.0;
-        for (int j = 0; j < d.n; j++) s += gptr(minv)[(int64_t)i * d.n + j] * (double)b[j];
+        for (int j0 = 0; j0 < d.n; j0 += 8) { // 8 loads in flight, summed in order
+          double mm[8];
+          T bb[8];
+#pragma unroll
+          for (int k = 0; k < 8; k++)
+            if (j0 + k < d.n) {
+              mm[k] = m[j0 + k];
+              bb[k] = b[j0 + k];
+            }
+#pragma unroll
+          for (int k = 0; k < 8; k++)
+            if (j0 + k < d.n) s += mm[k] * (double)bb[k];
+        }
         x[i] = (T)s;
       }
     } else if (coarse_mode == 1) {
@@ -1132,8 +1145,21 @@ __global__ __launch_bounds__(BLK) void k_coarse_gemv(int n, const double *__rest
   const int lane = threadIdx.x % WAVE;
   double s = 0.0;
   if (row < n) {
+    // loads of 8 strides issued together, then summed in the same order
     const double *m = minv + (int64_t)row * n;
-    for (int j = lane; j < n; j += WAVE) s += m[j] * (double)b[j];
+    for (int j0 = lane; j0 < n; j0 += 8 * WAVE) {
+      double mm[8];
+      T bb[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (j0 + k * WAVE < n) {
+          mm[k] = m[j0 + k * WAVE];
+          bb[k] = b[j0 + k * WAVE];
+        }
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (j0 + k * WAVE < n) s += mm[k] * (double)bb[k];
+    }
   }
   // wavefront reduction through LDS (fixed order)
   sh[threadIdx.x] = s;
