@@ -188,8 +188,7 @@ __global__ __launch_bounds__(256) void kb_finalize(const int32_t *__restrict__ f
   __shared__ double sh[256];
   const int m = blockIdx.x;
   for (int k = 0; k < nsum; k++) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < count[m]; i += blockDim.x) s += part[k * stride + first[m] + i];
+    double s = strided_sum(part + (int64_t)k * stride + first[m], count[m], threadIdx.x, blockDim.x);
     s = block_sum(s, sh);
     if (threadIdx.x == 0) (&sc[m]->rz)[field + k] = s;
   }

@@ -473,12 +473,27 @@ __global__ __launch_bounds__(256) void k_sum_partial(int64_t n, const double *__
 // one workgroup of FIN_THREADS sums np block partials per scalar (fixed
 // order: strided per-thread sums, then a tree)
 constexpr int FIN_THREADS = 1024;
+
+// a thread's strided sum of part[i], i = t, t + stride, ... < np, in that
+// order; the loads are issued eight at a time
+__device__ inline double strided_sum(const double *__restrict__ part, int np, int t, int stride) {
+  double s = 0.0;
+  for (int i0 = t; i0 < np; i0 += 8 * stride) {
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (i0 + k * stride < np) v[k] = part[i0 + k * stride];
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (i0 + k * stride < np) s += v[k];
+  }
+  return s;
+}
 __global__ __launch_bounds__(FIN_THREADS) void k_finalize(const double *__restrict__ part, int np, int nsum,
                                                           double *__restrict__ out) {
   __shared__ double sh[FIN_THREADS];
   for (int k = 0; k < nsum; k++) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[k * np + i];
+    double s = strided_sum(part + (int64_t)k * np, np, threadIdx.x, blockDim.x);
     s = block_sum(s, sh);
     if (threadIdx.x == 0) out[k] = s;
   }
@@ -1485,11 +1500,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize_beta(const double *__r
                                                                CgScalars *__restrict__ sc, int first) {
   __shared__ double sh[FIN_THREADS];
   double v[2];
-  for (int k = 0; k < 2; k++) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[k * np + i];
-    v[k] = block_sum(s, sh);
-  }
+  for (int k = 0; k < 2; k++) v[k] = block_sum(strided_sum(part + (int64_t)k * np, np, threadIdx.x, blockDim.x), sh);
   if (threadIdx.x == 0) {
     sc->rz_new = v[0];
     sc->rz_cross = v[1];
