@@ -1476,12 +1476,36 @@ __global__ __launch_bounds__(256) void k_take_z(int64_t n, const T *__restrict__
                                                 double *__restrict__ part) {
   __shared__ double sh[256];
   double a = 0.0, b = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const double zn = (double)x0[iperm ? iperm[i] : i];
-    a += r[i] * zn;
-    b += r[i] * z[i];
-    z[i] = zn;
+  // four grid strides at a time: the index loads, then the gathers, then the
+  // sums in stride order (the same sums as one stride at a time)
+  constexpr int U = 4;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += U * stride) {
+    int32_t ip[U];
+    double rv[U], zo[U];
+    T xv[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      const int64_t i = i0 + k * stride;
+      if (i < n) {
+        ip[k] = iperm ? iperm[i] : (int32_t)i;
+        rv[k] = r[i];
+        zo[k] = z[i];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++)
+      if (i0 + k * stride < n) xv[k] = x0[ip[k]];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      const int64_t i = i0 + k * stride;
+      if (i < n) {
+        const double zn = (double)xv[k];
+        a += rv[k] * zn;
+        b += rv[k] * zo[k];
+        z[i] = zn;
+      }
+    }
   }
   a = block_sum(a, sh);
   b = block_sum(b, sh);
@@ -1491,11 +1515,10 @@ __global__ __launch_bounds__(256) void k_take_z(int64_t n, const T *__restrict__
   }
 }
 
-// beta = ((r, z_new) - (r, z_old)) / rz: flexible (Polak-Ribiere) CG, robust
-// to the FP32 preconditioner; rz = (r, z_new).  first != 0: beta = 0 (p = z)
 // k_finalize of the (r, z_new) and (r, z_old) partials into rz_new and
-// rz_cross, then (the same thread, in order) the flexible beta =
-// (rz_new - rz_cross) / rz (0 on the first direction) and rz = rz_new
+// rz_cross, then (the same thread, in order) the flexible (Polak-Ribiere)
+// beta = (rz_new - rz_cross) / rz, robust to the FP32 preconditioner
+// (first != 0: beta = 0, p = z), and rz = rz_new
 __global__ __launch_bounds__(FIN_THREADS) void k_finalize_beta(const double *__restrict__ part, int np,
                                                                CgScalars *__restrict__ sc, int first) {
   __shared__ double sh[FIN_THREADS];
