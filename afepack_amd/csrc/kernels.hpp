@@ -1376,15 +1376,35 @@ __global__ __launch_bounds__(BLK) void k_residual_norms_dd(const SliceInfo *__re
       const uint16_t *c2 = c16 ? c16 + q.off + lane : nullptr;
       const int32_t cb = c16 ? cbase[s] : 0;
       const double *v = vals + q.off + lane;
-      for (int j = 0; j < q.width; j++) {
-        const int32_t cj = c16 ? cb + (int32_t)c2[j * WAVE] : c[j * WAVE];
-        const double a = v[j * WAVE], xv = x[cj];
-        const double pr = a * xv, pe = fma(a, xv, -pr);
-        double t, e;
-        two_sum(hi, -pr, t, e);
-        hi = t;
-        lo += e - pe;
-        if (xl) lo -= a * xl[cj];
+      // chunks of CH entries: the column and value loads, then the gathers,
+      // then the compensated products in stored order (as one at a time)
+      constexpr int CH = SELL_CHUNK_SPMV;
+      for (int j0 = 0; j0 < q.width; j0 += CH) {
+        int32_t cc[CH];
+        double vv[CH], xx[CH], xe[CH];
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+          if (j0 + k < q.width) {
+            cc[k] = c16 ? cb + (int32_t)c2[(j0 + k) * WAVE] : c[(j0 + k) * WAVE];
+            vv[k] = v[(j0 + k) * WAVE];
+          }
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+          if (j0 + k < q.width) {
+            xx[k] = x[cc[k]];
+            xe[k] = xl ? xl[cc[k]] : 0.0;
+          }
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+          if (j0 + k < q.width) {
+            const double a = vv[k], xv = xx[k];
+            const double pr = a * xv, pe = fma(a, xv, -pr);
+            double t, e;
+            two_sum(hi, -pr, t, e);
+            hi = t;
+            lo += e - pe;
+            if (xl) lo -= a * xe[k];
+          }
       }
       const double rv = hi + lo;
       if (r) r[row] = rv;
