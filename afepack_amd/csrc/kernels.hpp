@@ -825,14 +825,15 @@ template <class T> struct LevelDev {
 // issued before a barrier stay in flight across it (the barrier waits for
 // the LDS counter), and LDS accesses take the short LDS path.
 
+// Entries per lane loaded ahead for a colour step (most tail rows fit in
+// one chunk) and the chunk of the other row products; together they keep
+// the tail within 128 VGPRs.
+template <class T> constexpr int TailChunk = sizeof(T) == 8 ? 6 : 12;
+template <class T> constexpr int TailDotChunk = sizeof(T) == 8 ? 4 : 6;
+
 // One wavefront's share of a colour step: the slice's first chunk of column
 // and value loads, 1/diag and b of its row, issued one step ahead (none of
 // them depends on x).
-// entries per lane loaded ahead for a colour step (most tail rows fit in
-// one) and the chunk of the other row products; together they keep the tail
-// within 128 VGPRs
-template <class T> constexpr int TailChunk = sizeof(T) == 8 ? 6 : 12;
-template <class T> constexpr int TailDotChunk = sizeof(T) == 8 ? 4 : 6;
 template <class T> struct TailRow {
   SliceInfo q;
   int32_t cc[TailChunk<T>];
