@@ -14,9 +14,17 @@ something that does not restate its own formula:
   L2Discretize        exact integrals of interpolated polynomials
   Dirichlet apply     the algebraic identity: the eliminated system has the
                       solution of the constrained original system
-  Project / getPMPt   textbook 1-D result (C points every other node, weights
-                      1/2), dense P A P^T, unit C rows, P >= 0, P^T = P'
-  lazyProject         partition of unity of the interpolation columns
+  Project             textbook 1-D result (C points every other node, weights
+                      1/2); an anisotropic 5-point operator (semi-coarsening
+                      with weak-sum weights 1/2 at eps = 0.1, full coarsening
+                      at eps = 0.25 exactly on the strength threshold and at
+                      0.3); a hand-derived 6-node matrix that takes the F-F
+                      promotion and the distance-two weights; unit C rows,
+                      P >= 0, P^T = P'
+  getPMPt             dense P A P^T and the structural product pattern (explicit
+                      zeros kept, diagonal first, rest ascending)
+  lazyProject         core points = the lexicographically first maximal
+                      independent set (subset enumeration); partition of unity
   Gauss-Seidel        triangular solve (D+L)^-1 (r - U x) with scipy
   V-cycle             two-level error-propagation operator built densely
   dense inverse       A A^-1 = I
@@ -420,6 +428,214 @@ def test_hierarchy_invariants(lazy, kind, n, jit, acc):
         assert m <= 50 and Ac.nnz / (m * m) >= 0.382
         Minv = g.coarse_inverse()
         assert np.allclose(Minv @ Ac.dense(), np.eye(m), atol=1e-10)
+
+
+def _aniso5(nx, ny, eps):
+    """5-point operator -eps u_xx - u_yy on an nx x ny grid, natural order
+    i = x + nx y, constant diagonal 2 (1 + eps) (boundary rows eliminated)"""
+    n = nx * ny
+    D = np.zeros((n, n))
+    for y in range(ny):
+        for x in range(nx):
+            i = x + nx * y
+            D[i, i] = 2.0 * (1.0 + eps)
+            if x > 0:
+                D[i, i - 1] = -eps
+            if x + 1 < nx:
+                D[i, i + 1] = -eps
+            if y > 0:
+                D[i, i - nx] = -1.0
+            if y + 1 < ny:
+                D[i, i + nx] = -1.0
+    return D
+
+
+def test_project_anisotropic_semi_coarsening():
+    """eps = 0.1: the x couplings are weak (0.1 < 0.25 * 1), so only the y
+    couplings are strong (src/AMGSolver.cpp:43-46,55) and the greedy split
+    makes every even grid line y = 2j coarse.  An F point keeps its two C
+    neighbours above and below, and its weak x couplings are added to the
+    diagonal (:181-185): weight 1 / (2 (1 + eps) - eps * (#x neighbours)),
+    which is exactly 1/2 inside and 1/(2 + eps) on the left and right edges."""
+    nx, ny, eps = 7, 9, 0.1
+    D = _aniso5(nx, ny, eps)
+    g = orc.Amg(_csr_from_dense(D), nmpo=nx * ny - 1, nmpsd=0.0)
+    P = g.P(0).dense()
+    coarse = [x + nx * y for y in range(0, ny, 2) for x in range(nx)]
+    assert P.shape == (len(coarse), nx * ny)
+    Pexp = np.zeros_like(P)
+    for c, k in enumerate(coarse):
+        x, y = k % nx, k // nx
+        Pexp[c, k] = 1.0
+        w = 0.5 if 0 < x < nx - 1 else 1.0 / (2.0 + eps)
+        for yy in (y - 1, y + 1):
+            if 0 <= yy < ny:
+                Pexp[c, x + nx * yy] = w
+    assert np.allclose(P, Pexp, rtol=1e-14, atol=0)
+    # the pattern of P is exactly the nonzeros: C point, its F neighbours
+    rp, col, _ = g.P(0).csr()
+    for c in range(len(coarse)):
+        assert list(col[rp[c]:rp[c + 1]]) == list(np.nonzero(Pexp[c])[0])
+
+
+@pytest.mark.parametrize("eps", [0.25, 0.3])
+def test_project_anisotropic_full_coarsening(eps):
+    """eps >= 0.25: every coupling is strong.  eps = 0.25 sits exactly on the
+    threshold 0.25 * max|a_ij| = 0.25, which counts as strong because the
+    reference tests |a_ij| < beta for weak (src/AMGSolver.cpp:55,99,182): a
+    '<=' there would turn this case into semi-coarsening.  The greedy split
+    is the checkerboard x + y even; F weights are |a_il| / (2 (1 + eps))
+    (no weak couplings), 1/(2+2eps) vertically and eps/(2+2eps) horizontally."""
+    nx, ny = 6, 5
+    D = _aniso5(nx, ny, eps)
+    g = orc.Amg(_csr_from_dense(D), nmpo=nx * ny - 1, nmpsd=0.0)
+    P = g.P(0).dense()
+    coarse = [i for i in range(nx * ny) if (i % nx + i // nx) % 2 == 0]
+    assert P.shape == (len(coarse), nx * ny)
+    Pexp = np.zeros_like(P)
+    for c, k in enumerate(coarse):
+        Pexp[c, k] = 1.0
+        for j in np.nonzero(D[k])[0]:
+            if j != k:
+                Pexp[c, j] = abs(D[k, j]) / (2.0 * (1.0 + eps))
+    assert np.allclose(P, Pexp, rtol=1e-14, atol=0)
+    assert np.allclose(g.A(1).dense(), Pexp @ D @ Pexp.T, rtol=1e-13, atol=1e-15)
+
+
+def _ff_matrix():
+    """6 x 6 symmetric matrix worked out by hand below"""
+    A = np.zeros((6, 6))
+    for i, d in enumerate([4.0, 6.0, 4.0, 7.0, 3.0, 3.0]):
+        A[i, i] = d
+    for i, j, v in [(0, 2, -1.0), (0, 3, -1.0), (0, 4, -1.0), (1, 2, -1.0), (1, 3, -3.0),
+                    (1, 5, -1.0), (2, 3, -2.0), (2, 5, -0.25), (4, 5, -1.0)]:
+        A[i, j] = A[j, i] = v
+    return A
+
+
+def test_project_ff_promotion_and_distance_two_weights():
+    """Hand-derived Project of a 6-node matrix (src/AMGSolver.cpp:18-226).
+
+    Thresholds beta_i = 0.25 max_j |a_ij|: rows 0, 4, 5 -> 0.25, row 1 ->
+    0.75, row 2 -> 0.5, row 3 -> 0.75.
+    Step 1 (greedy, index order): 0 is C and marks its strong neighbours
+    2, 3, 4 as F; 1 is C and marks 2, 3, 5.
+    Step 2 (F-F check, :71-122): 2 has strong C neighbours {0, 1} and a strong
+    F neighbour 3 whose row holds C point 0 -> stays F (its coupling to 5,
+    0.25 < 0.5, is weak).  3 only sees F neighbour 2 < 3 -> stays F.  4 has
+    strong C neighbours {0} and strong F neighbour 5 whose row holds C point
+    1 only -> F-F without a common C point: 4 becomes C.  5 then has strong C
+    neighbours {1, 4} -> stays F.  C = {0, 1, 4}, F = {2, 3, 5}.
+    P rows: C point k plus the F points of its row (:150-160):
+    {0, 2, 3}, {1, 2, 3, 5}, {4, 5}.
+    Weights (:171-215), a~ = a_ii + sum of weak a_ij:
+    node 2: a~ = 4 - 0.25 = 3.75; C neighbours 0, 1 get 1/3.75 each; F
+      neighbour 3 (common C points {0, 1}, d = a30 + a31 = -4) adds
+      |a23 a30 / (d a~)| = 2/15 to C0 and |a23 a31 / (d a~)| = 6/15 to C1:
+      P(0,2) = 4/15 + 2/15 = 2/5, P(1,2) = 4/15 + 6/15 = 2/3.
+    node 3: a~ = 7; C neighbours: 1/7 to C0, 3/7 to C1; F neighbour 2
+      (common {0, 1}, d = a20 + a21 = -2) adds 2/14 to each:
+      P(0,3) = 2/7, P(1,3) = 4/7.
+    node 5: |a52| = 0.25 equals its threshold 0.25, so the coupling is
+      strong (weak means < beta) and a~ = 3.  C neighbours 1 and 4 get 1/3
+      each; F neighbour 2 (common C point {1}, d = a21 = -1) adds
+      |a52 a21 / (d a~)| = 1/12 to C1: P(1,5) = 5/12, P(2,5) = 1/3."""
+    A = _ff_matrix()
+    g = orc.Amg(_csr_from_dense(A), nmpo=3, nmpsd=0.0)
+    assert g.n_project == 1
+    rp, col, val = g.P(0).csr()
+    assert list(rp) == [0, 3, 7, 9]
+    assert list(col) == [0, 2, 3, 1, 2, 3, 5, 4, 5]
+    exp = [1.0, 2 / 5, 2 / 7, 1.0, 2 / 3, 4 / 7, 5 / 12, 1.0, 1 / 3]
+    assert np.allclose(val, exp, rtol=1e-15, atol=0)
+    Pexp = g.P(0).dense()
+    # coarse operator: diagonal first, then ascending (:306-336)
+    rp1, col1, val1 = g.A(1).csr()
+    assert list(col1) == [0, 1, 2, 1, 0, 2, 2, 0, 1]
+    assert np.allclose(g.A(1).dense(), Pexp @ A @ Pexp.T, rtol=1e-14, atol=0)
+
+
+def _lex_first_mis(adj):
+    """lexicographically first maximal independent set, by enumeration"""
+    n = len(adj)
+    nb = [sum(1 << j for j in adj[i]) for i in range(n)]
+    best = None
+    for S in range(1 << n):
+        ok = True
+        cover = S
+        for i in range(n):
+            if S >> i & 1:
+                if nb[i] & S:
+                    ok = False
+                    break
+                cover |= nb[i]
+        if not ok or cover != (1 << n) - 1:
+            continue
+        t = tuple(i for i in range(n) if S >> i & 1)
+        if best is None or t < best:
+            best = t
+    return best
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_lazy_project_core_set_is_lex_first_mis(seed):
+    """lazyProject (src/AMGSolver.cpp:235-284): its core points are the
+    lexicographically first maximal independent set of the pattern graph
+    (found here by enumerating all subsets); P's row for core k is k's closed
+    neighbourhood with weight 1 at k and 1/#(core neighbours of l) at l."""
+    rng = np.random.default_rng(100 + seed)
+    n = 13
+    U = np.triu(rng.random((n, n)) < 0.28, 1)
+    adj = [sorted(set(np.nonzero(U[i])[0]) | set(np.nonzero(U[:, i])[0])) for i in range(n)]
+    D = np.zeros((n, n))
+    for i in range(n):
+        for j in adj[i]:
+            D[i, j] = -rng.uniform(0.5, 1.5)
+    D = 0.5 * (D + D.T)
+    np.fill_diagonal(D, 1.0 + np.abs(D).sum(axis=1))
+    g = orc.Amg(_csr_from_dense(D), lazy=True, nmpo=n - 1, nmpsd=0.0)
+    cores = _lex_first_mis(adj)
+    assert g.n_project >= 1 or len(cores) == n
+    if len(cores) == n:
+        return
+    rp, col, val = g.P(0).csr()
+    assert len(rp) - 1 == len(cores)
+    for c, k in enumerate(cores):
+        nbhd = sorted([k] + adj[k])
+        assert list(col[rp[c]:rp[c + 1]]) == nbhd
+        for q in range(rp[c], rp[c + 1]):
+            l = col[q]
+            w = 1.0 if l == k else 1.0 / len(set(adj[l]) & set(cores))
+            assert val[q] == w
+
+
+@pytest.mark.parametrize("lazy", [False, True])
+@pytest.mark.parametrize("kind,n,acc", [(1, 8, 4), (2, 6, 6), (3, 3, 2)])
+def test_galerkin_pattern_is_structural_product(lazy, kind, n, acc):
+    """getPMPt (src/AMGSolver.cpp:288-358): the pattern of P A P^T is the
+    structural product of the three patterns (entries whose values cancel
+    stay as explicit zeros), stored diagonal first and then ascending."""
+    import scipy.sparse as sps
+    c, ev, vm = mesh(kind, n, 0.0, 0)
+    s = orc.Space(kind, c, ev, vm)
+    A = s.build(orc.OP_STIFF, acc)
+    g = orc.Amg(A, lazy=lazy, nmpo=10, nmpsd=0.0)
+    zeros = 0
+    for l in range(g.n_project):
+        def pat(M):
+            rp, col, _ = M.csr()
+            return sps.csr_matrix((np.ones(len(col)), col, rp), shape=(M.m, M.n))
+        S = (pat(g.P(l)) @ pat(g.A(l)) @ pat(g.Pt(l))).tocsr()
+        S.sort_indices()
+        rp, col, val = g.A(l + 1).csr()
+        assert np.array_equal(rp, S.indptr)
+        for i in range(len(rp) - 1):
+            row = list(col[rp[i]:rp[i + 1]])
+            assert row[0] == i and row[1:] == sorted(row[1:])
+            assert sorted(row) == list(S.indices[S.indptr[i]:S.indptr[i + 1]])
+        zeros += int((val == 0).sum())
+    # the meshes are chosen so that some coarse entries cancel exactly
+    assert zeros > 0
 
 
 def test_gauss_seidel_is_triangular_solve():
