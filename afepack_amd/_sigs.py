@@ -53,6 +53,7 @@ SIGS = {
     "afe_mesh_displace": (C.c_int, [I32, I64, P, P, F64, I32]),
     "afe_field_random": (C.c_int, [I32, I32, F64, U64, P, P, P, PF64]),
     "afe_space_create": (C.c_int, [P, I32, I64, P, I64, P, P, PP]),
+    "afe_space_create_sides": (C.c_int, [P, I32, I64, P, I64, P, P, I64, P, P, PP]),
     "afe_space_destroy": (C.c_int, [P]),
     "afe_space_info": (C.c_int, [P, PI64, PI64, PI32]),
     "afe_space_dofs": (C.c_int, [P, P, P, P]),
@@ -67,6 +68,7 @@ SIGS = {
     "afe_matrix_set_values": (C.c_int, [P, P]),
     "afe_matrix_device_values": (C.c_void_p, [P]),
     "afe_matrix_vmult": (C.c_int, [P, P, P]),
+    "afe_matrix_tvmult": (C.c_int, [P, P, P]),
     "afe_rhs": (C.c_int, [P, I32, I32, PField, P, P]),
     "afe_apply_dirichlet": (C.c_int, [P, P, P]),
     "afe_space_qpoints": (C.c_int, [P, I32, PI64, P, P]),
@@ -89,8 +91,10 @@ SIGS = {
     "afe_amg_pcg": (C.c_int, [P, P, P, F64, I32, I32, C.POINTER(SolveStats)]),
     "afe_amg_vcycle": (C.c_int, [P, P, P, I32]),
     "afe_amg_set_smooth": (C.c_int, [P, I32]),
+    "afe_amg_set_coarse_exact": (C.c_int, [P, I32]),
     "afe_set_tuning": (C.c_int, [I32, I32, I64]),
     "afe_amg_kernel_timing": (C.c_int, [P, I32, I32, PF64, PF64]),
+    "afe_assemble_kernel_timing": (C.c_int, [P, I32, I32, P, I32, PF64, PF64, PF64]),
     "afe_copy_kernel_timing": (C.c_int, [P, I64, I32, PF64]),
     "afe_amg_set_tail_rows": (C.c_int, [P, I64]),
     "afe_amg_live_timing": (C.c_int, [P, I32, PF64, PI64]),

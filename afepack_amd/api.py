@@ -196,7 +196,9 @@ class DeviceVector:
 class Space:
     """FE space (DOF numbering + pattern); ctx=None builds it on the host only."""
 
-    def __init__(self, ctx, kind, coords, elem_vert, vert_mark):
+    def __init__(self, ctx, kind, coords, elem_vert, vert_mark, sides=None):
+        """sides: optional (side_vert (n_side x 2), side_mark (n_side)) of the
+        mesh; P2 edge DOFs then take their side's boundary mark"""
         self.ctx = ctx
         self.kind = kind
         self.dim = 3 if kind == P1_TET else 2
@@ -206,8 +208,15 @@ class Space:
         nv = len(self.vert_mark)
         ne = len(self.elem_vert) // (4 if kind == P1_TET else 3)
         h = C.c_void_p()
-        _chk(_lib().afe_space_create(ctx.h if ctx else None, kind, nv, _ptr(self.coords), ne,
-                                     _ptr(self.elem_vert), _ptr(self.vert_mark), C.byref(h)))
+        if sides is None:
+            _chk(_lib().afe_space_create(ctx.h if ctx else None, kind, nv, _ptr(self.coords), ne,
+                                         _ptr(self.elem_vert), _ptr(self.vert_mark), C.byref(h)))
+        else:
+            sv = np.ascontiguousarray(sides[0], dtype=np.int32).ravel()
+            sm = np.ascontiguousarray(sides[1], dtype=np.int32)
+            _chk(_lib().afe_space_create_sides(ctx.h if ctx else None, kind, nv, _ptr(self.coords), ne,
+                                               _ptr(self.elem_vert), _ptr(self.vert_mark), len(sm), _ptr(sv),
+                                               _ptr(sm), C.byref(h)))
         self.h = h
         self.n_vert, self.n_elem = nv, ne
         nd, nz, nl = C.c_int64(), C.c_int64(), C.c_int32()
@@ -332,6 +341,17 @@ class Matrix:
     def vmult(self, x, y):
         _chk(_lib().afe_matrix_vmult(self.h, _ptr(x), _ptr(y)))
 
+    def tvmult(self, x, y):
+        _chk(_lib().afe_matrix_tvmult(self.h, _ptr(x), _ptr(y)))
+
+    def assemble_timing(self, op, acc, field=None, reps=10):
+        """(element kernel ms, gather kernel ms, algorithmic bytes) of afe_assemble"""
+        fp, _keep = _field_ptr(field)
+        te, tg, by = C.c_double(), C.c_double(), C.c_double()
+        _chk(_lib().afe_assemble_kernel_timing(self.h, op, acc, fp, reps, C.byref(te), C.byref(tg),
+                                               C.byref(by)))
+        return te.value, tg.value, by.value
+
     def close(self):
         if self.h:
             _lib().afe_matrix_destroy(self.h)
@@ -445,6 +465,11 @@ class Amg(_AmgBase):
 
     def set_smooth(self, s):
         _chk(_lib().afe_amg_set_smooth(self.h, s))
+
+    def set_coarse_exact(self, exact):
+        """AMGSolver::isSolveMostProjectExactly(): False smooths the coarsest
+        level with Gauss-Seidel sweeps instead of solving it exactly"""
+        _chk(_lib().afe_amg_set_coarse_exact(self.h, 1 if exact else 0))
 
     def live_timing(self, enable):
         """Start (True) or stop (False) live timing of the CG's level-0 SpMV;

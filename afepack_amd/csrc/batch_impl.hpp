@@ -433,12 +433,7 @@ template <class T> static void batch_step_impl(afe_batch *b, int *n_active) {
   b->t_epoch += std::chrono::duration<double>(t1 - t0).count();
   if (use_graphs()) {
     if (!b->graph) {
-      hipGraph_t graph;
-      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      batch_issue_step<T>(b, s);
-      HIPCHK(hipStreamEndCapture(s, &graph));
-      HIPCHK(hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0));
-      HIPCHK(hipGraphDestroy(graph));
+      b->graph = capture_graph(s, [&]() { batch_issue_step<T>(b, s); });
       b->t_graph += std::chrono::duration<double>(clk::now() - t1).count();
     }
     HIPCHK(hipGraphLaunch(b->graph, s));

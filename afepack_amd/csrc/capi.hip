@@ -7,6 +7,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <mutex>
 #include <thread>
 #include <cmath>
 #include <cstdio>
@@ -79,6 +80,7 @@ struct afe_ctx {
   std::recursive_mutex mu;
   int64_t bytes = 0;
   double *d_quad = nullptr; // all quadrature rules
+  double *d_bary = nullptr; // reference barycentrics of every rule point (same indexing)
   int64_t lds_bytes = 0;    // LDS granted to the coarse-tail workgroup
   std::map<void *, int64_t> user_bytes; // afe_dev_alloc buffers
 };
@@ -144,6 +146,7 @@ struct Tuning {
   int spmv_blocks = 2048;
   int64_t tail_rows = 4096;          // levels at most this size run in the tail
   int64_t stream_bytes = 128 << 20;  // matrices above this use streaming loads
+  uint64_t epoch = 0;                // bumped by every afe_set_tuning (captured graphs go stale)
 };
 static Tuning g_tune;
 
@@ -202,6 +205,24 @@ extern "C" int afe_ctx_create(int device, afe_ctx **out) {
   HIPCHK(hipMalloc((void **)&c->d_quad, sizeof(kQuadData)));
   c->bytes += sizeof(kQuadData);
   HIPCHK(hipMemcpy(c->d_quad, kQuadData, sizeof(kQuadData), hipMemcpyHostToDevice));
+  {
+    // CoordTransform's barycentric coordinates of each rule point, computed
+    // once here with the arithmetic the kernels would repeat per element
+    std::vector<double> bary(sizeof(kQuadData) / sizeof(double), 0.0);
+    for (int r = 0; r < kNumQuadRules; r++) {
+      const QuadRule &q = kQuadRules[r];
+      for (int l = 0; l < q.n; l++) {
+        const int o = q.off + l * (q.tdim + 1);
+        if (q.tdim == 2)
+          ref_bary<2>(&kQuadData[o], &bary[o]);
+        else
+          ref_bary<3>(&kQuadData[o], &bary[o]);
+      }
+    }
+    HIPCHK(hipMalloc((void **)&c->d_bary, sizeof(kQuadData)));
+    c->bytes += sizeof(kQuadData);
+    HIPCHK(hipMemcpy(c->d_bary, bary.data(), sizeof(kQuadData), hipMemcpyHostToDevice));
+  }
   *out = c;
   API_END
 }
@@ -212,6 +233,7 @@ extern "C" int afe_ctx_destroy(afe_ctx *c) {
   (void)hipStreamSynchronize(c->stream);
   for (auto &kv : c->user_bytes) (void)hipFree(kv.first);
   (void)hipFree(c->d_quad);
+  (void)hipFree(c->d_bary);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
   (void)hipStreamDestroy(c->stream);
@@ -378,17 +400,19 @@ struct afe_space {
   DevBuf<uint8_t> slot;
 };
 
-extern "C" int afe_space_create(afe_ctx *ctx, int32_t kind, int64_t n_vert, const double *coords,
-                                int64_t n_elem, const int32_t *elem_vert, const int32_t *vert_mark,
-                                afe_space **out) {
+extern "C" int afe_space_create_sides(afe_ctx *ctx, int32_t kind, int64_t n_vert, const double *coords,
+                                      int64_t n_elem, const int32_t *elem_vert, const int32_t *vert_mark,
+                                      int64_t n_side, const int32_t *side_vert, const int32_t *side_mark,
+                                      afe_space **out) {
   API_BEGIN
   if (!out || !coords || !elem_vert || !vert_mark || n_vert <= 0 || n_elem <= 0)
     return fail(AFE_E_ARG, "bad space arguments");
   if (kind != AFE_P1_TRI && kind != AFE_P2_TRI && kind != AFE_P1_TET)
     return fail(AFE_E_ARG, "unknown element kind");
+  if (n_side < 0 || (n_side > 0 && (!side_vert || !side_mark))) return fail(AFE_E_ARG, "bad side list");
   auto s = std::make_unique<afe_space>();
   s->ctx = ctx;
-  space_build(s->h, kind, n_vert, coords, n_elem, elem_vert, vert_mark);
+  space_build(s->h, kind, n_vert, coords, n_elem, elem_vert, vert_mark, n_side, side_vert, side_mark);
   if (!ctx) { // host-only space: numbering and pattern, nothing uploaded
     *out = s.release();
     return AFE_OK;
@@ -413,6 +437,12 @@ extern "C" int afe_space_create(afe_ctx *ctx, int32_t kind, int64_t n_vert, cons
   HIPCHK(hipStreamSynchronize(ctx->stream));
   *out = s.release();
   API_END
+}
+
+extern "C" int afe_space_create(afe_ctx *ctx, int32_t kind, int64_t n_vert, const double *coords,
+                                int64_t n_elem, const int32_t *elem_vert, const int32_t *vert_mark,
+                                afe_space **out) {
+  return afe_space_create_sides(ctx, kind, n_vert, coords, n_elem, elem_vert, vert_mark, 0, nullptr, nullptr, out);
 }
 
 extern "C" int afe_space_destroy(afe_space *s) {
@@ -482,6 +512,9 @@ struct afe_matrix {
   DevBuf<int32_t> own_col, own_mark;
   DevBuf<double> val;
   DevBuf<double> Ke; // element matrices of the last assembly
+  int ke_layout = 0;  // how Ke holds them (k_gather's LAYOUT)
+  DevBuf<int64_t> t_ptr, t_k; // transposed index map (afe_matrix_tvmult)
+  DevBuf<int32_t> t_row;
   const int64_t *rp() const { return s ? s->rp.p : own_rp.p; }
   const int32_t *col() const { return s ? s->col.p : own_col.p; }
   const std::vector<int64_t> &hrp() const { return s ? s->h.rp : h_rp; }
@@ -571,31 +604,39 @@ static void launch_elements(afe_matrix *a, int op, const QuadRule *q, FieldDev f
   afe_space *s = a->s;
   const int64_t ne = s->h.n_elem;
   const double *qd = s->ctx->d_quad + q->off;
+  const double *qb = s->ctx->d_bary + q->off;
   hipStream_t st = s->ctx->stream;
   if (op == AFE_OP_STIFF)
-    k_element_matrices<KIND, 0><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, qd, q->n, fd, a->Ke.p);
+    k_element_matrices<KIND, 0><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, qd, qb, q->n, fd, a->Ke.p);
   else if (op == AFE_OP_MASS)
-    k_element_matrices<KIND, 1><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, qd, q->n, fd, a->Ke.p);
+    k_element_matrices<KIND, 1><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, qd, qb, q->n, fd, a->Ke.p);
   else
-    k_element_matrices<KIND, 2><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, qd, q->n, fd, a->Ke.p);
+    k_element_matrices<KIND, 2><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, qd, qb, q->n, fd, a->Ke.p);
   HIPCHK(hipGetLastError());
+  a->ke_layout = op == AFE_OP_MASS ? 2 : 1;
 }
 
-static void launch_gather(afe_matrix *a) {
+template <int NL, int LAYOUT> static void launch_gather_nl(afe_matrix *a) {
   afe_space *s = a->s;
   const int64_t n = s->h.n_dof;
   const size_t lds = (size_t)s->seg_max * sizeof(double);
-  hipStream_t st = s->ctx->stream;
-  switch (s->h.n_loc) {
-  case 3:
-    k_gather<3><<<nblk(n), 256, lds, st>>>(n, s->rp.p, s->adj_ptr.p, s->adj.p, s->slot.p, a->Ke.p, a->val.p);
-    break;
-  case 4:
-    k_gather<4><<<nblk(n), 256, lds, st>>>(n, s->rp.p, s->adj_ptr.p, s->adj.p, s->slot.p, a->Ke.p, a->val.p);
-    break;
-  default:
-    k_gather<6><<<nblk(n), 256, lds, st>>>(n, s->rp.p, s->adj_ptr.p, s->adj.p, s->slot.p, a->Ke.p, a->val.p);
-    break;
+  k_gather<NL, LAYOUT><<<nblk(n), 256, lds, s->ctx->stream>>>(n, s->rp.p, s->adj_ptr.p, s->adj.p, s->slot.p,
+                                                              s->h.n_elem, a->Ke.p, a->val.p);
+}
+
+template <int NL> static void launch_gather_layout(afe_matrix *a) {
+  switch (a->ke_layout) {
+  case 0: launch_gather_nl<NL, 0>(a); break;
+  case 1: launch_gather_nl<NL, 1>(a); break;
+  default: launch_gather_nl<NL, 2>(a); break;
+  }
+}
+
+static void launch_gather(afe_matrix *a) {
+  switch (a->s->h.n_loc) {
+  case 3: launch_gather_layout<3>(a); break;
+  case 4: launch_gather_layout<4>(a); break;
+  default: launch_gather_layout<6>(a); break;
   }
   HIPCHK(hipGetLastError());
 }
@@ -624,6 +665,58 @@ extern "C" int afe_assemble(afe_matrix *a, int32_t op, int32_t acc, const afe_fi
   API_END
 }
 
+// average durations of the two assembly kernels of afe_assemble (element
+// matrices, CSR gather) over `reps` assemblies, from HIP events on the
+// context stream, and the assembly's algorithmic bytes: vertex coordinates,
+// element vertex and DOF indices read, CSR values written
+extern "C" int afe_assemble_kernel_timing(afe_matrix *a, int32_t op, int32_t acc, const afe_field *coef,
+                                          int32_t reps, double *elem_ms, double *gather_ms, double *bytes) {
+  API_BEGIN
+  if (!a || !a->s || reps < 1) return fail(AFE_E_ARG, "bad arguments");
+  if (op < AFE_OP_STIFF || op > AFE_OP_VARSTIFF) return fail(AFE_E_ARG, "unknown operator");
+  if (op == AFE_OP_VARSTIFF && (!coef || coef->n_modes <= 0 || coef->dim != a->s->h.tdim))
+    return fail(AFE_E_ARG, "variable coefficient needs a field of the space dimension");
+  afe_space *s = a->s;
+  const QuadRule *q = quad_find(s->h.tdim, acc);
+  if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
+  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  HIPCHK(hipSetDevice(a->ctx->device));
+  DevBuf<double> fstore;
+  FieldDev fd = upload_field(a->ctx, op == AFE_OP_VARSTIFF ? coef : nullptr, fstore);
+  a->Ke.alloc(a->ctx, (size_t)s->h.n_elem * s->h.n_loc * s->h.n_loc);
+  hipStream_t st = a->ctx->stream;
+  hipEvent_t ev[3];
+  for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+  double te = 0, tg = 0;
+  for (int r = 0; r < reps; r++) {
+    HIPCHK(hipEventRecord(ev[0], st));
+    switch (s->h.kind) {
+    case P1_TRI: launch_elements<P1_TRI>(a, op, q, fd); break;
+    case P2_TRI: launch_elements<P2_TRI>(a, op, q, fd); break;
+    default: launch_elements<P1_TET>(a, op, q, fd); break;
+    }
+    HIPCHK(hipEventRecord(ev[1], st));
+    launch_gather(a);
+    HIPCHK(hipEventRecord(ev[2], st));
+    HIPCHK(hipEventSynchronize(ev[2]));
+    float m0 = 0, m1 = 0;
+    HIPCHK(hipEventElapsedTime(&m0, ev[0], ev[1]));
+    HIPCHK(hipEventElapsedTime(&m1, ev[1], ev[2]));
+    te += m0;
+    tg += m1;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  if (elem_ms) *elem_ms = te / reps;
+  if (gather_ms) *gather_ms = tg / reps;
+  if (bytes) {
+    const Space &h = s->h;
+    const int nv = h.kind == P1_TET ? 4 : 3;
+    *bytes = (double)h.coords.size() * 8.0 + (double)h.n_elem * (nv + h.n_loc) * 4.0 +
+             (double)h.rp[h.n_dof] * 8.0;
+  }
+  API_END
+}
+
 extern "C" int afe_assemble_elements(afe_matrix *a, const double *h_elem_mat) {
   API_BEGIN
   if (!a || !a->s || !h_elem_mat) return fail(AFE_E_ARG, "null argument");
@@ -631,6 +724,7 @@ extern "C" int afe_assemble_elements(afe_matrix *a, const double *h_elem_mat) {
   HIPCHK(hipSetDevice(a->ctx->device));
   const Space &h = a->s->h;
   a->Ke.upload(a->ctx, h_elem_mat, (size_t)h.n_elem * h.n_loc * h.n_loc);
+  a->ke_layout = 0;
   launch_gather(a);
   HIPCHK(hipStreamSynchronize(a->ctx->stream));
   API_END
@@ -658,6 +752,12 @@ extern "C" int afe_matrix_set_values(afe_matrix *a, const double *h_val) {
 
 extern "C" double *afe_matrix_device_values(afe_matrix *a) { return a ? a->val.p : nullptr; }
 
+// SparseMatrix::vmult / Tvmult (SparseMatrix.h:100-132): products summed in
+// the reference's order without contraction, so the results are the
+// reference's bit for bit.  Tvmult's scatter y[col[k]] += v[k] x[i] over rows
+// i in ascending order is a gather here: column j sums its entries in
+// ascending row order through the transposed index map (tptr, trow, tk).
+#pragma clang fp contract(off)
 __global__ __launch_bounds__(256) void k_csr_vmult(int64_t m, const int64_t *__restrict__ rp,
                                                    const int32_t *__restrict__ col,
                                                    const double *__restrict__ val,
@@ -667,6 +767,50 @@ __global__ __launch_bounds__(256) void k_csr_vmult(int64_t m, const int64_t *__r
   double s = 0.0;
   for (int64_t k = rp[i]; k < rp[i + 1]; k++) s += val[k] * x[col[k]];
   y[i] = s;
+}
+
+__global__ __launch_bounds__(256) void k_csr_tvmult(int64_t n, const int64_t *__restrict__ tptr,
+                                                    const int32_t *__restrict__ trow,
+                                                    const int64_t *__restrict__ tk,
+                                                    const double *__restrict__ val,
+                                                    const double *__restrict__ x, double *__restrict__ y) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double s = 0.0;
+  for (int64_t t = tptr[j]; t < tptr[j + 1]; t++) s += val[tk[t]] * x[trow[t]];
+  y[j] = s;
+}
+#pragma clang fp contract(on)
+
+extern "C" int afe_matrix_tvmult(afe_matrix *a, const double *d_x, double *d_y) {
+  API_BEGIN
+  if (!a || !d_x || !d_y) return fail(AFE_E_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  HIPCHK(hipSetDevice(a->ctx->device));
+  if (!a->t_ptr.p) {
+    // transposed index map of the pattern, built once per matrix: column j
+    // lists (row, CSR position) in ascending row order
+    const std::vector<int64_t> &rp = a->hrp();
+    const std::vector<int32_t> &col = a->hcol();
+    std::vector<int64_t> tptr((size_t)a->n + 1, 0), tk((size_t)a->nnz);
+    std::vector<int32_t> trow((size_t)a->nnz);
+    for (int64_t k = 0; k < a->nnz; k++) tptr[(size_t)col[k] + 1]++;
+    for (int64_t j = 0; j < a->n; j++) tptr[j + 1] += tptr[j];
+    std::vector<int64_t> fill(tptr.begin(), tptr.end() - 1);
+    for (int64_t i = 0; i < a->m; i++)
+      for (int64_t k = rp[i]; k < rp[i + 1]; k++) {
+        const int64_t t = fill[col[k]]++;
+        trow[t] = (int32_t)i;
+        tk[t] = k;
+      }
+    a->t_ptr.upload(a->ctx, tptr);
+    a->t_row.upload(a->ctx, trow);
+    a->t_k.upload(a->ctx, tk);
+  }
+  k_csr_tvmult<<<nblk(a->n), 256, 0, a->ctx->stream>>>(a->n, a->t_ptr.p, a->t_row.p, a->t_k.p, a->val.p, d_x, d_y);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(a->ctx->stream));
+  API_END
 }
 
 extern "C" int afe_matrix_vmult(afe_matrix *a, const double *d_x, double *d_y) {
@@ -815,6 +959,31 @@ extern "C" int afe_apply_dirichlet_values(afe_matrix *a, const int32_t *marks, i
   API_END
 }
 
+// the error functional's kernels: per-block partial sums of the squared
+// errors into part (RED_BLOCKS doubles), then sqrt of their sum into d_err
+static void launch_error_qp(afe_space *s, const QuadRule *q, int h1, const double *d_uh, const double *d_exact,
+                            double *part, double *d_err) {
+  const int64_t ne = s->h.n_elem;
+  const double *qd = s->ctx->d_quad + q->off;
+  hipStream_t st = s->ctx->stream;
+  switch (s->h.kind) {
+  case P1_TRI:
+    k_error_qp_part<P1_TRI><<<RED_BLOCKS, 256, 0, st>>>(ne, s->coords.p, s->ev.p, s->elem_dof.p, qd, q->n, h1, d_uh,
+                                                        d_exact, part);
+    break;
+  case P2_TRI:
+    k_error_qp_part<P2_TRI><<<RED_BLOCKS, 256, 0, st>>>(ne, s->coords.p, s->ev.p, s->elem_dof.p, qd, q->n, h1, d_uh,
+                                                        d_exact, part);
+    break;
+  default:
+    k_error_qp_part<P1_TET><<<RED_BLOCKS, 256, 0, st>>>(ne, s->coords.p, s->ev.p, s->elem_dof.p, qd, q->n, h1, d_uh,
+                                                        d_exact, part);
+    break;
+  }
+  k_finalize_sqrt<<<1, FIN_THREADS, 0, st>>>(part, RED_BLOCKS, d_err);
+  HIPCHK(hipGetLastError());
+}
+
 extern "C" int afe_error_qp(afe_space *s, const double *d_uh, int32_t acc, int32_t h1, const double *h_exact,
                             double *err) {
   API_BEGIN
@@ -824,24 +993,13 @@ extern "C" int afe_error_qp(afe_space *s, const double *d_uh, int32_t acc, int32
   std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
   HIPCHK(hipSetDevice(s->ctx->device));
   const int64_t ne = s->h.n_elem;
-  DevBuf<double> ex, pe, part, tot;
+  DevBuf<double> ex, part, tot;
   ex.upload(s->ctx, h_exact, (size_t)ne * q->n * (h1 ? s->h.tdim : 1));
-  pe.alloc(s->ctx, ne);
   part.alloc(s->ctx, RED_BLOCKS);
   tot.alloc(s->ctx, 1);
-  const double *qd = s->ctx->d_quad + q->off;
-  hipStream_t st = s->ctx->stream;
-  switch (s->h.kind) {
-  case P1_TRI: k_element_error_qp<P1_TRI><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, s->elem_dof.p, qd, q->n, h1, d_uh, ex.p, pe.p); break;
-  case P2_TRI: k_element_error_qp<P2_TRI><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, s->elem_dof.p, qd, q->n, h1, d_uh, ex.p, pe.p); break;
-  default: k_element_error_qp<P1_TET><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, s->elem_dof.p, qd, q->n, h1, d_uh, ex.p, pe.p); break;
-  }
-  k_sum_partial<<<RED_BLOCKS, 256, 0, st>>>(ne, pe.p, part.p);
-  k_finalize<<<1, FIN_THREADS, 0, st>>>(part.p, RED_BLOCKS, 1, tot.p);
-  double h = 0;
-  HIPCHK(hipMemcpyAsync(&h, tot.p, sizeof(double), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  *err = std::sqrt(std::fabs(h));
+  launch_error_qp(s, q, h1, d_uh, ex.p, part.p, tot.p);
+  HIPCHK(hipMemcpyAsync(err, tot.p, sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
   API_END
 }
 
@@ -922,17 +1080,25 @@ struct DevLevel {
 struct afe_amg {
   afe_ctx *ctx = nullptr;
   afe_matrix *A0 = nullptr;
+  // the pattern the hierarchy was built on, for the same-pattern rule of
+  // lazyReinit: the space of a space matrix, else the CSR matrix object.
+  // Kept apart from A0 so that the check never reads a matrix the caller
+  // has destroyed since.
+  const afe_space *key_space = nullptr;
+  const afe_matrix *key_mat = nullptr;
+  uint64_t graph_epoch = 0; // g_tune.epoch when the graphs below were captured
   // live timing of the CG's level-0 SpMV inside the solves (events recorded
   // around it in every iteration, summed after each iteration's sync)
   bool live = false;
   hipEvent_t live0 = nullptr, live1 = nullptr;
   double live_ms = 0;
-  int64_t live_n = 0;
+  int64_t live_n = 0, live_missed = 0;
   afe_amg_params prm{};
   Hierarchy h;
   std::vector<DevLevel> lv;
   int tail0 = 0;       // first level handled by the single-workgroup tail
   int coarse_mode = 0; // 0 dense inverse, 1 diagonal, 2 GS
+  bool coarse_exact = true; // AMGSolver::isSolveMostProjectExactly()
   bool big_coarse = false; // dense coarsest level solved by k_coarse_gemv
   DevBuf<double> minv;
   DevBuf<LevelDev<double>> ld64;
@@ -968,6 +1134,13 @@ struct afe_amg {
 };
 
 static void drop_graphs(afe_amg *g) {
+  bool any = false;
+  for (auto &row : g->vc_graph)
+    for (auto &e : row) any = any || e;
+  for (auto &row : g->cg_graph)
+    for (auto &e : row) any = any || e;
+  // no executable graph is destroyed while a launch of it may still run
+  if (any && g->ctx) (void)hipStreamSynchronize(g->ctx->stream);
   for (auto &row : g->vc_graph)
     for (auto &e : row)
       if (e) {
@@ -1223,6 +1396,21 @@ static void build_layout(const Hierarchy &h, HostLayout &lay, bool maps) {
 }
 
 // uploads a prepared layout (caller holds the context lock)
+// coarsest-level solve (src/AMGSolver.cpp:590-605): the dense inverse when the
+// coarsening ended on a full matrix, diagonal scaling when it stalled, and
+// Gauss-Seidel sweeps from zero (multicolour, forward colour order, smoothing
+// count per side) when the caller turned the exact solve off
+static void set_coarse_mode(afe_amg *g) {
+  const int L = (int)g->lv.size();
+  if (!g->coarse_exact) {
+    g->coarse_mode = 2;
+    g->big_coarse = false;
+  } else {
+    g->coarse_mode = g->h.full ? 0 : 1;
+    g->big_coarse = g->h.full && g->lv[L - 1].n > COARSE_TAIL_MAX;
+  }
+}
+
 static void upload_layout(afe_amg *g, HostLayout &lay) {
   afe_ctx *ctx = g->ctx;
   const Hierarchy &h = g->h;
@@ -1328,8 +1516,7 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
   }
   // first level of the coarse tail; a large dense coarsest level is solved by
   // the multi-block gemv between the tail's down and up legs
-  g->coarse_mode = h.full ? 0 : 1;
-  g->big_coarse = h.full && g->lv[L - 1].n > COARSE_TAIL_MAX;
+  set_coarse_mode(g);
   g->tail0 = L - 1;
   while (g->tail0 > 0 && g->lv[g->tail0 - 1].n <= g_tune.tail_rows) g->tail0--;
   if (h.full) g->minv.upload(ctx, lay.minv_perm);
@@ -1382,6 +1569,8 @@ extern "C" int afe_amg_create(afe_matrix *a, const afe_amg_params *p, afe_amg **
   auto g = std::make_unique<afe_amg>();
   g->ctx = a->ctx;
   g->A0 = a;
+  g->key_space = a->s;
+  g->key_mat = a->s ? nullptr : a;
   afe_amg_params prm{0.25, 50u, 0.382, 0, 3, 1};
   if (p) prm = *p;
   if (prm.smooth < 0 || prm.nmpo < 1) return fail(AFE_E_ARG, "bad AMG parameters");
@@ -1549,7 +1738,7 @@ extern "C" int afe_amg_lazy_reinit(afe_amg *g, afe_matrix *a) {
   API_BEGIN
   if (!g || !a) return fail(AFE_E_ARG, "null argument");
   if (!g->prm.lazy) return fail(AFE_E_ARG, "hierarchy was not built in lazy mode");
-  if (a->s != g->A0->s || (a->s == nullptr && a != g->A0))
+  if (a->s != g->key_space || (a->s == nullptr && a != g->key_mat))
     return fail(AFE_E_PATTERN, "The solver is not initialized with the same sparsity pattern.");
   if (g->dev_refresh) {
     device_lazy_refresh(g, a);
@@ -1740,6 +1929,27 @@ static void issue_vcycle(afe_amg *g, int nu, bool symmetric, hipStream_t st, boo
   }
 }
 
+// captures what issue() puts on stream st into an executable graph.  If
+// issue() throws, the capture is ended and its partial graph destroyed
+// before the error propagates, so the stream never stays in capture mode.
+template <class F> static hipGraphExec_t capture_graph(hipStream_t st, F issue) {
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  try {
+    issue();
+  } catch (...) {
+    hipGraph_t partial = nullptr;
+    if (hipStreamEndCapture(st, &partial) == hipSuccess && partial) (void)hipGraphDestroy(partial);
+    throw;
+  }
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamEndCapture(st, &graph));
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIPCHK(e);
+  return ex;
+}
+
 // AFE_NO_GRAPH=1 issues the captured sequences as plain launches (for tools
 // that do not follow graph launches)
 static bool use_graphs() {
@@ -1756,21 +1966,22 @@ static void launch_graph_or_issue(afe_amg *g, int prec, bool symmetric, int nu) 
       issue_vcycle<double>(g, nu, symmetric, st, symmetric);
     return;
   }
+  if (g->graph_epoch != g_tune.epoch) { // launch shapes changed since capture
+    drop_graphs(g);
+    g->graph_epoch = g_tune.epoch;
+  }
   hipGraphExec_t &ex = g->vc_graph[prec][symmetric ? 1 : 0];
   if (ex && g->graph_nu[prec][symmetric ? 1 : 0] != nu) {
     (void)hipGraphExecDestroy(ex);
     ex = nullptr;
   }
   if (!ex) {
-    hipGraph_t graph;
-    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    if (prec == 1)
-      issue_vcycle<float>(g, nu, symmetric, st, symmetric);
-    else
-      issue_vcycle<double>(g, nu, symmetric, st, symmetric);
-    HIPCHK(hipStreamEndCapture(st, &graph));
-    HIPCHK(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
-    HIPCHK(hipGraphDestroy(graph));
+    ex = capture_graph(st, [&]() {
+      if (prec == 1)
+        issue_vcycle<float>(g, nu, symmetric, st, symmetric);
+      else
+        issue_vcycle<double>(g, nu, symmetric, st, symmetric);
+    });
     g->graph_nu[prec][symmetric ? 1 : 0] = nu;
   }
   HIPCHK(hipGraphLaunch(ex, st));
@@ -1939,18 +2150,17 @@ template <class F> static void launch_part(afe_amg *g, int part, int prec, F iss
     issue();
     return;
   }
+  if (g->graph_epoch != g_tune.epoch) {
+    drop_graphs(g);
+    g->graph_epoch = g_tune.epoch;
+  }
   hipGraphExec_t &ex = g->cg_graph[part][prec];
   if (ex && g->cg_nu[part][prec] != g->prm.smooth) {
     (void)hipGraphExecDestroy(ex);
     ex = nullptr;
   }
   if (!ex) {
-    hipGraph_t graph;
-    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    issue();
-    HIPCHK(hipStreamEndCapture(st, &graph));
-    HIPCHK(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
-    HIPCHK(hipGraphDestroy(graph));
+    ex = capture_graph(st, issue);
     g->cg_nu[part][prec] = g->prm.smooth;
   }
   HIPCHK(hipGraphLaunch(ex, st));
@@ -2004,10 +2214,17 @@ static int pcg_impl(afe_amg *g, double *d_x, const double *d_b, double rtol, int
       launch_cg_iteration<T>(g);
       HIPCHK(hipEventSynchronize(g->it_ev));
       if (g->live) {
+        // the pair is recorded by the kernel's own dispatch; a tool that
+        // replaces the dispatch's completion signal leaves it unrecorded,
+        // and that launch is then counted as untimed instead of failing
         float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, g->live0, g->live1));
-        g->live_ms += ms;
-        g->live_n++;
+        if (hipEventElapsedTime(&ms, g->live0, g->live1) == hipSuccess) {
+          g->live_ms += ms;
+          g->live_n++;
+        } else {
+          (void)hipGetLastError();
+          g->live_missed++;
+        }
       }
       it++;
       rr = g->h_sc->rr;
@@ -2183,6 +2400,20 @@ extern "C" int afe_amg_set_smooth(afe_amg *g, int32_t smooth) {
   return AFE_OK;
 }
 
+// AMGSolver::isSolveMostProjectExactly() (AMGSolver.h:100-101): exact = 0
+// replaces the coarsest solve by smoothing-count Gauss-Seidel sweeps from zero
+// (src/AMGSolver.cpp:601-605)
+extern "C" int afe_amg_set_coarse_exact(afe_amg *g, int32_t exact) {
+  if (!g || !g->ctx) return fail(AFE_E_ARG, "bad arguments");
+  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  if ((exact != 0) != g->coarse_exact) {
+    drop_graphs(g);
+    g->coarse_exact = exact != 0;
+    set_coarse_mode(g);
+  }
+  return AFE_OK;
+}
+
 // live timing of the level-0 SpMV inside afe_amg_pcg: enable = 1 resets and
 // starts recording, 0 stops; the average so far is returned either way
 extern "C" int afe_amg_live_timing(afe_amg *g, int32_t enable, double *avg_ms, int64_t *launches) {
@@ -2199,6 +2430,7 @@ extern "C" int afe_amg_live_timing(afe_amg *g, int32_t enable, double *avg_ms, i
   if (enable) {
     g->live_ms = 0;
     g->live_n = 0;
+    g->live_missed = 0;
   }
   if ((enable != 0) != g->live) {
     g->live = enable != 0;
@@ -2239,6 +2471,7 @@ extern "C" int afe_set_tuning(int32_t xcd, int32_t spmv_blocks, int64_t tail_row
   g_tune.xcd = xcd;
   g_tune.spmv_blocks = spmv_blocks;
   g_tune.tail_rows = tail_rows;
+  g_tune.epoch++;
   return AFE_OK;
 }
 

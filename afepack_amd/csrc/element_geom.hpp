@@ -85,23 +85,43 @@ template <int D> AFE_HD inline double tmpl_volume() {
   }
 }
 
-// CoordTransform::local_to_global of the reference point lp
-template <int D> AFE_HD inline void local_to_global(const double *lp, const double v[][D], double *x) {
+// CoordTransform::local_to_global of the reference point lp, in two steps:
+// the barycentric coordinates of lp on the reference simplex (area / volume
+// ratios; they do not depend on the element, so a kernel may take them from
+// a table computed once per quadrature rule), then their combination of the
+// physical vertices.  Same operations in the same order as the one-step form.
+template <int D> AFE_HD inline void ref_bary(const double *lp, double *lam) {
   if constexpr (D == 2) {
     const double r0[2] = {0.0, 0.0}, r1[2] = {1.0, 0.0}, r2[2] = {0.0, 1.0};
     const double ar = area2(r0, r1, r2);
-    const double l0 = area2(lp, r1, r2) / ar, l1 = area2(lp, r2, r0) / ar, l2 = area2(lp, r0, r1) / ar;
-    x[0] = l0 * v[0][0] + l1 * v[1][0] + l2 * v[2][0];
-    x[1] = l0 * v[0][1] + l1 * v[1][1] + l2 * v[2][1];
+    lam[0] = area2(lp, r1, r2) / ar;
+    lam[1] = area2(lp, r2, r0) / ar;
+    lam[2] = area2(lp, r0, r1) / ar;
   } else {
     const double r0[3] = {0.0, 0.0, 0.0}, r1[3] = {1.0, 0.0, 0.0}, r2[3] = {0.0, 1.0, 0.0},
                  r3[3] = {0.0, 0.0, 1.0};
     const double vol = det3(r0, r1, r2, r3) / 6.;
-    const double l0 = det3(lp, r1, r2, r3) / 6. / vol, l1 = det3(r0, lp, r2, r3) / 6. / vol;
-    const double l2 = det3(r0, r1, lp, r3) / 6. / vol, l3 = det3(r0, r1, r2, lp) / 6. / vol;
-AFE_UNROLL
-    for (int d = 0; d < 3; d++) x[d] = l0 * v[0][d] + l1 * v[1][d] + l2 * v[2][d] + l3 * v[3][d];
+    lam[0] = det3(lp, r1, r2, r3) / 6. / vol;
+    lam[1] = det3(r0, lp, r2, r3) / 6. / vol;
+    lam[2] = det3(r0, r1, lp, r3) / 6. / vol;
+    lam[3] = det3(r0, r1, r2, lp) / 6. / vol;
   }
+}
+
+template <int D> AFE_HD inline void bary_to_global(const double *lam, const double v[][D], double *x) {
+  if constexpr (D == 2) {
+    x[0] = lam[0] * v[0][0] + lam[1] * v[1][0] + lam[2] * v[2][0];
+    x[1] = lam[0] * v[0][1] + lam[1] * v[1][1] + lam[2] * v[2][1];
+  } else {
+AFE_UNROLL
+    for (int d = 0; d < 3; d++) x[d] = lam[0] * v[0][d] + lam[1] * v[1][d] + lam[2] * v[2][d] + lam[3] * v[3][d];
+  }
+}
+
+template <int D> AFE_HD inline void local_to_global(const double *lp, const double v[][D], double *x) {
+  double lam[D + 1];
+  ref_bary<D>(lp, lam);
+  bary_to_global<D>(lam, v, x);
 }
 
 // basis values at the physical point p (local order of the .bas_fun files)

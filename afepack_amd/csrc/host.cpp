@@ -106,7 +106,8 @@ void mesh_cube(int n, double jitter, uint64_t seed, double *coords, int32_t *ev,
 // (BilinearOperator.templates.h:64-111): the diagonal and every DOF sharing an
 // element, diagonal first, the rest ascending.
 void space_build(Space &s, int kind, int64_t n_vert, const double *coords,
-                 int64_t n_elem, const int32_t *ev, const int32_t *vmark) {
+                 int64_t n_elem, const int32_t *ev, const int32_t *vmark,
+                 int64_t n_side, const int32_t *side_vert, const int32_t *side_mark) {
   s.kind = kind;
   s.tdim = kind_tdim(kind);
   s.n_loc = kind_nloc(kind);
@@ -151,6 +152,31 @@ void space_build(Space &s, int kind, int64_t n_vert, const double *coords,
     edge_cnt.assign(n_edge, 0);
     for (int64_t i = 0; i < n_elem * 3; i++) edge_cnt[edge_of[i]]++;
   }
+  // buildDofBoundaryMark (FEMSpace.templates.h:689-701) gives an edge DOF the
+  // boundary mark of its mesh side: look each element edge up in the side
+  // list by its vertex pair
+  std::vector<int32_t> edge_mark;
+  if (kind == P2_TRI && n_side > 0) {
+    if (!side_vert || !side_mark) throw Error{-1, "side list without vertices or marks"};
+    std::vector<std::pair<int64_t, int32_t>> key(n_side);
+    for (int64_t i = 0; i < n_side; i++) {
+      const int32_t a = side_vert[2 * i], b = side_vert[2 * i + 1];
+      if (a < 0 || b < 0 || a >= n_vert || b >= n_vert) throw Error{-1, "side vertex index out of range"};
+      key[i] = {((int64_t)std::min(a, b) << 32) | std::max(a, b), side_mark[i]};
+    }
+    std::sort(key.begin(), key.end(),
+              [](const std::pair<int64_t, int32_t> &x, const std::pair<int64_t, int32_t> &y) { return x.first < y.first; });
+    edge_mark.assign(edge_cnt.size(), 0);
+    for (int64_t e = 0; e < n_elem; e++)
+      for (int k = 0; k < 3; k++) {
+        const int32_t p = ev[3 * e + (k + 1) % 3], q = ev[3 * e + (k + 2) % 3];
+        const int64_t kk = ((int64_t)std::min(p, q) << 32) | std::max(p, q);
+        auto it = std::lower_bound(key.begin(), key.end(), kk,
+                                   [](const std::pair<int64_t, int32_t> &x, int64_t v) { return x.first < v; });
+        if (it == key.end() || it->first != kk) throw Error{-1, "an element edge is missing from the side list"};
+        edge_mark[edge_of[3 * e + k]] = it->second;
+      }
+  }
 
   // first-touch numbering
   s.elem_dof.assign(n_elem * nl, -1);
@@ -184,7 +210,7 @@ void space_build(Space &s, int kind, int64_t n_vert, const double *coords,
     if (kind == P2_TRI)
       for (int j = 0; j < 3; j++) {
         int32_t d = s.elem_dof[nl * e + 3 + j];
-        s.dof_mark[d] = edge_cnt[edge_of[3 * e + j]] == 1 ? 1 : 0;
+        s.dof_mark[d] = edge_mark.empty() ? (edge_cnt[edge_of[3 * e + j]] == 1 ? 1 : 0) : edge_mark[edge_of[3 * e + j]];
         int32_t p = ev[3 * e + (j + 1) % 3], q = ev[3 * e + (j + 2) % 3];
         for (int c = 0; c < 2; c++)
           s.dof_point[(size_t)d * 2 + c] = 0.5 * coords[(size_t)p * 2 + c] + 0.5 * coords[(size_t)q * 2 + c];

@@ -68,9 +68,14 @@ struct Space {
   std::vector<uint8_t> slot;
 };
 
-// FEMSpace::buildDof first-touch numbering + pattern + gather map
+// FEMSpace::buildDof first-touch numbering + pattern + gather map.  With a
+// side list (n_side vertex pairs and their boundary marks, as the mesh files
+// give them), a P2 edge DOF takes the mark of its side; without one, an edge
+// lying in a single element is marked 1.
 void space_build(Space &s, int kind, int64_t n_vert, const double *coords,
-                 int64_t n_elem, const int32_t *ev, const int32_t *vmark);
+                 int64_t n_elem, const int32_t *ev, const int32_t *vmark,
+                 int64_t n_side = 0, const int32_t *side_vert = nullptr,
+                 const int32_t *side_mark = nullptr);
 
 struct AmgParams {
   double alpha = 0.25;

@@ -58,14 +58,25 @@ __device__ inline double field_g(const FieldDev &f, const double *x, double *gra
 // element matrices: StiffMatrix / MassMatrix / variable-coefficient stiffness
 // (BilinearOperator.templates.h:342-406, EllipticEquation.cpp:65-108):
 //   K(j,k) += w_l * jacobian * volume [* a(x_l)] * <g_j, g_k>  (or * v_j v_k)
-// one thread per element, K row-major per element (NL*NL doubles); for the
-// stiffness forms the lower triangle mirrors the upper one (<g_j,g_k> and
-// <g_k,g_j> are the same products), the mass form computes every entry
+// one thread per element.  The matrices are stored entry-major (entry t of
+// element e at Ke[t * n_elem + e]), so a wavefront's stores are contiguous.
+// The stiffness forms store the upper triangle only (t over j <= k, row by
+// row): <g_j,g_k> and <g_k,g_j> are the same products, so the lower triangle
+// is the upper one bit for bit.  The mass form (Jxw v_j) v_k is not
+// symmetric in rounding and stores all NL*NL entries.  qb holds the reference
+// barycentric coordinates of the rule's points (ref_bary, computed once per
+// rule), qd the points and weights.
 // ---------------------------------------------------------------------------
+template <int NL> __host__ __device__ constexpr int upper_index(int j, int k) { // j <= k
+  return j * NL - j * (j - 1) / 2 + (k - j);
+}
+template <int NL> constexpr int upper_count() { return NL * (NL + 1) / 2; }
+
 template <int KIND, int OP>
 __global__ __launch_bounds__(256) void k_element_matrices(
     int64_t n_elem, const double *__restrict__ coords, const int32_t *__restrict__ ev,
-    const double *__restrict__ qd, int nq, FieldDev fld, double *__restrict__ Ke) {
+    const double *__restrict__ qd, const double *__restrict__ qb, int nq, FieldDev fld,
+    double *__restrict__ Ke) {
   constexpr int D = KindInfo<KIND>::D, NL = KindInfo<KIND>::NL;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_elem) return;
@@ -81,7 +92,7 @@ __global__ __launch_bounds__(256) void k_element_matrices(
   for (int l = 0; l < nq; l++) {
     const double *q = qd + l * (D + 1);
     double x[D];
-    local_to_global<D>(q, v, x);
+    bary_to_global<D>(qb + l * (D + 1), v, x);
     const double Jxw = q[D] * jac * volume;
     if constexpr (OP == 1) { // mass
       double val[NL];
@@ -109,23 +120,40 @@ __global__ __launch_bounds__(256) void k_element_matrices(
         }
     }
   }
-  double *out = Ke + e * (NL * NL);
+  if constexpr (OP == 1) {
 #pragma unroll
-  for (int j = 0; j < NL; j++)
+    for (int j = 0; j < NL; j++)
 #pragma unroll
-    for (int k = 0; k < NL; k++) out[j * NL + k] = (OP == 1 || k >= j) ? K[j][k] : K[k][j];
+      for (int k = 0; k < NL; k++) Ke[(int64_t)(j * NL + k) * n_elem + e] = K[j][k];
+  } else {
+#pragma unroll
+    for (int j = 0; j < NL; j++)
+#pragma unroll
+      for (int k = j; k < NL; k++) Ke[(int64_t)upper_index<NL>(j, k) * n_elem + e] = K[j][k];
+  }
 }
 
 // ---------------------------------------------------------------------------
 // element -> CSR gather: one thread per row, 256 rows per block; the block's
 // contiguous slice of the value array is accumulated in LDS (each slot in
 // ascending element order, the summation order of buildSparseMatrix) and
-// written back coalesced.  NL is the element DOF count.
+// written back coalesced.  NL is the element DOF count; LAYOUT names how Ke
+// holds the element matrices: 0 element-major NL x NL row-major (matrices
+// given by the caller), 1 entry-major upper triangle (stiffness forms),
+// 2 entry-major NL x NL (mass form).
 // ---------------------------------------------------------------------------
-template <int NL>
+template <int NL, int LAYOUT>
+__device__ __forceinline__ double ke_entry(const double *__restrict__ Ke, int64_t n_elem, int64_t e, int a, int b) {
+  if constexpr (LAYOUT == 0) return Ke[(e * NL + a) * NL + b];
+  if constexpr (LAYOUT == 2) return Ke[(int64_t)(a * NL + b) * n_elem + e];
+  const int j = a < b ? a : b, k = a < b ? b : a;
+  return Ke[(int64_t)upper_index<NL>(j, k) * n_elem + e];
+}
+
+template <int NL, int LAYOUT>
 __global__ __launch_bounds__(256) void k_gather(
     int64_t n_rows, const int64_t *__restrict__ rp, const int64_t *__restrict__ adj_ptr,
-    const int32_t *__restrict__ adj, const uint8_t *__restrict__ slot,
+    const int32_t *__restrict__ adj, const uint8_t *__restrict__ slot, int64_t n_elem,
     const double *__restrict__ Ke, double *__restrict__ val) {
   extern __shared__ double seg[];
   const int64_t r0 = (int64_t)blockIdx.x * 256;
@@ -139,10 +167,14 @@ __global__ __launch_bounds__(256) void k_gather(
     double *row = seg + (rp[i] - base);
     for (int64_t p = adj_ptr[i]; p < adj_ptr[i + 1]; p++) {
       const int32_t ea = adj[p];
-      const double *kr = Ke + ((int64_t)(ea >> 3) * NL + (ea & 7)) * NL;
+      const int64_t e = ea >> 3;
+      const int a = ea & 7;
       const uint8_t *sl = slot + p * NL;
+      double kv[NL];
 #pragma unroll
-      for (int b = 0; b < NL; b++) row[sl[b]] += kr[b];
+      for (int b = 0; b < NL; b++) kv[b] = ke_entry<NL, LAYOUT>(Ke, n_elem, e, a, b);
+#pragma unroll
+      for (int b = 0; b < NL; b++) row[sl[b]] += kv[b];
     }
   }
   __syncthreads();
@@ -270,16 +302,13 @@ __global__ __launch_bounds__(256) void k_qpoints(int64_t n_elem, const double *_
 // errors against an exact solution given at the quadrature points:
 // h1 = 0: uq[e*nq+l]; h1 = 1: gq[(e*nq+l)*D + d]
 template <int KIND>
-__global__ __launch_bounds__(256) void k_element_error_qp(int64_t n_elem, const double *__restrict__ coords,
-                                                          const int32_t *__restrict__ ev,
-                                                          const int32_t *__restrict__ edof,
-                                                          const double *__restrict__ qd, int nq, int h1,
-                                                          const double *__restrict__ uh,
-                                                          const double *__restrict__ exact,
-                                                          double *__restrict__ out) {
+__device__ __forceinline__ double element_error_qp(int64_t e, const double *__restrict__ coords,
+                                                   const int32_t *__restrict__ ev,
+                                                   const int32_t *__restrict__ edof,
+                                                   const double *__restrict__ qd, int nq, int h1,
+                                                   const double *__restrict__ uh,
+                                                   const double *__restrict__ exact) {
   constexpr int D = KindInfo<KIND>::D, NL = KindInfo<KIND>::NL;
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n_elem) return;
   double v[D + 1][D];
   load_elem<KIND>(e, coords, ev, v);
   const double volume = tmpl_volume<D>();
@@ -314,7 +343,7 @@ __global__ __launch_bounds__(256) void k_element_error_qp(int64_t n_elem, const 
       }
     }
   }
-  out[e] = err;
+  return err;
 }
 
 __global__ __launch_bounds__(256) void k_gather_vec(int64_t n, int nl, const int64_t *__restrict__ adj_ptr,
@@ -497,6 +526,38 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(const double *__restri
     s = block_sum(s, sh);
     if (threadIdx.x == 0) out[k] = s;
   }
+}
+
+// Functional::L2Error / H1SemiError (Functional.templates.h:296-317,399-426)
+// with the exact values at the quadrature points given: each thread sums the
+// squared errors of its elements (grid-stride, ascending), the block adds
+// its threads' sums in a fixed tree, and part[block] receives the block's
+// sum; gridDim.x partial sums, no per-element array
+template <int KIND>
+__global__ __launch_bounds__(256) void k_error_qp_part(int64_t n_elem, const double *__restrict__ coords,
+                                                       const int32_t *__restrict__ ev,
+                                                       const int32_t *__restrict__ edof,
+                                                       const double *__restrict__ qd, int nq, int h1,
+                                                       const double *__restrict__ uh,
+                                                       const double *__restrict__ exact,
+                                                       double *__restrict__ part) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n_elem;
+       e += (int64_t)gridDim.x * blockDim.x)
+    s += element_error_qp<KIND>(e, coords, ev, edof, qd, nq, h1, uh, exact);
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// out = sqrt(|sum of part[0 .. np)|), the sum in k_finalize's order and the
+// square root correctly rounded (the value a host sqrt gives)
+__global__ __launch_bounds__(FIN_THREADS) void k_finalize_sqrt(const double *__restrict__ part, int np,
+                                                               double *__restrict__ out) {
+  __shared__ double sh[FIN_THREADS];
+  double s = strided_sum(part, np, threadIdx.x, blockDim.x);
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) out[0] = __dsqrt_rn(fabs(s));
 }
 
 // ---------------------------------------------------------------------------

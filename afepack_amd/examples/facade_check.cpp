@@ -5,14 +5,23 @@
 // usage: facade_check <kind 1|2|3> <n> <outdir> [easymesh-base]
 //   kind 1 = triangle.1 (P1), 2 = triangle.2 (P2), 3 = tetrahedron.1; with an
 //   EasyMesh base name (2D) the mesh is read from <base>.n/.s/.e instead of
-//   generated
+//   generated, with a name ending in .mesh from that file in the Mesh text
+//   format
 // files: rp.i64 col.i64 (pattern), K.f64 (StiffMatrix), KH.f64 (the same
 // form evaluated on the host by a derived class), K2.f64 (an override that
 // doubles it), M.f64 (MassMatrix), f.f64 (L2Discretize), Ka.f64 fa.f64 ua.f64
 // (after BoundaryConditionAdmin::apply), g.f64 (Dirichlet data per DOF),
 // x.f64 (AMGSolver::solve), xp.f64 (AMG-preconditioned CG), xv.f64
 // (AMGPreconditioner::vmult of f from zero), err.txt (L2Error, H1SemiError of
-// x against prod sin(pi x_d)).
+// x against prod sin(pi x_d)), L2.f64 (L2InnerProduct on one space), KM.f64
+// (a user operator declared BilinearOperator<DIM, double>: stiffness plus
+// mass, evaluated on the host), info.txt (nMaxCouplingDof), xg.f64
+// (AMGSolver::solve with isSolveMostProjectExactly() = false), cm.f64 and
+// Km.f64 (moved vertex coordinates and the stiffness re-assembled on them),
+// lz1.f64 (level-1 values of a lazy hierarchy after lazyReinit on the moved
+// matrix), hl1.f64 (level-1 values after two lazyReinit calls on a
+// host-assembled matrix with values 3 K), foreign.txt (1 when lazyReinit with
+// another pattern object reached the reference's abort).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -75,16 +84,55 @@ template <int DIM> struct DoubledStiff : public StiffMatrix<DIM, double> {
   }
 };
 
+// a user operator written against the reference's default second value
+// type (BilinearOperator<DIM, double>): stiffness plus mass in one loop
+template <int DIM> struct StiffPlusMass : public BilinearOperator<DIM, double> {
+  StiffPlusMass(FEMSpace<double, DIM> &sp) : BilinearOperator<DIM, double>(sp, sp) {}
+  void getElementMatrix(const Element<double, DIM> &e0, const Element<double, DIM> &e1,
+                        const typename ActiveElementPairIterator<DIM>::State =
+                            ActiveElementPairIterator<DIM>::EQUAL) override {
+    const int n0 = (int)e0.dof().size(), n1 = (int)e1.dof().size();
+    const double volume = e0.templateElement().volume();
+    const auto qi = e0.findQuadratureInfo(this->algebricAccuracy());
+    const auto jac = e0.local_to_global_jacobian(qi.quadraturePoint());
+    const auto q = e0.local_to_global(qi.quadraturePoint());
+    const auto bv = e0.basis_function_value(q);
+    const auto bg = e0.basis_function_gradient(q);
+    for (int l = 0; l < qi.n_quadraturePoint(); l++) {
+      const double Jxw = qi.weight(l) * jac[l] * volume;
+      for (int j = 0; j < n0; j++)
+        for (int k = 0; k < n1; k++)
+          this->elementMatrix(j, k) += Jxw * (innerProduct(bg[j][l], bg[k][l]) + bv[j][l] * bv[k][l]);
+    }
+  }
+};
+
+struct ForeignPattern {};
+static void throw_foreign(const char *) { throw ForeignPattern{}; }
+
+// level-1 values of a hierarchy (afe_amg_level_csr)
+static void dump_level1(const std::string &path, afe_amg *g) {
+  int64_t m, n, nnz;
+  detail::check(afe_amg_level_csr(g, 1, 0, &m, &n, &nnz, nullptr, nullptr, nullptr));
+  std::vector<double> v((size_t)nnz);
+  detail::check(afe_amg_level_csr(g, 1, 0, &m, &n, &nnz, nullptr, nullptr, v.data()));
+  dump(path, v.data(), v.size());
+}
+
 template <int DIM> static int run(int kind, int n, const std::string &out, const std::string &easymesh) {
   g_dim = DIM;
   Mesh<DIM, DIM> mesh;
   if constexpr (DIM == 2) {
-    if (!easymesh.empty()) {
+    if (easymesh.size() > 5 && easymesh.substr(easymesh.size() - 5) == ".mesh") {
+      mesh.readData(easymesh); // the Mesh text format
+    } else if (!easymesh.empty()) {
       EasyMesh em;
       em.readData(easymesh);
       mesh.coords = em.coords;
       mesh.elem = em.elem;
       mesh.vmark = em.vmark;
+      mesh.side_vert = em.side_vert;
+      mesh.side_mark = em.side_mark;
     } else {
       buildSquareMesh(mesh, n, 0.2, 7);
     }
@@ -172,6 +220,70 @@ template <int DIM> static int run(int kind, int n, const std::string &out, const
   std::ofstream es(out + "/err.txt");
   es.precision(17);
   es << l2 << " " << h1 << " " << its << "\n";
+
+  // L2InnerProduct on one space: the device mass kernel
+  L2InnerProduct<DIM, double> L2(sp, sp);
+  L2.algebricAccuracy() = acc;
+  L2.build();
+  dump_mat(out + "/L2.f64", L2);
+  StiffPlusMass<DIM> KM(sp);
+  KM.algebricAccuracy() = acc;
+  KM.build();
+  dump_mat(out + "/KM.f64", KM);
+  std::ofstream(out + "/info.txt") << KM.nMaxCouplingDof() << " " << KM.nDof0() << " " << KM.nDof1() << "\n";
+
+  // the coarsest level smoothed instead of solved exactly
+  AMGSolver gsolver(K);
+  gsolver.isSolveMostProjectExactly() = false;
+  Vector<double> xg(sp.n_dof());
+  gsolver.solve(xg, f, 1.0e-12, 400, 1);
+  dump_vec(out + "/xg.f64", xg);
+
+  if (n <= 0) { // a mesh read from a file: the generated-mesh checks end here
+    std::printf("facade_check kind %d file mesh: %d DOFs, pcg %d its, L2 %.6e\n", kind, sp.n_dof(), its, l2);
+    return 0;
+  }
+  // moving mesh: a lazy hierarchy on K, the interior vertices displaced,
+  // K re-assembled on its own pattern object, lazyReinit
+  AMGSolver lz;
+  lz.lazyReinit(K);
+  {
+    double h = 1.0 / n;
+    detail::check(afe_mesh_displace(DIM, mesh.n_point(), mesh.coords.data(), mesh.vmark.data(), 0.1 * h, 2));
+  }
+  dump(out + "/cm.f64", mesh.coords.data(), mesh.coords.size());
+  K.build();
+  dump_mat(out + "/Km.f64", K);
+  lz.lazyReinit(K);
+  dump_level1(out + "/lz1.f64", lz.handle());
+  // another pattern object: the reference prints its message and aborts
+  int foreign = 0;
+  detail::fatal_hook() = &throw_foreign;
+  try {
+    lz.lazyReinit(M);
+  } catch (const ForeignPattern &) {
+    foreign = 1;
+  }
+  detail::fatal_hook() = nullptr;
+  std::ofstream(out + "/foreign.txt") << foreign << "\n";
+
+  // a host-assembled matrix (no device binding) refreshed twice in place
+  SparsityPattern hpat;
+  {
+    const SparsityPattern &kp = K.getSparsityPattern();
+    hpat.assign_compressed(kp.n_rows(), kp.n_cols(), kp.row_ptr(), kp.col_indices());
+  }
+  SparseMatrix<double> HM(hpat);
+  const std::vector<double> kv = K.values();
+  for (size_t k = 0; k < kv.size(); k++) HM.global_entry(k) = kv[k];
+  AMGSolver hl;
+  hl.lazyReinit(HM);
+  for (int pass = 2; pass <= 3; pass++) {
+    for (size_t k = 0; k < kv.size(); k++) HM.global_entry(k) = pass * kv[k];
+    hl.lazyReinit(HM);
+  }
+  dump_level1(out + "/hl1.f64", hl.handle());
+
   std::printf("facade_check kind %d n %d: %d DOFs, pcg %d its, L2 %.6e H1 %.6e\n", kind, n, sp.n_dof(), its, l2, h1);
   return 0;
 }

@@ -31,7 +31,7 @@ public:
     if (own_) afe_matrix_destroy(own_);
     g_ = nullptr;
     own_ = nullptr;
-    src_ = nullptr;
+    pat_ = nullptr;
   }
   double tolerence() const { return toler_; }
   double &tolerence() { return toler_; }
@@ -42,16 +42,40 @@ public:
 
   // AMGSolver::reinit: full coarsening (src/AMGSolver.cpp:384-438)
   void reinit(const Matrix &M) { build(M, 0); }
-  // AMGSolver::lazyReinit (src/AMGSolver.cpp:442-545): the first call builds
-  // the lazy hierarchy, later calls with a matrix of the same pattern keep P
-  // and recompute the Galerkin products
+  // AMGSolver::lazyReinit (src/AMGSolver.cpp:505-545).  The first call builds
+  // the lazy hierarchy (lazyInit).  Later calls must pass a matrix on the
+  // same SparsityPattern object: P is kept and the Galerkin products are
+  // recomputed from the new values; another pattern prints the reference's
+  // message and aborts.  (After reinit, whose hierarchy has no kept P^T,
+  // the reference's refresh reads past its arrays; here it builds the lazy
+  // hierarchy anew.)
   void lazyReinit(const Matrix &M) {
-    if (g_ && lazy_ && src_ == &M) {
-      M.push_to_device();
-      SparseMatrix<double>::check(afe_amg_lazy_reinit(g_, device_of(M)));
+    if (!g_) {
+      build(M, 1);
       return;
     }
-    build(M, 1);
+    if (&M.get_sparsity_pattern() != pat_) detail::fatal("The solver is not initialized with the same sparsity pattern.");
+    if (!lazy_) {
+      build(M, 1);
+      return;
+    }
+    M.push_to_device();
+    afe_matrix *a = M.device_matrix();
+    if (!a && own_) {
+      // host-assembled values: refresh the device copy the hierarchy was
+      // built on in place, so the hierarchy never points at a freed matrix
+      detail::check(afe_matrix_set_values(own_, M.values().data()));
+      a = own_;
+    }
+    const int rc = a ? afe_amg_lazy_reinit(g_, a) : AFE_E_PATTERN;
+    if (rc == AFE_E_PATTERN) {
+      // same pattern object, but the matrix is now bound to another device
+      // copy: lazyProject depends on the pattern only, so a fresh lazy build
+      // gives the same P and the same Galerkin products
+      build(M, 1);
+      return;
+    }
+    detail::check(rc);
   }
   // AMGSolver::solve (src/AMGSolver.cpp:549-639): exactly step+1 V(s,s)
   // cycles, mode 0 reports the final residual on stderr
@@ -82,28 +106,30 @@ public:
 protected:
   void prepare(const Vector<double> &x, const Vector<double> &r) const {
     if (!g_) throw std::runtime_error("AMGSolver used before reinit");
-    if (!exact_coarse_) throw std::runtime_error("only the exact coarsest solve is supported on this path");
     if (x.size() != r.size() || x.size() != n_) throw std::runtime_error("AMGSolver: size mismatch");
     if (smooth_ != smooth_set_) {
       detail::check(afe_amg_set_smooth(g_, (int32_t)smooth_));
       smooth_set_ = smooth_;
     }
-  }
-  afe_matrix *device_of(const Matrix &M) {
-    if (M.device_matrix()) return M.device_matrix();
-    // a host-assembled matrix: upload its CSR
-    const auto &pat = M.get_sparsity_pattern();
-    std::vector<int64_t> rp(pat.row_ptr().begin(), pat.row_ptr().end()),
-        col(pat.col_indices().begin(), pat.col_indices().end());
-    if (own_) afe_matrix_destroy(own_);
-    own_ = nullptr;
-    detail::check(afe_matrix_create_csr(detail::device(), (int64_t)M.m(), rp.data(), col.data(), M.values().data(), &own_));
-    return own_;
+    if (exact_coarse_ != exact_set_) {
+      detail::check(afe_amg_set_coarse_exact(g_, exact_coarse_ ? 1 : 0));
+      exact_set_ = exact_coarse_;
+    }
   }
   void build(const Matrix &M, int lazy) {
     clear();
     M.push_to_device();
-    afe_matrix *a = device_of(M);
+    afe_matrix *a = M.device_matrix();
+    if (!a) {
+      // a host-assembled matrix: a device copy of its CSR, kept for the
+      // lifetime of the hierarchy (the hierarchy borrows its fine matrix)
+      const auto &pat = M.get_sparsity_pattern();
+      std::vector<int64_t> rp(pat.row_ptr().begin(), pat.row_ptr().end()),
+          col(pat.col_indices().begin(), pat.col_indices().end());
+      detail::check(
+          afe_matrix_create_csr(detail::device(), (int64_t)M.m(), rp.data(), col.data(), M.values().data(), &own_));
+      a = own_;
+    }
     afe_amg_params p;
     p.alpha = alpha_;
     p.nmpo = nmpo_;
@@ -113,14 +139,15 @@ protected:
     p.build_fp32 = 1;
     detail::check(afe_amg_create(a, &p, &g_));
     lazy_ = lazy;
-    src_ = &M;
+    pat_ = &M.get_sparsity_pattern();
     n_ = M.m();
     smooth_set_ = smooth_;
+    exact_set_ = true;
   }
 
   afe_amg *g_ = nullptr;
   afe_matrix *own_ = nullptr;
-  const Matrix *src_ = nullptr;
+  const SparsityPattern *pat_ = nullptr; // the pattern object of the fine matrix
   size_t n_ = 0;
   int lazy_ = 0;
   double toler_ = 1.0e-12;
@@ -128,6 +155,7 @@ protected:
   mutable u_int smooth_set_ = 3;
   double nmpsd_ = 0.382, alpha_ = 0.25;
   bool exact_coarse_ = true;
+  mutable bool exact_set_ = true;
 };
 
 // AMGSolver used as a preconditioner: vmult = iterate_step cycles from the

@@ -15,13 +15,15 @@
 // The matrix keeps a device copy; the host values are copied back lazily on
 // the first host access.
 #pragma once
+#include <algorithm>
 #include <typeinfo>
 
 #include "FEMSpace.h"
 
 namespace AFEPack {
 
-template <int DIM, class value_type0, class value_type1, int DOW = DIM, int TDIM0 = DIM, int TDIM1 = DIM>
+template <int DIM, class value_type0, class value_type1 = value_type0, int DOW = DIM, int TDIM0 = DIM,
+          int TDIM1 = DIM>
 class BilinearOperator : public SparseMatrix<double> {
 public:
   using space0_t = FEMSpace<value_type0, DIM, DOW, TDIM0>;
@@ -44,6 +46,23 @@ public:
   space1_t *FEMSpace1() { return sp1_; }
   const SparsityPattern &getSparsityPattern() const { return pattern_store_; }
   SparsityPattern &getSparsityPattern() { return pattern_store_; }
+
+  // BilinearOperator.h:101-116: sizes, the coupling bound of the pattern and
+  // the DOFs of the element pair being assembled
+  const int &nDof0() const { return n_dof0_; }
+  const int &nDof1() const { return n_dof1_; }
+  const int &nMaxCouplingDof() const { return n_max_coupling_; }
+  int &nDof0() { return n_dof0_; }
+  int &nDof1() { return n_dof1_; }
+  int &nMaxCouplingDof() { return n_max_coupling_; }
+  const std::vector<int> &elementDof0() const { return *element_dof0_; }
+  const std::vector<int> &elementDof1() const { return *element_dof1_; }
+  const int &elementDof0(const int &i) const { return (*element_dof0_)[i]; }
+  const int &elementDof1(const int &i) const { return (*element_dof1_)[i]; }
+  void getElementPattern(const element0_t &e0, const element1_t &e1) {
+    element_dof0_ = &e0.dof();
+    element_dof1_ = &e1.dof();
+  }
 
   const DenseMatrix<double> &elementMatrix() const { return element_matrix_; }
   DenseMatrix<double> &elementMatrix() { return element_matrix_; }
@@ -70,6 +89,13 @@ public:
     check(afe_space_pattern(h, rp.data(), col.data()));
     pattern_store_.assign_compressed((size_t)nd, (size_t)nd, std::vector<size_t>(rp.begin(), rp.end()),
                                      std::vector<size_t>(col.begin(), col.end()));
+    // buildDofInfo (BilinearOperator.templates.h:115-172): the largest sum of
+    // element DOF counts over the elements sharing a DOF, capped at nDof1
+    n_dof0_ = n_dof1_ = (int)nd;
+    std::vector<int> coupling((size_t)nd, 0);
+    for (int e = 0; e < sp0_->n_element(); e++)
+      for (int d : sp0_->element(e).dof()) coupling[d] += (int)sp0_->element(e).dof().size();
+    n_max_coupling_ = nd ? std::min(*std::max_element(coupling.begin(), coupling.end()), (int)nd) : 0;
   }
   // values on the device (BilinearOperator.templates.h:176-255)
   virtual void buildSparseMatrix() {
@@ -89,6 +115,7 @@ public:
       const int nl = n_elem ? (int)sp0_->element(0).dof().size() : 0;
       std::vector<double> em((size_t)n_elem * nl * nl);
       for (int e = 0; e < n_elem; e++) {
+        getElementPattern(sp0_->element(e), sp1_->element(e));
         element_matrix_.reinit(nl, nl);
         getElementMatrix(sp0_->element(e), sp1_->element(e));
         std::copy(element_matrix_.data(), element_matrix_.data() + (size_t)nl * nl, em.begin() + (size_t)e * nl * nl);
@@ -108,6 +135,47 @@ private:
   SparsityPattern pattern_store_;
   DenseMatrix<double> element_matrix_;
   int acc_ = 3;
+  int n_dof0_ = 0, n_dof1_ = 0, n_max_coupling_ = 0;
+  const std::vector<int> *element_dof0_ = nullptr, *element_dof1_ = nullptr;
+};
+
+// L2InnerProduct (BilinearOperator.h:142-156, BilinearOperator.templates.h:
+// 272-326): the L2 inner product of the basis functions.  On one space its
+// element matrix is the mass matrix with the same arithmetic (Jxw * phi_j *
+// phi_k, quadrature of element0), so it runs the device mass kernel.
+template <int DIM, class value_type0, class value_type1 = value_type0, int DOW = DIM, int TDIM0 = DIM,
+          int TDIM1 = DIM>
+class L2InnerProduct : public BilinearOperator<DIM, value_type0, value_type1, DOW, TDIM0, TDIM1> {
+  using base = BilinearOperator<DIM, value_type0, value_type1, DOW, TDIM0, TDIM1>;
+
+public:
+  L2InnerProduct() = default;
+  L2InnerProduct(FEMSpace<value_type0, DIM, DOW, TDIM0> &sp0, FEMSpace<value_type1, DIM, DOW, TDIM1> &sp1)
+      : base(sp0, sp1) {}
+  virtual ~L2InnerProduct() = default;
+  void reinit(FEMSpace<value_type0, DIM, DOW, TDIM0> &sp0, FEMSpace<value_type1, DIM, DOW, TDIM1> &sp1) {
+    base::reinit(sp0, sp1);
+  }
+  // host evaluation, used by classes derived from this one
+  virtual void getElementMatrix(const typename base::element0_t &e0, const typename base::element1_t &e1,
+                                const typename ActiveElementPairIterator<DIM>::State =
+                                    ActiveElementPairIterator<DIM>::EQUAL) {
+    const int n0 = (int)e0.dof().size(), n1 = (int)e1.dof().size();
+    const double volume = e0.templateElement().volume();
+    const auto qi = e0.findQuadratureInfo(this->algebricAccuracy());
+    const auto jac = e0.local_to_global_jacobian(qi.quadraturePoint());
+    const auto q = e0.local_to_global(qi.quadraturePoint());
+    const auto bv0 = e0.basis_function_value(q);
+    const auto bv1 = e1.basis_function_value(q);
+    for (int l = 0; l < qi.n_quadraturePoint(); l++) {
+      const double Jxw = qi.weight(l) * jac[l] * volume;
+      for (int j = 0; j < n0; j++)
+        for (int k = 0; k < n1; k++) this->elementMatrix(j, k) += Jxw * bv0[j][l] * bv1[k][l];
+    }
+  }
+
+protected:
+  int device_operator() const override { return typeid(*this) == typeid(L2InnerProduct) ? AFE_OP_MASS : -1; }
 };
 
 template <int DIM, class value_type, int DOW = DIM, int TDIM = DIM>

@@ -43,6 +43,21 @@ inline void check(int rc) {
   if (rc != AFE_OK) throw std::runtime_error(afe_last_error());
 }
 
+// where the reference prints a message and calls abort() (e.g. lazyReinit
+// with a foreign pattern, src/AMGSolver.cpp:509-513), the drop-in prints the
+// same message and calls this hook; it aborts unless a program installs its
+// own handler (which must not return)
+using fatal_handler = void (*)(const char *msg);
+inline fatal_handler &fatal_hook() {
+  static fatal_handler h = nullptr;
+  return h;
+}
+[[noreturn]] inline void fatal(const char *msg) {
+  std::cerr << msg << std::endl;
+  if (fatal_hook()) fatal_hook()(msg);
+  abort();
+}
+
 // HBM vector owned by the drop-in layer
 struct DeviceVector {
   double *p = nullptr;
@@ -73,18 +88,90 @@ template <int DIM, int DOW = DIM> class Mesh {
 public:
   int n_point() const { return (int)(coords.size() / DOW); }
   int n_element() const { return (int)(elem.size() / (DIM + 1)); }
-  int n_geometry(int dim) const { return dim == 0 ? n_point() : dim == DIM ? n_element() : 0; }
-  std::vector<double> coords;  // n_point x DOW
-  std::vector<int32_t> elem;   // n_element x (DIM+1)
-  std::vector<int32_t> vmark;  // boundary mark per point
+  int n_geometry(int dim) const {
+    return dim == 0 ? n_point() : dim == DIM ? n_element() : dim == 1 ? (int)side_mark.size() : 0;
+  }
+  std::vector<double> coords;      // vertex coordinates, n_point x DOW
+  std::vector<int32_t> elem;       // element vertices, n_element x (DIM+1)
+  std::vector<int32_t> vmark;      // boundary mark per vertex
+  std::vector<int32_t> side_vert;  // 1-dimensional geometries (sides / edges), 2 vertices each
+  std::vector<int32_t> side_mark;  // and their boundary marks (empty: none read)
+
+  // Mesh::readData (Geometry.templates.h:417-423, 470-497): the text format
+  // of Mesh::writeData -- the points, then for each dimension 0..DIM the
+  // number of geometries and each geometry as "index, vertex count,
+  // vertices, boundary count, boundaries, boundary mark" (GeometryBM,
+  // src/Geometry.cpp:111-125,174-179).  Geometries may come in any order
+  // (each is stored at its index).  Vertices here are the 0-dimensional
+  // geometries, placed at the point each one names.  As the reference, the
+  // element orientation is kept as read.
+  void readData(const std::string &filename) {
+    std::ifstream in(filename);
+    if (!in) throw std::runtime_error("cannot open " + filename);
+    int np = 0;
+    in >> np;
+    std::vector<double> pts((size_t)np * DOW);
+    for (auto &x : pts) in >> x;
+    struct G {
+      std::vector<int32_t> v;
+      int32_t bm = 0;
+      bool seen = false;
+    };
+    std::vector<std::vector<G>> geo(DIM + 1);
+    for (int d = 0; d <= DIM; d++) {
+      int n = 0;
+      in >> n;
+      if (!in || n < 0) throw std::runtime_error("bad geometry count in " + filename);
+      geo[d].resize(n);
+      for (int k = 0; k < n; k++) {
+        int idx = 0, nv = 0, nb = 0;
+        in >> idx >> nv;
+        if (!in || idx < 0 || idx >= n || nv < 0) throw std::runtime_error("bad geometry record in " + filename);
+        G &g = geo[d][idx];
+        g.v.resize(nv);
+        for (auto &x : g.v) in >> x;
+        in >> nb;
+        for (int b = 0, t = 0; b < nb; b++) in >> t;
+        in >> g.bm;
+        g.seen = true;
+      }
+      for (const G &g : geo[d])
+        if (!g.seen) throw std::runtime_error("a geometry index is missing in " + filename);
+    }
+    if (!in) throw std::runtime_error("truncated mesh file " + filename);
+    const int nv = (int)geo[0].size();
+    coords.resize((size_t)nv * DOW);
+    vmark.resize(nv);
+    for (int i = 0; i < nv; i++) {
+      const G &g = geo[0][i];
+      if (g.v.size() != 1 || g.v[0] < 0 || g.v[0] >= np) throw std::runtime_error("bad vertex geometry");
+      for (int c = 0; c < DOW; c++) coords[(size_t)i * DOW + c] = pts[(size_t)g.v[0] * DOW + c];
+      vmark[i] = g.bm;
+    }
+    side_vert.clear();
+    side_mark.clear();
+    for (const G &g : geo[1]) {
+      if (g.v.size() != 2) throw std::runtime_error("a 1-dimensional geometry needs 2 vertices");
+      side_vert.push_back(g.v[0]);
+      side_vert.push_back(g.v[1]);
+      side_mark.push_back(g.bm);
+    }
+    elem.clear();
+    for (const G &g : geo[DIM]) {
+      if (g.v.size() != (size_t)(DIM + 1)) throw std::runtime_error("only simplex elements are supported");
+      for (int32_t x : g.v) {
+        if (x < 0 || x >= nv) throw std::runtime_error("element vertex out of range");
+        elem.push_back(x);
+      }
+    }
+  }
 };
 
 // EasyMesh .n/.s/.e reader (EasyMesh.templates.h:16-94): triangles stored
 // clockwise are flipped to counter-clockwise; node marks give the vertex DOF
-// marks (side marks are read and kept)
+// marks and side marks the P2 edge DOF marks
 class EasyMesh : public Mesh<2, 2> {
 public:
-  std::vector<std::array<int32_t, 3>> side; // a, b, mark
   void readData(const std::string &base) {
     std::ifstream in(base + ".n");
     if (!in) throw std::runtime_error("cannot open " + base + ".n");
@@ -99,14 +186,21 @@ public:
       in >> idx >> coords[2 * i] >> coords[2 * i + 1] >> vmark[i];
     }
     std::ifstream is(base + ".s");
+    if (!is) throw std::runtime_error("cannot open " + base + ".s");
     int ns2;
     is >> ns2;
-    side.resize(ns2);
+    side_vert.assign(2 * (size_t)ns2, 0);
+    side_mark.assign(ns2, 0);
     for (int i = 0; i < ns2; i++) {
-      int idx, ea, eb;
-      is >> idx >> side[i][0] >> side[i][1] >> ea >> eb >> side[i][2];
+      int idx, a, b, ea, eb, m;
+      is >> idx >> a >> b >> ea >> eb >> m;
+      if (idx < 0 || idx >= ns2) throw std::runtime_error("bad side index in " + base + ".s");
+      side_vert[2 * idx] = a;
+      side_vert[2 * idx + 1] = b;
+      side_mark[idx] = m;
     }
     std::ifstream ie(base + ".e");
+    if (!ie) throw std::runtime_error("cannot open " + base + ".e");
     int ne2, nn2, ns3;
     ie >> ne2 >> nn2 >> ns3;
     std::getline(ie, rest);
@@ -328,8 +422,12 @@ public:
   void buildDof() {
     if (h_) afe_space_destroy(h_);
     const int kind = (*tmpl_)[0].kind;
-    detail::check(afe_space_create(detail::device(), kind, mesh_->n_point(), mesh_->coords.data(), mesh_->n_element(),
-                           mesh_->elem.data(), mesh_->vmark.data(), &h_));
+    // the side marks, when the mesh file had sides, mark the P2 edge DOFs
+    detail::check(afe_space_create_sides(detail::device(), kind, mesh_->n_point(), mesh_->coords.data(),
+                                         mesh_->n_element(), mesh_->elem.data(), mesh_->vmark.data(),
+                                         (int64_t)mesh_->side_mark.size(),
+                                         mesh_->side_vert.empty() ? nullptr : mesh_->side_vert.data(),
+                                         mesh_->side_mark.empty() ? nullptr : mesh_->side_mark.data(), &h_));
     int64_t nd, nnz;
     int32_t nl;
     detail::check(afe_space_info(h_, &nd, &nnz, &nl));
@@ -339,6 +437,7 @@ public:
     point_.resize((size_t)nd * DOW);
     detail::check(afe_space_dofs(h_, ed.data(), mark_.data(), point_.data()));
     for (int e = 0; e < n_element(); e++) elem_[e].dof().assign(ed.begin() + (size_t)e * nl, ed.begin() + (size_t)(e + 1) * nl);
+    uploaded_ = mesh_->coords;
   }
   void buildDofBoundaryMark() {}
   int n_dof() const { return n_dof_; }
@@ -348,9 +447,20 @@ public:
     for (int d = 0; d < DOW; d++) p[d] = point_[(size_t)i * DOW + d];
     return p;
   }
-  afe_space *handle() const { return h_; }
+  // the device space, with the mesh's current vertex coordinates: a moved
+  // mesh (same connectivity, as in the moving-mesh solvers) is pushed to
+  // the device before the next device use, so the pattern, the gather map
+  // and any lazy AMG hierarchy built on this space stay valid
+  afe_space *handle() const {
+    if (h_ && mesh_ && mesh_->coords != uploaded_) {
+      detail::check(afe_space_update_coords(h_, mesh_->coords.data()));
+      uploaded_ = mesh_->coords;
+    }
+    return h_;
+  }
 
 private:
+  mutable std::vector<double> uploaded_;
   Mesh<DIM, DOW> *mesh_ = nullptr;
   std::vector<template_t> *tmpl_ = nullptr;
   std::vector<element_t> elem_;

@@ -228,6 +228,12 @@ def run_c2(args):
         A.assemble(afe.OP_STIFF, acc)
         asm.append(time.perf_counter() - t)
     t_asm = min(asm)
+    # the two assembly kernels timed by HIP events; algorithmic bytes per
+    # assembly: coordinates + element vertex and DOF indices + CSR values
+    te_ms, tg_ms, asm_bytes = A.assemble_timing(afe.OP_STIFF, acc, reps=10)
+    asm_gbs = asm_bytes / ((te_ms + tg_ms) * 1e6)
+    log(f"assembly: element kernel {1e3*te_ms:.1f} us, gather {1e3*tg_ms:.1f} us, "
+        f"{asm_bytes/1e6:.0f} MB algorithmic -> {asm_gbs:.0f} GB/s")
     b = sp.rhs(acc, afe.F_POISSON)
     A.apply_dirichlet(b)
     x = afe.DeviceVector(ctx, sp.n_dof)
@@ -377,6 +383,11 @@ def run_c2(args):
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
+    asm_traffic = None
+    apath = os.path.join(ROOT, "profiles", "traffic_assembly_p2_4m.json")
+    if os.path.exists(apath):
+        with open(apath) as fh:
+            asm_traffic = json.load(fh)
 
     # achievable streaming bandwidth next to it (plain 1 GiB copy)
     ms_c = ctx.copy_kernel_timing(1 << 30, reps=10)
@@ -430,6 +441,11 @@ def run_c2(args):
         "mixed_vs_pcg64": value / (sp.n_dof / t64),
         "mixed_vs_fp64_solution_rel_l2": diff,
         "assembly_ms": 1e3 * t_asm,
+        "assembly": {"wall_ms": 1e3 * t_asm, "element_kernel_us": 1e3 * te_ms, "gather_kernel_us": 1e3 * tg_ms,
+                     "algorithmic_bytes": asm_bytes, "achieved_gbs": asm_gbs, "frac_of_hbm_peak": asm_gbs / HBM_PEAK_GBS,
+                     "traffic": asm_traffic,
+                     "note": ("bytes: coordinates + element vertex/DOF indices + CSR values; the element "
+                              "kernel is FP64-VALU bound (reference operation order, contraction off)")},
         "setup_s": {"space_pattern": t_space, "amg_host_and_upload": t_amg},
         "kernels_us": {k: 1e3 * v[0] for k, v in k_ms.items()},
         "kernels_algorithmic_gbs": {k: v[1] / v[0] / 1e6 for k, v in k_ms.items()},

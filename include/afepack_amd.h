@@ -139,6 +139,17 @@ int afe_space_create(afe_ctx *ctx, int32_t kind, int64_t n_vert,
                      const double *coords, int64_t n_elem,
                      const int32_t *elem_vert, const int32_t *vert_mark,
                      afe_space **out);
+/* the same with the mesh's side list (Mesh geometry(1): n_side vertex pairs
+ * side_vert and their boundary marks side_mark, as EasyMesh .s files and
+ * the Mesh text format give them): a P2 edge DOF takes the mark of its side
+ * (FEMSpace::buildDofBoundaryMark, FEMSpace.templates.h:689-701); an element
+ * edge missing from the list is an error.  afe_space_create without sides
+ * marks an edge 1 when it lies in a single element. */
+int afe_space_create_sides(afe_ctx *ctx, int32_t kind, int64_t n_vert,
+                           const double *coords, int64_t n_elem,
+                           const int32_t *elem_vert, const int32_t *vert_mark,
+                           int64_t n_side, const int32_t *side_vert,
+                           const int32_t *side_mark, afe_space **out);
 int afe_space_destroy(afe_space *s);
 int afe_space_info(const afe_space *s, int64_t *n_dof, int64_t *nnz,
                    int32_t *n_loc);
@@ -172,8 +183,12 @@ int afe_assemble_elements(afe_matrix *a, const double *h_elem_mat);
 int afe_matrix_values(const afe_matrix *a, double *h_val);     /* D2H copy */
 int afe_matrix_set_values(afe_matrix *a, const double *h_val); /* H2D copy */
 double *afe_matrix_device_values(afe_matrix *a);
-/* SparseMatrix::vmult (SparseMatrix.h:100-114): d_y = A d_x */
+/* SparseMatrix::vmult (SparseMatrix.h:100-114): d_y = A d_x (m entries),
+ * summed in stored order as the reference (bitwise its result) */
 int afe_matrix_vmult(afe_matrix *a, const double *d_x, double *d_y);
+/* SparseMatrix::Tvmult (SparseMatrix.h:116-132): d_y = A^T d_x (n entries),
+ * each entry summed in ascending row order as the reference's scatter */
+int afe_matrix_tvmult(afe_matrix *a, const double *d_x, double *d_y);
 
 /* ---- right-hand side and Dirichlet elimination -------------------------- */
 /* Operator::L2Discretize (Operator.discretize.templates.h:80-152) into d_f. */
@@ -234,7 +249,10 @@ int afe_amg_lazy_reinit_host(afe_amg *g, const double *val);
 int afe_amg_coarse_inverse(const afe_amg *g, double *minv);
 /* AMGSolver::lazyReinit (src/AMGSolver.cpp:505-545): same pattern object
  * required (AFE_E_PATTERN where the reference aborts); P is kept and the
- * Galerkin products are recomputed. */
+ * Galerkin products are recomputed.  The pattern object is the afe_space of
+ * a space matrix (any matrix of that space qualifies, so a re-assembled
+ * matrix may replace the destroyed one the hierarchy was built on) and the
+ * afe_matrix itself for a matrix made by afe_matrix_create_csr. */
 int afe_amg_lazy_reinit(afe_amg *g, afe_matrix *a);
 int afe_amg_info(const afe_amg *g, int32_t *n_levels, int64_t *level_n,
                  int64_t *level_nnz, int32_t *coarse_full);
@@ -269,6 +287,12 @@ int afe_amg_pcg(afe_amg *g, double *d_x, const double *d_b, double rtol,
 int afe_amg_vcycle(afe_amg *g, double *d_z, const double *d_r, int32_t prec);
 /* AMGSolver::smoothStep() = s (AMGSolver.h:98-99) after construction */
 int afe_amg_set_smooth(afe_amg *g, int32_t smooth);
+/* AMGSolver::isSolveMostProjectExactly() = exact (AMGSolver.h:100-101):
+ * exact = 1 (the default) solves the coarsest level with the dense inverse
+ * (or by diagonal scaling when the coarsening stalled); exact = 0 runs
+ * smoothStep() Gauss-Seidel sweeps from zero there instead
+ * (src/AMGSolver.cpp:590-605), multicolour in forward colour order. */
+int afe_amg_set_coarse_exact(afe_amg *g, int32_t exact);
 
 /* ---- batched solves (no counterpart in the reference) --------------------
  * Continuous batching of afe_amg_pcg solves on hierarchies of one context:
@@ -359,6 +383,14 @@ int afe_server_timing(afe_server *s, double *out);
  * 2 FP64 sweep, 3 FP32 residual.  *bytes = algorithmic bytes per launch. */
 int afe_amg_kernel_timing(afe_amg *g, int32_t which, int32_t reps,
                           double *avg_ms, double *bytes);
+/* afe_assemble(a, op, acc, coef) `reps` times: the average durations of its
+ * two kernels (element matrices, CSR gather) from HIP events on the context
+ * stream, and its algorithmic bytes per assembly (vertex coordinates,
+ * element vertex and DOF indices read, CSR values written) */
+int afe_assemble_kernel_timing(afe_matrix *a, int32_t op, int32_t acc,
+                               const afe_field *coef, int32_t reps,
+                               double *elem_ms, double *gather_ms,
+                               double *bytes);
 /* live timing of the level-0 FP64 SpMV inside afe_amg_pcg (a start/stop
  * HIP event pair attached to that kernel's dispatch on the solver stream in
  * every CG iteration; the iterate is unchanged): enable = 1 resets and

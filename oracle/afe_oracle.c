@@ -462,6 +462,46 @@ or_space *or_space_new(int kind, int64_t n_vert, const double *coords,
   return s;
 }
 
+/* or_space_new with the mesh's side list (Mesh geometry(1): vertex pairs and
+ * boundary marks, as EasyMesh .s files give them): buildDofBoundaryMark
+ * (FEMSpace.templates.h:689-701) gives an edge DOF the boundary mark of its
+ * side.  NULL when an element edge is not among the sides. */
+static int64_t side_key(int64_t a, int64_t b) { return a < b ? (a << 32) | b : (b << 32) | a; }
+typedef struct {
+  int64_t key;
+  int32_t mark;
+} side_rec;
+static int side_cmp(const void *x, const void *y) {
+  const side_rec *a = x, *b = y;
+  return a->key < b->key ? -1 : a->key > b->key;
+}
+or_space *or_space_new_sides(int kind, int64_t n_vert, const double *coords,
+                             int64_t n_elem, const int32_t *elem_vert,
+                             const int32_t *vert_mark, int64_t n_side,
+                             const int32_t *side_vert, const int32_t *side_mark) {
+  or_space *s = or_space_new(kind, n_vert, coords, n_elem, elem_vert, vert_mark);
+  if (kind != OR_P2_TRI || n_side <= 0) return s;
+  side_rec *sr = malloc(sizeof(side_rec) * n_side);
+  for (int64_t i = 0; i < n_side; i++) {
+    sr[i].key = side_key(side_vert[2 * i], side_vert[2 * i + 1]);
+    sr[i].mark = side_mark[i];
+  }
+  qsort(sr, n_side, sizeof *sr, side_cmp);
+  for (int64_t e = 0; e < n_elem; e++)
+    for (int j = 0; j < 3; j++) {
+      side_rec k = {side_key(elem_vert[e * 3 + (j + 1) % 3], elem_vert[e * 3 + (j + 2) % 3]), 0};
+      const side_rec *hit = bsearch(&k, sr, n_side, sizeof *sr, side_cmp);
+      if (!hit) {
+        free(sr);
+        or_space_free(s);
+        return NULL;
+      }
+      s->dof_mark[s->elem_dof[e * 6 + 3 + j]] = hit->mark;
+    }
+  free(sr);
+  return s;
+}
+
 void or_space_free(or_space *s) {
   if (!s) return;
   free(s->coords);

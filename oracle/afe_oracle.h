@@ -56,6 +56,10 @@ typedef struct or_amg or_amg;
 or_space *or_space_new(int kind, int64_t n_vert, const double *coords,
                        int64_t n_elem, const int32_t *elem_vert,
                        const int32_t *vert_mark);
+or_space *or_space_new_sides(int kind, int64_t n_vert, const double *coords,
+                             int64_t n_elem, const int32_t *elem_vert,
+                             const int32_t *vert_mark, int64_t n_side,
+                             const int32_t *side_vert, const int32_t *side_mark);
 void or_space_free(or_space *s);
 int64_t or_space_n_dof(const or_space *s);
 int or_space_n_loc(const or_space *s);

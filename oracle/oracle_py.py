@@ -29,6 +29,7 @@ class OrField(C.Structure):
 
 _SIGS = {
     "or_space_new": (P, [C.c_int, I64, P, I64, P, P]),
+    "or_space_new_sides": (P, [C.c_int, I64, P, I64, P, P, I64, P, P]),
     "or_space_free": (None, [P]),
     "or_space_n_dof": (I64, [P]),
     "or_space_n_loc": (C.c_int, [P]),
@@ -161,6 +162,12 @@ class Mat:
         lib().or_mat_vmult(self.h, _p(x), _p(y))
         return y
 
+    def tvmult(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.n)
+        lib().or_mat_tvmult(self.h, _p(x), _p(y))
+        return y
+
     def free(self):
         if self.h and self.owner is None:
             lib().or_mat_free(self.h)
@@ -168,7 +175,9 @@ class Mat:
 
 
 class Space:
-    def __init__(self, kind, coords, elem_vert, vert_mark):
+    def __init__(self, kind, coords, elem_vert, vert_mark, sides=None):
+        """sides: optional (side_vert (n_side x 2), side_mark): P2 edge DOFs
+        take their side's boundary mark"""
         self.kind = kind
         self.dim = 3 if kind == P1_TET else 2
         self.coords = np.ascontiguousarray(coords, dtype=np.float64)
@@ -176,8 +185,16 @@ class Space:
         self.vm = np.ascontiguousarray(vert_mark, dtype=np.int32)
         nvtx = 4 if kind == P1_TET else 3
         self.n_elem = len(self.ev) // nvtx
-        self.h = lib().or_space_new(kind, len(self.vm), _p(self.coords), self.n_elem, _p(self.ev),
-                                    _p(self.vm))
+        if sides is None:
+            self.h = lib().or_space_new(kind, len(self.vm), _p(self.coords), self.n_elem, _p(self.ev),
+                                        _p(self.vm))
+        else:
+            sv = np.ascontiguousarray(sides[0], dtype=np.int32).ravel()
+            sm = np.ascontiguousarray(sides[1], dtype=np.int32)
+            self.h = lib().or_space_new_sides(kind, len(self.vm), _p(self.coords), self.n_elem, _p(self.ev),
+                                              _p(self.vm), len(sm), _p(sv), _p(sm))
+            if not self.h:
+                raise ValueError("an element edge is missing from the side list")
         self.n_dof = lib().or_space_n_dof(self.h)
         self.n_loc = lib().or_space_n_loc(self.h)
 

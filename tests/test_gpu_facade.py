@@ -150,20 +150,61 @@ def test_cpp_facade_matches_oracle(tmp_path, kind, n):
     o2, o1 = osp.l2_error(xp, acc), osp.h1_error(xp, acc)
     assert abs(float(l2) - o2) <= 1e-12 * o2 and abs(float(h1) - o1) <= 1e-12 * o1
 
+    # L2InnerProduct on one space: the mass matrix, bit for bit
+    om = osp.build(orc.OP_MASS, acc).csr()[2]
+    assert np.array_equal(_read(tmp_path, "L2.f64"), om)
+    # a user operator declared BilinearOperator<DIM, double> (the default
+    # second value type), stiffness plus mass in one host loop
+    from tests.conftest import rel_err_rows
+    assert rel_err_rows(_read(tmp_path, "KM.f64"), osp.build(orc.OP_STIFF, acc).csr()[2] + om, rp) <= 1e-13
+    # nMaxCouplingDof (BilinearOperator.templates.h:115-172): the largest sum
+    # of element DOF counts over the elements holding a DOF, capped at nDof
+    ed, _, _ = osp.dofs()
+    nl = ed.size // (len(ev) // (4 if kind == 3 else 3))
+    coupling = np.bincount(ed, minlength=len(rp) - 1) * nl
+    mc, nd0, nd1 = (int(t) for t in open(os.path.join(tmp_path, "info.txt")).read().split())
+    assert (mc, nd0, nd1) == (min(int(coupling.max()), len(rp) - 1), len(rp) - 1, len(rp) - 1)
+    # AMGSolver with isSolveMostProjectExactly() = false (Gauss-Seidel on the
+    # coarsest level): 401 cycles reach the direct solution
+    xg = _read(tmp_path, "xg.f64")
+    assert np.linalg.norm(xg - xd) <= 1e-9 * np.linalg.norm(xd)
+    # moving mesh: the stiffness re-assembled on the moved vertices equals the
+    # oracle's assembly there, and the lazy hierarchy refreshed on it equals
+    # the oracle's lazyReinit of the same values (src/AMGSolver.cpp:505-545)
+    cm = _read(tmp_path, "cm.f64")
+    osp_m = orc.Space(kind, cm, ev, vm)
+    km = osp_m.build(orc.OP_STIFF, acc).csr()[2]
+    assert np.array_equal(_read(tmp_path, "Km.f64"), km)
+    og = orc.Amg(orc.Mat.from_csr(len(rp) - 1, len(rp) - 1, rp, col, _read(tmp_path, "Ka.f64")), lazy=True)
+    o_before = og.A(1).csr()[2].copy()
+    assert og.lazy_reinit(og.mat.share_pattern(km)) == 0
+    lz1 = _read(tmp_path, "lz1.f64")
+    assert np.array_equal(lz1, og.A(1).csr()[2]) and not np.array_equal(lz1, o_before)
+    # another SparsityPattern object: the reference's abort path was taken
+    assert open(os.path.join(tmp_path, "foreign.txt")).read().split() == ["1"]
+    # a host-assembled matrix refreshed twice in place (values 2 K, then 3 K)
+    oh = orc.Amg(orc.Mat.from_csr(len(rp) - 1, len(rp) - 1, rp, col, 3.0 * km), lazy=True)
+    assert np.array_equal(_read(tmp_path, "hl1.f64"), oh.A(1).csr()[2])
 
-@pytest.mark.parametrize("prog,limit", [("poisson_equation", 1e-2), ("elliptic_equation", 1.0)])
-def test_cpp_examples_run(tmp_path, prog, limit):
-    exe = os.path.join(BIN, prog)
+
+def test_moving_mesh_example_runs(tmp_path):
+    """afepack_amd/examples/moving_mesh_cg: a user CG loop preconditioned by
+    AMGPreconditioner on a moving P2 mesh with a variable coefficient; every
+    position converges and the L2 error is at the P2 level for h = 1/32."""
+    exe = os.path.join(BIN, "moving_mesh_cg")
     if not os.path.exists(exe):
-        pytest.fail(f"afepack_amd/examples/bin/{prog} is not built")
-    print(f"case example {prog}", flush=True)
-    r = subprocess.run([exe, "64"], capture_output=True, text=True, timeout=120, cwd=tmp_path)
-    print(r.stdout[-1000:], r.stderr[-2000:], flush=True)
+        pytest.fail("afepack_amd/examples/bin/moving_mesh_cg is not built")
+    print("case example moving_mesh_cg", flush=True)
+    r = subprocess.run([exe, "32", "3"], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0
-    line = [s for s in r.stderr.splitlines() if "L2 error" in s][-1]
-    err = float(line.split("=")[1])
-    assert 0 < err < limit
-    assert os.path.getsize(os.path.join(tmp_path, "u.dx")) > 0
+    lines = [s for s in r.stdout.splitlines() if s.startswith("mesh ")]
+    assert len(lines) == 3
+    for s in lines:
+        its = int(s.split("pcg ")[1].split()[0])
+        assert 0 < its < 100
+        l2 = float(s.split("L2 ")[1].split(",")[0])
+        assert 0 < l2 < 1e-4
 
 
 def _write_easymesh(base, c, ev, vm, flip):
@@ -193,6 +234,45 @@ def _write_easymesh(base, c, ev, vm, flip):
         for i, e in enumerate(E):
             s = [sid[tuple(sorted((int(e[(k + 1) % 3]), int(e[(k + 2) % 3]))))] for k in range(3)]
             f.write(f"{i} {e[0]} {e[1]} {e[2]} -1 -1 -1 {s[0]} {s[1]} {s[2]}\n")
+
+
+@pytest.mark.parametrize("name,fmt", [("D", "easymesh"), ("Lshape", "easymesh"), ("Lshape", "mesh")])
+@pytest.mark.parametrize("kind", [1, 2])
+def test_cpp_facade_generated_meshes(tmp_path, kind, name, fmt):
+    """Meshes made by the reference's own generator (tests/golden/easymesh),
+    read by the drop-in's EasyMesh reader or, rewritten in the Mesh text
+    format, by Mesh::readData: pattern and stiffness bit-exact and the
+    right-hand side within 1e-13 of the oracle's on the same mesh, and the
+    Dirichlet elimination on mark 1 -- with P2 edge DOFs marked by their
+    sides (several boundary segments) -- bit-exact."""
+    from oracle import oracle_py as orc
+    from tests.test_mesh_ingest import GOLD, read_easymesh, write_mesh_format
+    exe = os.path.join(BIN, "facade_check")
+    if not os.path.exists(exe):
+        pytest.fail("afepack_amd/examples/bin/facade_check is not built")
+    print(f"case generated mesh {name} {fmt} kind={kind}", flush=True)
+    coords, vmark, elem, sv, sm = read_easymesh(os.path.join(GOLD, name))
+    src = os.path.join(GOLD, name)
+    if fmt == "mesh":
+        src = str(tmp_path / (name + ".mesh"))
+        write_mesh_format(src, coords, vmark, elem, sv, sm, np.random.default_rng(11))
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([exe, str(kind), "0", str(out), src], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    osp = orc.Space(kind, coords.ravel(), elem.ravel(), vmark, sides=(sv, sm))
+    OK = osp.build(orc.OP_STIFF, 4)
+    rp, col, kval = OK.csr()
+    assert np.array_equal(_read(out, "rp.i64", np.int64), rp)
+    assert np.array_equal(_read(out, "col.i64", np.int64), col)
+    assert np.array_equal(_read(out, "K.f64"), kval)
+    of = osp.rhs(4, orc.F_POISSON)
+    assert np.abs(_read(out, "f.f64") - of).max() <= 1e-13 * np.abs(of).max()
+    g = _read(out, "g.f64")
+    osp.apply_dirichlet_values(OK, [1], g, of)
+    assert np.array_equal(_read(out, "Ka.f64"), OK.csr()[2])
+    assert np.abs(_read(out, "fa.f64") - of).max() <= 1e-13 * np.abs(of).max()
 
 
 @pytest.mark.parametrize("kind", [1, 2])
