@@ -30,7 +30,8 @@ class SolveStats(C.Structure):
 class Request(C.Structure):
     _fields_ = [("mesh_key", I64), ("kind", I32), ("n", I32), ("jitter", F64), ("seed", U64),
                 ("move_amp", F64), ("move_k", I32), ("acc", I32), ("op", I32), ("fkind", I32),
-                ("coef", Field), ("rtol", F64), ("max_it", I32), ("keep_x", I32)]
+                ("coef", Field), ("rtol", F64), ("max_it", I32), ("keep_x", I32),
+                ("n_vert", I64), ("coords", P), ("n_elem", I64), ("elem_vert", P), ("vert_mark", P)]
 
 
 PField = C.POINTER(Field)
@@ -83,6 +84,8 @@ SIGS = {
     "afe_amg_lazy_reinit_host": (C.c_int, [P, P]),
     "afe_amg_coarse_inverse": (C.c_int, [P, P]),
     "afe_amg_lazy_reinit": (C.c_int, [P, P]),
+    "afe_space_clone": (C.c_int, [P, P, PP]),
+    "afe_amg_clone": (C.c_int, [P, P, PP]),
     "afe_amg_info": (C.c_int, [P, PI32, P, P, PI32]),
     "afe_amg_level_csr": (C.c_int, [P, I32, I32, PI64, PI64, PI64, P, P, P]),
     "afe_amg_level_colours": (C.c_int, [P, I32, P]),
@@ -106,6 +109,7 @@ SIGS = {
     "afe_server_release": (C.c_int, [P, I64]),
     "afe_server_info": (C.c_int, [P, PI64, PI64, PI64, PI64]),
     "afe_server_timing": (C.c_int, [P, PF64]),
+    "afe_server_timing_n": (C.c_int, [P, P, I32]),
     "afe_batch_create": (C.c_int, [P, I32, I32, PP]),
     "afe_batch_destroy": (C.c_int, [P]),
     "afe_batch_add": (C.c_int, [P, P, P, P, F64, I32, PI32]),
@@ -113,6 +117,7 @@ SIGS = {
     "afe_batch_run": (C.c_int, [P]),
     "afe_batch_result": (C.c_int, [P, I32, PI32, P]),
     "afe_batch_remove": (C.c_int, [P, I32]),
+    "afe_batch_set_tiles": (C.c_int, [P, I32, I32]),
     "afe_amg_device_info": (C.c_int, [P, I32, PI64, PI32, PI64, PI32, PI32]),
 }
 

@@ -240,6 +240,18 @@ class Space:
         self.coords = np.ascontiguousarray(coords, dtype=np.float64)
         _chk(_lib().afe_space_update_coords(self.h, _ptr(self.coords)))
 
+    def clone(self, coords=None):
+        """a space of the same connectivity (numbering, pattern and gather map
+        copied) with new vertex coordinates"""
+        c = self.coords if coords is None else np.ascontiguousarray(coords, dtype=np.float64)
+        h = C.c_void_p()
+        _chk(_lib().afe_space_clone(self.h, _ptr(c), C.byref(h)))
+        out = Space.__new__(Space)
+        out.__dict__.update(self.__dict__)
+        out.coords = c
+        out.h = h
+        return out
+
     def rhs(self, acc, fkind, field=None, nodal=None, out=None):
         if out is None:
             out = DeviceVector(self.ctx, self.n_dof, zero=False)
@@ -378,6 +390,13 @@ class _AmgBase:
         return {"levels": L.value, "n": n[:L.value].tolist(), "nnz": nz[:L.value].tolist(),
                 "coarse_full": bool(full.value)}
 
+    def level_shape(self, level, which=0):
+        """(m, n, nnz) of A_l (which 0) or P_l (which 1) without copying it"""
+        m, n, nnz = C.c_int64(), C.c_int64(), C.c_int64()
+        _chk(_lib().afe_amg_level_csr(self.h, level, which, C.byref(m), C.byref(n), C.byref(nnz),
+                                      None, None, None))
+        return m.value, n.value, nnz.value
+
     def level_csr(self, level, which=0):
         m, n, nnz = C.c_int64(), C.c_int64(), C.c_int64()
         _chk(_lib().afe_amg_level_csr(self.h, level, which, C.byref(m), C.byref(n), C.byref(nnz),
@@ -444,6 +463,16 @@ class Amg(_AmgBase):
     def lazy_reinit(self, matrix):
         _chk(_lib().afe_amg_lazy_reinit(self.h, matrix.h))
         self.matrix = matrix
+
+    def clone(self, matrix):
+        """the hierarchy's structure for another matrix of its pattern, with
+        that matrix's Galerkin products (equal to a fresh lazy build)"""
+        h = C.c_void_p()
+        _chk(_lib().afe_amg_clone(self.h, matrix.h, C.byref(h)))
+        out = Amg.__new__(Amg)
+        out.h = h
+        out.matrix = matrix
+        return out
 
     def solve(self, x, b, tol=0.0, step=20, mode=1, rtol=0.0):
         st = _sigs.SolveStats()
@@ -536,6 +565,10 @@ class Batch:
         _chk(_lib().afe_batch_remove(self.h, slot))
         self._keep.pop(slot, None)
 
+    def set_tiles(self, tile_level0, tile_coarse):
+        """slices per wavefront of the batched level launches (1, 2, 4, 8)"""
+        _chk(_lib().afe_batch_set_tiles(self.h, tile_level0, tile_coarse))
+
     def close(self):
         if self.h:
             _lib().afe_batch_destroy(self.h)
@@ -557,14 +590,23 @@ class Server:
         _chk(_lib().afe_server_create(device, int(hbm_budget), max_slots, setup_threads, smooth, C.byref(h)))
         self.h = h
 
-    def submit(self, mesh_key, kind, n, jitter=0.0, seed=0, move_amp=0.0, move_k=1, acc=4,
-               op=OP_STIFF, fkind=F_POISSON, field=None, rtol=1e-10, max_it=1000, keep_x=False):
+    def submit(self, mesh_key, kind, n=0, jitter=0.0, seed=0, move_amp=0.0, move_k=1, acc=4,
+               op=OP_STIFF, fkind=F_POISSON, field=None, rtol=1e-10, max_it=1000, keep_x=False,
+               mesh=None):
+        """a structured mesh (kind, n, jitter, seed), or mesh=(coords,
+        elem_vert, vert_mark) of the caller's (copied by the call)"""
         r = _sigs.Request()
         r.mesh_key, r.kind, r.n, r.jitter, r.seed = mesh_key, kind, n, jitter, seed
         r.move_amp, r.move_k, r.acc, r.op, r.fkind = move_amp, move_k, acc, op, fkind
         r.rtol, r.max_it, r.keep_x = rtol, max_it, 1 if keep_x else 0
         if field is not None:
             r.coef = field.cstruct()  # the field's arrays are copied by the call
+        if mesh is not None:
+            c = np.ascontiguousarray(mesh[0], dtype=np.float64)
+            ev = np.ascontiguousarray(mesh[1], dtype=np.int32)
+            vm = np.ascontiguousarray(mesh[2], dtype=np.int32)
+            r.n_vert, r.n_elem = len(vm), len(ev) // (4 if kind == P1_TET else 3)
+            r.coords, r.elem_vert, r.vert_mark = _ptr(c), _ptr(ev), _ptr(vm)
         rid = C.c_int64()
         _chk(_lib().afe_server_submit(self.h, C.byref(r), C.byref(rid)))
         return rid.value
@@ -594,9 +636,10 @@ class Server:
         return {"device_bytes": b.value, "cached": c.value, "completed": d.value, "evictions": e.value}
 
     def timing(self):
-        t = (C.c_double * 7)()
-        _chk(_lib().afe_server_timing(self.h, t))
-        keys = ["setup_new_s", "setup_moving_s", "wait_setup_s", "wait_slot_s", "solve_s", "n_new", "n_moving"]
+        t = (C.c_double * 9)()
+        _chk(_lib().afe_server_timing_n(self.h, t, 9))
+        keys = ["setup_new_s", "setup_moving_s", "wait_setup_s", "wait_slot_s", "solve_s", "n_new", "n_moving",
+                "setup_cloned_s", "n_cloned"]
         return dict(zip(keys, list(t)))
 
     def close(self):

@@ -32,6 +32,7 @@ struct BatchMember {
 struct BLaunch {
   int seg_off = 0, nseg = 0, grid = 0;
   int lv_off = 0, lvn_off = 0; // LevelDev tables (level l, level l+1)
+  int tile = 1;                // slices per wavefront (V-cycle level launches)
 };
 
 } // namespace
@@ -65,6 +66,9 @@ struct afe_batch {
   size_t h_gath_n = 0;
   hipGraphExec_t graph = nullptr;
   int nu = 1;
+  // slices per wavefront of the V-cycle level launches, by level bucket:
+  // [0] level 0, [1] the coarser levels run as launches
+  int tile[2] = {1, 1};
   std::recursive_mutex mu;
   // host time of the epochs (tables, capture and instantiation) and the
   // steps, printed at destruction with AFE_VERBOSE
@@ -282,6 +286,7 @@ template <class T> static void batch_build_epoch(afe_batch *b) {
     }
   }
   for (int l = 0; l < Lmulti; l++) {
+    const int tile = b->tile[l == 0 ? 0 : 1];
     int maxcol = 0;
     for (int k = 0; k < K; k++) {
       afe_amg *g = b->slots[b->act[k]]->g;
@@ -295,8 +300,9 @@ template <class T> static void batch_build_epoch(afe_batch *b) {
         s0 = g->lv[l].colour_slice[c];
         s1 = g->lv[l].colour_slice[c + 1];
         return true;
-      }, WPB);
+      }, WPB * tile);
       bl.lv_off = lv_off[l];
+      bl.tile = tile;
       b->down_gs.push_back(bl);
       b->down_gs_level.push_back(l);
       BLaunch bu;
@@ -307,8 +313,9 @@ template <class T> static void batch_build_epoch(afe_batch *b) {
         s0 = g->lv[l].colour_slice[cr];
         s1 = g->lv[l].colour_slice[cr + 1];
         return true;
-      }, WPB);
+      }, WPB * tile);
       bu.lv_off = lv_off[l];
+      bu.tile = tile;
       b->up_gs.push_back(bu);
       b->up_gs_level.push_back(l);
     }
@@ -318,23 +325,26 @@ template <class T> static void batch_build_epoch(afe_batch *b) {
       if (l >= g->tail0) return false;
       s0 = 0; s1 = g->lv[l].ns;
       return true;
-    }, WPB);
+    }, WPB * tile);
     br.lv_off = lv_off[l];
+    br.tile = tile;
     add_launch(bp, [&](int k, int &s0, int &s1) {
       afe_amg *g = b->slots[b->act[k]]->g;
       if (l >= g->tail0) return false;
       s0 = 0; s1 = g->lv[l].p_ns;
       return true;
-    }, WPB);
+    }, WPB * tile);
     bp.lv_off = lv_off[l];
+    bp.tile = tile;
     bp.lvn_off = lv_off[l + 1];
     add_launch(bt, [&](int k, int &s0, int &s1) {
       afe_amg *g = b->slots[b->act[k]]->g;
       if (l >= g->tail0) return false;
       s0 = 0; s1 = g->lv[l].t_ns;
       return true;
-    }, WPB);
+    }, WPB * tile);
     bt.lv_off = lv_off[l];
+    bt.tile = tile;
     bt.lvn_off = lv_off[l + 1];
     b->down_res.push_back(br);
     b->down_restrict.push_back(bp);
@@ -387,25 +397,27 @@ template <class T> static void batch_issue_step(afe_batch *b, hipStream_t s) {
         // a level below 0 starts from the first colour step that kb_restrict took
         if (l > 0 && it == 0 && gi == g0) continue;
         const BLaunch &bl = b->down_gs[gi];
-        if (bl.grid) kb_gs<T><<<bl.grid, BLK, 0, s>>>(S + bl.seg_off, bl.nseg, LV + bl.lv_off);
+        if (bl.grid) kb_gs<T><<<bl.grid, BLK, 0, s>>>(S + bl.seg_off, bl.nseg, LV + bl.lv_off, bl.tile);
       }
     }
     const BLaunch &br = b->down_res[l];
-    if (br.grid) kb_residual<T><<<br.grid, BLK, 0, s>>>(S + br.seg_off, br.nseg, LV + br.lv_off);
+    if (br.grid) kb_residual<T><<<br.grid, BLK, 0, s>>>(S + br.seg_off, br.nseg, LV + br.lv_off, br.tile);
     const BLaunch &bp = b->down_restrict[l];
-    if (bp.grid) kb_restrict<T><<<bp.grid, BLK, 0, s>>>(S + bp.seg_off, bp.nseg, LV + bp.lv_off, LV + bp.lvn_off);
+    if (bp.grid) kb_restrict<T><<<bp.grid, BLK, 0, s>>>(S + bp.seg_off, bp.nseg, LV + bp.lv_off, LV + bp.lvn_off,
+                                                    bp.tile);
   }
   const int lds_cap = (int)b->ctx->lds_bytes;
   kb_tail<T><<<K, TAIL_THREADS, (size_t)lds_cap, s>>>(tj_buf<T>(b).p, tl_buf<T>(b).p, b->nu, lds_cap);
   for (int l = L - 1; l >= 0; l--) {
     const BLaunch &bt = b->up_prolong[l];
-    if (bt.grid) kb_prolong<T><<<bt.grid, BLK, 0, s>>>(S + bt.seg_off, bt.nseg, LV + bt.lv_off, LV + bt.lvn_off);
+    if (bt.grid) kb_prolong<T><<<bt.grid, BLK, 0, s>>>(S + bt.seg_off, bt.nseg, LV + bt.lv_off, LV + bt.lvn_off,
+                                                   bt.tile);
     size_t first = 0;
     while (first < b->up_gs.size() && b->up_gs_level[first] != l) first++;
     for (int it = 0; it < b->nu; it++)
       for (size_t j = first; j < b->up_gs.size() && b->up_gs_level[j] == l; j++) {
         const BLaunch &bl = b->up_gs[j];
-        if (bl.grid) kb_gs<T><<<bl.grid, BLK, 0, s>>>(S + bl.seg_off, bl.nseg, LV + bl.lv_off);
+        if (bl.grid) kb_gs<T><<<bl.grid, BLK, 0, s>>>(S + bl.seg_off, bl.nseg, LV + bl.lv_off, bl.tile);
       }
   }
   // z, (r, z) and the flexible beta, new direction
@@ -487,6 +499,21 @@ extern "C" int afe_batch_create(afe_ctx *ctx, int32_t prec, int32_t smooth, afe_
   b->nu = smooth;
   *out = b.release();
   API_END
+}
+
+// slices per wavefront of the batched V-cycle level launches: tile_level0 on
+// level 0, tile_coarse on the coarser levels run as launches (1, 2, 4 or 8);
+// the launch tables are rebuilt at the next step.  Results do not depend on it.
+extern "C" int afe_batch_set_tiles(afe_batch *b, int32_t tile_level0, int32_t tile_coarse) {
+  auto ok = [](int t) { return t == 1 || t == 2 || t == 4 || t == 8; };
+  if (!b || !ok(tile_level0) || !ok(tile_coarse)) return fail(AFE_E_ARG, "tiles must be 1, 2, 4 or 8");
+  std::lock_guard<std::recursive_mutex> lk(b->mu);
+  if (b->tile[0] != tile_level0 || b->tile[1] != tile_coarse) {
+    b->tile[0] = tile_level0;
+    b->tile[1] = tile_coarse;
+    b->dirty = true;
+  }
+  return AFE_OK;
 }
 
 extern "C" int afe_batch_destroy(afe_batch *b) {

@@ -46,65 +46,75 @@ template <class T> struct MemberDev {
 };
 
 // ---- V-cycle level kernels: one wavefront per slice, 2^gs lanes per row ---
+// A block covers tile * WPB consecutive slices of its segment: wavefront w
+// takes slices w, w + WPB, ... (tile of them).  The rows of one launch are
+// independent, so the results do not depend on the tile.
 template <class T>
 __global__ __launch_bounds__(BLK) void kb_gs(const Seg *__restrict__ segs, int nseg,
-                                             const LevelDev<T> *__restrict__ lv) {
+                                             const LevelDev<T> *__restrict__ lv, int tile) {
   const Seg g = find_seg(segs, nseg);
-  const int s = g.s0 + (blockIdx.x - g.block0) * WPB + wave_id();
-  if (s >= g.s1) return;
   const LevelDev<T> &d = lv[g.member];
-  const SliceInfo q = load_si(gptr(d.si), s);
   const int lane = threadIdx.x % WAVE;
   const int gs = d.gs;
-  if ((lane >> gs) >= q.nrow) return;
-  const int r = q.row0 + (lane >> gs);
-  const T br = gptr(d.b)[r], ir = gptr(d.invd)[r];
-  const T acc = row_group_sum(sell_dot<T>(q, lane, gptr(d.cols), gptr(d.vals), gptr((const T *)d.x)), gs);
-  if ((lane & ((1 << gs) - 1)) == 0) gptr(d.x)[r] = (br - acc) * ir;
+  for (int t = 0; t < tile; t++) {
+    const int s = g.s0 + ((blockIdx.x - g.block0) * tile + t) * WPB + wave_id();
+    if (s >= g.s1) return;
+    const SliceInfo q = load_si(gptr(d.si), s);
+    if ((lane >> gs) >= q.nrow) continue;
+    const int r = q.row0 + (lane >> gs);
+    const T br = gptr(d.b)[r], ir = gptr(d.invd)[r];
+    const T acc = row_group_sum(sell_dot<T>(q, lane, gptr(d.cols), gptr(d.vals), gptr((const T *)d.x)), gs);
+    if ((lane & ((1 << gs) - 1)) == 0) gptr(d.x)[r] = (br - acc) * ir;
+  }
 }
 
 template <class T>
 __global__ __launch_bounds__(BLK) void kb_residual(const Seg *__restrict__ segs, int nseg,
-                                                   const LevelDev<T> *__restrict__ lv) {
+                                                   const LevelDev<T> *__restrict__ lv, int tile) {
   const Seg g = find_seg(segs, nseg);
-  const int s = g.s0 + (blockIdx.x - g.block0) * WPB + wave_id();
-  if (s >= g.s1) return;
   const LevelDev<T> &d = lv[g.member];
-  const SliceInfo q = load_si(gptr(d.si), s);
   const int lane = threadIdx.x % WAVE;
   const int gs = d.gs;
-  if ((lane >> gs) >= q.nrow) return;
-  const int r = q.row0 + (lane >> gs);
-  if (s >= d.zs) { // the last colour after the forward sweep (the batch smooths at least once)
-    if ((lane & ((1 << gs) - 1)) == 0) gptr(d.t)[r] = 0;
-    return;
+  for (int t = 0; t < tile; t++) {
+    const int s = g.s0 + ((blockIdx.x - g.block0) * tile + t) * WPB + wave_id();
+    if (s >= g.s1) return;
+    const SliceInfo q = load_si(gptr(d.si), s);
+    if ((lane >> gs) >= q.nrow) continue;
+    const int r = q.row0 + (lane >> gs);
+    if (s >= d.zs) { // the last colour after the forward sweep (the batch smooths at least once)
+      if ((lane & ((1 << gs) - 1)) == 0) gptr(d.t)[r] = 0;
+      continue;
+    }
+    const T br = gptr(d.b)[r], dr = gptr(d.diag)[r], xr = gptr(d.x)[r];
+    const T acc = row_group_sum(sell_dot<T>(q, lane, gptr(d.cols), gptr(d.vals), gptr((const T *)d.x)), gs);
+    if ((lane & ((1 << gs) - 1)) == 0) gptr(d.t)[r] = br - dr * xr - acc;
   }
-  const T br = gptr(d.b)[r], dr = gptr(d.diag)[r], xr = gptr(d.x)[r];
-  const T acc = row_group_sum(sell_dot<T>(q, lane, gptr(d.cols), gptr(d.vals), gptr((const T *)d.x)), gs);
-  if ((lane & ((1 << gs) - 1)) == 0) gptr(d.t)[r] = br - dr * xr - acc;
 }
 
 // b_{l+1} = P t_l, x_{l+1} = 0 (lv: level l, lvn: level l+1 of each member)
 template <class T>
 __global__ __launch_bounds__(BLK) void kb_restrict(const Seg *__restrict__ segs, int nseg,
                                                    const LevelDev<T> *__restrict__ lv,
-                                                   const LevelDev<T> *__restrict__ lvn) {
+                                                   const LevelDev<T> *__restrict__ lvn, int tile) {
   const Seg g = find_seg(segs, nseg);
-  const int s = g.s0 + (blockIdx.x - g.block0) * WPB + wave_id();
-  if (s >= g.s1) return;
   const LevelDev<T> &d = lv[g.member];
   const LevelDev<T> &dn = lvn[g.member];
-  const SliceInfo q = load_si(gptr(d.p_si), s);
   const int lane = threadIdx.x % WAVE;
   const int gs = d.p_gs;
-  if ((lane >> gs) >= q.nrow) return;
-  const int r = q.row0 + (lane >> gs);
-  const T acc = row_group_sum(sell_dot<T>(q, lane, gptr(d.p_cols), gptr(d.p_vals), gptr((const T *)d.t)), gs);
-  if ((lane & ((1 << gs) - 1)) == 0) {
-    // the next level's first colour from zero (as k_restrict); a coarse
-    // tail recomputes the same values when it sweeps that colour
-    gptr(dn.b)[r] = acc;
-    gptr(dn.x)[r] = r < dn.rows_c0 ? acc * gptr(dn.invd)[r] : (T)0;
+  for (int t = 0; t < tile; t++) {
+    const int s = g.s0 + ((blockIdx.x - g.block0) * tile + t) * WPB + wave_id();
+    if (s >= g.s1) return;
+    const SliceInfo q = load_si(gptr(d.p_si), s);
+    if ((lane >> gs) >= q.nrow) continue;
+    const int r = q.row0 + (lane >> gs);
+    const T acc =
+        row_group_sum(sell_dot<T>(q, lane, gptr(d.p_cols), gptr(d.p_vals), gptr((const T *)d.t)), gs);
+    if ((lane & ((1 << gs) - 1)) == 0) {
+      // the next level's first colour from zero (as k_restrict); a coarse
+      // tail recomputes the same values when it sweeps that colour
+      gptr(dn.b)[r] = acc;
+      gptr(dn.x)[r] = r < dn.rows_c0 ? acc * gptr(dn.invd)[r] : (T)0;
+    }
   }
 }
 
@@ -112,19 +122,21 @@ __global__ __launch_bounds__(BLK) void kb_restrict(const Seg *__restrict__ segs,
 template <class T>
 __global__ __launch_bounds__(BLK) void kb_prolong(const Seg *__restrict__ segs, int nseg,
                                                   const LevelDev<T> *__restrict__ lv,
-                                                  const LevelDev<T> *__restrict__ lvn) {
+                                                  const LevelDev<T> *__restrict__ lvn, int tile) {
   const Seg g = find_seg(segs, nseg);
-  const int s = g.s0 + (blockIdx.x - g.block0) * WPB + wave_id();
-  if (s >= g.s1) return;
   const LevelDev<T> &d = lv[g.member];
   const LevelDev<T> &dn = lvn[g.member];
-  const SliceInfo q = load_si(gptr(d.t_si), s);
   const int lane = threadIdx.x % WAVE;
-  if (lane >= q.nrow) return;
-  const int r = q.row0 + lane;
-  if (r >= d.rows_lc) return; // the last colour: the reverse post-sweep overwrites it unread
-  const T xr = gptr(d.x)[r];
-  gptr(d.x)[r] = xr + sell_dot<T>(q, lane, gptr(d.t_cols), gptr(d.t_vals), gptr((const T *)dn.x));
+  for (int t = 0; t < tile; t++) {
+    const int s = g.s0 + ((blockIdx.x - g.block0) * tile + t) * WPB + wave_id();
+    if (s >= g.s1) return;
+    const SliceInfo q = load_si(gptr(d.t_si), s);
+    if (lane >= q.nrow) continue;
+    const int r = q.row0 + lane;
+    if (r >= d.rows_lc) continue; // the last colour: the reverse post-sweep overwrites it unread
+    const T xr = gptr(d.x)[r];
+    gptr(d.x)[r] = xr + sell_dot<T>(q, lane, gptr(d.t_cols), gptr(d.t_vals), gptr((const T *)dn.x));
+  }
 }
 
 // the coarse tail of each member: one workgroup per member

@@ -1411,6 +1411,8 @@ static void set_coarse_mode(afe_amg *g) {
   }
 }
 
+static void finish_levels(afe_amg *g);
+
 static void upload_layout(afe_amg *g, HostLayout &lay) {
   afe_ctx *ctx = g->ctx;
   const Hierarchy &h = g->h;
@@ -1520,7 +1522,14 @@ static void upload_layout(afe_amg *g, HostLayout &lay) {
   g->tail0 = L - 1;
   while (g->tail0 > 0 && g->lv[g->tail0 - 1].n <= g_tune.tail_rows) g->tail0--;
   if (h.full) g->minv.upload(ctx, lay.minv_perm);
-  // level descriptors for the tail kernel
+  finish_levels(g);
+}
+
+// the level descriptors of the tail and batch kernels (device pointers into
+// the levels' buffers) and the CG work vectors
+static void finish_levels(afe_amg *g) {
+  afe_ctx *ctx = g->ctx;
+  const int L = (int)g->lv.size();
   std::vector<LevelDev<double>> d64(L);
   std::vector<LevelDev<float>> d32(L);
   for (int l = 0; l < L; l++) {
@@ -1759,6 +1768,152 @@ extern "C" int afe_amg_lazy_reinit(afe_amg *g, afe_matrix *a) {
   HIPCHK(hipSetDevice(g->ctx->device));
   g->A0 = a;
   upload_layout(g, lay);
+  API_END
+}
+
+// ---------------------------------------------------------------------------
+// structure sharing: the numbering, pattern and gather map of a space, and the
+// interpolation, colouring and layouts of a lazy hierarchy, depend on the
+// mesh connectivity only (lazyProject uses the pattern, not the values), so a
+// mesh of the same connectivity can copy them and compute only its values
+// ---------------------------------------------------------------------------
+template <class T> static void clone_buf(afe_ctx *ctx, DevBuf<T> &dst, const DevBuf<T> &src) {
+  if (!src.p || src.n == 0) {
+    dst.release();
+    return;
+  }
+  dst.alloc(ctx, src.n);
+  HIPCHK(hipMemcpyAsync(dst.p, src.p, src.n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+}
+
+extern "C" int afe_space_clone(const afe_space *src, const double *coords, afe_space **out) {
+  API_BEGIN
+  if (!src || !out) return fail(AFE_E_ARG, "null argument");
+  if (!src->ctx) return fail(AFE_E_ARG, "host-only space");
+  afe_ctx *ctx = src->ctx;
+  auto s = std::make_unique<afe_space>();
+  s->ctx = ctx;
+  s->h = src->h;
+  if (coords) std::memcpy(s->h.coords.data(), coords, s->h.coords.size() * sizeof(double));
+  s->seg_max = src->seg_max;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  s->coords.upload(ctx, s->h.coords);
+  clone_buf(ctx, s->ev, src->ev);
+  clone_buf(ctx, s->elem_dof, src->elem_dof);
+  clone_buf(ctx, s->dof_mark, src->dof_mark);
+  clone_buf(ctx, s->adj, src->adj);
+  clone_buf(ctx, s->col, src->col);
+  clone_buf(ctx, s->adj_ptr, src->adj_ptr);
+  clone_buf(ctx, s->rp, src->rp);
+  clone_buf(ctx, s->slot, src->slot);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *out = s.release();
+  API_END
+}
+
+extern "C" int afe_amg_clone(const afe_amg *src, afe_matrix *a, afe_amg **out) {
+  API_BEGIN
+  if (!src || !a || !out || !src->ctx) return fail(AFE_E_ARG, "null argument");
+  if (!src->prm.lazy || !src->dev_refresh)
+    return fail(AFE_E_ARG, "only a lazy hierarchy with device refresh maps can be cloned");
+  if (a->ctx != src->ctx) return fail(AFE_E_ARG, "matrix of another context");
+  const Csr &F = src->h.A[0];
+  if (a->m != F.m || a->nnz != F.nnz() || a->hrp() != F.rp || a->hcol() != F.col)
+    return fail(AFE_E_PATTERN, "the matrix does not have the hierarchy's sparsity pattern");
+  afe_ctx *ctx = src->ctx;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  auto g = std::make_unique<afe_amg>();
+  g->ctx = ctx;
+  g->A0 = a;
+  g->key_space = a->s;
+  g->key_mat = a->s ? nullptr : a;
+  g->prm = src->prm;
+  g->h = src->h;
+  g->has32 = src->has32;
+  g->tail0 = src->tail0;
+  g->coarse_exact = src->coarse_exact;
+  g->coarse_mode = src->coarse_mode;
+  g->big_coarse = src->big_coarse;
+  g->dev_refresh = true;
+  g->nat_ns = src->nat_ns;
+  g->graph_epoch = g_tune.epoch;
+  g->lv.resize(src->lv.size());
+  for (size_t l = 0; l < src->lv.size(); l++) {
+    DevLevel &d = g->lv[l];
+    const DevLevel &o = src->lv[l];
+    d.n = o.n;
+    d.ncol = o.ncol;
+    d.colour_slice = o.colour_slice;
+    d.rows_c0 = o.rows_c0;
+    d.colour_row = o.colour_row;
+    d.ns = o.ns;
+    d.p_ns = o.p_ns;
+    d.t_ns = o.t_ns;
+    d.gs = o.gs;
+    d.p_gs = o.p_gs;
+    d.zs = o.zs;
+    d.cshift = o.cshift;
+    d.perm = o.perm;
+    d.stream = o.stream;
+    clone_buf(ctx, d.si, o.si);
+    clone_buf(ctx, d.p_si, o.p_si);
+    clone_buf(ctx, d.t_si, o.t_si);
+    clone_buf(ctx, d.cols, o.cols);
+    clone_buf(ctx, d.p_cols, o.p_cols);
+    clone_buf(ctx, d.t_cols, o.t_cols);
+    clone_buf(ctx, d.col_slice, o.col_slice);
+    clone_buf(ctx, d.c16, o.c16);
+    clone_buf(ctx, d.cb16, o.cb16);
+    // values too: the refresh below rewrites every stored entry, and the
+    // copies bring the zero padding of the sliced-ELL slices with them
+    clone_buf(ctx, d.v64, o.v64);
+    clone_buf(ctx, d.invd64, o.invd64);
+    clone_buf(ctx, d.diag64, o.diag64);
+    clone_buf(ctx, d.p_v64, o.p_v64);
+    clone_buf(ctx, d.t_v64, o.t_v64);
+    clone_buf(ctx, d.v32, o.v32);
+    clone_buf(ctx, d.invd32, o.invd32);
+    clone_buf(ctx, d.diag32, o.diag32);
+    clone_buf(ctx, d.p_v32, o.p_v32);
+    clone_buf(ctx, d.t_v32, o.t_v32);
+    clone_buf(ctx, d.c_rp, o.c_rp);
+    clone_buf(ctx, d.P_rp, o.P_rp);
+    clone_buf(ctx, d.Pt_rp, o.Pt_rp);
+    clone_buf(ctx, d.c_col, o.c_col);
+    clone_buf(ctx, d.P_col, o.P_col);
+    clone_buf(ctx, d.Pt_col, o.Pt_col);
+    clone_buf(ctx, d.smap, o.smap);
+    clone_buf(ctx, d.dsrc, o.dsrc);
+    clone_buf(ctx, d.c_val, o.c_val);
+    clone_buf(ctx, d.P_val, o.P_val);
+    clone_buf(ctx, d.Pt_val, o.Pt_val);
+    d.x64.alloc(ctx, d.n);
+    d.b64.alloc(ctx, d.n);
+    d.t64.alloc(ctx, d.n);
+    if (g->has32) {
+      d.x32.alloc(ctx, d.n);
+      d.b32.alloc(ctx, d.n);
+      d.t32.alloc(ctx, d.n);
+    }
+  }
+  clone_buf(ctx, g->perm0, src->perm0);
+  clone_buf(ctx, g->iperm0, src->iperm0);
+  clone_buf(ctx, g->nat_si, src->nat_si);
+  clone_buf(ctx, g->nat_cols, src->nat_cols);
+  clone_buf(ctx, g->nat_rowmap, src->nat_rowmap);
+  clone_buf(ctx, g->nat_smap, src->nat_smap);
+  clone_buf(ctx, g->nat_dsrc, src->nat_dsrc);
+  clone_buf(ctx, g->nat_c16, src->nat_c16);
+  clone_buf(ctx, g->nat_cbase, src->nat_cbase);
+  clone_buf(ctx, g->nat_v64, src->nat_v64);
+  clone_buf(ctx, g->nat_diag, src->nat_diag);
+  clone_buf(ctx, g->minv, src->minv);
+  HIPCHK(hipHostMalloc((void **)&g->h_sc, sizeof(CgScalars), hipHostMallocDefault));
+  finish_levels(g.get());
+  device_lazy_refresh(g.get(), a);
+  *out = g.release();
   API_END
 }
 

@@ -1,10 +1,12 @@
 // server_impl.hpp -- a solve server on one device (part of capi.hip).
 //
-// Requests (structured meshes, optionally a moving-mesh update of a cached
-// mesh) are set up by a pool of host threads -- mesh, FE space, device
-// assembly, right-hand side, Dirichlet elimination and the lazy AMG
-// hierarchy, or for a cached mesh only the new coordinates, the values and
-// AMGSolver::lazyReinit -- and then admitted into one continuously running
+// Requests (structured meshes or the caller's own, optionally a moving-mesh
+// update of a cached mesh) are set up by a pool of host threads -- mesh, FE
+// space, device assembly, right-hand side, Dirichlet elimination and the lazy
+// AMG hierarchy; for a cached mesh only the new coordinates, the values and
+// AMGSolver::lazyReinit; for a new mesh whose connectivity matches a cached
+// entry, a copy of that entry's space and hierarchy structure with the new
+// values computed on the device -- and then admitted into one continuously running
 // batch (afe_batch_*): requests that arrive while others are solving join at
 // the next step, finished ones leave, and every step advances all of them
 // with shared launches.  Meshes, matrices and hierarchies stay cached in HBM
@@ -17,22 +19,31 @@ struct SrvEntry {
   int32_t kind = 0, n = 0;
   double jitter = 0;
   uint64_t seed = 0;
+  bool own_mesh = false;      // the caller's mesh (ev and vmark kept to compare)
+  uint64_t struct_key = 0;    // connectivity key (kind, elements, marks)
   afe_space *sp = nullptr;
   afe_matrix *A = nullptr;
   afe_amg *g = nullptr;
   double *d_b = nullptr, *d_x = nullptr;
   std::vector<double> base_coords;
-  std::vector<int32_t> vmark;
+  std::vector<int32_t> vmark, ev;
   int64_t n_dof = 0, est_bytes = 0;
-  bool busy = false;
+  bool busy = false;          // being set up or solved
+  bool in_setup = false;      // its space / hierarchy being written
+  int pins = 0;               // set-ups copying its structure right now
   uint64_t last_use = 0;
 };
 
 struct SrvRequest {
   int64_t id = 0;
   bool moving = false; // set up as a moving-mesh re-solve of a cached mesh
+  bool cloned = false; // set up from a cached entry of the same connectivity
   afe_request r{};
   std::vector<double> fk, fphi, fw; // owned copies of the coefficient field
+  std::vector<double> mcoords;      // owned copies of the caller's mesh
+  std::vector<int32_t> mev, mvmark;
+  uint64_t struct_key = 0;
+  SrvEntry *tmpl = nullptr;         // entry whose structure the set-up copies
   int state = 0;                    // 0 pending, 1 setup, 2 ready, 3 solving, 4 done
   int status = AFE_OK;
   std::string err; // message of a failed set-up or solve
@@ -70,6 +81,38 @@ static int64_t entry_bytes(int kind, int n) {
   return 96 * nnz + 480 * ndof + 96 * ne;
 }
 
+// the same for a caller's mesh, with upper bounds from its element count:
+// at most n_loc^2 stored entries per element, one DOF per vertex plus (P2)
+// three edges per element
+static int64_t entry_bytes_mesh(int kind, int64_t nv, int64_t ne) {
+  const int nl = kind == AFE_P2_TRI ? 6 : kind == AFE_P1_TET ? 4 : 3;
+  const int64_t ndof = nv + (kind == AFE_P2_TRI ? 3 * ne : 0);
+  return 96 * (int64_t)nl * nl * ne + 480 * ndof + 96 * ne;
+}
+
+// connectivity keys: FNV-1a over the words that fix the numbering, pattern
+// and lazy hierarchy (equal keys are confirmed by comparing the arrays)
+static uint64_t fnv(uint64_t h, const void *p, size_t bytes) {
+  const unsigned char *c = (const unsigned char *)p;
+  for (size_t i = 0; i < bytes; i++) {
+    h ^= c[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+static uint64_t struct_key_structured(int kind, int n) {
+  uint64_t h = 1469598103934665603ull;
+  const int32_t w[3] = {0x5354, kind, n};
+  return fnv(h, w, sizeof w);
+}
+static uint64_t struct_key_mesh(int kind, const std::vector<int32_t> &ev, const std::vector<int32_t> &vm) {
+  uint64_t h = 1469598103934665603ull;
+  const int64_t w[3] = {0x4d45, kind, (int64_t)vm.size()};
+  h = fnv(h, w, sizeof w);
+  h = fnv(h, ev.data(), ev.size() * sizeof(int32_t));
+  return fnv(h, vm.data(), vm.size() * sizeof(int32_t));
+}
+
 } // namespace
 
 struct afe_server {
@@ -92,6 +135,8 @@ struct afe_server {
   // set-up thread, waiting for a batch slot, solving; counts of new / moved
   double t_setup_new = 0, t_setup_moving = 0, t_wait_setup = 0, t_wait_slot = 0, t_solve = 0;
   int64_t n_new = 0, n_moving = 0;
+  double t_setup_cloned = 0; // the part of t_setup_new spent on meshes set up by structure copies
+  int64_t n_cloned = 0;
 };
 
 static void srv_free_entry(afe_server *s, SrvEntry *e) {
@@ -124,7 +169,7 @@ static bool srv_make_room(afe_server *s, int64_t need) {
     SrvEntry *victim = nullptr;
     for (auto &kv : s->cache) {
       SrvEntry *e = kv.second.get();
-      if (e->busy || !e->g) continue;
+      if (e->busy || e->pins > 0 || !e->g) continue;
       if (!victim || e->last_use < victim->last_use) victim = e;
     }
     if (!victim) return false;
@@ -135,6 +180,41 @@ static bool srv_make_room(afe_server *s, int64_t need) {
   return true;
 }
 
+// the request's mesh: the structured generator (then, for a re-solve, the
+// moving-mesh displacement of the base coordinates) or the caller's copy;
+// the elements only for a new entry (want_ev)
+static int srv_mesh(SrvEntry *e, const SrvRequest *q, std::vector<double> &c, std::vector<int32_t> &ev,
+                    bool want_ev) {
+  const afe_request &r = q->r;
+  const int dim = r.kind == AFE_P1_TET ? 3 : 2;
+  if (e->own_mesh) {
+    c = q->mcoords;
+    ev = q->mev;
+    if (e->base_coords.empty()) {
+      e->vmark = q->mvmark;
+      e->base_coords = q->mcoords;
+    }
+    return AFE_OK;
+  }
+  int64_t nv, ne, ndof, nnz;
+  mesh_sizes(r.kind, r.n, nv, ne, ndof, nnz);
+  if (e->base_coords.empty() || want_ev) {
+    std::vector<double> bc((size_t)nv * dim);
+    std::vector<int32_t> vm(nv);
+    ev.resize((size_t)ne * (dim + 1));
+    const int rc = r.kind == AFE_P1_TET ? afe_mesh_cube(r.n, r.jitter, r.seed, bc.data(), ev.data(), vm.data())
+                                        : afe_mesh_square(r.n, r.jitter, r.seed, bc.data(), ev.data(), vm.data());
+    if (rc) return rc;
+    if (e->base_coords.empty()) {
+      e->base_coords = std::move(bc);
+      e->vmark = std::move(vm);
+    }
+  }
+  c = e->base_coords;
+  if (r.move_amp != 0) afe_mesh_displace(dim, nv, c.data(), e->vmark.data(), r.move_amp / r.n, r.move_k);
+  return AFE_OK;
+}
+
 static int srv_setup(afe_server *s, SrvRequest *q) {
   SrvEntry *e = q->e;
   const afe_request &r = q->r;
@@ -143,20 +223,18 @@ static int srv_setup(afe_server *s, SrvRequest *q) {
   coef.phi = q->fphi.empty() ? nullptr : q->fphi.data();
   coef.w = q->fw.empty() ? nullptr : q->fw.data();
   const afe_field *cf = coef.n_modes > 0 ? &coef : nullptr;
-  const int dim = r.kind == AFE_P1_TET ? 3 : 2;
   int rc;
+  std::vector<double> c;
+  std::vector<int32_t> ev;
   if (!e->g) {
-    int64_t nv, ne, ndof, nnz;
-    mesh_sizes(r.kind, r.n, nv, ne, ndof, nnz);
-    e->base_coords.resize((size_t)nv * dim);
-    e->vmark.resize(nv);
-    std::vector<int32_t> ev((size_t)ne * (dim + 1));
-    rc = r.kind == AFE_P1_TET ? afe_mesh_cube(r.n, r.jitter, r.seed, e->base_coords.data(), ev.data(), e->vmark.data())
-                              : afe_mesh_square(r.n, r.jitter, r.seed, e->base_coords.data(), ev.data(), e->vmark.data());
+    if ((rc = srv_mesh(e, q, c, ev, true))) return rc;
+    const int64_t nv = (int64_t)e->vmark.size();
+    const int64_t ne = (int64_t)ev.size() / (r.kind == AFE_P1_TET ? 4 : 3);
+    if (q->tmpl) // same connectivity as a cached entry: copy its numbering, pattern and gather map
+      rc = afe_space_clone(q->tmpl->sp, c.data(), &e->sp);
+    else
+      rc = afe_space_create(s->ctx, r.kind, nv, c.data(), ne, ev.data(), e->vmark.data(), &e->sp);
     if (rc) return rc;
-    std::vector<double> c = e->base_coords;
-    if (r.move_amp != 0) afe_mesh_displace(dim, nv, c.data(), e->vmark.data(), r.move_amp / r.n, r.move_k);
-    if ((rc = afe_space_create(s->ctx, r.kind, nv, c.data(), ne, ev.data(), e->vmark.data(), &e->sp))) return rc;
     if ((rc = afe_matrix_create(e->sp, &e->A))) return rc;
     int32_t nl;
     int64_t nz;
@@ -166,15 +244,18 @@ static int srv_setup(afe_server *s, SrvRequest *q) {
     if ((rc = afe_assemble(e->A, r.op, r.acc, cf))) return rc;
     if ((rc = afe_rhs(e->sp, r.acc, r.fkind, cf, nullptr, e->d_b))) return rc;
     if ((rc = afe_apply_dirichlet(e->A, nullptr, e->d_b))) return rc;
-    afe_amg_params p{0.25, 50u, 0.382, 1, 1, 1};
-    if ((rc = afe_amg_create(e->A, &p, &e->g))) return rc;
+    if (q->tmpl) { // its hierarchy's structure, with this matrix's Galerkin products
+      if ((rc = afe_amg_clone(q->tmpl->g, e->A, &e->g))) return rc;
+      afe_amg_set_smooth(e->g, 1);
+    } else {
+      afe_amg_params p{0.25, 50u, 0.382, 1, 1, 1};
+      if ((rc = afe_amg_create(e->A, &p, &e->g))) return rc;
+    }
     afe_amg_set_tail_rows(e->g, g_tune.tail_rows);
   } else {
     // moving-mesh re-solve: new coordinates, same connectivity and pattern;
     // values-only assembly and the lazy hierarchy refresh
-    std::vector<double> c = e->base_coords;
-    const int64_t nv = (int64_t)e->vmark.size();
-    if (r.move_amp != 0) afe_mesh_displace(dim, nv, c.data(), e->vmark.data(), r.move_amp / r.n, r.move_k);
+    if ((rc = srv_mesh(e, q, c, ev, false))) return rc;
     if ((rc = afe_space_update_coords(e->sp, c.data()))) return rc;
     if ((rc = afe_assemble(e->A, r.op, r.acc, cf))) return rc;
     if ((rc = afe_rhs(e->sp, r.acc, r.fkind, cf, nullptr, e->d_b))) return rc;
@@ -184,16 +265,64 @@ static int srv_setup(afe_server *s, SrvRequest *q) {
   return afe_dev_zero(s->ctx, e->d_x, e->n_dof * 8);
 }
 
+// the cached entry holds this request's mesh (same key and connectivity,
+// and for a structured mesh the same base mesh)
+static bool srv_same_mesh(const SrvEntry *e, const SrvRequest *q) {
+  if (!e->g || e->kind != q->r.kind || e->struct_key != q->struct_key) return false;
+  if (q->r.coords) return e->own_mesh && e->ev == q->mev && e->vmark == q->mvmark;
+  return !e->own_mesh && e->n == q->r.n && e->jitter == q->r.jitter && e->seed == q->r.seed;
+}
+
+// an idle-or-solving entry of the same connectivity whose structure a new
+// mesh may copy (not while its own space or hierarchy is being written)
+static SrvEntry *srv_template(afe_server *s, const SrvRequest *q) {
+  for (auto &kv : s->cache) {
+    SrvEntry *t = kv.second.get();
+    if (!t->g || t->in_setup || t->kind != q->r.kind || t->struct_key != q->struct_key) continue;
+    if (q->r.coords ? (!t->own_mesh || t->ev != q->mev || t->vmark != q->mvmark) : (t->own_mesh || t->n != q->r.n))
+      continue;
+    return t;
+  }
+  return nullptr;
+}
+
+// true when something other than entry e holds or reserves device memory
+// that will be given back (another set-up or a solve in progress)
+static bool srv_others_in_flight(afe_server *s, const SrvEntry *e) {
+  if (s->reserved > 0) return true;
+  for (auto &kv : s->cache)
+    if (kv.second.get() != e && kv.second->busy) return true;
+  return false;
+}
+
+static void srv_finish_failed(afe_server *s, SrvRequest *q, SrvEntry *e, int rc, const std::string &msg) {
+  q->status = rc;
+  q->err = msg;
+  q->state = 4;
+  q->t_done = std::chrono::steady_clock::now();
+  q->n_dof = e->n_dof;
+  q->e = nullptr;
+  if (q->tmpl) {
+    q->tmpl->pins--;
+    q->tmpl = nullptr;
+  }
+  srv_free_entry(s, e);
+  s->cache.erase(e->key);
+  s->completed++;
+  s->cv_done.notify_all();
+  s->cv_work.notify_all();
+}
+
 static void srv_worker(afe_server *s) {
   // the pool works on many requests at once: one host thread per set-up
   set_host_threads(1);
   std::unique_lock<std::mutex> lk(s->m);
   while (!s->stop) {
-    // first pending request whose mesh is not being solved or set up
+    // first pending request whose mesh is not being solved, set up or copied
     SrvRequest *q = nullptr;
     for (auto it = s->pending.begin(); it != s->pending.end(); ++it) {
       auto ce = s->cache.find((*it)->r.mesh_key);
-      if (ce != s->cache.end() && ce->second->busy) continue;
+      if (ce != s->cache.end() && (ce->second->busy || ce->second->pins > 0)) continue;
       q = *it;
       s->pending.erase(it);
       break;
@@ -207,8 +336,7 @@ static void srv_worker(afe_server *s) {
     auto ce = s->cache.find(q->r.mesh_key);
     SrvEntry *e;
     bool fresh = false;
-    if (ce != s->cache.end() && ce->second->g && ce->second->kind == q->r.kind && ce->second->n == q->r.n &&
-        ce->second->jitter == q->r.jitter && ce->second->seed == q->r.seed) {
+    if (ce != s->cache.end() && srv_same_mesh(ce->second.get(), q)) {
       e = ce->second.get();
     } else {
       if (ce != s->cache.end()) { // same key, different mesh: replace
@@ -221,38 +349,66 @@ static void srv_worker(afe_server *s) {
       ne->n = q->r.n;
       ne->jitter = q->r.jitter;
       ne->seed = q->r.seed;
-      ne->est_bytes = entry_bytes(q->r.kind, q->r.n);
+      ne->own_mesh = q->r.coords != nullptr;
+      ne->struct_key = q->struct_key;
+      if (ne->own_mesh) ne->ev = q->mev;
+      ne->est_bytes = ne->own_mesh ? entry_bytes_mesh(q->r.kind, (int64_t)q->mvmark.size(), (int64_t)q->mev.size() /
+                                                      (q->r.kind == AFE_P1_TET ? 4 : 3))
+                                   : entry_bytes(q->r.kind, q->r.n);
       e = ne.get();
       s->cache[ne->key] = std::move(ne);
       fresh = true;
+      // a cached entry of the same connectivity lends its structure
+      if (SrvEntry *t = srv_template(s, q)) {
+        t->pins++;
+        q->tmpl = t;
+        e->est_bytes = t->est_bytes; // measured on the template
+      }
     }
     e->busy = true;
+    e->in_setup = true;
     q->e = e;
     q->moving = !fresh;
     int64_t need = fresh ? e->est_bytes : 0;
     // HBM admission: wait until the budget has room (finished requests free
-    // their entries for eviction)
-    while (fresh && !s->stop && !srv_make_room(s, need)) s->cv_done.wait(lk);
+    // their entries for eviction); when nothing else holds memory that will
+    // come back, waiting cannot help: drop the template (it may be evicted
+    // then) or fail the request
+    bool refused = false;
+    while (fresh && !s->stop && !srv_make_room(s, need)) {
+      if (!srv_others_in_flight(s, e)) {
+        if (q->tmpl) {
+          q->tmpl->pins--;
+          q->tmpl = nullptr;
+          continue;
+        }
+        refused = true;
+        break;
+      }
+      s->cv_done.wait(lk);
+    }
     if (s->stop) break;
+    if (refused) {
+      srv_finish_failed(s, q, e, AFE_E_BUDGET, "request does not fit the HBM budget with every idle mesh evicted");
+      continue;
+    }
+    q->cloned = q->tmpl != nullptr;
     s->reserved += need;
     lk.unlock();
     t_dev_bytes = 0;
     const int rc = srv_setup(s, q);
+    const std::string msg = rc != AFE_OK ? std::string(afe_last_error()) : std::string();
     const int64_t kept = t_dev_bytes;
     lk.lock();
     s->reserved -= need;
+    e->in_setup = false;
+    if (q->tmpl) {
+      q->tmpl->pins--;
+      q->tmpl = nullptr;
+    }
     if (fresh && rc == AFE_OK) e->est_bytes = kept; // measured from now on
     if (rc != AFE_OK) {
-      q->status = rc;
-      q->err = afe_last_error();
-      q->state = 4;
-      q->t_done = std::chrono::steady_clock::now();
-      q->n_dof = e->n_dof;
-      q->e = nullptr;
-      srv_free_entry(s, e);
-      s->cache.erase(e->key);
-      s->completed++;
-      s->cv_done.notify_all();
+      srv_finish_failed(s, q, e, rc, msg);
       continue;
     }
     q->state = 2;
@@ -318,6 +474,10 @@ static void srv_engine(afe_server *s) {
       q->t_done = std::chrono::steady_clock::now();
       (q->moving ? s->t_setup_moving : s->t_setup_new) += seconds_between(q->t_setup, q->t_ready);
       (q->moving ? s->n_moving : s->n_new)++;
+      if (q->cloned) {
+        s->t_setup_cloned += seconds_between(q->t_setup, q->t_ready);
+        s->n_cloned++;
+      }
       s->t_wait_setup += seconds_between(q->t_submit, q->t_setup);
       s->t_wait_slot += seconds_between(q->t_ready, q->t_admit);
       s->t_solve += seconds_between(q->t_admit, q->t_done);
@@ -376,17 +536,34 @@ extern "C" int afe_server_destroy(afe_server *s) {
 extern "C" int afe_server_submit(afe_server *s, const afe_request *r, int64_t *id) {
   API_BEGIN
   if (!s || !r || !id) return fail(AFE_E_ARG, "null argument");
-  if (r->kind < AFE_P1_TRI || r->kind > AFE_P1_TET || r->n < 1 || r->rtol <= 0 || r->max_it < 1 ||
+  if (r->kind < AFE_P1_TRI || r->kind > AFE_P1_TET || (!r->coords && r->n < 1) || r->rtol <= 0 || r->max_it < 1 ||
       (r->op != AFE_OP_STIFF && r->op != AFE_OP_VARSTIFF) ||
       (r->fkind != AFE_F_POISSON && r->fkind != AFE_F_VARCOEF && r->fkind != AFE_F_ONE))
     return fail(AFE_E_ARG, "bad request");
   if ((r->op == AFE_OP_VARSTIFF || r->fkind == AFE_F_VARCOEF) &&
       (r->coef.n_modes < 1 || !r->coef.k || !r->coef.phi || !r->coef.w))
     return fail(AFE_E_ARG, "variable coefficient without a field");
-  const int64_t est = entry_bytes(r->kind, r->n);
+  if (r->coords && (r->n_vert < 1 || r->n_elem < 1 || !r->elem_vert || !r->vert_mark))
+    return fail(AFE_E_ARG, "a caller's mesh needs vertices, elements and vertex marks");
+  const int64_t est = r->coords ? entry_bytes_mesh(r->kind, r->n_vert, r->n_elem) : entry_bytes(r->kind, r->n);
   if (est > s->budget) return fail(AFE_E_BUDGET, "request larger than the HBM budget");
   std::unique_ptr<SrvRequest> q(new SrvRequest);
   q->r = *r;
+  if (r->coords) {
+    const int dim = r->kind == AFE_P1_TET ? 3 : 2, nvtx = r->kind == AFE_P1_TET ? 4 : 3;
+    q->mcoords.assign(r->coords, r->coords + (size_t)r->n_vert * dim);
+    q->mev.assign(r->elem_vert, r->elem_vert + (size_t)r->n_elem * nvtx);
+    q->mvmark.assign(r->vert_mark, r->vert_mark + (size_t)r->n_vert);
+    for (int32_t v : q->mev)
+      if (v < 0 || v >= r->n_vert) return fail(AFE_E_ARG, "element vertex index out of range");
+    q->struct_key = struct_key_mesh(r->kind, q->mev, q->mvmark);
+  } else {
+    q->struct_key = struct_key_structured(r->kind, r->n);
+  }
+  // the copies above are the request's: the caller's arrays are not kept
+  q->r.coords = nullptr;
+  q->r.elem_vert = q->r.vert_mark = nullptr;
+  if (r->coords) q->r.coords = q->mcoords.data();
   if (r->coef.n_modes > 0) {
     const int nm = r->coef.n_modes, dm = r->coef.dim;
     q->fk.assign(r->coef.k, r->coef.k + (size_t)nm * dm);
@@ -461,15 +638,13 @@ extern "C" int afe_server_info(afe_server *s, int64_t *device_bytes, int64_t *ca
 // out[1] set-up of moving-mesh re-solves, out[2] waiting for a set-up
 // thread, out[3] waiting for a batch slot, out[4] solving; out[5], out[6]
 // the counts of new and moving-mesh requests
-extern "C" int afe_server_timing(afe_server *s, double *out) {
-  if (!s || !out) return fail(AFE_E_ARG, "null argument");
+extern "C" int afe_server_timing_n(afe_server *s, double *out, int32_t n_out) {
+  if (!s || !out || n_out < 0) return fail(AFE_E_ARG, "bad arguments");
   std::lock_guard<std::mutex> lk(s->m);
-  out[0] = s->t_setup_new;
-  out[1] = s->t_setup_moving;
-  out[2] = s->t_wait_setup;
-  out[3] = s->t_wait_slot;
-  out[4] = s->t_solve;
-  out[5] = (double)s->n_new;
-  out[6] = (double)s->n_moving;
+  const double v[9] = {s->t_setup_new, s->t_setup_moving, s->t_wait_setup, s->t_wait_slot, s->t_solve,
+                       (double)s->n_new, (double)s->n_moving, s->t_setup_cloned, (double)s->n_cloned};
+  for (int i = 0; i < n_out && i < 9; i++) out[i] = v[i];
   return AFE_OK;
 }
+
+extern "C" int afe_server_timing(afe_server *s, double *out) { return afe_server_timing_n(s, out, 7); }

@@ -35,43 +35,95 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def iteration_bytes(g, s, sv=4):
+    """Algorithmic HBM bytes of one mixed CG iteration (the byte formulas of
+    the measurement plan): the level-0 FP64 SpMV with 32-bit columns; per
+    V-cycle level 2 s Gauss-Seidel sweeps, the residual, the restriction and
+    the prolongation in the V-cycle precision (sv bytes per value, 4-byte
+    columns and row pointers); the dense coarse solve; the CG vector work
+    (x, r and p updates 3 N 8 each, the (r, r) and (r, z) dots 2 N 8 each,
+    the V-cycle's right-hand side in and z out N (8 + sv) each)."""
+    info = g.info()
+    L = info["levels"]
+    n, nnz = info["n"], info["nnz"]
+    n0 = n[0]
+    total = nnz[0] * 12 + (n0 + 1) * 4 + 2 * n0 * 8
+    for l in range(L - 1):
+        m = n[l]
+        pm, pn, pnz = g.level_shape(l, 1)
+        sweep = nnz[l] * (sv + 4) + (m + 1) * 4 + 3 * m * sv
+        resid = nnz[l] * (sv + 4) + (m + 1) * 4 + 3 * m * sv
+        restr = pnz * (sv + 4) + (pm + 1) * 4 + pn * sv + pm * sv
+        prol = pnz * (sv + 4) + (pn + 1) * 4 + pm * sv + 2 * pn * sv
+        total += 2 * s * sweep + resid + restr + prol
+    total += n[L - 1] ** 2 * 8
+    total += (3 * 3 + 2 * 2) * n0 * 8 + 2 * n0 * (8 + sv)
+    return total
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(rp, col, val, b, budget_s, n_dof):
     """Oracle AMGSolver (lazy hierarchy, lexicographic V(3,3)) on the same
-    matrix: time a bounded number of cycles and scale by the cycle count the
-    oracle's own contraction needs for 1e-10 (L1, the reference's norm)."""
+    matrix, pinned to one host core: as many cycles as fit the time budget,
+    then the cycle count to 1e-10 from the measured L1 contraction (the
+    reference's norm) over the second half of those cycles."""
     import numpy as np
     from oracle import oracle_py as orc
-    t0 = time.time()
-    M = orc.Mat.from_csr(len(rp) - 1, len(rp) - 1, rp, col, val)
-    g = orc.Amg(M, lazy=True)
-    t_setup = time.time() - t0
-    x = np.zeros(n_dof)
-    # probe: one cycle to size the sample
-    t1 = time.perf_counter()
-    h1 = g.solve(x, b, smooth=3, step=0)
-    t_cyc = time.perf_counter() - t1
-    k = max(2, min(12, int(budget_s / max(t_cyc, 1e-3))))
-    x[:] = 0
-    t1 = time.perf_counter()
-    hist = g.solve(x, b, smooth=3, step=k - 1)
-    t_k = time.perf_counter() - t1
-    rho = (hist[k] / hist[1]) ** (1.0 / (k - 1)) if k > 1 and hist[1] > 0 else float("nan")
-    cycles = math.ceil(math.log(1e-10) / math.log(rho)) if 0 < rho < 1 else float("inf")
+    old_aff = os.sched_getaffinity(0)
+    core = min(old_aff)
+    os.sched_setaffinity(0, {core})
+    try:
+        t0 = time.time()
+        M = orc.Mat.from_csr(len(rp) - 1, len(rp) - 1, rp, col, val)
+        g = orc.Amg(M, lazy=True)
+        t_setup = time.time() - t0
+        x = np.zeros(n_dof)
+        # probe: one cycle to size the sample
+        t1 = time.perf_counter()
+        g.solve(x, b, smooth=3, step=0)
+        t_cyc = time.perf_counter() - t1
+        k = max(4, min(400, int(budget_s / max(t_cyc, 1e-3))))
+        x[:] = 0
+        t1 = time.perf_counter()
+        hist = g.solve(x, b, smooth=3, step=k - 1)
+        t_k = time.perf_counter() - t1
+        g.free()
+    finally:
+        os.sched_setaffinity(0, old_aff)
+    h = k // 2
+    rel = hist[k] / hist[0] if hist[0] > 0 else float("nan")
+    rho = (hist[k] / hist[h]) ** (1.0 / (k - h)) if hist[h] > 0 else float("nan")
+    if rel <= 1e-10:
+        cycles = next(i for i in range(1, k + 1) if hist[i] / hist[0] <= 1e-10)
+    elif 0 < rho < 1:
+        cycles = k + math.ceil(math.log(1e-10 / rel) / math.log(rho))
+    else:
+        cycles = float("inf")
     per_cycle = t_k / k
     value = n_dof / (cycles * per_cycle) if math.isfinite(cycles) else 0.0
-    g.free()
     return {
         "value": value, "unit": "DOF/s", "cores": 1, "kind": "port",
-        "sample": (f"{k} stationary V(3,3) cycles of the oracle AMGSolver (lazy hierarchy, "
-                   f"lexicographic Gauss-Seidel) on the same {n_dof}-DOF system: "
-                   f"{per_cycle:.3f} s/cycle, L1 contraction {rho:.4f}/cycle -> "
-                   f"{cycles} cycles to 1e-10 (an upper bound on its rate: its FP64 iterate "
-                   f"stops near the rounding floor of this system); hierarchy setup {t_setup:.1f} s untimed; "
-                   f"cpu {os.cpu_count()} visible, 1 used"),
+        "cpu_model": _cpu_model(), "pinned_core": core,
+        "sample": (f"{k} stationary V(3,3) cycles ({t_k:.1f} s) of the oracle AMGSolver (lazy hierarchy, "
+                   f"lexicographic Gauss-Seidel) on the same {n_dof}-DOF system, pinned to core {core}: "
+                   f"{per_cycle:.3f} s/cycle, relative L1 residual {rel:.2e} after them, contraction "
+                   f"{rho:.4f}/cycle over the last {k - h} -> {cycles} cycles to 1e-10 (an upper bound on its "
+                   f"rate: its FP64 iterate stops near the rounding floor of this system); hierarchy set-up "
+                   f"{t_setup:.1f} s untimed; {os.cpu_count()} cpus visible, 1 used"),
     }
 
 
@@ -327,6 +379,10 @@ def run_c2(args):
     ms = 1e3 * statistics.mean(lat)
     value = sp.n_dof / statistics.mean(lat)
     log(f"mixed: {ms:.2f} ms/solve, {its[-1]} its, rel res {rel:.2e}, {value:.3e} DOF/s")
+    it_bytes = iteration_bytes(g, s_mixed)
+    it_ms = ms / its[-1]
+    it_gbs = it_bytes / it_ms / 1e6
+    log(f"per CG iteration: {it_ms*1e3:.0f} us, {it_bytes/1e6:.0f} MB algorithmic -> {it_gbs:.0f} GB/s")
 
     # FP64 PCG with the same hierarchy and smoothing
     solve(afe.PREC_FP64, s_mixed)
@@ -434,6 +490,9 @@ def run_c2(args):
             "parallelism": "single GPU",
         },
         "mixed": {"iterations": its[-1], "rel_res": rel, "p50_ms": 1e3 * p50, "p99_ms": 1e3 * p99},
+        "iteration": {"us": 1e3 * it_ms, "algorithmic_bytes": it_bytes, "achieved_gbs": it_gbs,
+                      "frac_of_hbm_peak": it_gbs / HBM_PEAK_GBS,
+                      "note": "whole mixed CG iteration (SpMV, vector work, FP32 V-cycle) over its own time"},
         "pcg64": {"seconds": t64, "iterations": st64.iterations, "dof_per_s": sp.n_dof / t64},
         "plain": plain,
         "mixed_vs_plain_stationary": (value / plain["dof_per_s"]) if plain and plain["dof_per_s"] else None,
@@ -592,12 +651,13 @@ def run_c4(args):
     total = sum(p["sp"].n_dof for p in probs)
     log(f"c4 setup: {K} problems, {total} DOFs, {t_setup:.1f} s")
 
-    def batch_run(s, tail):
+    def batch_run(s, tail, tiles=(1, 1)):
         for p in probs:
             p["g"].set_tail_rows(tail)
             p["x"].zero()
         ctx.synchronize()
         bt = afe.Batch(ctx, afe.PREC_FP32, smooth=s)
+        bt.set_tiles(*tiles)
         t = time.perf_counter()
         slots = [bt.add(p["g"], p["x"], p["b"], rtol=args.rtol, max_it=1000) for p in probs]
         bt.run()
@@ -609,19 +669,33 @@ def run_c4(args):
     # autotune: smoothing count and the size below which a member's levels
     # run in its own single-workgroup tail (one workgroup per member)
     best = None
+    tune_log = []
     for s in (1, 2):
         for tail in (4096, 32768, 262144):
             dt, res = batch_run(s, tail)
             ok = all(r[0] and r[2] == 0 for r in res)
             log(f"c4 tune V({s},{s}) tail {tail}: {dt*1e3:.1f} ms, all converged {ok}")
+            tune_log.append({"smooth": s, "tail_rows": tail, "ms": 1e3 * dt, "converged": ok})
             if ok and (best is None or dt < best[0]):
                 best = (dt, s, tail)
     _, s_best, tail_best = best
+    # autotune 2: launch buckets of the batched level kernels -- slices per
+    # wavefront on level 0 and on the coarser levels run as launches (the
+    # results are the same for every choice; only the time differs)
+    tile_log = []
+    best_tiles = None
+    for tiles in ((1, 1), (2, 1), (4, 1), (1, 2), (2, 2), (1, 4), (4, 4)):
+        dt = min(batch_run(s_best, tail_best, tiles)[0] for _ in range(2))
+        tile_log.append({"level0_slices_per_wave": tiles[0], "coarse_slices_per_wave": tiles[1], "ms": 1e3 * dt})
+        log(f"c4 tune tiles level0 {tiles[0]} coarse {tiles[1]}: {dt*1e3:.1f} ms")
+        if best_tiles is None or dt < best_tiles[0]:
+            best_tiles = (dt, tiles)
+    tiles_best = best_tiles[1]
     for _ in range(args.warmup):
-        batch_run(s_best, tail_best)
+        batch_run(s_best, tail_best, tiles_best)
     runs = []
     for _ in range(args.steps):
-        dt, res = batch_run(s_best, tail_best)
+        dt, res = batch_run(s_best, tail_best, tiles_best)
         runs.append((dt, res))
     dts = [r[0] for r in runs]
     ms = 1e3 * statistics.mean(dts)
@@ -653,7 +727,9 @@ def run_c4(args):
                                 "even P1 acc 4 / odd P2 acc 6, a = exp(8-mode field, amplitude 0.5), "
                                 "f ~ U[-1, 1] at the DOFs"),
                    "total_dof": total, "rtol": args.rtol, "pcg_smooth": s_best, "tail_rows": tail_best,
+                   "buckets": {"level0_slices_per_wave": tiles_best[0], "coarse_slices_per_wave": tiles_best[1]},
                    "parallelism": "single GPU"},
+        "autotune": {"smooth_and_tail": tune_log, "buckets": tile_log},
         "p50_ms": p50, "p99_ms": p99,
         "iterations": {"min": min(its), "median": statistics.median(its), "max": max(its)},
         "worst_rel_res": worst,

@@ -254,6 +254,18 @@ int afe_amg_coarse_inverse(const afe_amg *g, double *minv);
  * matrix may replace the destroyed one the hierarchy was built on) and the
  * afe_matrix itself for a matrix made by afe_matrix_create_csr. */
 int afe_amg_lazy_reinit(afe_amg *g, afe_matrix *a);
+/* Structure sharing (no counterpart in the reference): the numbering,
+ * pattern and gather map of a space, and the interpolation, colouring and
+ * device layouts of a lazy hierarchy, depend only on the mesh connectivity.
+ * afe_space_clone copies a space for a mesh of the same connectivity with
+ * new vertex coordinates (NULL: the same ones); afe_amg_clone copies a lazy
+ * hierarchy for matrix a, which must have its pattern (AFE_E_PATTERN
+ * otherwise), and computes a's Galerkin products on the device -- the result
+ * equals afe_amg_create(a, lazy) bit for bit, without the host coarsening,
+ * colouring and layout.  The copies are independent of their sources. */
+int afe_space_clone(const afe_space *src, const double *coords,
+                    afe_space **out);
+int afe_amg_clone(const afe_amg *src, afe_matrix *a, afe_amg **out);
 int afe_amg_info(const afe_amg *g, int32_t *n_levels, int64_t *level_n,
                  int64_t *level_nnz, int32_t *coarse_full);
 /* host copies of a level of the hierarchy (reference patterns, including the
@@ -326,20 +338,30 @@ int afe_batch_run(afe_batch *b);
 int afe_batch_result(afe_batch *b, int32_t slot, int32_t *done,
                      afe_solve_stats *st);
 int afe_batch_remove(afe_batch *b, int32_t slot);
+/* launch shape of the batched V-cycle level kernels, by level bucket: slices
+ * per wavefront on level 0 (tile_level0) and on the coarser levels run as
+ * launches (tile_coarse), each 1, 2, 4 or 8; chosen by timing (the C4
+ * autotuner).  Results do not depend on it. */
+int afe_batch_set_tiles(afe_batch *b, int32_t tile_level0, int32_t tile_coarse);
 
 /* ---- solve server (no counterpart in the reference) ----------------------
  * One device, a pool of host set-up threads and one continuously running
  * batch.  A request names a structured mesh (n x n squares split in two
  * triangles, or n^3 cubes split in six tetrahedra, interior jitter and
- * seed) by a cache key; a later request with the same key re-solves that
- * mesh after a moving-mesh update (interior vertices displaced by
- * move_amp * h * sin(2 pi move_k x_d) from the base mesh): the pattern,
- * gather map and lazy hierarchy are kept, only values are reassembled and
- * AMGSolver::lazyReinit refreshes the Galerkin products.  Requests are set
- * up concurrently, admitted into the batch as slots free up, and solved
- * with FP64 CG + FP32 V-cycle to rtol; cached meshes stay in HBM within
- * hbm_budget bytes (idle ones evicted, least recently used first).  A
- * request's result does not depend on the requests solved with it. */
+ * seed) or brings a mesh of its own, under a cache key; a later request with
+ * the same key re-solves that mesh after a moving-mesh update (structured:
+ * interior vertices displaced by move_amp * h * sin(2 pi move_k x_d) from the
+ * base mesh; own mesh: the request's coordinates): the pattern, gather map
+ * and lazy hierarchy are kept, only values are reassembled and
+ * AMGSolver::lazyReinit refreshes the Galerkin products.  A new mesh whose
+ * connectivity equals a cached one's copies that entry's structure
+ * (afe_space_clone, afe_amg_clone) instead of numbering and coarsening on
+ * the host.  Requests are set up concurrently, admitted into the batch as
+ * slots free up, and solved with FP64 CG + FP32 V-cycle to rtol; cached
+ * meshes stay in HBM within hbm_budget bytes (idle ones evicted, least
+ * recently used first); a request that cannot fit even with everything else
+ * evicted fails with AFE_E_BUDGET.  A request's result does not depend on
+ * the requests solved with it, nor on the entry its structure came from. */
 typedef struct afe_request {
   int64_t mesh_key;  /* cache key of the mesh                              */
   int32_t kind;      /* AFE_P1_TRI, AFE_P2_TRI, AFE_P1_TET                  */
@@ -355,6 +377,16 @@ typedef struct afe_request {
   double rtol;       /* relative residual target                           */
   int32_t max_it;    /* CG iteration limit                                 */
   int32_t keep_x;    /* 1: keep the solution on the host for afe_server_wait */
+  /* a mesh of the caller's instead of a structured one (coords != NULL):
+   * n_vert vertices (coords, vert_mark) and n_elem elements of `kind`
+   * (elem_vert), copied at submit; n, jitter, seed and move_* are unused.
+   * A later request with the same mesh_key and the same elements and marks
+   * re-solves that mesh on its new coordinates (values only). */
+  int64_t n_vert;
+  const double *coords;
+  int64_t n_elem;
+  const int32_t *elem_vert;
+  const int32_t *vert_mark;
 } afe_request;
 typedef struct afe_server afe_server;
 int afe_server_create(int32_t device, int64_t hbm_budget, int32_t max_slots,
@@ -375,6 +407,10 @@ int afe_server_info(afe_server *s, int64_t *device_bytes, int64_t *cached,
  * thread, out[3] waiting for a batch slot, out[4] solving; out[5], out[6]
  * the counts of new and moving-mesh requests */
 int afe_server_timing(afe_server *s, double *out);
+/* the same sums, the first n_out of: the seven above, then out[7] the set-up
+ * of new meshes that copied a cached entry's structure (part of out[0]) and
+ * out[8] their count */
+int afe_server_timing_n(afe_server *s, double *out, int32_t n_out);
 
 /* ---- measurement hooks (no counterpart in the reference) ------------------ */
 /* average duration of `reps` back-to-back launches of a level-0 kernel as the

@@ -132,3 +132,69 @@ def test_batch_matches_unbatched_iterations(ctx, problems):
         assert abs(st.iterations - res[i][1]) <= 1
         xr = x.numpy()
         assert np.linalg.norm(xr - res[i][0]) <= 1e-6 * np.linalg.norm(xr)
+
+
+@pytest.fixture(scope="module")
+def c4_members(ctx):
+    """32 members drawn as the 256-problem batch is: n = round(exp(U[ln 32,
+    ln 512])), even members P1 (accuracy 4) and odd P2 (accuracy 6), a =
+    exp(8-mode field of amplitude 0.5), f uniform in [-1, 1] at the DOFs"""
+    import afepack_amd as afe
+    rng = np.random.default_rng(4321)
+    sizes = np.rint(np.exp(rng.uniform(np.log(32), np.log(512), 32))).astype(int)
+    out = []
+    for i, n in enumerate(sizes):
+        kind, acc = (afe.P1_TRI, 4) if i % 2 == 0 else (afe.P2_TRI, 6)
+        c, ev, vm = afe.mesh_square(int(n), 0.0, 0)
+        sp = afe.Space(ctx, kind, c, ev, vm)
+        field = afe.field_random(8, 2, 0.5, 7000 + i)
+        A = afe.Matrix(sp)
+        A.assemble(afe.OP_VARSTIFF, acc, field)
+        b = sp.rhs(acc, afe.F_NODAL, nodal=np.random.default_rng(8000 + i).uniform(-1, 1, sp.n_dof))
+        A.apply_dirichlet(b)
+        g = afe.Amg(A, afe.AmgParams(lazy=1, smooth=1, build_fp32=1))
+        out.append({"sp": sp, "A": A, "b": b, "g": g})
+    return out
+
+
+def _run_members(ctx, probs, idx, tiles=(1, 1)):
+    import afepack_amd as afe
+    bt = afe.Batch(ctx, afe.PREC_FP32, smooth=1)
+    bt.set_tiles(*tiles)
+    xs, slots = {}, {}
+    for i in idx:
+        xs[i] = afe.DeviceVector(ctx, probs[i]["sp"].n_dof)
+        slots[i] = bt.add(probs[i]["g"], xs[i], probs[i]["b"], rtol=1e-10, max_it=1000)
+    bt.run()
+    out = {}
+    for i in idx:
+        done, st, code = bt.result(slots[i])
+        assert done and code == 0 and st.converged == 1 and st.rel_res_l2 <= 1e-10, (i, code)
+        out[i] = xs[i].numpy()
+    bt.close()
+    return out
+
+
+def test_c4_distribution_batch(ctx, c4_members):
+    """32 members of the C4 distribution in one batch: launch buckets
+    (slices per wavefront on level 0 and on the coarser levels) change no
+    bit of any member; members solved alone equal the mix bit for bit; each
+    member's true residual, computed here by scipy from its assembled matrix,
+    is at most 1e-10 of its right-hand side."""
+    import scipy.sparse as sps
+    probs = c4_members
+    idx = list(range(len(probs)))
+    print(f"case c4 distribution: {sum(p['sp'].n_dof for p in probs)} DOFs", flush=True)
+    base = _run_members(ctx, probs, idx)
+    for tiles in ((4, 2), (2, 8)):
+        other = _run_members(ctx, probs, idx, tiles)
+        for i in idx:
+            assert np.array_equal(other[i], base[i]), (tiles, i)
+    for i in (0, 1, 17, 30):
+        assert np.array_equal(_run_members(ctx, probs, [i])[i], base[i]), i
+    for i in idx:
+        p = probs[i]
+        rp, col = p["sp"].pattern()
+        A = sps.csr_matrix((p["A"].values(), col, rp), shape=(p["sp"].n_dof,) * 2)
+        b = p["b"].numpy()
+        assert np.linalg.norm(b - A @ base[i]) <= 1e-10 * np.linalg.norm(b), i
