@@ -147,6 +147,8 @@ def main():
     ap.add_argument("--xcd", type=int, default=-1,
                     help="slice schedule of the level-0 grid-stride kernels (0, 1, 2); -1 = autotune")
     ap.add_argument("--spmv-blocks", type=int, default=16384, help="grid of those kernels when --xcd is given")
+    ap.add_argument("--tail-rows", type=int, default=0,
+                    help="first level of the single-workgroup tail (rows); 0 = autotune")
     ap.add_argument("--kernels-only", action="store_true",
                     help="build the C2 system and hierarchy, then only time the level-0 kernels and a "
                          "plain copy (for profiler runs); prints one JSON line")
@@ -340,7 +342,7 @@ def run_c2(args):
     # the first level of the single-workgroup tail
     choices = [(50, 0.382), (2048, 0.0)] if args.nmpo == 0 else [(args.nmpo, args.nmpsd)]
     smooths = (1, 2) if args.smooth == 0 else (args.smooth,)
-    tails = (4096, 12288)
+    tails = (4096, 12288) if args.tail_rows == 0 else (args.tail_rows,)
     best = None
     for nmpo, nmpsd in choices:
         gg, t_b = build(nmpo, nmpsd)
