@@ -173,6 +173,36 @@ static inline unsigned sblk(int64_t nslices) {
   return g_tune.xcd == 1 ? (b + 7) / 8 * 8 : b;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of afe_error_qp_noalloc: the partial sums of its reduction live in
+// this file-scope device array (one copy per GPU, held until the process
+// ends), so that entry point allocates nothing.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in): it uses the
+// device address fetched once per device by hipGetSymbolAddress at context
+// creation, with that device current.
+// ---------------------------------------------------------------------------
+constexpr int NOALLOC_SCRATCH_DOUBLES = RED_BLOCKS; // one partial sum per reduction block
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+constexpr int kMaxDevices = 64;
+static double *g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// with `device` current: its copy of the scratch array, looked up once
+static void scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) throw afe::Error{AFE_E_NODEVICE, "device index beyond the scratch table"};
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void *p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = (double *)p;
+}
+
+static double *scratch_of(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
 extern "C" int afe_ctx_create(int device, afe_ctx **out) {
   API_BEGIN
   if (!out) return fail(AFE_E_ARG, "null output");
@@ -185,6 +215,7 @@ extern "C" int afe_ctx_create(int device, afe_ctx **out) {
   auto *c = new afe_ctx();
   c->device = device;
   HIPCHK(hipSetDevice(device));
+  scratch_init(device);
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&c->ev0));
   HIPCHK(hipEventCreate(&c->ev1));
@@ -999,6 +1030,25 @@ extern "C" int afe_error_qp(afe_space *s, const double *d_uh, int32_t acc, int32
   tot.alloc(s->ctx, 1);
   launch_error_qp(s, q, h1, d_uh, ex.p, part.p, tot.p);
   HIPCHK(hipMemcpyAsync(err, tot.p, sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  API_END
+}
+
+// afe_error_qp with the exact values and the result on the device and the
+// reduction's partial sums in the file-scope scratch array: no allocation
+extern "C" int afe_error_qp_noalloc(afe_space *s, const double *d_uh, int32_t acc, int32_t h1,
+                                    const double *d_exact, double *d_err) {
+  API_BEGIN
+  if (!s || !s->ctx || !d_uh || !d_exact || !d_err) return fail(AFE_E_ARG, "null argument");
+  const QuadRule *q = quad_find(s->h.tdim, acc);
+  if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
+  static_assert(RED_BLOCKS <= NOALLOC_SCRATCH_DOUBLES, "the reduction's partial sums must fit the scratch array");
+  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  HIPCHK(hipSetDevice(s->ctx->device));
+  double *scr = scratch_of(s->ctx->device);
+  if (!scr) return fail(AFE_E_HIP, "scratch array of this device not initialised (no context created on it)");
+  // every block overwrites its partial sum: nothing to clear between calls
+  launch_error_qp(s, q, h1, d_uh, d_exact, scr, d_err);
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
   API_END
 }

@@ -220,6 +220,21 @@ int afe_apply_dirichlet_values(afe_matrix *a, const int32_t *marks,
 int afe_error_qp(afe_space *s, const double *d_uh, int32_t acc, int32_t h1,
                  const double *h_exact, double *err);
 
+/* The same functional with everything on the device, allocating nothing:
+ * no counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  d_exact holds the exact values (gradients for h1 = 1) at the
+ * quadrature points as afe_error_qp's h_exact, d_err receives the error (one
+ * double, device memory).  The reduction's partial sums live in a
+ * file-scope device array of 1024 doubles, one per GPU, held until the
+ * process ends; the array's size does not depend on the mesh, so no mesh is
+ * too large for it.  Every partial sum is overwritten by each call (nothing
+ * to clear); calls that overlap on one GPU race on the array.  Returns with
+ * the context stream synchronised; the result equals afe_error_qp's bit for
+ * bit (same kernels). */
+int afe_error_qp_noalloc(afe_space *s, const double *d_uh, int32_t acc,
+                         int32_t h1, const double *d_exact, double *d_err);
+
 /* ---- error functionals -------------------------------------------------- */
 /* Functional::L2Error / H1SemiError (Functional.templates.h:296-317,399-426)
  * of the FE function d_uh against u = prod sin(pi x_d). */
