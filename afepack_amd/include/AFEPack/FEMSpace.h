@@ -2,12 +2,13 @@
 //
 // Mirrors the interface the reference path uses (include/AFEPack/
 // {Geometry,EasyMesh,TemplateElement,FEMSpace}.h): meshes, template elements
-// named by their data files (triangle.1 / triangle.2 / tetrahedron.1),
+// described by their data files (triangle.1 / triangle.2 / tetrahedron.1),
 // FEMSpace::buildElement/buildDof/buildDofBoundaryMark and the Element
 // queries a getElementMatrix override needs.  The DOF numbering and pattern
 // come from the C ABI (afe_space_create); basis functions are evaluated on the
 // host here only for user-written element matrices.
 #pragma once
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdlib>
@@ -237,33 +238,142 @@ inline void buildCubeMesh(Mesh<3, 3> &m, int n, double jitter = 0.0, uint64_t se
 
 // ---- template elements --------------------------------------------------------
 // The reference reads geometry / DOF / basis descriptions from data files and
-// dlopens their plugins; here the file names select one of the supported
-// template elements and the numerics live in the library.
+// dlopens their plugins (TemplateElement.templates.h).  Here readData looks
+// the file up as the reference does (the directories of
+// AFEPACK_TEMPLATE_PATH, then ".", src/Miscellaneous.cpp:145-185) and, when
+// it is found, reads what selects the element: the template geometry's
+// vertex count, the DOF layout per geometry dimension and the basis
+// functions' count and polynomial order.  TemplateElement::reinit picks the
+// supported element from that content (any file name), and fails loudly on
+// content no supported element has.  A file that is not found leaves only
+// its name, and the name then selects the element (triangle.1,
+// triangle.2, tetrahedron.1), so that programs run where the template
+// directory is absent; the numerics of the three elements live in the
+// library either way.
+namespace detail {
+inline std::string find_template_file(const std::string &name) {
+  std::vector<std::string> dirs;
+  if (const char *p = std::getenv("AFEPACK_TEMPLATE_PATH")) {
+    std::string all = p, cur;
+    for (char c : all) {
+      if (c == ':') {
+        if (!cur.empty()) dirs.push_back(cur);
+        cur.clear();
+      } else {
+        cur += c;
+      }
+    }
+    if (!cur.empty()) dirs.push_back(cur);
+  }
+  dirs.push_back(".");
+  for (const auto &d : dirs) {
+    const std::string f = d + "/" + name;
+    std::ifstream in(f);
+    if (in) return f;
+  }
+  return "";
+}
+
+// whitespace-separated tokens with '#' comments (to the end of the line)
+// removed, as the reference's filtered stream reads these files
+inline std::vector<std::string> template_tokens(const std::string &path) {
+  std::ifstream in(path);
+  std::vector<std::string> tok;
+  std::string line;
+  while (std::getline(in, line)) {
+    const size_t h = line.find('#');
+    if (h != std::string::npos) line.resize(h);
+    std::istringstream ls(line);
+    std::string t;
+    while (ls >> t) tok.push_back(t);
+  }
+  return tok;
+}
+
+inline int template_int(const std::vector<std::string> &tok, size_t i, const std::string &file) {
+  if (i >= tok.size()) throw std::runtime_error("template file ends early: " + file);
+  try {
+    return std::stoi(tok[i]);
+  } catch (const std::exception &) {
+    throw std::runtime_error("template file: integer expected at token " + std::to_string(i) + " of " + file);
+  }
+}
+} // namespace detail
+
+// TemplateGeometry::readData (.tmp_geo, TemplateElement.templates.h): the
+// volume function's library and name, then the vertex count and vertices
 template <int TDIM> class TemplateGeometry {
 public:
-  void readData(const std::string &f) { file = f; }
+  void readData(const std::string &f) {
+    file = f;
+    n_vertex = -1;
+    const std::string path = detail::find_template_file(f);
+    if (path.empty()) return;
+    const auto tok = detail::template_tokens(path);
+    n_vertex = detail::template_int(tok, 2, path);
+  }
   std::string file;
+  int n_vertex = -1; // -1: no file read
 };
 template <int TDIM, int DOW = TDIM> class CoordTransform {
 public:
   void readData(const std::string &f) { file = f; }
   std::string file;
 };
+// TemplateDOF::readData (.tmp_dof, TemplateElement.templates.h:420-430): the
+// entry count, then (dimension, geometry index, DOF count) per entry
 template <int TDIM> class TemplateDOF {
 public:
   TemplateDOF() = default;
   explicit TemplateDOF(TemplateGeometry<TDIM> &) {}
   void reinit(TemplateGeometry<TDIM> &) {}
-  void readData(const std::string &f) { file = f; }
+  void readData(const std::string &f) {
+    file = f;
+    n_dof = -1;
+    per_dim.fill(0);
+    const std::string path = detail::find_template_file(f);
+    if (path.empty()) return;
+    const auto tok = detail::template_tokens(path);
+    const int n = detail::template_int(tok, 0, path);
+    if (n < 0) throw std::runtime_error("negative DOF entry count: " + path);
+    n_dof = 0;
+    for (int e = 0; e < n; e++) {
+      const int d = detail::template_int(tok, 1 + 3 * e, path), c = detail::template_int(tok, 3 + 3 * e, path);
+      if (d < 0 || d > TDIM) throw std::runtime_error("template DOF of dimension " + std::to_string(d) + ": " + path);
+      per_dim[d] += c;
+      n_dof += c;
+    }
+  }
   std::string file;
+  int n_dof = -1; // -1: no file read
+  std::array<int, 4> per_dim{};
 };
+// BasisFunctionAdmin::readData (.bas_fun): the basis count, then per basis
+// its DOF's geometry (dimension, index), the interpolation point, the
+// identity (order, TDIM exponents, flag), the plugin and its two functions
 template <class value_type, int DOW, int TDIM = DOW> class BasisFunctionAdmin {
 public:
   BasisFunctionAdmin() = default;
   explicit BasisFunctionAdmin(TemplateDOF<TDIM> &) {}
   void reinit(TemplateDOF<TDIM> &) {}
-  void readData(const std::string &f) { file = f; }
+  void readData(const std::string &f) {
+    file = f;
+    n_basis = -1;
+    max_order = 0;
+    const std::string path = detail::find_template_file(f);
+    if (path.empty()) return;
+    const auto tok = detail::template_tokens(path);
+    const int n = detail::template_int(tok, 0, path);
+    if (n < 0) throw std::runtime_error("negative basis count: " + path);
+    const size_t per = 2 + TDIM + (TDIM + 2) + 3;
+    for (int b = 0; b < n; b++)
+      max_order = std::max(max_order, detail::template_int(tok, 1 + b * per + 2 + TDIM, path));
+    if (tok.size() < 1 + (size_t)n * per) throw std::runtime_error("template file ends early: " + path);
+    n_basis = n;
+  }
   std::string file;
+  int n_basis = -1; // -1: no file read
+  int max_order = 0;
 };
 
 template <int TDIM> struct QuadratureInfo {
@@ -286,6 +396,10 @@ template <class value_type, int DOW, int TDIM = DOW> class TemplateElement {
 public:
   void reinit(TemplateGeometry<TDIM> &g, TemplateDOF<TDIM> &d, CoordTransform<TDIM, DOW> &,
               BasisFunctionAdmin<value_type, DOW, TDIM> &b) {
+    if (b.n_basis >= 0 || d.n_dof >= 0 || g.n_vertex >= 0) {
+      kind = from_content(g, d, b);
+      return;
+    }
     const std::string name = b.file.empty() ? d.file : b.file;
     if (TDIM == 3 && name.find("tetrahedron.1") != std::string::npos)
       kind = AFE_P1_TET;
@@ -305,6 +419,26 @@ public:
   }
   int n_dof() const { return kind == AFE_P1_TRI ? 3 : kind == AFE_P2_TRI ? 6 : 4; }
   int kind = 0;
+
+private:
+  // the supported elements as their data files describe them: a simplex
+  // (TDIM + 1 vertices); P1: one DOF per vertex, TDIM + 1 bases of order 1;
+  // P2 (triangles): one DOF per vertex and per edge, 6 bases of order 2
+  static int from_content(const TemplateGeometry<TDIM> &g, const TemplateDOF<TDIM> &d,
+                          const BasisFunctionAdmin<value_type, DOW, TDIM> &b) {
+    const std::string what = g.file + " / " + d.file + " / " + b.file;
+    if (g.n_vertex >= 0 && g.n_vertex != TDIM + 1)
+      throw std::runtime_error("template element not supported on this path (not a simplex): " + what);
+    if (d.n_dof >= 0 && b.n_basis >= 0 && d.n_dof != b.n_basis)
+      throw std::runtime_error("template DOF and basis files disagree on the DOF count: " + what);
+    const int n = b.n_basis >= 0 ? b.n_basis : d.n_dof;
+    const bool vertex_dofs = d.n_dof < 0 || d.per_dim[0] == TDIM + 1;
+    if (n == TDIM + 1 && vertex_dofs && (d.n_dof < 0 || d.per_dim[1] == 0) && (b.n_basis < 0 || b.max_order == 1))
+      return TDIM == 2 ? AFE_P1_TRI : AFE_P1_TET;
+    if (TDIM == 2 && n == 6 && vertex_dofs && (d.n_dof < 0 || d.per_dim[1] == 3) && (b.n_basis < 0 || b.max_order == 2))
+      return AFE_P2_TRI;
+    throw std::runtime_error("template element not supported on this path: " + what);
+  }
 };
 
 template <class value_type, int DIM, int DOW = DIM, int TDIM = DIM> class FEMSpace;
