@@ -62,6 +62,9 @@ SIGS = {
     "afe_space_update_coords": (C.c_int, [P, P]),
     "afe_matrix_create": (C.c_int, [P, PP]),
     "afe_matrix_create_csr": (C.c_int, [P, I64, P, P, P, PP]),
+    "afe_matrix_create_pair": (C.c_int, [P, P, P]),
+    "afe_matrix_info": (C.c_int, [P, P, P, P, P]),
+    "afe_matrix_pattern": (C.c_int, [P, P, P]),
     "afe_matrix_destroy": (C.c_int, [P]),
     "afe_assemble": (C.c_int, [P, I32, I32, PField]),
     "afe_assemble_elements": (C.c_int, [P, P]),

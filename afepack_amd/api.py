@@ -308,11 +308,18 @@ class Space:
 
 
 class Matrix:
-    def __init__(self, space=None, ctx=None, csr=None):
+    def __init__(self, space=None, ctx=None, csr=None, space1=None):
         h = C.c_void_p()
-        if space is not None:
+        if space is not None and space1 is not None:
+            # BilinearOperator(sp0, sp1): two spaces on one mesh
+            _chk(_lib().afe_matrix_create_pair(space.h, space1.h, C.byref(h)))
+            m, n, nnz, mc = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
+            _chk(_lib().afe_matrix_info(h, C.byref(m), C.byref(n), C.byref(nnz), C.byref(mc)))
+            self.m, self.n, self.nnz, self.max_coupling = m.value, n.value, nnz.value, mc.value
+        elif space is not None:
             _chk(_lib().afe_matrix_create(space.h, C.byref(h)))
             self.n, self.nnz = space.n_dof, space.nnz
+            self.m = self.n
         else:
             rp, col, val = csr
             rp = np.ascontiguousarray(rp, dtype=np.int64)
@@ -321,8 +328,17 @@ class Matrix:
             _chk(_lib().afe_matrix_create_csr(ctx.h, len(rp) - 1, _ptr(rp), _ptr(col), _ptr(val),
                                               C.byref(h)))
             self.n, self.nnz = len(rp) - 1, int(rp[-1])
+            self.m = self.n
         self.space = space
+        self.space1 = space1
         self.h = h
+
+    def pattern(self):
+        """(row_ptr, col) of the compressed pattern"""
+        rp = np.empty(self.m + 1, dtype=np.int64)
+        col = np.empty(self.nnz, dtype=np.int64)
+        _chk(_lib().afe_matrix_pattern(self.h, _ptr(rp), _ptr(col)))
+        return rp, col
 
     def assemble(self, op, acc, field=None):
         fp, _keep = _field_ptr(field)

@@ -546,6 +546,13 @@ struct afe_matrix {
   int ke_layout = 0;  // how Ke holds them (k_gather's LAYOUT)
   DevBuf<int64_t> t_ptr, t_k; // transposed index map (afe_matrix_tvmult)
   DevBuf<int32_t> t_row;
+  // a matrix of two FE spaces on one mesh (afe_matrix_create_pair): rows are
+  // DOFs of ps0, columns DOFs of ps1; its pattern sits in h_rp/h_col and
+  // own_rp/own_col, s stays null
+  afe_space *ps0 = nullptr, *ps1 = nullptr;
+  DevBuf<uint8_t> pair_slot; // PairPattern::slot
+  int64_t pair_seg = 0;
+  int pair_max_coupling = 0;
   const int64_t *rp() const { return s ? s->rp.p : own_rp.p; }
   const int32_t *col() const { return s ? s->col.p : own_col.p; }
   const std::vector<int64_t> &hrp() const { return s ? s->h.rp : h_rp; }
@@ -601,6 +608,59 @@ extern "C" int afe_matrix_create_csr(afe_ctx *ctx, int64_t m, const int64_t *row
     HIPCHK(hipMemsetAsync(a->val.p, 0, a->nnz * sizeof(double), ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   *out = a.release();
+  API_END
+}
+
+// BilinearOperator over two FE spaces of one mesh (the same-mesh branch of
+// BilinearOperator.templates.h:64-172): the rectangular pattern and the
+// gather map of its element matrices, values zero
+extern "C" int afe_matrix_create_pair(afe_space *s0, afe_space *s1, afe_matrix **out) {
+  API_BEGIN
+  if (!s0 || !s1 || !out) return fail(AFE_E_ARG, "null argument");
+  if (!s0->ctx || s0->ctx != s1->ctx) return fail(AFE_E_ARG, "the two spaces need one device context");
+  std::lock_guard<std::recursive_mutex> lk(s0->ctx->mu);
+  HIPCHK(hipSetDevice(s0->ctx->device));
+  PairPattern pp;
+  pair_build(pp, s0->h, s1->h);
+  if (pp.seg_max * 8 > 64 * 1024) return fail(AFE_E_ARG, "rows too long for the LDS gather");
+  auto a = std::make_unique<afe_matrix>();
+  a->ctx = s0->ctx;
+  a->ps0 = s0;
+  a->ps1 = s1;
+  a->m = pp.m;
+  a->n = pp.n;
+  a->nnz = pp.rp[pp.m];
+  a->h_rp = std::move(pp.rp);
+  a->h_col = std::move(pp.col);
+  a->pair_seg = pp.seg_max;
+  a->pair_max_coupling = pp.max_coupling;
+  a->own_rp.upload(a->ctx, a->h_rp);
+  a->own_col.upload(a->ctx, a->h_col);
+  a->pair_slot.upload(a->ctx, pp.slot);
+  a->val.alloc(a->ctx, a->nnz);
+  HIPCHK(hipMemsetAsync(a->val.p, 0, a->nnz * sizeof(double), a->ctx->stream));
+  HIPCHK(hipStreamSynchronize(a->ctx->stream));
+  *out = a.release();
+  API_END
+}
+
+extern "C" int afe_matrix_info(const afe_matrix *a, int64_t *m, int64_t *n, int64_t *nnz, int32_t *max_coupling) {
+  API_BEGIN
+  if (!a) return fail(AFE_E_ARG, "null matrix");
+  if (m) *m = a->m;
+  if (n) *n = a->n;
+  if (nnz) *nnz = a->nnz;
+  if (max_coupling) *max_coupling = a->ps0 ? a->pair_max_coupling : -1;
+  API_END
+}
+
+extern "C" int afe_matrix_pattern(const afe_matrix *a, int64_t *rp, int64_t *col) {
+  API_BEGIN
+  if (!a || !rp || !col) return fail(AFE_E_ARG, "null argument");
+  const std::vector<int64_t> &r = a->hrp();
+  const std::vector<int32_t> &c = a->hcol();
+  for (int64_t i = 0; i <= a->m; i++) rp[i] = r[i];
+  for (int64_t k = 0; k < a->nnz; k++) col[k] = c[k];
   API_END
 }
 
@@ -672,8 +732,57 @@ static void launch_gather(afe_matrix *a) {
   HIPCHK(hipGetLastError());
 }
 
+// the two-space element matrices (entry-major, LAYOUT 2) or the caller's
+// (element-major n0 x n1, LAYOUT 0) gathered into the rectangular CSR values
+template <int N0, int N1, int LAYOUT> static void launch_gather_pair_nl(afe_matrix *a) {
+  afe_space *s0 = a->ps0;
+  const size_t lds = (size_t)a->pair_seg * sizeof(double);
+  k_gather_pair<N0, N1, LAYOUT><<<nblk(a->m), 256, lds, a->ctx->stream>>>(
+      a->m, a->own_rp.p, s0->adj_ptr.p, s0->adj.p, a->pair_slot.p, s0->h.n_elem, a->Ke.p, a->val.p);
+}
+
+template <int LAYOUT> static void launch_gather_pair(afe_matrix *a) {
+  const int n0 = a->ps0->h.n_loc, n1 = a->ps1->h.n_loc;
+  if (n0 == 3 && n1 == 3) launch_gather_pair_nl<3, 3, LAYOUT>(a);
+  else if (n0 == 3 && n1 == 6) launch_gather_pair_nl<3, 6, LAYOUT>(a);
+  else if (n0 == 6 && n1 == 3) launch_gather_pair_nl<6, 3, LAYOUT>(a);
+  else if (n0 == 6 && n1 == 6) launch_gather_pair_nl<6, 6, LAYOUT>(a);
+  else launch_gather_pair_nl<4, 4, LAYOUT>(a);
+  HIPCHK(hipGetLastError());
+}
+
+template <int K0, int K1> static void launch_mass_pair(afe_matrix *a, const QuadRule *q) {
+  afe_space *s0 = a->ps0;
+  const int64_t ne = s0->h.n_elem;
+  k_element_mass_pair<K0, K1><<<nblk(ne), 256, 0, a->ctx->stream>>>(
+      ne, s0->coords.p, s0->ev.p, s0->ctx->d_quad + q->off, s0->ctx->d_bary + q->off, q->n, a->Ke.p);
+  HIPCHK(hipGetLastError());
+}
+
+// L2InnerProduct of two spaces on one mesh (BilinearOperator.templates.h:
+// 300-326, the quadrature of element 0)
+static int assemble_pair(afe_matrix *a, int32_t op, int32_t acc) {
+  if (op != AFE_OP_MASS) return fail(AFE_E_ARG, "two-space matrices assemble the L2 inner product (AFE_OP_MASS) only");
+  afe_space *s0 = a->ps0;
+  const QuadRule *q = quad_find(s0->h.tdim, acc);
+  if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
+  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  HIPCHK(hipSetDevice(a->ctx->device));
+  a->Ke.alloc(a->ctx, (size_t)s0->h.n_elem * s0->h.n_loc * a->ps1->h.n_loc);
+  const int k0 = s0->h.kind, k1 = a->ps1->h.kind;
+  if (k0 == P1_TRI && k1 == P1_TRI) launch_mass_pair<P1_TRI, P1_TRI>(a, q);
+  else if (k0 == P1_TRI && k1 == P2_TRI) launch_mass_pair<P1_TRI, P2_TRI>(a, q);
+  else if (k0 == P2_TRI && k1 == P1_TRI) launch_mass_pair<P2_TRI, P1_TRI>(a, q);
+  else if (k0 == P2_TRI && k1 == P2_TRI) launch_mass_pair<P2_TRI, P2_TRI>(a, q);
+  else launch_mass_pair<P1_TET, P1_TET>(a, q);
+  launch_gather_pair<2>(a);
+  HIPCHK(hipStreamSynchronize(a->ctx->stream));
+  return AFE_OK;
+}
+
 extern "C" int afe_assemble(afe_matrix *a, int32_t op, int32_t acc, const afe_field *coef) {
   API_BEGIN
+  if (a && a->ps0) return assemble_pair(a, op, acc);
   if (!a || !a->s) return fail(AFE_E_ARG, "matrix has no FE space");
   if (op < AFE_OP_STIFF || op > AFE_OP_VARSTIFF) return fail(AFE_E_ARG, "unknown operator");
   if (op == AFE_OP_VARSTIFF && (!coef || coef->n_modes <= 0 || coef->dim != a->s->h.tdim))
@@ -703,6 +812,7 @@ extern "C" int afe_assemble(afe_matrix *a, int32_t op, int32_t acc, const afe_fi
 extern "C" int afe_assemble_kernel_timing(afe_matrix *a, int32_t op, int32_t acc, const afe_field *coef,
                                           int32_t reps, double *elem_ms, double *gather_ms, double *bytes) {
   API_BEGIN
+  if (a && a->ps0) return fail(AFE_E_ARG, "kernel timing covers one-space assembly only");
   if (!a || !a->s || reps < 1) return fail(AFE_E_ARG, "bad arguments");
   if (op < AFE_OP_STIFF || op > AFE_OP_VARSTIFF) return fail(AFE_E_ARG, "unknown operator");
   if (op == AFE_OP_VARSTIFF && (!coef || coef->n_modes <= 0 || coef->dim != a->s->h.tdim))
@@ -750,6 +860,14 @@ extern "C" int afe_assemble_kernel_timing(afe_matrix *a, int32_t op, int32_t acc
 
 extern "C" int afe_assemble_elements(afe_matrix *a, const double *h_elem_mat) {
   API_BEGIN
+  if (a && a->ps0 && h_elem_mat) {
+    std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+    HIPCHK(hipSetDevice(a->ctx->device));
+    a->Ke.upload(a->ctx, h_elem_mat, (size_t)a->ps0->h.n_elem * a->ps0->h.n_loc * a->ps1->h.n_loc);
+    launch_gather_pair<0>(a);
+    HIPCHK(hipStreamSynchronize(a->ctx->stream));
+    return AFE_OK;
+  }
   if (!a || !a->s || !h_elem_mat) return fail(AFE_E_ARG, "null argument");
   std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
@@ -899,6 +1017,7 @@ extern "C" int afe_rhs(afe_space *s, int32_t acc, int32_t fkind, const afe_field
 extern "C" int afe_apply_dirichlet(afe_matrix *a, double *d_u, double *d_f) {
   API_BEGIN
   (void)d_u;
+  if (a && a->ps0) return fail(AFE_E_ARG, "Dirichlet elimination needs the square matrix of one space");
   if (!a || !a->s || !d_f) return fail(AFE_E_ARG, "matrix without FE space or null rhs");
   std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
@@ -966,6 +1085,7 @@ extern "C" int afe_apply_dirichlet_values(afe_matrix *a, const int32_t *marks, i
                                           double *d_u, double *d_f) {
   API_BEGIN
   (void)d_u;
+  if (a && a->ps0) return fail(AFE_E_ARG, "Dirichlet elimination needs the square matrix of one space");
   if (!a || !a->s || !d_f || !h_g || (n_marks > 0 && !marks)) return fail(AFE_E_ARG, "bad arguments");
   std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
@@ -1625,6 +1745,7 @@ static void finish_levels(afe_amg *g) {
 extern "C" int afe_amg_create(afe_matrix *a, const afe_amg_params *p, afe_amg **out) {
   API_BEGIN
   if (!a || !out) return fail(AFE_E_ARG, "null argument");
+  if (a->m != a->n || a->ps0) return fail(AFE_E_ARG, "AMG needs the square matrix of one space");
   auto g = std::make_unique<afe_amg>();
   g->ctx = a->ctx;
   g->A0 = a;

@@ -237,18 +237,24 @@ void space_build(Space &s, int kind, int64_t n_vert, const double *coords,
 #pragma omp parallel for schedule(dynamic, 1)
   for (int64_t c = 0; c < nch; c++) {
     std::vector<int32_t> buf;
+    // seen[d] == i: d already listed for row i (duplicates are dropped
+    // before the sort, which then orders ~15 distinct columns, not ~100)
+    std::vector<int32_t> seen(n_dof, -1);
     std::vector<int32_t> &out = chunk_cols[c];
     for (int64_t i = c * CH; i < std::min<int64_t>(n_dof, (c + 1) * CH); i++) {
       buf.clear();
+      seen[i] = (int32_t)i;
       for (int64_t p = s.adj_ptr[i]; p < s.adj_ptr[i + 1]; p++) {
         int64_t e = s.adj[p] >> 3;
         for (int b = 0; b < nl; b++) {
           int32_t d = s.elem_dof[nl * e + b];
-          if (d != i) buf.push_back(d);
+          if (seen[d] != i) {
+            seen[d] = (int32_t)i;
+            buf.push_back(d);
+          }
         }
       }
       std::sort(buf.begin(), buf.end());
-      buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
       rowlen[i] = (int32_t)buf.size() + 1;
       out.push_back((int32_t)i);
       out.insert(out.end(), buf.begin(), buf.end());
@@ -284,6 +290,65 @@ void space_build(Space &s, int kind, int64_t n_vert, const double *coords,
       }
     }
   }
+}
+
+void pair_build(PairPattern &pp, const Space &s0, const Space &s1) {
+  // BilinearOperator::buildSparsityPattern, two spaces on one mesh
+  // (BilinearOperator.templates.h:64-95): element e couples every space-0
+  // DOF of e with every space-1 DOF of e; SparsityPattern::reinit/compress
+  // (SparsityPattern.h:16-70,193-235) keeps a square pattern's diagonal
+  // first in each row (present even where no element couples it) and sorts
+  // the other columns ascending
+  if (s0.n_elem != s1.n_elem || s0.ev != s1.ev || s0.coords != s1.coords)
+    throw Error{-1, "the two spaces must be built on the same mesh"};
+  const int n0 = s0.n_loc, n1 = s1.n_loc;
+  pp.m = s0.n_dof;
+  pp.n = s1.n_dof;
+  pp.n1 = n1;
+  const bool diag_first = pp.m == pp.n;
+  pp.rp.assign(pp.m + 1, 0);
+  pp.col.clear();
+  pp.slot.assign(s0.adj.size() * n1, 0);
+  std::vector<int32_t> row;
+  for (int64_t i = 0; i < pp.m; i++) {
+    row.clear();
+    for (int64_t p = s0.adj_ptr[i]; p < s0.adj_ptr[i + 1]; p++) {
+      const int64_t e = s0.adj[p] >> 3;
+      for (int b = 0; b < n1; b++) row.push_back(s1.elem_dof[e * n1 + b]);
+    }
+    std::sort(row.begin(), row.end());
+    row.erase(std::unique(row.begin(), row.end()), row.end());
+    if (diag_first) {
+      auto it = std::lower_bound(row.begin(), row.end(), (int32_t)i);
+      if (it != row.end() && *it == i) row.erase(it);
+      row.insert(row.begin(), (int32_t)i);
+    }
+    if (row.size() > 255) throw Error{-1, "a row of the two-space pattern exceeds 255 entries"};
+    const size_t off = diag_first ? 1 : 0; // the sorted part of the row
+    for (int64_t p = s0.adj_ptr[i]; p < s0.adj_ptr[i + 1]; p++) {
+      const int64_t e = s0.adj[p] >> 3;
+      for (int b = 0; b < n1; b++) {
+        const int32_t c = s1.elem_dof[e * n1 + b];
+        pp.slot[p * n1 + b] = (uint8_t)(diag_first && c == i
+                                            ? 0
+                                            : std::lower_bound(row.begin() + off, row.end(), c) - row.begin());
+      }
+    }
+    pp.col.insert(pp.col.end(), row.begin(), row.end());
+    pp.rp[i + 1] = (int64_t)pp.col.size();
+  }
+  pp.seg_max = 0;
+  for (int64_t r0 = 0; r0 < pp.m; r0 += 256)
+    pp.seg_max = std::max(pp.seg_max, pp.rp[std::min(r0 + 256, pp.m)] - pp.rp[r0]);
+  // buildDofInfo, same mesh (BilinearOperator.templates.h:131-171): each
+  // element adds its space-1 DOF count to the coupling count of each of its
+  // space-0 DOFs; the maximum, capped at the column count
+  std::vector<int64_t> cnt(pp.m, 0);
+  for (int64_t e = 0; e < s0.n_elem; e++)
+    for (int a = 0; a < n0; a++) cnt[s0.elem_dof[e * n0 + a]] += n1;
+  int64_t mx = 0;
+  for (int64_t c : cnt) mx = std::max(mx, c);
+  pp.max_coupling = (int)std::min<int64_t>(mx, pp.n);
 }
 
 // ===========================================================================

@@ -77,6 +77,22 @@ void space_build(Space &s, int kind, int64_t n_vert, const double *coords,
                  int64_t n_side = 0, const int32_t *side_vert = nullptr,
                  const int32_t *side_mark = nullptr);
 
+// BilinearOperator::buildSparsityPattern with two FE spaces on one mesh
+// (BilinearOperator.templates.h:64-111 and buildDofInfo :115-172, the
+// same-mesh branch): row i (a DOF of space 0) couples with every DOF of
+// space 1 on the elements holding i; the compressed pattern sorts each row
+// ascending, after the diagonal when the pattern is square.
+struct PairPattern {
+  int64_t m = 0, n = 0;
+  int n1 = 0;
+  std::vector<int64_t> rp;
+  std::vector<int32_t> col;
+  std::vector<uint8_t> slot; // slot[p * n1 + b]: position in row i of column b of adjacency entry p of i
+  int64_t seg_max = 0;       // largest 256-row value segment (gather LDS)
+  int max_coupling = 0;      // nMaxCouplingDof
+};
+void pair_build(PairPattern &pp, const Space &s0, const Space &s1);
+
 struct AmgParams {
   double alpha = 0.25;
   unsigned nmpo = 50;

@@ -182,6 +182,86 @@ __global__ __launch_bounds__(256) void k_gather(
 }
 
 // ---------------------------------------------------------------------------
+// two FE spaces on one mesh (BilinearOperator's same-mesh branch):
+// L2InnerProduct's element matrix (BilinearOperator.templates.h:300-326, the
+// quadrature and Jacobian of element 0): K(j,k) += Jxw * v0_j * v1_k,
+// stored entry-major (entry j * N1 + k of element e at Ke[.. * n_elem + e])
+// ---------------------------------------------------------------------------
+template <int K0, int K1>
+__global__ __launch_bounds__(256) void k_element_mass_pair(int64_t n_elem, const double *__restrict__ coords,
+                                                           const int32_t *__restrict__ ev,
+                                                           const double *__restrict__ qd,
+                                                           const double *__restrict__ qb, int nq,
+                                                           double *__restrict__ Ke) {
+  constexpr int D = KindInfo<K0>::D, N0 = KindInfo<K0>::NL, N1 = KindInfo<K1>::NL;
+  static_assert(KindInfo<K1>::D == D, "the two spaces share the mesh");
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_elem) return;
+  double v[D + 1][D];
+  load_elem<K0>(e, coords, ev, v);
+  const double volume = tmpl_volume<D>();
+  const double jac = l2g_jacobian<D>(v);
+  double K[N0][N1];
+#pragma unroll
+  for (int j = 0; j < N0; j++)
+#pragma unroll
+    for (int k = 0; k < N1; k++) K[j][k] = 0.0;
+  for (int l = 0; l < nq; l++) {
+    double x[D];
+    bary_to_global<D>(qb + l * (D + 1), v, x);
+    const double Jxw = qd[l * (D + 1) + D] * jac * volume;
+    double v0[N0], v1[N1];
+    basis_val<K0>(x, v, v0);
+    basis_val<K1>(x, v, v1);
+#pragma unroll
+    for (int j = 0; j < N0; j++)
+#pragma unroll
+      for (int k = 0; k < N1; k++) K[j][k] += Jxw * v0[j] * v1[k];
+  }
+#pragma unroll
+  for (int j = 0; j < N0; j++)
+#pragma unroll
+    for (int k = 0; k < N1; k++) Ke[(int64_t)(j * N1 + k) * n_elem + e] = K[j][k];
+}
+
+// the rectangular gather: row i of space 0 sums, in ascending element
+// order, row a of each adjacent element matrix (N1 entries) into its slots.
+// LAYOUT 0: element-major N0 x N1 row-major (caller's matrices), 2:
+// entry-major as k_element_mass_pair writes them.
+template <int N0, int N1, int LAYOUT>
+__global__ __launch_bounds__(256) void k_gather_pair(int64_t n_rows, const int64_t *__restrict__ rp,
+                                                     const int64_t *__restrict__ adj_ptr,
+                                                     const int32_t *__restrict__ adj,
+                                                     const uint8_t *__restrict__ slot, int64_t n_elem,
+                                                     const double *__restrict__ Ke, double *__restrict__ val) {
+  extern __shared__ double seg[];
+  const int64_t r0 = (int64_t)blockIdx.x * 256;
+  const int64_t r1 = r0 + 256 < n_rows ? r0 + 256 : n_rows;
+  const int64_t base = rp[r0];
+  const int64_t segn = rp[r1] - base;
+  for (int64_t k = threadIdx.x; k < segn; k += 256) seg[k] = 0.0;
+  __syncthreads();
+  const int64_t i = r0 + threadIdx.x;
+  if (i < r1) {
+    double *row = seg + (rp[i] - base);
+    for (int64_t p = adj_ptr[i]; p < adj_ptr[i + 1]; p++) {
+      const int32_t ea = adj[p];
+      const int64_t e = ea >> 3;
+      const int a = ea & 7;
+      const uint8_t *sl = slot + p * N1;
+      double kv[N1];
+#pragma unroll
+      for (int b = 0; b < N1; b++)
+        kv[b] = LAYOUT == 0 ? Ke[(e * N0 + a) * N1 + b] : Ke[(int64_t)(a * N1 + b) * n_elem + e];
+#pragma unroll
+      for (int b = 0; b < N1; b++) row[sl[b]] += kv[b];
+    }
+  }
+  __syncthreads();
+  for (int64_t k = threadIdx.x; k < segn; k += 256) val[base + k] = seg[k];
+}
+
+// ---------------------------------------------------------------------------
 // right-hand side: element load vectors, then a per-DOF gather in ascending
 // element order (Operator::L2Discretize, Operator.discretize.templates.h:80-152)
 // ---------------------------------------------------------------------------

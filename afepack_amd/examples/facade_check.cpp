@@ -21,7 +21,11 @@
 // lz1.f64 (level-1 values of a lazy hierarchy after lazyReinit on the moved
 // matrix), hl1.f64 (level-1 values after two lazyReinit calls on a
 // host-assembled matrix with values 3 K), foreign.txt (1 when lazyReinit with
-// another pattern object reached the reference's abort).
+// another pattern object reached the reference's abort), L2x_rp.i64
+// L2x_col.i64 L2x.f64 (L2InnerProduct of this space and a second one on the
+// same mesh: the other order in 2D, another P1 space in 3D), L2xd.f64 (a
+// derived two-space operator doubling it, evaluated on the host), infox.txt
+// (its nMaxCouplingDof, nDof0, nDof1, m, n).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -104,6 +108,19 @@ template <int DIM> struct StiffPlusMass : public BilinearOperator<DIM, double> {
         for (int k = 0; k < n1; k++)
           this->elementMatrix(j, k) += Jxw * (innerProduct(bg[j][l], bg[k][l]) + bv[j][l] * bv[k][l]);
     }
+  }
+};
+
+// a user operator on two spaces: L2InnerProduct's element matrix, doubled
+template <int DIM> struct DoubledL2 : public L2InnerProduct<DIM, double> {
+  DoubledL2(FEMSpace<double, DIM> &s0, FEMSpace<double, DIM> &s1) : L2InnerProduct<DIM, double>(s0, s1) {}
+  void getElementMatrix(const Element<double, DIM> &e0, const Element<double, DIM> &e1,
+                        const typename ActiveElementPairIterator<DIM>::State s =
+                            ActiveElementPairIterator<DIM>::EQUAL) override {
+    L2InnerProduct<DIM, double>::getElementMatrix(e0, e1, s);
+    DenseMatrix<double> &m = this->elementMatrix();
+    for (size_t i = 0; i < m.m(); i++)
+      for (size_t j = 0; j < m.n(); j++) m(i, j) *= 2.0;
   }
 };
 
@@ -231,6 +248,37 @@ template <int DIM> static int run(int kind, int n, const std::string &out, const
   KM.build();
   dump_mat(out + "/KM.f64", KM);
   std::ofstream(out + "/info.txt") << KM.nMaxCouplingDof() << " " << KM.nDof0() << " " << KM.nDof1() << "\n";
+
+  // two FE spaces on one mesh: the other order in 2D, a second P1 space of
+  // tetrahedra in 3D; L2InnerProduct(sp, sp2) on the device, a derived
+  // class that doubles it on the host
+  TemplateDOF<DIM> tdof2(geo);
+  BasisFunctionAdmin<double, DIM, DIM> bas2(tdof2);
+  const std::string order2 = DIM == 3 ? ".1" : (kind == AFE_P2_TRI ? ".1" : ".2");
+  tdof2.readData(base + order2 + ".tmp_dof");
+  bas2.readData(base + order2 + ".bas_fun");
+  std::vector<TemplateElement<double, DIM, DIM>> tmpl2(1);
+  tmpl2[0].reinit(geo, tdof2, crd, bas2);
+  FEMSpace<double, DIM> sp2(mesh, tmpl2);
+  sp2.element().resize(n_element);
+  for (int i = 0; i < n_element; i++) sp2.element(i).reinit(sp2, i, 0);
+  sp2.buildElement();
+  sp2.buildDof();
+  sp2.buildDofBoundaryMark();
+  L2InnerProduct<DIM, double> L2x(sp, sp2);
+  L2x.algebricAccuracy() = acc;
+  L2x.build();
+  const SparsityPattern &px = L2x.getSparsityPattern();
+  std::vector<int64_t> rpx(px.row_ptr().begin(), px.row_ptr().end()), colx(px.col_indices().begin(), px.col_indices().end());
+  dump(out + "/L2x_rp.i64", rpx.data(), rpx.size());
+  dump(out + "/L2x_col.i64", colx.data(), colx.size());
+  dump_mat(out + "/L2x.f64", L2x);
+  DoubledL2<DIM> L2xd(sp, sp2);
+  L2xd.algebricAccuracy() = acc;
+  L2xd.build();
+  dump_mat(out + "/L2xd.f64", L2xd);
+  std::ofstream(out + "/infox.txt") << L2x.nMaxCouplingDof() << " " << L2x.nDof0() << " " << L2x.nDof1() << " "
+                                    << L2x.m() << " " << L2x.n() << "\n";
 
   // the coarsest level smoothed instead of solved exactly
   AMGSolver gsolver(K);

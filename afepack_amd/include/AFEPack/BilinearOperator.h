@@ -2,9 +2,12 @@
 // (include/AFEPack/BilinearOperator.h:76-224 and
 // BilinearOperator.templates.h:55-255,300-406 of the reference).
 //
-// build() = buildSparsityPattern() + buildSparseMatrix().  The pattern is the
-// one afe_space_create computed (first-touch numbering, diagonal first, rest
-// ascending).  buildSparseMatrix() assembles on the device:
+// build() = buildSparsityPattern() + buildSparseMatrix().  On one space the
+// pattern is the one afe_space_create computed (first-touch numbering,
+// diagonal first, rest ascending); on two spaces of one mesh it is the
+// rectangular pattern of afe_matrix_create_pair (the reference's same-mesh
+// branch; spaces on different meshes are not supported).
+// buildSparseMatrix() assembles on the device:
 //   * StiffMatrix and MassMatrix objects run the device element kernels
 //     (afe_assemble, op STIFF / MASS);
 //   * a derived class -- the reference's extension point, which overrides
@@ -33,10 +36,10 @@ public:
 
   BilinearOperator() = default;
   BilinearOperator(space0_t &s0, space1_t &s1) { reinit(s0, s1); }
-  virtual ~BilinearOperator() = default;
+  virtual ~BilinearOperator() {
+    if (pair_dev_) afe_matrix_destroy(pair_dev_);
+  }
   void reinit(space0_t &s0, space1_t &s1) {
-    if ((void *)&s0 != (void *)&s1)
-      throw std::runtime_error("BilinearOperator: two different FE spaces are not supported on this path");
     sp0_ = &s0;
     sp1_ = &s1;
   }
@@ -78,10 +81,14 @@ public:
     buildSparsityPattern();
     buildSparseMatrix();
   }
-  // the CSR pattern of the space (BilinearOperator.templates.h:64-172)
+  // the CSR pattern (BilinearOperator.templates.h:64-172)
   virtual void buildSparsityPattern() {
     afe_space *h = sp0_->handle();
     if (!h) throw std::runtime_error("buildDof() must run before build()");
+    if (!same_space()) {
+      build_pair_pattern();
+      return;
+    }
     int64_t nd, nnz;
     int32_t nl;
     check(afe_space_info(h, &nd, &nnz, &nl));
@@ -99,6 +106,10 @@ public:
   }
   // values on the device (BilinearOperator.templates.h:176-255)
   virtual void buildSparseMatrix() {
+    if (!same_space()) {
+      build_pair_matrix();
+      return;
+    }
     SparseMatrix<double>::reinit(pattern_store_);
     afe_matrix *a = nullptr;
     check(afe_matrix_create(sp0_->handle(), &a));
@@ -132,6 +143,56 @@ protected:
   space1_t *sp1_ = nullptr;
 
 private:
+  bool same_space() const { return (const void *)sp0_ == (const void *)sp1_; }
+  // two spaces: the device matrix owns the pattern; it is made here and the
+  // values are filled by build_pair_matrix
+  void build_pair_pattern() {
+    if ((const void *)&sp0_->mesh() != (const void *)&sp1_->mesh())
+      throw std::runtime_error("BilinearOperator: FE spaces on different meshes are not supported on this path");
+    afe_space *h1 = sp1_->handle();
+    if (!h1) throw std::runtime_error("buildDof() must run before build()");
+    afe_matrix *a = nullptr;
+    check(afe_matrix_create_pair(sp0_->handle(), h1, &a));
+    int64_t m, n, nnz;
+    int32_t mc;
+    check(afe_matrix_info(a, &m, &n, &nnz, &mc));
+    std::vector<int64_t> rp(m + 1), col(nnz);
+    check(afe_matrix_pattern(a, rp.data(), col.data()));
+    pattern_store_.assign_compressed((size_t)m, (size_t)n, std::vector<size_t>(rp.begin(), rp.end()),
+                                     std::vector<size_t>(col.begin(), col.end()));
+    n_dof0_ = (int)m;
+    n_dof1_ = (int)n;
+    n_max_coupling_ = mc;
+    if (pair_dev_) afe_matrix_destroy(pair_dev_);
+    pair_dev_ = a;
+  }
+  void build_pair_matrix() {
+    if (!pair_dev_) build_pair_pattern();
+    afe_matrix *a = pair_dev_;
+    pair_dev_ = nullptr;
+    SparseMatrix<double>::reinit(pattern_store_);
+    bind_device(a, true);
+    const int op = device_operator();
+    if (op == AFE_OP_MASS) {
+      check(afe_assemble(a, op, acc_, nullptr));
+    } else {
+      if (op >= 0) throw std::runtime_error("BilinearOperator: this operator needs one FE space");
+      const int n_elem = sp0_->n_element();
+      const int n0 = n_elem ? (int)sp0_->element(0).dof().size() : 0;
+      const int n1 = n_elem ? (int)sp1_->element(0).dof().size() : 0;
+      std::vector<double> em((size_t)n_elem * n0 * n1);
+      for (int e = 0; e < n_elem; e++) {
+        getElementPattern(sp0_->element(e), sp1_->element(e));
+        element_matrix_.reinit(n0, n1);
+        getElementMatrix(sp0_->element(e), sp1_->element(e));
+        std::copy(element_matrix_.data(), element_matrix_.data() + (size_t)n0 * n1,
+                  em.begin() + (size_t)e * n0 * n1);
+      }
+      check(afe_assemble_elements(a, em.data()));
+    }
+    mark_device_newer();
+  }
+  afe_matrix *pair_dev_ = nullptr; // made by buildSparsityPattern, taken by buildSparseMatrix
   SparsityPattern pattern_store_;
   DenseMatrix<double> element_matrix_;
   int acc_ = 3;
@@ -140,9 +201,10 @@ private:
 };
 
 // L2InnerProduct (BilinearOperator.h:142-156, BilinearOperator.templates.h:
-// 272-326): the L2 inner product of the basis functions.  On one space its
-// element matrix is the mass matrix with the same arithmetic (Jxw * phi_j *
-// phi_k, quadrature of element0), so it runs the device mass kernel.
+// 272-326): the L2 inner product of the basis functions, Jxw * phi0_j *
+// phi1_k on the quadrature of element 0.  On one space that is the mass
+// matrix's arithmetic, so it runs the device mass kernel; on two spaces of
+// one mesh the device two-space kernel.
 template <int DIM, class value_type0, class value_type1 = value_type0, int DOW = DIM, int TDIM0 = DIM,
           int TDIM1 = DIM>
 class L2InnerProduct : public BilinearOperator<DIM, value_type0, value_type1, DOW, TDIM0, TDIM1> {

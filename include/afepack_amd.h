@@ -174,11 +174,30 @@ int afe_matrix_create(afe_space *s, afe_matrix **out);
 int afe_matrix_create_csr(afe_ctx *ctx, int64_t m, const int64_t *row_ptr,
                           const int64_t *col, const double *val,
                           afe_matrix **out);
+/* BilinearOperator(sp0, sp1) with two FE spaces on one mesh (the same-mesh
+ * branch of buildSparsityPattern / buildDofInfo, BilinearOperator.templates.h:
+ * 64-172): rows are the DOFs of s0, columns those of s1; element e couples
+ * each s0 DOF of e with each s1 DOF of e; rows sorted ascending (diagonal
+ * first when the pattern is square, as SparsityPattern.h:39-70 stores it).
+ * The spaces must share context, elements and coordinates.  afe_assemble
+ * takes AFE_OP_MASS only (L2InnerProduct, BilinearOperator.templates.h:
+ * 300-326: Jxw * phi0_j * phi1_k on the quadrature of element 0);
+ * afe_assemble_elements takes n_elem x n_loc0 x n_loc1 element matrices;
+ * vmult / tvmult / values work as on a square matrix.  Dirichlet
+ * elimination and AMG reject it. */
+int afe_matrix_create_pair(afe_space *s0, afe_space *s1, afe_matrix **out);
+/* rows, columns and stored entries; max_coupling = nMaxCouplingDof
+ * (BilinearOperator.templates.h:169-170) of a two-space matrix, -1 else */
+int afe_matrix_info(const afe_matrix *a, int64_t *m, int64_t *n, int64_t *nnz,
+                    int32_t *max_coupling);
+/* the compressed pattern (row_ptr: m + 1 entries, col: nnz) */
+int afe_matrix_pattern(const afe_matrix *a, int64_t *row_ptr, int64_t *col);
 int afe_matrix_destroy(afe_matrix *a);
 int afe_assemble(afe_matrix *a, int32_t op, int32_t acc, const afe_field *coef);
 /* user-supplied element matrices (a getElementMatrix override evaluated by
- * the caller): h_elem_mat is n_elem x n_loc x n_loc row-major on the host;
- * the CSR gather runs on the device. */
+ * the caller): h_elem_mat is n_elem x n_loc x n_loc row-major on the host
+ * (n_loc0 x n_loc1 for a two-space matrix); the CSR gather runs on the
+ * device. */
 int afe_assemble_elements(afe_matrix *a, const double *h_elem_mat);
 int afe_matrix_values(const afe_matrix *a, double *h_val);     /* D2H copy */
 int afe_matrix_set_values(afe_matrix *a, const double *h_val); /* H2D copy */

@@ -806,6 +806,73 @@ or_mat *or_build(const or_space *s, int op, int acc, const or_field *coef) {
   return a;
 }
 
+/* L2InnerProduct::getElementMatrix of two spaces on one element
+ * (BilinearOperator.templates.h:300-326, the branch for equal elements):
+ * quadrature and Jacobian of element 0, entry (j, k) accumulates
+ * Jxw * phi0_j * phi1_k; K is n_loc0 x n_loc1 row-major */
+void or_element_matrix_pair(int kind0, int kind1, int acc, const double *gv, double *K) {
+  int tdim = kind_tdim(kind0), n0 = kind_nloc(kind0), n1 = kind_nloc(kind1);
+  const qrule *q = quad_find(tdim, acc);
+  double volume = tmpl_volume(tdim);
+  double jac = l2g_jacobian(tdim, gv);
+  for (int i = 0; i < n0 * n1; i++) K[i] = 0.0;
+  for (int l = 0; l < q->n; l++) {
+    double lp[3], xp[3], v0[6], v1[6];
+    for (int d = 0; d < tdim; d++) lp[d] = q->d[l * (tdim + 1) + d];
+    double w = q->d[l * (tdim + 1) + tdim];
+    local_to_global(tdim, lp, gv, xp);
+    double Jxw = w * jac * volume;
+    basis_value(kind0, xp, gv, v0);
+    basis_value(kind1, xp, gv, v1);
+    for (int j = 0; j < n0; j++)
+      for (int k = 0; k < n1; k++) K[j * n1 + k] += Jxw * v0[j] * v1[k];
+  }
+}
+
+/* BilinearOperator::build over two spaces of one mesh (the same-mesh branches
+ * of buildDofInfo, buildSparsityPattern and buildSparseMatrix,
+ * BilinearOperator.templates.h:71-81,131-143,187-197):
+ *  - coupling count of a space-0 DOF = sum over its elements of the space-1
+ *    DOF count of the element; the maximum, capped at the column count, is
+ *    the row length given to reinit(N0, N1, .);
+ *  - each element adds (dof0_i, dof1_j) for all i, j; compress;
+ *  - values: the element matrices added entry by entry, elements in order.
+ * op < 0 builds the pattern only; op = OR_OP_MASS is L2InnerProduct.
+ * *max_coupling (if not NULL) receives nMaxCouplingDof. */
+or_mat *or_build_pair(const or_space *s0, const or_space *s1, int op, int acc, int64_t *max_coupling) {
+  if (s0->n_elem != s1->n_elem) return NULL;
+  int64_t N0 = s0->n_dof, N1 = s1->n_dof;
+  int n0 = s0->n_loc, n1 = s1->n_loc;
+  int64_t *cnt = calloc(N0 > 0 ? N0 : 1, sizeof(int64_t));
+  for (int64_t e = 0; e < s0->n_elem; e++)
+    for (int i = 0; i < n0; i++) cnt[s0->elem_dof[e * n0 + i]] += n1;
+  int64_t maxc = 0;
+  for (int64_t i = 0; i < N0; i++)
+    if (cnt[i] > maxc) maxc = cnt[i];
+  if (maxc > N1) maxc = N1;
+  free(cnt);
+  if (max_coupling) *max_coupling = maxc;
+  or_sp *sp = sp_new_max(N0, N1, maxc);
+  for (int64_t e = 0; e < s0->n_elem; e++) {
+    const int32_t *e0 = s0->elem_dof + e * n0, *e1 = s1->elem_dof + e * n1;
+    for (int i = 0; i < n0; i++)
+      for (int j = 0; j < n1; j++) sp_add(sp, e0[i], e1[j]);
+  }
+  sp_compress(sp);
+  sp->col = realloc(sp->col, sizeof(int64_t) * (sp->nnz > 0 ? sp->nnz : 1));
+  or_mat *a = mat_new(sp);
+  if (op < 0) return a;
+  double gv[12], K[36];
+  for (int64_t e = 0; e < s0->n_elem; e++) {
+    elem_coords(s0, e, gv);
+    or_element_matrix_pair(s0->kind, s1->kind, acc, gv, K);
+    const int32_t *e0 = s0->elem_dof + e * n0, *e1 = s1->elem_dof + e * n1;
+    for (int i = 0; i < n0; i++)
+      for (int j = 0; j < n1; j++) mat_add(a, e0[i], e1[j], K[i * n1 + j]);
+  }
+  return a;
+}
+
 /* ====================================================================== */
 /* right-hand side, Dirichlet elimination, error functionals               */
 /* ====================================================================== */

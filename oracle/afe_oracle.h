@@ -82,6 +82,9 @@ double or_template_volume(int tdim);
 void or_element_matrix(int kind, int op, int acc, const double *vert_coords,
                        const or_field *coef, double *K);
 or_mat *or_build(const or_space *s, int op, int acc, const or_field *coef);
+/* two spaces on one mesh: L2InnerProduct (op OR_OP_MASS) or pattern only (op < 0) */
+void or_element_matrix_pair(int kind0, int kind1, int acc, const double *vert_coords, double *K);
+or_mat *or_build_pair(const or_space *s0, const or_space *s1, int op, int acc, int64_t *max_coupling);
 
 /* ---- CSR access ---------------------------------------------------------- */
 or_mat *or_mat_from_csr(int64_t m, int64_t n, const int64_t *rp,

@@ -42,6 +42,8 @@ _SIGS = {
     "or_quad_get": (None, [C.c_int, C.c_int, P, P]),
     "or_element_matrix": (None, [C.c_int, C.c_int, C.c_int, P, C.POINTER(OrField), P]),
     "or_build": (P, [P, C.c_int, C.c_int, C.POINTER(OrField)]),
+    "or_element_matrix_pair": (None, [C.c_int, C.c_int, C.c_int, P, P]),
+    "or_build_pair": (P, [P, P, C.c_int, C.c_int, P]),
     "or_mat_from_csr": (P, [I64, I64, P, P, P]),
     "or_mat_share_pattern": (P, [P, P]),
     "or_mat_free": (None, [P]),
@@ -208,6 +210,14 @@ class Space:
     def build(self, op, acc, field=None):
         return Mat(lib().or_build(self.h, op, acc, _fp(field)))
 
+    def build_pair(self, other, op, acc):
+        """BilinearOperator(self, other) on one mesh: (matrix, nMaxCouplingDof)"""
+        mc = np.zeros(1, dtype=np.int64)
+        h = lib().or_build_pair(self.h, other.h, op, acc, _p(mc))
+        if not h:
+            raise ValueError("the two spaces are not on one mesh")
+        return Mat(h), int(mc[0])
+
     def rhs(self, acc, fkind, field=None, nodal=None):
         f = np.empty(self.n_dof)
         nd = None if nodal is None else np.ascontiguousarray(nodal, dtype=np.float64)
@@ -240,6 +250,14 @@ def element_matrix(kind, op, acc, vert_coords, field=None):
     K = np.empty(nl * nl)
     lib().or_element_matrix(kind, op, acc, _p(v), _fp(field), _p(K))
     return K.reshape(nl, nl)
+
+
+def element_matrix_pair(kind0, kind1, acc, vert_coords):
+    n0, n1 = ({P1_TRI: 3, P2_TRI: 6, P1_TET: 4}[k] for k in (kind0, kind1))
+    v = np.ascontiguousarray(vert_coords, dtype=np.float64).ravel()
+    K = np.empty(n0 * n1)
+    lib().or_element_matrix_pair(kind0, kind1, acc, _p(v), _p(K))
+    return K.reshape(n0, n1)
 
 
 def basis_eval(kind, p, verts):

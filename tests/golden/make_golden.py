@@ -10,7 +10,8 @@ Fixtures (data only):
                   reference's C plugins at seeded points of seeded simplices
   patterns.npz -- CSR patterns built by the reference's SparsityPattern class
                   (reinit with the max coupling count, add per element pair,
-                  compress) for the 8x8 P1, 4x4 P2 and 3^3 tetrahedron meshes
+                  compress) for the 8x8 P1, 4x4 P2 and 3^3 tetrahedron meshes,
+                  and for two spaces on the 4x4 mesh (P1 x P2, P2 x P1)
 """
 import ctypes as C
 import os
@@ -157,6 +158,34 @@ def pattern_fixture():
         L.refsp_get(sp, rp.ctypes.data_as(C.c_void_p), col.ctypes.data_as(C.c_void_p))
         L.refsp_free(sp)
         out[tag + "_elem_dof"] = ed.astype(np.int32)
+        out[tag + "_row_ptr"] = rp.astype(np.int32)
+        out[tag + "_col"] = col.astype(np.int32)
+    # two spaces on one mesh (BilinearOperator's same-mesh branch): the
+    # reference class reinit to (N0, N1, max coupling), element e adds
+    # (dof0_i, dof1_j) for all i, j, then compress
+    for tag, k0, k1, n in (("p1p2_4", 1, 2, 4), ("p2p1_4", 2, 1, 4)):
+        mesh = afe.mesh_square(n, 0.0, 0)
+        from oracle import oracle_py as orc
+        e0 = orc.Space(k0, *mesh).dofs()[0].reshape(2 * n * n, -1)
+        s1 = orc.Space(k1, *mesh)
+        e1 = s1.dofs()[0].reshape(2 * n * n, -1)
+        N0, N1 = int(e0.max()) + 1, s1.n_dof
+        cnt = np.zeros(N0, dtype=np.int64)
+        for a in e0:
+            cnt[a] += e1.shape[1]
+        maxc = min(int(cnt.max()), N1)
+        sp = L.refsp_new_max(N0, N1, maxc)
+        for a, b in zip(e0, e1):
+            for i in a:
+                for j in b:
+                    L.refsp_add(sp, int(i), int(j))
+        L.refsp_compress(sp)
+        nnz = L.refsp_nnz(sp)
+        rp = np.empty(N0 + 1, dtype=np.int64)
+        col = np.empty(nnz, dtype=np.int64)
+        L.refsp_get(sp, rp.ctypes.data_as(C.c_void_p), col.ctypes.data_as(C.c_void_p))
+        L.refsp_free(sp)
+        out[tag + "_max_coupling"] = np.array([maxc], dtype=np.int32)
         out[tag + "_row_ptr"] = rp.astype(np.int32)
         out[tag + "_col"] = col.astype(np.int32)
     return out

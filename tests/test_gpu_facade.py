@@ -153,6 +153,19 @@ def test_cpp_facade_matches_oracle(tmp_path, kind, n):
     # L2InnerProduct on one space: the mass matrix, bit for bit
     om = osp.build(orc.OP_MASS, acc).csr()[2]
     assert np.array_equal(_read(tmp_path, "L2.f64"), om)
+    # L2InnerProduct of this space and a second one on the same mesh (the
+    # other order in 2D, another P1 space in 3D): pattern, values and
+    # nMaxCouplingDof of the oracle's two-space build; the host-evaluated
+    # derived class doubling it gathers to exactly twice the values
+    osp2 = orc.Space(kind if kind == 3 else 3 - kind, c, ev, vm)
+    OX, omc = osp.build_pair(osp2, orc.OP_MASS, acc)
+    xrp, xcol, xval = OX.csr()
+    assert np.array_equal(_read(tmp_path, "L2x_rp.i64", np.int64), xrp)
+    assert np.array_equal(_read(tmp_path, "L2x_col.i64", np.int64), xcol)
+    assert np.array_equal(_read(tmp_path, "L2x.f64"), xval)
+    assert np.array_equal(_read(tmp_path, "L2xd.f64"), 2.0 * xval)
+    info = tuple(map(int, open(os.path.join(tmp_path, "infox.txt")).read().split()))
+    assert info == (omc, osp.n_dof, osp2.n_dof, osp.n_dof, osp2.n_dof)
     # a user operator declared BilinearOperator<DIM, double> (the default
     # second value type), stiffness plus mass in one host loop
     from tests.conftest import rel_err_rows

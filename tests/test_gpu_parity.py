@@ -352,3 +352,25 @@ def test_variable_coefficient_solution_matches_oracle(ctx, kind, n, jitter, acc,
     e2, e1 = sp.error_l2(x, acc), sp.error_h1(x, acc)
     oe2, oe1 = osp.l2_error(ox, acc), osp.h1_error(ox, acc)
     assert abs(e2 - oe2) <= 1e-6 * oe2 and abs(e1 - oe1) <= 1e-6 * oe1
+
+
+def test_full_size_p2_assembly_matches_oracle(ctx):
+    """The benchmark's own system, 1024x1024 squares x 2 P2 triangles
+    (4,198,401 DOFs, 48,250,881 entries, jitter 0.2, seed 20261, acc 6):
+    pattern and stiffness values bit-exact against the oracle's assembly of
+    the same mesh"""
+    import afepack_amd as afe
+    from oracle import oracle_py as orc
+    print("case full-size P2 assembly n=1024", flush=True)
+    c, ev, vm = afe.mesh_square(1024, 0.2, 20261)
+    sp = afe.Space(ctx, afe.P2_TRI, c, ev, vm)
+    A = afe.Matrix(sp)
+    A.assemble(afe.OP_STIFF, 6)
+    val = A.values()
+    prp, pcol = sp.pattern()
+    del A
+    osp = orc.Space(afe.P2_TRI, c, ev, vm)
+    rp, col, oval = osp.build(orc.OP_STIFF, 6).csr()
+    assert len(rp) == 4198402 and rp[-1] == 48250881
+    assert np.array_equal(prp, rp) and np.array_equal(pcol, col)
+    assert np.array_equal(val, oval)

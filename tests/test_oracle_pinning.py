@@ -722,3 +722,85 @@ def test_error_functionals_analytic_and_rates():
         r2 = math.log2(errs[0][0] / errs[1][0])
         r1 = math.log2(errs[0][1] / errs[1][1])
         assert abs(r2 - pl2) < 0.15 and abs(r1 - ph1) < 0.15
+
+
+# ------------------------------------------------ two spaces on one mesh
+# BilinearOperator(sp0, sp1), same-mesh branch (BilinearOperator.templates.h:
+# 64-172) and L2InnerProduct (:300-326)
+@pytest.mark.parametrize("tag,k0,k1", [("p1p2_4", 1, 2), ("p2p1_4", 2, 1)])
+def test_pair_pattern_equals_reference_sparsity_class(tag, k0, k1):
+    g = np.load(os.path.join(HERE, "golden", "patterns.npz"))
+    c, ev, vm = mesh(1, 4)
+    s0, s1 = orc.Space(k0, c, ev, vm), orc.Space(k1, c, ev, vm)
+    A, mc = s0.build_pair(s1, -1, 2)
+    rp, col, _ = A.csr()
+    assert (A.m, A.n) == (s0.n_dof, s1.n_dof)
+    assert np.array_equal(rp, g[tag + "_row_ptr"]) and np.array_equal(col, g[tag + "_col"])
+    assert mc == g[tag + "_max_coupling"][0]
+
+
+def test_pair_pattern_square_keeps_diagonal_first():
+    """two distinct P1 spaces on one mesh: a square pattern, so the
+    reference's SparsityPattern stores the diagonal first and the other
+    columns ascending -- the one-space pattern exactly"""
+    c, ev, vm = mesh(1, 5, 0.2, 3)
+    s0, s1 = orc.Space(1, c, ev, vm), orc.Space(1, c, ev, vm)
+    rp, col, _ = s0.build_pair(s1, -1, 2)[0].csr()
+    rq, cq, _ = s0.build(orc.OP_MASS, 2).csr()
+    assert np.array_equal(rp, rq) and np.array_equal(col, cq)
+
+
+def _p1p2_closed_form(area):
+    """int_T lambda_i phi_k for the P2 basis (vertex phi_k = lambda_k
+    (2 lambda_k - 1), edge DOF 3+k = 4 lambda_a lambda_b on the edge opposite
+    vertex k), from int_T lambda^a lambda^b lambda^c = 2|T| a! b! c! /
+    (a+b+c+2)!"""
+    K = np.empty((3, 6))
+    for i in range(3):
+        for k in range(3):
+            K[i, k] = area / 30 if i == k else -area / 60
+        for k in range(3):
+            K[i, 3 + k] = area / 15 if i == k else 2 * area / 15
+    return K
+
+
+def test_pair_element_matrix_closed_forms():
+    rng = np.random.default_rng(11)
+    for _ in range(8):
+        v = rng.uniform(-1, 1, (3, 2))
+        if _tri_area(v) < 0:
+            v = v[[0, 2, 1]]
+        a = _tri_area(v)
+        if a < 0.05:
+            continue
+        K12 = orc.element_matrix_pair(orc.P1_TRI, orc.P2_TRI, 4, v)
+        assert np.allclose(K12, _p1p2_closed_form(a), rtol=1e-12, atol=1e-14 * a)
+        K21 = orc.element_matrix_pair(orc.P2_TRI, orc.P1_TRI, 4, v)
+        assert np.allclose(K21, K12.T, rtol=1e-12, atol=1e-14 * a)
+        K11 = orc.element_matrix_pair(orc.P1_TRI, orc.P1_TRI, 2, v)
+        assert np.allclose(K11, a / 12 * (np.ones((3, 3)) + np.eye(3)), rtol=1e-12)
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=float) * 0.7 + 0.1
+    vol = 0.7 ** 3 / 6
+    K = orc.element_matrix_pair(orc.P1_TET, orc.P1_TET, 2, v)
+    assert np.allclose(K, vol / 20 * (np.ones((4, 4)) + np.eye(4)), rtol=1e-12)
+
+
+@pytest.mark.parametrize("k0,k1", [(1, 2), (2, 1), (2, 2)])
+def test_pair_matrix_integrates_products_of_linear_functions(k0, k1):
+    """x0^T A x1 = int f g for linear f (nodal values in space 0) and g
+    (space 1), and the row sums are int phi0_i (partition of unity of
+    space 1) = the row sums of the one-space mass matrix"""
+    c, ev, vm = mesh(1, 6, 0.2, 5)
+    s0, s1 = orc.Space(k0, c, ev, vm), orc.Space(k1, c, ev, vm)
+    A, _ = s0.build_pair(s1, orc.OP_MASS, 4)
+    rp, col, val = A.csr()
+    p0 = s0.dofs()[2].reshape(-1, 2)
+    p1 = s1.dofs()[2].reshape(-1, 2)
+    f = 1.0 + p0[:, 0]
+    gg = 2.0 - p1[:, 1]
+    y = A.vmult(np.ascontiguousarray(gg))
+    assert abs(f @ y - 2.25) < 1e-12
+    rs = np.add.reduceat(val, rp[:-1])
+    M0 = s0.build(orc.OP_MASS, 4)
+    r0, _, v0 = M0.csr()
+    assert np.allclose(rs, np.add.reduceat(v0, r0[:-1]), rtol=1e-12, atol=1e-16)
