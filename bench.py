@@ -820,7 +820,11 @@ def run_c5(args):
             log(f"c5: {n_req} submitted at t={time.perf_counter() - t0:.1f} s, {len(results)} done")
     t_sub_end = time.perf_counter()
     done_evt.set()
-    th.join()
+    while th.is_alive():
+        th.join(10.0)
+        if th.is_alive():
+            log(f"c5: draining, {len(results)} of {n_req} done in submission order at "
+                f"t={time.perf_counter() - t0:.1f} s")
     t_end = max(r[6] for r in results.values())
     info = srv.info()
     tm = srv.timing()
