@@ -596,15 +596,18 @@ static void lazy_project(const Csr &M, Csr &P, Csr &Pt) {
   }
   const int64_t nc = (int64_t)core_point.size();
   const int64_t cap = max_entries_per_row(M);
-  // P rows: core k then its neighbours (add order), capped, then sorted
+  // P rows: core k then its neighbours (add order; a repeated column adds to
+  // its entry), capped, then sorted by column
   P = Csr();
   P.m = nc;
   P.n = M.n;
   P.rp.assign(nc + 1, 0);
-  std::vector<std::vector<std::pair<int32_t, double>>> prow(nc);
+  P.col.reserve(M.nnz());
+  P.val.reserve(M.nnz());
+  std::vector<std::pair<int32_t, double>> r;
   for (int64_t c = 0; c < nc; c++) {
     const int64_t k = core_point[c];
-    auto &r = prow[c];
+    r.clear();
     auto add = [&](int32_t j, double w) {
       for (auto &e : r)
         if (e.first == j) {
@@ -615,48 +618,51 @@ static void lazy_project(const Csr &M, Csr &P, Csr &Pt) {
     };
     add((int32_t)k, 1.0);
     for (int64_t j = M.rp[k] + 1; j < M.rp[k + 1]; j++) add(M.col[j], 1.0 / counter[M.col[j]]);
-    std::sort(r.begin(), r.end(), [](auto &a, auto &b) { return a.first < b.first; });
-    P.rp[c + 1] = P.rp[c] + (int64_t)r.size();
-  }
-  P.col.resize(P.rp[nc]);
-  P.val.resize(P.rp[nc]);
-  for (int64_t c = 0; c < nc; c++) {
-    int64_t p = P.rp[c];
-    for (auto &e : prow[c]) {
-      P.col[p] = e.first;
-      P.val[p++] = e.second;
+    std::sort(r.begin(), r.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+    for (auto &e : r) {
+      P.col.push_back(e.first);
+      P.val.push_back(e.second);
     }
+    P.rp[c + 1] = (int64_t)P.col.size();
   }
-  // P^T rows receive entries in core order; capped likewise
+  // P^T rows receive entries in core order, capped likewise: a row keeps the
+  // first cap cores that reach it; one core reaches a row once (a repeated
+  // column of the core's row adds to that entry, which is the row's last)
   Pt = Csr();
   Pt.m = M.n;
   Pt.n = nc;
   std::vector<int64_t> cnt(M.n, 0);
-  std::vector<std::vector<std::pair<int32_t, double>>> trow(M.n);
+  std::vector<int32_t> last(M.n, -1);
+  auto reach = [&](int64_t l, int32_t c) {
+    if (last[l] != c) {
+      last[l] = c;
+      if (cnt[l] < cap) cnt[l]++;
+    }
+  };
   for (int64_t c = 0; c < nc; c++) {
     const int64_t k = core_point[c];
-    auto add = [&](int64_t l, double w) {
-      auto &r = trow[l];
-      for (auto &e : r)
-        if (e.first == c) {
-          e.second += w;
-          return;
-        }
-      if ((int64_t)r.size() < cap) r.push_back({(int32_t)c, w});
-    };
-    add(k, 1.0);
-    for (int64_t j = M.rp[k] + 1; j < M.rp[k + 1]; j++) add(M.col[j], 1.0 / counter[M.col[j]]);
+    reach(k, (int32_t)c);
+    for (int64_t j = M.rp[k] + 1; j < M.rp[k + 1]; j++) reach(M.col[j], (int32_t)c);
   }
   Pt.rp.assign(M.n + 1, 0);
-  for (int64_t l = 0; l < M.n; l++) Pt.rp[l + 1] = Pt.rp[l] + (int64_t)trow[l].size();
-  Pt.col.resize(Pt.rp[M.n]);
-  Pt.val.resize(Pt.rp[M.n]);
-  for (int64_t l = 0; l < M.n; l++) {
-    int64_t p = Pt.rp[l];
-    for (auto &e : trow[l]) {
-      Pt.col[p] = e.first;
-      Pt.val[p++] = e.second;
+  for (int64_t l = 0; l < M.n; l++) Pt.rp[l + 1] = Pt.rp[l] + cnt[l];
+  Pt.col.assign(Pt.rp[M.n], 0);
+  Pt.val.assign(Pt.rp[M.n], 0.0);
+  std::vector<int64_t> fill(Pt.rp.begin(), Pt.rp.end() - 1);
+  auto put = [&](int64_t l, int32_t c, double w) {
+    if (fill[l] > Pt.rp[l] && Pt.col[fill[l] - 1] == c) {
+      Pt.val[fill[l] - 1] += w;
+      return;
     }
+    if (fill[l] < Pt.rp[l + 1]) {
+      Pt.col[fill[l]] = c;
+      Pt.val[fill[l]++] = w;
+    }
+  };
+  for (int64_t c = 0; c < nc; c++) {
+    const int64_t k = core_point[c];
+    put(k, (int32_t)c, 1.0);
+    for (int64_t j = M.rp[k] + 1; j < M.rp[k + 1]; j++) put(M.col[j], (int32_t)c, 1.0 / counter[M.col[j]]);
   }
 }
 
