@@ -78,6 +78,7 @@ SIGS = {
     "afe_space_qpoints": (C.c_int, [P, I32, PI64, P, P]),
     "afe_rhs_qp": (C.c_int, [P, I32, P, P]),
     "afe_apply_dirichlet_values": (C.c_int, [P, P, I32, P, P, P]),
+    "afe_apply_dirichlet_bc": (C.c_int, [P, P, I32, P, P, P, I32]),
     "afe_error_qp": (C.c_int, [P, P, I32, I32, P, PF64]),
     "afe_error_qp_noalloc": (C.c_int, [P, P, I32, I32, P, P]),
     "afe_error_l2": (C.c_int, [P, P, I32, PF64]),

@@ -360,11 +360,14 @@ class Matrix:
     def apply_dirichlet(self, f, u=None):
         _chk(_lib().afe_apply_dirichlet(self.h, _ptr(u), _ptr(f)))
 
-    def apply_dirichlet_values(self, marks, g, f, u=None):
-        """Dirichlet elimination on the DOFs whose mark is in marks, data g per DOF."""
+    def apply_dirichlet_values(self, marks, g, f, u=None, preserve_symmetry=True):
+        """Dirichlet elimination on the DOFs whose mark is in marks, data g per
+        DOF (BoundaryConditionAdmin::apply; without preserve_symmetry only the
+        marked rows change)."""
         m = np.ascontiguousarray(marks, dtype=np.int32)
         gg = np.ascontiguousarray(g, dtype=np.float64)
-        _chk(_lib().afe_apply_dirichlet_values(self.h, _ptr(m), len(m), _ptr(gg), _ptr(u), _ptr(f)))
+        _chk(_lib().afe_apply_dirichlet_bc(self.h, _ptr(m), len(m), _ptr(gg), _ptr(u), _ptr(f),
+                                           1 if preserve_symmetry else 0))
 
     def vmult(self, x, y):
         _chk(_lib().afe_matrix_vmult(self.h, _ptr(x), _ptr(y)))

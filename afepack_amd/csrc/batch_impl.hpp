@@ -87,10 +87,9 @@ static double seconds_between(std::chrono::steady_clock::time_point a, std::chro
 
 // restart the member's CG from the residual in its r: the next batched step
 // preconditions it (see BatchMember::fresh)
-static void batch_restart(BatchMember &m) {
+static void batch_restart(BatchMember &m, hipStream_t s) {
   afe_amg *g = m.g;
   const int64_t n = g->lv[0].n;
-  hipStream_t s = g->ctx->stream;
   k_fill<double><<<nblk(n), 256, 0, s>>>(n, g->wp.p, 0.0);
   k_fill<double><<<nblk(n), 256, 0, s>>>(n, g->wz.p, 0.0);
   k_fill<double><<<1, 64, 0, s>>>(1, &g->sc.p->rz, 0.0);
@@ -109,14 +108,14 @@ static void batch_check_all(afe_batch *b, const std::vector<BatchMember *> &list
       HIPCHK(hipMemcpyAsync(m->g->h_sc, m->g->sc.p, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
   };
-  for (BatchMember *m : list) residual_norms(m->g, true, m->g->wb.p, m->g->wx.p, m->g->wr.p, m->g->wxl.p);
+  for (BatchMember *m : list) residual_norms(m->g, true, m->g->wb.p, m->g->wx.p, m->g->wr.p, m->g->wxl.p, s);
   fetch(list);
   std::vector<BatchMember *> round, deliver;
   for (BatchMember *m : list) {
     m->rr = m->g->h_sc->rr;
     const bool pair_conv = m->rr <= m->pthr;
     if (!pair_conv && ++m->restarts <= 20 && m->it < m->max_it)
-      batch_restart(*m);
+      batch_restart(*m, s);
     else
       round.push_back(m);
   }
@@ -125,7 +124,7 @@ static void batch_check_all(afe_batch *b, const std::vector<BatchMember *> &list
       afe_amg *g = m->g;
       const int64_t n = g->lv[0].n;
       k_dd_round<<<nblk(n), 256, 0, s>>>(n, g->wx.p, g->wxl.p);
-      residual_norms(g, true, g->wb.p, g->wx.p, g->wr.p, nullptr);
+      residual_norms(g, true, g->wb.p, g->wx.p, g->wr.p, nullptr, s);
     }
     fetch(round);
     std::vector<BatchMember *> again;
@@ -141,7 +140,7 @@ static void batch_check_all(afe_batch *b, const std::vector<BatchMember *> &list
       m->outer++;
       m->pthr *= 0.25;
       if (m->rr > m->pthr)
-        batch_restart(*m);
+        batch_restart(*m, s);
       else
         again.push_back(m);
     }
@@ -534,7 +533,10 @@ extern "C" int afe_batch_add(afe_batch *b, afe_amg *g, double *d_x, const double
                              int32_t max_it, int32_t *slot) {
   API_BEGIN
   if (!b || !g || !d_x || !d_b || !slot || rtol <= 0 || max_it < 1) return fail(AFE_E_ARG, "bad arguments");
-  if (g->ctx != b->ctx) return fail(AFE_E_ARG, "the hierarchy belongs to another context");
+  // a hierarchy of another context on the same device joins too: the batch
+  // works on its own stream, and the caller has finished with the
+  // hierarchy's stream (every call returns with it synchronised)
+  if (!g->ctx || g->ctx->device != b->ctx->device) return fail(AFE_E_ARG, "the hierarchy belongs to another device");
   if (b->prec == AFE_PREC_FP32 && !g->has32) return fail(AFE_E_ARG, "hierarchy built without FP32 copies");
   std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
   for (auto &p : b->slots)
@@ -556,7 +558,7 @@ extern "C" int afe_batch_add(afe_batch *b, afe_amg *g, double *d_x, const double
   // r = b - A x and (r, r), (b, b), |r|_1 into the member's scalars; the
   // first batched step preconditions r (no V-cycle of its own here, and the
   // thresholds come from that step's gathered scalars)
-  residual_norms(g, true, g->wb.p, g->wx.p, g->wr.p, g->wxl.p);
+  residual_norms(g, true, g->wb.p, g->wx.p, g->wr.p, g->wxl.p, s);
   m->thr = m->pthr = -1.0;
   m->active = true;
   int idx = -1;
@@ -572,7 +574,7 @@ extern "C" int afe_batch_add(afe_batch *b, afe_amg *g, double *d_x, const double
   b->slots[idx] = std::move(m);
   BatchMember &mm = *b->slots[idx];
   b->dirty = true;
-  batch_restart(mm);
+  batch_restart(mm, s);
   HIPCHK(hipStreamSynchronize(s));
   *slot = idx;
   API_END

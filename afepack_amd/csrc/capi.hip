@@ -13,7 +13,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <map>
+#include <unordered_map>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -78,7 +80,7 @@ struct afe_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::recursive_mutex mu;
-  int64_t bytes = 0;
+  std::atomic<int64_t> bytes{0}; // device bytes held (read without the lock by the server)
   double *d_quad = nullptr; // all quadrature rules
   double *d_bary = nullptr; // reference barycentrics of every rule point (same indexing)
   int64_t lds_bytes = 0;    // LDS granted to the coarse-tail workgroup
@@ -1083,6 +1085,11 @@ extern "C" int afe_rhs_qp(afe_space *s, int32_t acc, const double *h_fq, double 
 
 extern "C" int afe_apply_dirichlet_values(afe_matrix *a, const int32_t *marks, int32_t n_marks, const double *h_g,
                                           double *d_u, double *d_f) {
+  return afe_apply_dirichlet_bc(a, marks, n_marks, h_g, d_u, d_f, 1);
+}
+
+extern "C" int afe_apply_dirichlet_bc(afe_matrix *a, const int32_t *marks, int32_t n_marks, const double *h_g,
+                                      double *d_u, double *d_f, int32_t preserve_symmetry) {
   API_BEGIN
   (void)d_u;
   if (a && a->ps0) return fail(AFE_E_ARG, "Dirichlet elimination needs the square matrix of one space");
@@ -1104,7 +1111,8 @@ extern "C" int afe_apply_dirichlet_values(afe_matrix *a, const int32_t *marks, i
   diag0.alloc(a->ctx, n);
   hipStream_t st = a->ctx->stream;
   k_csr_diag<<<nblk(n), 256, 0, st>>>(n, s->rp.p, a->val.p, diag0.p);
-  k_dirichlet_values<<<nblk(n), 256, 0, st>>>(n, s->rp.p, s->col.p, dflag.p, diag0.p, dg.p, a->val.p, d_f);
+  k_dirichlet_values<<<nblk(n), 256, 0, st>>>(n, s->rp.p, s->col.p, dflag.p, diag0.p, dg.p, a->val.p, d_f,
+                                              preserve_symmetry ? 1 : 0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));
   API_END
@@ -1957,11 +1965,14 @@ template <class T> static void clone_buf(afe_ctx *ctx, DevBuf<T> &dst, const Dev
   HIPCHK(hipMemcpyAsync(dst.p, src.p, src.n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
 }
 
-extern "C" int afe_space_clone(const afe_space *src, const double *coords, afe_space **out) {
+// a copy of src (another mesh of the same connectivity) owned by context
+// ctx, on src's device: its device arrays are copied on ctx's stream, so src
+// must not be written meanwhile
+static int space_clone_into(afe_ctx *ctx, const afe_space *src, const double *coords, afe_space **out) {
   API_BEGIN
-  if (!src || !out) return fail(AFE_E_ARG, "null argument");
+  if (!src || !out || !ctx) return fail(AFE_E_ARG, "null argument");
   if (!src->ctx) return fail(AFE_E_ARG, "host-only space");
-  afe_ctx *ctx = src->ctx;
+  if (src->ctx->device != ctx->device) return fail(AFE_E_ARG, "space of another device");
   auto s = std::make_unique<afe_space>();
   s->ctx = ctx;
   s->h = src->h;
@@ -1983,16 +1994,23 @@ extern "C" int afe_space_clone(const afe_space *src, const double *coords, afe_s
   API_END
 }
 
+extern "C" int afe_space_clone(const afe_space *src, const double *coords, afe_space **out) {
+  if (!src) return fail(AFE_E_ARG, "null argument");
+  return space_clone_into(src->ctx, src, coords, out);
+}
+
 extern "C" int afe_amg_clone(const afe_amg *src, afe_matrix *a, afe_amg **out) {
   API_BEGIN
   if (!src || !a || !out || !src->ctx) return fail(AFE_E_ARG, "null argument");
   if (!src->prm.lazy || !src->dev_refresh)
     return fail(AFE_E_ARG, "only a lazy hierarchy with device refresh maps can be cloned");
-  if (a->ctx != src->ctx) return fail(AFE_E_ARG, "matrix of another context");
+  if (!a->ctx || a->ctx->device != src->ctx->device) return fail(AFE_E_ARG, "matrix of another device");
   const Csr &F = src->h.A[0];
   if (a->m != F.m || a->nnz != F.nnz() || a->hrp() != F.rp || a->hcol() != F.col)
     return fail(AFE_E_PATTERN, "the matrix does not have the hierarchy's sparsity pattern");
-  afe_ctx *ctx = src->ctx;
+  // the copy belongs to the matrix's context (the source's may be another
+  // one on the same device: its buffers are read on this stream)
+  afe_ctx *ctx = a->ctx;
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   HIPCHK(hipSetDevice(ctx->device));
   auto g = std::make_unique<afe_amg>();
@@ -2318,9 +2336,9 @@ static void launch_graph_or_issue(afe_amg *g, int prec, bool symmetric, int nu) 
 // natural = true: vectors in the natural numbering (the CG), else in the
 // level-0 colour order (the stationary solver)
 static void residual_norms(afe_amg *g, bool natural, const double *b, const double *x, double *r,
-                           const double *xl = nullptr) {
+                           const double *xl = nullptr, hipStream_t stream = nullptr) {
   DevLevel &d = g->lv[0];
-  hipStream_t st = g->ctx->stream;
+  hipStream_t st = stream ? stream : g->ctx->stream;
   const int G = g_tune.spmv_blocks;
   if (natural)
     k_residual_norms_dd<<<G, BLK, 0, st>>>(g->nat_si.p, g->nat_ns, g->nat_cols.p, g->nat_c16.p, g->nat_cbase.p,

@@ -460,22 +460,23 @@ __global__ __launch_bounds__(256) void k_dirichlet_zero(int64_t n, const int64_t
 
 // BoundaryConditionAdmin::apply with Dirichlet data g on the flagged DOFs
 // (FEMSpace.templates.h:1338-1373): a flagged row gets f_i = A_ii g_i and
-// keeps only its diagonal; every other row k subtracts, column by column in
-// ascending order (the order in which the reference visits the boundary
-// DOFs), A_ki * (A_ii g_i) / A_ii and zeroes A_ki.  Flagged rows read the
+// keeps only its diagonal; with sym (preserve_symmetry) every other row k
+// subtracts, column by column in ascending order (the order in which the
+// reference visits the boundary DOFs), A_ki * (A_ii g_i) / A_ii and zeroes
+// A_ki, without it the other rows stay as they are.  Flagged rows read the
 // pre-elimination values of their own row only.
 __global__ __launch_bounds__(256) void k_dirichlet_values(int64_t n, const int64_t *__restrict__ rp,
                                                           const int32_t *__restrict__ col,
                                                           const uint8_t *__restrict__ flag,
                                                           const double *__restrict__ diag0,
                                                           const double *__restrict__ g,
-                                                          double *__restrict__ val, double *__restrict__ f) {
+                                                          double *__restrict__ val, double *__restrict__ f, int sym) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (flag[i]) {
     f[i] = val[rp[i]] * g[i];
     for (int64_t j = rp[i] + 1; j < rp[i + 1]; j++) val[j] = 0.0;
-  } else {
+  } else if (sym) {
     double fi = f[i];
     for (int64_t j = rp[i] + 1; j < rp[i + 1]; j++) {
       const int32_t c = col[j];

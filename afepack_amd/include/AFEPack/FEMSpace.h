@@ -714,10 +714,9 @@ public:
       if (c->boundaryMark() == b.boundaryMark()) throw std::runtime_error("a condition for this mark exists already");
     bcs_.push_back(&b);
   }
-  // preserve_symmetry = true elimination on the device (the matrix's device
-  // copy, or a temporary one for a host-only matrix)
+  // the elimination on the device copy of the matrix (FEMSpace.templates.h:
+  // 1338-1373); preserve_symmetry = false changes the boundary rows only
   void apply(SparseMatrix<double> &A, Vector<double> &u, Vector<double> &f, bool preserve_symmetry = true) {
-    if (!preserve_symmetry) throw std::runtime_error("apply(..., false) is not supported on this path");
     const size_t n = (size_t)sp_->n_dof();
     if (A.m() != n || A.n() != n || u.size() != n || f.size() != n) throw std::runtime_error("apply: size mismatch");
     std::vector<int32_t> marks;
@@ -735,7 +734,8 @@ public:
     detail::DeviceVector du(n), df(n);
     du.upload(u.data());
     df.upload(f.data());
-    const int rc = afe_apply_dirichlet_values(a, marks.data(), (int32_t)marks.size(), g.data(), du.p, df.p);
+    const int rc = afe_apply_dirichlet_bc(a, marks.data(), (int32_t)marks.size(), g.data(), du.p, df.p,
+                                          preserve_symmetry ? 1 : 0);
     A.mark_device_newer();
     if (rc == AFE_OK) {
       du.download(u.data());

@@ -862,7 +862,10 @@ def run_c5(args):
             "wait_for_setup_thread": tm["wait_setup_s"] / max(1, tm["n_new"] + tm["n_moving"]),
             "wait_for_slot": tm["wait_slot_s"] / max(1, tm["n_new"] + tm["n_moving"]),
             "solve": tm["solve_s"] / max(1, tm["n_new"] + tm["n_moving"]),
+            "setup_new_mesh_copying_a_cached_structure": tm["setup_cloned_s"] / max(1, tm["n_cloned"]),
         },
+        "server_counts": {"new_mesh": tm["n_new"], "new_mesh_copying_a_cached_structure": tm["n_cloned"],
+                          "moving_mesh": tm["n_moving"]},
     }
     print(json.dumps(out), flush=True)
 

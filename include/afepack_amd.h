@@ -234,6 +234,14 @@ int afe_rhs_qp(afe_space *s, int32_t acc, const double *h_fq, double *d_f);
 int afe_apply_dirichlet_values(afe_matrix *a, const int32_t *marks,
                                int32_t n_marks, const double *h_g, double *d_u,
                                double *d_f);
+/* BoundaryConditionAdmin::apply(A, u, f, preserve_symmetry)
+ * (FEMSpace.templates.h:1338-1373) with the data of afe_apply_dirichlet_values:
+ * preserve_symmetry = 1 is afe_apply_dirichlet_values; 0 sets the flagged
+ * rows (f_i = A_ii g_i, off-diagonals zero) and leaves every other row and
+ * its right-hand side as they are. */
+int afe_apply_dirichlet_bc(afe_matrix *a, const int32_t *marks, int32_t n_marks,
+                           const double *h_g, double *d_u, double *d_f,
+                           int32_t preserve_symmetry);
 /* Functional::L2Error (h1 = 0, exact values n_elem x n_qp) and H1SemiError
  * (h1 = 1, exact gradients n_elem x n_qp x dim) */
 int afe_error_qp(afe_space *s, const double *d_uh, int32_t acc, int32_t h1,

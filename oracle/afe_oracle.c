@@ -964,6 +964,13 @@ void or_apply_dirichlet(const or_space *s, or_mat *a, double *u, double *f) {
  * elimination as above.  u is not touched. */
 void or_apply_dirichlet_values(const or_space *s, or_mat *a, const int32_t *marks, int n_marks,
                                const double *g, double *f) {
+  or_apply_dirichlet_bc(s, a, marks, n_marks, g, f, 1);
+}
+
+/* the same with the reference's preserve_symmetry argument: without it only
+ * the boundary rows change (FEMSpace.templates.h:1357-1358) */
+void or_apply_dirichlet_bc(const or_space *s, or_mat *a, const int32_t *marks, int n_marks,
+                           const double *g, double *f, int preserve_symmetry) {
   const int64_t *rp = a->sp->rp, *col = a->sp->col;
   for (int64_t i = 0; i < s->n_dof; i++) {
     int hit = 0;
@@ -971,6 +978,7 @@ void or_apply_dirichlet_values(const or_space *s, or_mat *a, const int32_t *mark
     if (!hit) continue;
     f[i] = a->val[rp[i]] * g[i];
     for (int64_t j = rp[i] + 1; j < rp[i + 1]; j++) a->val[j] = 0.0;
+    if (!preserve_symmetry) continue;
     for (int64_t j = rp[i] + 1; j < rp[i + 1]; j++) {
       int64_t k = col[j];
       for (int64_t p = rp[k] + 1; p < rp[k + 1]; p++)

@@ -104,6 +104,8 @@ void or_rhs(const or_space *s, int acc, int fkind, const or_field *coef,
 void or_apply_dirichlet(const or_space *s, or_mat *a, double *u, double *f);
 void or_apply_dirichlet_values(const or_space *s, or_mat *a, const int32_t *marks, int n_marks,
                                const double *g, double *f);
+void or_apply_dirichlet_bc(const or_space *s, or_mat *a, const int32_t *marks, int n_marks,
+                           const double *g, double *f, int preserve_symmetry);
 double or_l2_error(const or_space *s, const double *uh, int acc, int ukind);
 double or_h1_error(const or_space *s, const double *uh, int acc, int ukind);
 

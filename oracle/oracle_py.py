@@ -56,6 +56,7 @@ _SIGS = {
     "or_rhs": (None, [P, C.c_int, C.c_int, C.POINTER(OrField), P, P]),
     "or_apply_dirichlet": (None, [P, P, P, P]),
     "or_apply_dirichlet_values": (None, [P, P, P, C.c_int, P, P]),
+    "or_apply_dirichlet_bc": (None, [P, P, P, C.c_int, P, P, C.c_int]),
     "or_l2_error": (F64, [P, P, C.c_int, C.c_int]),
     "or_h1_error": (F64, [P, P, C.c_int, C.c_int]),
     "or_amg_new": (P, [P, C.c_int, F64, C.c_uint, F64]),
@@ -227,10 +228,10 @@ class Space:
     def apply_dirichlet(self, mat, f, u=None):
         lib().or_apply_dirichlet(self.h, mat.h, _p(u), _p(f))
 
-    def apply_dirichlet_values(self, mat, marks, g, f):
+    def apply_dirichlet_values(self, mat, marks, g, f, preserve_symmetry=True):
         m = np.ascontiguousarray(marks, dtype=np.int32)
         gg = np.ascontiguousarray(g, dtype=np.float64)
-        lib().or_apply_dirichlet_values(self.h, mat.h, _p(m), len(m), _p(gg), _p(f))
+        lib().or_apply_dirichlet_bc(self.h, mat.h, _p(m), len(m), _p(gg), _p(f), 1 if preserve_symmetry else 0)
 
     def l2_error(self, uh, acc, ukind=U_SINPROD):
         return lib().or_l2_error(self.h, _p(np.ascontiguousarray(uh, dtype=np.float64)), acc, ukind)

@@ -92,6 +92,23 @@ def test_batch_alone_equals_mixed_bitwise(ctx, problems):
         assert alone[i][1] == mixed[i][1] == late[i][1]
 
 
+def test_batch_members_from_other_contexts(ctx, problems):
+    """hierarchies built in two other contexts (own streams and locks, the
+    server's set-up threads) join a batch of this context and give the same
+    solutions bit for bit as the ones built here"""
+    import afepack_amd as afe
+    print("case batch members from other contexts", flush=True)
+    idx = list(range(len(problems)))
+    here = _solve_batch(ctx, problems, idx, 1e-10)
+    others = [afe.Context(0), afe.Context(0)]
+    theirs = [_problem(others[i % 2], *p, seed=31 + i) for i, p in enumerate(PROBLEMS)]
+    away = _solve_batch(ctx, theirs, idx, 1e-10)
+    for i in idx:
+        assert np.array_equal(here[i][0], away[i][0]), f"member {i} differs when built in another context"
+        assert here[i][1] == away[i][1]
+    del theirs
+
+
 @pytest.mark.parametrize("tail", [4096, 0])
 def test_batch_solutions_match_direct(ctx, problems, tail):
     """tail 0: every level of every member but its coarsest runs in the

@@ -13,6 +13,10 @@
   AMGSolver::lazyReinit          the device refresh against the oracle's
                                  lazyReinit of the same values: every level
                                  bit-exact
+  BoundaryConditionAdmin::apply  afe_apply_dirichlet_bc with and without
+                                 preserve_symmetry, nonzero data on part of
+                                 the boundary: matrix bit-exact, right-hand
+                                 side within 1e-13 of its scale
 """
 import numpy as np
 import pytest
@@ -169,3 +173,29 @@ def test_device_lazy_reinit_matches_oracle(ctx, kind, n, jitter, op):
         orp, ocol, oval = og.A(l).csr()
         assert np.array_equal(lrp, orp) and np.array_equal(lcol, ocol)
         assert np.array_equal(lval, oval), f"level {l}"
+
+
+@pytest.mark.parametrize("kind,n,jitter,acc", [(1, 20, 0.1, 4), (2, 10, 0.2, 6), (3, 5, 0.05, 2)])
+@pytest.mark.parametrize("sym", [True, False])
+def test_dirichlet_with_and_without_symmetry_matches_oracle(ctx, kind, n, jitter, acc, sym):
+    import afepack_amd as afe
+    from oracle import oracle_py as orc
+    print(f"case dirichlet kind={kind} sym={sym}", flush=True)
+    c, ev, vm = make_mesh(kind, n, jitter, 17)
+    dim = 3 if kind == 3 else 2
+    cc = c.reshape(-1, dim)
+    vm = vm.copy()
+    vm[(vm != 0) & (cc[:, 0] == 0.0)] = 2  # data on the face x = 0 only
+    sp = afe.Space(ctx, kind, c, ev, vm)
+    osp = orc.Space(kind, c, ev, vm)
+    A = afe.Matrix(sp)
+    A.assemble(afe.OP_STIFF, acc)
+    OA = osp.build(orc.OP_STIFF, acc)
+    of = osp.rhs(acc, orc.F_POISSON)
+    P = sp.dofs()[2].reshape(-1, dim)
+    g = 2.0 + np.sin(3.0 * P[:, 1])
+    df = afe.DeviceVector.from_numpy(ctx, of)
+    A.apply_dirichlet_values([2], g, df, preserve_symmetry=sym)
+    osp.apply_dirichlet_values(OA, [2], g, of, preserve_symmetry=sym)
+    assert np.array_equal(A.values(), OA.csr()[2])
+    assert np.abs(df.numpy() - of).max() <= 1e-13 * np.abs(of).max()

@@ -370,6 +370,34 @@ def test_dirichlet_values_partial_marks_constrained_system():
     assert np.allclose(x1[B], g[B], rtol=1e-12) and np.allclose(x1[I], xi, rtol=1e-10)
 
 
+def test_dirichlet_without_symmetry_changes_only_boundary_rows():
+    """apply(A, u, f, false): the boundary rows become A_ii u_i = A_ii g_i
+    and every other row and its right-hand side stay, so the system solves
+    to the constrained solution u_B = g_B, A_II u_I = f_I - A_IB g_B (the
+    same solution as the symmetric elimination) and is not symmetric"""
+    c, ev, vm = mesh(orc.P2_TRI, 4, 0.2, 5)
+    cc = c.reshape(-1, 2)
+    vm = vm.copy()
+    vm[(vm != 0) & (cc[:, 0] == 0.0)] = 2
+    s = orc.Space(orc.P2_TRI, c, ev, vm)
+    A = s.build(orc.OP_STIFF, 6)
+    f = s.rhs(6, orc.F_POISSON)
+    D0, f0 = A.dense(), f.copy()
+    _, dm, dp = s.dofs()
+    g = np.sin(3.0 * dp.reshape(-1, 2)[:, 1]) + 2.0
+    s.apply_dirichlet_values(A, [2], g, f, preserve_symmetry=False)
+    D1 = A.dense()
+    B = dm == 2
+    I = ~B
+    assert np.array_equal(D1[I], D0[I]) and np.array_equal(f[I], f0[I])
+    assert np.array_equal(D1[np.ix_(B, B)], np.diag(np.diag(D0)[B])) and not np.any(D1[np.ix_(B, I)])
+    assert np.array_equal(f[B], np.diag(D0)[B] * g[B])
+    assert not np.allclose(D1, D1.T)
+    x1 = np.linalg.solve(D1, f)
+    xi = np.linalg.solve(D0[np.ix_(I, I)], f0[I] - D0[np.ix_(I, B)] @ g[B])
+    assert np.allclose(x1[B], g[B], rtol=1e-12) and np.allclose(x1[I], xi, rtol=1e-10)
+
+
 # ----------------------------------------------------------------------- AMG
 def _csr_from_dense(D):
     n = D.shape[0]
