@@ -328,11 +328,14 @@ static SrvEntry *srv_template(afe_server *s, const SrvRequest *q) {
 }
 
 // true when something other than entry e holds or reserves device memory
-// that will be given back (another set-up or a solve in progress)
+// that will be given back: an admitted set-up, entries being freed, or an
+// entry with memory that is being solved or re-set-up.  Entries still
+// waiting for admission hold nothing and do not count (otherwise set-ups
+// all waiting for admission would wait for each other)
 static bool srv_others_in_flight(afe_server *s, const SrvEntry *e) {
   if (s->reserved > 0 || s->freeing > 0) return true;
   for (auto &kv : s->cache)
-    if (kv.second.get() != e && kv.second->busy) return true;
+    if (kv.second.get() != e && kv.second->busy && kv.second->g) return true;
   return false;
 }
 

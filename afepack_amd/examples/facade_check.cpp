@@ -10,7 +10,8 @@
 // files: rp.i64 col.i64 (pattern), K.f64 (StiffMatrix), KH.f64 (the same
 // form evaluated on the host by a derived class), K2.f64 (an override that
 // doubles it), M.f64 (MassMatrix), f.f64 (L2Discretize), Ka.f64 fa.f64 ua.f64
-// (after BoundaryConditionAdmin::apply), g.f64 (Dirichlet data per DOF),
+// (after BoundaryConditionAdmin::apply), Kn.f64 fn.f64 (after apply with
+// preserve_symmetry = false), g.f64 (Dirichlet data per DOF),
 // x.f64 (AMGSolver::solve), xp.f64 (AMG-preconditioned CG), xv.f64
 // (AMGPreconditioner::vmult of f from zero), err.txt (L2Error, H1SemiError of
 // x against prod sin(pi x_d)), L2.f64 (L2InnerProduct on one space), KM.f64
@@ -214,6 +215,18 @@ template <int DIM> static int run(int kind, int n, const std::string &out, const
   BoundaryConditionAdmin<double, DIM> admin(sp);
   admin.add(bc);
   FEMFunction<double, DIM> x(sp);
+  {
+    // preserve_symmetry = false on a copy of the system: the boundary rows only
+    HostStiff<DIM> Kn(sp);
+    Kn.algebricAccuracy() = acc;
+    Kn.build();
+    Vector<double> fn(f.size());
+    for (size_t i = 0; i < f.size(); i++) fn(i) = f(i);
+    FEMFunction<double, DIM> xn(sp);
+    admin.apply(Kn, xn, fn, false);
+    dump_mat(out + "/Kn.f64", Kn);
+    dump_vec(out + "/fn.f64", fn);
+  }
   admin.apply(K, x, f);
   dump_mat(out + "/Ka.f64", K);
   dump_vec(out + "/fa.f64", f);

@@ -129,6 +129,12 @@ def test_cpp_facade_matches_oracle(tmp_path, kind, n):
     f = _read(tmp_path, "f.f64")
     assert np.abs(f - of).max() <= 1e-13 * np.abs(of).max()
     g = _read(tmp_path, "g.f64")
+    # apply(..., false): the boundary rows only, on a copy of the system
+    OKn = osp.build(orc.OP_STIFF, acc)
+    ofn = of.copy()
+    osp.apply_dirichlet_values(OKn, [1], g, ofn, preserve_symmetry=False)
+    assert np.array_equal(_read(tmp_path, "Kn.f64"), OKn.csr()[2])
+    assert np.abs(_read(tmp_path, "fn.f64") - ofn).max() <= 1e-13 * np.abs(ofn).max()
     osp.apply_dirichlet_values(OK, [1], g, of)
     assert np.array_equal(_read(tmp_path, "Ka.f64"), OK.csr()[2])
     fa = _read(tmp_path, "fa.f64")
