@@ -90,6 +90,10 @@ struct afe_ctx {
 // device bytes allocated minus freed by the calling thread (the server
 // measures what one request's set-up keeps)
 static thread_local int64_t t_dev_bytes = 0;
+// hierarchies built by the calling thread colour every level first-fit in
+// index order (DSatur's fewer colours cost host time a busy set-up pool
+// cannot spare; the server's set-up threads set it)
+static thread_local bool t_first_fit_only = false;
 
 template <class T> struct DevBuf {
   T *p = nullptr;
@@ -1447,7 +1451,7 @@ static void build_layout(const Hierarchy &h, HostLayout &lay, bool maps) {
     const Csr &A = h.A[l];
     HostLevel &hl = lay.lv[l];
     std::vector<int32_t> colour;
-    const int ncol = colour_rows(A, colour, l == 0 ? DSATUR_MAX_ROWS : DSATUR_MAX_ROWS_COARSE);
+    const int ncol = colour_rows(A, colour, t_first_fit_only ? 0 : l == 0 ? DSATUR_MAX_ROWS : DSATUR_MAX_ROWS_COARSE);
     tick(tph[0]);
     std::vector<int32_t> len(A.m, 0);
     for (int64_t i = 0; i < A.m; i++)
@@ -2011,15 +2015,15 @@ extern "C" int afe_amg_clone(const afe_amg *src, afe_matrix *a, afe_amg **out) {
   // the copy belongs to the matrix's context (the source's may be another
   // one on the same device: its buffers are read on this stream)
   afe_ctx *ctx = a->ctx;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  HIPCHK(hipSetDevice(ctx->device));
   auto g = std::make_unique<afe_amg>();
   g->ctx = ctx;
   g->A0 = a;
   g->key_space = a->s;
   g->key_mat = a->s ? nullptr : a;
   g->prm = src->prm;
-  g->h = src->h;
+  g->h = src->h; // host copy before the context lock: other callers keep the device meanwhile
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
   g->has32 = src->has32;
   g->tail0 = src->tail0;
   g->coarse_exact = src->coarse_exact;

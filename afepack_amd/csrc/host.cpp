@@ -274,19 +274,20 @@ void space_build(Space &s, int kind, int64_t n_vert, const double *coords,
   for (int64_t i = 0; i < n_dof; i++) s.max_row = std::max(s.max_row, rowlen[i]);
   if (s.max_row > 255) throw Error{-1, "row longer than 255 entries: unsupported gather map"};
 
-  // gather map: slot offset of (row i, column elem_dof[e][b]) inside row i
+  // gather map: slot offset of (row i, column elem_dof[e][b]) inside row i,
+  // looked up in a per-thread column -> offset table filled from row i
   s.slot.assign((size_t)s.adj.size() * nl, 0);
-#pragma omp parallel for schedule(static, 4096)
-  for (int64_t i = 0; i < n_dof; i++) {
-    const int32_t *c0 = s.col.data() + s.rp[i];
-    int64_t len = s.rp[i + 1] - s.rp[i];
-    for (int64_t p = s.adj_ptr[i]; p < s.adj_ptr[i + 1]; p++) {
-      int64_t e = s.adj[p] >> 3;
-      for (int b = 0; b < nl; b++) {
-        int32_t d = s.elem_dof[nl * e + b];
-        int64_t o = 0;
-        if (d != i) o = std::lower_bound(c0 + 1, c0 + len, d) - c0;
-        s.slot[(size_t)p * nl + b] = (uint8_t)o;
+#pragma omp parallel
+  {
+    std::vector<uint8_t> at(n_dof, 0);
+#pragma omp for schedule(static, 4096)
+    for (int64_t i = 0; i < n_dof; i++) {
+      const int32_t *c0 = s.col.data() + s.rp[i];
+      const int64_t len = s.rp[i + 1] - s.rp[i];
+      for (int64_t o = 0; o < len; o++) at[c0[o]] = (uint8_t)o;
+      for (int64_t p = s.adj_ptr[i]; p < s.adj_ptr[i + 1]; p++) {
+        const int64_t e = s.adj[p] >> 3;
+        for (int b = 0; b < nl; b++) s.slot[(size_t)p * nl + b] = at[s.elem_dof[nl * e + b]];
       }
     }
   }

@@ -11,10 +11,6 @@
 // the next step, finished ones leave, and every step advances all of them
 // with shared launches.  Meshes, matrices and hierarchies stay cached in HBM
 // under a byte budget (least recently used idle entries are evicted).
-// Each set-up thread has a context (stream and lock) of its own on the
-// device, so set-ups neither wait for each other's device work nor for the
-// batch's steps; an entry's objects belong to the context that created
-// them, and the batch runs them on its own stream once set up.
 
 namespace {
 
@@ -25,7 +21,6 @@ struct SrvEntry {
   uint64_t seed = 0;
   bool own_mesh = false;      // the caller's mesh (ev and vmark kept to compare)
   uint64_t struct_key = 0;    // connectivity key (kind, elements, marks)
-  afe_ctx *ctx = nullptr;      // the context owning sp, A, g, d_b, d_x
   afe_space *sp = nullptr;
   afe_matrix *A = nullptr;
   afe_amg *g = nullptr;
@@ -121,8 +116,7 @@ static uint64_t struct_key_mesh(int kind, const std::vector<int32_t> &ev, const 
 } // namespace
 
 struct afe_server {
-  afe_ctx *ctx = nullptr;           // the batch's context
-  std::vector<afe_ctx *> wctx;      // one per set-up thread
+  afe_ctx *ctx = nullptr;
   afe_batch *batch = nullptr;
   int64_t budget = 0, reserved = 0;
   int max_slots = 0;
@@ -135,7 +129,6 @@ struct afe_server {
   std::thread engine;
   bool stop = false;
   int64_t next_id = 1, completed = 0, evictions = 0;
-  int64_t freeing = 0; // accounted bytes of evicted entries not yet freed
   uint64_t clock = 0;
   std::string engine_error;
   // sums over finished requests: set-up of new / moved meshes, waiting for a
@@ -150,74 +143,41 @@ static void srv_free_entry(afe_server *s, SrvEntry *e) {
   if (e->g) afe_amg_destroy(e->g);
   if (e->A) afe_matrix_destroy(e->A);
   if (e->sp) afe_space_destroy(e->sp);
-  if (e->d_b) afe_dev_free(e->ctx, e->d_b);
-  if (e->d_x) afe_dev_free(e->ctx, e->d_x);
+  if (e->d_b) afe_dev_free(s->ctx, e->d_b);
+  if (e->d_x) afe_dev_free(s->ctx, e->d_x);
   e->g = nullptr; e->A = nullptr; e->sp = nullptr; e->d_b = e->d_x = nullptr;
 }
 
-// live device allocations of the server's contexts
-static int64_t srv_live_bytes(afe_server *s) {
-  int64_t live = s->ctx->bytes.load();
-  for (afe_ctx *c : s->wctx) live += c->bytes.load();
-  return live;
-}
-
 // device bytes held: the larger of the cache's accounted sizes and the
-// contexts' live allocations less the evicted entries not yet freed, plus
-// the reservations of in-flight set-ups
+// context's live allocations, plus the reservations of in-flight set-ups
 static int64_t srv_bytes(afe_server *s) {
   int64_t b = 0;
   for (auto &kv : s->cache)
     if (kv.second->g) b += kv.second->est_bytes;
-  return std::max(b, srv_live_bytes(s) - s->freeing) + s->reserved;
+  int64_t live;
+  {
+    std::lock_guard<std::recursive_mutex> lc(s->ctx->mu);
+    live = s->ctx->bytes;
+  }
+  return std::max(b, live) + s->reserved;
 }
 
-// make room for `need` bytes: evict idle entries, least recently used first,
-// but entries whose connectivity another cached entry also holds before the
-// last holder of a connectivity (that one can still lend its structure to a
-// new mesh).  Caller holds s->m.  The victims leave the cache into `doomed`
-// and are counted in s->freeing until srv_release frees them without the
-// server lock: freeing device memory synchronises the device, which would
-// otherwise hold up every submit and set-up for as long as the device is
-// busy.  False when only busy entries are left.
-static bool srv_make_room(afe_server *s, int64_t need, std::vector<std::unique_ptr<SrvEntry>> &doomed) {
+// make room for `need` bytes: evict idle entries, least recently used first
+// (caller holds s->m); false when only busy entries are left
+static bool srv_make_room(afe_server *s, int64_t need) {
   while (srv_bytes(s) + need > s->budget) {
-    std::unordered_map<uint64_t, int> holders;
-    for (auto &kv : s->cache)
-      if (kv.second->g) holders[kv.second->struct_key]++;
     SrvEntry *victim = nullptr;
-    bool victim_dup = false;
     for (auto &kv : s->cache) {
       SrvEntry *e = kv.second.get();
       if (e->busy || e->pins > 0 || !e->g) continue;
-      const bool dup = holders[e->struct_key] > 1;
-      if (!victim || (dup && !victim_dup) || (dup == victim_dup && e->last_use < victim->last_use)) {
-        victim = e;
-        victim_dup = dup;
-      }
+      if (!victim || e->last_use < victim->last_use) victim = e;
     }
     if (!victim) return false;
+    srv_free_entry(s, victim);
     s->evictions++;
-    s->freeing += victim->est_bytes;
-    auto it = s->cache.find(victim->key);
-    doomed.push_back(std::move(it->second));
-    s->cache.erase(it);
+    s->cache.erase(victim->key);
   }
   return true;
-}
-
-// frees evicted entries; called without the server lock
-static void srv_release(afe_server *s, std::vector<std::unique_ptr<SrvEntry>> &doomed) {
-  if (doomed.empty()) return;
-  int64_t est = 0;
-  for (auto &e : doomed) {
-    srv_free_entry(s, e.get());
-    est += e->est_bytes;
-  }
-  doomed.clear();
-  std::lock_guard<std::mutex> lk(s->m);
-  s->freeing -= est;
-  s->cv_done.notify_all();
 }
 
 // the request's mesh: the structured generator (then, for a re-solve, the
@@ -255,7 +215,7 @@ static int srv_mesh(SrvEntry *e, const SrvRequest *q, std::vector<double> &c, st
   return AFE_OK;
 }
 
-static int srv_setup(SrvRequest *q, afe_ctx *wc) {
+static int srv_setup(afe_server *s, SrvRequest *q) {
   SrvEntry *e = q->e;
   const afe_request &r = q->r;
   afe_field coef = r.coef;
@@ -270,18 +230,17 @@ static int srv_setup(SrvRequest *q, afe_ctx *wc) {
     if ((rc = srv_mesh(e, q, c, ev, true))) return rc;
     const int64_t nv = (int64_t)e->vmark.size();
     const int64_t ne = (int64_t)ev.size() / (r.kind == AFE_P1_TET ? 4 : 3);
-    e->ctx = wc;
     if (q->tmpl) // same connectivity as a cached entry: copy its numbering, pattern and gather map
-      rc = space_clone_into(wc, q->tmpl->sp, c.data(), &e->sp);
+      rc = afe_space_clone(q->tmpl->sp, c.data(), &e->sp);
     else
-      rc = afe_space_create(wc, r.kind, nv, c.data(), ne, ev.data(), e->vmark.data(), &e->sp);
+      rc = afe_space_create(s->ctx, r.kind, nv, c.data(), ne, ev.data(), e->vmark.data(), &e->sp);
     if (rc) return rc;
     if ((rc = afe_matrix_create(e->sp, &e->A))) return rc;
     int32_t nl;
     int64_t nz;
     afe_space_info(e->sp, &e->n_dof, &nz, &nl);
-    if ((rc = afe_dev_alloc(wc, e->n_dof * 8, (void **)&e->d_b))) return rc;
-    if ((rc = afe_dev_alloc(wc, e->n_dof * 8, (void **)&e->d_x))) return rc;
+    if ((rc = afe_dev_alloc(s->ctx, e->n_dof * 8, (void **)&e->d_b))) return rc;
+    if ((rc = afe_dev_alloc(s->ctx, e->n_dof * 8, (void **)&e->d_x))) return rc;
     if ((rc = afe_assemble(e->A, r.op, r.acc, cf))) return rc;
     if ((rc = afe_rhs(e->sp, r.acc, r.fkind, cf, nullptr, e->d_b))) return rc;
     if ((rc = afe_apply_dirichlet(e->A, nullptr, e->d_b))) return rc;
@@ -303,7 +262,7 @@ static int srv_setup(SrvRequest *q, afe_ctx *wc) {
     if ((rc = afe_apply_dirichlet(e->A, nullptr, e->d_b))) return rc;
     if ((rc = afe_amg_lazy_reinit(e->g, e->A))) return rc;
   }
-  return afe_dev_zero(e->ctx, e->d_x, e->n_dof * 8);
+  return afe_dev_zero(s->ctx, e->d_x, e->n_dof * 8);
 }
 
 // the cached entry holds this request's mesh (same key and connectivity,
@@ -328,14 +287,11 @@ static SrvEntry *srv_template(afe_server *s, const SrvRequest *q) {
 }
 
 // true when something other than entry e holds or reserves device memory
-// that will be given back: an admitted set-up, entries being freed, or an
-// entry with memory that is being solved or re-set-up.  Entries still
-// waiting for admission hold nothing and do not count (otherwise set-ups
-// all waiting for admission would wait for each other)
+// that will be given back (another set-up or a solve in progress)
 static bool srv_others_in_flight(afe_server *s, const SrvEntry *e) {
-  if (s->reserved > 0 || s->freeing > 0) return true;
+  if (s->reserved > 0) return true;
   for (auto &kv : s->cache)
-    if (kv.second.get() != e && kv.second->busy && kv.second->g) return true;
+    if (kv.second.get() != e && kv.second->busy) return true;
   return false;
 }
 
@@ -357,9 +313,11 @@ static void srv_finish_failed(afe_server *s, SrvRequest *q, SrvEntry *e, int rc,
   s->cv_work.notify_all();
 }
 
-static void srv_worker(afe_server *s, afe_ctx *wc) {
-  // the pool works on many requests at once: one host thread per set-up
+static void srv_worker(afe_server *s) {
+  // the pool works on many requests at once: one host thread per set-up,
+  // first-fit colouring (the host is the server's bottleneck)
   set_host_threads(1);
+  t_first_fit_only = true;
   std::unique_lock<std::mutex> lk(s->m);
   while (!s->stop) {
     // first pending request whose mesh is not being solved, set up or copied
@@ -419,14 +377,7 @@ static void srv_worker(afe_server *s, afe_ctx *wc) {
     // come back, waiting cannot help: drop the template (it may be evicted
     // then) or fail the request
     bool refused = false;
-    std::vector<std::unique_ptr<SrvEntry>> doomed;
-    while (fresh && !s->stop && !srv_make_room(s, need, doomed)) {
-      if (!doomed.empty()) { // free what this round evicted before waiting for more
-        lk.unlock();
-        srv_release(s, doomed);
-        lk.lock();
-        continue;
-      }
+    while (fresh && !s->stop && !srv_make_room(s, need)) {
       if (!srv_others_in_flight(s, e)) {
         if (q->tmpl) {
           q->tmpl->pins--;
@@ -438,12 +389,7 @@ static void srv_worker(afe_server *s, afe_ctx *wc) {
       }
       s->cv_done.wait(lk);
     }
-    if (s->stop) {
-      lk.unlock();
-      srv_release(s, doomed);
-      lk.lock();
-      break;
-    }
+    if (s->stop) break;
     if (refused) {
       srv_finish_failed(s, q, e, AFE_E_BUDGET, "request does not fit the HBM budget with every idle mesh evicted");
       continue;
@@ -451,9 +397,8 @@ static void srv_worker(afe_server *s, afe_ctx *wc) {
     q->cloned = q->tmpl != nullptr;
     s->reserved += need;
     lk.unlock();
-    srv_release(s, doomed);
     t_dev_bytes = 0;
-    const int rc = srv_setup(q, wc);
+    const int rc = srv_setup(s, q);
     const std::string msg = rc != AFE_OK ? std::string(afe_last_error()) : std::string();
     const int64_t kept = t_dev_bytes;
     lk.lock();
@@ -565,19 +510,9 @@ extern "C" int afe_server_create(int32_t device, int64_t hbm_budget, int32_t max
   }
   s->budget = hbm_budget;
   s->max_slots = max_slots;
-  for (int i = 0; i < setup_threads; i++) {
-    afe_ctx *c = nullptr;
-    if ((rc = afe_ctx_create(device, &c))) {
-      for (afe_ctx *w : s->wctx) afe_ctx_destroy(w);
-      afe_batch_destroy(s->batch);
-      afe_ctx_destroy(s->ctx);
-      return rc;
-    }
-    s->wctx.push_back(c);
-  }
   afe_server *sp = s.get();
   s->engine = std::thread(srv_engine, sp);
-  for (int i = 0; i < setup_threads; i++) s->workers.emplace_back(srv_worker, sp, s->wctx[i]);
+  for (int i = 0; i < setup_threads; i++) s->workers.emplace_back(srv_worker, sp);
   *out = s.release();
   API_END
 }
@@ -595,7 +530,6 @@ extern "C" int afe_server_destroy(afe_server *s) {
   afe_batch_destroy(s->batch);
   for (auto &kv : s->cache) srv_free_entry(s, kv.second.get());
   s->cache.clear();
-  for (afe_ctx *w : s->wctx) afe_ctx_destroy(w);
   afe_ctx_destroy(s->ctx);
   delete s;
   return AFE_OK;
@@ -692,7 +626,10 @@ extern "C" int afe_server_info(afe_server *s, int64_t *device_bytes, int64_t *ca
                                int64_t *evictions) {
   if (!s) return fail(AFE_E_ARG, "null argument");
   std::lock_guard<std::mutex> lk(s->m);
-  if (device_bytes) *device_bytes = srv_live_bytes(s); // includes the afe_dev_alloc buffers
+  if (device_bytes) {
+    std::lock_guard<std::recursive_mutex> lc(s->ctx->mu);
+    *device_bytes = s->ctx->bytes; // includes the afe_dev_alloc buffers
+  }
   if (cached) *cached = (int64_t)s->cache.size();
   if (completed) *completed = s->completed;
   if (evictions) *evictions = s->evictions;
