@@ -99,3 +99,31 @@ def test_two_space_matrix_rejections(ctx):
     other = afe.Space(ctx, 2, c, ev, vm)
     with pytest.raises(afe.AfeError):
         afe.Matrix(s0, space1=other)
+
+
+def test_pair_transpose_and_row_sums_at_full_size(ctx):
+    """size-independent checks on a 512 x 512 mesh (0.79M P1 x 1.05M P2):
+    the P1 x P2 and P2 x P1 matrices are each other's transposes (within
+    rounding: the products Jxw phi0 phi1 and Jxw phi1 phi0 round apart), the
+    P1 x P2 row sums are the P1 mass row sums (partition of unity of P2)"""
+    import afepack_amd as afe
+    import scipy.sparse as sps
+    print("case pair full size n=512", flush=True)
+    c, ev, vm = make_mesh(1, 512, 0.2, 20261)
+    s1, s2 = afe.Space(ctx, 1, c, ev, vm), afe.Space(ctx, 2, c, ev, vm)
+    A12, A21, M1 = afe.Matrix(s1, space1=s2), afe.Matrix(s2, space1=s1), afe.Matrix(s1)
+    A12.assemble(afe.OP_MASS, 4)
+    A21.assemble(afe.OP_MASS, 4)
+    M1.assemble(afe.OP_MASS, 4)
+    rp, col = A12.pattern()
+    X = sps.csr_matrix((A12.values(), col, rp), shape=(A12.m, A12.n))
+    rp, col = A21.pattern()
+    Y = sps.csr_matrix((A21.values(), col, rp), shape=(A21.m, A21.n))
+    D = (X - Y.T).tocsr()
+    assert abs(D).max() <= 1e-15 * abs(X).max()
+    assert (X != 0).nnz == (Y != 0).nnz
+    rp, _ = s1.pattern()
+    rs = np.add.reduceat(A12.values(), A12.pattern()[0][:-1])
+    ms = np.add.reduceat(M1.values(), rp[:-1])
+    assert np.allclose(rs, ms, rtol=1e-12, atol=1e-20)
+    assert abs(rs.sum() - 1.0) < 1e-12

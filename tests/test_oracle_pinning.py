@@ -813,6 +813,18 @@ def test_pair_element_matrix_closed_forms():
     assert np.allclose(K, vol / 20 * (np.ones((4, 4)) + np.eye(4)), rtol=1e-12)
 
 
+@pytest.mark.parametrize("kind,acc", [(orc.P1_TRI, 3), (orc.P2_TRI, 6), (orc.P1_TET, 2)])
+def test_pair_of_one_kind_equals_mass_matrix(kind, acc):
+    """two spaces of the same element on one mesh: L2InnerProduct's
+    element matrix is the mass matrix's arithmetic (Jxw * phi_j * phi_k), so
+    the two-space build equals the one-space mass matrix bit for bit"""
+    c, ev, vm = mesh(kind, 3 if kind == orc.P1_TET else 5, 0.05 if kind == orc.P1_TET else 0.2, 8)
+    s0, s1 = orc.Space(kind, c, ev, vm), orc.Space(kind, c, ev, vm)
+    rp, col, val = s0.build_pair(s1, orc.OP_MASS, acc)[0].csr()
+    rq, cq, vq = s0.build(orc.OP_MASS, acc).csr()
+    assert np.array_equal(rp, rq) and np.array_equal(col, cq) and np.array_equal(val, vq)
+
+
 @pytest.mark.parametrize("k0,k1", [(1, 2), (2, 1), (2, 2)])
 def test_pair_matrix_integrates_products_of_linear_functions(k0, k1):
     """x0^T A x1 = int f g for linear f (nodal values in space 0) and g
