@@ -65,6 +65,7 @@ struct afe_batch {
   CgScalars *h_gath = nullptr;
   size_t h_gath_n = 0;
   hipGraphExec_t graph = nullptr;
+  hipEvent_t step_ev = nullptr; // completion of the step in flight
   int nu = 1;
   // slices per wavefront of the V-cycle level launches, by level bucket:
   // [0] level 0, [1] the coarser levels run as launches
@@ -77,6 +78,7 @@ struct afe_batch {
   double t_epoch = 0, t_graph = 0, t_step = 0;
   ~afe_batch() {
     if (graph) (void)hipGraphExecDestroy(graph);
+    if (step_ev) (void)hipEventDestroy(step_ev);
     if (h_gath) (void)hipHostFree(h_gath);
   }
 };
@@ -426,7 +428,8 @@ template <class T> static void batch_issue_step(afe_batch *b, hipStream_t s) {
   kb_gather_scalars<<<(K + 63) / 64, 64, 0, s>>>(b->scp.p, K, b->gath.p);
 }
 
-template <class T> static void batch_step_impl(afe_batch *b, int *n_active) {
+template <class T>
+static void batch_step_impl(afe_batch *b, int *n_active, std::unique_lock<std::recursive_mutex> &lk) {
   afe_ctx *ctx = b->ctx;
   hipStream_t s = ctx->stream;
   using clk = std::chrono::steady_clock;
@@ -452,7 +455,14 @@ template <class T> static void batch_step_impl(afe_batch *b, int *n_active) {
     batch_issue_step<T>(b, s);
   }
   HIPCHK(hipMemcpyAsync(b->h_gath, b->gath.p, sizeof(CgScalars) * K, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  HIPCHK(hipEventRecord(b->step_ev, s));
+  // the context is free while the step runs: other callers may queue their
+  // work behind it on the stream (the capture above is complete; the batch
+  // itself stays locked by b->mu)
+  lk.unlock();
+  const hipError_t we = hipEventSynchronize(b->step_ev);
+  lk.lock();
+  HIPCHK(we);
   b->n_steps++;
   b->t_step += std::chrono::duration<double>(clk::now() - t1).count();
   const std::vector<int> act = b->act;
@@ -496,6 +506,11 @@ extern "C" int afe_batch_create(afe_ctx *ctx, int32_t prec, int32_t smooth, afe_
   b->ctx = ctx;
   b->prec = prec;
   b->nu = smooth;
+  {
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipEventCreateWithFlags(&b->step_ev, hipEventDisableTiming));
+  }
   *out = b.release();
   API_END
 }
@@ -521,6 +536,7 @@ extern "C" int afe_batch_destroy(afe_batch *b) {
     fprintf(stderr, "batch: %lld steps, %d epochs; host time: epochs %.1f ms, graphs %.1f ms, steps %.1f ms\n",
             (long long)b->n_steps, b->n_epochs, 1e3 * b->t_epoch, 1e3 * b->t_graph, 1e3 * b->t_step);
   {
+    std::lock_guard<std::recursive_mutex> bl(b->mu);
     std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
@@ -538,6 +554,7 @@ extern "C" int afe_batch_add(afe_batch *b, afe_amg *g, double *d_x, const double
   // hierarchy's stream (every call returns with it synchronised)
   if (!g->ctx || g->ctx->device != b->ctx->device) return fail(AFE_E_ARG, "the hierarchy belongs to another device");
   if (b->prec == AFE_PREC_FP32 && !g->has32) return fail(AFE_E_ARG, "hierarchy built without FP32 copies");
+  std::lock_guard<std::recursive_mutex> bl(b->mu);
   std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
   for (auto &p : b->slots)
     if (p && p->active && p->g == g) return fail(AFE_E_ARG, "hierarchy already has an active solve in this batch");
@@ -583,11 +600,12 @@ extern "C" int afe_batch_add(afe_batch *b, afe_amg *g, double *d_x, const double
 extern "C" int afe_batch_step(afe_batch *b, int32_t *n_active) {
   API_BEGIN
   if (!b) return fail(AFE_E_ARG, "null batch");
-  std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+  std::lock_guard<std::recursive_mutex> bl(b->mu);
+  std::unique_lock<std::recursive_mutex> lk(b->ctx->mu);
   HIPCHK(hipSetDevice(b->ctx->device));
   int na = 0;
-  if (b->prec == AFE_PREC_FP32) batch_step_impl<float>(b, &na);
-  else batch_step_impl<double>(b, &na);
+  if (b->prec == AFE_PREC_FP32) batch_step_impl<float>(b, &na, lk);
+  else batch_step_impl<double>(b, &na, lk);
   if (n_active) *n_active = na;
   API_END
 }
@@ -603,7 +621,7 @@ extern "C" int afe_batch_run(afe_batch *b) {
 
 extern "C" int afe_batch_result(afe_batch *b, int32_t slot, int32_t *done, afe_solve_stats *st) {
   if (!b) return fail(AFE_E_ARG, "null batch");
-  std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+  std::lock_guard<std::recursive_mutex> lk(b->mu);
   if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot]) return fail(AFE_E_ARG, "bad slot");
   const BatchMember &m = *b->slots[slot];
   if (done) *done = m.done ? 1 : 0;
@@ -622,7 +640,7 @@ extern "C" int afe_batch_result(afe_batch *b, int32_t slot, int32_t *done, afe_s
 // forget a finished (or abandon an active) member; its slot is reused
 extern "C" int afe_batch_remove(afe_batch *b, int32_t slot) {
   if (!b) return fail(AFE_E_ARG, "null batch");
-  std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+  std::lock_guard<std::recursive_mutex> lk(b->mu);
   if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot]) return fail(AFE_E_ARG, "bad slot");
   if (b->slots[slot]->active) b->dirty = true;
   b->slots[slot].reset();
