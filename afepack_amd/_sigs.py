@@ -103,6 +103,7 @@ SIGS = {
     "afe_set_tuning": (C.c_int, [I32, I32, I64]),
     "afe_amg_kernel_timing": (C.c_int, [P, I32, I32, PF64, PF64]),
     "afe_assemble_kernel_timing": (C.c_int, [P, I32, I32, P, I32, PF64, PF64, PF64]),
+    "afe_assemble_timing": (C.c_int, [P, I32, I32, P, I32, PF64]),
     "afe_copy_kernel_timing": (C.c_int, [P, I64, I32, PF64]),
     "afe_amg_set_tail_rows": (C.c_int, [P, I64]),
     "afe_amg_live_timing": (C.c_int, [P, I32, PF64, PI64]),

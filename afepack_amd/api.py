@@ -383,6 +383,13 @@ class Matrix:
                                                C.byref(by)))
         return te.value, tg.value, by.value
 
+    def assemble_pipelined_ms(self, op, acc, field=None, reps=10):
+        """device ms of afe_assemble as it runs (element chunks overlapping the gathers)"""
+        fp, _keep = _field_ptr(field)
+        ms = C.c_double()
+        _chk(_lib().afe_assemble_timing(self.h, op, acc, fp, reps, C.byref(ms)))
+        return ms.value
+
     def close(self):
         if self.h:
             _lib().afe_matrix_destroy(self.h)

@@ -75,10 +75,16 @@ extern "C" int afe_version(void) { return 1; }
 // ===========================================================================
 // context and device buffers
 // ===========================================================================
+// chunks of a pipelined assembly (element kernels on the context stream,
+// gathers on the second stream)
+constexpr int kAsmChunksMax = 16;
+
 struct afe_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr; // the gathers of a pipelined assembly
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t asm_ev[kAsmChunksMax + 2] = {}; // pipelined assembly: chunk done, start, join
   std::recursive_mutex mu;
   std::atomic<int64_t> bytes{0}; // device bytes held (read without the lock by the server)
   double *d_quad = nullptr; // all quadrature rules
@@ -223,8 +229,10 @@ extern "C" int afe_ctx_create(int device, afe_ctx **out) {
   HIPCHK(hipSetDevice(device));
   scratch_init(device);
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&c->ev0));
   HIPCHK(hipEventCreate(&c->ev1));
+  for (auto &e : c->asm_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   {
     int lds = 0;
     HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
@@ -268,11 +276,15 @@ extern "C" int afe_ctx_destroy(afe_ctx *c) {
   if (!c) return AFE_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   for (auto &kv : c->user_bytes) (void)hipFree(kv.first);
   (void)hipFree(c->d_quad);
   (void)hipFree(c->d_bary);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
+  for (auto &e : c->asm_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return AFE_OK;
@@ -431,11 +443,43 @@ struct afe_space {
   afe_ctx *ctx = nullptr;
   Space h;
   int64_t seg_max = 0; // largest 256-row value segment (gather LDS)
+  // per 256-row gather block: the largest element adjacent to a row of this
+  // block or of an earlier one (the element chunks a block's gather waits for)
+  std::vector<int64_t> gather_dep;
+  bool distinct_dofs = false; // every element's DOFs are distinct (the batched gather)
   DevBuf<double> coords;
   DevBuf<int32_t> ev, elem_dof, dof_mark, adj, col;
   DevBuf<int64_t> adj_ptr, rp;
   DevBuf<uint8_t> slot;
 };
+
+// per 256-row block, the largest element adjacent to any row of blocks
+// 0..b (adjacency lists are ascending, so a row's last entry is its largest)
+static std::vector<int64_t> gather_deps(const Space &h) {
+  const int64_t nb = (h.n_dof + 255) / 256;
+  std::vector<int64_t> dep(nb);
+  int64_t m = -1;
+  for (int64_t b = 0; b < nb; b++) {
+    const int64_t r1 = std::min<int64_t>((b + 1) * 256, h.n_dof);
+    for (int64_t i = b * 256; i < r1; i++)
+      if (h.adj_ptr[i + 1] > h.adj_ptr[i]) m = std::max<int64_t>(m, h.adj[h.adj_ptr[i + 1] - 1] >> 3);
+    dep[b] = m;
+  }
+  return dep;
+}
+
+// no element lists a DOF twice (a degenerate element with a repeated vertex
+// would)
+static bool distinct_element_dofs(const Space &h) {
+  const int nl = h.n_loc;
+  for (int64_t e = 0; e < h.n_elem; e++) {
+    const int32_t *d = &h.elem_dof[(size_t)e * nl];
+    for (int a = 0; a < nl; a++)
+      for (int b = a + 1; b < nl; b++)
+        if (d[a] == d[b]) return false;
+  }
+  return true;
+}
 
 extern "C" int afe_space_create_sides(afe_ctx *ctx, int32_t kind, int64_t n_vert, const double *coords,
                                       int64_t n_elem, const int32_t *elem_vert, const int32_t *vert_mark,
@@ -461,6 +505,8 @@ extern "C" int afe_space_create_sides(afe_ctx *ctx, int32_t kind, int64_t n_vert
     const int64_t r1 = std::min<int64_t>(r0 + 256, s->h.n_dof);
     s->seg_max = std::max(s->seg_max, s->h.rp[r1] - s->h.rp[r0]);
   }
+  s->gather_dep = gather_deps(s->h);
+  s->distinct_dofs = distinct_element_dofs(s->h);
   if (s->seg_max * 8 > 64 * 1024) return fail(AFE_E_ARG, "rows too long for the LDS gather");
   s->coords.upload(ctx, s->h.coords);
   s->ev.upload(ctx, s->h.ev);
@@ -696,46 +742,138 @@ static FieldDev upload_field(afe_ctx *ctx, const afe_field *f, DevBuf<double> &s
   return fd;
 }
 
+// element matrices of the elements [e0, e1) on stream st
 template <int KIND>
-static void launch_elements(afe_matrix *a, int op, const QuadRule *q, FieldDev fd) {
+static void launch_elements_kind(afe_matrix *a, int op, const QuadRule *q, FieldDev fd, int64_t e0, int64_t e1,
+                                 hipStream_t st) {
   afe_space *s = a->s;
   const int64_t ne = s->h.n_elem;
   const double *qd = s->ctx->d_quad + q->off;
   const double *qb = s->ctx->d_bary + q->off;
-  hipStream_t st = s->ctx->stream;
+  const int nb = nblk(e1 - e0);
   if (op == AFE_OP_STIFF)
-    k_element_matrices<KIND, 0><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, qd, qb, q->n, fd, a->Ke.p);
+    k_element_matrices<KIND, 0><<<nb, 256, 0, st>>>(ne, e0, e1, s->coords.p, s->ev.p, qd, qb, q->n, fd, a->Ke.p);
   else if (op == AFE_OP_MASS)
-    k_element_matrices<KIND, 1><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, qd, qb, q->n, fd, a->Ke.p);
+    k_element_matrices<KIND, 1><<<nb, 256, 0, st>>>(ne, e0, e1, s->coords.p, s->ev.p, qd, qb, q->n, fd, a->Ke.p);
   else
-    k_element_matrices<KIND, 2><<<nblk(ne), 256, 0, st>>>(ne, s->coords.p, s->ev.p, qd, qb, q->n, fd, a->Ke.p);
+    k_element_matrices<KIND, 2><<<nb, 256, 0, st>>>(ne, e0, e1, s->coords.p, s->ev.p, qd, qb, q->n, fd, a->Ke.p);
   HIPCHK(hipGetLastError());
   a->ke_layout = op == AFE_OP_MASS ? 2 : 1;
 }
 
-template <int NL, int LAYOUT> static void launch_gather_nl(afe_matrix *a) {
-  afe_space *s = a->s;
-  const int64_t n = s->h.n_dof;
-  const size_t lds = (size_t)s->seg_max * sizeof(double);
-  k_gather<NL, LAYOUT><<<nblk(n), 256, lds, s->ctx->stream>>>(n, s->rp.p, s->adj_ptr.p, s->adj.p, s->slot.p,
-                                                              s->h.n_elem, a->Ke.p, a->val.p);
-}
-
-template <int NL> static void launch_gather_layout(afe_matrix *a) {
-  switch (a->ke_layout) {
-  case 0: launch_gather_nl<NL, 0>(a); break;
-  case 1: launch_gather_nl<NL, 1>(a); break;
-  default: launch_gather_nl<NL, 2>(a); break;
+static void launch_elements(afe_matrix *a, int op, const QuadRule *q, FieldDev fd, int64_t e0, int64_t e1,
+                            hipStream_t st) {
+  switch (a->s->h.kind) {
+  case P1_TRI: launch_elements_kind<P1_TRI>(a, op, q, fd, e0, e1, st); break;
+  case P2_TRI: launch_elements_kind<P2_TRI>(a, op, q, fd, e0, e1, st); break;
+  default: launch_elements_kind<P1_TET>(a, op, q, fd, e0, e1, st); break;
   }
 }
 
-static void launch_gather(afe_matrix *a) {
+// gather of the 256-row blocks [b0, b1) on stream st
+// Row blocks go to the XCDs in contiguous ranges (xcd = 1) on tetrahedral
+// meshes, whose rows have long element lists that share matrices through
+// one L2 (C3 gather 2.68 -> 2.52 ms), and round-robin on triangles (C2
+// 302 -> 310 us the other way); AFE_GATHER_XCD=0/1 forces either
+static int gather_xcd(int kind) {
+  static const int forced = [] {
+    const char *e = getenv("AFE_GATHER_XCD");
+    return e ? (atoi(e) != 0) : -1;
+  }();
+  return forced >= 0 ? forced : kind == P1_TET;
+}
+
+template <int NL, int LAYOUT> static void launch_gather_nl(afe_matrix *a, int64_t b0, int64_t b1, hipStream_t st) {
+  afe_space *s = a->s;
+  const size_t lds = (size_t)s->seg_max * sizeof(double);
+  const int xcd = gather_xcd(s->h.kind);
+  const int64_t nb = xcd ? (b1 - b0 + 7) / 8 * 8 : b1 - b0;
+  if (s->distinct_dofs)
+    k_gather<NL, LAYOUT, true><<<(unsigned)nb, 256, lds, st>>>(s->h.n_dof, b0, b1, xcd, s->rp.p, s->adj_ptr.p,
+                                                               s->adj.p, s->slot.p, s->h.n_elem, a->Ke.p, a->val.p);
+  else
+    k_gather<NL, LAYOUT, false><<<(unsigned)nb, 256, lds, st>>>(s->h.n_dof, b0, b1, xcd, s->rp.p, s->adj_ptr.p,
+                                                                s->adj.p, s->slot.p, s->h.n_elem, a->Ke.p, a->val.p);
+}
+
+template <int NL> static void launch_gather_layout(afe_matrix *a, int64_t b0, int64_t b1, hipStream_t st) {
+  switch (a->ke_layout) {
+  case 0: launch_gather_nl<NL, 0>(a, b0, b1, st); break;
+  case 1: launch_gather_nl<NL, 1>(a, b0, b1, st); break;
+  default: launch_gather_nl<NL, 2>(a, b0, b1, st); break;
+  }
+}
+
+static int64_t gather_blocks(const afe_space *s) { return (s->h.n_dof + 255) / 256; }
+
+static void launch_gather(afe_matrix *a, int64_t b0, int64_t b1, hipStream_t st) {
+  if (b1 <= b0) return;
   switch (a->s->h.n_loc) {
-  case 3: launch_gather_layout<3>(a); break;
-  case 4: launch_gather_layout<4>(a); break;
-  default: launch_gather_layout<6>(a); break;
+  case 3: launch_gather_layout<3>(a, b0, b1, st); break;
+  case 4: launch_gather_layout<4>(a, b0, b1, st); break;
+  default: launch_gather_layout<6>(a, b0, b1, st); break;
   }
   HIPCHK(hipGetLastError());
+}
+
+static void launch_gather(afe_matrix *a) { launch_gather(a, 0, gather_blocks(a->s), a->ctx->stream); }
+
+// number of element chunks of a pipelined assembly: 8 for tetrahedral
+// meshes from 2^18 elements up, where the gather (rows of ~24 elements) is
+// latency bound and leaves the chip room for the next chunk's element
+// matrices (C3, variable coefficient: 5.3 -> 4.0 ms; constant: no change);
+// 1 (no pipeline) on triangles, where the gather streams at bandwidth and
+// the overlap measured slower (C2 0.50 -> 0.53 ms, P2 variable coefficient
+// 1.37 -> 1.63 ms).  AFE_ASM_CHUNKS=k forces k chunks.
+static int asm_chunks(int kind, int64_t n_elem) {
+  static const int forced = [] {
+    const char *e = getenv("AFE_ASM_CHUNKS");
+    return e ? std::max(1, std::min(kAsmChunksMax, atoi(e))) : 0;
+  }();
+  if (forced) return forced;
+  return kind == P1_TET && n_elem >= (int64_t(1) << 18) ? 8 : 1;
+}
+
+// Element matrices and CSR gather of afe_assemble.  Pipelined: the elements
+// are computed in chunks on the context stream; after chunk k the gather of
+// every row block whose adjacent elements all lie in chunks 0..k runs on the
+// second stream, so the element work of the next chunk overlaps the gather
+// of the rows already complete.  Each row is still
+// summed by one thread in ascending element order, so the values are the
+// same bits as the two whole-mesh launches.  The context stream waits for
+// the last gather.
+static void launch_assembly(afe_matrix *a, int op, const QuadRule *q, FieldDev fd) {
+  afe_space *s = a->s;
+  afe_ctx *c = a->ctx;
+  const int64_t ne = s->h.n_elem, nb = gather_blocks(s);
+  const int K = (int64_t)s->gather_dep.size() == nb ? asm_chunks(s->h.kind, ne) : 1;
+  if (K <= 1) {
+    launch_elements(a, op, q, fd, 0, ne, c->stream);
+    launch_gather(a, 0, nb, c->stream);
+    return;
+  }
+  hipEvent_t ev_start = c->asm_ev[kAsmChunksMax], ev_join = c->asm_ev[kAsmChunksMax + 1];
+  HIPCHK(hipEventRecord(ev_start, c->stream)); // the gathers follow earlier work of the context stream
+  HIPCHK(hipStreamWaitEvent(c->stream2, ev_start, 0));
+  const int64_t ce = (ne + K - 1) / K;
+  int64_t b0 = 0;
+  for (int k = 0; k < K; k++) {
+    const int64_t e0 = k * ce, e1 = std::min(ne, e0 + ce);
+    if (e0 >= ne) break;
+    launch_elements(a, op, q, fd, e0, e1, c->stream);
+    // blocks whose largest adjacent element (prefix maximum) is below e1
+    const int64_t b1 = e1 == ne ? nb
+                                : (int64_t)(std::lower_bound(s->gather_dep.begin(), s->gather_dep.end(), e1) -
+                                            s->gather_dep.begin());
+    if (b1 > b0) {
+      HIPCHK(hipEventRecord(c->asm_ev[k], c->stream));
+      HIPCHK(hipStreamWaitEvent(c->stream2, c->asm_ev[k], 0));
+      launch_gather(a, b0, b1, c->stream2);
+      b0 = b1;
+    }
+  }
+  HIPCHK(hipEventRecord(ev_join, c->stream2));
+  HIPCHK(hipStreamWaitEvent(c->stream, ev_join, 0));
 }
 
 // the two-space element matrices (entry-major, LAYOUT 2) or the caller's
@@ -801,12 +939,7 @@ extern "C" int afe_assemble(afe_matrix *a, int32_t op, int32_t acc, const afe_fi
   DevBuf<double> fstore;
   FieldDev fd = upload_field(a->ctx, op == AFE_OP_VARSTIFF ? coef : nullptr, fstore);
   a->Ke.alloc(a->ctx, (size_t)s->h.n_elem * s->h.n_loc * s->h.n_loc);
-  switch (s->h.kind) {
-  case P1_TRI: launch_elements<P1_TRI>(a, op, q, fd); break;
-  case P2_TRI: launch_elements<P2_TRI>(a, op, q, fd); break;
-  default: launch_elements<P1_TET>(a, op, q, fd); break;
-  }
-  launch_gather(a);
+  launch_assembly(a, op, q, fd);
   HIPCHK(hipStreamSynchronize(a->ctx->stream));
   API_END
 }
@@ -837,11 +970,7 @@ extern "C" int afe_assemble_kernel_timing(afe_matrix *a, int32_t op, int32_t acc
   double te = 0, tg = 0;
   for (int r = 0; r < reps; r++) {
     HIPCHK(hipEventRecord(ev[0], st));
-    switch (s->h.kind) {
-    case P1_TRI: launch_elements<P1_TRI>(a, op, q, fd); break;
-    case P2_TRI: launch_elements<P2_TRI>(a, op, q, fd); break;
-    default: launch_elements<P1_TET>(a, op, q, fd); break;
-    }
+    launch_elements(a, op, q, fd, 0, s->h.n_elem, st);
     HIPCHK(hipEventRecord(ev[1], st));
     launch_gather(a);
     HIPCHK(hipEventRecord(ev[2], st));
@@ -861,6 +990,43 @@ extern "C" int afe_assemble_kernel_timing(afe_matrix *a, int32_t op, int32_t acc
     *bytes = (double)h.coords.size() * 8.0 + (double)h.n_elem * (nv + h.n_loc) * 4.0 +
              (double)h.rp[h.n_dof] * 8.0;
   }
+  API_END
+}
+
+// average device time of afe_assemble's kernels (element matrices and CSR
+// gather, pipelined as afe_assemble runs them) over `reps` assemblies, from
+// HIP events on the context stream, which waits for the last gather
+extern "C" int afe_assemble_timing(afe_matrix *a, int32_t op, int32_t acc, const afe_field *coef, int32_t reps,
+                                   double *ms) {
+  API_BEGIN
+  if (a && a->ps0) return fail(AFE_E_ARG, "assembly timing covers one-space assembly only");
+  if (!a || !a->s || reps < 1 || !ms) return fail(AFE_E_ARG, "bad arguments");
+  if (op < AFE_OP_STIFF || op > AFE_OP_VARSTIFF) return fail(AFE_E_ARG, "unknown operator");
+  if (op == AFE_OP_VARSTIFF && (!coef || coef->n_modes <= 0 || coef->dim != a->s->h.tdim))
+    return fail(AFE_E_ARG, "variable coefficient needs a field of the space dimension");
+  afe_space *s = a->s;
+  const QuadRule *q = quad_find(s->h.tdim, acc);
+  if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
+  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  HIPCHK(hipSetDevice(a->ctx->device));
+  DevBuf<double> fstore;
+  FieldDev fd = upload_field(a->ctx, op == AFE_OP_VARSTIFF ? coef : nullptr, fstore);
+  a->Ke.alloc(a->ctx, (size_t)s->h.n_elem * s->h.n_loc * s->h.n_loc);
+  hipStream_t st = a->ctx->stream;
+  hipEvent_t ev[2];
+  for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+  double t = 0;
+  for (int r = 0; r < reps; r++) {
+    HIPCHK(hipEventRecord(ev[0], st));
+    launch_assembly(a, op, q, fd);
+    HIPCHK(hipEventRecord(ev[1], st));
+    HIPCHK(hipEventSynchronize(ev[1]));
+    float m = 0;
+    HIPCHK(hipEventElapsedTime(&m, ev[0], ev[1]));
+    t += m;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  *ms = t / reps;
   API_END
 }
 
@@ -1982,6 +2148,8 @@ static int space_clone_into(afe_ctx *ctx, const afe_space *src, const double *co
   s->h = src->h;
   if (coords) std::memcpy(s->h.coords.data(), coords, s->h.coords.size() * sizeof(double));
   s->seg_max = src->seg_max;
+  s->gather_dep = src->gather_dep;
+  s->distinct_dofs = src->distinct_dofs;
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   HIPCHK(hipSetDevice(ctx->device));
   s->coords.upload(ctx, s->h.coords);

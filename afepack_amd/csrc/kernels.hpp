@@ -22,6 +22,15 @@ constexpr int BLK = WAVE * WPB;    // 256 threads
 constexpr int RED_BLOCKS = 1024;   // fixed grid of the reduction kernels
 constexpr int TAIL_THREADS = 1024; // single-workgroup coarse tail
 
+// Workgroups are dispatched round-robin over the 8 XCDs; with xcd != 0 the
+// logical block index makes consecutive logical blocks share an XCD, so each
+// XCD's L2 serves a contiguous (spatially local) range of slices.  gridDim.x
+// is a multiple of 8 when xcd != 0.
+__device__ inline int logical_block(int xcd) {
+  const int b = blockIdx.x;
+  return xcd == 1 ? (b & 7) * (int)(gridDim.x >> 3) + (b >> 3) : b;
+}
+
 struct FieldDev {
   int n_modes, dim;
   const double *k, *phi, *w;
@@ -72,14 +81,16 @@ template <int NL> __host__ __device__ constexpr int upper_index(int j, int k) { 
 }
 template <int NL> constexpr int upper_count() { return NL * (NL + 1) / 2; }
 
+// The launch covers the elements [e0, e1) (a chunk of a pipelined assembly);
+// n_elem is the stride of the entry-major store.
 template <int KIND, int OP>
 __global__ __launch_bounds__(256) void k_element_matrices(
-    int64_t n_elem, const double *__restrict__ coords, const int32_t *__restrict__ ev,
+    int64_t n_elem, int64_t e0, int64_t e1, const double *__restrict__ coords, const int32_t *__restrict__ ev,
     const double *__restrict__ qd, const double *__restrict__ qb, int nq, FieldDev fld,
     double *__restrict__ Ke) {
   constexpr int D = KindInfo<KIND>::D, NL = KindInfo<KIND>::NL;
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n_elem) return;
+  const int64_t e = e0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= e1) return;
   double v[D + 1][D];
   load_elem<KIND>(e, coords, ev, v);
   const double volume = tmpl_volume<D>();
@@ -137,7 +148,7 @@ __global__ __launch_bounds__(256) void k_element_matrices(
 // element -> CSR gather: one thread per row, 256 rows per block; the block's
 // contiguous slice of the value array is accumulated in LDS (each slot in
 // ascending element order, the summation order of buildSparseMatrix) and
-// written back coalesced.  NL is the element DOF count; LAYOUT names how Ke
+// written back coalesced.  The launch covers the row blocks [blk0, blk1).  NL is the element DOF count; LAYOUT names how Ke
 // holds the element matrices: 0 element-major NL x NL row-major (matrices
 // given by the caller), 1 entry-major upper triangle (stiffness forms),
 // 2 entry-major NL x NL (mass form).
@@ -150,13 +161,23 @@ __device__ __forceinline__ double ke_entry(const double *__restrict__ Ke, int64_
   return Ke[(int64_t)upper_index<NL>(j, k) * n_elem + e];
 }
 
-template <int NL, int LAYOUT>
+// BATCH: an element's NL DOFs are distinct, so its NL slots in row i are
+// distinct and the row's NL LDS updates of one element are independent: the
+// NL old values are read together, then the NL sums written (one LDS round
+// trip per element instead of NL dependent ones).  Each slot still takes its
+// contributions in ascending element order.  The next adjacency entry is
+// loaded while the current one is summed.
+template <int NL, int LAYOUT, bool BATCH>
 __global__ __launch_bounds__(256) void k_gather(
-    int64_t n_rows, const int64_t *__restrict__ rp, const int64_t *__restrict__ adj_ptr,
-    const int32_t *__restrict__ adj, const uint8_t *__restrict__ slot, int64_t n_elem,
-    const double *__restrict__ Ke, double *__restrict__ val) {
+    int64_t n_rows, int64_t blk0, int64_t blk1, int xcd, const int64_t *__restrict__ rp,
+    const int64_t *__restrict__ adj_ptr, const int32_t *__restrict__ adj, const uint8_t *__restrict__ slot,
+    int64_t n_elem, const double *__restrict__ Ke, double *__restrict__ val) {
   extern __shared__ double seg[];
-  const int64_t r0 = (int64_t)blockIdx.x * 256;
+  // with xcd = 1 each XCD takes a contiguous range of row blocks: the rows of
+  // one element (numbered close together) then read its matrix through one L2
+  const int64_t blk = blk0 + logical_block(xcd);
+  if (blk >= blk1) return; // the whole block (the grid is rounded up to a multiple of 8)
+  const int64_t r0 = blk * 256;
   const int64_t r1 = r0 + 256 < n_rows ? r0 + 256 : n_rows;
   const int64_t base = rp[r0];
   const int64_t segn = rp[r1] - base;
@@ -165,16 +186,31 @@ __global__ __launch_bounds__(256) void k_gather(
   const int64_t i = r0 + threadIdx.x;
   if (i < r1) {
     double *row = seg + (rp[i] - base);
-    for (int64_t p = adj_ptr[i]; p < adj_ptr[i + 1]; p++) {
-      const int32_t ea = adj[p];
+    const int64_t p0 = adj_ptr[i], p1 = adj_ptr[i + 1];
+    int32_t ea_next = p0 < p1 ? adj[p0] : 0;
+    for (int64_t p = p0; p < p1; p++) {
+      const int32_t ea = ea_next;
+      if (p + 1 < p1) ea_next = adj[p + 1];
       const int64_t e = ea >> 3;
       const int a = ea & 7;
       const uint8_t *sl = slot + p * NL;
+      int s[NL];
       double kv[NL];
 #pragma unroll
-      for (int b = 0; b < NL; b++) kv[b] = ke_entry<NL, LAYOUT>(Ke, n_elem, e, a, b);
+      for (int b = 0; b < NL; b++) {
+        s[b] = sl[b];
+        kv[b] = ke_entry<NL, LAYOUT>(Ke, n_elem, e, a, b);
+      }
+      if constexpr (BATCH) {
+        double old[NL];
 #pragma unroll
-      for (int b = 0; b < NL; b++) row[sl[b]] += kv[b];
+        for (int b = 0; b < NL; b++) old[b] = row[s[b]];
+#pragma unroll
+        for (int b = 0; b < NL; b++) row[s[b]] = old[b] + kv[b];
+      } else {
+#pragma unroll
+        for (int b = 0; b < NL; b++) row[s[b]] += kv[b];
+      }
     }
   }
   __syncthreads();
@@ -798,15 +834,6 @@ template <class T> __device__ inline T row_group_sum(T v, int gs) {
   if (gs >= 3) v += dpp_move<DPP_HALF_MIRROR>(v);
   if (gs >= 4) v += dpp_move<DPP_MIRROR>(v);
   return v;
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs; with xcd != 0 the
-// logical block index makes consecutive logical blocks share an XCD, so each
-// XCD's L2 serves a contiguous (spatially local) range of slices.  gridDim.x
-// is a multiple of 8 when xcd != 0.
-__device__ inline int logical_block(int xcd) {
-  const int b = blockIdx.x;
-  return xcd == 1 ? (b & 7) * (int)(gridDim.x >> 3) + (b >> 3) : b;
 }
 
 // slice schedule of the grid-stride level-0 kernels: sched 0 = contiguous

@@ -469,6 +469,12 @@ int afe_assemble_kernel_timing(afe_matrix *a, int32_t op, int32_t acc,
                                const afe_field *coef, int32_t reps,
                                double *elem_ms, double *gather_ms,
                                double *bytes);
+/* afe_assemble(a, op, acc, coef) `reps` times as afe_assemble runs it (the
+ * element chunks and the gathers of the rows they complete overlapping on
+ * two streams): *ms = the average device time from a HIP event before the
+ * first kernel to one after the last gather, on the context stream */
+int afe_assemble_timing(afe_matrix *a, int32_t op, int32_t acc,
+                        const afe_field *coef, int32_t reps, double *ms);
 /* live timing of the level-0 FP64 SpMV inside afe_amg_pcg (a start/stop
  * HIP event pair attached to that kernel's dispatch on the solver stream in
  * every CG iteration; the iterate is unchanged): enable = 1 resets and

@@ -31,10 +31,13 @@ def make_mesh(kind, n, jitter=0.0, seed=0):
 def rel_err_rows(a, b, rp):
     """max over rows of |a-b| / max|b in row| (entries that cancel to ~0 are
     measured against the row scale)"""
-    worst = 0.0
-    for i in range(len(rp) - 1):
-        s = slice(rp[i], rp[i + 1])
-        scale = np.abs(b[s]).max()
-        if scale > 0:
-            worst = max(worst, np.abs(a[s] - b[s]).max() / scale)
-    return worst
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    rp = np.asarray(rp, dtype=np.int64)
+    starts = rp[:-1][np.diff(rp) > 0]  # non-empty rows: each segment ends where the next begins
+    if len(starts) == 0:
+        return 0.0
+    scale = np.maximum.reduceat(np.abs(b), starts)
+    diff = np.maximum.reduceat(np.abs(a - b), starts)
+    ok = scale > 0
+    return float((diff[ok] / scale[ok]).max()) if ok.any() else 0.0

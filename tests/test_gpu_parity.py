@@ -354,6 +354,87 @@ def test_variable_coefficient_solution_matches_oracle(ctx, kind, n, jitter, acc,
     assert abs(e2 - oe2) <= 1e-6 * oe2 and abs(e1 - oe1) <= 1e-6 * oe1
 
 
+# meshes just above 2^18 elements, where afe_assemble pipelines the element
+# chunks and the gathers of the rows they complete (two streams)
+PIPELINE_CASES = [
+    (1, 364, 0.2, 4),  # 264,992 triangles
+    (2, 364, 0.2, 6),
+    (3, 36, 0.05, 2),  # 279,936 tetrahedra
+]
+
+
+@pytest.mark.parametrize("kind,n,jitter,acc", PIPELINE_CASES)
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_pipelined_assembly_matches_oracle(ctx, kind, n, jitter, acc, shuffle):
+    """Element chunks overlapping the gathers of completed rows give the
+    oracle's values bit for bit (stiffness and mass; the variable
+    coefficient within 1e-13 of the row scale), in the mesh's own element
+    order and with the elements shuffled (a row then waits for late chunks)"""
+    import afepack_amd as afe
+    from oracle import oracle_py as orc
+    print(f"case pipelined kind={kind} n={n} shuffle={shuffle}", flush=True)
+    c, ev, vm = make_mesh(kind, n, jitter, 7)
+    if shuffle:  # permute whole elements (rows of nv vertices)
+        nv = 4 if kind == 3 else 3
+        ev2 = np.asarray(ev).reshape(-1, nv)
+        ev = np.ascontiguousarray(ev2[np.random.default_rng(3).permutation(len(ev2))]).reshape(-1)
+    sp = afe.Space(ctx, kind, c, ev, vm)
+    osp = orc.Space(kind, c, ev, vm)
+    assert sp.n_elem >= 1 << 18
+    field = afe.field_random(8, sp.dim, 0.5, 5)
+    ofield = orc.Field(field.k, field.phi, field.w, field.scale)
+    A = afe.Matrix(sp)
+    for op in (0, 1, 2):
+        A.assemble(op, acc, field if op == 2 else None)
+        rp, col, oval = osp.build(op, acc, ofield if op == 2 else None).csr()
+        val = A.values()
+        if op == 2:
+            assert rel_err_rows(val, oval, rp) <= 1e-13
+        else:
+            assert np.array_equal(val, oval)
+
+
+def _assembly_cases_match_oracle():
+    """(child process) every kind at the pipeline size, three operators, in
+    the launch configuration the environment forces; prints one line per case"""
+    import afepack_amd as afe
+    from oracle import oracle_py as orc
+    from tests.conftest import rel_err_rows
+    c0 = afe.Context(0)
+    for kind, n, jitter, acc in PIPELINE_CASES:
+        c, ev, vm = make_mesh(kind, n, jitter, 7)
+        sp = afe.Space(c0, kind, c, ev, vm)
+        osp = orc.Space(kind, c, ev, vm)
+        field = afe.field_random(8, sp.dim, 0.5, 5)
+        ofield = orc.Field(field.k, field.phi, field.w, field.scale)
+        A = afe.Matrix(sp)
+        for op in (0, 1, 2):
+            A.assemble(op, acc, field if op == 2 else None)
+            rp, col, oval = osp.build(op, acc, ofield if op == 2 else None).csr()
+            val = A.values()
+            ok = rel_err_rows(val, oval, rp) <= 1e-13 if op == 2 else np.array_equal(val, oval)
+            print(f"kind {kind} op {op} {'ok' if ok else 'MISMATCH'}", flush=True)
+            assert ok
+
+
+@pytest.mark.parametrize("chunks,xcd", [("8", "1"), ("1", "0")])
+def test_forced_assembly_launches_match_oracle(chunks, xcd):
+    """The launch forms the defaults do not pick for a kind (pipelined
+    triangles with XCD-ranged gathers; tetrahedra in two whole-mesh launches
+    with round-robin gathers) give the oracle's values too (a child process,
+    since the forms are read from the environment once)"""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, AFE_ASM_CHUNKS=chunks, AFE_GATHER_XCD=xcd)
+    print(f"case forced chunks={chunks} xcd={xcd}", flush=True)
+    r = subprocess.run([sys.executable, "-c", "from tests.test_gpu_parity import _assembly_cases_match_oracle as f; f()"],
+                       cwd=root, env=env, capture_output=True, text=True, timeout=100)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0 and r.stdout.count(" ok") == 9
+
+
 def test_full_size_p2_assembly_matches_oracle(ctx):
     """The benchmark's own system, 1024x1024 squares x 2 P2 triangles
     (4,198,401 DOFs, 48,250,881 entries, jitter 0.2, seed 20261, acc 6):
