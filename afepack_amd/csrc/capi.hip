@@ -742,6 +742,17 @@ static FieldDev upload_field(afe_ctx *ctx, const afe_field *f, DevBuf<double> &s
   return fd;
 }
 
+// element matrices stored element-major on tetrahedral meshes (a row's
+// elements lie far apart in index there), entry-major on triangles;
+// AFE_KE_LAYOUT=entry / element forces either
+static bool ke_element_major(int kind) {
+  static const int forced = [] {
+    const char *e = getenv("AFE_KE_LAYOUT");
+    return e ? (std::strcmp(e, "element") == 0 ? 1 : 0) : -1;
+  }();
+  return forced >= 0 ? forced == 1 : kind == P1_TET;
+}
+
 // element matrices of the elements [e0, e1) on stream st
 template <int KIND>
 static void launch_elements_kind(afe_matrix *a, int op, const QuadRule *q, FieldDev fd, int64_t e0, int64_t e1,
@@ -751,14 +762,18 @@ static void launch_elements_kind(afe_matrix *a, int op, const QuadRule *q, Field
   const double *qd = s->ctx->d_quad + q->off;
   const double *qb = s->ctx->d_bary + q->off;
   const int nb = nblk(e1 - e0);
+  const bool em = ke_element_major(KIND);
+  auto go = [&](auto kern) {
+    kern<<<nb, 256, 0, st>>>(ne, e0, e1, s->coords.p, s->ev.p, qd, qb, q->n, fd, a->Ke.p);
+  };
   if (op == AFE_OP_STIFF)
-    k_element_matrices<KIND, 0><<<nb, 256, 0, st>>>(ne, e0, e1, s->coords.p, s->ev.p, qd, qb, q->n, fd, a->Ke.p);
+    em ? go(k_element_matrices<KIND, 0, true>) : go(k_element_matrices<KIND, 0, false>);
   else if (op == AFE_OP_MASS)
-    k_element_matrices<KIND, 1><<<nb, 256, 0, st>>>(ne, e0, e1, s->coords.p, s->ev.p, qd, qb, q->n, fd, a->Ke.p);
+    em ? go(k_element_matrices<KIND, 1, true>) : go(k_element_matrices<KIND, 1, false>);
   else
-    k_element_matrices<KIND, 2><<<nb, 256, 0, st>>>(ne, e0, e1, s->coords.p, s->ev.p, qd, qb, q->n, fd, a->Ke.p);
+    em ? go(k_element_matrices<KIND, 2, true>) : go(k_element_matrices<KIND, 2, false>);
   HIPCHK(hipGetLastError());
-  a->ke_layout = op == AFE_OP_MASS ? 2 : 1;
+  a->ke_layout = op == AFE_OP_MASS ? (em ? 0 : 2) : (em ? 3 : 1);
 }
 
 static void launch_elements(afe_matrix *a, int op, const QuadRule *q, FieldDev fd, int64_t e0, int64_t e1,
@@ -800,7 +815,8 @@ template <int NL> static void launch_gather_layout(afe_matrix *a, int64_t b0, in
   switch (a->ke_layout) {
   case 0: launch_gather_nl<NL, 0>(a, b0, b1, st); break;
   case 1: launch_gather_nl<NL, 1>(a, b0, b1, st); break;
-  default: launch_gather_nl<NL, 2>(a, b0, b1, st); break;
+  case 2: launch_gather_nl<NL, 2>(a, b0, b1, st); break;
+  default: launch_gather_nl<NL, 3>(a, b0, b1, st); break;
   }
 }
 

@@ -82,8 +82,12 @@ template <int NL> __host__ __device__ constexpr int upper_index(int j, int k) { 
 template <int NL> constexpr int upper_count() { return NL * (NL + 1) / 2; }
 
 // The launch covers the elements [e0, e1) (a chunk of a pipelined assembly);
-// n_elem is the stride of the entry-major store.
-template <int KIND, int OP>
+// n_elem is the stride of the entry-major store.  EM stores element-major
+// instead (an element's entries contiguous: the upper triangle, or the full
+// NL x NL row-major for the mass form), which suits meshes whose rows' elements
+// lie far apart in index (tetrahedra), where a wavefront's loads of one entry
+// over its rows' elements would each touch a separate line.
+template <int KIND, int OP, bool EM>
 __global__ __launch_bounds__(256) void k_element_matrices(
     int64_t n_elem, int64_t e0, int64_t e1, const double *__restrict__ coords, const int32_t *__restrict__ ev,
     const double *__restrict__ qd, const double *__restrict__ qb, int nq, FieldDev fld,
@@ -135,12 +139,22 @@ __global__ __launch_bounds__(256) void k_element_matrices(
 #pragma unroll
     for (int j = 0; j < NL; j++)
 #pragma unroll
-      for (int k = 0; k < NL; k++) Ke[(int64_t)(j * NL + k) * n_elem + e] = K[j][k];
+      for (int k = 0; k < NL; k++) {
+        if constexpr (EM)
+          Ke[e * (NL * NL) + j * NL + k] = K[j][k];
+        else
+          Ke[(int64_t)(j * NL + k) * n_elem + e] = K[j][k];
+      }
   } else {
 #pragma unroll
     for (int j = 0; j < NL; j++)
 #pragma unroll
-      for (int k = j; k < NL; k++) Ke[(int64_t)upper_index<NL>(j, k) * n_elem + e] = K[j][k];
+      for (int k = j; k < NL; k++) {
+        if constexpr (EM)
+          Ke[e * upper_count<NL>() + upper_index<NL>(j, k)] = K[j][k];
+        else
+          Ke[(int64_t)upper_index<NL>(j, k) * n_elem + e] = K[j][k];
+      }
   }
 }
 
@@ -150,14 +164,16 @@ __global__ __launch_bounds__(256) void k_element_matrices(
 // ascending element order, the summation order of buildSparseMatrix) and
 // written back coalesced.  The launch covers the row blocks [blk0, blk1).  NL is the element DOF count; LAYOUT names how Ke
 // holds the element matrices: 0 element-major NL x NL row-major (matrices
-// given by the caller), 1 entry-major upper triangle (stiffness forms),
-// 2 entry-major NL x NL (mass form).
+// given by the caller, or the element-major mass form), 1 entry-major upper
+// triangle (stiffness forms), 2 entry-major NL x NL (mass form), 3
+// element-major upper triangle (stiffness forms).
 // ---------------------------------------------------------------------------
 template <int NL, int LAYOUT>
 __device__ __forceinline__ double ke_entry(const double *__restrict__ Ke, int64_t n_elem, int64_t e, int a, int b) {
   if constexpr (LAYOUT == 0) return Ke[(e * NL + a) * NL + b];
   if constexpr (LAYOUT == 2) return Ke[(int64_t)(a * NL + b) * n_elem + e];
   const int j = a < b ? a : b, k = a < b ? b : a;
+  if constexpr (LAYOUT == 3) return Ke[e * upper_count<NL>() + upper_index<NL>(j, k)];
   return Ke[(int64_t)upper_index<NL>(j, k) * n_elem + e];
 }
 
