@@ -662,10 +662,11 @@ class Server:
         return {"device_bytes": b.value, "cached": c.value, "completed": d.value, "evictions": e.value}
 
     def timing(self):
-        t = (C.c_double * 9)()
-        _chk(_lib().afe_server_timing_n(self.h, t, 9))
+        t = (C.c_double * 13)()
+        _chk(_lib().afe_server_timing_n(self.h, t, 13))
         keys = ["setup_new_s", "setup_moving_s", "wait_setup_s", "wait_slot_s", "solve_s", "n_new", "n_moving",
-                "setup_cloned_s", "n_cloned"]
+                "setup_cloned_s", "n_cloned", "wait_room_s", "setup_cpu_user_s", "setup_cpu_sys_s",
+                "setup_lock_wait_s"]
         return dict(zip(keys, list(t)))
 
     def close(self):

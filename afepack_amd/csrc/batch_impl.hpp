@@ -429,7 +429,7 @@ template <class T> static void batch_issue_step(afe_batch *b, hipStream_t s) {
 }
 
 template <class T>
-static void batch_step_impl(afe_batch *b, int *n_active, std::unique_lock<std::recursive_mutex> &lk) {
+static void batch_step_impl(afe_batch *b, int *n_active, std::unique_lock<CtxMutex> &lk) {
   afe_ctx *ctx = b->ctx;
   hipStream_t s = ctx->stream;
   using clk = std::chrono::steady_clock;
@@ -507,7 +507,7 @@ extern "C" int afe_batch_create(afe_ctx *ctx, int32_t prec, int32_t smooth, afe_
   b->prec = prec;
   b->nu = smooth;
   {
-    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    std::lock_guard<CtxMutex> lk(ctx->mu);
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipEventCreateWithFlags(&b->step_ev, hipEventDisableTiming));
   }
@@ -537,7 +537,7 @@ extern "C" int afe_batch_destroy(afe_batch *b) {
             (long long)b->n_steps, b->n_epochs, 1e3 * b->t_epoch, 1e3 * b->t_graph, 1e3 * b->t_step);
   {
     std::lock_guard<std::recursive_mutex> bl(b->mu);
-    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+    std::lock_guard<CtxMutex> lk(b->ctx->mu);
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
   }
@@ -555,7 +555,7 @@ extern "C" int afe_batch_add(afe_batch *b, afe_amg *g, double *d_x, const double
   if (!g->ctx || g->ctx->device != b->ctx->device) return fail(AFE_E_ARG, "the hierarchy belongs to another device");
   if (b->prec == AFE_PREC_FP32 && !g->has32) return fail(AFE_E_ARG, "hierarchy built without FP32 copies");
   std::lock_guard<std::recursive_mutex> bl(b->mu);
-  std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+  std::lock_guard<CtxMutex> lk(b->ctx->mu);
   for (auto &p : b->slots)
     if (p && p->active && p->g == g) return fail(AFE_E_ARG, "hierarchy already has an active solve in this batch");
   HIPCHK(hipSetDevice(b->ctx->device));
@@ -601,7 +601,7 @@ extern "C" int afe_batch_step(afe_batch *b, int32_t *n_active) {
   API_BEGIN
   if (!b) return fail(AFE_E_ARG, "null batch");
   std::lock_guard<std::recursive_mutex> bl(b->mu);
-  std::unique_lock<std::recursive_mutex> lk(b->ctx->mu);
+  std::unique_lock<CtxMutex> lk(b->ctx->mu);
   HIPCHK(hipSetDevice(b->ctx->device));
   int na = 0;
   if (b->prec == AFE_PREC_FP32) batch_step_impl<float>(b, &na, lk);

@@ -23,6 +23,8 @@
 #include <string>
 #include <vector>
 
+#include <sys/resource.h>
+
 #include "../../include/afepack_amd.h"
 #include "host.hpp"
 #include "kernels.hpp"
@@ -79,13 +81,30 @@ extern "C" int afe_version(void) { return 1; }
 // gathers on the second stream)
 constexpr int kAsmChunksMax = 16;
 
+// seconds the calling thread has waited for context locks (the server
+// accounts its set-up threads' waits)
+static thread_local double t_ctx_lock_wait = 0;
+
+// the context lock: a recursive mutex whose contended acquisitions are timed
+struct CtxMutex {
+  std::recursive_mutex m;
+  void lock() {
+    if (m.try_lock()) return;
+    const auto t0 = std::chrono::steady_clock::now();
+    m.lock();
+    t_ctx_lock_wait += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  bool try_lock() { return m.try_lock(); }
+  void unlock() { m.unlock(); }
+};
+
 struct afe_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr; // the gathers of a pipelined assembly
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t asm_ev[kAsmChunksMax + 2] = {}; // pipelined assembly: chunk done, start, join
-  std::recursive_mutex mu;
+  CtxMutex mu;
   std::atomic<int64_t> bytes{0}; // device bytes held (read without the lock by the server)
   double *d_quad = nullptr; // all quadrature rules
   double *d_bary = nullptr; // reference barycentrics of every rule point (same indexing)
@@ -100,6 +119,21 @@ static thread_local int64_t t_dev_bytes = 0;
 // index order (DSatur's fewer colours cost host time a busy set-up pool
 // cannot spare; the server's set-up threads set it)
 static thread_local bool t_first_fit_only = false;
+
+// The library's own device buffers come from the device's stream-ordered
+// pool (hipMallocAsync / hipFreeAsync on the owning context's stream): a
+// set-up allocates and a cache eviction frees a few hundred buffers, and the
+// pool serves them without a device-wide allocation or free each time.  A
+// buffer is used on its context's stream, or on another stream only after an
+// event or synchronisation that follows its allocation; it is freed only when
+// no queued work uses it.  AFE_STREAM_ALLOC=0 uses hipMalloc / hipFree.
+static bool stream_ordered_alloc() {
+  static const bool v = [] {
+    const char *e = getenv("AFE_STREAM_ALLOC");
+    return !(e && atoi(e) == 0);
+  }();
+  return v;
+}
 
 template <class T> struct DevBuf {
   T *p = nullptr;
@@ -120,7 +154,10 @@ template <class T> struct DevBuf {
   ~DevBuf() { release(); }
   void release() {
     if (p) {
-      (void)hipFree(p);
+      if (ctx && stream_ordered_alloc())
+        (void)hipFreeAsync(p, ctx->stream);
+      else
+        (void)hipFree(p);
       if (ctx) ctx->bytes -= (int64_t)(n * sizeof(T));
       t_dev_bytes -= (int64_t)(n * sizeof(T));
     }
@@ -132,10 +169,12 @@ template <class T> struct DevBuf {
     release();
     ctx = c;
     if (count == 0) return;
-    if (hipMalloc((void **)&p, count * sizeof(T)) != hipSuccess) {
+    const hipError_t e = stream_ordered_alloc() ? hipMallocAsync((void **)&p, count * sizeof(T), c->stream)
+                                                : hipMalloc((void **)&p, count * sizeof(T));
+    if (e != hipSuccess) {
       p = nullptr;
       (void)hipGetLastError();
-      throw afe::Error{AFE_E_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed"};
+      throw afe::Error{AFE_E_NOMEM, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed"};
     }
     n = count;
     ctx->bytes += (int64_t)(n * sizeof(T));
@@ -230,6 +269,14 @@ extern "C" int afe_ctx_create(int device, afe_ctx **out) {
   scratch_init(device);
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+  if (stream_ordered_alloc()) {
+    // the pool keeps freed memory for reuse instead of returning it to the
+    // driver at every synchronisation
+    hipMemPool_t pool;
+    HIPCHK(hipDeviceGetDefaultMemPool(&pool, device));
+    uint64_t keep = UINT64_MAX;
+    HIPCHK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+  }
   HIPCHK(hipEventCreate(&c->ev0));
   HIPCHK(hipEventCreate(&c->ev1));
   for (auto &e : c->asm_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -293,14 +340,14 @@ extern "C" int afe_ctx_destroy(afe_ctx *c) {
 extern "C" int afe_ctx_synchronize(afe_ctx *c) {
   API_BEGIN
   if (!c) return fail(AFE_E_ARG, "null context");
-  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  std::lock_guard<CtxMutex> lk(c->mu);
   HIPCHK(hipStreamSynchronize(c->stream));
   API_END
 }
 
 extern "C" int afe_ctx_device_bytes(afe_ctx *c, int64_t *bytes) {
   if (!c || !bytes) return fail(AFE_E_ARG, "null argument");
-  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  std::lock_guard<CtxMutex> lk(c->mu);
   *bytes = c->bytes;
   return AFE_OK;
 }
@@ -316,7 +363,7 @@ extern "C" int afe_dev_alloc(afe_ctx *c, int64_t bytes, void **d_ptr) {
     *d_ptr = nullptr;
     return fail(AFE_E_NOMEM, "hipMalloc failed");
   }
-  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  std::lock_guard<CtxMutex> lk(c->mu);
   c->bytes += bytes;
   t_dev_bytes += bytes;
   c->user_bytes[*d_ptr] = bytes;
@@ -327,7 +374,7 @@ extern "C" int afe_dev_free(afe_ctx *c, void *d_ptr) {
   API_BEGIN
   if (!c) return fail(AFE_E_ARG, "null context");
   if (!d_ptr) return AFE_OK;
-  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  std::lock_guard<CtxMutex> lk(c->mu);
   auto it = c->user_bytes.find(d_ptr);
   if (it == c->user_bytes.end()) return fail(AFE_E_ARG, "pointer not allocated by afe_dev_alloc");
   HIPCHK(hipSetDevice(c->device));
@@ -342,7 +389,7 @@ extern "C" int afe_dev_free(afe_ctx *c, void *d_ptr) {
 extern "C" int afe_dev_upload(afe_ctx *c, void *d_dst, const void *h_src, int64_t bytes) {
   API_BEGIN
   if (!c || (bytes > 0 && (!d_dst || !h_src))) return fail(AFE_E_ARG, "null argument");
-  std::lock_guard<std::recursive_mutex> lk(c->mu); // stream work: never inside another thread's capture
+  std::lock_guard<CtxMutex> lk(c->mu); // stream work: never inside another thread's capture
   HIPCHK(hipSetDevice(c->device));
   if (bytes > 0) HIPCHK(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -352,7 +399,7 @@ extern "C" int afe_dev_upload(afe_ctx *c, void *d_dst, const void *h_src, int64_
 extern "C" int afe_dev_download(afe_ctx *c, void *h_dst, const void *d_src, int64_t bytes) {
   API_BEGIN
   if (!c || (bytes > 0 && (!d_src || !h_dst))) return fail(AFE_E_ARG, "null argument");
-  std::lock_guard<std::recursive_mutex> lk(c->mu); // stream work: never inside another thread's capture
+  std::lock_guard<CtxMutex> lk(c->mu); // stream work: never inside another thread's capture
   HIPCHK(hipSetDevice(c->device));
   if (bytes > 0) HIPCHK(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -362,7 +409,7 @@ extern "C" int afe_dev_download(afe_ctx *c, void *h_dst, const void *d_src, int6
 extern "C" int afe_dev_copy(afe_ctx *c, void *d_dst, const void *d_src, int64_t bytes) {
   API_BEGIN
   if (!c || (bytes > 0 && (!d_src || !d_dst))) return fail(AFE_E_ARG, "null argument");
-  std::lock_guard<std::recursive_mutex> lk(c->mu); // stream work: never inside another thread's capture
+  std::lock_guard<CtxMutex> lk(c->mu); // stream work: never inside another thread's capture
   HIPCHK(hipSetDevice(c->device));
   if (bytes > 0) HIPCHK(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -372,7 +419,7 @@ extern "C" int afe_dev_copy(afe_ctx *c, void *d_dst, const void *d_src, int64_t 
 extern "C" int afe_dev_zero(afe_ctx *c, void *d_ptr, int64_t bytes) {
   API_BEGIN
   if (!c || (bytes > 0 && !d_ptr)) return fail(AFE_E_ARG, "null argument");
-  std::lock_guard<std::recursive_mutex> lk(c->mu); // stream work: never inside another thread's capture
+  std::lock_guard<CtxMutex> lk(c->mu); // stream work: never inside another thread's capture
   HIPCHK(hipSetDevice(c->device));
   if (bytes > 0) HIPCHK(hipMemsetAsync(d_ptr, 0, bytes, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -498,7 +545,7 @@ extern "C" int afe_space_create_sides(afe_ctx *ctx, int32_t kind, int64_t n_vert
     *out = s.release();
     return AFE_OK;
   }
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  std::lock_guard<CtxMutex> lk(ctx->mu);
   HIPCHK(hipSetDevice(ctx->device));
   if (s->h.rp[s->h.n_dof] > INT32_MAX) return fail(AFE_E_ARG, "nnz exceeds 2^31");
   for (int64_t r0 = 0; r0 < s->h.n_dof; r0 += 256) {
@@ -534,7 +581,7 @@ extern "C" int afe_space_destroy(afe_space *s) {
     delete s;
     return AFE_OK;
   }
-  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  std::lock_guard<CtxMutex> lk(s->ctx->mu);
   (void)hipSetDevice(s->ctx->device);
   delete s;
   return AFE_OK;
@@ -574,7 +621,7 @@ extern "C" int afe_space_update_coords(afe_space *s, const double *coords) {
     std::memcpy(s->h.coords.data(), coords, s->h.coords.size() * sizeof(double));
     return AFE_OK;
   }
-  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  std::lock_guard<CtxMutex> lk(s->ctx->mu);
   HIPCHK(hipSetDevice(s->ctx->device));
   std::memcpy(s->h.coords.data(), coords, s->h.coords.size() * sizeof(double));
   s->coords.upload(s->ctx, s->h.coords);
@@ -615,7 +662,7 @@ extern "C" int afe_matrix_create(afe_space *s, afe_matrix **out) {
   API_BEGIN
   if (!s || !out) return fail(AFE_E_ARG, "null argument");
   if (!s->ctx) return fail(AFE_E_ARG, "host-only space (no device context)");
-  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  std::lock_guard<CtxMutex> lk(s->ctx->mu);
   HIPCHK(hipSetDevice(s->ctx->device));
   auto a = std::make_unique<afe_matrix>();
   a->ctx = s->ctx;
@@ -634,7 +681,7 @@ extern "C" int afe_matrix_create_csr(afe_ctx *ctx, int64_t m, const int64_t *row
                                      const int64_t *col, const double *val, afe_matrix **out) {
   API_BEGIN
   if (!ctx || !out || !row_ptr || !col || m <= 0) return fail(AFE_E_ARG, "null argument");
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  std::lock_guard<CtxMutex> lk(ctx->mu);
   HIPCHK(hipSetDevice(ctx->device));
   auto a = std::make_unique<afe_matrix>();
   a->ctx = ctx;
@@ -670,7 +717,7 @@ extern "C" int afe_matrix_create_pair(afe_space *s0, afe_space *s1, afe_matrix *
   API_BEGIN
   if (!s0 || !s1 || !out) return fail(AFE_E_ARG, "null argument");
   if (!s0->ctx || s0->ctx != s1->ctx) return fail(AFE_E_ARG, "the two spaces need one device context");
-  std::lock_guard<std::recursive_mutex> lk(s0->ctx->mu);
+  std::lock_guard<CtxMutex> lk(s0->ctx->mu);
   HIPCHK(hipSetDevice(s0->ctx->device));
   PairPattern pp;
   pair_build(pp, s0->h, s1->h);
@@ -718,7 +765,7 @@ extern "C" int afe_matrix_pattern(const afe_matrix *a, int64_t *rp, int64_t *col
 
 extern "C" int afe_matrix_destroy(afe_matrix *a) {
   if (!a) return AFE_OK;
-  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  std::lock_guard<CtxMutex> lk(a->ctx->mu);
   (void)hipSetDevice(a->ctx->device);
   delete a;
   return AFE_OK;
@@ -926,7 +973,7 @@ static int assemble_pair(afe_matrix *a, int32_t op, int32_t acc) {
   afe_space *s0 = a->ps0;
   const QuadRule *q = quad_find(s0->h.tdim, acc);
   if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
-  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  std::lock_guard<CtxMutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
   a->Ke.alloc(a->ctx, (size_t)s0->h.n_elem * s0->h.n_loc * a->ps1->h.n_loc);
   const int k0 = s0->h.kind, k1 = a->ps1->h.kind;
@@ -950,7 +997,7 @@ extern "C" int afe_assemble(afe_matrix *a, int32_t op, int32_t acc, const afe_fi
   afe_space *s = a->s;
   const QuadRule *q = quad_find(s->h.tdim, acc);
   if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
-  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  std::lock_guard<CtxMutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
   DevBuf<double> fstore;
   FieldDev fd = upload_field(a->ctx, op == AFE_OP_VARSTIFF ? coef : nullptr, fstore);
@@ -975,7 +1022,7 @@ extern "C" int afe_assemble_kernel_timing(afe_matrix *a, int32_t op, int32_t acc
   afe_space *s = a->s;
   const QuadRule *q = quad_find(s->h.tdim, acc);
   if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
-  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  std::lock_guard<CtxMutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
   DevBuf<double> fstore;
   FieldDev fd = upload_field(a->ctx, op == AFE_OP_VARSTIFF ? coef : nullptr, fstore);
@@ -1023,7 +1070,7 @@ extern "C" int afe_assemble_timing(afe_matrix *a, int32_t op, int32_t acc, const
   afe_space *s = a->s;
   const QuadRule *q = quad_find(s->h.tdim, acc);
   if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
-  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  std::lock_guard<CtxMutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
   DevBuf<double> fstore;
   FieldDev fd = upload_field(a->ctx, op == AFE_OP_VARSTIFF ? coef : nullptr, fstore);
@@ -1049,7 +1096,7 @@ extern "C" int afe_assemble_timing(afe_matrix *a, int32_t op, int32_t acc, const
 extern "C" int afe_assemble_elements(afe_matrix *a, const double *h_elem_mat) {
   API_BEGIN
   if (a && a->ps0 && h_elem_mat) {
-    std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+    std::lock_guard<CtxMutex> lk(a->ctx->mu);
     HIPCHK(hipSetDevice(a->ctx->device));
     a->Ke.upload(a->ctx, h_elem_mat, (size_t)a->ps0->h.n_elem * a->ps0->h.n_loc * a->ps1->h.n_loc);
     launch_gather_pair<0>(a);
@@ -1057,7 +1104,7 @@ extern "C" int afe_assemble_elements(afe_matrix *a, const double *h_elem_mat) {
     return AFE_OK;
   }
   if (!a || !a->s || !h_elem_mat) return fail(AFE_E_ARG, "null argument");
-  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  std::lock_guard<CtxMutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
   const Space &h = a->s->h;
   a->Ke.upload(a->ctx, h_elem_mat, (size_t)h.n_elem * h.n_loc * h.n_loc);
@@ -1070,7 +1117,7 @@ extern "C" int afe_assemble_elements(afe_matrix *a, const double *h_elem_mat) {
 extern "C" int afe_matrix_values(const afe_matrix *a, double *h_val) {
   API_BEGIN
   if (!a || !h_val) return fail(AFE_E_ARG, "null argument");
-  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  std::lock_guard<CtxMutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
   HIPCHK(hipMemcpyAsync(h_val, a->val.p, a->nnz * sizeof(double), hipMemcpyDeviceToHost, a->ctx->stream));
   HIPCHK(hipStreamSynchronize(a->ctx->stream));
@@ -1080,7 +1127,7 @@ extern "C" int afe_matrix_values(const afe_matrix *a, double *h_val) {
 extern "C" int afe_matrix_set_values(afe_matrix *a, const double *h_val) {
   API_BEGIN
   if (!a || !h_val) return fail(AFE_E_ARG, "null argument");
-  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  std::lock_guard<CtxMutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
   HIPCHK(hipMemcpyAsync(a->val.p, h_val, a->nnz * sizeof(double), hipMemcpyHostToDevice, a->ctx->stream));
   HIPCHK(hipStreamSynchronize(a->ctx->stream));
@@ -1122,7 +1169,7 @@ __global__ __launch_bounds__(256) void k_csr_tvmult(int64_t n, const int64_t *__
 extern "C" int afe_matrix_tvmult(afe_matrix *a, const double *d_x, double *d_y) {
   API_BEGIN
   if (!a || !d_x || !d_y) return fail(AFE_E_ARG, "null argument");
-  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  std::lock_guard<CtxMutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
   if (!a->t_ptr.p) {
     // transposed index map of the pattern, built once per matrix: column j
@@ -1153,7 +1200,7 @@ extern "C" int afe_matrix_tvmult(afe_matrix *a, const double *d_x, double *d_y) 
 extern "C" int afe_matrix_vmult(afe_matrix *a, const double *d_x, double *d_y) {
   API_BEGIN
   if (!a || !d_x || !d_y) return fail(AFE_E_ARG, "null argument");
-  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  std::lock_guard<CtxMutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
   k_csr_vmult<<<nblk(a->m), 256, 0, a->ctx->stream>>>(a->m, a->rp(), a->col(), a->val.p, d_x, d_y);
   HIPCHK(hipGetLastError());
@@ -1185,7 +1232,7 @@ extern "C" int afe_rhs(afe_space *s, int32_t acc, int32_t fkind, const afe_field
     return fail(AFE_E_ARG, "variable-coefficient rhs needs a field");
   const QuadRule *q = quad_find(s->h.tdim, acc);
   if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
-  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  std::lock_guard<CtxMutex> lk(s->ctx->mu);
   HIPCHK(hipSetDevice(s->ctx->device));
   DevBuf<double> fstore, Fe;
   FieldDev fd = upload_field(s->ctx, fkind == AFE_F_VARCOEF ? coef : nullptr, fstore);
@@ -1207,7 +1254,7 @@ extern "C" int afe_apply_dirichlet(afe_matrix *a, double *d_u, double *d_f) {
   (void)d_u;
   if (a && a->ps0) return fail(AFE_E_ARG, "Dirichlet elimination needs the square matrix of one space");
   if (!a || !a->s || !d_f) return fail(AFE_E_ARG, "matrix without FE space or null rhs");
-  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  std::lock_guard<CtxMutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
   afe_space *s = a->s;
   k_dirichlet_zero<<<nblk(s->h.n_dof), 256, 0, a->ctx->stream>>>(s->h.n_dof, s->rp.p, s->col.p, s->dof_mark.p,
@@ -1224,7 +1271,7 @@ extern "C" int afe_space_qpoints(afe_space *s, int32_t acc, int64_t *n_qp, doubl
   if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
   if (n_qp) *n_qp = q->n;
   if (!h_pts && !h_jxw) return AFE_OK;
-  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  std::lock_guard<CtxMutex> lk(s->ctx->mu);
   HIPCHK(hipSetDevice(s->ctx->device));
   const int64_t ne = s->h.n_elem;
   const int D = s->h.tdim;
@@ -1250,7 +1297,7 @@ extern "C" int afe_rhs_qp(afe_space *s, int32_t acc, const double *h_fq, double 
   if (!s || !s->ctx || !h_fq || !d_f) return fail(AFE_E_ARG, "null argument");
   const QuadRule *q = quad_find(s->h.tdim, acc);
   if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
-  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  std::lock_guard<CtxMutex> lk(s->ctx->mu);
   HIPCHK(hipSetDevice(s->ctx->device));
   const int64_t ne = s->h.n_elem;
   DevBuf<double> fq, Fe;
@@ -1280,7 +1327,7 @@ extern "C" int afe_apply_dirichlet_bc(afe_matrix *a, const int32_t *marks, int32
   (void)d_u;
   if (a && a->ps0) return fail(AFE_E_ARG, "Dirichlet elimination needs the square matrix of one space");
   if (!a || !a->s || !d_f || !h_g || (n_marks > 0 && !marks)) return fail(AFE_E_ARG, "bad arguments");
-  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  std::lock_guard<CtxMutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
   afe_space *s = a->s;
   const int64_t n = s->h.n_dof;
@@ -1335,7 +1382,7 @@ extern "C" int afe_error_qp(afe_space *s, const double *d_uh, int32_t acc, int32
   if (!s || !s->ctx || !d_uh || !h_exact || !err) return fail(AFE_E_ARG, "null argument");
   const QuadRule *q = quad_find(s->h.tdim, acc);
   if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
-  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  std::lock_guard<CtxMutex> lk(s->ctx->mu);
   HIPCHK(hipSetDevice(s->ctx->device));
   const int64_t ne = s->h.n_elem;
   DevBuf<double> ex, part, tot;
@@ -1357,7 +1404,7 @@ extern "C" int afe_error_qp_noalloc(afe_space *s, const double *d_uh, int32_t ac
   const QuadRule *q = quad_find(s->h.tdim, acc);
   if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
   static_assert(RED_BLOCKS <= NOALLOC_SCRATCH_DOUBLES, "the reduction's partial sums must fit the scratch array");
-  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  std::lock_guard<CtxMutex> lk(s->ctx->mu);
   HIPCHK(hipSetDevice(s->ctx->device));
   double *scr = scratch_of(s->ctx->device);
   if (!scr) return fail(AFE_E_HIP, "scratch array of this device not initialised (no context created on it)");
@@ -1381,7 +1428,7 @@ static int fe_error(afe_space *s, const double *d_uh, int acc, int h1, double *e
   if (!s->ctx) return fail(AFE_E_ARG, "host-only space (no device context)");
   const QuadRule *q = quad_find(s->h.tdim, acc);
   if (!q) return fail(AFE_E_ARG, "no quadrature rule of that accuracy");
-  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  std::lock_guard<CtxMutex> lk(s->ctx->mu);
   HIPCHK(hipSetDevice(s->ctx->device));
   DevBuf<double> pe, part, tot;
   pe.alloc(s->ctx, s->h.n_elem);
@@ -1957,7 +2004,7 @@ extern "C" int afe_amg_create(afe_matrix *a, const afe_amg_params *p, afe_amg **
   hp.lazy = prm.lazy;
   Csr A0;
   {
-    std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+    std::lock_guard<CtxMutex> lk(a->ctx->mu);
     HIPCHK(hipSetDevice(a->ctx->device));
     A0 = fine_csr(a);
   }
@@ -1969,7 +2016,7 @@ extern "C" int afe_amg_create(afe_matrix *a, const afe_amg_params *p, afe_amg **
   HostLayout lay;
   build_layout(g->h, lay, prm.lazy != 0);
   const auto t2 = std::chrono::steady_clock::now();
-  std::lock_guard<std::recursive_mutex> lk(a->ctx->mu);
+  std::lock_guard<CtxMutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
   HIPCHK(hipHostMalloc((void **)&g->h_sc, sizeof(CgScalars), hipHostMallocDefault));
   upload_layout(g.get(), lay);
@@ -2033,7 +2080,7 @@ extern "C" int afe_amg_destroy(afe_amg *g) {
     delete g;
     return AFE_OK;
   }
-  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  std::lock_guard<CtxMutex> lk(g->ctx->mu);
   (void)hipSetDevice(g->ctx->device);
   (void)hipStreamSynchronize(g->ctx->stream);
   drop_graphs(g);
@@ -2051,7 +2098,7 @@ extern "C" int afe_amg_destroy(afe_amg *g) {
 // coarsest values.  Layouts, launch tables and captured graphs stay valid.
 static void device_lazy_refresh(afe_amg *g, afe_matrix *a) {
   afe_ctx *ctx = g->ctx;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  std::lock_guard<CtxMutex> lk(ctx->mu);
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   Hierarchy &h = g->h;
@@ -2097,7 +2144,7 @@ static void device_lazy_refresh(afe_amg *g, afe_matrix *a) {
 // host copies of the level values after device refreshes
 static void sync_host_values(afe_amg *g) {
   if (!g->host_vals_stale || !g->ctx) return;
-  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  std::lock_guard<CtxMutex> lk(g->ctx->mu);
   hipStream_t s = g->ctx->stream;
   for (size_t l = 0; l < g->lv.size(); l++) {
     Csr &A = g->h.A[l];
@@ -2120,7 +2167,7 @@ extern "C" int afe_amg_lazy_reinit(afe_amg *g, afe_matrix *a) {
   }
   Csr A0;
   {
-    std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+    std::lock_guard<CtxMutex> lk(g->ctx->mu);
     HIPCHK(hipSetDevice(g->ctx->device));
     A0 = fine_csr(a);
   }
@@ -2129,7 +2176,7 @@ extern "C" int afe_amg_lazy_reinit(afe_amg *g, afe_matrix *a) {
   amg_lazy_refresh(g->h, A0);
   HostLayout lay;
   build_layout(g->h, lay, true);
-  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  std::lock_guard<CtxMutex> lk(g->ctx->mu);
   HIPCHK(hipSetDevice(g->ctx->device));
   g->A0 = a;
   upload_layout(g, lay);
@@ -2166,7 +2213,7 @@ static int space_clone_into(afe_ctx *ctx, const afe_space *src, const double *co
   s->seg_max = src->seg_max;
   s->gather_dep = src->gather_dep;
   s->distinct_dofs = src->distinct_dofs;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  std::lock_guard<CtxMutex> lk(ctx->mu);
   HIPCHK(hipSetDevice(ctx->device));
   s->coords.upload(ctx, s->h.coords);
   clone_buf(ctx, s->ev, src->ev);
@@ -2206,7 +2253,7 @@ extern "C" int afe_amg_clone(const afe_amg *src, afe_matrix *a, afe_amg **out) {
   g->key_mat = a->s ? nullptr : a;
   g->prm = src->prm;
   g->h = src->h; // host copy before the context lock: other callers keep the device meanwhile
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  std::lock_guard<CtxMutex> lk(ctx->mu);
   HIPCHK(hipSetDevice(ctx->device));
   g->has32 = src->has32;
   g->tail0 = src->tail0;
@@ -2324,7 +2371,7 @@ extern "C" int afe_amg_level_csr(const afe_amg *g, int32_t level, int32_t which,
   if (level < 0 || level >= L || (which == 1 && level >= L - 1) || which < 0 || which > 1)
     return fail(AFE_E_ARG, "no such level matrix");
   if (val && which == 0 && g->ctx) {
-    std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+    std::lock_guard<CtxMutex> lk(g->ctx->mu);
     try {
       sync_host_values(const_cast<afe_amg *>(g));
     } catch (const HipFail &h) {
@@ -2547,7 +2594,7 @@ extern "C" int afe_amg_solve(afe_amg *g, double *d_x, const double *d_r, double 
                              double rtol, afe_solve_stats *st) {
   API_BEGIN
   if (!g || !d_x || !d_r || step < 0) return fail(AFE_E_ARG, "bad solve arguments");
-  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  std::lock_guard<CtxMutex> lk(g->ctx->mu);
   HIPCHK(hipSetDevice(g->ctx->device));
   afe_ctx *ctx = g->ctx;
   hipStream_t s = ctx->stream;
@@ -2811,7 +2858,7 @@ extern "C" int afe_amg_pcg(afe_amg *g, double *d_x, const double *d_b, double rt
   API_BEGIN
   if (!g || !d_x || !d_b || rtol <= 0 || max_it < 1) return fail(AFE_E_ARG, "bad PCG arguments");
   if (prec == AFE_PREC_FP32 && !g->has32) return fail(AFE_E_ARG, "hierarchy built without FP32 copies");
-  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  std::lock_guard<CtxMutex> lk(g->ctx->mu);
   HIPCHK(hipSetDevice(g->ctx->device));
   return prec == AFE_PREC_FP32 ? pcg_impl<float>(g, d_x, d_b, rtol, max_it, st)
                                : pcg_impl<double>(g, d_x, d_b, rtol, max_it, st);
@@ -2822,7 +2869,7 @@ extern "C" int afe_amg_vcycle(afe_amg *g, double *d_z, const double *d_r, int32_
   API_BEGIN
   if (!g || !d_z || !d_r) return fail(AFE_E_ARG, "null argument");
   if (prec == AFE_PREC_FP32 && !g->has32) return fail(AFE_E_ARG, "hierarchy built without FP32 copies");
-  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  std::lock_guard<CtxMutex> lk(g->ctx->mu);
   HIPCHK(hipSetDevice(g->ctx->device));
   hipStream_t s = g->ctx->stream;
   DevLevel &d0 = g->lv[0];
@@ -2853,7 +2900,7 @@ extern "C" int afe_amg_kernel_timing(afe_amg *g, int32_t which, int32_t reps, do
   API_BEGIN
   if (!g || !g->ctx || reps < 1 || which < 0 || which > 3) return fail(AFE_E_ARG, "bad arguments");
   if ((which == 1 || which == 3) && !g->has32) return fail(AFE_E_ARG, "no FP32 copies");
-  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  std::lock_guard<CtxMutex> lk(g->ctx->mu);
   HIPCHK(hipSetDevice(g->ctx->device));
   afe_ctx *ctx = g->ctx;
   hipStream_t s = ctx->stream;
@@ -2905,7 +2952,7 @@ extern "C" int afe_amg_kernel_timing(afe_amg *g, int32_t which, int32_t reps, do
 extern "C" int afe_copy_kernel_timing(afe_ctx *ctx, int64_t bytes, int32_t reps, double *avg_ms) {
   API_BEGIN
   if (!ctx || bytes < 16 || reps < 1 || !avg_ms) return fail(AFE_E_ARG, "bad arguments");
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  std::lock_guard<CtxMutex> lk(ctx->mu);
   HIPCHK(hipSetDevice(ctx->device));
   const int64_t n2 = bytes / 16;
   DevBuf<double> a, b;
@@ -2937,7 +2984,7 @@ extern "C" int afe_amg_set_smooth(afe_amg *g, int32_t smooth) {
 // (src/AMGSolver.cpp:601-605)
 extern "C" int afe_amg_set_coarse_exact(afe_amg *g, int32_t exact) {
   if (!g || !g->ctx) return fail(AFE_E_ARG, "bad arguments");
-  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  std::lock_guard<CtxMutex> lk(g->ctx->mu);
   if ((exact != 0) != g->coarse_exact) {
     drop_graphs(g);
     g->coarse_exact = exact != 0;
@@ -2951,7 +2998,7 @@ extern "C" int afe_amg_set_coarse_exact(afe_amg *g, int32_t exact) {
 extern "C" int afe_amg_live_timing(afe_amg *g, int32_t enable, double *avg_ms, int64_t *launches) {
   API_BEGIN
   if (!g || !g->ctx) return fail(AFE_E_ARG, "bad arguments");
-  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  std::lock_guard<CtxMutex> lk(g->ctx->mu);
   HIPCHK(hipSetDevice(g->ctx->device));
   if (avg_ms) *avg_ms = g->live_n ? g->live_ms / g->live_n : 0.0;
   if (launches) *launches = g->live_n;
@@ -2974,7 +3021,7 @@ extern "C" int afe_amg_live_timing(afe_amg *g, int32_t enable, double *avg_ms, i
 // levels of at most `rows` rows run in the single-workgroup tail from now on
 extern "C" int afe_amg_set_tail_rows(afe_amg *g, int64_t rows) {
   if (!g || !g->ctx || rows < 0) return fail(AFE_E_ARG, "bad arguments");
-  std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+  std::lock_guard<CtxMutex> lk(g->ctx->mu);
   const int L = (int)g->lv.size();
   drop_graphs(g);
   g->tail0 = L - 1;

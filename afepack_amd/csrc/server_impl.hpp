@@ -52,8 +52,21 @@ struct SrvRequest {
   int32_t slot = -1;
   std::vector<double> x; // solution (when keep_x)
   afe_solve_stats st{};
-  std::chrono::steady_clock::time_point t_submit, t_setup, t_ready, t_admit, t_done;
+  // t_taken: a set-up thread took it; t_setup: HBM room found, set-up starts
+  std::chrono::steady_clock::time_point t_submit, t_taken, t_setup, t_ready, t_admit, t_done;
+  double cpu_user = 0, cpu_sys = 0, lock_wait = 0; // of the set-up thread during the set-up
 };
+
+// user and system CPU seconds of the calling thread
+static void thread_cpu(double &user, double &sys) {
+  struct rusage ru;
+  if (getrusage(RUSAGE_THREAD, &ru) != 0) {
+    user = sys = 0;
+    return;
+  }
+  user = ru.ru_utime.tv_sec + 1e-6 * ru.ru_utime.tv_usec;
+  sys = ru.ru_stime.tv_sec + 1e-6 * ru.ru_stime.tv_usec;
+}
 
 // structured-mesh sizes
 static void mesh_sizes(int kind, int n, int64_t &nv, int64_t &ne, int64_t &ndof, int64_t &nnz) {
@@ -128,6 +141,13 @@ struct afe_server {
   std::vector<std::thread> workers;
   std::thread engine;
   bool stop = false;
+  // wall seconds of the set-up steps summed over requests (AFE_VERBOSE
+  // prints their means at destroy): new meshes 0-7, moving meshes 8-14
+  double t_phase[15] = {};
+  int64_t n_phase_new = 0, n_phase_moving = 0;
+  // seconds the server lock was held for evictions and for the engine's
+  // batch calls (AFE_VERBOSE)
+  double t_evict = 0, t_engine_locked = 0;
   int64_t next_id = 1, completed = 0, evictions = 0;
   uint64_t clock = 0;
   std::string engine_error;
@@ -137,6 +157,9 @@ struct afe_server {
   int64_t n_new = 0, n_moving = 0;
   double t_setup_cloned = 0; // the part of t_setup_new spent on meshes set up by structure copies
   int64_t n_cloned = 0;
+  // waiting for HBM room before a set-up; the set-up threads' user and
+  // system CPU seconds and context-lock waits during set-ups
+  double t_wait_room = 0, cpu_user = 0, cpu_sys = 0, lock_wait = 0;
 };
 
 static void srv_free_entry(afe_server *s, SrvEntry *e) {
@@ -156,7 +179,7 @@ static int64_t srv_bytes(afe_server *s) {
     if (kv.second->g) b += kv.second->est_bytes;
   int64_t live;
   {
-    std::lock_guard<std::recursive_mutex> lc(s->ctx->mu);
+    std::lock_guard<CtxMutex> lc(s->ctx->mu);
     live = s->ctx->bytes;
   }
   return std::max(b, live) + s->reserved;
@@ -173,7 +196,9 @@ static bool srv_make_room(afe_server *s, int64_t need) {
       if (!victim || e->last_use < victim->last_use) victim = e;
     }
     if (!victim) return false;
+    const auto t0 = std::chrono::steady_clock::now();
     srv_free_entry(s, victim);
+    s->t_evict += seconds_between(t0, std::chrono::steady_clock::now());
     s->evictions++;
     s->cache.erase(victim->key);
   }
@@ -226,8 +251,24 @@ static int srv_setup(afe_server *s, SrvRequest *q) {
   int rc;
   std::vector<double> c;
   std::vector<int32_t> ev;
+  double ph[15] = {};
+  auto tp = std::chrono::steady_clock::now();
+  auto mark = [&](int i) {
+    const auto now = std::chrono::steady_clock::now();
+    ph[i] += std::chrono::duration<double>(now - tp).count();
+    tp = now;
+  };
+  struct Flush { // adds this set-up's step times to the server's sums on every return
+    afe_server *s; double *ph; bool moving;
+    ~Flush() {
+      std::lock_guard<std::mutex> lk(s->m);
+      for (int i = 0; i < 15; i++) s->t_phase[i] += ph[i];
+      (moving ? s->n_phase_moving : s->n_phase_new)++;
+    }
+  } flush{s, ph, e->g != nullptr};
   if (!e->g) {
     if ((rc = srv_mesh(e, q, c, ev, true))) return rc;
+    mark(0);
     const int64_t nv = (int64_t)e->vmark.size();
     const int64_t ne = (int64_t)ev.size() / (r.kind == AFE_P1_TET ? 4 : 3);
     if (q->tmpl) // same connectivity as a cached entry: copy its numbering, pattern and gather map
@@ -235,15 +276,20 @@ static int srv_setup(afe_server *s, SrvRequest *q) {
     else
       rc = afe_space_create(s->ctx, r.kind, nv, c.data(), ne, ev.data(), e->vmark.data(), &e->sp);
     if (rc) return rc;
+    mark(1);
     if ((rc = afe_matrix_create(e->sp, &e->A))) return rc;
     int32_t nl;
     int64_t nz;
     afe_space_info(e->sp, &e->n_dof, &nz, &nl);
     if ((rc = afe_dev_alloc(s->ctx, e->n_dof * 8, (void **)&e->d_b))) return rc;
     if ((rc = afe_dev_alloc(s->ctx, e->n_dof * 8, (void **)&e->d_x))) return rc;
+    mark(2);
     if ((rc = afe_assemble(e->A, r.op, r.acc, cf))) return rc;
+    mark(3);
     if ((rc = afe_rhs(e->sp, r.acc, r.fkind, cf, nullptr, e->d_b))) return rc;
+    mark(4);
     if ((rc = afe_apply_dirichlet(e->A, nullptr, e->d_b))) return rc;
+    mark(5);
     if (q->tmpl) { // its hierarchy's structure, with this matrix's Galerkin products
       if ((rc = afe_amg_clone(q->tmpl->g, e->A, &e->g))) return rc;
       afe_amg_set_smooth(e->g, 1);
@@ -252,17 +298,28 @@ static int srv_setup(afe_server *s, SrvRequest *q) {
       if ((rc = afe_amg_create(e->A, &p, &e->g))) return rc;
     }
     afe_amg_set_tail_rows(e->g, g_tune.tail_rows);
-  } else {
-    // moving-mesh re-solve: new coordinates, same connectivity and pattern;
-    // values-only assembly and the lazy hierarchy refresh
-    if ((rc = srv_mesh(e, q, c, ev, false))) return rc;
-    if ((rc = afe_space_update_coords(e->sp, c.data()))) return rc;
-    if ((rc = afe_assemble(e->A, r.op, r.acc, cf))) return rc;
-    if ((rc = afe_rhs(e->sp, r.acc, r.fkind, cf, nullptr, e->d_b))) return rc;
-    if ((rc = afe_apply_dirichlet(e->A, nullptr, e->d_b))) return rc;
-    if ((rc = afe_amg_lazy_reinit(e->g, e->A))) return rc;
+    mark(6);
+    rc = afe_dev_zero(s->ctx, e->d_x, e->n_dof * 8);
+    mark(7);
+    return rc;
   }
-  return afe_dev_zero(s->ctx, e->d_x, e->n_dof * 8);
+  // moving-mesh re-solve: new coordinates, same connectivity and pattern;
+  // values-only assembly and the lazy hierarchy refresh
+  if ((rc = srv_mesh(e, q, c, ev, false))) return rc;
+  mark(8);
+  if ((rc = afe_space_update_coords(e->sp, c.data()))) return rc;
+  mark(9);
+  if ((rc = afe_assemble(e->A, r.op, r.acc, cf))) return rc;
+  mark(10);
+  if ((rc = afe_rhs(e->sp, r.acc, r.fkind, cf, nullptr, e->d_b))) return rc;
+  mark(11);
+  if ((rc = afe_apply_dirichlet(e->A, nullptr, e->d_b))) return rc;
+  mark(12);
+  if ((rc = afe_amg_lazy_reinit(e->g, e->A))) return rc;
+  mark(13);
+  rc = afe_dev_zero(s->ctx, e->d_x, e->n_dof * 8);
+  mark(14);
+  return rc;
 }
 
 // the cached entry holds this request's mesh (same key and connectivity,
@@ -334,7 +391,7 @@ static void srv_worker(afe_server *s) {
       continue;
     }
     q->state = 1;
-    q->t_setup = std::chrono::steady_clock::now();
+    q->t_taken = std::chrono::steady_clock::now();
     auto ce = s->cache.find(q->r.mesh_key);
     SrvEntry *e;
     bool fresh = false;
@@ -398,7 +455,15 @@ static void srv_worker(afe_server *s) {
     s->reserved += need;
     lk.unlock();
     t_dev_bytes = 0;
+    q->t_setup = std::chrono::steady_clock::now();
+    double u0, s0, u1, s1;
+    thread_cpu(u0, s0);
+    const double w0 = t_ctx_lock_wait;
     const int rc = srv_setup(s, q);
+    thread_cpu(u1, s1);
+    q->cpu_user = u1 - u0;
+    q->cpu_sys = s1 - s0;
+    q->lock_wait = t_ctx_lock_wait - w0;
     const std::string msg = rc != AFE_OK ? std::string(afe_last_error()) : std::string();
     const int64_t kept = t_dev_bytes;
     lk.lock();
@@ -425,6 +490,7 @@ static void srv_engine(afe_server *s) {
   std::unique_lock<std::mutex> lk(s->m);
   while (!s->stop) {
     // admit every ready request while slots are free
+    const auto te0 = std::chrono::steady_clock::now();
     while (!s->ready.empty() && (int)solving.size() < s->max_slots) {
       SrvRequest *q = s->ready.front();
       s->ready.pop_front();
@@ -445,6 +511,7 @@ static void srv_engine(afe_server *s) {
       q->t_admit = std::chrono::steady_clock::now();
       solving.push_back(q);
     }
+    s->t_engine_locked += seconds_between(te0, std::chrono::steady_clock::now());
     if (solving.empty()) {
       s->cv_work.wait(lk);
       continue;
@@ -453,6 +520,7 @@ static void srv_engine(afe_server *s) {
     int32_t na = 0;
     const int rc = afe_batch_step(s->batch, &na);
     lk.lock();
+    const auto te1 = std::chrono::steady_clock::now();
     if (rc != AFE_OK) s->engine_error = afe_last_error();
     for (size_t k = 0; k < solving.size();) {
       SrvRequest *q = solving[k];
@@ -480,7 +548,11 @@ static void srv_engine(afe_server *s) {
         s->t_setup_cloned += seconds_between(q->t_setup, q->t_ready);
         s->n_cloned++;
       }
-      s->t_wait_setup += seconds_between(q->t_submit, q->t_setup);
+      s->t_wait_setup += seconds_between(q->t_submit, q->t_taken);
+      s->t_wait_room += seconds_between(q->t_taken, q->t_setup);
+      s->cpu_user += q->cpu_user;
+      s->cpu_sys += q->cpu_sys;
+      s->lock_wait += q->lock_wait;
       s->t_wait_slot += seconds_between(q->t_ready, q->t_admit);
       s->t_solve += seconds_between(q->t_admit, q->t_done);
       q->n_dof = q->e->n_dof;
@@ -493,6 +565,7 @@ static void srv_engine(afe_server *s) {
       s->cv_done.notify_all();
       s->cv_work.notify_all();
     }
+    s->t_engine_locked += seconds_between(te1, std::chrono::steady_clock::now());
   }
 }
 
@@ -527,6 +600,18 @@ extern "C" int afe_server_destroy(afe_server *s) {
   s->cv_done.notify_all();
   for (auto &t : s->workers) t.join();
   s->engine.join();
+  if (verbose()) {
+    static const char *names[15] = {"mesh", "space", "matrix+vectors", "assemble", "rhs", "dirichlet", "amg",
+                                    "zero", "mesh", "coords", "assemble", "rhs", "dirichlet", "lazy_reinit",
+                                    "zero"};
+    fprintf(stderr, "server set-up step means (s), new meshes (%lld):", (long long)s->n_phase_new);
+    for (int i = 0; i < 8; i++) fprintf(stderr, " %s %.4f", names[i], s->t_phase[i] / std::max<int64_t>(1, s->n_phase_new));
+    fprintf(stderr, "\nserver set-up step means (s), moving meshes (%lld):", (long long)s->n_phase_moving);
+    for (int i = 8; i < 15; i++)
+      fprintf(stderr, " %s %.4f", names[i], s->t_phase[i] / std::max<int64_t>(1, s->n_phase_moving));
+    fprintf(stderr, "\nserver lock held for evictions %.2f s (%lld), by the engine's batch calls %.2f s\n",
+            s->t_evict, (long long)s->evictions, s->t_engine_locked);
+  }
   afe_batch_destroy(s->batch);
   for (auto &kv : s->cache) srv_free_entry(s, kv.second.get());
   s->cache.clear();
@@ -627,7 +712,7 @@ extern "C" int afe_server_info(afe_server *s, int64_t *device_bytes, int64_t *ca
   if (!s) return fail(AFE_E_ARG, "null argument");
   std::lock_guard<std::mutex> lk(s->m);
   if (device_bytes) {
-    std::lock_guard<std::recursive_mutex> lc(s->ctx->mu);
+    std::lock_guard<CtxMutex> lc(s->ctx->mu);
     *device_bytes = s->ctx->bytes; // includes the afe_dev_alloc buffers
   }
   if (cached) *cached = (int64_t)s->cache.size();
@@ -643,9 +728,10 @@ extern "C" int afe_server_info(afe_server *s, int64_t *device_bytes, int64_t *ca
 extern "C" int afe_server_timing_n(afe_server *s, double *out, int32_t n_out) {
   if (!s || !out || n_out < 0) return fail(AFE_E_ARG, "bad arguments");
   std::lock_guard<std::mutex> lk(s->m);
-  const double v[9] = {s->t_setup_new, s->t_setup_moving, s->t_wait_setup, s->t_wait_slot, s->t_solve,
-                       (double)s->n_new, (double)s->n_moving, s->t_setup_cloned, (double)s->n_cloned};
-  for (int i = 0; i < n_out && i < 9; i++) out[i] = v[i];
+  const double v[13] = {s->t_setup_new, s->t_setup_moving, s->t_wait_setup, s->t_wait_slot, s->t_solve,
+                        (double)s->n_new, (double)s->n_moving, s->t_setup_cloned, (double)s->n_cloned,
+                        s->t_wait_room, s->cpu_user, s->cpu_sys, s->lock_wait};
+  for (int i = 0; i < n_out && i < 13; i++) out[i] = v[i];
   return AFE_OK;
 }
 

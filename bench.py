@@ -869,6 +869,10 @@ def run_c5(args):
             "wait_for_slot": tm["wait_slot_s"] / max(1, tm["n_new"] + tm["n_moving"]),
             "solve": tm["solve_s"] / max(1, tm["n_new"] + tm["n_moving"]),
             "setup_new_mesh_copying_a_cached_structure": tm["setup_cloned_s"] / max(1, tm["n_cloned"]),
+            "wait_for_hbm_room": tm["wait_room_s"] / max(1, tm["n_new"] + tm["n_moving"]),
+            "setup_cpu_user": tm["setup_cpu_user_s"] / max(1, tm["n_new"] + tm["n_moving"]),
+            "setup_cpu_system": tm["setup_cpu_sys_s"] / max(1, tm["n_new"] + tm["n_moving"]),
+            "setup_context_lock_wait": tm["setup_lock_wait_s"] / max(1, tm["n_new"] + tm["n_moving"]),
         },
         "server_counts": {"new_mesh": tm["n_new"], "new_mesh_copying_a_cached_structure": tm["n_cloned"],
                           "moving_mesh": tm["n_moving"]},

@@ -450,8 +450,11 @@ int afe_server_info(afe_server *s, int64_t *device_bytes, int64_t *cached,
  * the counts of new and moving-mesh requests */
 int afe_server_timing(afe_server *s, double *out);
 /* the same sums, the first n_out of: the seven above, then out[7] the set-up
- * of new meshes that copied a cached entry's structure (part of out[0]) and
- * out[8] their count */
+ * of new meshes that copied a cached entry's structure (part of out[0]),
+ * out[8] their count, out[9] waiting for HBM room before set-ups (a set-up
+ * thread holds the request meanwhile), out[10] and out[11] the set-up
+ * threads' user and system CPU seconds during set-ups, out[12] their waits
+ * for the context lock during set-ups */
 int afe_server_timing_n(afe_server *s, double *out, int32_t n_out);
 
 /* ---- measurement hooks (no counterpart in the reference) ------------------ */
