@@ -81,8 +81,6 @@ extern "C" int afe_version(void) { return 1; }
 // gathers on the second stream)
 constexpr int kAsmChunksMax = 16;
 
-constexpr int kPinSlots = 4096;
-
 // seconds the calling thread has waited for context locks (the server
 // accounts its set-up threads' waits)
 static thread_local double t_ctx_lock_wait = 0;
@@ -106,10 +104,6 @@ struct afe_ctx {
   hipStream_t stream2 = nullptr; // the gathers of a pipelined assembly
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t asm_ev[kAsmChunksMax + 2] = {}; // pipelined assembly: chunk done, start, join
-  // pinned host slots for the solvers' scalars, one per hierarchy, from one
-  // slab: creating and destroying a hierarchy then pins and unpins nothing
-  CgScalars *pin_slab = nullptr;
-  std::vector<int32_t> pin_free;
   CtxMutex mu;
   std::atomic<int64_t> bytes{0}; // device bytes held (read without the lock by the server)
   double *d_quad = nullptr; // all quadrature rules
@@ -117,30 +111,6 @@ struct afe_ctx {
   int64_t lds_bytes = 0;    // LDS granted to the coarse-tail workgroup
   std::map<void *, int64_t> user_bytes; // afe_dev_alloc buffers
 };
-
-// a pinned scalar slot of the context's slab (caller holds c->mu); null
-// when the slab is used up
-static CgScalars *pinned_take(afe_ctx *c) {
-  if (c->pin_free.empty()) return nullptr;
-  const int32_t i = c->pin_free.back();
-  c->pin_free.pop_back();
-  return c->pin_slab + i;
-}
-
-// gives a slot back, or frees a separately pinned block (caller holds c->mu)
-static void pinned_give(afe_ctx *c, CgScalars *p) {
-  if (!p) return;
-  if (p >= c->pin_slab && p < c->pin_slab + kPinSlots)
-    c->pin_free.push_back((int32_t)(p - c->pin_slab));
-  else
-    (void)hipHostFree(p);
-}
-
-static CgScalars *pinned_scalars(afe_ctx *c) {
-  CgScalars *p = pinned_take(c);
-  if (!p) HIPCHK(hipHostMalloc((void **)&p, sizeof(CgScalars), hipHostMallocDefault));
-  return p;
-}
 
 // device bytes allocated minus freed by the calling thread (the server
 // measures what one request's set-up keeps)
@@ -164,13 +134,6 @@ static bool stream_ordered_alloc() {
   }();
   return v;
 }
-
-// set-up calls made by the server's set-up threads end without waiting for
-// the stream: such a thread chains several calls and the first one that
-// reads results on the host (or the batch, on the same stream) comes after
-// them in stream order; buffers released meanwhile are freed in stream
-// order too.  Elsewhere every call returns with its work done.
-static thread_local bool t_defer_sync = false;
 
 template <class T> struct DevBuf {
   T *p = nullptr;
@@ -223,12 +186,6 @@ template <class T> struct DevBuf {
   }
   void upload(afe_ctx *c, const std::vector<T> &v) { upload(c, v.data(), v.size()); }
 };
-
-// the end of a set-up call on context c (see t_defer_sync)
-static void end_sync(afe_ctx *c) {
-  if (t_defer_sync && stream_ordered_alloc()) return;
-  HIPCHK(hipStreamSynchronize(c->stream));
-}
 
 static inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
@@ -323,8 +280,6 @@ extern "C" int afe_ctx_create(int device, afe_ctx **out) {
   HIPCHK(hipEventCreate(&c->ev0));
   HIPCHK(hipEventCreate(&c->ev1));
   for (auto &e : c->asm_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  HIPCHK(hipHostMalloc((void **)&c->pin_slab, sizeof(CgScalars) * kPinSlots, hipHostMallocDefault));
-  for (int i = kPinSlots - 1; i >= 0; i--) c->pin_free.push_back(i);
   {
     int lds = 0;
     HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
@@ -376,7 +331,6 @@ extern "C" int afe_ctx_destroy(afe_ctx *c) {
   (void)hipEventDestroy(c->ev1);
   for (auto &e : c->asm_ev)
     if (e) (void)hipEventDestroy(e);
-  if (c->pin_slab) (void)hipHostFree(c->pin_slab);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   (void)hipStreamDestroy(c->stream);
   delete c;
@@ -468,7 +422,7 @@ extern "C" int afe_dev_zero(afe_ctx *c, void *d_ptr, int64_t bytes) {
   std::lock_guard<CtxMutex> lk(c->mu); // stream work: never inside another thread's capture
   HIPCHK(hipSetDevice(c->device));
   if (bytes > 0) HIPCHK(hipMemsetAsync(d_ptr, 0, bytes, c->stream));
-  end_sync(c);
+  HIPCHK(hipStreamSynchronize(c->stream));
   API_END
 }
 
@@ -610,7 +564,7 @@ extern "C" int afe_space_create_sides(afe_ctx *ctx, int32_t kind, int64_t n_vert
   s->slot.upload(ctx, s->h.slot);
   s->rp.upload(ctx, s->h.rp);
   s->col.upload(ctx, s->h.col);
-  end_sync(ctx);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   *out = s.release();
   API_END
 }
@@ -671,7 +625,7 @@ extern "C" int afe_space_update_coords(afe_space *s, const double *coords) {
   HIPCHK(hipSetDevice(s->ctx->device));
   std::memcpy(s->h.coords.data(), coords, s->h.coords.size() * sizeof(double));
   s->coords.upload(s->ctx, s->h.coords);
-  end_sync(s->ctx);
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
   API_END
 }
 
@@ -717,7 +671,7 @@ extern "C" int afe_matrix_create(afe_space *s, afe_matrix **out) {
   a->nnz = s->h.rp[s->h.n_dof];
   a->val.alloc(s->ctx, a->nnz);
   HIPCHK(hipMemsetAsync(a->val.p, 0, a->nnz * sizeof(double), s->ctx->stream));
-  end_sync(s->ctx);
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
   *out = a.release();
   API_END
 }
@@ -1049,7 +1003,7 @@ extern "C" int afe_assemble(afe_matrix *a, int32_t op, int32_t acc, const afe_fi
   FieldDev fd = upload_field(a->ctx, op == AFE_OP_VARSTIFF ? coef : nullptr, fstore);
   a->Ke.alloc(a->ctx, (size_t)s->h.n_elem * s->h.n_loc * s->h.n_loc);
   launch_assembly(a, op, q, fd);
-  end_sync(a->ctx);
+  HIPCHK(hipStreamSynchronize(a->ctx->stream));
   API_END
 }
 
@@ -1291,7 +1245,7 @@ extern "C" int afe_rhs(afe_space *s, int32_t acc, int32_t fkind, const afe_field
   k_gather_vec<<<nblk(s->h.n_dof), 256, 0, s->ctx->stream>>>(s->h.n_dof, s->h.n_loc, s->adj_ptr.p, s->adj.p,
                                                               Fe.p, d_f);
   HIPCHK(hipGetLastError());
-  end_sync(s->ctx);
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
   API_END
 }
 
@@ -1306,7 +1260,7 @@ extern "C" int afe_apply_dirichlet(afe_matrix *a, double *d_u, double *d_f) {
   k_dirichlet_zero<<<nblk(s->h.n_dof), 256, 0, a->ctx->stream>>>(s->h.n_dof, s->rp.p, s->col.p, s->dof_mark.p,
                                                                   a->val.p, d_f);
   HIPCHK(hipGetLastError());
-  end_sync(a->ctx);
+  HIPCHK(hipStreamSynchronize(a->ctx->stream));
   API_END
 }
 
@@ -2064,7 +2018,7 @@ extern "C" int afe_amg_create(afe_matrix *a, const afe_amg_params *p, afe_amg **
   const auto t2 = std::chrono::steady_clock::now();
   std::lock_guard<CtxMutex> lk(a->ctx->mu);
   HIPCHK(hipSetDevice(a->ctx->device));
-  g->h_sc = pinned_scalars(a->ctx);
+  HIPCHK(hipHostMalloc((void **)&g->h_sc, sizeof(CgScalars), hipHostMallocDefault));
   upload_layout(g.get(), lay);
   if (verbose()) {
     const auto t3 = std::chrono::steady_clock::now();
@@ -2130,7 +2084,7 @@ extern "C" int afe_amg_destroy(afe_amg *g) {
   (void)hipSetDevice(g->ctx->device);
   (void)hipStreamSynchronize(g->ctx->stream);
   drop_graphs(g);
-  pinned_give(g->ctx, g->h_sc);
+  if (g->h_sc) (void)hipHostFree(g->h_sc);
   if (g->live0) (void)hipEventDestroy(g->live0);
   if (g->live1) (void)hipEventDestroy(g->live1);
   if (g->it_ev) (void)hipEventDestroy(g->it_ev);
@@ -2270,7 +2224,7 @@ static int space_clone_into(afe_ctx *ctx, const afe_space *src, const double *co
   clone_buf(ctx, s->adj_ptr, src->adj_ptr);
   clone_buf(ctx, s->rp, src->rp);
   clone_buf(ctx, s->slot, src->slot);
-  end_sync(ctx);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   *out = s.release();
   API_END
 }
@@ -2380,7 +2334,7 @@ extern "C" int afe_amg_clone(const afe_amg *src, afe_matrix *a, afe_amg **out) {
   clone_buf(ctx, g->nat_v64, src->nat_v64);
   clone_buf(ctx, g->nat_diag, src->nat_diag);
   clone_buf(ctx, g->minv, src->minv);
-  g->h_sc = pinned_scalars(ctx);
+  HIPCHK(hipHostMalloc((void **)&g->h_sc, sizeof(CgScalars), hipHostMallocDefault));
   finish_levels(g.get());
   device_lazy_refresh(g.get(), a);
   *out = g.release();

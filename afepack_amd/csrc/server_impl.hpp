@@ -145,10 +145,9 @@ struct afe_server {
   // prints their means at destroy): new meshes 0-7, moving meshes 8-14
   double t_phase[15] = {};
   int64_t n_phase_new = 0, n_phase_moving = 0;
-  // seconds spent freeing evicted entries and in the engine's batch calls
-  // other than the step (AFE_VERBOSE)
+  // seconds the server lock was held for evictions and for the engine's
+  // batch calls (AFE_VERBOSE)
   double t_evict = 0, t_engine_locked = 0;
-  int64_t freeing = 0; // bytes of evicted entries being freed (outside the server lock)
   int64_t next_id = 1, completed = 0, evictions = 0;
   uint64_t clock = 0;
   std::string engine_error;
@@ -178,17 +177,17 @@ static int64_t srv_bytes(afe_server *s) {
   int64_t b = 0;
   for (auto &kv : s->cache)
     if (kv.second->g) b += kv.second->est_bytes;
-  // live allocations (an atomic counter) less those of evicted entries
-  // whose buffers a set-up thread is freeing
-  const int64_t live = s->ctx->bytes.load() - s->freeing;
+  int64_t live;
+  {
+    std::lock_guard<CtxMutex> lc(s->ctx->mu);
+    live = s->ctx->bytes;
+  }
   return std::max(b, live) + s->reserved;
 }
 
-// make room for `need` bytes: idle entries leave the cache, least recently
-// used first, into `victims` (caller holds s->m, and frees the victims'
-// buffers without it, their bytes counted in s->freeing meanwhile); false
-// when only busy entries are left
-static bool srv_make_room(afe_server *s, int64_t need, std::vector<std::unique_ptr<SrvEntry>> &victims) {
+// make room for `need` bytes: evict idle entries, least recently used first
+// (caller holds s->m); false when only busy entries are left
+static bool srv_make_room(afe_server *s, int64_t need) {
   while (srv_bytes(s) + need > s->budget) {
     SrvEntry *victim = nullptr;
     for (auto &kv : s->cache) {
@@ -197,11 +196,11 @@ static bool srv_make_room(afe_server *s, int64_t need, std::vector<std::unique_p
       if (!victim || e->last_use < victim->last_use) victim = e;
     }
     if (!victim) return false;
-    auto it = s->cache.find(victim->key);
-    s->freeing += victim->est_bytes;
-    victims.push_back(std::move(it->second));
-    s->cache.erase(it);
+    const auto t0 = std::chrono::steady_clock::now();
+    srv_free_entry(s, victim);
+    s->t_evict += seconds_between(t0, std::chrono::steady_clock::now());
     s->evictions++;
+    s->cache.erase(victim->key);
   }
   return true;
 }
@@ -347,7 +346,7 @@ static SrvEntry *srv_template(afe_server *s, const SrvRequest *q) {
 // true when something other than entry e holds or reserves device memory
 // that will be given back (another set-up or a solve in progress)
 static bool srv_others_in_flight(afe_server *s, const SrvEntry *e) {
-  if (s->reserved > 0 || s->freeing > 0) return true;
+  if (s->reserved > 0) return true;
   for (auto &kv : s->cache)
     if (kv.second.get() != e && kv.second->busy) return true;
   return false;
@@ -376,7 +375,6 @@ static void srv_worker(afe_server *s) {
   // first-fit colouring (the host is the server's bottleneck)
   set_host_threads(1);
   t_first_fit_only = true;
-  t_defer_sync = true;
   std::unique_lock<std::mutex> lk(s->m);
   while (!s->stop) {
     // first pending request whose mesh is not being solved, set up or copied
@@ -436,24 +434,7 @@ static void srv_worker(afe_server *s) {
     // come back, waiting cannot help: drop the template (it may be evicted
     // then) or fail the request
     bool refused = false;
-    while (fresh && !s->stop) {
-      std::vector<std::unique_ptr<SrvEntry>> victims;
-      const bool room = srv_make_room(s, need, victims);
-      if (!victims.empty()) { // free them without the server lock
-        int64_t fb = 0;
-        for (auto &v : victims) fb += v->est_bytes;
-        lk.unlock();
-        const auto t0 = std::chrono::steady_clock::now();
-        for (auto &v : victims) srv_free_entry(s, v.get());
-        victims.clear();
-        const double dt = seconds_between(t0, std::chrono::steady_clock::now());
-        lk.lock();
-        s->freeing -= fb;
-        s->t_evict += dt;
-        s->cv_done.notify_all();
-        continue; // check again under the lock: others may have taken the room meanwhile
-      }
-      if (room) break; // room found and reserved below without releasing the lock
+    while (fresh && !s->stop && !srv_make_room(s, need)) {
       if (!srv_others_in_flight(s, e)) {
         if (q->tmpl) {
           q->tmpl->pins--;
@@ -505,85 +486,62 @@ static void srv_worker(afe_server *s) {
 }
 
 static void srv_engine(afe_server *s) {
-  std::vector<SrvRequest *> solving, admit, failed, finished;
+  std::vector<SrvRequest *> solving;
   std::unique_lock<std::mutex> lk(s->m);
   while (!s->stop) {
-    // the ready requests that fit the free slots
-    admit.clear();
-    while (!s->ready.empty() && (int)(solving.size() + admit.size()) < s->max_slots) {
-      admit.push_back(s->ready.front());
-      s->ready.pop_front();
-    }
-    if (solving.empty() && admit.empty()) {
-      s->cv_work.wait(lk);
-      continue;
-    }
-    // the batch calls run without the server lock: admitted and solving
-    // requests (and their busy entries) belong to this thread until it
-    // marks them finished under the lock
-    lk.unlock();
+    // admit every ready request while slots are free
     const auto te0 = std::chrono::steady_clock::now();
-    failed.clear();
-    finished.clear();
-    for (SrvRequest *q : admit) {
-      const int arc = afe_batch_add(s->batch, q->e->g, q->e->d_x, q->e->d_b, q->r.rtol, q->r.max_it, &q->slot);
-      if (arc != AFE_OK) {
-        q->status = arc;
+    while (!s->ready.empty() && (int)solving.size() < s->max_slots) {
+      SrvRequest *q = s->ready.front();
+      s->ready.pop_front();
+      const int rc = afe_batch_add(s->batch, q->e->g, q->e->d_x, q->e->d_b, q->r.rtol, q->r.max_it, &q->slot);
+      if (rc != AFE_OK) {
+        q->status = rc;
         q->err = afe_last_error();
+        q->state = 4;
         q->t_done = std::chrono::steady_clock::now();
-        failed.push_back(q);
+        q->n_dof = q->e->n_dof;
+        q->e->busy = false;
+        q->e = nullptr;
+        s->completed++;
+        s->cv_done.notify_all();
         continue;
       }
+      q->state = 3;
       q->t_admit = std::chrono::steady_clock::now();
       solving.push_back(q);
     }
-    double t_batch = seconds_between(te0, std::chrono::steady_clock::now());
-    int rc = AFE_OK;
-    std::string step_err;
-    if (!solving.empty()) {
-      int32_t na = 0;
-      rc = afe_batch_step(s->batch, &na);
-      if (rc != AFE_OK) step_err = afe_last_error();
-      const auto te1 = std::chrono::steady_clock::now();
-      for (size_t k = 0; k < solving.size();) {
-        SrvRequest *q = solving[k];
-        int32_t done = 0;
-        const int st = afe_batch_result(s->batch, q->slot, &done, &q->st);
-        if (!done && rc == AFE_OK) {
-          k++;
-          continue;
-        }
-        q->status = rc != AFE_OK ? rc : st;
-        if (q->r.keep_x) {
-          q->x.resize(q->e->n_dof);
-          if (afe_dev_download(s->ctx, q->x.data(), q->e->d_x, q->e->n_dof * 8) != AFE_OK && q->status == AFE_OK) {
-            q->status = AFE_E_HIP;
-            q->err = afe_last_error();
-          }
-        }
-        afe_batch_remove(s->batch, q->slot);
-        q->slot = -1;
-        q->t_done = std::chrono::steady_clock::now();
-        finished.push_back(q);
-        solving[k] = solving.back();
-        solving.pop_back();
-      }
-      t_batch += seconds_between(te1, std::chrono::steady_clock::now());
+    s->t_engine_locked += seconds_between(te0, std::chrono::steady_clock::now());
+    if (solving.empty()) {
+      s->cv_work.wait(lk);
+      continue;
     }
+    lk.unlock();
+    int32_t na = 0;
+    const int rc = afe_batch_step(s->batch, &na);
     lk.lock();
-    s->t_engine_locked += t_batch;
-    if (!step_err.empty()) s->engine_error = step_err;
-    for (SrvRequest *q : admit)
-      if (q->slot >= 0) q->state = 3;
-    for (SrvRequest *q : failed) {
+    const auto te1 = std::chrono::steady_clock::now();
+    if (rc != AFE_OK) s->engine_error = afe_last_error();
+    for (size_t k = 0; k < solving.size();) {
+      SrvRequest *q = solving[k];
+      int32_t done = 0;
+      const int st = afe_batch_result(s->batch, q->slot, &done, &q->st);
+      if (!done && rc == AFE_OK) {
+        k++;
+        continue;
+      }
+      q->status = rc != AFE_OK ? rc : st;
+      if (q->r.keep_x) {
+        q->x.resize(q->e->n_dof);
+        if (afe_dev_download(s->ctx, q->x.data(), q->e->d_x, q->e->n_dof * 8) != AFE_OK && q->status == AFE_OK) {
+          q->status = AFE_E_HIP;
+          q->err = afe_last_error();
+        }
+      }
+      afe_batch_remove(s->batch, q->slot);
+      q->slot = -1;
       q->state = 4;
-      q->n_dof = q->e->n_dof;
-      q->e->busy = false;
-      q->e = nullptr;
-      s->completed++;
-    }
-    for (SrvRequest *q : finished) {
-      q->state = 4;
+      q->t_done = std::chrono::steady_clock::now();
       (q->moving ? s->t_setup_moving : s->t_setup_new) += seconds_between(q->t_setup, q->t_ready);
       (q->moving ? s->n_moving : s->n_new)++;
       if (q->cloned) {
@@ -602,11 +560,12 @@ static void srv_engine(afe_server *s) {
       q->e->last_use = ++s->clock;
       q->e = nullptr;
       s->completed++;
-    }
-    if (!failed.empty() || !finished.empty()) {
+      solving[k] = solving.back();
+      solving.pop_back();
       s->cv_done.notify_all();
       s->cv_work.notify_all();
     }
+    s->t_engine_locked += seconds_between(te1, std::chrono::steady_clock::now());
   }
 }
 
@@ -650,7 +609,7 @@ extern "C" int afe_server_destroy(afe_server *s) {
     fprintf(stderr, "\nserver set-up step means (s), moving meshes (%lld):", (long long)s->n_phase_moving);
     for (int i = 8; i < 15; i++)
       fprintf(stderr, " %s %.4f", names[i], s->t_phase[i] / std::max<int64_t>(1, s->n_phase_moving));
-    fprintf(stderr, "\nserver: evictions %.2f s (%lld), engine batch calls outside the step %.2f s\n",
+    fprintf(stderr, "\nserver lock held for evictions %.2f s (%lld), by the engine's batch calls %.2f s\n",
             s->t_evict, (long long)s->evictions, s->t_engine_locked);
   }
   afe_batch_destroy(s->batch);
