@@ -120,17 +120,21 @@ static thread_local int64_t t_dev_bytes = 0;
 // cannot spare; the server's set-up threads set it)
 static thread_local bool t_first_fit_only = false;
 
-// The library's own device buffers come from the device's stream-ordered
-// pool (hipMallocAsync / hipFreeAsync on the owning context's stream): a
-// set-up allocates and a cache eviction frees a few hundred buffers, and the
-// pool serves them without a device-wide allocation or free each time.  A
-// buffer is used on its context's stream, or on another stream only after an
-// event or synchronisation that follows its allocation; it is freed only when
-// no queued work uses it.  AFE_STREAM_ALLOC=0 uses hipMalloc / hipFree.
+// AFE_STREAM_ALLOC=1 takes the library's own device buffers from the
+// device's stream-ordered pool (hipMallocAsync / hipFreeAsync on the owning
+// context's stream) instead of hipMalloc / hipFree: a serving set-up
+// allocates and a cache eviction frees a few hundred buffers, which the pool
+// serves without a device-wide allocation or free each time.  A buffer is
+// used on its context's stream, or on another stream only after an event or
+// synchronisation that follows its allocation; it is freed only when no
+// queued work uses it.  Off by default: it is covered by the GPU tests and
+// by short serving streams, and a long serving stream run with it (and with
+// further serving changes since taken out) ended in a host fault whose cause
+// was not found.
 static bool stream_ordered_alloc() {
   static const bool v = [] {
     const char *e = getenv("AFE_STREAM_ALLOC");
-    return !(e && atoi(e) == 0);
+    return e && atoi(e) != 0;
   }();
   return v;
 }
