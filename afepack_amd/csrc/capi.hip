@@ -849,12 +849,34 @@ static int gather_xcd(int kind) {
   return forced >= 0 ? forced : kind == P1_TET;
 }
 
+// The LDS-staged gather (a block's element contributions loaded together,
+// then summed row by row) on tetrahedral meshes, whose rows have ~24
+// elements; AFE_GATHER_STAGED=0/1 forces either form
+static bool gather_staged(int kind) {
+  static const int forced = [] {
+    const char *e = getenv("AFE_GATHER_STAGED");
+    return e ? (atoi(e) != 0) : -1;
+  }();
+  return forced >= 0 ? forced == 1 : kind == P1_TET;
+}
+
 template <int NL, int LAYOUT> static void launch_gather_nl(afe_matrix *a, int64_t b0, int64_t b1, hipStream_t st) {
   afe_space *s = a->s;
   const size_t lds = (size_t)s->seg_max * sizeof(double);
   const int xcd = gather_xcd(s->h.kind);
   const int64_t nb = xcd ? (b1 - b0 + 7) / 8 * 8 : b1 - b0;
-  if (s->distinct_dofs)
+  const int64_t seg_cap = (s->seg_max + 1) / 2 * 2; // keeps the staged values 16-byte aligned
+  const size_t lds_staged = (size_t)seg_cap * sizeof(double) + (size_t)GATHER_CHUNK * NL * (sizeof(double) + 1);
+  if (s->distinct_dofs && gather_staged(s->h.kind) && (int64_t)lds_staged <= a->ctx->lds_bytes) {
+    static std::once_flag once; // LDS above the default 64 KiB for this form
+    std::call_once(once, [&] {
+      HIPCHK(hipFuncSetAttribute((const void *)k_gather_staged<NL, LAYOUT>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)a->ctx->lds_bytes));
+    });
+    k_gather_staged<NL, LAYOUT><<<(unsigned)nb, 256, lds_staged, st>>>(s->h.n_dof, b0, b1, xcd, seg_cap, s->rp.p,
+                                                                       s->adj_ptr.p, s->adj.p, s->slot.p,
+                                                                       s->h.n_elem, a->Ke.p, a->val.p);
+  } else if (s->distinct_dofs)
     k_gather<NL, LAYOUT, true><<<(unsigned)nb, 256, lds, st>>>(s->h.n_dof, b0, b1, xcd, s->rp.p, s->adj_ptr.p,
                                                                s->adj.p, s->slot.p, s->h.n_elem, a->Ke.p, a->val.p);
   else

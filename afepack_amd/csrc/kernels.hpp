@@ -233,6 +233,70 @@ __global__ __launch_bounds__(256) void k_gather(
   for (int64_t k = threadIdx.x; k < segn; k += 256) val[base + k] = seg[k];
 }
 
+// The same gather with the block's element contributions staged in LDS:
+// the block's rows own the contiguous adjacency range [adj_ptr[r0],
+// adj_ptr[r1]), which is walked in chunks of GATHER_CHUNK entries.  All 256
+// threads load a chunk's element-matrix rows and slots together (each thread
+// several independent entries, so many loads are in flight), then each row
+// sums its own entries of the chunk from LDS in ascending order.  Chunks go
+// in order, so every slot still takes its contributions in ascending element
+// order.  For rows with long element lists (tetrahedra, ~24), where the
+// one-thread walk is a chain of dependent loads.  LDS: seg_cap doubles of
+// row values, then the chunk's NL values and NL slots per entry; the slots of
+// one element of a row are distinct (BATCH's condition).
+constexpr int GATHER_CHUNK = 1024;
+template <int NL, int LAYOUT>
+__global__ __launch_bounds__(256) void k_gather_staged(
+    int64_t n_rows, int64_t blk0, int64_t blk1, int xcd, int64_t seg_cap, const int64_t *__restrict__ rp,
+    const int64_t *__restrict__ adj_ptr, const int32_t *__restrict__ adj, const uint8_t *__restrict__ slot,
+    int64_t n_elem, const double *__restrict__ Ke, double *__restrict__ val) {
+  extern __shared__ double seg[];
+  double *skv = seg + seg_cap;
+  uint8_t *ssl = reinterpret_cast<uint8_t *>(skv + GATHER_CHUNK * NL);
+  const int64_t blk = blk0 + logical_block(xcd);
+  if (blk >= blk1) return; // the whole block (the grid is rounded up to a multiple of 8)
+  const int64_t r0 = blk * 256;
+  const int64_t r1 = r0 + 256 < n_rows ? r0 + 256 : n_rows;
+  const int64_t base = rp[r0];
+  const int64_t segn = rp[r1] - base;
+  for (int64_t k = threadIdx.x; k < segn; k += 256) seg[k] = 0.0;
+  const int64_t i = r0 + threadIdx.x;
+  const bool mine = i < r1;
+  double *row = mine ? seg + (rp[i] - base) : seg;
+  const int64_t pa = mine ? adj_ptr[i] : 0, pb = mine ? adj_ptr[i + 1] : 0;
+  const int64_t q0 = adj_ptr[r0], q1 = adj_ptr[r1];
+  for (int64_t c0 = q0; c0 < q1; c0 += GATHER_CHUNK) {
+    const int64_t c1 = c0 + GATHER_CHUNK < q1 ? c0 + GATHER_CHUNK : q1;
+    __syncthreads(); // the previous chunk is summed (and the row values cleared)
+    for (int64_t p = c0 + threadIdx.x; p < c1; p += 256) {
+      const int32_t ea = adj[p];
+      const int64_t e = ea >> 3;
+      const int a = ea & 7;
+      const int k = (int)(p - c0);
+#pragma unroll
+      for (int b = 0; b < NL; b++) {
+        skv[k * NL + b] = ke_entry<NL, LAYOUT>(Ke, n_elem, e, a, b);
+        ssl[k * NL + b] = slot[p * NL + b];
+      }
+    }
+    __syncthreads();
+    const int64_t lo = pa > c0 ? pa : c0, hi = pb < c1 ? pb : c1;
+    for (int64_t p = lo; p < hi; p++) {
+      const int k = (int)(p - c0);
+      int sl[NL];
+      double old[NL];
+#pragma unroll
+      for (int b = 0; b < NL; b++) sl[b] = ssl[k * NL + b];
+#pragma unroll
+      for (int b = 0; b < NL; b++) old[b] = row[sl[b]];
+#pragma unroll
+      for (int b = 0; b < NL; b++) row[sl[b]] = old[b] + skv[k * NL + b];
+    }
+  }
+  __syncthreads();
+  for (int64_t k = threadIdx.x; k < segn; k += 256) val[base + k] = seg[k];
+}
+
 // ---------------------------------------------------------------------------
 // two FE spaces on one mesh (BilinearOperator's same-mesh branch):
 // L2InnerProduct's element matrix (BilinearOperator.templates.h:300-326, the

@@ -417,19 +417,21 @@ def _assembly_cases_match_oracle():
             assert ok
 
 
-@pytest.mark.parametrize("chunks,xcd,layout", [("8", "1", "element"), ("1", "0", "entry")])
-def test_forced_assembly_launches_match_oracle(chunks, xcd, layout):
+@pytest.mark.parametrize("chunks,xcd,layout,staged", [("8", "1", "element", "1"), ("1", "0", "entry", "0")])
+def test_forced_assembly_launches_match_oracle(chunks, xcd, layout, staged):
     """The launch forms the defaults do not pick for a kind (pipelined
-    triangles with XCD-ranged gathers and element-major element matrices;
-    tetrahedra in two whole-mesh launches with round-robin gathers and
-    entry-major matrices) give the oracle's values too (a child process,
-    since the forms are read from the environment once)"""
+    triangles with XCD-ranged, LDS-staged gathers and element-major element
+    matrices; tetrahedra in two whole-mesh launches with round-robin
+    one-thread-per-row gathers and entry-major matrices) give the oracle's
+    values too (a child process, since the forms are read from the
+    environment once)"""
     import os
     import subprocess
     import sys
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    env = dict(os.environ, AFE_ASM_CHUNKS=chunks, AFE_GATHER_XCD=xcd, AFE_KE_LAYOUT=layout)
-    print(f"case forced chunks={chunks} xcd={xcd} layout={layout}", flush=True)
+    env = dict(os.environ, AFE_ASM_CHUNKS=chunks, AFE_GATHER_XCD=xcd, AFE_KE_LAYOUT=layout,
+               AFE_GATHER_STAGED=staged)
+    print(f"case forced chunks={chunks} xcd={xcd} layout={layout} staged={staged}", flush=True)
     r = subprocess.run([sys.executable, "-c", "from tests.test_gpu_parity import _assembly_cases_match_oracle as f; f()"],
                        cwd=root, env=env, capture_output=True, text=True, timeout=100)
     print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
