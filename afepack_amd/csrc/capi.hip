@@ -297,6 +297,16 @@ extern "C" int afe_ctx_create(int device, afe_ctx **out) {
                                (int)c->lds_bytes));
     HIPCHK(hipFuncSetAttribute((const void *)kb_tail<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)c->lds_bytes));
+    // the staged gather's row values and chunk exceed the default 64 KiB
+    const void *staged[] = {
+        (const void *)k_gather_staged<3, 0>, (const void *)k_gather_staged<3, 1>,
+        (const void *)k_gather_staged<3, 2>, (const void *)k_gather_staged<3, 3>,
+        (const void *)k_gather_staged<4, 0>, (const void *)k_gather_staged<4, 1>,
+        (const void *)k_gather_staged<4, 2>, (const void *)k_gather_staged<4, 3>,
+        (const void *)k_gather_staged<6, 0>, (const void *)k_gather_staged<6, 1>,
+        (const void *)k_gather_staged<6, 2>, (const void *)k_gather_staged<6, 3>};
+    for (const void *f : staged)
+      HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_bytes));
   }
   HIPCHK(hipMalloc((void **)&c->d_quad, sizeof(kQuadData)));
   c->bytes += sizeof(kQuadData);
@@ -868,11 +878,6 @@ template <int NL, int LAYOUT> static void launch_gather_nl(afe_matrix *a, int64_
   const int64_t seg_cap = (s->seg_max + 1) / 2 * 2; // keeps the staged values 16-byte aligned
   const size_t lds_staged = (size_t)seg_cap * sizeof(double) + (size_t)GATHER_CHUNK * NL * (sizeof(double) + 1);
   if (s->distinct_dofs && gather_staged(s->h.kind) && (int64_t)lds_staged <= a->ctx->lds_bytes) {
-    static std::once_flag once; // LDS above the default 64 KiB for this form
-    std::call_once(once, [&] {
-      HIPCHK(hipFuncSetAttribute((const void *)k_gather_staged<NL, LAYOUT>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)a->ctx->lds_bytes));
-    });
     k_gather_staged<NL, LAYOUT><<<(unsigned)nb, 256, lds_staged, st>>>(s->h.n_dof, b0, b1, xcd, seg_cap, s->rp.p,
                                                                        s->adj_ptr.p, s->adj.p, s->slot.p,
                                                                        s->h.n_elem, a->Ke.p, a->val.p);
