@@ -394,6 +394,28 @@ def test_pipelined_assembly_matches_oracle(ctx, kind, n, jitter, acc, shuffle):
             assert np.array_equal(val, oval)
 
 
+@pytest.mark.parametrize("kind,n,jitter,acc", [PIPELINE_CASES[1], PIPELINE_CASES[2]])
+def test_assembly_timing_entries_leave_the_assembly(ctx, kind, n, jitter, acc):
+    """afe_assemble_timing (the assembly as it runs, pipelined on tetrahedra)
+    and afe_assemble_kernel_timing (its kernels as whole-mesh launches)
+    report positive times and the algorithmic bytes (coordinates, element
+    vertex and DOF indices, CSR values), and leave the values of a plain
+    assembly bit for bit"""
+    import afepack_amd as afe
+    print(f"case assembly timing kind={kind} n={n}", flush=True)
+    c, ev, vm = make_mesh(kind, n, jitter, 3)
+    sp = afe.Space(ctx, kind, c, ev, vm)
+    A = afe.Matrix(sp)
+    A.assemble(afe.OP_STIFF, acc)
+    ref = A.values()
+    ms = A.assemble_pipelined_ms(afe.OP_STIFF, acc, reps=2)
+    assert ms > 0 and np.array_equal(A.values(), ref)
+    te, tg, by = A.assemble_timing(afe.OP_STIFF, acc, reps=2)
+    assert te > 0 and tg > 0 and np.array_equal(A.values(), ref)
+    nv = 4 if kind == afe.P1_TET else 3
+    assert by == len(c) * 8 + sp.n_elem * (nv + sp.n_loc) * 4 + sp.nnz * 8
+
+
 def _assembly_cases_match_oracle():
     """(child process) every kind at the pipeline size, three operators, in
     the launch configuration the environment forces; prints one line per case"""
