@@ -2736,8 +2736,7 @@ template <class T> static void issue_cg_update(afe_amg *g) {
   k_cg_update<T><<<RED_BLOCKS, 256, 0, s>>>(n, g->sc.p, g->wx.p, g->wxl.p, g->wr.p, g->wp.p, g->wq.p,
                                              Prec<T>::b(d0), g->tail0 > 0 ? nullptr : Prec<T>::x(d0),
                                              g->iperm0.p, g->part.p);
-  k_finalize<<<1, FIN_THREADS, 0, s>>>(g->part.p, RED_BLOCKS, 1, &g->sc.p->rr);
-  HIPCHK(hipMemcpyAsync(g->h_sc, g->sc.p, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
+  k_finalize_host<<<1, FIN_THREADS, 0, s>>>(g->part.p, RED_BLOCKS, &g->sc.p->rr, g->sc.p, g->h_sc);
 }
 
 // part 1: z = M r (the V-cycle, issued inline: nested capture is not used),

@@ -1689,6 +1689,23 @@ struct CgScalars {
   double rz, pq, rz_new, rz_cross, rr, bb, l1, beta;
 };
 
+// k_finalize of one sum into *out (a field of *sc), then the scalars copied
+// to pinned host memory by the thread that wrote the sum, so the host reads
+// them once the launch is complete without a separate copy (vector stores
+// to host-coherent memory; the launch's completion makes them visible)
+__global__ __launch_bounds__(FIN_THREADS) void k_finalize_host(const double *__restrict__ part, int np,
+                                                               double *out, const CgScalars *sc,
+                                                               CgScalars *__restrict__ h_sc) {
+  __shared__ double sh[FIN_THREADS];
+  double s = strided_sum(part, np, threadIdx.x, blockDim.x);
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) {
+    *out = s;
+    const CgScalars v = *sc;
+    *h_sc = v;
+  }
+}
+
 #pragma clang fp contract(off)
 // alpha = rz / pq; x += alpha p kept as the unevaluated pair x_hi + x_lo
 // (the sum and the product errors go to x_lo); r -= alpha q;
