@@ -107,6 +107,7 @@ SIGS = {
     "afe_copy_kernel_timing": (C.c_int, [P, I64, I32, PF64]),
     "afe_amg_set_tail_rows": (C.c_int, [P, I64]),
     "afe_amg_live_timing": (C.c_int, [P, I32, PF64, PI64]),
+    "afe_amg_part_timing": (C.c_int, [P, I32, PF64, PF64, PI64]),
     "afe_server_create": (C.c_int, [I32, I64, I32, I32, I32, PP]),
     "afe_server_destroy": (C.c_int, [P]),
     "afe_server_submit": (C.c_int, [P, C.POINTER(Request), PI64]),

@@ -533,6 +533,13 @@ class Amg(_AmgBase):
         _chk(_lib().afe_amg_live_timing(self.h, 1 if enable else 0, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def part_timing(self, enable):
+        """Start (True) or stop (False) graphs-on timing of the two parts of
+        every CG iteration; returns (part 0 ms, part 1 ms, iterations timed)"""
+        a, b, n = C.c_double(), C.c_double(), C.c_int64()
+        _chk(_lib().afe_amg_part_timing(self.h, 1 if enable else 0, C.byref(a), C.byref(b), C.byref(n)))
+        return a.value, b.value, n.value
+
     def set_tail_rows(self, rows):
         _chk(_lib().afe_amg_set_tail_rows(self.h, int(rows)))
 

@@ -1535,6 +1535,14 @@ struct afe_amg {
   hipEvent_t live0 = nullptr, live1 = nullptr;
   double live_ms = 0;
   int64_t live_n = 0, live_missed = 0;
+  // part timing (afe_amg_part_timing): events before part 0, between the
+  // parts and after part 1 of a CG iteration, in two alternating sets (an
+  // iteration's set is read once the next iteration's part 0 has finished)
+  bool parts = false, part_pending = false;
+  int part_set = 0;
+  hipEvent_t pev[2][3] = {};
+  double part_ms[2] = {0, 0};
+  int64_t part_n = 0;
   afe_amg_params prm{};
   Hierarchy h;
   std::vector<DevLevel> lv;
@@ -2118,6 +2126,9 @@ extern "C" int afe_amg_destroy(afe_amg *g) {
   if (g->h_sc) (void)hipHostFree(g->h_sc);
   if (g->live0) (void)hipEventDestroy(g->live0);
   if (g->live1) (void)hipEventDestroy(g->live1);
+  for (auto &set : g->pev)
+    for (auto &e : set)
+      if (e) (void)hipEventDestroy(e);
   if (g->it_ev) (void)hipEventDestroy(g->it_ev);
   delete g;
   return AFE_OK;
@@ -2784,13 +2795,40 @@ template <class T> static void launch_cg_iteration(afe_amg *g) {
   const int prec = std::is_same<T, float>::value ? 1 : 0;
   hipStream_t st = g->ctx->stream;
   if (!g->it_ev) HIPCHK(hipEventCreateWithFlags(&g->it_ev, hipEventDisableTiming));
+  hipEvent_t *pe = g->parts ? g->pev[g->part_set] : nullptr;
+  if (pe) HIPCHK(hipEventRecord(pe[0], st));
   if (g->live) issue_cg_spmv(g, st, g->live0, g->live1);
   launch_part(g, 0, prec, [&]() {
     if (!g->live) issue_cg_spmv(g, st);
     issue_cg_update<T>(g);
   });
   HIPCHK(hipEventRecord(g->it_ev, st));
+  if (pe) HIPCHK(hipEventRecord(pe[1], st));
   launch_part(g, 1, prec, [&]() { issue_cg_precondition<T>(g); });
+  if (pe) HIPCHK(hipEventRecord(pe[2], st));
+}
+
+// adds the part times of the iteration recorded in event set `set` (its
+// events have completed)
+static void part_read(afe_amg *g, int set) {
+  float a = 0, b = 0;
+  if (hipEventElapsedTime(&a, g->pev[set][0], g->pev[set][1]) == hipSuccess &&
+      hipEventElapsedTime(&b, g->pev[set][1], g->pev[set][2]) == hipSuccess) {
+    g->part_ms[0] += a;
+    g->part_ms[1] += b;
+    g->part_n++;
+  } else {
+    (void)hipGetLastError();
+  }
+}
+
+// after the host's wait for an iteration's part 0: the previous iteration's
+// set is complete (its part 1 preceded this part 0 on the stream)
+static void part_step(afe_amg *g) {
+  if (!g->parts) return;
+  if (g->part_pending) part_read(g, g->part_set ^ 1);
+  g->part_pending = true;
+  g->part_set ^= 1;
 }
 
 template <class T>
@@ -2822,6 +2860,7 @@ static int pcg_impl(afe_amg *g, double *d_x, const double *d_b, double rtol, int
     while (it < max_it) {
       launch_cg_iteration<T>(g);
       HIPCHK(hipEventSynchronize(g->it_ev));
+      part_step(g);
       if (g->live) {
         // the pair is recorded by the kernel's own dispatch; a tool that
         // replaces the dispatch's completion signal leaves it unrecorded,
@@ -2842,6 +2881,7 @@ static int pcg_impl(afe_amg *g, double *d_x, const double *d_b, double rtol, int
                 g->h_sc->pq, g->h_sc->rz, g->h_sc->beta);
       if (!(rr == rr)) {
         HIPCHK(hipStreamSynchronize(s)); // part 1 of the iteration is still queued
+        g->part_pending = false;
         return fail(AFE_E_NOTCONVERGED, "CG breakdown (NaN residual)");
       }
       if (rr <= pthr) break;
@@ -2869,6 +2909,10 @@ static int pcg_impl(afe_amg *g, double *d_x, const double *d_b, double rtol, int
   HIPCHK(hipMemcpyAsync(d_x, g->wx.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   HIPCHK(hipEventRecord(ctx->ev1, s));
   HIPCHK(hipEventSynchronize(ctx->ev1));
+  if (g->parts && g->part_pending) { // the last iteration's set
+    part_read(g, g->part_set ^ 1);
+    g->part_pending = false;
+  }
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   if (st) {
@@ -3045,6 +3089,29 @@ extern "C" int afe_amg_live_timing(afe_amg *g, int32_t enable, double *avg_ms, i
     g->live = enable != 0;
     drop_graphs(g); // the captured iteration loses or regains its SpMV launch
   }
+  API_END
+}
+
+// graphs-on timing of the two parts of every CG iteration of afe_amg_pcg
+// (HIP events recorded between the captured graphs, outside them)
+extern "C" int afe_amg_part_timing(afe_amg *g, int32_t enable, double *part0_ms, double *part1_ms,
+                                   int64_t *iterations) {
+  API_BEGIN
+  if (!g || !g->ctx) return fail(AFE_E_ARG, "bad arguments");
+  std::lock_guard<CtxMutex> lk(g->ctx->mu);
+  HIPCHK(hipSetDevice(g->ctx->device));
+  if (part0_ms) *part0_ms = g->part_n ? g->part_ms[0] / g->part_n : 0.0;
+  if (part1_ms) *part1_ms = g->part_n ? g->part_ms[1] / g->part_n : 0.0;
+  if (iterations) *iterations = g->part_n;
+  if (enable && !g->pev[0][0])
+    for (auto &set : g->pev)
+      for (auto &e : set) HIPCHK(hipEventCreate(&e));
+  if (enable) {
+    g->part_ms[0] = g->part_ms[1] = 0;
+    g->part_n = 0;
+  }
+  g->parts = enable != 0;
+  g->part_pending = false;
   API_END
 }
 

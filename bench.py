@@ -380,6 +380,13 @@ def run_c2(args):
         rel = st.rel_res_l2
     live_ms, live_n = g.live_timing(False)
     xs = x.numpy()
+    # one more (untimed) solve with HIP events between the captured graphs of
+    # each iteration: the graphs-on split of an iteration
+    g.part_timing(True)
+    solve(afe.PREC_FP32, s_mixed)
+    p0_ms, p1_ms, p_n = g.part_timing(False)
+    log(f"iteration parts (graphs on): SpMV + update {1e3*p0_ms:.1f} us, V-cycle + z + p {1e3*p1_ms:.1f} us "
+        f"over {p_n} iterations")
     ms = 1e3 * statistics.mean(lat)
     value = sp.n_dof / statistics.mean(lat)
     log(f"mixed: {ms:.2f} ms/solve, {its[-1]} its, rel res {rel:.2e}, {value:.3e} DOF/s")
@@ -496,7 +503,10 @@ def run_c2(args):
         "mixed": {"iterations": its[-1], "rel_res": rel, "p50_ms": 1e3 * p50, "p99_ms": 1e3 * p99},
         "iteration": {"us": 1e3 * it_ms, "algorithmic_bytes": it_bytes, "achieved_gbs": it_gbs,
                       "frac_of_hbm_peak": it_gbs / HBM_PEAK_GBS,
-                      "note": "whole mixed CG iteration (SpMV, vector work, FP32 V-cycle) over its own time"},
+                      "parts_us": {"spmv_update": 1e3 * p0_ms, "vcycle_z_p": 1e3 * p1_ms, "iterations": p_n},
+                      "note": ("whole mixed CG iteration (SpMV, vector work, FP32 V-cycle) over its own time; "
+                               "parts_us: graphs-on split from HIP events between the two captured graphs "
+                               "of each iteration, in one more untimed solve")},
         "pcg64": {"seconds": t64, "iterations": st64.iterations, "dof_per_s": sp.n_dof / t64},
         "plain": plain,
         "mixed_vs_plain_stationary": (value / plain["dof_per_s"]) if plain and plain["dof_per_s"] else None,

@@ -484,6 +484,13 @@ int afe_assemble_timing(afe_matrix *a, int32_t op, int32_t acc,
  * starts, 0 stops; *avg_ms / *launches = the average so far */
 int afe_amg_live_timing(afe_amg *g, int32_t enable, double *avg_ms,
                         int64_t *launches);
+/* graphs-on timing of the two captured parts of every CG iteration of
+ * afe_amg_pcg: part 0 (SpMV with its dot, x/r update, scalars) and part 1
+ * (the preconditioner V-cycle, z, beta, p), from HIP events recorded between
+ * the graph launches; enable = 1 resets and starts, 0 stops; the averages
+ * so far and the iterations timed are returned */
+int afe_amg_part_timing(afe_amg *g, int32_t enable, double *part0_ms,
+                        double *part1_ms, int64_t *iterations);
 /* average duration of `reps` launches of a plain streaming copy of `bytes`
  * (read + write, 16-byte vectors): calibrates the HBM byte counters and
  * gives the achievable copy bandwidth next to the solver kernels */

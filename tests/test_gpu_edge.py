@@ -95,6 +95,32 @@ def test_live_timing_keeps_results(ctx):
     assert n2 == n and ms2 == ms
 
 
+def test_part_timing_keeps_results(ctx):
+    """Events between the two captured graphs of each CG iteration leave the
+    iterate unchanged and time every iteration's two parts"""
+    import afepack_amd as afe
+    print("case part timing P2 n=64", flush=True)
+    c, ev, vm = make_mesh(2, 64, 0.2, 3)
+    sp = afe.Space(ctx, 2, c, ev, vm)
+    A = afe.Matrix(sp)
+    A.assemble(afe.OP_STIFF, 4)
+    b = sp.rhs(4, afe.F_POISSON)
+    A.apply_dirichlet(b)
+    g = afe.Amg(A, afe.AmgParams(lazy=1, smooth=1, build_fp32=1))
+    x0 = afe.DeviceVector(ctx, sp.n_dof)
+    st0 = g.pcg(x0, b, rtol=1e-10, max_it=200, prec=afe.PREC_FP32)
+    g.part_timing(True)
+    x1 = afe.DeviceVector(ctx, sp.n_dof)
+    st1 = g.pcg(x1, b, rtol=1e-10, max_it=200, prec=afe.PREC_FP32)
+    p0, p1, n = g.part_timing(False)
+    assert st1.iterations == st0.iterations and np.array_equal(x1.numpy(), x0.numpy())
+    assert n == st1.iterations and p0 > 0.0 and p1 > 0.0
+    assert (p0 + p1) * n <= 1e3 * st1.seconds * 1.05  # the parts lie inside the solve
+    x2 = afe.DeviceVector(ctx, sp.n_dof)
+    g.pcg(x2, b, rtol=1e-10, max_it=200, prec=afe.PREC_FP32)
+    assert g.part_timing(False)[2] == n  # stopped
+
+
 @pytest.mark.parametrize("kind,n,stream", [(2, 48, None), (3, 10, None), (2, 48, "0"), (3, 10, "0")])
 def test_natural_columns_16_and_32_bit_agree(ctx, kind, n, stream):
     """The natural-order FP64 copy of level 0 stores 16-bit column offsets
