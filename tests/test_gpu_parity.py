@@ -416,6 +416,37 @@ def test_assembly_timing_entries_leave_the_assembly(ctx, kind, n, jitter, acc):
     assert by == len(c) * 8 + sp.n_elem * (nv + sp.n_loc) * 4 + sp.nnz * 8
 
 
+@pytest.mark.parametrize("kind,n,jitter", [(2, 128, 0.2), (3, 36, 0.05)])
+def test_caller_element_matrices_gathered_in_element_order(ctx, kind, n, jitter):
+    """afe_assemble_elements (the caller's element matrices, element-major
+    NL x NL, not symmetric) sums every CSR entry over its elements in
+    ascending element order from zero: equal bit for bit to a sequential
+    numpy accumulation (np.add.at in element order); tetrahedra go through
+    the LDS-staged gather"""
+    import afepack_amd as afe
+    print(f"case caller element matrices kind={kind} n={n}", flush=True)
+    c, ev, vm = make_mesh(kind, n, jitter, 9)
+    sp = afe.Space(ctx, kind, c, ev, vm)
+    nl = sp.n_loc
+    ed = sp.dofs()[0].reshape(-1, nl).astype(np.int64)
+    K = np.random.default_rng(4).standard_normal((sp.n_elem, nl, nl))
+    A = afe.Matrix(sp)
+    A.assemble_elements(K)
+    val = A.values()
+    rp, col = sp.pattern()
+    rp = rp.astype(np.int64)
+    nd = sp.n_dof
+    rows = np.repeat(np.arange(nd, dtype=np.int64), np.diff(rp))
+    keys = rows * nd + col.astype(np.int64)
+    order = np.argsort(keys)
+    ck = (ed[:, :, None] * nd + ed[:, None, :]).reshape(-1)  # element-major, (a, b) inside
+    pos = order[np.searchsorted(keys[order], ck)]
+    assert np.array_equal(keys[pos], ck)
+    ref = np.zeros(len(col))
+    np.add.at(ref, pos, K.reshape(-1))
+    assert np.array_equal(val, ref)
+
+
 def _assembly_cases_match_oracle():
     """(child process) every kind at the pipeline size, three operators, in
     the launch configuration the environment forces; prints one line per case"""
