@@ -181,8 +181,8 @@ __device__ __forceinline__ double ke_entry(const double *__restrict__ Ke, int64_
 // distinct and the row's NL LDS updates of one element are independent: the
 // NL old values are read together, then the NL sums written (one LDS round
 // trip per element instead of NL dependent ones).  Each slot still takes its
-// contributions in ascending element order.  The next adjacency entry is
-// loaded while the current one is summed.
+// contributions in ascending element order.
+constexpr int GATHER_GROUP = 4;
 template <int NL, int LAYOUT, bool BATCH>
 __global__ __launch_bounds__(256) void k_gather(
     int64_t n_rows, int64_t blk0, int64_t blk1, int xcd, const int64_t *__restrict__ rp,
@@ -203,30 +203,42 @@ __global__ __launch_bounds__(256) void k_gather(
   if (i < r1) {
     double *row = seg + (rp[i] - base);
     const int64_t p0 = adj_ptr[i], p1 = adj_ptr[i + 1];
-    int32_t ea_next = p0 < p1 ? adj[p0] : 0;
-    for (int64_t p = p0; p < p1; p++) {
-      const int32_t ea = ea_next;
-      if (p + 1 < p1) ea_next = adj[p + 1];
-      const int64_t e = ea >> 3;
-      const int a = ea & 7;
-      const uint8_t *sl = slot + p * NL;
-      int s[NL];
-      double kv[NL];
+    // the row's elements in groups of GATHER_GROUP: a group's adjacency
+    // entries, then its element-matrix rows and slots, are loaded together
+    // (two dependent loads per group instead of per element); the group's
+    // elements are then summed one after the other in ascending order
+    for (int64_t pg = p0; pg < p1; pg += GATHER_GROUP) {
+      int32_t ea[GATHER_GROUP];
+      int s[GATHER_GROUP][NL];
+      double kv[GATHER_GROUP][NL];
 #pragma unroll
-      for (int b = 0; b < NL; b++) {
-        s[b] = sl[b];
-        kv[b] = ke_entry<NL, LAYOUT>(Ke, n_elem, e, a, b);
-      }
-      if constexpr (BATCH) {
-        double old[NL];
+      for (int g = 0; g < GATHER_GROUP; g++) ea[g] = pg + g < p1 ? adj[pg + g] : 0;
 #pragma unroll
-        for (int b = 0; b < NL; b++) old[b] = row[s[b]];
+      for (int g = 0; g < GATHER_GROUP; g++)
+        if (pg + g < p1) {
+          const int64_t e = ea[g] >> 3;
+          const int a = ea[g] & 7;
+          const uint8_t *sl = slot + (pg + g) * NL;
 #pragma unroll
-        for (int b = 0; b < NL; b++) row[s[b]] = old[b] + kv[b];
-      } else {
+          for (int b = 0; b < NL; b++) {
+            s[g][b] = sl[b];
+            kv[g][b] = ke_entry<NL, LAYOUT>(Ke, n_elem, e, a, b);
+          }
+        }
 #pragma unroll
-        for (int b = 0; b < NL; b++) row[s[b]] += kv[b];
-      }
+      for (int g = 0; g < GATHER_GROUP; g++)
+        if (pg + g < p1) {
+          if constexpr (BATCH) {
+            double old[NL];
+#pragma unroll
+            for (int b = 0; b < NL; b++) old[b] = row[s[g][b]];
+#pragma unroll
+            for (int b = 0; b < NL; b++) row[s[g][b]] = old[b] + kv[g][b];
+          } else {
+#pragma unroll
+            for (int b = 0; b < NL; b++) row[s[g][b]] += kv[g][b];
+          }
+        }
     }
   }
   __syncthreads();
