@@ -932,10 +932,9 @@ static int asm_chunks(int kind, int64_t n_elem) {
 // are computed in chunks on the context stream; after chunk k the gather of
 // every row block whose adjacent elements all lie in chunks 0..k runs on the
 // second stream, so the element work of the next chunk overlaps the gather
-// of the rows already complete.  Each row is still
-// summed by one thread in ascending element order, so the values are the
-// same bits as the two whole-mesh launches.  The context stream waits for
-// the last gather.
+// of the rows already complete.  Each row is still summed by one thread in
+// ascending element order, so the values are the same bits as the two
+// whole-mesh launches.  The context stream waits for the last gather.
 static void launch_assembly(afe_matrix *a, int op, const QuadRule *q, FieldDev fd) {
   afe_space *s = a->s;
   afe_ctx *c = a->ctx;
