@@ -196,10 +196,11 @@ def run_c1(args):
 
     # each form timed with the small levels as launches below 4096 rows, and
     # with the whole hierarchy in the single-workgroup tail; the faster kept
-    def timed(run):
+    def timed(run, gg=None):
+        gg = gg or g
         best = None
         for tail in (4096, 1 << 14):
-            g.set_tail_rows(tail)
+            gg.set_tail_rows(tail)
             run()  # warm-up (graph capture)
             x.zero()
             t0 = time.perf_counter()
@@ -215,8 +216,18 @@ def run_c1(args):
     probe = g.solve(x, b, step=200, mode=1, rtol=rtol)
     k_cycles = probe.iterations
     t_plain, st, tail_plain, e_plain = timed(lambda: g.solve(x, b, step=k_cycles - 1, mode=1))
+    # the mixed form (FP32 V(1,1) in FP64 CG) on the reference's default
+    # hierarchy and on one whose coarsest level may hold up to 2048 rows
+    # (solved by the dense inverse), as the large configuration's autotuner
+    # chooses; the faster is reported
     g.set_smooth(1)
-    t_mixed, stm, tail_mixed, e_mixed = timed(lambda: g.pcg(x, b, rtol=rtol, max_it=200, prec=afe.PREC_FP32))
+    g2 = afe.Amg(A, afe.AmgParams(lazy=0, nmpo=2048, nmpsd=0.0, smooth=1, build_fp32=1))
+    mixed = []
+    for name, gg in (("AMGSolver(A) defaults", g), ("coarsest order <= 2048, dense inverse", g2)):
+        r = timed(lambda gg=gg: gg.pcg(x, b, rtol=rtol, max_it=200, prec=afe.PREC_FP32), gg)
+        log(f"c1 mixed on {name}: {1e3*r[0]:.2f} ms, {r[1].iterations} its, level sizes {gg.info()['n']}")
+        mixed.append((r, name, gg.info()["n"]))
+    (t_mixed, stm, tail_mixed, e_mixed), mixed_name, mixed_levels = min(mixed, key=lambda m: m[0][0])
     log(f"c1 device: stationary V(3,3) {st.iterations} cycles {1e3*t_plain:.2f} ms (rel res {st.rel_res_l2:.2e}, "
         f"cycles from a probe at 1e-10, then run without checks), "
         f"mixed PCG {stm.iterations} its {1e3*t_mixed:.2f} ms; L2 errors {e_plain:.6e} / {e_mixed:.6e}")
@@ -252,7 +263,9 @@ def run_c1(args):
         "plain": {"form": "stationary FP64 V(3,3) (AMGSolver::solve)", "cycles": st.iterations,
                   "seconds": t_plain, "rel_res_l2": st.rel_res_l2, "l2_error": e_plain, "tail_rows": tail_plain},
         "mixed": {"iterations": stm.iterations, "seconds": t_mixed, "rel_res_l2": stm.rel_res_l2,
-                  "l2_error": e_mixed, "tail_rows": tail_mixed},
+                  "l2_error": e_mixed, "tail_rows": tail_mixed, "hierarchy": mixed_name,
+                  "level_sizes": mixed_levels,
+                  "rejected": [{"hierarchy": n, "seconds": r[0]} for r, n, _ in mixed if n != mixed_name]},
         "setup_s": t_setup,
         "cpu_baseline": {"value": (sp.n_dof / t_cpu_to) if t_cpu_to else 0.0, "unit": "DOF/s", "cores": 1,
                          "kind": "port", "cycles": cycles_cpu, "seconds": t_cpu_to, "l2_error": e_cpu,
