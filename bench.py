@@ -675,7 +675,10 @@ def run_c4(args):
         fr = np.random.default_rng(args.seed + 2000 + i).uniform(-1.0, 1.0, sp.n_dof)
         b = sp.rhs(acc, afe.F_NODAL, nodal=fr)
         A.apply_dirichlet(b)
-        g = afe.Amg(A, afe.AmgParams(lazy=1, smooth=1, build_fp32=1))
+        if args.nmpo > 0:  # coarsest order bound given on the command line
+            g = afe.Amg(A, afe.AmgParams(lazy=1, nmpo=args.nmpo, nmpsd=args.nmpsd, smooth=1, build_fp32=1))
+        else:
+            g = afe.Amg(A, afe.AmgParams(lazy=1, smooth=1, build_fp32=1))
         x = afe.DeviceVector(ctx, sp.n_dof)
         probs.append({"sp": sp, "A": A, "b": b, "g": g, "x": x, "field": field, "kind": kind})
     t_setup = time.time() - t0
@@ -710,6 +713,31 @@ def run_c4(args):
             if ok and (best is None or dt < best[0]):
                 best = (dt, s, tail)
     _, s_best, tail_best = best
+    # the hierarchies themselves: the reference's defaults (coarsest order
+    # <= 50) against a coarsest level of up to 256 rows solved by the dense
+    # inverse inside each member's tail (fewer many-colour tiny levels)
+    hier_log = [{"nmpo": 50, "nmpsd": 0.382, "ms": 1e3 * best[0], "converged": True}]
+    hier_best = "coarsest order <= 50 (AMGSolver defaults)"
+    if args.nmpo == 0:
+        t1 = time.time()
+        alt = [afe.Amg(p["A"], afe.AmgParams(lazy=1, nmpo=256, nmpsd=0.0, smooth=1, build_fp32=1)) for p in probs]
+        t_alt = time.time() - t1
+        old = [p["g"] for p in probs]
+        for p, g2 in zip(probs, alt):
+            p["g"] = g2
+        dt2, res2 = batch_run(s_best, tail_best)
+        ok2 = all(r[0] and r[2] == 0 for r in res2)
+        hier_log.append({"nmpo": 256, "nmpsd": 0.0, "ms": 1e3 * dt2, "converged": ok2, "setup_s": t_alt})
+        log(f"c4 tune hierarchy coarsest <= 256: {dt2*1e3:.1f} ms against {best[0]*1e3:.1f} ms, all converged {ok2}")
+        if ok2 and dt2 < best[0]:
+            hier_best = "coarsest order <= 256, dense inverse"
+            for g in old:
+                g.close()
+        else:
+            for p, g in zip(probs, old):
+                p["g"] = g
+            for g in alt:
+                g.close()
     # autotune 2: launch buckets of the batched level kernels -- slices per
     # wavefront on level 0 and on the coarser levels run as launches (the
     # results are the same for every choice; only the time differs)
@@ -760,7 +788,8 @@ def run_c4(args):
                    "total_dof": total, "rtol": args.rtol, "pcg_smooth": s_best, "tail_rows": tail_best,
                    "buckets": {"level0_slices_per_wave": tiles_best[0], "coarse_slices_per_wave": tiles_best[1]},
                    "parallelism": "single GPU"},
-        "autotune": {"smooth_and_tail": tune_log, "buckets": tile_log},
+        "autotune": {"smooth_and_tail": tune_log, "hierarchy": hier_log, "hierarchy_chosen": hier_best,
+                     "buckets": tile_log},
         "p50_ms": p50, "p99_ms": p99,
         "iterations": {"min": min(its), "median": statistics.median(its), "max": max(its)},
         "worst_rel_res": worst,
