@@ -579,7 +579,10 @@ def run_c3(args):
     log(f"c3 space: {sp.n_dof} DOFs, nnz {sp.nnz}, {t_space:.1f} s, assembly {1e3*min(asm):.2f} ms")
     x = afe.DeviceVector(ctx, sp.n_dof)
     best = None
-    choices = ((50, 0.382), (2048, 0.0)) if args.nmpo == 0 else ((args.nmpo, args.nmpsd),)
+    # coarsest order bounds: the reference's default, and coarsest levels of
+    # up to 2048 / 4096 rows solved by the dense inverse (the 3D coarse
+    # operators are dense enough that their small levels need 18-28 colours)
+    choices = ((50, 0.382), (2048, 0.0), (4096, 0.0)) if args.nmpo == 0 else ((args.nmpo, args.nmpsd),)
     smooths = (1, 2) if args.smooth == 0 else (args.smooth,)
     for nmpo, nmpsd in choices:
         t = time.time()
