@@ -125,9 +125,15 @@ __global__ __launch_bounds__(256) void k_element_matrices(
       for (int j = 0; j < NL; j++)
 #pragma unroll
         for (int k = j; k < NL; k++) {
-          double ip = 0;
+          // innerProduct sums from +0.0; starting from the first product
+          // instead changes ip only when that sum is +0.0 and the product
+          // -0.0, and a +-0.0 term then adds to K, which starts at +0.0 and
+          // never holds -0.0 (a round-to-nearest sum is -0.0 only when both
+          // addends are), so every K bit is the same, with 21 fewer FP64 adds
+          // per quadrature point on P2
+          double ip = g[j][0] * g[k][0];
 #pragma unroll
-          for (int d = 0; d < D; d++) ip += g[j][d] * g[k][d];
+          for (int d = 1; d < D; d++) ip += g[j][d] * g[k][d];
           if constexpr (OP == 2)
             K[j][k] += Jxw * coef * ip;
           else
